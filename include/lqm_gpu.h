@@ -1,0 +1,129 @@
+/*
+ * lqm_gpu.h -- C ABI of liblqmgpu.so, the MI355X (gfx950) k-space band path of
+ * LatticeQM.jl: Bloch assembly H(k) = sum_R t_R e^{2 pi i k.R}, batched Hermitian
+ * eigensolve over a whole k-grid, DOS, chemical potential and the density matrix
+ * rho(R) = sum_k w_k e^{-2 pi i k.R} U f(eps) U^H.
+ *
+ * Conventions (all entry points):
+ *   - Plain C types only.  Complex numbers are interleaved (re, im) doubles, the
+ *     layout of Julia's ComplexF64 and C99 double _Complex.
+ *   - Arrays are column-major like Julia's:
+ *       R      int32  [d x nR]       lattice vectors (keys of the Hops dict)
+ *       T      cplx   [N x N x nR]   hopping blocks, block r = T + 2*r*N*N doubles
+ *       ks     double [d x Nk]       fractional k-points (Structure.points(ks))
+ *       bands  double [N x Nk]       ascending eigenvalues, column j = k_j
+ *       evecs  cplx   [N x N x Nk]   column n of block j = eigenvector of bands[n, j]
+ *       rho    cplx   [N x N x nRr]  density-matrix blocks for the keys Rr
+ *   - Every function returns an lqm_status; on failure lqm_last_error() returns a
+ *     message for the calling thread.  No exceptions or callbacks cross the ABI.
+ *   - Host-buffer functions are synchronous: pointers are read/written only during
+ *     the call.  *_dev functions take device pointers and are asynchronous on the
+ *     context's stream (call lqm_context_synchronize before reading results).
+ *   - A context is used by one host thread at a time; several may coexist.
+ */
+#ifndef LQM_GPU_H
+#define LQM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    LQM_OK = 0,
+    LQM_ERR_ARG = 1,         /* shape / argument error (DimensionMismatch, ArgumentError) */
+    LQM_ERR_DEVICE = 2,      /* HIP runtime error or device allocation failure */
+    LQM_ERR_UNSUPPORTED = 3, /* size or mode this build does not handle */
+    LQM_ERR_NOCONV = 4,      /* eigensolver did not converge (LAPACK info > 0 analogue) */
+    LQM_ERR_NONFINITE = 5    /* NaN/Inf met in the input or the results */
+} lqm_status;
+
+typedef enum {
+    LQM_PREC_FP64 = 0 /* FP64 complex everywhere: the plain path */
+} lqm_precision;
+
+typedef struct lqm_context lqm_context;
+typedef struct lqm_model lqm_model;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *lqm_last_error(void);
+/* ABI version, bumped on incompatible changes. */
+int lqm_abi_version(void);
+
+/* ---- context ------------------------------------------------------------------ */
+/* Replaces the executor choice of Parallel.to_executor (Parallel/Parallel.jl:85-109):
+ * a context is the GPUExec a multimode=:gpu call resolves to. */
+lqm_status lqm_context_create(int device, int precision, lqm_context **out);
+lqm_status lqm_context_destroy(lqm_context *ctx);
+/* Run on an existing hipStream_t (NULL restores the context's own stream). */
+lqm_status lqm_context_set_stream(lqm_context *ctx, void *hip_stream);
+lqm_status lqm_context_synchronize(lqm_context *ctx);
+
+/* ---- model: the Hops operator, resident in HBM --------------------------------- */
+/* Upload of Hops{Vector{Int},Matrix{ComplexF64}} (TightBinding/types.jl:54-60):
+ * nR keys R (d x nR) and their N x N blocks T.  Key order is the summation order. */
+lqm_status lqm_model_create(lqm_context *ctx, int N, int d, int nR, const int32_t *R,
+                            const double *T, lqm_model **out);
+/* Same, T already in device memory (copied into the model). */
+lqm_status lqm_model_create_dev(lqm_context *ctx, int N, int d, int nR, const int32_t *R,
+                                const double *T_dev, lqm_model **out);
+lqm_status lqm_model_destroy(lqm_model *m);
+
+/* ---- H(k) ------------------------------------------------------------------------ */
+/* fouriersum! (TightBinding/types.jl:18-25) at Nk points: Hk [N x N x Nk]. */
+lqm_status lqm_fouriersum(lqm_model *m, int Nk, const double *ks, double *Hk);
+lqm_status lqm_fouriersum_dev(lqm_model *m, int Nk, const double *ks_dev, double *Hk_dev);
+
+/* ---- bands ------------------------------------------------------------------------ */
+/* bandmatrix(H, ks) with no projectors (Spectrum/bands.jl:164-227): eigenvalues
+ * of Hermitian(H(k)) (upper triangle), ascending.  evecs may be NULL; when given,
+ * the eigenvectors are returned too (the geteigen! branch, bands.jl:222-224). */
+lqm_status lqm_bandmatrix(lqm_model *m, int Nk, const double *ks, double *bands, double *evecs);
+lqm_status lqm_bandmatrix_dev(lqm_model *m, int Nk, const double *ks_dev, double *bands_dev,
+                              double *evecs_dev);
+
+/* ---- DOS -------------------------------------------------------------------------- */
+/* dos_compute! (Spectrum/dos.jl:202-232): DOS(w) = sum_k wk sum_n Im[1/(w - i G - e_nk)],
+ * divided by Nk when kweights == NULL (the unweighted branch, dos.jl:221). */
+lqm_status lqm_dos(lqm_model *m, int Nk, const double *ks, const double *kweights, int Nw,
+                   const double *omegas, double gamma, double *dos);
+/* DOS from eigenvalues already on the device (bands_dev [N x Nk]). */
+lqm_status lqm_dos_from_bands_dev(lqm_context *ctx, int N, int Nk, const double *bands_dev,
+                                  const double *kweights_dev, int Nw, const double *omegas_dev,
+                                  double gamma, double *dos_dev);
+
+/* ---- chemical potential ----------------------------------------------------------- */
+/* chemicalpotential(H, ks, filling; T) (Spectrum/chemicalpotential.jl:6-65):
+ * T == 0: mean of the i-th and (i+1)-th smallest of all N*Nk eigenvalues with
+ * i = 1 + floor(filling*(L-1)); T > 0: bisection of getfilling(mu) = filling on
+ * [min, max] down to adjacent doubles.  kweights NULL = unweighted mean. */
+lqm_status lqm_chemicalpotential(lqm_model *m, int Nk, const double *ks, const double *kweights,
+                                 double filling, double T, double *mu);
+/* Same from eigenvalues already on the device; *mu is a host scalar. */
+lqm_status lqm_chemicalpotential_from_bands_dev(lqm_context *ctx, int N, int Nk,
+                                                const double *bands_dev,
+                                                const double *kweights_dev, double filling,
+                                                double T, double *mu);
+
+/* ---- density matrix --------------------------------------------------------------- */
+/* getdensitymatrix!(rho, H, ks, [kweights,] mu; T) (Operators/densitymatrix.jl:17-145):
+ * rho (keys Rr, d x nRr) is overwritten with sum_k w_k e^{-2 pi i k.Rr} U f U^H,
+ * f = fermidirac(e - mu; T), w_k = 1/Nk when kweights == NULL.  *eband receives
+ * sum_k w_k sum_n e_n f_n, the value the reference returns. */
+lqm_status lqm_densitymatrix(lqm_model *m, int Nk, const double *ks, const double *kweights,
+                             double mu, double T, int nRr, const int32_t *Rr, double *rho,
+                             double *eband);
+/* Device variant: ks_dev, kweights_dev (may be NULL), rho_dev, eband_dev (one double)
+ * are device pointers; Rr stays a host array.  bands_dev, if not NULL, receives
+ * the ascending eigenvalues [N x Nk] of the same pass. */
+lqm_status lqm_densitymatrix_dev(lqm_model *m, int Nk, const double *ks_dev,
+                                 const double *kweights_dev, double mu, double T, int nRr,
+                                 const int32_t *Rr, double *rho_dev, double *eband_dev,
+                                 double *bands_dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LQM_GPU_H */

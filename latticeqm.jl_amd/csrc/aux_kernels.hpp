@@ -1,0 +1,227 @@
+// k-grid reductions of the band path on gfx950: the phase contraction that turns the
+// per-k rho_k stack into rho(R), the DOS, chemical-potential reductions, and plain
+// H(k) materialisation.  Every reduction is two-level (fixed split, then a fixed-order
+// fold) so results are bitwise reproducible run to run.
+#pragma once
+#include "lqm_common.hpp"
+
+namespace lqm {
+
+// rho(R)[e] = sum_k w_k e^{-2 pi i k.R} rho_k[e]   (Operators/densitymatrix.jl:141-143)
+// grid: x = element tiles of 256, y = k-splits, z = key groups of RG keys.
+// part[(z*S + y)*RG + r][e] receives the split's partial sum.
+constexpr int kPhaseK = 32;  // k-points whose phases are staged in LDS at a time
+
+template <int RG>
+__global__ __launch_bounds__(256) void k_phase_contract(
+    int NN, int Nkc, int d, const double* __restrict__ ks, const double* __restrict__ kw,
+    double wuni, int nRr, const int* __restrict__ Rr, const cx<double>* __restrict__ rhok,
+    int ksplit, cx<double>* __restrict__ part) {
+    __shared__ cx<double> sph[kPhaseK][RG];
+    const int tid = threadIdx.x;
+    const int e = blockIdx.x * 256 + tid;
+    const int s = blockIdx.y;
+    const int S = gridDim.y;
+    const int rg0 = blockIdx.z * RG;
+    const int nr = min(RG, nRr - rg0);
+    const int k0 = s * ksplit;
+    const int k1 = min(k0 + ksplit, Nkc);
+    cx<double> acc[RG];
+#pragma unroll
+    for (int r = 0; r < RG; ++r) acc[r] = {0.0, 0.0};
+    for (int kb = k0; kb < k1; kb += kPhaseK) {
+        __syncthreads();
+        for (int t = tid; t < kPhaseK * RG; t += 256) {
+            const int kl = t / RG, r = t % RG, k = kb + kl;
+            cx<double> ph = {0.0, 0.0};
+            if (k < k1 && r < nr) {
+                double x = 0.0;
+                for (int a = 0; a < d; ++a) x += ks[(size_t)k * d + a] * (double)Rr[(rg0 + r) * d + a];
+                const double w = kw ? kw[k] : wuni;
+                ph = w * cis2pi(-x);
+            }
+            sph[kl][r] = ph;
+        }
+        __syncthreads();
+        if (e < NN) {
+            const int kn = min(kPhaseK, k1 - kb);
+            for (int kl = 0; kl < kn; ++kl) {
+                const cx<double> v = rhok[(size_t)(kb + kl) * NN + e];
+#pragma unroll
+                for (int r = 0; r < RG; ++r) acc[r] = cmad(acc[r], sph[kl][r], v);
+            }
+        }
+    }
+    if (e < NN) {
+#pragma unroll
+        for (int r = 0; r < RG; ++r)
+            if (r < nr) part[((size_t)(blockIdx.z * S + s) * RG + r) * NN + e] = acc[r];
+    }
+}
+
+// rho[(rg0 + r)][e] += sum_s part[..s..][r][e], fixed order over s.
+template <int RG>
+__global__ void k_phase_fold(int NN, int nRr, int S, const cx<double>* __restrict__ part,
+                             cx<double>* __restrict__ rho) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t total = (size_t)nRr * NN;
+    if (idx >= total) return;
+    const int key = (int)(idx / NN);
+    const int e = (int)(idx % NN);
+    const int g = key / RG, r = key % RG;
+    cx<double> acc = rho[idx];
+    for (int s = 0; s < S; ++s) {
+        const cx<double> v = part[((size_t)(g * S + s) * RG + r) * NN + e];
+        acc.re += v.re;
+        acc.im += v.im;
+    }
+    rho[idx] = acc;
+}
+
+// Sum of n doubles into out[0] (+= when accumulate), one block, fixed order.
+__global__ __launch_bounds__(256) void k_sum(int n, const double* __restrict__ x,
+                                             double* __restrict__ out, int accumulate) {
+    __shared__ double sred[4];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) a += x[i];
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x / 64] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sred[0] + sred[1] + sred[2] + sred[3];
+        out[0] = accumulate ? out[0] + t : t;
+    }
+}
+
+// DOS(w) partials: part[s][o] = sum over eigenvalue slice s of
+// w_k * Im[1/(w_o - i G - e)] = w_k G / ((w_o - e)^2 + G^2)   (Spectrum/dos.jl:185-188)
+__global__ __launch_bounds__(256) void k_dos(int N, int Ne, const double* __restrict__ bands,
+                                             const double* __restrict__ kw, int Nw,
+                                             const double* __restrict__ om, double gamma,
+                                             int esplit, double* __restrict__ part) {
+    __shared__ double se[256];
+    __shared__ double sw[256];
+    const int o = blockIdx.x * 256 + threadIdx.x;
+    const int s = blockIdx.y;
+    const int e0 = s * esplit, e1 = min(e0 + esplit, Ne);
+    const double w_o = o < Nw ? om[o] : 0.0;
+    const double g2 = gamma * gamma;
+    double acc = 0.0;
+    for (int eb = e0; eb < e1; eb += 256) {
+        __syncthreads();
+        const int ei = eb + threadIdx.x;
+        if (ei < e1) {
+            se[threadIdx.x] = bands[ei];
+            sw[threadIdx.x] = kw ? kw[ei / N] : 1.0;
+        }
+        __syncthreads();
+        const int n = min(256, e1 - eb);
+        for (int t = 0; t < n; ++t) {
+            const double x = w_o - se[t];
+            acc += sw[t] * gamma / (x * x + g2);
+        }
+    }
+    if (o < Nw) part[(size_t)s * Nw + o] = acc;
+}
+
+__global__ void k_dos_fold(int Nw, int S, const double* __restrict__ part, double scale,
+                           double* __restrict__ dos) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= Nw) return;
+    double a = 0.0;
+    for (int s = 0; s < S; ++s) a += part[(size_t)s * Nw + o];
+    dos[o] = a * scale;
+}
+
+// getfilling partials (Spectrum/chemicalpotential.jl:74-78): per block,
+// sum_k w_k sum_n f(e_nk - mu) (w_k = 1 when kw == NULL).
+__global__ __launch_bounds__(256) void k_filling(int N, int Ne, const double* __restrict__ bands,
+                                                 const double* __restrict__ kw, double mu,
+                                                 double T, double* __restrict__ part) {
+    __shared__ double sred[4];
+    double a = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < Ne; i += gridDim.x * 256) {
+        const double x = bands[i] - mu;
+        double f;
+        if (T > 0.0) f = 1.0 / (exp(x / T) + 1.0);
+        else f = x < 0.0 ? 1.0 : (x > 0.0 ? 0.0 : 0.5);
+        a += kw ? kw[i / N] * f : f;
+    }
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x / 64] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sred[0] + sred[1] + sred[2] + sred[3];
+}
+
+// min / max of n doubles: part[2*b] = min, part[2*b+1] = max over block b's slice.
+__global__ __launch_bounds__(256) void k_minmax(int n, const double* __restrict__ x,
+                                                double* __restrict__ part) {
+    __shared__ double smin[4], smax[4];
+    double lo = INFINITY, hi = -INFINITY;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        lo = fmin(lo, x[i]);
+        hi = fmax(hi, x[i]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, o));
+        hi = fmax(hi, __shfl_xor(hi, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        smin[threadIdx.x / 64] = lo;
+        smax[threadIdx.x / 64] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = fmin(fmin(smin[0], smin[1]), fmin(smin[2], smin[3]));
+        part[2 * blockIdx.x + 1] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+    }
+}
+
+// Order-preserving map double -> uint64 (total order on non-NaN values).
+__device__ __forceinline__ unsigned long long dkey(double v) {
+    unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// One radix-select pass: histogram of byte `shift` among keys whose bits above the
+// byte equal `prefix` under `mask`.
+__global__ __launch_bounds__(256) void k_radix_hist(int n, const double* __restrict__ x,
+                                                    unsigned long long prefix,
+                                                    unsigned long long mask, int shift,
+                                                    unsigned int* __restrict__ hist) {
+    __shared__ unsigned int sh[256];
+    sh[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const unsigned long long k = dkey(x[i]);
+        if ((k & mask) == prefix) atomicAdd(&sh[(k >> shift) & 0xff], 1u);
+    }
+    __syncthreads();
+    if (sh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], sh[threadIdx.x]);
+}
+
+// Plain H(k) = sum_R T_R e^{2 pi i k.R} (TightBinding/types.jl:18-25), one block per k.
+__global__ __launch_bounds__(256) void k_fouriersum(int N, int d, int nR, const int* __restrict__ Rk,
+                                                    const cx<double>* __restrict__ T,
+                                                    const double* __restrict__ ks,
+                                                    cx<double>* __restrict__ Hk) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<double>* sph = reinterpret_cast<cx<double>*>(smem);
+    const int kk = blockIdx.x;
+    const double* k = ks + (size_t)kk * d;
+    for (int r = threadIdx.x; r < nR; r += 256) {
+        double x = 0.0;
+        for (int a = 0; a < d; ++a) x += k[a] * (double)Rk[r * d + a];
+        sph[r] = cis2pi(x);
+    }
+    __syncthreads();
+    const size_t NN = (size_t)N * N;
+    for (size_t e = threadIdx.x; e < NN; e += 256) {
+        cx<double> h = {0.0, 0.0};
+        for (int r = 0; r < nR; ++r) h = cmad(h, T[r * NN + e], sph[r]);
+        Hk[(size_t)kk * NN + e] = h;
+    }
+}
+
+}  // namespace lqm

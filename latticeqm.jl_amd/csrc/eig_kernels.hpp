@@ -1,0 +1,478 @@
+// Batched Hermitian eigensolver for whole k-grids on gfx950.
+//
+// The path of one k-point (Eigen.jl:14-39 replaces LAPACK zheevr with this):
+//   k_hk_tridiag        H(k) = sum_R t_R e^{2 pi i k.R} assembled straight into the
+//                       workgroup's matrix tile (upper triangle read, as Hermitian(M)
+//                       does) and reduced to real tridiagonal form by Householder
+//                       reflections H_j = I - tau_j v_j v_j^H (one workgroup per k).
+//   k_tridiag_ql        implicit QL with Wilkinson shifts on (d, e), one LANE per
+//                       k-point, so the serial recurrence of 64 k-points runs in one
+//                       wave; optionally logs every Givens rotation.
+//   k_backtransform     replays the rotation log on Z = I, applies the reflectors
+//                       (V = H_0 ... H_{N-2} Z), then either writes the eigenvectors
+//                       or forms rho_k = V diag(f) V^H for the density matrix.
+#pragma once
+#include "lqm_common.hpp"
+
+namespace lqm {
+
+// ---------------------------------------------------------------------------------
+// K_A: Bloch assembly + Householder tridiagonalization.
+// Matrix tile: column-major, leading dimension lda, interleaved complex.  In LDS
+// when it fits (LDSMAT), else in a per-k global workspace.
+// Outputs (per k-point kk of the chunk):
+//   dd[i*Nkc + kk], ee[i*Nkc + kk]   diagonal / off-diagonal, structure-of-arrays so
+//                                    the lane-per-k QL reads them coalesced;
+//   refl[kk*N*N + j*N + i]           v_j at rows i = j+1..N-1 (v_j[j+1] = 1);
+//   tau[kk*N + j]                    tau_j.
+// ---------------------------------------------------------------------------------
+template <typename R, bool LDSMAT>
+__global__ __launch_bounds__(kTPB) void k_hk_tridiag(
+    int N, int d, int nR, const int* __restrict__ Rk, const cx<double>* __restrict__ T,
+    const double* __restrict__ ks, int Nkc, cx<R>* __restrict__ gA, R* __restrict__ dd,
+    R* __restrict__ ee, cx<R>* __restrict__ refl, cx<R>* __restrict__ tau) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int kk = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const int lda = LDSMAT ? N + 1 : N;
+
+    cx<R>* A;
+    unsigned char* p = smem;
+    if constexpr (LDSMAT) {
+        A = reinterpret_cast<cx<R>*>(p);
+        p += sizeof(cx<R>) * (size_t)N * lda;
+    } else {
+        A = gA + (size_t)kk * N * lda;
+    }
+    cx<R>* sv = reinterpret_cast<cx<R>*>(p);
+    p += sizeof(cx<R>) * N;
+    cx<R>* sp = reinterpret_cast<cx<R>*>(p);
+    p += sizeof(cx<R>) * N;
+    cx<double>* sph = reinterpret_cast<cx<double>*>(p);
+    p += sizeof(cx<double>) * nR;
+    cx<R>* sred = reinterpret_cast<cx<R>*>(p);  // [4] wave partials + [2] tau (double-buffered)
+    auto At = [&](int i, int j) -> cx<R>& { return A[(size_t)j * lda + i]; };
+
+    // phases e^{2 pi i k.R}
+    const double* k = ks + (size_t)kk * d;
+    for (int r = tid; r < nR; r += kTPB) {
+        double x = 0.0;
+        for (int a = 0; a < d; ++a) x += k[a] * (double)Rk[r * d + a];
+        sph[r] = cis2pi(x);
+    }
+    __syncthreads();
+
+    // H(k), upper triangle summed in FP64, mirrored into the lower triangle
+    const size_t NN = (size_t)N * N;
+    for (int e = tid; e < N * N; e += kTPB) {
+        int i = e % N, j = e / N;
+        if (i > j) continue;
+        cx<double> h = {0.0, 0.0};
+        const cx<double>* t = T + (size_t)j * N + i;
+        for (int r = 0; r < nR; ++r) h = cmad(h, t[r * NN], sph[r]);
+        if (i == j) {
+            At(i, i) = mkc<R>((R)h.re, (R)0);
+        } else {
+            At(i, j) = mkc<R>((R)h.re, (R)h.im);
+            At(j, i) = mkc<R>((R)h.re, (R)-h.im);
+        }
+    }
+    __syncthreads();
+
+    cx<R>* rf = refl + (size_t)kk * NN;
+    for (int j = 0; j + 1 < N; ++j) {
+        const int m = N - 1 - j;  // length of the reflected column x = A[j+1:N, j]
+        // (a) Householder vector of x, computed by wave 0 (zlarfg semantics:
+        //     H^H x = beta e_1 with beta real).
+        if (wave == 0) {
+            R xn2 = 0;
+            for (int i = j + 2 + lane; i < N; i += kWave) xn2 += abs2(At(i, j));
+            xn2 = wave_sum(xn2);
+            cx<R> alpha = At(j + 1, j);
+            cx<R> tj = {0, 0}, scale = {0, 0};
+            R beta = alpha.re;
+            if (!(xn2 == (R)0 && alpha.im == (R)0)) {
+                R nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+                beta = alpha.re >= (R)0 ? -nrm : nrm;
+                tj = mkc<R>((beta - alpha.re) / beta, -alpha.im / beta);
+                scale = crecip(mkc<R>(alpha.re - beta, alpha.im));
+            }
+            const bool triv = (tj.re == (R)0 && tj.im == (R)0);
+            for (int ip = lane; ip < m; ip += kWave) {
+                cx<R> v = (ip == 0) ? mkc<R>(1, 0) : (triv ? mkc<R>(0, 0) : At(j + 1 + ip, j) * scale);
+                sv[ip] = v;
+                rf[(size_t)j * N + j + 1 + ip] = v;
+            }
+            if (lane == 0) {
+                sred[4 + (j & 1)] = tj;
+                tau[(size_t)kk * N + j] = tj;
+                dd[(size_t)j * Nkc + kk] = At(j, j).re;
+                ee[(size_t)j * Nkc + kk] = beta;
+            }
+        }
+        __syncthreads();
+        // tau slot alternates so a wave still reading step j never races wave 0's
+        // write for step j+1 on the skip path (no barrier there)
+        const cx<R> tj = sred[4 + (j & 1)];
+        if (tj.re == (R)0 && tj.im == (R)0) continue;  // H_j = I (uniform branch)
+
+        // (b) p = tau * A22 v  (4 lanes per row, shuffle-reduced) and partial p^H v
+        cx<R> dotp = {0, 0};
+        {
+            const int q = tid & 3;
+            for (int r = tid >> 2; r < m; r += kTPB / 4) {
+                cx<R> acc = {0, 0};
+                const cx<R>* arow = &At(j + 1 + r, j + 1);
+                for (int c = q; c < m; c += 4) acc = cmad(acc, arow[(size_t)c * lda], sv[c]);
+                acc.re += __shfl_xor(acc.re, 1);
+                acc.im += __shfl_xor(acc.im, 1);
+                acc.re += __shfl_xor(acc.re, 2);
+                acc.im += __shfl_xor(acc.im, 2);
+                if (q == 0) {
+                    cx<R> pr = tj * acc;
+                    sp[r] = pr;
+                    dotp = cmadc(dotp, pr, sv[r]);
+                }
+            }
+        }
+        dotp = wave_sum(dotp);
+        if (lane == 0) sred[wave] = dotp;
+        __syncthreads();
+        cx<R> dot = sred[0] + sred[1] + sred[2] + sred[3];
+        const cx<R> alpha2 = (R)-0.5 * (tj * dot);
+
+        // (c) A22 -= v w^H + w v^H with w = p + alpha2 v
+        {
+            const int rr = tid & (kWave - 1);
+            const int cq = tid / kWave;
+            for (int r = rr; r < m; r += kWave) {
+                const cx<R> vr = sv[r];
+                const cx<R> wr = cmad(sp[r], alpha2, vr);
+                for (int c = cq; c < m; c += kTPB / kWave) {
+                    const cx<R> vc = sv[c];
+                    const cx<R> wc = cmad(sp[c], alpha2, vc);
+                    cx<R> a = At(j + 1 + r, j + 1 + c);
+                    cx<R> u = {0, 0};
+                    u = cmadcb(u, vr, wc);
+                    u = cmadcb(u, wr, vc);
+                    At(j + 1 + r, j + 1 + c) = a - u;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        dd[(size_t)(N - 1) * Nkc + kk] = At(N - 1, N - 1).re;
+        ee[(size_t)(N - 1) * Nkc + kk] = (R)0;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// K_B: implicit QL on the real symmetric tridiagonal (d, e), one lane per k-point.
+// Working copies live in LDS as [i][lane] so that lanes at the same index hit
+// consecutive banks.  Each QL sweep is logged as a header (first index << 16 | count)
+// followed by its (c, s) pairs: the rotation at index i maps columns (i, i+1) of Z to
+//   z_{i+1} <- s z_i + c z_{i+1},   z_i <- c z_i - s z_{i+1}.
+// Eigenvalues are written ascending (evals[kk*N + n]) with perm[kk*N + n] = the Z
+// column that belongs to them.
+// ---------------------------------------------------------------------------------
+template <typename R, bool LOG>
+__global__ void k_tridiag_ql(int N, int Nkc, const R* __restrict__ dd, const R* __restrict__ ee,
+                             R* __restrict__ evals, int* __restrict__ perm, R* __restrict__ rot,
+                             int* __restrict__ sweeps, int* __restrict__ nsweeps, int cap_rot,
+                             int cap_sw, int* __restrict__ status) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int L = blockDim.x;
+    const int lane = threadIdx.x;
+    const int kk = blockIdx.x * L + lane;
+    R* sd = reinterpret_cast<R*>(smem);
+    R* se = sd + (size_t)N * L;
+    if (kk >= Nkc) return;  // no barriers below: early exit is safe
+    auto D = [&](int i) -> R& { return sd[(size_t)i * L + lane]; };
+    auto E = [&](int i) -> R& { return se[(size_t)i * L + lane]; };
+    for (int i = 0; i < N; ++i) {
+        D(i) = dd[(size_t)i * Nkc + kk];
+        E(i) = ee[(size_t)i * Nkc + kk];
+    }
+    int flag = 0;
+    R* lrot = LOG ? rot + (size_t)kk * cap_rot * 2 : nullptr;
+    int* lsw = LOG ? sweeps + (size_t)kk * cap_sw : nullptr;
+    int pos = 0, nsw = 0;
+    const R heps = num<R>::half_eps;
+    for (int l = 0; l < N; ++l) {
+        int iter = 0;
+        int m;
+        for (;;) {
+            for (m = l; m < N - 1; ++m) {
+                R dsum = fabs(D(m)) + fabs(D(m + 1));
+                if (fabs(E(m)) <= heps * dsum) break;
+            }
+            if (m == l) break;
+            if (++iter > 30) {
+                flag |= kStatNoConv;
+                break;
+            }
+            // Wilkinson shift from the leading 2x2 block
+            R g = (D(l + 1) - D(l)) / (R(2) * E(l));
+            R r = hypot(g, R(1));
+            g = D(m) - D(l) + E(l) / (g + copysign(r, g));
+            R s = 1, c = 1, pacc = 0;
+            int i;
+            const bool logit = LOG && (pos + (m - l) <= cap_rot) && (nsw < cap_sw);
+            if (LOG && !logit) flag |= kStatLogFull;
+            int cnt = 0;
+            bool broke = false;
+            for (i = m - 1; i >= l; --i) {
+                R f = s * E(i);
+                R b = c * E(i);
+                r = hypot(f, g);
+                E(i + 1) = r;
+                if (r == R(0)) {
+                    D(i + 1) -= pacc;
+                    E(m) = 0;
+                    broke = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = D(i + 1) - pacc;
+                r = (D(i) - g) * s + R(2) * c * b;
+                pacc = s * r;
+                D(i + 1) = g + pacc;
+                g = c * r - b;
+                if (logit) {
+                    lrot[2 * (pos + cnt)] = c;
+                    lrot[2 * (pos + cnt) + 1] = s;
+                }
+                ++cnt;
+            }
+            if (logit && cnt > 0) {
+                lsw[nsw++] = ((m - 1) << 16) | cnt;
+                pos += cnt;
+            }
+            if (broke) continue;
+            D(l) -= pacc;
+            E(l) = g;
+            E(m) = 0;
+        }
+        if (flag & kStatNoConv) break;
+    }
+    // ascending order by rank counting (ties broken by index)
+    for (int i = 0; i < N; ++i) {
+        R di = D(i);
+        if (!(di == di) || isinf(di)) flag |= kStatNonFinite;
+        int rank = 0;
+        for (int jx = 0; jx < N; ++jx) {
+            R dj = D(jx);
+            rank += (dj < di) || (dj == di && jx < i);
+        }
+        rank = rank < N ? rank : N - 1;
+        evals[(size_t)kk * N + rank] = di;
+        if (perm) perm[(size_t)kk * N + rank] = i;
+    }
+    if (LOG) nsweeps[kk] = nsw;
+    if (flag) atomicOr(status, flag);
+}
+
+// Fermi-Dirac occupation (Utils/fermidirac.jl:1-5): T > 0 logistic, T = 0 step
+// with 1/2 exactly at zero.
+__device__ __forceinline__ double fermidirac(double e, double mu, double T) {
+    double x = e - mu;
+    if (T > 0.0) return 1.0 / (exp(x / T) + 1.0);
+    return x < 0.0 ? 1.0 : (x > 0.0 ? 0.0 : 0.5);
+}
+
+// ---------------------------------------------------------------------------------
+// K_C: eigenvectors (and rho_k).  V is kept planar (real and imaginary planes,
+// leading dimension ldv) so lanes walking consecutive rows hit consecutive banks.
+//   1. Z = I; replay the QL sweeps, one row of Z per thread (carry trick: one LDS
+//      read and one write per rotation);
+//   2. V = H_0 H_1 ... H_{N-2} Z, reflectors applied from the last one;
+//   3. MODE 0: write evecs (ascending order); MODE 1: rho_k = V diag(f) V^H with
+//      f = fermidirac(e - mu; T), written as FP64, and eb[kk] = w_k sum_n e_n f_n.
+// ---------------------------------------------------------------------------------
+constexpr int kRotChunk = 512;
+
+template <typename R, bool LDSMAT, int MODE>
+__global__ __launch_bounds__(kTPB) void k_backtransform(
+    int N, int Nkc, const cx<R>* __restrict__ refl, const cx<R>* __restrict__ tau,
+    const R* __restrict__ rot, const int* __restrict__ sweeps, const int* __restrict__ nsweeps,
+    int cap_rot, int cap_sw, const R* __restrict__ evals, const int* __restrict__ perm,
+    const double* __restrict__ kw, double wuni, double mu, double Temp, R* __restrict__ gV,
+    cx<double>* __restrict__ out, double* __restrict__ eb) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int kk = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int ldv = LDSMAT ? N + 1 : N;
+    const size_t plane = (size_t)N * ldv;
+    unsigned char* p = smem;
+    R* Vr;
+    if constexpr (LDSMAT) {
+        Vr = reinterpret_cast<R*>(p);
+        p += sizeof(R) * 2 * plane;
+    } else {
+        Vr = gV + (size_t)kk * 2 * plane;
+    }
+    R* Vi = Vr + plane;
+    R* srot = reinterpret_cast<R*>(p);          // [kRotChunk][2]
+    p += sizeof(R) * 2 * kRotChunk;
+    cx<R>* sv = reinterpret_cast<cx<R>*>(p);    // [N]
+    p += sizeof(cx<R>) * N;
+    cx<R>* sy = reinterpret_cast<cx<R>*>(p);    // [N]
+    p += sizeof(cx<R>) * N;
+    double* sf = reinterpret_cast<double*>(p);  // [N]
+    p += sizeof(double) * N;
+    double* sred = reinterpret_cast<double*>(p);  // [4]
+
+    // 1. Z = I, then the logged QL rotations
+    for (int e = tid; e < N * N; e += kTPB) {
+        int i = e % N, c = e / N;
+        Vr[(size_t)c * ldv + i] = (i == c) ? R(1) : R(0);
+        Vi[(size_t)c * ldv + i] = R(0);
+    }
+    __syncthreads();
+    const R* lrot = rot + (size_t)kk * cap_rot * 2;
+    const int* lsw = sweeps + (size_t)kk * cap_sw;
+    const int nsw = nsweeps[kk];
+    int pos = 0;
+    int cbase = -kRotChunk;  // first log position held in srot
+    for (int sidx = 0; sidx < nsw; ++sidx) {
+        const int h = lsw[sidx];
+        const int i0 = h >> 16, cnt = h & 0xffff;
+        for (int row = 0; row * kTPB < N || row == 0; ++row) {
+            // every thread walks the sweep so the chunk reloads stay uniform
+            const int rr = row * kTPB + tid;
+            const bool act = rr < N;
+            R carry = act ? Vr[(size_t)(i0 + 1) * ldv + rr] : R(0);
+            for (int t = 0; t < cnt; ++t) {
+                const int pp = pos + t;
+                if (pp >= cbase + kRotChunk || pp < cbase) {
+                    __syncthreads();
+                    const int nload = min(kRotChunk, cap_rot - pp);
+                    for (int q = tid; q < 2 * nload; q += kTPB) srot[q] = lrot[2 * (size_t)pp + q];
+                    cbase = pp;
+                    __syncthreads();
+                }
+                const R c = srot[2 * (pp - cbase)], s = srot[2 * (pp - cbase) + 1];
+                const int i = i0 - t;
+                if (act) {
+                    R zi = Vr[(size_t)i * ldv + rr];
+                    Vr[(size_t)(i + 1) * ldv + rr] = s * zi + c * carry;
+                    carry = c * zi - s * carry;
+                }
+            }
+            if (act) Vr[(size_t)(i0 - cnt + 1) * ldv + rr] = carry;
+            if ((row + 1) * kTPB >= N) break;
+        }
+        pos += cnt;
+    }
+    __syncthreads();
+
+    // 2. V = H_0 ... H_{N-2} Z
+    const cx<R>* rf = refl + (size_t)kk * N * N;
+    for (int j = N - 2; j >= 0; --j) {
+        const int m = N - 1 - j;
+        const cx<R> tj = tau[(size_t)kk * N + j];
+        if (tj.re == R(0) && tj.im == R(0)) continue;
+        for (int i = tid; i < m; i += kTPB) sv[i] = rf[(size_t)j * N + j + 1 + i];
+        __syncthreads();
+        // y_c = tau * v^H V[j+1:, c]   (4 lanes per column)
+        {
+            const int q = tid & 3;
+            for (int c = tid >> 2; c < N; c += kTPB / 4) {
+                cx<R> acc = {0, 0};
+                const R* vr = Vr + (size_t)c * ldv + j + 1;
+                const R* vi = Vi + (size_t)c * ldv + j + 1;
+                for (int i = q; i < m; i += 4) acc = cmadc(acc, sv[i], mkc<R>(vr[i], vi[i]));
+                acc.re += __shfl_xor(acc.re, 1);
+                acc.im += __shfl_xor(acc.im, 1);
+                acc.re += __shfl_xor(acc.re, 2);
+                acc.im += __shfl_xor(acc.im, 2);
+                if (q == 0) sy[c] = tj * acc;
+            }
+        }
+        __syncthreads();
+        // V[j+1:, :] -= v y
+        {
+            const int rr = tid & (kWave - 1);
+            const int cq = tid / kWave;
+            for (int r = rr; r < m; r += kWave) {
+                const cx<R> vr = sv[r];
+                for (int c = cq; c < N; c += kTPB / kWave) {
+                    const size_t o = (size_t)c * ldv + j + 1 + r;
+                    cx<R> u = vr * sy[c];
+                    Vr[o] -= u.re;
+                    Vi[o] -= u.im;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    const R* ev = evals + (size_t)kk * N;
+    const int* pm = perm + (size_t)kk * N;
+    const size_t NN = (size_t)N * N;
+    if constexpr (MODE == 0) {
+        cx<double>* o = out + (size_t)kk * NN;
+        for (int e = tid; e < N * N; e += kTPB) {
+            int i = e % N, n = e / N;
+            int c = pm[n];
+            o[e] = mkc<double>((double)Vr[(size_t)c * ldv + i], (double)Vi[(size_t)c * ldv + i]);
+        }
+    } else {
+        // occupations per Z column and the band-energy term
+        double ebp = 0.0;
+        for (int n = tid; n < N; n += kTPB) {
+            double en = (double)ev[n];
+            double f = fermidirac(en, mu, Temp);
+            sf[pm[n]] = f;
+            ebp += en * f;
+        }
+        ebp = wave_sum(ebp);
+        if ((tid & (kWave - 1)) == 0) sred[tid / kWave] = ebp;
+        __syncthreads();
+        if (tid == 0) {
+            double w = kw ? kw[kk] : wuni;
+            eb[kk] = w * (sred[0] + sred[1] + sred[2] + sred[3]);
+        }
+        // rho_k(i, j) = sum_c f_c V(i, c) conj(V(j, c)), 4x4 register blocks
+        const int nb = (N + 3) / 4;
+        cx<double>* o = out + (size_t)kk * NN;
+        for (int b = tid; b < nb * nb; b += kTPB) {
+            const int bi = b % nb, bj = b / nb;
+            cx<R> acc[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = mkc<R>(0, 0);
+            for (int c = 0; c < N; ++c) {
+                const R f = (R)sf[c];
+                if (f == R(0)) continue;
+                cx<R> a[4], bb[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    int i = 4 * bi + u;
+                    int jj = 4 * bj + u;
+                    a[u] = (i < N) ? mkc<R>(Vr[(size_t)c * ldv + i], Vi[(size_t)c * ldv + i]) : mkc<R>(0, 0);
+                    bb[u] = (jj < N) ? f * mkc<R>(Vr[(size_t)c * ldv + jj], Vi[(size_t)c * ldv + jj]) : mkc<R>(0, 0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[u][v] = cmadcb(acc[u][v], a[u], bb[v]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    int i = 4 * bi + u, jj = 4 * bj + v;
+                    if (i < N && jj < N)
+                        o[(size_t)jj * N + i] = mkc<double>((double)acc[u][v].re, (double)acc[u][v].im);
+                }
+        }
+    }
+}
+
+}  // namespace lqm

@@ -1,0 +1,106 @@
+// Shared device helpers for the gfx950 k-space band path: a minimal complex type,
+// wave64 reductions and the phase factor e^{2 pi i x}.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lqm {
+
+constexpr int kWave = 64;   // CDNA wavefront width
+constexpr int kTPB = 256;   // threads per block of the per-k kernels (4 waves)
+
+template <typename R>
+struct cx {
+    R re, im;
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> mkc(R a, R b) { return cx<R>{a, b}; }
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> operator*(cx<R> a, cx<R> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> operator*(R s, cx<R> a) { return {s * a.re, s * a.im}; }
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> conj(cx<R> a) { return {a.re, -a.im}; }
+template <typename R>
+__host__ __device__ __forceinline__ R abs2(cx<R> a) { return a.re * a.re + a.im * a.im; }
+// acc + a*b with fused multiply-adds
+template <typename R>
+__device__ __forceinline__ cx<R> cmad(cx<R> acc, cx<R> a, cx<R> b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+    return acc;
+}
+// acc + conj(a)*b
+template <typename R>
+__device__ __forceinline__ cx<R> cmadc(cx<R> acc, cx<R> a, cx<R> b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(-a.im, b.re, acc.im);
+    return acc;
+}
+// acc + a*conj(b)
+template <typename R>
+__device__ __forceinline__ cx<R> cmadcb(cx<R> acc, cx<R> a, cx<R> b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(a.im, b.im, acc.re);
+    acc.im = fma(a.im, b.re, acc.im);
+    acc.im = fma(-a.re, b.im, acc.im);
+    return acc;
+}
+// 1/z
+template <typename R>
+__device__ __forceinline__ cx<R> crecip(cx<R> z) {
+    R den = z.re * z.re + z.im * z.im;
+    return {z.re / den, -z.im / den};
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+template <typename R>
+__device__ __forceinline__ cx<R> wave_sum(cx<R> v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        v.re += __shfl_xor(v.re, o);
+        v.im += __shfl_xor(v.im, o);
+    }
+    return v;
+}
+
+// e^{2 pi i x} in FP64 (sincospi keeps the argument reduction exact).
+__device__ __forceinline__ cx<double> cis2pi(double x) {
+    double s, c;
+    sincospi(2.0 * x, &s, &c);
+    return {c, s};
+}
+
+// Numerical-type constants used by the eigensolver.
+template <typename R> struct num;
+template <> struct num<double> {
+    static constexpr double half_eps = 1.1102230246251565e-16;  // 2^-53
+};
+template <> struct num<float> {
+    static constexpr float half_eps = 5.9604645e-08f;  // 2^-24
+};
+
+// Device-side status bits, OR-ed into a per-call flag word.
+enum : int {
+    kStatNoConv = 1,       // QL exceeded its iteration budget
+    kStatLogFull = 2,      // rotation log capacity exceeded
+    kStatNonFinite = 4,    // NaN/Inf met
+};
+
+}  // namespace lqm

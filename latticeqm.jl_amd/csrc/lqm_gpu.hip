@@ -1,0 +1,754 @@
+// liblqmgpu.so: host side of the gfx950 k-space band path behind the C ABI in
+// include/lqm_gpu.h.  One context = one device + one stream + a grow-only workspace;
+// one model = one Hops operator resident in HBM.  Work over a k-grid is cut into
+// chunks sized to a workspace budget; each chunk runs
+//   K_A k_hk_tridiag -> K_B k_tridiag_ql -> [K_C k_backtransform] -> [K_D phase contraction]
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lqm_gpu.h"
+#include "aux_kernels.hpp"
+#include "eig_kernels.hpp"
+
+using lqm::cx;
+
+namespace {
+
+thread_local std::string g_err;
+
+lqm_status fail(lqm_status st, const std::string& msg) {
+    g_err = msg;
+    return st;
+}
+
+#define LQM_HIP(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(LQM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+enum Slot {
+    W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
+    W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_NSLOTS
+};
+
+constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
+
+}  // namespace
+
+struct lqm_context {
+    int device = 0;
+    int precision = 0;
+    int lds_max = 65536;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    void* buf[W_NSLOTS] = {};
+    size_t cap[W_NSLOTS] = {};
+    int* d_status = nullptr;
+};
+
+struct lqm_model {
+    lqm_context* ctx = nullptr;
+    int N = 0, d = 0, nR = 0;
+    std::vector<int> R;
+    int* dR = nullptr;
+    cx<double>* dT = nullptr;
+};
+
+namespace {
+
+lqm_status ws(lqm_context* c, Slot s, size_t bytes, void** out) {
+    if (bytes == 0) bytes = 16;
+    if (c->cap[s] < bytes) {
+        if (c->buf[s]) {
+            hipError_t e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) return fail(LQM_ERR_DEVICE, hipGetErrorString(e));
+            (void)hipFree(c->buf[s]);
+            c->buf[s] = nullptr;
+            c->cap[s] = 0;
+        }
+        hipError_t e = hipMalloc(&c->buf[s], bytes);
+        if (e != hipSuccess)
+            return fail(LQM_ERR_DEVICE, "workspace allocation of " + std::to_string(bytes) +
+                                            " bytes failed: " + hipGetErrorString(e));
+        c->cap[s] = bytes;
+    }
+    *out = c->buf[s];
+    return LQM_OK;
+}
+
+template <typename T>
+lqm_status wsT(lqm_context* c, Slot s, size_t count, T** out) {
+    void* p = nullptr;
+    lqm_status st = ws(c, s, count * sizeof(T), &p);
+    *out = static_cast<T*>(p);
+    return st;
+}
+
+struct Plan {
+    int N = 0, nR = 0;
+    bool ldsA = false, ldsC = false;
+    size_t shA = 0, shC = 0, shB = 0;
+    int laneB = 64;
+    int cap_rot = 0, cap_sw = 0;
+    int Kc = 0;  // k-points per chunk
+};
+
+Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool rho) {
+    Plan p;
+    p.N = N;
+    p.nR = nR;
+    const size_t cR = sizeof(double);
+    p.shA = sizeof(cx<double>) * ((size_t)N * (N + 1) + 2 * N + 6) + sizeof(cx<double>) * nR;
+    p.ldsA = p.shA <= (size_t)c->lds_max;
+    if (!p.ldsA) p.shA = sizeof(cx<double>) * (2 * N + 6) + sizeof(cx<double>) * nR;
+    const size_t smallC = cR * 2 * lqm::kRotChunk + sizeof(cx<double>) * 2 * N + 8 * N + 64;
+    p.shC = 2 * cR * (size_t)N * (N + 1) + smallC;
+    p.ldsC = p.shC <= (size_t)c->lds_max;
+    if (!p.ldsC) p.shC = smallC;
+    int L = 64;
+    while (L > 1 && 2 * cR * (size_t)N * L > (size_t)c->lds_max) L >>= 1;
+    p.laneB = L;
+    p.shB = 2 * cR * (size_t)N * L;
+    p.cap_rot = 3 * N * N + 64 * N;
+    p.cap_sw = 30 * N + 8;
+    const size_t NN = (size_t)N * N;
+    size_t perk = 16 * NN + 16 * N + 16 * N + 8 * N + 4 * N + 8;
+    if (vectors || rho) perk += 16 * (size_t)p.cap_rot + 4 * (size_t)p.cap_sw + 4;
+    if (!p.ldsA) perk += 16 * NN;
+    if ((vectors || rho) && !p.ldsC) perk += 16 * NN;
+    if (rho) perk += 16 * NN;
+    size_t kc = kWorkspaceBudget / perk;
+    if (kc < 1) kc = 1;
+    p.Kc = (int)std::min<size_t>(kc, (size_t)Nk);
+    return p;
+}
+
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+    if (bytes > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+// Eigenvalues (and optionally the QL log) of one chunk: K_A then K_B.
+lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks, double* evals,
+                          bool logq) {
+    lqm_context* c = m->ctx;
+    const int N = m->N;
+    double *dd, *ee;
+    cx<double>*refl, *tau, *gA = nullptr;
+    int* perm = nullptr;
+    double* rot = nullptr;
+    int *sw = nullptr, *nsw = nullptr;
+    lqm_status st;
+    if ((st = wsT(c, W_D, (size_t)N * Kc, &dd))) return st;
+    if ((st = wsT(c, W_E, (size_t)N * Kc, &ee))) return st;
+    if ((st = wsT(c, W_REFL, (size_t)N * N * Kc, &refl))) return st;
+    if ((st = wsT(c, W_TAU, (size_t)N * Kc, &tau))) return st;
+    if ((st = wsT(c, W_PERM, (size_t)N * Kc, &perm))) return st;
+    if (!P.ldsA && (st = wsT(c, W_GA, (size_t)N * N * Kc, &gA))) return st;
+    if (logq) {
+        if ((st = wsT(c, W_ROT, (size_t)2 * P.cap_rot * Kc, &rot))) return st;
+        if ((st = wsT(c, W_SW, (size_t)P.cap_sw * Kc, &sw))) return st;
+        if ((st = wsT(c, W_NSW, (size_t)Kc, &nsw))) return st;
+    }
+    if (P.ldsA) {
+        auto kA = lqm::k_hk_tridiag<double, true>;
+        allow_lds(kA, P.shA);
+        hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kTPB), P.shA, c->stream, N, m->d, m->nR, m->dR,
+                           m->dT, ks, Kc, gA, dd, ee, refl, tau);
+    } else {
+        auto kA = lqm::k_hk_tridiag<double, false>;
+        allow_lds(kA, P.shA);
+        hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kTPB), P.shA, c->stream, N, m->d, m->nR, m->dR,
+                           m->dT, ks, Kc, gA, dd, ee, refl, tau);
+    }
+    LQM_HIP(hipGetLastError());
+    const int L = P.laneB;
+    const int gB = (Kc + L - 1) / L;
+    if (logq) {
+        auto kB = lqm::k_tridiag_ql<double, true>;
+        allow_lds(kB, P.shB);
+        hipLaunchKernelGGL(kB, dim3(gB), dim3(L), P.shB, c->stream, N, Kc, dd, ee, evals, perm, rot,
+                           sw, nsw, P.cap_rot, P.cap_sw, c->d_status);
+    } else {
+        auto kB = lqm::k_tridiag_ql<double, false>;
+        allow_lds(kB, P.shB);
+        hipLaunchKernelGGL(kB, dim3(gB), dim3(L), P.shB, c->stream, N, Kc, dd, ee, evals, perm, rot,
+                           sw, nsw, P.cap_rot, P.cap_sw, c->d_status);
+    }
+    LQM_HIP(hipGetLastError());
+    return LQM_OK;
+}
+
+template <int MODE>
+lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const double* evals,
+                                const double* kw, double wuni, double mu, double T,
+                                cx<double>* out, double* eb) {
+    lqm_context* c = m->ctx;
+    const int N = m->N;
+    double* gV = nullptr;
+    if (!P.ldsC) {
+        lqm_status st = wsT(c, W_GV, (size_t)2 * N * N * Kc, &gV);
+        if (st) return st;
+    }
+    auto* refl = static_cast<cx<double>*>(c->buf[W_REFL]);
+    auto* tau = static_cast<cx<double>*>(c->buf[W_TAU]);
+    auto* rot = static_cast<double*>(c->buf[W_ROT]);
+    auto* sw = static_cast<int*>(c->buf[W_SW]);
+    auto* nsw = static_cast<int*>(c->buf[W_NSW]);
+    auto* perm = static_cast<int*>(c->buf[W_PERM]);
+    if (P.ldsC) {
+        auto kC = lqm::k_backtransform<double, true, MODE>;
+        allow_lds(kC, P.shC);
+        hipLaunchKernelGGL(kC, dim3(Kc), dim3(lqm::kTPB), P.shC, c->stream, N, Kc, refl, tau, rot, sw,
+                           nsw, P.cap_rot, P.cap_sw, evals, perm, kw, wuni, mu, T, gV, out, eb);
+    } else {
+        auto kC = lqm::k_backtransform<double, false, MODE>;
+        allow_lds(kC, P.shC);
+        hipLaunchKernelGGL(kC, dim3(Kc), dim3(lqm::kTPB), P.shC, c->stream, N, Kc, refl, tau, rot, sw,
+                           nsw, P.cap_rot, P.cap_sw, evals, perm, kw, wuni, mu, T, gV, out, eb);
+    }
+    LQM_HIP(hipGetLastError());
+    return LQM_OK;
+}
+
+template <int RG>
+void launch_contract(hipStream_t s, int NN, int Kc, int d, const double* ks, const double* kw,
+                     double wuni, int nRr, const int* Rr, const cx<double>* rhok, int ksplit,
+                     int S, int groups, cx<double>* part, cx<double>* rho) {
+    dim3 grid((NN + 255) / 256, S, groups);
+    hipLaunchKernelGGL(lqm::k_phase_contract<RG>, grid, dim3(256), 0, s, NN, Kc, d, ks, kw, wuni, nRr,
+                       Rr, rhok, ksplit, part);
+    const size_t total = (size_t)nRr * NN;
+    hipLaunchKernelGGL(lqm::k_phase_fold<RG>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       NN, nRr, S, part, rho);
+}
+
+lqm_status check_status(lqm_context* c, const char* what) {
+    int h = 0;
+    LQM_HIP(hipMemcpyAsync(&h, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    if (h) {
+        LQM_HIP(hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+        if (h & lqm::kStatNoConv)
+            return fail(LQM_ERR_NOCONV, std::string(what) + ": tridiagonal QL did not converge");
+        if (h & lqm::kStatLogFull)
+            return fail(LQM_ERR_NOCONV, std::string(what) + ": QL rotation log capacity exceeded");
+        if (h & lqm::kStatNonFinite)
+            return fail(LQM_ERR_NONFINITE, std::string(what) + ": non-finite eigenvalue (NaN/Inf in H(k))");
+    }
+    return LQM_OK;
+}
+
+lqm_status check_model(lqm_model* m) {
+    if (!m || !m->ctx) return fail(LQM_ERR_ARG, "null model");
+    return LQM_OK;
+}
+
+// bands (+ evecs) over the whole grid, device pointers.
+lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, double* evecs) {
+    const int N = m->N;
+    if (Nk < 0) return fail(LQM_ERR_ARG, "Nk < 0");
+    if (Nk == 0) return LQM_OK;
+    if (!ks || !bands) return fail(LQM_ERR_ARG, "null ks or bands");
+    Plan P = make_plan(m->ctx, N, m->nR, Nk, evecs != nullptr, false);
+    for (int k0 = 0; k0 < Nk; k0 += P.Kc) {
+        const int Kc = std::min(P.Kc, Nk - k0);
+        const double* ksc = ks + (size_t)k0 * m->d;
+        double* ev = bands + (size_t)k0 * N;
+        lqm_status st = launch_eigvals(m, P, Kc, ksc, ev, evecs != nullptr);
+        if (st) return st;
+        if (evecs) {
+            st = launch_backtransform<0>(m, P, Kc, ev, nullptr, 0.0, 0.0, 0.0,
+                                         reinterpret_cast<cx<double>*>(evecs) + (size_t)k0 * N * N,
+                                         nullptr);
+            if (st) return st;
+        }
+    }
+    return check_status(m->ctx, "bandmatrix");
+}
+
+lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
+                   int nRr, const int32_t* Rr, double* rho, double* eband, double* bands_out) {
+    lqm_context* c = m->ctx;
+    const int N = m->N, d = m->d;
+    if (Nk <= 0) return fail(LQM_ERR_ARG, "Nk must be positive");
+    if (nRr < 0 || (nRr > 0 && !Rr)) return fail(LQM_ERR_ARG, "bad rho key list");
+    if (!ks || !rho || !eband) return fail(LQM_ERR_ARG, "null ks, rho or eband");
+    const size_t NN = (size_t)N * N;
+    LQM_HIP(hipMemsetAsync(rho, 0, sizeof(cx<double>) * NN * std::max(nRr, 0), c->stream));
+    LQM_HIP(hipMemsetAsync(eband, 0, sizeof(double), c->stream));
+    int* dRr = nullptr;
+    lqm_status st;
+    if ((st = wsT(c, W_RR, (size_t)std::max(1, nRr * d), &dRr))) return st;
+    if (nRr > 0)
+        LQM_HIP(hipMemcpyAsync(dRr, Rr, sizeof(int) * nRr * d, hipMemcpyHostToDevice, c->stream));
+    Plan P = make_plan(c, N, m->nR, Nk, true, true);
+    const int RG = nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : 32;
+    const int groups = std::max(1, (nRr + RG - 1) / RG);
+    const int tilesE = (int)((NN + 255) / 256);
+    int S = std::max(1, std::min(P.Kc, 1024 / std::max(1, tilesE * groups)));
+    cx<double>* part = nullptr;
+    if ((st = wsT(c, W_PART, (size_t)groups * S * RG * NN, &part))) return st;
+    const double wuni = 1.0 / (double)Nk;
+    for (int k0 = 0; k0 < Nk; k0 += P.Kc) {
+        const int Kc = std::min(P.Kc, Nk - k0);
+        const double* ksc = ks + (size_t)k0 * d;
+        const double* kwc = kw ? kw + k0 : nullptr;
+        double* ev = nullptr;
+        if (bands_out) {
+            ev = bands_out + (size_t)k0 * N;
+        } else if ((st = wsT(c, W_EVALS, (size_t)N * P.Kc, &ev))) {
+            return st;
+        }
+        if ((st = launch_eigvals(m, P, Kc, ksc, ev, true))) return st;
+        cx<double>* rhok;
+        double* eb;
+        if ((st = wsT(c, W_RHOK, NN * P.Kc, &rhok))) return st;
+        if ((st = wsT(c, W_EB, (size_t)P.Kc, &eb))) return st;
+        if ((st = launch_backtransform<1>(m, P, Kc, ev, kwc, wuni, mu, T, rhok, eb))) return st;
+        hipLaunchKernelGGL(lqm::k_sum, dim3(1), dim3(256), 0, c->stream, Kc, eb, eband, 1);
+        if (nRr > 0) {
+            const int Sc = std::min(S, Kc);
+            const int ksplit = (Kc + Sc - 1) / Sc;
+            const int Sused = (Kc + ksplit - 1) / ksplit;
+            auto* rh = reinterpret_cast<cx<double>*>(rho);
+            switch (RG) {
+                case 1: launch_contract<1>(c->stream, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
+                case 4: launch_contract<4>(c->stream, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
+                case 8: launch_contract<8>(c->stream, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
+                case 16: launch_contract<16>(c->stream, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
+                default: launch_contract<32>(c->stream, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
+            }
+            LQM_HIP(hipGetLastError());
+        }
+    }
+    return check_status(c, "densitymatrix");
+}
+
+lqm_status dos_bands_dev(lqm_context* c, int N, int Nk, const double* bands, const double* kw,
+                         int Nw, const double* om, double gamma, double* dos) {
+    if (Nw <= 0) return LQM_OK;
+    const int Ne = N * Nk;
+    const int tilesO = (Nw + 255) / 256;
+    int S = std::max(1, std::min((Ne + 255) / 256, 1024 / tilesO));
+    int esplit = (Ne + S - 1) / S;
+    S = (Ne + esplit - 1) / esplit;
+    double* part;
+    lqm_status st = wsT(c, W_PART, (size_t)S * Nw + 1, &part);
+    if (st) return st;
+    hipLaunchKernelGGL(lqm::k_dos, dim3(tilesO, S), dim3(256), 0, c->stream, N, Ne, bands, kw, Nw, om,
+                       gamma, esplit, part);
+    const double scale = kw ? 1.0 : 1.0 / (double)Nk;
+    hipLaunchKernelGGL(lqm::k_dos_fold, dim3(tilesO), dim3(256), 0, c->stream, Nw, S, part, scale,
+                       dos);
+    LQM_HIP(hipGetLastError());
+    return LQM_OK;
+}
+
+// Roots.jl-style midpoint: 0 across a sign change, else the midpoint of the
+// bit patterns of |a| and |b| (so bisection ends in <= 64 steps).
+double bit_middle(double a, double b) {
+    if ((a < 0) != (b < 0) && a != 0 && b != 0) return 0.0;
+    uint64_t ia, ib;
+    double aa = std::fabs(a), bb = std::fabs(b);
+    std::memcpy(&ia, &aa, 8);
+    std::memcpy(&ib, &bb, 8);
+    uint64_t im = (ia >> 1) + (ib >> 1) + (ia & ib & 1);
+    double mid;
+    std::memcpy(&mid, &im, 8);
+    return (a + b) < 0 ? -mid : mid;
+}
+
+lqm_status filling_eval(lqm_context* c, int N, int Nk, const double* bands, const double* kw,
+                        double mu, double T, double* out) {
+    const int Ne = N * Nk;
+    const int G = std::max(1, std::min(1024, (Ne + 255) / 256));
+    double* part;
+    lqm_status st = wsT(c, W_SMALL, (size_t)G + 8, &part);
+    if (st) return st;
+    hipLaunchKernelGGL(lqm::k_filling, dim3(G), dim3(256), 0, c->stream, N, Ne, bands, kw, mu, T, part);
+    hipLaunchKernelGGL(lqm::k_sum, dim3(1), dim3(256), 0, c->stream, G, part, part + G, 0);
+    double h = 0;
+    LQM_HIP(hipMemcpyAsync(&h, part + G, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    // getfilling: unweighted mean over N*Nk, or sum_k w_k sum_n f / N
+    *out = kw ? h / (double)N : h / (double)Ne;
+    return LQM_OK;
+}
+
+uint64_t host_dkey(double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+double host_dkey_inv(uint64_t k) {
+    uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    double v;
+    std::memcpy(&v, &u, 8);
+    return v;
+}
+
+// r-th smallest (0-based) of n device doubles, by 8 byte-wise histogram passes.
+lqm_status select_rank(lqm_context* c, int n, const double* x, long long r, double* out) {
+    unsigned int* hist;
+    lqm_status st = wsT(c, W_OUT, 256, &hist);
+    if (st) return st;
+    uint64_t prefix = 0, mask = 0;
+    unsigned int h[256];
+    const int G = std::max(1, std::min(1024, (n + 255) / 256));
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        LQM_HIP(hipMemsetAsync(hist, 0, sizeof(unsigned int) * 256, c->stream));
+        hipLaunchKernelGGL(lqm::k_radix_hist, dim3(G), dim3(256), 0, c->stream, n, x,
+                           (unsigned long long)prefix, (unsigned long long)mask, shift, hist);
+        LQM_HIP(hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        LQM_HIP(hipStreamSynchronize(c->stream));
+        int b = 0;
+        for (; b < 256; ++b) {
+            if (r < (long long)h[b]) break;
+            r -= h[b];
+        }
+        if (b == 256) return fail(LQM_ERR_NONFINITE, "order statistic: rank out of range (NaN in bands?)");
+        prefix |= (uint64_t)b << shift;
+        mask |= (uint64_t)0xff << shift;
+    }
+    *out = host_dkey_inv(prefix);
+    return LQM_OK;
+}
+
+lqm_status mu_from_bands(lqm_context* c, int N, int Nk, const double* bands, const double* kw,
+                         double filling, double T, double* mu) {
+    const long long L = (long long)N * Nk;
+    if (L <= 0) return fail(LQM_ERR_ARG, "empty band set");
+    if (T == 0.0) {
+        // chemicalpotential_0! (Spectrum/chemicalpotential.jl:9-23)
+        double a, b;
+        lqm_status st;
+        if (filling >= 1.0) return select_rank(c, (int)L, bands, L - 1, mu);
+        if (filling <= 0.0) {
+            if ((st = select_rank(c, (int)L, bands, 0, &a))) return st;
+            *mu = a - 1.0;
+            return LQM_OK;
+        }
+        long long i = 1 + (long long)std::floor(filling * (double)(L - 1));  // 1-based
+        if ((st = select_rank(c, (int)L, bands, i - 1, &a))) return st;
+        if ((st = select_rank(c, (int)L, bands, std::min(i, L - 1), &b))) return st;
+        *mu = (a + b) / 2;
+        return LQM_OK;
+    }
+    // T > 0: bracketing bisection of getfilling(mu) - filling on [min, max]
+    const int G = std::max(1, std::min(1024, (int)((L + 255) / 256)));
+    double* part;
+    lqm_status st = wsT(c, W_SMALL, (size_t)2 * G + 8, &part);
+    if (st) return st;
+    hipLaunchKernelGGL(lqm::k_minmax, dim3(G), dim3(256), 0, c->stream, (int)L, bands, part);
+    std::vector<double> mm(2 * G);
+    LQM_HIP(hipMemcpyAsync(mm.data(), part, sizeof(double) * 2 * G, hipMemcpyDeviceToHost, c->stream));
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    double lo = INFINITY, hi = -INFINITY;
+    for (int g = 0; g < G; ++g) {
+        lo = std::min(lo, mm[2 * g]);
+        hi = std::max(hi, mm[2 * g + 1]);
+    }
+    if (!std::isfinite(lo) || !std::isfinite(hi)) return fail(LQM_ERR_NONFINITE, "non-finite bands");
+    auto F = [&](double x, double* fx) -> lqm_status {
+        double f;
+        lqm_status s2 = filling_eval(c, N, Nk, bands, kw, x, T, &f);
+        *fx = f - filling;
+        return s2;
+    };
+    double fa, fb;
+    if ((st = F(lo, &fa))) return st;
+    if ((st = F(hi, &fb))) return st;
+    if (fa == 0) { *mu = lo; return LQM_OK; }
+    if (fb == 0) { *mu = hi; return LQM_OK; }
+    if ((fa < 0) == (fb < 0)) return fail(LQM_ERR_ARG, "filling not bracketed by the band range");
+    double a = lo, b = hi;
+    for (int it = 0; it < 200; ++it) {
+        double mid = bit_middle(a, b);
+        if (!(mid > a && mid < b)) break;
+        double fm;
+        if ((st = F(mid, &fm))) return st;
+        if (fm == 0) { *mu = mid; return LQM_OK; }
+        if ((fm < 0) == (fa < 0)) { a = mid; fa = fm; }
+        else { b = mid; fb = fm; }
+    }
+    *mu = std::fabs(fa) <= std::fabs(fb) ? a : b;
+    return LQM_OK;
+}
+
+// host <-> device staging helpers for the synchronous host-buffer entry points
+lqm_status d2h(lqm_context* c, void* dst, const void* src, size_t bytes) {
+    if (bytes) LQM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    return LQM_OK;
+}
+
+lqm_status stage_in(lqm_context* c, Slot s, const void* h, size_t bytes, void** dptr) {
+    lqm_status st = ws(c, s, bytes, dptr);
+    if (st) return st;
+    if (bytes && h) LQM_HIP(hipMemcpyAsync(*dptr, h, bytes, hipMemcpyHostToDevice, c->stream));
+    return LQM_OK;
+}
+
+lqm_status model_init(lqm_context* ctx, int N, int d, int nR, const int32_t* R, lqm_model** out) {
+    if (!ctx || !out) return fail(LQM_ERR_ARG, "null context or output pointer");
+    if (N <= 0 || d <= 0 || nR <= 0) return fail(LQM_ERR_ARG, "N, d and nR must be positive");
+    if (N > 32767) return fail(LQM_ERR_UNSUPPORTED, "N above 32767");
+    if (!R) return fail(LQM_ERR_ARG, "null key array");
+    auto* m = new lqm_model();
+    m->ctx = ctx;
+    m->N = N;
+    m->d = d;
+    m->nR = nR;
+    m->R.assign(R, R + (size_t)d * nR);
+    hipError_t e = hipMalloc(&m->dR, sizeof(int) * d * nR);
+    if (e == hipSuccess) e = hipMalloc(&m->dT, sizeof(cx<double>) * (size_t)N * N * nR);
+    if (e != hipSuccess) {
+        (void)hipFree(m->dR);
+        delete m;
+        return fail(LQM_ERR_DEVICE, std::string("model allocation: ") + hipGetErrorString(e));
+    }
+    *out = m;
+    return LQM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lqm_last_error(void) { return g_err.c_str(); }
+int lqm_abi_version(void) { return 1; }
+
+lqm_status lqm_context_create(int device, int precision, lqm_context** out) {
+    if (!out) return fail(LQM_ERR_ARG, "null output pointer");
+    if (precision != LQM_PREC_FP64) return fail(LQM_ERR_UNSUPPORTED, "precision mode not built");
+    int n = 0;
+    LQM_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(LQM_ERR_ARG, "device ordinal out of range");
+    LQM_HIP(hipSetDevice(device));
+    auto* c = new lqm_context();
+    c->device = device;
+    c->precision = precision;
+    int lds = 65536;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess)
+        c->lds_max = lds;
+    hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&c->d_status, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->d_status, 0, sizeof(int));
+    if (e != hipSuccess) {
+        if (c->own) (void)hipStreamDestroy(c->own);
+        delete c;
+        return fail(LQM_ERR_DEVICE, std::string("context init: ") + hipGetErrorString(e));
+    }
+    c->stream = c->own;
+    *out = c;
+    return LQM_OK;
+}
+
+lqm_status lqm_context_destroy(lqm_context* c) {
+    if (!c) return LQM_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (int s = 0; s < W_NSLOTS; ++s)
+        if (c->buf[s]) (void)hipFree(c->buf[s]);
+    (void)hipFree(c->d_status);
+    (void)hipStreamDestroy(c->own);
+    delete c;
+    return LQM_OK;
+}
+
+lqm_status lqm_context_set_stream(lqm_context* c, void* s) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    c->stream = s ? static_cast<hipStream_t>(s) : c->own;
+    return LQM_OK;
+}
+
+lqm_status lqm_context_synchronize(lqm_context* c) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    return LQM_OK;
+}
+
+lqm_status lqm_model_create(lqm_context* ctx, int N, int d, int nR, const int32_t* R,
+                            const double* T, lqm_model** out) {
+    if (!T) return fail(LQM_ERR_ARG, "null hopping array");
+    lqm_status st = model_init(ctx, N, d, nR, R, out);
+    if (st) return st;
+    lqm_model* m = *out;
+    LQM_HIP(hipMemcpyAsync(m->dR, R, sizeof(int) * d * nR, hipMemcpyHostToDevice, ctx->stream));
+    LQM_HIP(hipMemcpyAsync(m->dT, T, sizeof(cx<double>) * (size_t)N * N * nR, hipMemcpyHostToDevice,
+                           ctx->stream));
+    LQM_HIP(hipStreamSynchronize(ctx->stream));
+    return LQM_OK;
+}
+
+lqm_status lqm_model_create_dev(lqm_context* ctx, int N, int d, int nR, const int32_t* R,
+                                const double* T_dev, lqm_model** out) {
+    if (!T_dev) return fail(LQM_ERR_ARG, "null hopping array");
+    lqm_status st = model_init(ctx, N, d, nR, R, out);
+    if (st) return st;
+    lqm_model* m = *out;
+    LQM_HIP(hipMemcpyAsync(m->dR, R, sizeof(int) * d * nR, hipMemcpyHostToDevice, ctx->stream));
+    LQM_HIP(hipMemcpyAsync(m->dT, T_dev, sizeof(cx<double>) * (size_t)N * N * nR,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    LQM_HIP(hipStreamSynchronize(ctx->stream));
+    return LQM_OK;
+}
+
+lqm_status lqm_model_destroy(lqm_model* m) {
+    if (!m) return LQM_OK;
+    (void)hipStreamSynchronize(m->ctx->stream);
+    (void)hipFree(m->dR);
+    (void)hipFree(m->dT);
+    delete m;
+    return LQM_OK;
+}
+
+lqm_status lqm_fouriersum_dev(lqm_model* m, int Nk, const double* ks_dev, double* Hk_dev) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0) return Nk == 0 ? LQM_OK : fail(LQM_ERR_ARG, "Nk < 0");
+    if (!ks_dev || !Hk_dev) return fail(LQM_ERR_ARG, "null buffer");
+    const size_t sh = sizeof(cx<double>) * m->nR;
+    auto kH = lqm::k_fouriersum;
+    allow_lds(kH, sh);
+    hipLaunchKernelGGL(kH, dim3(Nk), dim3(256), sh, m->ctx->stream, m->N, m->d, m->nR, m->dR, m->dT,
+                       ks_dev, reinterpret_cast<cx<double>*>(Hk_dev));
+    LQM_HIP(hipGetLastError());
+    LQM_HIP(hipStreamSynchronize(m->ctx->stream));
+    return LQM_OK;
+}
+
+lqm_status lqm_fouriersum(lqm_model* m, int Nk, const double* ks, double* Hk) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0) return Nk == 0 ? LQM_OK : fail(LQM_ERR_ARG, "Nk < 0");
+    lqm_context* c = m->ctx;
+    void *dks, *dH;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if ((st = ws(c, W_OUT, sizeof(cx<double>) * (size_t)m->N * m->N * Nk, &dH))) return st;
+    if ((st = lqm_fouriersum_dev(m, Nk, (const double*)dks, (double*)dH))) return st;
+    return d2h(c, Hk, dH, sizeof(cx<double>) * (size_t)m->N * m->N * Nk);
+}
+
+lqm_status lqm_bandmatrix_dev(lqm_model* m, int Nk, const double* ks_dev, double* bands_dev,
+                              double* evecs_dev) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    return run_bands(m, Nk, ks_dev, bands_dev, evecs_dev);
+}
+
+lqm_status lqm_bandmatrix(lqm_model* m, int Nk, const double* ks, double* bands, double* evecs) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0) return Nk == 0 ? LQM_OK : fail(LQM_ERR_ARG, "Nk < 0");
+    lqm_context* c = m->ctx;
+    const size_t N = m->N;
+    void *dks, *db, *dv = nullptr;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if ((st = ws(c, W_BANDS, sizeof(double) * N * Nk, &db))) return st;
+    if (evecs && (st = ws(c, W_OUT, sizeof(cx<double>) * N * N * Nk, &dv))) return st;
+    if ((st = run_bands(m, Nk, (const double*)dks, (double*)db, (double*)dv))) return st;
+    if ((st = d2h(c, bands, db, sizeof(double) * N * Nk))) return st;
+    if (evecs) return d2h(c, evecs, dv, sizeof(cx<double>) * N * N * Nk);
+    return LQM_OK;
+}
+
+lqm_status lqm_dos_from_bands_dev(lqm_context* c, int N, int Nk, const double* bands_dev,
+                                  const double* kweights_dev, int Nw, const double* omegas_dev,
+                                  double gamma, double* dos_dev) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    if (N <= 0 || Nk <= 0 || Nw < 0) return fail(LQM_ERR_ARG, "bad sizes");
+    lqm_status st = dos_bands_dev(c, N, Nk, bands_dev, kweights_dev, Nw, omegas_dev, gamma, dos_dev);
+    if (st) return st;
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    return LQM_OK;
+}
+
+lqm_status lqm_dos(lqm_model* m, int Nk, const double* ks, const double* kweights, int Nw,
+                   const double* omegas, double gamma, double* dos) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0 || Nw < 0) return fail(LQM_ERR_ARG, "bad sizes");
+    lqm_context* c = m->ctx;
+    const size_t N = m->N;
+    void *dks, *dkw = nullptr, *dom, *db, *dd;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    if ((st = stage_in(c, W_OM, omegas, sizeof(double) * std::max(Nw, 1), &dom))) return st;
+    if ((st = ws(c, W_BANDS, sizeof(double) * N * Nk, &db))) return st;
+    if ((st = ws(c, W_OUT, sizeof(double) * std::max(Nw, 1), &dd))) return st;
+    if ((st = run_bands(m, Nk, (const double*)dks, (double*)db, nullptr))) return st;
+    if ((st = dos_bands_dev(c, (int)N, Nk, (double*)db, (double*)dkw, Nw, (double*)dom, gamma,
+                            (double*)dd)))
+        return st;
+    return d2h(c, dos, dd, sizeof(double) * Nw);
+}
+
+lqm_status lqm_chemicalpotential_from_bands_dev(lqm_context* c, int N, int Nk, const double* bands_dev,
+                                                const double* kweights_dev, double filling, double T,
+                                                double* mu) {
+    if (!c || !mu || !bands_dev) return fail(LQM_ERR_ARG, "null argument");
+    if (N <= 0 || Nk <= 0) return fail(LQM_ERR_ARG, "bad sizes");
+    if (T < 0) return fail(LQM_ERR_ARG, "negative temperature");
+    return mu_from_bands(c, N, Nk, bands_dev, kweights_dev, filling, T, mu);
+}
+
+lqm_status lqm_chemicalpotential(lqm_model* m, int Nk, const double* ks, const double* kweights,
+                                 double filling, double T, double* mu) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0) return fail(LQM_ERR_ARG, "Nk must be positive");
+    lqm_context* c = m->ctx;
+    void *dks, *dkw = nullptr, *db;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    if ((st = ws(c, W_BANDS, sizeof(double) * m->N * Nk, &db))) return st;
+    if ((st = run_bands(m, Nk, (const double*)dks, (double*)db, nullptr))) return st;
+    return lqm_chemicalpotential_from_bands_dev(c, m->N, Nk, (double*)db, (double*)dkw, filling, T, mu);
+}
+
+lqm_status lqm_densitymatrix_dev(lqm_model* m, int Nk, const double* ks_dev,
+                                 const double* kweights_dev, double mu, double T, int nRr,
+                                 const int32_t* Rr, double* rho_dev, double* eband_dev,
+                                 double* bands_dev) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    return run_rho(m, Nk, ks_dev, kweights_dev, mu, T, nRr, Rr, rho_dev, eband_dev, bands_dev);
+}
+
+lqm_status lqm_densitymatrix(lqm_model* m, int Nk, const double* ks, const double* kweights,
+                             double mu, double T, int nRr, const int32_t* Rr, double* rho,
+                             double* eband) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0) return fail(LQM_ERR_ARG, "Nk must be positive");
+    if (!rho && nRr > 0) return fail(LQM_ERR_ARG, "null rho");
+    lqm_context* c = m->ctx;
+    const size_t NN = (size_t)m->N * m->N;
+    void *dks, *dkw = nullptr, *drho, *deb;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    if ((st = ws(c, W_OUT, sizeof(cx<double>) * NN * std::max(nRr, 1) + 16, &drho))) return st;
+    deb = static_cast<char*>(drho) + sizeof(cx<double>) * NN * std::max(nRr, 1);
+    if ((st = run_rho(m, Nk, (const double*)dks, (const double*)dkw, mu, T, nRr, Rr, (double*)drho,
+                      (double*)deb, nullptr)))
+        return st;
+    if (nRr > 0 && (st = d2h(c, rho, drho, sizeof(cx<double>) * NN * nRr))) return st;
+    double h = 0;
+    if ((st = d2h(c, &h, deb, sizeof(double)))) return st;
+    if (eband) *eband = h;
+    return LQM_OK;
+}
+
+}  // extern "C"
