@@ -17,8 +17,9 @@
  *   - Every function returns an lqm_status; on failure lqm_last_error() returns a
  *     message for the calling thread.  No exceptions or callbacks cross the ABI.
  *   - Host-buffer functions are synchronous: pointers are read/written only during
- *     the call.  *_dev functions take device pointers and are asynchronous on the
- *     context's stream (call lqm_context_synchronize before reading results).
+ *     the call.  *_dev functions take device pointers, enqueue their kernels on the
+ *     context's stream and return once that stream has reached the end of the
+ *     call (the device-side status word is read back then).
  *   - A context is used by one host thread at a time; several may coexist.
  */
 #ifndef LQM_GPU_H
@@ -59,6 +60,16 @@ lqm_status lqm_context_destroy(lqm_context *ctx);
 /* Run on an existing hipStream_t (NULL restores the context's own stream). */
 lqm_status lqm_context_set_stream(lqm_context *ctx, void *hip_stream);
 lqm_status lqm_context_synchronize(lqm_context *ctx);
+/* Kernel-stage timing (no reference counterpart: the reference has no tracer).
+ * When enabled every kernel launch is bracketed by HIP events on the context
+ * stream.  lqm_context_stage_times returns, per stage, the accumulated device
+ * milliseconds and launch counts since the last query, then resets them.
+ * Stages: 0 hk_tridiag, 1 tridiag_ql, 2 backtransform, 3 phase_contract,
+ * 4 phase_fold, 5 dos, 6 reduce (LQM_NSTAGES = 7). */
+#define LQM_NSTAGES 7
+lqm_status lqm_context_set_timing(lqm_context *ctx, int enable);
+lqm_status lqm_context_stage_times(lqm_context *ctx, double *ms, int64_t *launches, int n);
+const char *lqm_stage_name(int stage);
 
 /* ---- model: the Hops operator, resident in HBM --------------------------------- */
 /* Upload of Hops{Vector{Int},Matrix{ComplexF64}} (TightBinding/types.jl:54-60):

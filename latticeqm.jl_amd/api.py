@@ -1,0 +1,264 @@
+"""Host API of the GPU band path, mirroring the reference's entry points.
+
+Each function has the name, argument meaning and error behaviour of its
+LatticeQM.jl counterpart, with ``multimode=:gpu`` as the only executor:
+
+=====================  =================================================
+this module            reference
+=====================  =================================================
+``to_executor``        Parallel.to_executor (Parallel/Parallel.jl:85-109)
+``fouriersum``         TightBinding.fouriersum! (TightBinding/types.jl:18-25)
+``bandmatrix``         Spectrum.bandmatrix (Spectrum/bands.jl:164-204)
+``getbands``           Spectrum.getbands (Spectrum/bands.jl:264-267)
+``getdos``             Spectrum.getdos (Spectrum/dos.jl:127-232)
+``chemicalpotential``  Spectrum.chemicalpotential (Spectrum/chemicalpotential.jl:6-65)
+``getfilling``         Spectrum.getfilling / filling (chemicalpotential.jl:74-94)
+``getdensitymatrix``   Operators.getdensitymatrix! (Operators/densitymatrix.jl:17-49)
+``setfilling``         Operators.setfilling! (Operators/chemicalpotential.jl:20-46)
+=====================  =================================================
+
+All numerics run in liblqmgpu.so on the GPU; this module only packs arrays and
+maps status codes to exceptions.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .hops import Hops, Mesh, points, regulargrid
+
+_c = ctypes
+
+
+def _vp(a: Optional[np.ndarray]):
+    return None if a is None else _c.c_void_p(a.ctypes.data)
+
+
+class GPUExec:
+    """The executor multimode=:gpu resolves to: one device context + stream."""
+
+    def __init__(self, device: int = 0, precision: int = _lib.LQM_PREC_FP64):
+        self.device = device
+        self.precision = precision
+        h = _c.c_void_p()
+        _lib.call("lqm_context_create", device, precision, _c.byref(h))
+        self.handle = h
+
+    def synchronize(self) -> None:
+        _lib.call("lqm_context_synchronize", self.handle)
+
+    def set_stream(self, stream_ptr: int | None) -> None:
+        _lib.call("lqm_context_set_stream", self.handle, _c.c_void_p(stream_ptr or 0))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            _lib.lib().lqm_context_destroy(self.handle)
+            self.handle = _c.c_void_p()
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order varies
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_executors: dict = {}
+
+
+def to_executor(multimode="gpu", device: int = 0) -> GPUExec:
+    """Resolve ``multimode`` (``:gpu``/``"gpu"`` or a GPUExec) to the shared executor."""
+    if isinstance(multimode, GPUExec):
+        return multimode
+    mode = str(multimode).lstrip(":")
+    if mode != "gpu":
+        raise ValueError(f"this package provides multimode=:gpu only, got {multimode!r}")
+    ex = _executors.get(device)
+    if ex is None:
+        ex = _executors[device] = GPUExec(device)
+    return ex
+
+
+class GPUModel:
+    """A Hops operator uploaded once and kept resident in device memory."""
+
+    def __init__(self, H: Hops, executor: GPUExec | None = None):
+        self.exec = executor or to_executor()
+        R, T = H.pack()
+        self.R, self.N, self.d, self.nR = R, H.N, H.d, R.shape[1]
+        self.keys = list(H.keys())
+        h = _c.c_void_p()
+        _lib.call("lqm_model_create", self.exec.handle, self.N, self.d, self.nR, _vp(R), _vp(T),
+                  _c.byref(h))
+        self.handle = h
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            _lib.lib().lqm_model_destroy(self.handle)
+            self.handle = _c.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _model(H, executor) -> GPUModel:
+    if isinstance(H, GPUModel):
+        return H
+    if not isinstance(H, Hops):
+        raise TypeError("H must be a Hops or a GPUModel")
+    return GPUModel(H, to_executor(executor or "gpu"))
+
+
+def _kpts(ks, d: int) -> np.ndarray:
+    k = np.asfortranarray(points(ks), dtype=np.float64)
+    if k.shape[0] != d:
+        raise ValueError(f"k-points have dimension {k.shape[0]}, the model has {d}")
+    return k
+
+
+def _weights(ks, kweights):
+    if kweights is not None:
+        return np.ascontiguousarray(kweights, dtype=np.float64)
+    if isinstance(ks, Mesh):
+        return np.ascontiguousarray(ks.weights(), dtype=np.float64)
+    return None
+
+
+def fouriersum(H, ks, executor=None) -> np.ndarray:
+    """H(k) for every column of ks: complex array (N, N, Nk)."""
+    m = _model(H, executor)
+    k = _kpts(ks, m.d)
+    Nk = k.shape[1]
+    out = np.zeros((m.N, m.N, Nk), dtype=np.complex128, order="F")
+    _lib.call("lqm_fouriersum", m.handle, Nk, _vp(k), _vp(out))
+    return out
+
+
+def bandmatrix(H, ks, *projectors, executor=None, multimode="gpu"):
+    """Eigenvalues of Hermitian(H(k)) per k: returns (bands (N, Nk), obs (N, Nk, 0))."""
+    if projectors:
+        raise NotImplementedError("projector expectation values are not part of this build")
+    m = _model(H, executor or multimode)
+    k = _kpts(ks, m.d)
+    Nk = k.shape[1]
+    bands = np.zeros((m.N, Nk), order="F")
+    _lib.call("lqm_bandmatrix", m.handle, Nk, _vp(k), _vp(bands), None)
+    return bands, np.zeros((m.N, Nk, 0))
+
+
+def eigen(H, ks, executor=None):
+    """Eigenvalues and eigenvectors per k (the geteigen! branch, bands.jl:222-224):
+    (bands (N, Nk), vecs (N, N, Nk)) with vecs[:, n, j] the eigenvector of bands[n, j]."""
+    m = _model(H, executor)
+    k = _kpts(ks, m.d)
+    Nk = k.shape[1]
+    bands = np.zeros((m.N, Nk), order="F")
+    vecs = np.zeros((m.N, m.N, Nk), dtype=np.complex128, order="F")
+    _lib.call("lqm_bandmatrix", m.handle, Nk, _vp(k), _vp(bands), _vp(vecs))
+    return bands, vecs
+
+
+@dataclass
+class BandData:
+    """Spectrum.BandData (Spectrum/types.jl:23-29)."""
+    bands: np.ndarray
+    obs: np.ndarray
+    path: object = field(default=None)
+
+
+def getbands(H, ks, *projectors, **kw) -> BandData:
+    bands, obs = bandmatrix(H, ks, *projectors, **kw)
+    return BandData(bands, obs, ks)
+
+
+def getdos(h, *args, klin: int | None = None, Gamma: float | None = None, kweights=None,
+           executor=None, **kw):
+    """DOS with a Lorentzian of width Gamma and no 1/pi (Spectrum/dos.jl:127-232).
+
+    getdos(h, emin, emax, num=500; klin, Gamma) -> (omegas, DOS)
+    getdos(h, omegas; klin, Gamma)              -> DOS
+    getdos(h, omegas, ks; Gamma, kweights)       -> DOS
+    """
+    if Gamma is None:
+        Gamma = kw.pop("Γ", None)
+    if Gamma is None:
+        raise ValueError("keyword Gamma (Γ) is required")
+    if len(args) >= 2 and np.isscalar(args[0]) and np.isscalar(args[1]):
+        emin, emax = float(args[0]), float(args[1])
+        num = int(args[2]) if len(args) > 2 else 500
+        om = np.linspace(emin, emax, num)
+        return om, getdos(h, om, klin=klin, Gamma=Gamma, executor=executor)
+    om = np.ascontiguousarray(args[0], dtype=np.float64)
+    if len(args) > 1:
+        ks = args[1]
+    else:
+        if klin is None:
+            raise ValueError("keyword klin is required")
+        ks = regulargrid(nk=klin ** 2)
+    m = _model(h, executor)
+    k = _kpts(ks, m.d)
+    w = _weights(ks, kweights)
+    out = np.zeros(len(om))
+    _lib.call("lqm_dos", m.handle, k.shape[1], _vp(k), _vp(w), len(om), _vp(om), float(Gamma), _vp(out))
+    return out
+
+
+def chemicalpotential(H, ks, filling: float, T: float = 0.0, executor=None, kweights=None) -> float:
+    """mu such that the filling of the bands on ks equals ``filling``."""
+    m = _model(H, executor)
+    k = _kpts(ks, m.d)
+    w = _weights(ks, kweights)
+    mu = _c.c_double()
+    _lib.call("lqm_chemicalpotential", m.handle, k.shape[1], _vp(k), _vp(w), float(filling), float(T),
+              _c.byref(mu))
+    return mu.value
+
+
+def getdensitymatrix(rhos: Hops, H, ks, mu: float = 0.0, T: float = 0.01, kweights=None,
+                     executor=None) -> float:
+    """Overwrite every block of ``rhos`` with rho(R) and return E_band.
+
+    rho(R) = sum_k w_k e^{-2 pi i k.R} U_k f(e - mu) U_k^H with w_k = 1/Nk (or the
+    Mesh / explicit weights); only the keys already present in ``rhos`` are filled.
+    """
+    m = _model(H, executor)
+    k = _kpts(ks, m.d)
+    w = _weights(ks, kweights)
+    keys = list(rhos.keys())
+    if keys and len(keys[0]) != m.d:
+        raise ValueError("rho keys and model keys differ in dimension")
+    for key in keys:
+        if rhos[key].shape != (m.N, m.N):
+            raise ValueError("rho blocks must be N x N")
+    nRr = len(keys)
+    Rr = np.asfortranarray(np.array(keys, dtype=np.int32).T.reshape(m.d, nRr)) if nRr else np.zeros((m.d, 1), np.int32)
+    rho = np.zeros((m.N, m.N, max(nRr, 1)), dtype=np.complex128, order="F")
+    eb = _c.c_double()
+    _lib.call("lqm_densitymatrix", m.handle, k.shape[1], _vp(k), _vp(w), float(mu), float(T), nRr,
+              _vp(Rr), _vp(rho), _c.byref(eb))
+    for r, key in enumerate(keys):
+        rhos.data[key] = np.array(rho[:, :, r])
+    return eb.value
+
+
+def addchemicalpotential(H: Hops, mu: float) -> Hops:
+    """Add mu to the onsite diagonal (Operators/chemicalpotential.jl:48-58)."""
+    zk = H.zerokey()
+    if zk not in H:
+        H[zk] = np.zeros((H.N, H.N), dtype=np.complex128)
+    blk = H.data[zk]
+    blk[np.diag_indices(H.N)] += mu
+    return H
+
+
+def setfilling(H: Hops, filling: float, nk: int = 100, T: float = 0.0, executor=None) -> float:
+    """Shift H so that its filling on regulargrid(nk) is ``filling``; returns mu."""
+    mu = chemicalpotential(H, regulargrid(nk=nk, dim=H.d), filling, T=T, executor=executor)
+    addchemicalpotential(H, -mu)
+    return mu

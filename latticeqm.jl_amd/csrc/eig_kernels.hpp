@@ -418,7 +418,7 @@ __global__ __launch_bounds__(kTPB) void k_backtransform(
         cx<double>* o = out + (size_t)kk * NN;
         for (int e = tid; e < N * N; e += kTPB) {
             int i = e % N, n = e / N;
-            int c = pm[n];
+            int c = min(max(pm[n], 0), N - 1);  // perm is only garbage after a flagged NaN
             o[e] = mkc<double>((double)Vr[(size_t)c * ldv + i], (double)Vi[(size_t)c * ldv + i]);
         }
     } else {
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(kTPB) void k_backtransform(
         for (int n = tid; n < N; n += kTPB) {
             double en = (double)ev[n];
             double f = fermidirac(en, mu, Temp);
-            sf[pm[n]] = f;
+            sf[min(max(pm[n], 0), N - 1)] = f;
             ebp += en * f;
         }
         ebp = wave_sum(ebp);

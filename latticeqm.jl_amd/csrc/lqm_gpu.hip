@@ -43,7 +43,21 @@ constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch
 
 }  // namespace
 
+// Kernel stages timed with HIP events when timing is on.
+enum Stage { ST_TRIDIAG, ST_QL, ST_BACKTRANSFORM, ST_CONTRACT, ST_FOLD, ST_DOS, ST_REDUCE, ST_N };
+const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "backtransform",
+                                       "phase_contract", "phase_fold", "dos", "reduce"};
+
+struct TimedLaunch {
+    int stage;
+    hipEvent_t a, b;
+};
+
 struct lqm_context {
+    int timing = 0;
+    std::vector<TimedLaunch> pending;
+    double stage_ms[ST_N] = {};
+    long long stage_n[ST_N] = {};
     int device = 0;
     int precision = 0;
     int lds_max = 65536;
@@ -131,6 +145,26 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     return p;
 }
 
+// Brackets one kernel launch with events on the context stream (timing mode only).
+struct StageScope {
+    lqm_context* c;
+    TimedLaunch t{};
+    StageScope(lqm_context* ctx, int stage) : c(ctx) {
+        if (!c->timing) return;
+        t.stage = stage;
+        if (hipEventCreate(&t.a) != hipSuccess || hipEventCreate(&t.b) != hipSuccess) {
+            c = nullptr;
+            return;
+        }
+        (void)hipEventRecord(t.a, c->stream);
+    }
+    ~StageScope() {
+        if (!c || !c->timing) return;
+        (void)hipEventRecord(t.b, c->stream);
+        c->pending.push_back(t);
+    }
+};
+
 template <typename K>
 void allow_lds(K kernel, size_t bytes) {
     if (bytes > 65536)
@@ -160,6 +194,8 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
         if ((st = wsT(c, W_SW, (size_t)P.cap_sw * Kc, &sw))) return st;
         if ((st = wsT(c, W_NSW, (size_t)Kc, &nsw))) return st;
     }
+    {
+    StageScope sc(c, ST_TRIDIAG);
     if (P.ldsA) {
         auto kA = lqm::k_hk_tridiag<double, true>;
         allow_lds(kA, P.shA);
@@ -171,9 +207,12 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
         hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kTPB), P.shA, c->stream, N, m->d, m->nR, m->dR,
                            m->dT, ks, Kc, gA, dd, ee, refl, tau);
     }
+    }
     LQM_HIP(hipGetLastError());
     const int L = P.laneB;
     const int gB = (Kc + L - 1) / L;
+    {
+    StageScope sc(c, ST_QL);
     if (logq) {
         auto kB = lqm::k_tridiag_ql<double, true>;
         allow_lds(kB, P.shB);
@@ -184,6 +223,7 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
         allow_lds(kB, P.shB);
         hipLaunchKernelGGL(kB, dim3(gB), dim3(L), P.shB, c->stream, N, Kc, dd, ee, evals, perm, rot,
                            sw, nsw, P.cap_rot, P.cap_sw, c->d_status);
+    }
     }
     LQM_HIP(hipGetLastError());
     return LQM_OK;
@@ -206,6 +246,7 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const doubl
     auto* sw = static_cast<int*>(c->buf[W_SW]);
     auto* nsw = static_cast<int*>(c->buf[W_NSW]);
     auto* perm = static_cast<int*>(c->buf[W_PERM]);
+    StageScope sc(c, ST_BACKTRANSFORM);
     if (P.ldsC) {
         auto kC = lqm::k_backtransform<double, true, MODE>;
         allow_lds(kC, P.shC);
@@ -222,13 +263,18 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const doubl
 }
 
 template <int RG>
-void launch_contract(hipStream_t s, int NN, int Kc, int d, const double* ks, const double* kw,
+void launch_contract(lqm_context* c, int NN, int Kc, int d, const double* ks, const double* kw,
                      double wuni, int nRr, const int* Rr, const cx<double>* rhok, int ksplit,
                      int S, int groups, cx<double>* part, cx<double>* rho) {
+    hipStream_t s = c->stream;
     dim3 grid((NN + 255) / 256, S, groups);
-    hipLaunchKernelGGL(lqm::k_phase_contract<RG>, grid, dim3(256), 0, s, NN, Kc, d, ks, kw, wuni, nRr,
-                       Rr, rhok, ksplit, part);
+    {
+        StageScope sc(c, ST_CONTRACT);
+        hipLaunchKernelGGL(lqm::k_phase_contract<RG>, grid, dim3(256), 0, s, NN, Kc, d, ks, kw, wuni,
+                           nRr, Rr, rhok, ksplit, part);
+    }
     const size_t total = (size_t)nRr * NN;
+    StageScope sc(c, ST_FOLD);
     hipLaunchKernelGGL(lqm::k_phase_fold<RG>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
                        NN, nRr, S, part, rho);
 }
@@ -316,18 +362,21 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
         if ((st = wsT(c, W_RHOK, NN * P.Kc, &rhok))) return st;
         if ((st = wsT(c, W_EB, (size_t)P.Kc, &eb))) return st;
         if ((st = launch_backtransform<1>(m, P, Kc, ev, kwc, wuni, mu, T, rhok, eb))) return st;
-        hipLaunchKernelGGL(lqm::k_sum, dim3(1), dim3(256), 0, c->stream, Kc, eb, eband, 1);
+        {
+            StageScope sc(c, ST_REDUCE);
+            hipLaunchKernelGGL(lqm::k_sum, dim3(1), dim3(256), 0, c->stream, Kc, eb, eband, 1);
+        }
         if (nRr > 0) {
             const int Sc = std::min(S, Kc);
             const int ksplit = (Kc + Sc - 1) / Sc;
             const int Sused = (Kc + ksplit - 1) / ksplit;
             auto* rh = reinterpret_cast<cx<double>*>(rho);
             switch (RG) {
-                case 1: launch_contract<1>(c->stream, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
-                case 4: launch_contract<4>(c->stream, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
-                case 8: launch_contract<8>(c->stream, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
-                case 16: launch_contract<16>(c->stream, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
-                default: launch_contract<32>(c->stream, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
+                case 1: launch_contract<1>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
+                case 4: launch_contract<4>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
+                case 8: launch_contract<8>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
+                case 16: launch_contract<16>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
+                default: launch_contract<32>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
             }
             LQM_HIP(hipGetLastError());
         }
@@ -346,6 +395,7 @@ lqm_status dos_bands_dev(lqm_context* c, int N, int Nk, const double* bands, con
     double* part;
     lqm_status st = wsT(c, W_PART, (size_t)S * Nw + 1, &part);
     if (st) return st;
+    StageScope sc(c, ST_DOS);
     hipLaunchKernelGGL(lqm::k_dos, dim3(tilesO, S), dim3(256), 0, c->stream, N, Ne, bands, kw, Nw, om,
                        gamma, esplit, part);
     const double scale = kw ? 1.0 : 1.0 / (double)Nk;
@@ -572,6 +622,36 @@ lqm_status lqm_context_set_stream(lqm_context* c, void* s) {
     c->stream = s ? static_cast<hipStream_t>(s) : c->own;
     return LQM_OK;
 }
+
+lqm_status lqm_context_set_timing(lqm_context* c, int enable) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    c->timing = enable ? 1 : 0;
+    return LQM_OK;
+}
+
+lqm_status lqm_context_stage_times(lqm_context* c, double* ms, int64_t* launches, int n) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    for (auto& t : c->pending) {
+        float e = 0.f;
+        if (hipEventElapsedTime(&e, t.a, t.b) == hipSuccess) {
+            c->stage_ms[t.stage] += e;
+            c->stage_n[t.stage] += 1;
+        }
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    c->pending.clear();
+    for (int i = 0; i < n && i < ST_N; ++i) {
+        if (ms) ms[i] = c->stage_ms[i];
+        if (launches) launches[i] = c->stage_n[i];
+        c->stage_ms[i] = 0;
+        c->stage_n[i] = 0;
+    }
+    return LQM_OK;
+}
+
+const char* lqm_stage_name(int stage) { return (stage >= 0 && stage < ST_N) ? kStageNames[stage] : ""; }
 
 lqm_status lqm_context_synchronize(lqm_context* c) {
     if (!c) return fail(LQM_ERR_ARG, "null context");

@@ -1,0 +1,97 @@
+"""Input generators for the band path (host-side set-up, run once per model).
+
+These build the exact ``Hops`` the tests and the benchmark feed to the device:
+
+* nearest-neighbour square, triangular and honeycomb models, with the keys and
+  blocks that ``Operators.nearestneighbor!`` produces on ``Geometries.square``,
+  ``triangular`` and ``honeycomb`` (Operators/nearestneighbor.jl:22-62,
+  Operators/gethops.jl:33-140: block (i, j) of key L couples site i of cell L to
+  site j of cell 0; the -L block is the adjoint);
+* ``addspin`` -- kron(t, sigma_0), spin as the fast index (TightBinding/types.jl:258-270);
+* a seeded random dense tight-binding model: complex Gaussian blocks with
+  standard deviation exp(-|R|) on a (2 rmax + 1)^d key box, Hermitian by
+  construction (t_{-R} = t_R^H, t_0 = (G + G^H)/2).
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+from .hops import Hops
+
+
+def square_nn(t: float = -1.0) -> Hops:
+    """1 orbital, keys +-(1,0), +-(0,1): e(k) = 2t(cos 2pi k1 + cos 2pi k2)."""
+    one = np.array([[t]], dtype=np.complex128)
+    return Hops({(1, 0): one, (-1, 0): one, (0, 1): one, (0, -1): one})
+
+
+def triangular_nn(t: float = -1.0) -> Hops:
+    """1 orbital, A = [(cos30, -sin30), (cos30, sin30)]: neighbours a1, a2, a1 - a2."""
+    one = np.array([[t]], dtype=np.complex128)
+    keys = [(1, 0), (-1, 0), (0, 1), (0, -1), (1, -1), (-1, 1)]
+    return Hops({k: one for k in keys})
+
+
+def honeycomb_nn(t: float = -1.0) -> Hops:
+    """Geometries.honeycomb(): A = sqrt(3)[(cos30, -sin30), (cos30, sin30)], sites
+    A at 0 and B at (a1 + a2)/3, bond length 1.  Keys 0, +-(1,0), +-(0,1)."""
+    z = np.zeros((2, 2), dtype=np.complex128)
+    t0 = z.copy()
+    t0[0, 1] = t0[1, 0] = t
+    # B in cell (-1,0) / (0,-1) neighbours A in cell 0: block[B, A]
+    tm = z.copy()
+    tm[1, 0] = t
+    h = Hops()
+    h[(0, 0)] = t0
+    h[(-1, 0)] = tm
+    h[(1, 0)] = tm.conj().T
+    h[(0, -1)] = tm
+    h[(0, 1)] = tm.conj().T
+    return h
+
+
+def addspin(h: Hops, mode: str = "spinhalf") -> Hops:
+    """kron(t_R, sigma) per block (TightBinding/types.jl:258-270)."""
+    if mode in ("nospin", "id"):
+        return h
+    if mode in ("spinhalf", "sigma0"):
+        s = np.eye(2, dtype=np.complex128)
+    elif mode == "sigmax":
+        s = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+    else:
+        raise ValueError(f"unknown spin mode {mode!r}")
+    return Hops({R: np.kron(t, s) for R, t in h.items()})
+
+
+def _cgauss(rng, shape) -> np.ndarray:
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2.0)
+
+
+def random_tb(N: int, rmax: int = 2, seed: int = 2, dim: int = 2) -> Hops:
+    """Random dense tight-binding model on the key box |R_a| <= rmax.
+
+    For R in the half space (first nonzero coordinate positive) t_R is complex
+    Gaussian with std exp(-|R|_2); t_{-R} = t_R^H; t_0 = (G + G^H)/2 with G complex
+    Gaussian.  Keys are inserted in lexicographic order.
+    """
+    rng = np.random.default_rng(seed)
+    keys = list(itertools.product(range(-rmax, rmax + 1), repeat=dim))
+    h = Hops()
+    blocks = {}
+    for R in keys:
+        if all(x == 0 for x in R):
+            G = _cgauss(rng, (N, N))
+            blocks[R] = (G + G.conj().T) / 2
+            continue
+        first = next(x for x in R if x != 0)
+        if first > 0:
+            blocks[R] = _cgauss(rng, (N, N)) * np.exp(-np.linalg.norm(R))
+    for R in keys:
+        if R in blocks:
+            h[R] = blocks[R]
+        else:
+            mR = tuple(-x for x in R)
+            h[R] = blocks[mR].conj().T
+    return h

@@ -83,6 +83,7 @@ class _OneBlasThread:
     """Pin OpenBLAS to one thread per caller thread (Parallel.configure_blas!)."""
 
     def __enter__(self):
+        lib()  # load scipy's OpenBLAS first so that the limit below applies to it
         try:
             from threadpoolctl import threadpool_limits
 

@@ -1,0 +1,109 @@
+"""GPU density matrix, DOS and chemical potential vs the FP64 oracle.
+
+Tolerances: rho to 1e-9 absolute (entries are O(1)), E_band to 1e-10 relative,
+DOS to 1e-10 relative, mu: T=0 exact (order statistic of the same eigenvalues to
+1e-12), T>0 the defining equation getfilling(mu) = filling to 1e-12.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _zero_like(pkg, h, keys=None):
+    keys = list(h.keys()) if keys is None else keys
+    return pkg.Hops({R: np.zeros((h.N, h.N), complex) for R in keys})
+
+
+def _rho_vs_oracle(pkg, oracle, h, ks, mu, T, kweights=None, keys=None, tol=1e-9):
+    rho = _zero_like(pkg, h, keys)
+    eb = pkg.api.getdensitymatrix(rho, h, ks, mu, T=T, kweights=kweights)
+    R, Tt = h.pack()
+    Rr = np.array(list(rho.keys()), dtype=np.int32).T
+    ref, eb_ref = oracle.densitymatrix(R, Tt, ks, Rr, mu=mu, Temp=T, kweights=kweights)
+    err = max(np.max(np.abs(rho[tuple(Rr[:, r])] - ref[:, :, r])) for r in range(Rr.shape[1]))
+    assert err < tol, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+    return rho, eb
+
+
+@pytest.mark.parametrize("N,nk,mu,T", [(2, 36, 0.0, 0.01), (4, 64, 0.3, 0.0), (8, 49, -0.2, 0.05),
+                                       (33, 25, 0.0, 0.02), (64, 64, 0.0, 0.02), (96, 9, 0.1, 0.02)])
+def test_rho_random_models(pkg, oracle, gpu, N, nk, mu, T):
+    h = pkg.models.random_tb(N, rmax=1 if N > 8 else 2, seed=N)
+    _rho_vs_oracle(pkg, oracle, h, pkg.regulargrid(nk=nk), mu, T)
+
+
+def test_rho_zeeman_graphene_T0(pkg, oracle, gpu):
+    # the test_operators.jl:91-127 configuration
+    h = pkg.models.addspin(pkg.models.honeycomb_nn(-1.0), "spinhalf")
+    sy = np.array([[0, -1j], [1j, 0]])
+    h[(0, 0)] = h[(0, 0)] + np.kron(np.eye(2), 0.4 * sy)
+    nk = 9
+    ks = np.array([[i / nk, j / nk] for i in range(nk) for j in range(nk)]).T
+    _rho_vs_oracle(pkg, oracle, h, ks, -0.5, 0.0)
+
+
+def test_rho_weights_and_extra_keys(pkg, oracle, gpu):
+    h = pkg.models.random_tb(6, rmax=1, seed=6)
+    ks = pkg.regulargrid(nk=16)
+    w = np.random.default_rng(0).random(16)
+    keys = [(0, 0), (2, -1), (3, 3), (-1, 0)]
+    _rho_vs_oracle(pkg, oracle, h, ks, 0.0, 0.03, kweights=w, keys=keys)
+
+
+def test_rho_many_keys(pkg, oracle, gpu):
+    # more keys than one register group of the phase contraction
+    h = pkg.models.random_tb(5, rmax=3, seed=8)  # 49 keys
+    _rho_vs_oracle(pkg, oracle, h, pkg.regulargrid(nk=16), 0.1, 0.05)
+
+
+def test_rho_invariants_cfg2_shape(pkg, gpu):
+    # N=64, 25 keys, at a grid the oracle check above does not cover: size-free identities
+    h = pkg.models.random_tb(64, rmax=2, seed=2)
+    ks = pkg.regulargrid(nk=32 ** 2)
+    rho = _zero_like(pkg, h)
+    eb = pkg.api.getdensitymatrix(rho, h, ks, 0.0, T=0.02)
+    for R in rho.keys():
+        mR = tuple(-x for x in R)
+        assert np.max(np.abs(rho[mR] - rho[R].conj().T)) < 1e-12
+    pair = sum(np.trace(rho[R] @ h[tuple(-x for x in R)]) for R in rho.keys())
+    # keys of rho = keys of h cover only part of the Fourier content, so E_band is
+    # sum over the complete key set; with all |R| <= 2 blocks of h present the pairing is exact
+    assert abs(np.real(pair) - eb) < 1e-10 * max(1, abs(eb))
+
+
+def test_dos_vs_oracle(pkg, oracle, gpu):
+    h = pkg.models.honeycomb_nn(-1.0)
+    R, T = h.pack()
+    ks = pkg.regulargrid(nk=60 ** 2)
+    om = np.linspace(-3.05, 3.05, 401)
+    dos = pkg.api.getdos(h, om, ks, Gamma=0.05)
+    ref = oracle.dos(R, T, ks, om, 0.05)
+    assert np.max(np.abs(dos - ref)) < 1e-10 * np.max(ref)
+    om2, dos2 = pkg.api.getdos(h, -3.05, 3.05, 401, klin=60, Gamma=0.05)
+    assert np.allclose(om2, om) and np.max(np.abs(dos2 - ref)) < 1e-10 * np.max(ref)
+    w = np.full(ks.shape[1], 1.0 / ks.shape[1])
+    dw = pkg.api.getdos(h, om, ks, Gamma=0.05, kweights=w)
+    assert np.max(np.abs(dw - ref)) < 1e-10 * np.max(ref)
+
+
+@pytest.mark.parametrize("filling,T", [(0.5, 0.0), (0.25, 0.0), (1.0, 0.0), (0.0, 0.0), (0.5, 0.01), (0.37, 0.2)])
+def test_chemicalpotential_vs_oracle(pkg, oracle, gpu, filling, T):
+    h = pkg.models.random_tb(6, rmax=1, seed=12)
+    ks = pkg.regulargrid(nk=49)
+    R, Tt = h.pack()
+    bands = oracle.bands(R, Tt, ks)
+    mu = pkg.api.chemicalpotential(h, ks, filling, T=T)
+    if T == 0:
+        assert abs(mu - oracle.chemicalpotential(bands, filling, 0.0)) < 1e-12
+    else:
+        assert abs(oracle.getfilling(bands, mu, T) - filling) < 1e-12
+        assert abs(mu - oracle.chemicalpotential(bands, filling, T)) < 1e-10
+
+
+def test_setfilling_graphene(pkg, gpu):
+    # test_spectrum_graphene.jl: half filling of NN graphene puts mu at the Dirac point
+    h = pkg.models.honeycomb_nn(-1.0)
+    mu = pkg.api.setfilling(h, 0.5, nk=100)
+    assert abs(mu) < 1e-12
