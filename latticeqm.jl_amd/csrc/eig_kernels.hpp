@@ -293,13 +293,68 @@ __device__ __forceinline__ double fermidirac(double e, double mu, double T) {
 //   3. MODE 0: write evecs (ascending order); MODE 1: rho_k = V diag(f) V^H with
 //      f = fermidirac(e - mu; T), written as FP64, and eb[kk] = w_k sum_n e_n f_n.
 // ---------------------------------------------------------------------------------
-constexpr int kRotChunk = 512;
+constexpr int kRotChunk = 512;  // default rotation-staging capacity (>= longest sweep)
+constexpr int kHdrChunk = 256;  // sweep headers staged per LDS load
+
+// Apply one logged sweep to row rr of Z (column-major, leading dimension ldv):
+// rotations at i = i0, i0-1, ..., i0-cnt+1 with (c, s) pairs at cs[0..2cnt).
+// Four rotations per group; the next group's Z loads are issued before the
+// current group's stores so the carry chain is the only serial dependency.
+template <typename R>
+__device__ __forceinline__ void replay_sweep(R* __restrict__ zc, int ldv, int i0, int cnt,
+                                             const R* __restrict__ cs) {
+    R carry = zc[(size_t)(i0 + 1) * ldv];
+    int t = 0;
+    R z0 = 0, z1 = 0, z2 = 0, z3 = 0;
+    if (cnt >= 4) {
+        z0 = zc[(size_t)(i0)*ldv];
+        z1 = zc[(size_t)(i0 - 1) * ldv];
+        z2 = zc[(size_t)(i0 - 2) * ldv];
+        z3 = zc[(size_t)(i0 - 3) * ldv];
+    }
+    for (; t + 4 <= cnt; t += 4) {
+        R y0 = 0, y1 = 0, y2 = 0, y3 = 0;
+        if (t + 8 <= cnt) {
+            y0 = zc[(size_t)(i0 - t - 4) * ldv];
+            y1 = zc[(size_t)(i0 - t - 5) * ldv];
+            y2 = zc[(size_t)(i0 - t - 6) * ldv];
+            y3 = zc[(size_t)(i0 - t - 7) * ldv];
+        }
+        const R c0 = cs[2 * t], s0 = cs[2 * t + 1];
+        const R c1 = cs[2 * t + 2], s1 = cs[2 * t + 3];
+        const R c2 = cs[2 * t + 4], s2 = cs[2 * t + 5];
+        const R c3 = cs[2 * t + 6], s3 = cs[2 * t + 7];
+        const R o0 = s0 * z0 + c0 * carry;
+        carry = c0 * z0 - s0 * carry;
+        const R o1 = s1 * z1 + c1 * carry;
+        carry = c1 * z1 - s1 * carry;
+        const R o2 = s2 * z2 + c2 * carry;
+        carry = c2 * z2 - s2 * carry;
+        const R o3 = s3 * z3 + c3 * carry;
+        carry = c3 * z3 - s3 * carry;
+        zc[(size_t)(i0 - t + 1) * ldv] = o0;
+        zc[(size_t)(i0 - t) * ldv] = o1;
+        zc[(size_t)(i0 - t - 1) * ldv] = o2;
+        zc[(size_t)(i0 - t - 2) * ldv] = o3;
+        z0 = y0;
+        z1 = y1;
+        z2 = y2;
+        z3 = y3;
+    }
+    for (; t < cnt; ++t) {
+        const R zi = zc[(size_t)(i0 - t) * ldv];
+        const R c = cs[2 * t], s = cs[2 * t + 1];
+        zc[(size_t)(i0 - t + 1) * ldv] = s * zi + c * carry;
+        carry = c * zi - s * carry;
+    }
+    zc[(size_t)(i0 - cnt + 1) * ldv] = carry;
+}
 
 template <typename R, bool LDSMAT, int MODE>
 __global__ __launch_bounds__(kTPB) void k_backtransform(
     int N, int Nkc, const cx<R>* __restrict__ refl, const cx<R>* __restrict__ tau,
     const R* __restrict__ rot, const int* __restrict__ sweeps, const int* __restrict__ nsweeps,
-    int cap_rot, int cap_sw, const R* __restrict__ evals, const int* __restrict__ perm,
+    int cap_rot, int cap_sw, int rch, const R* __restrict__ evals, const int* __restrict__ perm,
     const double* __restrict__ kw, double wuni, double mu, double Temp, R* __restrict__ gV,
     cx<double>* __restrict__ out, double* __restrict__ eb) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -316,15 +371,19 @@ __global__ __launch_bounds__(kTPB) void k_backtransform(
         Vr = gV + (size_t)kk * 2 * plane;
     }
     R* Vi = Vr + plane;
-    R* srot = reinterpret_cast<R*>(p);          // [kRotChunk][2]
-    p += sizeof(R) * 2 * kRotChunk;
-    cx<R>* sv = reinterpret_cast<cx<R>*>(p);    // [N]
+    R* srot = reinterpret_cast<R*>(p);          // [rch][2]
+    p += sizeof(R) * 2 * rch;
+    cx<R>* stau = reinterpret_cast<cx<R>*>(p);  // [N]
     p += sizeof(cx<R>) * N;
+    cx<R>* sv = reinterpret_cast<cx<R>*>(p);    // [2][N] double-buffered reflector
+    p += sizeof(cx<R>) * 2 * N;
     cx<R>* sy = reinterpret_cast<cx<R>*>(p);    // [N]
     p += sizeof(cx<R>) * N;
     double* sf = reinterpret_cast<double*>(p);  // [N]
     p += sizeof(double) * N;
     double* sred = reinterpret_cast<double*>(p);  // [4]
+    p += sizeof(double) * 4;
+    int* shdr = reinterpret_cast<int*>(p);        // [kHdrChunk]
 
     // 1. Z = I, then the logged QL rotations
     for (int e = tid; e < N * N; e += kTPB) {
@@ -332,60 +391,59 @@ __global__ __launch_bounds__(kTPB) void k_backtransform(
         Vr[(size_t)c * ldv + i] = (i == c) ? R(1) : R(0);
         Vi[(size_t)c * ldv + i] = R(0);
     }
-    __syncthreads();
+    const cx<R>* rf = refl + (size_t)kk * N * N;
+    for (int j = tid; j < N; j += kTPB) stau[j] = tau[(size_t)kk * N + j];
     const R* lrot = rot + (size_t)kk * cap_rot * 2;
     const int* lsw = sweeps + (size_t)kk * cap_sw;
     const int nsw = nsweeps[kk];
-    int pos = 0;
-    int cbase = -kRotChunk;  // first log position held in srot
+    int pos = 0, cbase = 0, cend = 0;  // srot holds log positions [cbase, cend)
+    int hbase = 0, hend = 0;           // shdr holds sweep headers [hbase, hend)
+    __syncthreads();
     for (int sidx = 0; sidx < nsw; ++sidx) {
-        const int h = lsw[sidx];
-        const int i0 = h >> 16, cnt = h & 0xffff;
-        for (int row = 0; row * kTPB < N || row == 0; ++row) {
-            // every thread walks the sweep so the chunk reloads stay uniform
-            const int rr = row * kTPB + tid;
-            const bool act = rr < N;
-            R carry = act ? Vr[(size_t)(i0 + 1) * ldv + rr] : R(0);
-            for (int t = 0; t < cnt; ++t) {
-                const int pp = pos + t;
-                if (pp >= cbase + kRotChunk || pp < cbase) {
-                    __syncthreads();
-                    const int nload = min(kRotChunk, cap_rot - pp);
-                    for (int q = tid; q < 2 * nload; q += kTPB) srot[q] = lrot[2 * (size_t)pp + q];
-                    cbase = pp;
-                    __syncthreads();
-                }
-                const R c = srot[2 * (pp - cbase)], s = srot[2 * (pp - cbase) + 1];
-                const int i = i0 - t;
-                if (act) {
-                    R zi = Vr[(size_t)i * ldv + rr];
-                    Vr[(size_t)(i + 1) * ldv + rr] = s * zi + c * carry;
-                    carry = c * zi - s * carry;
-                }
-            }
-            if (act) Vr[(size_t)(i0 - cnt + 1) * ldv + rr] = carry;
-            if ((row + 1) * kTPB >= N) break;
+        if (sidx >= hend) {  // uniform: every thread walks the same sweeps
+            __syncthreads();
+            const int nh = min(kHdrChunk, cap_sw - sidx);
+            for (int q = tid; q < nh; q += kTPB) shdr[q] = lsw[sidx + q];
+            hbase = sidx;
+            hend = sidx + nh;
+            __syncthreads();
         }
+        const int h = shdr[sidx - hbase];
+        const int i0 = h >> 16, cnt = h & 0xffff;
+        if (pos + cnt > cend) {
+            __syncthreads();
+            const int nload = min(rch, cap_rot - pos);
+            for (int q = tid; q < 2 * nload; q += kTPB) srot[q] = lrot[2 * (size_t)pos + q];
+            cbase = pos;
+            cend = pos + nload;
+            __syncthreads();
+        }
+        const R* cs = srot + 2 * (pos - cbase);
+        for (int rr = tid; rr < N; rr += kTPB) replay_sweep(Vr + rr, ldv, i0, cnt, cs);
         pos += cnt;
     }
-    __syncthreads();
 
-    // 2. V = H_0 ... H_{N-2} Z
-    const cx<R>* rf = refl + (size_t)kk * N * N;
+    // 2. V = H_0 ... H_{N-2} Z.  Reflector j-1 is fetched into the other half of sv
+    //    while reflector j is applied, so no global latency sits between barriers.
+    {
+        const int j0 = N - 2;
+        if (j0 >= 0)
+            for (int i = tid; i < N - 1 - j0; i += kTPB) sv[(j0 & 1) * N + i] = rf[(size_t)j0 * N + j0 + 1 + i];
+    }
+    __syncthreads();
     for (int j = N - 2; j >= 0; --j) {
         const int m = N - 1 - j;
-        const cx<R> tj = tau[(size_t)kk * N + j];
-        if (tj.re == R(0) && tj.im == R(0)) continue;
-        for (int i = tid; i < m; i += kTPB) sv[i] = rf[(size_t)j * N + j + 1 + i];
-        __syncthreads();
+        const cx<R> tj = stau[j];
+        const cx<R>* svj = sv + (j & 1) * N;
+        const bool skip = (tj.re == R(0) && tj.im == R(0));
         // y_c = tau * v^H V[j+1:, c]   (4 lanes per column)
-        {
+        if (!skip) {
             const int q = tid & 3;
             for (int c = tid >> 2; c < N; c += kTPB / 4) {
                 cx<R> acc = {0, 0};
                 const R* vr = Vr + (size_t)c * ldv + j + 1;
                 const R* vi = Vi + (size_t)c * ldv + j + 1;
-                for (int i = q; i < m; i += 4) acc = cmadc(acc, sv[i], mkc<R>(vr[i], vi[i]));
+                for (int i = q; i < m; i += 4) acc = cmadc(acc, svj[i], mkc<R>(vr[i], vi[i]));
                 acc.re += __shfl_xor(acc.re, 1);
                 acc.im += __shfl_xor(acc.im, 1);
                 acc.re += __shfl_xor(acc.re, 2);
@@ -394,12 +452,18 @@ __global__ __launch_bounds__(kTPB) void k_backtransform(
             }
         }
         __syncthreads();
+        // prefetch reflector j-1 into the other buffer (its last reader was step j+1)
+        if (j >= 1) {
+            const int jn = j - 1, mn = N - 1 - jn;
+            cx<R>* dst = sv + (jn & 1) * N;
+            for (int i = tid; i < mn; i += kTPB) dst[i] = rf[(size_t)jn * N + jn + 1 + i];
+        }
         // V[j+1:, :] -= v y
-        {
+        if (!skip) {
             const int rr = tid & (kWave - 1);
             const int cq = tid / kWave;
             for (int r = rr; r < m; r += kWave) {
-                const cx<R> vr = sv[r];
+                const cx<R> vr = svj[r];
                 for (int c = cq; c < N; c += kTPB / kWave) {
                     const size_t o = (size_t)c * ldv + j + 1 + r;
                     cx<R> u = vr * sy[c];

@@ -112,7 +112,8 @@ struct Plan {
     size_t shA = 0, shC = 0, shB = 0;
     int laneB = 64;
     int cap_rot = 0, cap_sw = 0;
-    int Kc = 0;  // k-points per chunk
+    int rch = 0;  // rotation-staging capacity of K_C
+    int Kc = 0;   // k-points per chunk
 };
 
 Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool rho) {
@@ -123,7 +124,9 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     p.shA = sizeof(cx<double>) * ((size_t)N * (N + 1) + 2 * N + 6) + sizeof(cx<double>) * nR;
     p.ldsA = p.shA <= (size_t)c->lds_max;
     if (!p.ldsA) p.shA = sizeof(cx<double>) * (2 * N + 6) + sizeof(cx<double>) * nR;
-    const size_t smallC = cR * 2 * lqm::kRotChunk + sizeof(cx<double>) * 2 * N + 8 * N + 64;
+    p.rch = std::max(lqm::kRotChunk, N);
+    const size_t smallC = cR * 2 * p.rch + sizeof(cx<double>) * 4 * N + 8 * N + 32 +
+                          sizeof(int) * lqm::kHdrChunk + 16;
     p.shC = 2 * cR * (size_t)N * (N + 1) + smallC;
     p.ldsC = p.shC <= (size_t)c->lds_max;
     if (!p.ldsC) p.shC = smallC;
@@ -251,12 +254,12 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const doubl
         auto kC = lqm::k_backtransform<double, true, MODE>;
         allow_lds(kC, P.shC);
         hipLaunchKernelGGL(kC, dim3(Kc), dim3(lqm::kTPB), P.shC, c->stream, N, Kc, refl, tau, rot, sw,
-                           nsw, P.cap_rot, P.cap_sw, evals, perm, kw, wuni, mu, T, gV, out, eb);
+                           nsw, P.cap_rot, P.cap_sw, P.rch, evals, perm, kw, wuni, mu, T, gV, out, eb);
     } else {
         auto kC = lqm::k_backtransform<double, false, MODE>;
         allow_lds(kC, P.shC);
         hipLaunchKernelGGL(kC, dim3(Kc), dim3(lqm::kTPB), P.shC, c->stream, N, Kc, refl, tau, rot, sw,
-                           nsw, P.cap_rot, P.cap_sw, evals, perm, kw, wuni, mu, T, gV, out, eb);
+                           nsw, P.cap_rot, P.cap_sw, P.rch, evals, perm, kw, wuni, mu, T, gV, out, eb);
     }
     LQM_HIP(hipGetLastError());
     return LQM_OK;
