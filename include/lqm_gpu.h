@@ -64,9 +64,9 @@ lqm_status lqm_context_synchronize(lqm_context *ctx);
  * When enabled every kernel launch is bracketed by HIP events on the context
  * stream.  lqm_context_stage_times returns, per stage, the accumulated device
  * milliseconds and launch counts since the last query, then resets them.
- * Stages: 0 hk_tridiag, 1 tridiag_ql, 2 backtransform, 3 phase_contract,
- * 4 phase_fold, 5 dos, 6 reduce (LQM_NSTAGES = 7). */
-#define LQM_NSTAGES 7
+ * Stages: 0 hk_tridiag, 1 tridiag_ql, 2 replay, 3 backtransform,
+ * 4 phase_contract, 5 phase_fold, 6 dos, 7 reduce (LQM_NSTAGES = 8). */
+#define LQM_NSTAGES 8
 lqm_status lqm_context_set_timing(lqm_context *ctx, int enable);
 lqm_status lqm_context_stage_times(lqm_context *ctx, double *ms, int64_t *launches, int n);
 const char *lqm_stage_name(int stage);

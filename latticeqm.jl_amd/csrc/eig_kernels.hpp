@@ -350,13 +350,88 @@ __device__ __forceinline__ void replay_sweep(R* __restrict__ zc, int ldv, int i0
     zc[(size_t)(i0 - cnt + 1) * ldv] = carry;
 }
 
-template <typename R, bool LDSMAT, int MODE>
+// 64-bit broadcast from a wave-uniform lane (v_readlane into scalar registers).
+template <typename R>
+__device__ __forceinline__ R bcast_lane(R v, int lane);
+template <>
+__device__ __forceinline__ double bcast_lane<double>(double v, int lane) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)x, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(x >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <>
+__device__ __forceinline__ float bcast_lane<float>(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// ---------------------------------------------------------------------------------
+// K_R: QL rotation replay for N <= NM <= 64, one wave per k-point.  Lane r keeps row r
+// of Z in registers (fully unrolled over the NM-1 rotation positions; positions
+// outside a sweep are skipped by a wave-uniform branch).  Each sweep (<= 63
+// rotations) arrives with one coalesced load -- lane t holds pair t -- issued one
+// sweep ahead, and pairs are broadcast with readlane, so no LDS is used and many
+// waves share a CU.  Output: Z[kk][c*N + r] (column-major, real).
+// ---------------------------------------------------------------------------------
+template <typename R, int NM>
+__global__ __launch_bounds__(kWave) void k_replay(int N, const R* __restrict__ rot,
+                                                   const int* __restrict__ sweeps,
+                                                   const int* __restrict__ nsweeps, int cap_rot,
+                                                   int cap_sw, R* __restrict__ Z) {
+    const int kk = blockIdx.x;
+    const int lane = threadIdx.x;
+    R z[NM];
+#pragma unroll
+    for (int c = 0; c < NM; ++c) z[c] = (c == lane) ? R(1) : R(0);
+    const R* lrot = rot + (size_t)kk * cap_rot * 2;
+    const int* lsw = sweeps + (size_t)kk * cap_sw;
+    const int nsw = nsweeps[kk];
+    int pos = 0;
+    // pairs of the current sweep: lane t holds (c_t, s_t)
+    R cc = 0, ss = 0;
+    if (nsw > 0 && lane < min(kWave, cap_rot)) {
+        cc = lrot[2 * lane];
+        ss = lrot[2 * lane + 1];
+    }
+    for (int sidx = 0; sidx < nsw; ++sidx) {
+        const int h = lsw[sidx];
+        const int i0 = h >> 16, cnt = h & 0xffff;
+        // prefetch the next sweep's pairs
+        const int pn = pos + cnt;
+        R nc = 0, ns = 0;
+        if (sidx + 1 < nsw && pn + lane < cap_rot) {
+            nc = lrot[2 * (size_t)(pn + lane)];
+            ns = lrot[2 * (size_t)(pn + lane) + 1];
+        }
+#pragma unroll
+        for (int i = NM - 2; i >= 0; --i) {
+            if (i <= i0 && i > i0 - cnt) {
+                const int t = i0 - i;
+                const R c = bcast_lane(cc, t), s = bcast_lane(ss, t);
+                const R zi = z[i], zi1 = z[i + 1];
+                z[i + 1] = s * zi + c * zi1;
+                z[i] = c * zi - s * zi1;
+            }
+        }
+        cc = nc;
+        ss = ns;
+        pos = pn;
+    }
+    if (lane < N) {
+        R* zo = Z + (size_t)kk * N * N + lane;
+#pragma unroll
+        for (int c = 0; c < NM; ++c)
+            if (c < N) zo[(size_t)c * N] = z[c];
+    }
+}
+
+template <typename R, bool LDSMAT, int MODE, bool ZIN>
 __global__ __launch_bounds__(kTPB) void k_backtransform(
     int N, int Nkc, const cx<R>* __restrict__ refl, const cx<R>* __restrict__ tau,
     const R* __restrict__ rot, const int* __restrict__ sweeps, const int* __restrict__ nsweeps,
     int cap_rot, int cap_sw, int rch, const R* __restrict__ evals, const int* __restrict__ perm,
     const double* __restrict__ kw, double wuni, double mu, double Temp, R* __restrict__ gV,
-    cx<double>* __restrict__ out, double* __restrict__ eb) {
+    const R* __restrict__ zin, cx<double>* __restrict__ out, double* __restrict__ eb) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int kk = blockIdx.x;
     const int tid = threadIdx.x;
@@ -385,14 +460,22 @@ __global__ __launch_bounds__(kTPB) void k_backtransform(
     p += sizeof(double) * 4;
     int* shdr = reinterpret_cast<int*>(p);        // [kHdrChunk]
 
-    // 1. Z = I, then the logged QL rotations
+    // 1. Z: replayed already (ZIN, from k_replay) or Z = I and the logged QL rotations
+    const cx<R>* rf = refl + (size_t)kk * N * N;
+    for (int j = tid; j < N; j += kTPB) stau[j] = tau[(size_t)kk * N + j];
+    if constexpr (ZIN) {
+        const R* zk = zin + (size_t)kk * N * N;
+        for (int e = tid; e < N * N; e += kTPB) {
+            int i = e % N, c = e / N;
+            Vr[(size_t)c * ldv + i] = zk[e];
+            Vi[(size_t)c * ldv + i] = R(0);
+        }
+    } else {
     for (int e = tid; e < N * N; e += kTPB) {
         int i = e % N, c = e / N;
         Vr[(size_t)c * ldv + i] = (i == c) ? R(1) : R(0);
         Vi[(size_t)c * ldv + i] = R(0);
     }
-    const cx<R>* rf = refl + (size_t)kk * N * N;
-    for (int j = tid; j < N; j += kTPB) stau[j] = tau[(size_t)kk * N + j];
     const R* lrot = rot + (size_t)kk * cap_rot * 2;
     const int* lsw = sweeps + (size_t)kk * cap_sw;
     const int nsw = nsweeps[kk];
@@ -421,6 +504,7 @@ __global__ __launch_bounds__(kTPB) void k_backtransform(
         const R* cs = srot + 2 * (pos - cbase);
         for (int rr = tid; rr < N; rr += kTPB) replay_sweep(Vr + rr, ldv, i0, cnt, cs);
         pos += cnt;
+    }
     }
 
     // 2. V = H_0 ... H_{N-2} Z.  Reflector j-1 is fetched into the other half of sv
@@ -452,11 +536,14 @@ __global__ __launch_bounds__(kTPB) void k_backtransform(
             }
         }
         __syncthreads();
-        // prefetch reflector j-1 into the other buffer (its last reader was step j+1)
-        if (j >= 1) {
-            const int jn = j - 1, mn = N - 1 - jn;
-            cx<R>* dst = sv + (jn & 1) * N;
-            for (int i = tid; i < mn; i += kTPB) dst[i] = rf[(size_t)jn * N + jn + 1 + i];
+        // fetch reflector j-1 into registers now, store it to the other buffer (last
+        // read in step j+1) after the update, so the global latency overlaps the update
+        const int jn = j - 1, mn = N - 1 - jn;
+        R pre_re = 0, pre_im = 0;  // scalars: a struct here ends up in scratch
+        if (j >= 1 && tid < mn) {
+            const R* src = reinterpret_cast<const R*>(rf + (size_t)jn * N + jn + 1 + tid);
+            pre_re = src[0];
+            pre_im = src[1];
         }
         // V[j+1:, :] -= v y
         if (!skip) {
@@ -471,6 +558,11 @@ __global__ __launch_bounds__(kTPB) void k_backtransform(
                     Vi[o] -= u.im;
                 }
             }
+        }
+        if (j >= 1) {
+            cx<R>* dst = sv + (jn & 1) * N;
+            if (tid < mn) dst[tid] = mkc<R>(pre_re, pre_im);
+            for (int i = tid + kTPB; i < mn; i += kTPB) dst[i] = rf[(size_t)jn * N + jn + 1 + i];
         }
         __syncthreads();
     }

@@ -36,7 +36,7 @@ lqm_status fail(lqm_status st, const std::string& msg) {
 
 enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
-    W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_NSLOTS
+    W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -44,8 +44,8 @@ constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch
 }  // namespace
 
 // Kernel stages timed with HIP events when timing is on.
-enum Stage { ST_TRIDIAG, ST_QL, ST_BACKTRANSFORM, ST_CONTRACT, ST_FOLD, ST_DOS, ST_REDUCE, ST_N };
-const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "backtransform",
+enum Stage { ST_TRIDIAG, ST_QL, ST_REPLAY, ST_BACKTRANSFORM, ST_CONTRACT, ST_FOLD, ST_DOS, ST_REDUCE, ST_N };
+const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "backtransform",
                                        "phase_contract", "phase_fold", "dos", "reduce"};
 
 struct TimedLaunch {
@@ -113,6 +113,7 @@ struct Plan {
     int laneB = 64;
     int cap_rot = 0, cap_sw = 0;
     int rch = 0;  // rotation-staging capacity of K_C
+    int nm = 0;   // register row length of k_replay (0: replay inside K_C)
     int Kc = 0;   // k-points per chunk
 };
 
@@ -142,6 +143,8 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     if (!p.ldsA) perk += 16 * NN;
     if ((vectors || rho) && !p.ldsC) perk += 16 * NN;
     if (rho) perk += 16 * NN;
+    p.nm = N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : N <= 64 ? 64 : 0;
+    if ((vectors || rho) && p.nm) perk += 8 * NN;
     size_t kc = kWorkspaceBudget / perk;
     if (kc < 1) kc = 1;
     p.Kc = (int)std::min<size_t>(kc, (size_t)Nk);
@@ -232,6 +235,35 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
     return LQM_OK;
 }
 
+template <int NM>
+void launch_replay(lqm_context* c, int N, int Kc, const double* rot, const int* sw, const int* nsw,
+                   int cap_rot, int cap_sw, double* Z) {
+    StageScope sc(c, ST_REPLAY);
+    auto kR = lqm::k_replay<double, NM>;
+    hipLaunchKernelGGL(kR, dim3(Kc), dim3(lqm::kWave), 0, c->stream, N, rot, sw, nsw, cap_rot, cap_sw, Z);
+}
+
+template <int MODE, bool ZIN>
+void launch_kc(lqm_context* c, const Plan& P, int N, int Kc, const cx<double>* refl,
+               const cx<double>* tau, const double* rot, const int* sw, const int* nsw,
+               const double* evals, const int* perm, const double* kw, double wuni, double mu,
+               double T, double* gV, const double* zin, cx<double>* out, double* eb) {
+    StageScope sc(c, ST_BACKTRANSFORM);
+    if (P.ldsC) {
+        auto kC = lqm::k_backtransform<double, true, MODE, ZIN>;
+        allow_lds(kC, P.shC);
+        hipLaunchKernelGGL(kC, dim3(Kc), dim3(lqm::kTPB), P.shC, c->stream, N, Kc, refl, tau, rot, sw,
+                           nsw, P.cap_rot, P.cap_sw, P.rch, evals, perm, kw, wuni, mu, T, gV, zin, out,
+                           eb);
+    } else {
+        auto kC = lqm::k_backtransform<double, false, MODE, ZIN>;
+        allow_lds(kC, P.shC);
+        hipLaunchKernelGGL(kC, dim3(Kc), dim3(lqm::kTPB), P.shC, c->stream, N, Kc, refl, tau, rot, sw,
+                           nsw, P.cap_rot, P.cap_sw, P.rch, evals, perm, kw, wuni, mu, T, gV, zin, out,
+                           eb);
+    }
+}
+
 template <int MODE>
 lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const double* evals,
                                 const double* kw, double wuni, double mu, double T,
@@ -249,17 +281,22 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const doubl
     auto* sw = static_cast<int*>(c->buf[W_SW]);
     auto* nsw = static_cast<int*>(c->buf[W_NSW]);
     auto* perm = static_cast<int*>(c->buf[W_PERM]);
-    StageScope sc(c, ST_BACKTRANSFORM);
-    if (P.ldsC) {
-        auto kC = lqm::k_backtransform<double, true, MODE>;
-        allow_lds(kC, P.shC);
-        hipLaunchKernelGGL(kC, dim3(Kc), dim3(lqm::kTPB), P.shC, c->stream, N, Kc, refl, tau, rot, sw,
-                           nsw, P.cap_rot, P.cap_sw, P.rch, evals, perm, kw, wuni, mu, T, gV, out, eb);
+    if (P.nm) {
+        double* Z;
+        lqm_status st = wsT(c, W_Z, (size_t)N * N * Kc, &Z);
+        if (st) return st;
+        switch (P.nm) {
+            case 8: launch_replay<8>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+            case 16: launch_replay<16>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+            case 32: launch_replay<32>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+            default: launch_replay<64>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+        }
+        LQM_HIP(hipGetLastError());
+        launch_kc<MODE, true>(c, P, N, Kc, refl, tau, rot, sw, nsw, evals, perm, kw, wuni, mu, T, gV, Z,
+                              out, eb);
     } else {
-        auto kC = lqm::k_backtransform<double, false, MODE>;
-        allow_lds(kC, P.shC);
-        hipLaunchKernelGGL(kC, dim3(Kc), dim3(lqm::kTPB), P.shC, c->stream, N, Kc, refl, tau, rot, sw,
-                           nsw, P.cap_rot, P.cap_sw, P.rch, evals, perm, kw, wuni, mu, T, gV, out, eb);
+        launch_kc<MODE, false>(c, P, N, Kc, refl, tau, rot, sw, nsw, evals, perm, kw, wuni, mu, T, gV,
+                               nullptr, out, eb);
     }
     LQM_HIP(hipGetLastError());
     return LQM_OK;
