@@ -111,6 +111,7 @@ def main() -> int:
                                      vp(bands.data_ptr()) if bands is not None else None)
         _lib.check(st, "lqm_densitymatrix_dev")
 
+    chosen = api.autotune(model, ks_d.data_ptr(), min(Nk, 2048))
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -189,7 +190,8 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic (seeded random tight-binding model)",
         "config": {"workload": args.workload, "N": N, "nR": nR, "nRrho": nRr, "Nk": Nk, "mu": mu,
-                   "kT": T, "precision": "fp64 plain path", "parallelism": "single GPU"},
+                   "kT": T, "precision": "fp64 plain path", "parallelism": "single GPU",
+                   "kernel_variants": {"tridiag": chosen[0], "rho": chosen[1]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages": stages,

@@ -64,12 +64,24 @@ lqm_status lqm_context_synchronize(lqm_context *ctx);
  * When enabled every kernel launch is bracketed by HIP events on the context
  * stream.  lqm_context_stage_times returns, per stage, the accumulated device
  * milliseconds and launch counts since the last query, then resets them.
- * Stages: 0 hk_tridiag, 1 tridiag_ql, 2 replay, 3 backtransform,
- * 4 phase_contract, 5 phase_fold, 6 dos, 7 reduce (LQM_NSTAGES = 8). */
-#define LQM_NSTAGES 8
+ * Stages: 0 hk_tridiag, 1 tridiag_ql, 2 replay, 3 backtransform, 4 zfz,
+ * 5 rho_twosided, 6 phase_contract, 7 phase_fold, 8 dos, 9 reduce (LQM_NSTAGES = 10);
+ * lqm_stage_name gives the names. */
+#define LQM_NSTAGES 10
 lqm_status lqm_context_set_timing(lqm_context *ctx, int enable);
 lqm_status lqm_context_stage_times(lqm_context *ctx, double *ms, int64_t *launches, int n);
 const char *lqm_stage_name(int stage);
+
+/* Kernel variants (no reference counterpart; the reference's only tunable is
+ * the executor).  kind 0: tridiagonalization (0 = 256-thread full tile, 1 = one
+ * wave with the packed Hermitian tile); kind 1: rho_k (0 = V = QZ then V f V^H,
+ * 1 = two-sided reflectors on Z f Z^T).  variant -1 returns to the tuned/default
+ * choice.  lqm_autotune times every variant of every kind on Nk device k-points of
+ * the model and records the fastest per orbital count N in the context;
+ * lqm_context_get_variant reports the choice in force (-1: default, not tuned). */
+lqm_status lqm_context_set_variant(lqm_context *ctx, int kind, int variant);
+lqm_status lqm_context_get_variant(lqm_context *ctx, int kind, int N, int *variant);
+lqm_status lqm_autotune(lqm_model *m, int Nk, const double *ks_dev);
 
 /* ---- model: the Hops operator, resident in HBM --------------------------------- */
 /* Upload of Hops{Vector{Int},Matrix{ComplexF64}} (TightBinding/types.jl:54-60):

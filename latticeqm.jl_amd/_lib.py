@@ -38,6 +38,9 @@ _PROTOS = {
     "lqm_context_set_timing": [P, I],
     "lqm_context_stage_times": [P, P, P, I],
     "lqm_stage_name": [I],
+    "lqm_context_set_variant": [P, I, I],
+    "lqm_context_get_variant": [P, I, I, P],
+    "lqm_autotune": [P, I, P],
     "lqm_model_create": [P, I, I, I, P, P, PP],
     "lqm_model_create_dev": [P, I, I, I, P, P, PP],
     "lqm_model_destroy": [P],
@@ -53,7 +56,7 @@ _PROTOS = {
     "lqm_densitymatrix_dev": [P, I, P, P, D, D, I, P, P, P, P],
 }
 _RESTYPE = {"lqm_last_error": ctypes.c_char_p, "lqm_abi_version": I, "lqm_stage_name": ctypes.c_char_p}
-NSTAGES = 8
+NSTAGES = 10
 
 
 def header_symbols(path: str = HEADER) -> list[str]:

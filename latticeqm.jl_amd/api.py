@@ -51,6 +51,16 @@ class GPUExec:
     def synchronize(self) -> None:
         _lib.call("lqm_context_synchronize", self.handle)
 
+    def set_variant(self, kind: int, variant: int) -> None:
+        """Force kernel variant ``variant`` (-1: tuned/default) for kind 0 (tridiagonalization)
+        or kind 1 (rho_k construction)."""
+        _lib.call("lqm_context_set_variant", self.handle, int(kind), int(variant))
+
+    def get_variant(self, kind: int, N: int) -> int:
+        v = _c.c_int()
+        _lib.call("lqm_context_get_variant", self.handle, int(kind), int(N), _c.byref(v))
+        return v.value
+
     def set_stream(self, stream_ptr: int | None) -> None:
         _lib.call("lqm_context_set_stream", self.handle, _c.c_void_p(stream_ptr or 0))
 
@@ -105,6 +115,12 @@ class GPUModel:
             self.close()
         except Exception:
             pass
+
+
+def autotune(model: GPUModel, ks_dev_ptr: int, Nk: int) -> dict:
+    """Time every kernel variant on Nk device-resident k-points and keep the fastest."""
+    _lib.call("lqm_autotune", model.handle, int(Nk), _c.c_void_p(ks_dev_ptr))
+    return {kind: model.exec.get_variant(kind, model.N) for kind in (0, 1)}
 
 
 def _model(H, executor) -> GPUModel:

@@ -9,12 +9,14 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
 #include "lqm_gpu.h"
 #include "aux_kernels.hpp"
 #include "eig_kernels.hpp"
+#include "wave_kernels.hpp"
 
 using lqm::cx;
 
@@ -36,7 +38,7 @@ lqm_status fail(lqm_status st, const std::string& msg) {
 
 enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
-    W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_NSLOTS
+    W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -44,16 +46,24 @@ constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch
 }  // namespace
 
 // Kernel stages timed with HIP events when timing is on.
-enum Stage { ST_TRIDIAG, ST_QL, ST_REPLAY, ST_BACKTRANSFORM, ST_CONTRACT, ST_FOLD, ST_DOS, ST_REDUCE, ST_N };
-const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "backtransform",
-                                       "phase_contract", "phase_fold", "dos", "reduce"};
+enum Stage {
+    ST_TRIDIAG, ST_QL, ST_REPLAY, ST_BACKTRANSFORM, ST_ZFZ, ST_RHO2, ST_CONTRACT, ST_FOLD, ST_DOS,
+    ST_REDUCE, ST_N
+};
+const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "backtransform", "zfz",
+                                       "rho_twosided", "phase_contract", "phase_fold", "dos", "reduce"};
 
 struct TimedLaunch {
     int stage;
     hipEvent_t a, b;
 };
 
+// Kernel variants the autotuner chooses between (per N).
+enum VariantKind { VK_TRIDIAG = 0, VK_RHO = 1, VK_N };
+
 struct lqm_context {
+    std::map<int, int> tuned;  // (kind << 20 | N) -> variant
+    int forced[VK_N] = {-1, -1};
     int timing = 0;
     std::vector<TimedLaunch> pending;
     double stage_ms[ST_N] = {};
@@ -114,6 +124,10 @@ struct Plan {
     int cap_rot = 0, cap_sw = 0;
     int rch = 0;  // rotation-staging capacity of K_C
     int nm = 0;   // register row length of k_replay (0: replay inside K_C)
+    int nqA = 0;  // rows per lane of the one-wave tridiagonalization (0: 256-thread kernel)
+    size_t shAw = 0;
+    bool rho2 = false;  // rho_k by two-sided reflectors on the packed Z f Z^T
+    size_t shM = 0, shR2 = 0;
     int Kc = 0;   // k-points per chunk
 };
 
@@ -145,6 +159,21 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     if (rho) perk += 16 * NN;
     p.nm = N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : N <= 64 ? 64 : 0;
     if ((vectors || rho) && p.nm) perk += 8 * NN;
+    const size_t np = (size_t)N * (N + 1) / 2;
+    p.shAw = sizeof(cx<double>) * (np + 2 * N + nR);
+    auto variant = [&](int kind) {
+        if (c->forced[kind] >= 0) return c->forced[kind];
+        auto it = c->tuned.find((kind << 20) | N);
+        return it == c->tuned.end() ? 0 : it->second;
+    };
+    // tridiagonalization: 0 = 256-thread full-tile kernel, 1 = one wave, packed tile
+    if (variant(VK_TRIDIAG) == 1 && N <= 128 && p.shAw <= (size_t)c->lds_max) p.nqA = N <= 64 ? 1 : 2;
+    p.shM = sizeof(double) * ((size_t)N * (N + 1) + 2 + N + 4);
+    p.shR2 = sizeof(cx<double>) * (np + 2 * N);
+    // rho_k: 0 = V = QZ then V f V^H, 1 = two-sided reflectors on Z f Z^T
+    p.rho2 = variant(VK_RHO) == 1 && rho && p.nm && p.shM <= (size_t)c->lds_max &&
+             p.shR2 <= (size_t)c->lds_max;
+    if (p.rho2) perk += 8 * NN;
     size_t kc = kWorkspaceBudget / perk;
     if (kc < 1) kc = 1;
     p.Kc = (int)std::min<size_t>(kc, (size_t)Nk);
@@ -202,7 +231,17 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
     }
     {
     StageScope sc(c, ST_TRIDIAG);
-    if (P.ldsA) {
+    if (P.nqA == 1) {
+        auto kA = lqm::k_hk_tridiag_w<double, 1>;
+        allow_lds(kA, P.shAw);
+        hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kWave), P.shAw, c->stream, N, m->d, m->nR, m->dR,
+                           m->dT, ks, Kc, dd, ee, refl, tau);
+    } else if (P.nqA == 2) {
+        auto kA = lqm::k_hk_tridiag_w<double, 2>;
+        allow_lds(kA, P.shAw);
+        hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kWave), P.shAw, c->stream, N, m->d, m->nR, m->dR,
+                           m->dT, ks, Kc, dd, ee, refl, tau);
+    } else if (P.ldsA) {
         auto kA = lqm::k_hk_tridiag<double, true>;
         allow_lds(kA, P.shA);
         hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kTPB), P.shA, c->stream, N, m->d, m->nR, m->dR,
@@ -297,6 +336,46 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const doubl
     } else {
         launch_kc<MODE, false>(c, P, N, Kc, refl, tau, rot, sw, nsw, evals, perm, kw, wuni, mu, T, gV,
                                nullptr, out, eb);
+    }
+    LQM_HIP(hipGetLastError());
+    return LQM_OK;
+}
+
+// rho_k for N <= 64: replay -> M = Z f Z^T -> two-sided reflectors on packed M.
+lqm_status launch_rho2(lqm_model* m, const Plan& P, int Kc, const double* evals, const double* kw,
+                       double wuni, double mu, double T, cx<double>* rhok, double* eb) {
+    lqm_context* c = m->ctx;
+    const int N = m->N;
+    auto* refl = static_cast<cx<double>*>(c->buf[W_REFL]);
+    auto* tau = static_cast<cx<double>*>(c->buf[W_TAU]);
+    auto* rot = static_cast<double*>(c->buf[W_ROT]);
+    auto* sw = static_cast<int*>(c->buf[W_SW]);
+    auto* nsw = static_cast<int*>(c->buf[W_NSW]);
+    auto* perm = static_cast<int*>(c->buf[W_PERM]);
+    double *Z, *M;
+    lqm_status st;
+    if ((st = wsT(c, W_Z, (size_t)N * N * Kc, &Z))) return st;
+    if ((st = wsT(c, W_M, (size_t)N * N * Kc, &M))) return st;
+    switch (P.nm) {
+        case 8: launch_replay<8>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+        case 16: launch_replay<16>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+        case 32: launch_replay<32>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+        default: launch_replay<64>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+    }
+    LQM_HIP(hipGetLastError());
+    {
+        StageScope sc(c, ST_ZFZ);
+        auto kM = lqm::k_zfz<double>;
+        allow_lds(kM, P.shM);
+        hipLaunchKernelGGL(kM, dim3(Kc), dim3(lqm::kTPB), P.shM, c->stream, N, Z, evals, perm, kw, wuni,
+                           mu, T, M, eb);
+    }
+    LQM_HIP(hipGetLastError());
+    {
+        StageScope sc(c, ST_RHO2);
+        auto kR2 = lqm::k_rho_w<double, 1>;
+        allow_lds(kR2, P.shR2);
+        hipLaunchKernelGGL(kR2, dim3(Kc), dim3(lqm::kWave), P.shR2, c->stream, N, refl, tau, M, rhok);
     }
     LQM_HIP(hipGetLastError());
     return LQM_OK;
@@ -401,7 +480,11 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
         double* eb;
         if ((st = wsT(c, W_RHOK, NN * P.Kc, &rhok))) return st;
         if ((st = wsT(c, W_EB, (size_t)P.Kc, &eb))) return st;
-        if ((st = launch_backtransform<1>(m, P, Kc, ev, kwc, wuni, mu, T, rhok, eb))) return st;
+        if (P.rho2) {
+            if ((st = launch_rho2(m, P, Kc, ev, kwc, wuni, mu, T, rhok, eb))) return st;
+        } else if ((st = launch_backtransform<1>(m, P, Kc, ev, kwc, wuni, mu, T, rhok, eb))) {
+            return st;
+        }
         {
             StageScope sc(c, ST_REDUCE);
             hipLaunchKernelGGL(lqm::k_sum, dim3(1), dim3(256), 0, c->stream, Kc, eb, eband, 1);
@@ -691,6 +774,25 @@ lqm_status lqm_context_stage_times(lqm_context* c, double* ms, int64_t* launches
     return LQM_OK;
 }
 
+lqm_status lqm_context_set_variant(lqm_context* c, int kind, int variant) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    if (kind < 0 || kind >= VK_N || variant < -1 || variant > 1) return fail(LQM_ERR_ARG, "bad variant");
+    c->forced[kind] = variant;
+    return LQM_OK;
+}
+
+lqm_status lqm_context_get_variant(lqm_context* c, int kind, int N, int* variant) {
+    if (!c || !variant) return fail(LQM_ERR_ARG, "null argument");
+    if (kind < 0 || kind >= VK_N) return fail(LQM_ERR_ARG, "bad variant kind");
+    if (c->forced[kind] >= 0) {
+        *variant = c->forced[kind];
+        return LQM_OK;
+    }
+    auto it = c->tuned.find((kind << 20) | N);
+    *variant = it == c->tuned.end() ? -1 : it->second;
+    return LQM_OK;
+}
+
 const char* lqm_stage_name(int stage) { return (stage >= 0 && stage < ST_N) ? kStageNames[stage] : ""; }
 
 lqm_status lqm_context_synchronize(lqm_context* c) {
@@ -836,6 +938,56 @@ lqm_status lqm_chemicalpotential(lqm_model* m, int Nk, const double* ks, const d
     if ((st = ws(c, W_BANDS, sizeof(double) * m->N * Nk, &db))) return st;
     if ((st = run_bands(m, Nk, (const double*)dks, (double*)db, nullptr))) return st;
     return lqm_chemicalpotential_from_bands_dev(c, m->N, Nk, (double*)db, (double*)dkw, filling, T, mu);
+}
+
+lqm_status lqm_autotune(lqm_model* m, int Nk, const double* ks_dev) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0 || !ks_dev) return fail(LQM_ERR_ARG, "autotune needs a non-empty device k sample");
+    lqm_context* c = m->ctx;
+    const int N = m->N;
+    const int saved[VK_N] = {c->forced[0], c->forced[1]};
+    const int saved_timing = c->timing;
+    c->timing = 0;
+    double* rho;
+    if ((st = wsT(c, W_OUT, (size_t)2 * N * N + 2, &rho))) return st;
+    const int32_t R0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int nr = m->d <= 8 ? 1 : 0;
+    hipEvent_t a, b;
+    LQM_HIP(hipEventCreate(&a));
+    LQM_HIP(hipEventCreate(&b));
+    // one kind at a time, the other left at its current choice
+    for (int kind = 0; kind < VK_N; ++kind) {
+        float best = 1e30f;
+        int bestv = 0;
+        for (int v = 0; v <= 1; ++v) {
+            c->forced[kind] = v;
+            float t = 0.f;
+            for (int rep = 0; rep < 3 && !st; ++rep) {
+                LQM_HIP(hipEventRecord(a, c->stream));
+                st = run_rho(m, Nk, ks_dev, nullptr, 0.0, 0.01, nr, R0, rho, rho + 2 * N * N, nullptr);
+                LQM_HIP(hipEventRecord(b, c->stream));
+                LQM_HIP(hipEventSynchronize(b));
+                float e = 0.f;
+                LQM_HIP(hipEventElapsedTime(&e, a, b));
+                if (rep > 0) t += e;  // first run warms up
+            }
+            if (st) break;
+            if (t < best) {
+                best = t;
+                bestv = v;
+            }
+        }
+        if (st) break;
+        c->tuned[(kind << 20) | N] = bestv;
+        c->forced[kind] = saved[kind];
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    c->forced[0] = saved[0];
+    c->forced[1] = saved[1];
+    c->timing = saved_timing;
+    return st;
 }
 
 lqm_status lqm_densitymatrix_dev(lqm_model* m, int Nk, const double* ks_dev,

@@ -39,9 +39,16 @@ def test_square_triangular_analytic(pkg, gpu):
     assert np.max(np.abs(b[0] - tri)) < 1e-12
 
 
+@pytest.fixture(params=[0, 1], ids=["tri-block", "tri-wave"])
+def tri_variant(request, gpu):
+    gpu.set_variant(0, request.param)
+    yield request.param
+    gpu.set_variant(0, -1)
+
+
 @pytest.mark.parametrize("N,rmax,nk", [(1, 1, 9), (2, 1, 16), (3, 2, 16), (8, 1, 64), (17, 2, 25),
                                        (64, 2, 256), (65, 1, 16), (96, 1, 8), (128, 1, 8)])
-def test_random_model_vs_oracle(pkg, oracle, gpu, N, rmax, nk):
+def test_random_model_vs_oracle(pkg, oracle, gpu, tri_variant, N, rmax, nk):
     h = pkg.models.random_tb(N, rmax=rmax, seed=100 + N)
     ks = pkg.regulargrid(nk=nk)
     R, T = h.pack()
@@ -51,7 +58,7 @@ def test_random_model_vs_oracle(pkg, oracle, gpu, N, rmax, nk):
 
 
 @pytest.mark.parametrize("N", [4, 30, 64, 100])
-def test_eigenvectors(pkg, gpu, N):
+def test_eigenvectors(pkg, gpu, tri_variant, N):
     h = pkg.models.random_tb(N, rmax=1, seed=N)
     ks = pkg.regulargrid(nk=9)
     b, V = pkg.api.eigen(h, ks)

@@ -27,11 +27,38 @@ def _rho_vs_oracle(pkg, oracle, h, ks, mu, T, kweights=None, keys=None, tol=1e-9
     return rho, eb
 
 
+@pytest.fixture(params=[(0, 0), (1, 1), (0, 1), (1, 0)], ids=lambda v: f"tri{v[0]}-rho{v[1]}")
+def variants(request, gpu):
+    """Force each (tridiagonalization, rho_k) kernel variant pair, then restore."""
+    gpu.set_variant(0, request.param[0])
+    gpu.set_variant(1, request.param[1])
+    yield request.param
+    gpu.set_variant(0, -1)
+    gpu.set_variant(1, -1)
+
+
 @pytest.mark.parametrize("N,nk,mu,T", [(2, 36, 0.0, 0.01), (4, 64, 0.3, 0.0), (8, 49, -0.2, 0.05),
                                        (33, 25, 0.0, 0.02), (64, 64, 0.0, 0.02), (96, 9, 0.1, 0.02)])
-def test_rho_random_models(pkg, oracle, gpu, N, nk, mu, T):
+def test_rho_random_models(pkg, oracle, gpu, variants, N, nk, mu, T):
     h = pkg.models.random_tb(N, rmax=1 if N > 8 else 2, seed=N)
     _rho_vs_oracle(pkg, oracle, h, pkg.regulargrid(nk=nk), mu, T)
+
+
+def test_rho_degenerate_spinful_all_variants(pkg, oracle, gpu, variants):
+    h = pkg.models.addspin(pkg.models.random_tb(5, rmax=1, seed=3), "spinhalf")
+    _rho_vs_oracle(pkg, oracle, h, pkg.regulargrid(nk=36), 0.1, 0.0)
+
+
+def test_autotune_picks_valid_variants(pkg, oracle, gpu):
+    import torch
+
+    h = pkg.models.random_tb(16, rmax=1, seed=4)
+    ks = pkg.regulargrid(nk=256)
+    m = pkg.api.GPUModel(h, gpu)
+    ks_d = torch.from_numpy(np.ascontiguousarray(ks.T)).cuda()
+    chosen = pkg.api.autotune(m, ks_d.data_ptr(), ks.shape[1])
+    assert set(chosen.values()) <= {0, 1}
+    _rho_vs_oracle(pkg, oracle, h, ks, 0.0, 0.05)
 
 
 def test_rho_zeeman_graphene_T0(pkg, oracle, gpu):
