@@ -41,7 +41,9 @@ typedef enum {
 } lqm_status;
 
 typedef enum {
-    LQM_PREC_FP64 = 0 /* FP64 complex everywhere: the plain path */
+    LQM_PREC_FP64 = 0,         /* FP64 complex everywhere: the plain path */
+    LQM_PREC_FP32_REFINED = 1  /* FP32 H(k) + eigensolve, FP64 Ogita-Aishima refinement of
+                                  the eigenpairs, FP64 occupations, rho_k and contraction */
 } lqm_precision;
 
 typedef struct lqm_context lqm_context;
@@ -80,6 +82,17 @@ const char *lqm_stage_name(int stage);
  * the model and records the fastest per orbital count N in the context;
  * lqm_context_get_variant reports the choice in force (-1: default, not tuned). */
 lqm_status lqm_context_set_variant(lqm_context *ctx, int kind, int variant);
+
+/* Mixed precision (LQM_PREC_FP32_REFINED; no reference counterpart).  Eigenpairs
+ * from the FP32 eigensolver are refined in FP64 by Ogita-Aishima sweeps
+ * (R = I - X^H X, S = X^H A X, lambda_i = s_ii/(1-r_ii), X <- X + X E); a further
+ * sweep runs only while some k-point has a near-degenerate pair the last sweep
+ * could not separate, up to max_sweeps (default 3).  The report of the last call:
+ * out[0] sweeps used, out[1] max_k ||I - X^H X||_F, out[2] max_k delta (cluster
+ * threshold), out[3] k-points still holding near-degenerate pairs.  Eigenvector
+ * output (bandmatrix with evecs) always uses the FP64 path. */
+lqm_status lqm_context_set_refinement(lqm_context *ctx, int max_sweeps);
+lqm_status lqm_context_refinement_report(lqm_context *ctx, double *out);
 lqm_status lqm_context_get_variant(lqm_context *ctx, int kind, int N, int *variant);
 lqm_status lqm_autotune(lqm_model *m, int Nk, const double *ks_dev);
 

@@ -19,6 +19,7 @@ HEADER = os.path.join(os.path.dirname(PKG_DIR), "include", "lqm_gpu.h")
 
 LQM_OK, LQM_ERR_ARG, LQM_ERR_DEVICE, LQM_ERR_UNSUPPORTED, LQM_ERR_NOCONV, LQM_ERR_NONFINITE = range(6)
 LQM_PREC_FP64 = 0
+LQM_PREC_FP32_REFINED = 1
 
 _lib = None
 
@@ -39,6 +40,8 @@ _PROTOS = {
     "lqm_context_stage_times": [P, P, P, I],
     "lqm_stage_name": [I],
     "lqm_context_set_variant": [P, I, I],
+    "lqm_context_set_refinement": [P, I],
+    "lqm_context_refinement_report": [P, P],
     "lqm_context_get_variant": [P, I, I, P],
     "lqm_autotune": [P, I, P],
     "lqm_model_create": [P, I, I, I, P, P, PP],

@@ -61,6 +61,17 @@ class GPUExec:
         _lib.call("lqm_context_get_variant", self.handle, int(kind), int(N), _c.byref(v))
         return v.value
 
+    def set_refinement(self, max_sweeps: int) -> None:
+        """Largest number of Ogita-Aishima sweeps of the mixed-precision path."""
+        _lib.call("lqm_context_set_refinement", self.handle, int(max_sweeps))
+
+    def refinement_report(self) -> dict:
+        """Diagnostics of the last mixed-precision call."""
+        out = (_c.c_double * 4)()
+        _lib.call("lqm_context_refinement_report", self.handle, out)
+        return {"sweeps": int(out[0]), "max_orth_residual": out[1], "max_cluster_delta": out[2],
+                "kpoints_near_degenerate": int(out[3])}
+
     def set_stream(self, stream_ptr: int | None) -> None:
         _lib.call("lqm_context_set_stream", self.handle, _c.c_void_p(stream_ptr or 0))
 
@@ -79,16 +90,25 @@ class GPUExec:
 _executors: dict = {}
 
 
-def to_executor(multimode="gpu", device: int = 0) -> GPUExec:
-    """Resolve ``multimode`` (``:gpu``/``"gpu"`` or a GPUExec) to the shared executor."""
+_PRECISIONS = {"fp64": _lib.LQM_PREC_FP64, "mixed": _lib.LQM_PREC_FP32_REFINED}
+
+
+def to_executor(multimode="gpu", device: int = 0, precision: str = "fp64") -> GPUExec:
+    """Resolve ``multimode`` (``:gpu``/``"gpu"`` or a GPUExec) to a shared executor.
+
+    precision "fp64" is the plain path; "mixed" runs the eigensolve in FP32 and
+    refines the eigenpairs to FP64 accuracy (Ogita-Aishima)."""
     if isinstance(multimode, GPUExec):
         return multimode
     mode = str(multimode).lstrip(":")
     if mode != "gpu":
         raise ValueError(f"this package provides multimode=:gpu only, got {multimode!r}")
-    ex = _executors.get(device)
+    if precision not in _PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
+    key = (device, precision)
+    ex = _executors.get(key)
     if ex is None:
-        ex = _executors[device] = GPUExec(device)
+        ex = _executors[key] = GPUExec(device, _PRECISIONS[precision])
     return ex
 
 

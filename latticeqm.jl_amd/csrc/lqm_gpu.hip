@@ -17,6 +17,7 @@
 #include "aux_kernels.hpp"
 #include "eig_kernels.hpp"
 #include "wave_kernels.hpp"
+#include "refine_kernels.hpp"
 
 using lqm::cx;
 
@@ -38,7 +39,8 @@ lqm_status fail(lqm_status st, const std::string& msg) {
 
 enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
-    W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_NSLOTS
+    W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
+    W_S, W_G, W_LAM, W_STAT, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -48,10 +50,11 @@ constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch
 // Kernel stages timed with HIP events when timing is on.
 enum Stage {
     ST_TRIDIAG, ST_QL, ST_REPLAY, ST_BACKTRANSFORM, ST_ZFZ, ST_RHO2, ST_CONTRACT, ST_FOLD, ST_DOS,
-    ST_REDUCE, ST_N
+    ST_REDUCE, ST_HK64, ST_GEMM, ST_OA, ST_N
 };
 const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "backtransform", "zfz",
-                                       "rho_twosided", "phase_contract", "phase_fold", "dos", "reduce"};
+                                       "rho_twosided", "phase_contract", "phase_fold", "dos", "reduce",
+                                       "herm_hk_fp64", "zgemm_fp64", "oa_correct"};
 
 struct TimedLaunch {
     int stage;
@@ -76,6 +79,8 @@ struct lqm_context {
     void* buf[W_NSLOTS] = {};
     size_t cap[W_NSLOTS] = {};
     int* d_status = nullptr;
+    int refine_max = 3;      // Ogita-Aishima sweeps allowed per call (mixed precision)
+    double report[4] = {};   // last mixed call: sweeps, max ||R||_F, max delta, near-cluster k
 };
 
 struct lqm_model {
@@ -131,17 +136,17 @@ struct Plan {
     int Kc = 0;   // k-points per chunk
 };
 
-Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool rho) {
+Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool rho,
+               size_t cR = sizeof(double)) {
     Plan p;
     p.N = N;
     p.nR = nR;
-    const size_t cR = sizeof(double);
-    p.shA = sizeof(cx<double>) * ((size_t)N * (N + 1) + 2 * N + 6) + sizeof(cx<double>) * nR;
+    const size_t cC = 2 * cR;  // complex element of the eigensolver's working precision
+    p.shA = cC * ((size_t)N * (N + 1) + 2 * N + 6) + sizeof(cx<double>) * nR;
     p.ldsA = p.shA <= (size_t)c->lds_max;
-    if (!p.ldsA) p.shA = sizeof(cx<double>) * (2 * N + 6) + sizeof(cx<double>) * nR;
+    if (!p.ldsA) p.shA = cC * (2 * N + 6) + sizeof(cx<double>) * nR;
     p.rch = std::max(lqm::kRotChunk, N);
-    const size_t smallC = cR * 2 * p.rch + sizeof(cx<double>) * 4 * N + 8 * N + 32 +
-                          sizeof(int) * lqm::kHdrChunk + 16;
+    const size_t smallC = cR * 2 * p.rch + cC * 4 * N + 8 * N + 32 + sizeof(int) * lqm::kHdrChunk + 16;
     p.shC = 2 * cR * (size_t)N * (N + 1) + smallC;
     p.ldsC = p.shC <= (size_t)c->lds_max;
     if (!p.ldsC) p.shC = smallC;
@@ -152,15 +157,15 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     p.cap_rot = 3 * N * N + 64 * N;
     p.cap_sw = 30 * N + 8;
     const size_t NN = (size_t)N * N;
-    size_t perk = 16 * NN + 16 * N + 16 * N + 8 * N + 4 * N + 8;
-    if (vectors || rho) perk += 16 * (size_t)p.cap_rot + 4 * (size_t)p.cap_sw + 4;
-    if (!p.ldsA) perk += 16 * NN;
-    if ((vectors || rho) && !p.ldsC) perk += 16 * NN;
+    size_t perk = cC * NN + cC * N + 2 * cR * N + cR * N + 4 * N + 8;
+    if (vectors || rho) perk += 2 * cR * (size_t)p.cap_rot + 4 * (size_t)p.cap_sw + 4;
+    if (!p.ldsA) perk += cC * NN;
+    if ((vectors || rho) && !p.ldsC) perk += cC * NN;
     if (rho) perk += 16 * NN;
     p.nm = N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : N <= 64 ? 64 : 0;
-    if ((vectors || rho) && p.nm) perk += 8 * NN;
+    if ((vectors || rho) && p.nm) perk += cR * NN;
     const size_t np = (size_t)N * (N + 1) / 2;
-    p.shAw = sizeof(cx<double>) * (np + 2 * N + nR);
+    p.shAw = cC * (np + 2 * N) + sizeof(cx<double>) * nR;
     auto variant = [&](int kind) {
         if (c->forced[kind] >= 0) return c->forced[kind];
         auto it = c->tuned.find((kind << 20) | N);
@@ -207,15 +212,17 @@ void allow_lds(K kernel, size_t bytes) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-// Eigenvalues (and optionally the QL log) of one chunk: K_A then K_B.
-lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks, double* evals,
+// Eigenvalues (and optionally the QL log) of one chunk: K_A then K_B, in the
+// working precision R (double: the plain path; float: the bulk of the mixed path).
+template <typename R>
+lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks, R* evals,
                           bool logq) {
     lqm_context* c = m->ctx;
     const int N = m->N;
-    double *dd, *ee;
-    cx<double>*refl, *tau, *gA = nullptr;
+    R *dd, *ee;
+    cx<R>*refl, *tau, *gA = nullptr;
     int* perm = nullptr;
-    double* rot = nullptr;
+    R* rot = nullptr;
     int *sw = nullptr, *nsw = nullptr;
     lqm_status st;
     if ((st = wsT(c, W_D, (size_t)N * Kc, &dd))) return st;
@@ -230,72 +237,72 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
         if ((st = wsT(c, W_NSW, (size_t)Kc, &nsw))) return st;
     }
     {
-    StageScope sc(c, ST_TRIDIAG);
-    if (P.nqA == 1) {
-        auto kA = lqm::k_hk_tridiag_w<double, 1>;
-        allow_lds(kA, P.shAw);
-        hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kWave), P.shAw, c->stream, N, m->d, m->nR, m->dR,
-                           m->dT, ks, Kc, dd, ee, refl, tau);
-    } else if (P.nqA == 2) {
-        auto kA = lqm::k_hk_tridiag_w<double, 2>;
-        allow_lds(kA, P.shAw);
-        hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kWave), P.shAw, c->stream, N, m->d, m->nR, m->dR,
-                           m->dT, ks, Kc, dd, ee, refl, tau);
-    } else if (P.ldsA) {
-        auto kA = lqm::k_hk_tridiag<double, true>;
-        allow_lds(kA, P.shA);
-        hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kTPB), P.shA, c->stream, N, m->d, m->nR, m->dR,
-                           m->dT, ks, Kc, gA, dd, ee, refl, tau);
-    } else {
-        auto kA = lqm::k_hk_tridiag<double, false>;
-        allow_lds(kA, P.shA);
-        hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kTPB), P.shA, c->stream, N, m->d, m->nR, m->dR,
-                           m->dT, ks, Kc, gA, dd, ee, refl, tau);
-    }
+        StageScope sc(c, ST_TRIDIAG);
+        if (P.nqA == 1) {
+            auto kA = lqm::k_hk_tridiag_w<R, 1>;
+            allow_lds(kA, P.shAw);
+            hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kWave), P.shAw, c->stream, N, m->d, m->nR, m->dR,
+                               m->dT, ks, Kc, dd, ee, refl, tau);
+        } else if (P.nqA == 2) {
+            auto kA = lqm::k_hk_tridiag_w<R, 2>;
+            allow_lds(kA, P.shAw);
+            hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kWave), P.shAw, c->stream, N, m->d, m->nR, m->dR,
+                               m->dT, ks, Kc, dd, ee, refl, tau);
+        } else if (P.ldsA) {
+            auto kA = lqm::k_hk_tridiag<R, true>;
+            allow_lds(kA, P.shA);
+            hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kTPB), P.shA, c->stream, N, m->d, m->nR, m->dR,
+                               m->dT, ks, Kc, gA, dd, ee, refl, tau);
+        } else {
+            auto kA = lqm::k_hk_tridiag<R, false>;
+            allow_lds(kA, P.shA);
+            hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kTPB), P.shA, c->stream, N, m->d, m->nR, m->dR,
+                               m->dT, ks, Kc, gA, dd, ee, refl, tau);
+        }
     }
     LQM_HIP(hipGetLastError());
     const int L = P.laneB;
     const int gB = (Kc + L - 1) / L;
     {
-    StageScope sc(c, ST_QL);
-    if (logq) {
-        auto kB = lqm::k_tridiag_ql<double, true>;
-        allow_lds(kB, P.shB);
-        hipLaunchKernelGGL(kB, dim3(gB), dim3(L), P.shB, c->stream, N, Kc, dd, ee, evals, perm, rot,
-                           sw, nsw, P.cap_rot, P.cap_sw, c->d_status);
-    } else {
-        auto kB = lqm::k_tridiag_ql<double, false>;
-        allow_lds(kB, P.shB);
-        hipLaunchKernelGGL(kB, dim3(gB), dim3(L), P.shB, c->stream, N, Kc, dd, ee, evals, perm, rot,
-                           sw, nsw, P.cap_rot, P.cap_sw, c->d_status);
-    }
+        StageScope sc(c, ST_QL);
+        if (logq) {
+            auto kB = lqm::k_tridiag_ql<R, true>;
+            allow_lds(kB, P.shB);
+            hipLaunchKernelGGL(kB, dim3(gB), dim3(L), P.shB, c->stream, N, Kc, dd, ee, evals, perm, rot,
+                               sw, nsw, P.cap_rot, P.cap_sw, c->d_status);
+        } else {
+            auto kB = lqm::k_tridiag_ql<R, false>;
+            allow_lds(kB, P.shB);
+            hipLaunchKernelGGL(kB, dim3(gB), dim3(L), P.shB, c->stream, N, Kc, dd, ee, evals, perm, rot,
+                               sw, nsw, P.cap_rot, P.cap_sw, c->d_status);
+        }
     }
     LQM_HIP(hipGetLastError());
     return LQM_OK;
 }
 
-template <int NM>
-void launch_replay(lqm_context* c, int N, int Kc, const double* rot, const int* sw, const int* nsw,
-                   int cap_rot, int cap_sw, double* Z) {
+template <typename R, int NM>
+void launch_replay(lqm_context* c, int N, int Kc, const R* rot, const int* sw, const int* nsw,
+                   int cap_rot, int cap_sw, R* Z) {
     StageScope sc(c, ST_REPLAY);
-    auto kR = lqm::k_replay<double, NM>;
+    auto kR = lqm::k_replay<R, NM>;
     hipLaunchKernelGGL(kR, dim3(Kc), dim3(lqm::kWave), 0, c->stream, N, rot, sw, nsw, cap_rot, cap_sw, Z);
 }
 
-template <int MODE, bool ZIN>
-void launch_kc(lqm_context* c, const Plan& P, int N, int Kc, const cx<double>* refl,
-               const cx<double>* tau, const double* rot, const int* sw, const int* nsw,
-               const double* evals, const int* perm, const double* kw, double wuni, double mu,
-               double T, double* gV, const double* zin, cx<double>* out, double* eb) {
+template <typename R, int MODE, bool ZIN>
+void launch_kc(lqm_context* c, const Plan& P, int N, int Kc, const cx<R>* refl, const cx<R>* tau,
+               const R* rot, const int* sw, const int* nsw, const R* evals, const int* perm,
+               const double* kw, double wuni, double mu, double T, R* gV, const R* zin,
+               cx<double>* out, double* eb) {
     StageScope sc(c, ST_BACKTRANSFORM);
     if (P.ldsC) {
-        auto kC = lqm::k_backtransform<double, true, MODE, ZIN>;
+        auto kC = lqm::k_backtransform<R, true, MODE, ZIN>;
         allow_lds(kC, P.shC);
         hipLaunchKernelGGL(kC, dim3(Kc), dim3(lqm::kTPB), P.shC, c->stream, N, Kc, refl, tau, rot, sw,
                            nsw, P.cap_rot, P.cap_sw, P.rch, evals, perm, kw, wuni, mu, T, gV, zin, out,
                            eb);
     } else {
-        auto kC = lqm::k_backtransform<double, false, MODE, ZIN>;
+        auto kC = lqm::k_backtransform<R, false, MODE, ZIN>;
         allow_lds(kC, P.shC);
         hipLaunchKernelGGL(kC, dim3(Kc), dim3(lqm::kTPB), P.shC, c->stream, N, Kc, refl, tau, rot, sw,
                            nsw, P.cap_rot, P.cap_sw, P.rch, evals, perm, kw, wuni, mu, T, gV, zin, out,
@@ -303,39 +310,39 @@ void launch_kc(lqm_context* c, const Plan& P, int N, int Kc, const cx<double>* r
     }
 }
 
-template <int MODE>
-lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const double* evals,
+template <typename R, int MODE>
+lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const R* evals,
                                 const double* kw, double wuni, double mu, double T,
                                 cx<double>* out, double* eb) {
     lqm_context* c = m->ctx;
     const int N = m->N;
-    double* gV = nullptr;
+    R* gV = nullptr;
     if (!P.ldsC) {
         lqm_status st = wsT(c, W_GV, (size_t)2 * N * N * Kc, &gV);
         if (st) return st;
     }
-    auto* refl = static_cast<cx<double>*>(c->buf[W_REFL]);
-    auto* tau = static_cast<cx<double>*>(c->buf[W_TAU]);
-    auto* rot = static_cast<double*>(c->buf[W_ROT]);
+    auto* refl = static_cast<cx<R>*>(c->buf[W_REFL]);
+    auto* tau = static_cast<cx<R>*>(c->buf[W_TAU]);
+    auto* rot = static_cast<R*>(c->buf[W_ROT]);
     auto* sw = static_cast<int*>(c->buf[W_SW]);
     auto* nsw = static_cast<int*>(c->buf[W_NSW]);
     auto* perm = static_cast<int*>(c->buf[W_PERM]);
     if (P.nm) {
-        double* Z;
+        R* Z;
         lqm_status st = wsT(c, W_Z, (size_t)N * N * Kc, &Z);
         if (st) return st;
         switch (P.nm) {
-            case 8: launch_replay<8>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
-            case 16: launch_replay<16>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
-            case 32: launch_replay<32>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
-            default: launch_replay<64>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+            case 8: launch_replay<R, 8>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+            case 16: launch_replay<R, 16>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+            case 32: launch_replay<R, 32>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+            default: launch_replay<R, 64>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
         }
         LQM_HIP(hipGetLastError());
-        launch_kc<MODE, true>(c, P, N, Kc, refl, tau, rot, sw, nsw, evals, perm, kw, wuni, mu, T, gV, Z,
-                              out, eb);
+        launch_kc<R, MODE, true>(c, P, N, Kc, refl, tau, rot, sw, nsw, evals, perm, kw, wuni, mu, T, gV, Z,
+                                 out, eb);
     } else {
-        launch_kc<MODE, false>(c, P, N, Kc, refl, tau, rot, sw, nsw, evals, perm, kw, wuni, mu, T, gV,
-                               nullptr, out, eb);
+        launch_kc<R, MODE, false>(c, P, N, Kc, refl, tau, rot, sw, nsw, evals, perm, kw, wuni, mu, T, gV,
+                                  nullptr, out, eb);
     }
     LQM_HIP(hipGetLastError());
     return LQM_OK;
@@ -357,10 +364,10 @@ lqm_status launch_rho2(lqm_model* m, const Plan& P, int Kc, const double* evals,
     if ((st = wsT(c, W_Z, (size_t)N * N * Kc, &Z))) return st;
     if ((st = wsT(c, W_M, (size_t)N * N * Kc, &M))) return st;
     switch (P.nm) {
-        case 8: launch_replay<8>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
-        case 16: launch_replay<16>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
-        case 32: launch_replay<32>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
-        default: launch_replay<64>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+        case 8: launch_replay<double, 8>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+        case 16: launch_replay<double, 16>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+        case 32: launch_replay<double, 32>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+        default: launch_replay<double, 64>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
     }
     LQM_HIP(hipGetLastError());
     {
@@ -420,11 +427,17 @@ lqm_status check_model(lqm_model* m) {
 }
 
 // bands (+ evecs) over the whole grid, device pointers.
+lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
+                     int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands);
+
 lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, double* evecs) {
     const int N = m->N;
     if (Nk < 0) return fail(LQM_ERR_ARG, "Nk < 0");
     if (Nk == 0) return LQM_OK;
     if (!ks || !bands) return fail(LQM_ERR_ARG, "null ks or bands");
+    // eigenvector output always comes from the FP64 path
+    if (m->ctx->precision == LQM_PREC_FP32_REFINED && !evecs)
+        return run_mixed(m, Nk, ks, nullptr, 0.0, 0.0, 0, nullptr, nullptr, nullptr, bands);
     Plan P = make_plan(m->ctx, N, m->nR, Nk, evecs != nullptr, false);
     for (int k0 = 0; k0 < Nk; k0 += P.Kc) {
         const int Kc = std::min(P.Kc, Nk - k0);
@@ -433,7 +446,7 @@ lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, doub
         lqm_status st = launch_eigvals(m, P, Kc, ksc, ev, evecs != nullptr);
         if (st) return st;
         if (evecs) {
-            st = launch_backtransform<0>(m, P, Kc, ev, nullptr, 0.0, 0.0, 0.0,
+            st = launch_backtransform<double, 0>(m, P, Kc, ev, nullptr, 0.0, 0.0, 0.0,
                                          reinterpret_cast<cx<double>*>(evecs) + (size_t)k0 * N * N,
                                          nullptr);
             if (st) return st;
@@ -457,6 +470,9 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
     if ((st = wsT(c, W_RR, (size_t)std::max(1, nRr * d), &dRr))) return st;
     if (nRr > 0)
         LQM_HIP(hipMemcpyAsync(dRr, Rr, sizeof(int) * nRr * d, hipMemcpyHostToDevice, c->stream));
+    if (c->precision == LQM_PREC_FP32_REFINED)
+        return run_mixed(m, Nk, ks, kw, mu, T, nRr, dRr, reinterpret_cast<cx<double>*>(rho), eband,
+                         bands_out);
     Plan P = make_plan(c, N, m->nR, Nk, true, true);
     const int RG = nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : 32;
     const int groups = std::max(1, (nRr + RG - 1) / RG);
@@ -482,7 +498,7 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
         if ((st = wsT(c, W_EB, (size_t)P.Kc, &eb))) return st;
         if (P.rho2) {
             if ((st = launch_rho2(m, P, Kc, ev, kwc, wuni, mu, T, rhok, eb))) return st;
-        } else if ((st = launch_backtransform<1>(m, P, Kc, ev, kwc, wuni, mu, T, rhok, eb))) {
+        } else if ((st = launch_backtransform<double, 1>(m, P, Kc, ev, kwc, wuni, mu, T, rhok, eb))) {
             return st;
         }
         {
@@ -505,6 +521,149 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
         }
     }
     return check_status(c, "densitymatrix");
+}
+
+// ---- mixed precision: FP32 eigensolve + FP64 Ogita-Aishima refinement ---------------
+template <int OPA, int OPB>
+void zgemm(lqm_context* c, int N, int Kc, const cx<double>* A, const cx<double>* B, cx<double>* C,
+           cx<double> alpha, cx<double> beta) {
+    StageScope sc(c, ST_GEMM);
+    const int t = (N + lqm::kGT - 1) / lqm::kGT;
+    auto kG = lqm::k_zgemm_batched<OPA, OPB>;
+    hipLaunchKernelGGL(kG, dim3(t, t, Kc), dim3(256), 0, c->stream, N, A, B, C, alpha, beta);
+}
+
+// One pass over the grid.  Outputs (any may be null): sorted refined eigenvalues
+// (bands), rho on keys dRr (rho: accumulated into, zeroed by the caller) and eband.
+lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
+                     int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands) {
+    lqm_context* c = m->ctx;
+    const int N = m->N, d = m->d;
+    if (N > 512) return fail(LQM_ERR_UNSUPPORTED, "mixed precision path supports N <= 512");
+    Plan P = make_plan(c, N, m->nR, Nk, true, false, sizeof(float));
+    const size_t NN = (size_t)N * N;
+    // refinement buffers: X, A, W, S, G (+ rho_k) in FP64 complex per k-point
+    const size_t per64 = 16 * NN * (rho ? 6 : 5) + 48 * N;
+    P.Kc = (int)std::max<size_t>(1, std::min<size_t>((size_t)P.Kc, kWorkspaceBudget / per64));
+    const double wuni = 1.0 / (double)Nk;
+    const cx<double> one = {1.0, 0.0}, zero = {0.0, 0.0};
+    lqm_status st;
+    int sweeps_max = 0;
+    double rmax = 0.0, dmax = 0.0, nnear = 0.0;
+    const int RG = nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : 32;
+    const int groups = std::max(1, (nRr + RG - 1) / RG);
+    const int tilesE = (int)((NN + 255) / 256);
+    const int S = std::max(1, std::min(P.Kc, 1024 / std::max(1, tilesE * groups)));
+    cx<double>* part = nullptr;
+    if (rho && nRr > 0 && (st = wsT(c, W_PART, (size_t)groups * S * RG * NN, &part))) return st;
+    std::vector<double> hstat;
+    for (int k0 = 0; k0 < Nk; k0 += P.Kc) {
+        const int Kc = std::min(P.Kc, Nk - k0);
+        const double* ksc = ks + (size_t)k0 * d;
+        const double* kwc = kw ? kw + k0 : nullptr;
+        float* ev32;
+        cx<double> *X, *A, *W, *Sm, *G;
+        double *lam, *stat;
+        if ((st = wsT(c, W_EV32, (size_t)N * P.Kc, &ev32))) return st;
+        if ((st = wsT(c, W_X, NN * P.Kc, &X))) return st;
+        if ((st = wsT(c, W_HA, NN * P.Kc, &A))) return st;
+        if ((st = wsT(c, W_W, NN * P.Kc, &W))) return st;
+        if ((st = wsT(c, W_S, NN * P.Kc, &Sm))) return st;
+        if ((st = wsT(c, W_G, NN * P.Kc, &G))) return st;
+        if ((st = wsT(c, W_LAM, (size_t)N * P.Kc, &lam))) return st;
+        if ((st = wsT(c, W_STAT, (size_t)4 * P.Kc, &stat))) return st;
+        // FP32 eigenvectors, ascending, written as FP64 complex
+        if ((st = launch_eigvals<float>(m, P, Kc, ksc, ev32, true))) return st;
+        if ((st = launch_backtransform<float, 0>(m, P, Kc, ev32, nullptr, 0.0, 0.0, 0.0, X, nullptr)))
+            return st;
+        {
+            StageScope sc(c, ST_HK64);
+            const size_t sh = sizeof(cx<double>) * m->nR;
+            allow_lds(lqm::k_herm_hk, sh);
+            hipLaunchKernelGGL(lqm::k_herm_hk, dim3(Kc), dim3(256), sh, c->stream, N, d, m->nR, m->dR,
+                               m->dT, ksc, A);
+        }
+        LQM_HIP(hipGetLastError());
+        int sweeps = 0;
+        for (;;) {
+            zgemm<0, 0>(c, N, Kc, A, X, W, one, zero);   // W = A X
+            zgemm<1, 0>(c, N, Kc, X, W, Sm, one, zero);  // S = X^H A X
+            zgemm<1, 0>(c, N, Kc, X, X, G, one, zero);   // G = X^H X
+            {
+                StageScope sc(c, ST_OA);
+                const size_t sh = lqm::oa_lds_bytes(N);
+                allow_lds(lqm::k_oa_correct, sh);
+                hipLaunchKernelGGL(lqm::k_oa_correct, dim3(Kc), dim3(256), sh, c->stream, N, Sm, G, W, lam,
+                                   stat);
+            }
+            LQM_HIP(hipGetLastError());
+            // X <- X + X E  (E in W; G is free after the correction)
+            LQM_HIP(hipMemcpyAsync(G, X, sizeof(cx<double>) * NN * Kc, hipMemcpyDeviceToDevice, c->stream));
+            zgemm<0, 0>(c, N, Kc, X, W, G, one, one);
+            std::swap(X, G);
+            ++sweeps;
+            hstat.resize((size_t)4 * Kc);
+            LQM_HIP(hipMemcpyAsync(hstat.data(), stat, sizeof(double) * 4 * Kc, hipMemcpyDeviceToHost, c->stream));
+            LQM_HIP(hipStreamSynchronize(c->stream));
+            int rotated = 0, unresolved = 0;
+            double offmax = 0.0;
+            for (int k = 0; k < Kc; ++k) {
+                rmax = std::max(rmax, hstat[4 * k]);
+                dmax = std::max(dmax, hstat[4 * k + 1]);
+                const int fl = (int)hstat[4 * k + 2];
+                rotated += (fl & 1) != 0;
+                unresolved += (fl & 2) != 0;
+                offmax = std::max(offmax, hstat[4 * k + 3]);
+                if (!std::isfinite(hstat[4 * k]) || !std::isfinite(hstat[4 * k + 3]))
+                    return fail(LQM_ERR_NONFINITE, "mixed precision: non-finite refinement residual");
+            }
+            // Two sweeps take FP32 eigenvectors to FP64 accuracy; more run while a
+            // cluster was rotated from a still inaccurate subspace.
+            const bool done = sweeps >= 2 && rotated == 0 && unresolved == 0;
+            if (done || sweeps >= std::max(2, c->refine_max)) {
+                nnear += unresolved;
+                if (unresolved)
+                    return fail(LQM_ERR_NOCONV, "mixed precision: a near-degenerate cluster wider than 8 "
+                                                "eigenvalues could not be resolved; use the FP64 path");
+                break;
+            }
+        }
+        sweeps_max = std::max(sweeps_max, sweeps);
+        if (bands) {
+            StageScope sc(c, ST_REDUCE);
+            hipLaunchKernelGGL(lqm::k_sort_evals, dim3(Kc), dim3(256), 0, c->stream, N, lam,
+                               bands + (size_t)k0 * N);
+        }
+        if (rho || eband) {
+            double* eb;
+            if ((st = wsT(c, W_EB, (size_t)P.Kc, &eb))) return st;
+            {
+                StageScope sc(c, ST_REDUCE);
+                hipLaunchKernelGGL(lqm::k_scale_occ, dim3(Kc), dim3(256), 0, c->stream, N, X, lam, kwc, wuni,
+                                   mu, T, G, eb);
+                hipLaunchKernelGGL(lqm::k_sum, dim3(1), dim3(256), 0, c->stream, Kc, eb, eband, 1);
+            }
+            zgemm<0, 1>(c, N, Kc, G, X, Sm, one, zero);  // rho_k = X f X^H
+            if (rho && nRr > 0) {
+                const int Sc = std::min(S, Kc);
+                const int ksplit = (Kc + Sc - 1) / Sc;
+                const int Sused = (Kc + ksplit - 1) / ksplit;
+                switch (RG) {
+                    case 1: launch_contract<1>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho); break;
+                    case 4: launch_contract<4>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho); break;
+                    case 8: launch_contract<8>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho); break;
+                    case 16: launch_contract<16>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho); break;
+                    default: launch_contract<32>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho); break;
+                }
+            }
+            LQM_HIP(hipGetLastError());
+        }
+    }
+    c->report[0] = sweeps_max;
+    c->report[1] = rmax;
+    c->report[2] = dmax;
+    c->report[3] = nnear;
+    return check_status(c, "mixed-precision pass");
 }
 
 lqm_status dos_bands_dev(lqm_context* c, int N, int Nk, const double* bands, const double* kw,
@@ -704,7 +863,8 @@ int lqm_abi_version(void) { return 1; }
 
 lqm_status lqm_context_create(int device, int precision, lqm_context** out) {
     if (!out) return fail(LQM_ERR_ARG, "null output pointer");
-    if (precision != LQM_PREC_FP64) return fail(LQM_ERR_UNSUPPORTED, "precision mode not built");
+    if (precision != LQM_PREC_FP64 && precision != LQM_PREC_FP32_REFINED)
+        return fail(LQM_ERR_UNSUPPORTED, "unknown precision mode");
     int n = 0;
     LQM_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(LQM_ERR_ARG, "device ordinal out of range");
@@ -771,6 +931,19 @@ lqm_status lqm_context_stage_times(lqm_context* c, double* ms, int64_t* launches
         c->stage_ms[i] = 0;
         c->stage_n[i] = 0;
     }
+    return LQM_OK;
+}
+
+lqm_status lqm_context_set_refinement(lqm_context* c, int max_sweeps) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    if (max_sweeps < 1 || max_sweeps > 8) return fail(LQM_ERR_ARG, "max_sweeps must be in 1..8");
+    c->refine_max = max_sweeps;
+    return LQM_OK;
+}
+
+lqm_status lqm_context_refinement_report(lqm_context* c, double* out) {
+    if (!c || !out) return fail(LQM_ERR_ARG, "null argument");
+    for (int i = 0; i < 4; ++i) out[i] = c->report[i];
     return LQM_OK;
 }
 

@@ -1,0 +1,416 @@
+// FP64 refinement of FP32 eigenpairs (Ogita & Aishima, "Iterative refinement for
+// symmetric eigenvalue decomposition", JJIAM 35 (2018), Algorithm 1, in its Hermitian
+// form) and the FP64 pieces around it:
+//
+//   k_herm_hk        A = Hermitian(H(k)) in FP64 from the upper triangle
+//   k_zgemm_batched  C = alpha op(A) op(B) + beta C per k-point (N <= 128 tiles)
+//   k_oa_correct     lambda_i = s_ii / g_ii, delta = 2(||S - D|| + ||A|| ||R||),
+//                    E_ij = (s_ij + lambda_j r_ij)/(lambda_j - lambda_i) or r_ij / 2
+//                    with R = I - G, G = X^H X, S = X^H A X
+//   k_scale_occ      Y = X diag(f(lambda)), f = fermidirac(lambda - mu; T) in FP64,
+//                    and eb[k] = w_k sum_n lambda_n f_n
+//   k_sort_evals     ascending copy of the refined eigenvalues
+#pragma once
+#include "lqm_common.hpp"
+
+namespace lqm {
+
+// Hermitian(H(k)) with H(k) = sum_R T_R e^{2 pi i k.R}; one block per k-point.
+__global__ __launch_bounds__(256) void k_herm_hk(int N, int d, int nR, const int* __restrict__ Rk,
+                                                 const cx<double>* __restrict__ T,
+                                                 const double* __restrict__ ks,
+                                                 cx<double>* __restrict__ A) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<double>* sph = reinterpret_cast<cx<double>*>(smem);
+    const int kk = blockIdx.x;
+    const double* k = ks + (size_t)kk * d;
+    for (int r = threadIdx.x; r < nR; r += 256) {
+        double x = 0.0;
+        for (int a = 0; a < d; ++a) x += k[a] * (double)Rk[r * d + a];
+        sph[r] = cis2pi(x);
+    }
+    __syncthreads();
+    const size_t NN = (size_t)N * N;
+    cx<double>* Ak = A + (size_t)kk * NN;
+    for (int e = threadIdx.x; e < N * N; e += 256) {
+        const int i = e % N, j = e / N;
+        if (i > j) continue;
+        cx<double> h = {0.0, 0.0};
+        for (int r = 0; r < nR; ++r) h = cmad(h, T[r * NN + (size_t)j * N + i], sph[r]);
+        if (i == j) {
+            Ak[e] = {h.re, 0.0};
+        } else {
+            Ak[e] = h;
+            Ak[(size_t)i * N + j] = {h.re, -h.im};
+        }
+    }
+}
+
+// Batched C = alpha op(A) op(B) + beta C, square N x N column-major FP64 complex,
+// batch stride N*N.  op 0 = as is, 1 = conjugate transpose.  64 x 64 output tile
+// per block, K-steps of 16 staged in LDS, 4 x 4 complex register block per thread
+// (rows ti + 16u, columns tj + 16v).  grid = (tiles, tiles, batch).
+constexpr int kGT = 64, kGK = 16;
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(256) void k_zgemm_batched(int N, const cx<double>* __restrict__ A,
+                                                       const cx<double>* __restrict__ B,
+                                                       cx<double>* __restrict__ C,
+                                                       cx<double> alpha, cx<double> beta) {
+    __shared__ cx<double> As[kGK][kGT + 1];  // As[k][i] = op(A)(i0 + i, k0 + k)
+    __shared__ cx<double> Bs[kGK][kGT + 1];  // Bs[k][j] = op(B)(k0 + k, j0 + j)
+    const int tid = threadIdx.x;
+    const int ti = tid & 15, tj = tid >> 4;
+    const int i0 = blockIdx.x * kGT, j0 = blockIdx.y * kGT;
+    const size_t NN = (size_t)N * N;
+    const cx<double>* Ak = A + (size_t)blockIdx.z * NN;
+    const cx<double>* Bk = B + (size_t)blockIdx.z * NN;
+    cx<double>* Ck = C + (size_t)blockIdx.z * NN;
+    cx<double> acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = {0.0, 0.0};
+    for (int k0 = 0; k0 < N; k0 += kGK) {
+        // 64 x 16 tiles: 1024 elements, 4 per thread
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            // A tile element (i, k): i fastest when op = N (coalesced column reads)
+            int i, kx;
+            if (OPA == 0) { i = e & 63; kx = e >> 6; } else { kx = e & 15; i = e >> 4; }
+            const int gi = i0 + i, gk = k0 + kx;
+            cx<double> a = {0.0, 0.0};
+            if (gi < N && gk < N) {
+                if (OPA == 0) a = Ak[(size_t)gk * N + gi];
+                else { a = Ak[(size_t)gi * N + gk]; a.im = -a.im; }
+            }
+            As[kx][i] = a;
+            int j, ky;
+            if (OPB == 0) { ky = e & 15; j = e >> 4; } else { j = e & 63; ky = e >> 6; }
+            const int gj = j0 + j, gk2 = k0 + ky;
+            cx<double> b = {0.0, 0.0};
+            if (gj < N && gk2 < N) {
+                if (OPB == 0) b = Bk[(size_t)gj * N + gk2];
+                else { b = Bk[(size_t)gk2 * N + gj]; b.im = -b.im; }
+            }
+            Bs[ky][j] = b;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kx = 0; kx < kGK; ++kx) {
+            cx<double> a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] = As[kx][ti + 16 * u];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) b[v] = Bs[kx][tj + 16 * v];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = cmad(acc[u][v], a[u], b[v]);
+        }
+        __syncthreads();
+    }
+    const bool zero_beta = (beta.re == 0.0 && beta.im == 0.0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int gi = i0 + ti + 16 * u, gj = j0 + tj + 16 * v;
+            if (gi < N && gj < N) {
+                cx<double>* c = Ck + (size_t)gj * N + gi;
+                cx<double> r = alpha * acc[u][v];
+                if (!zero_beta) r = cmad(r, beta, *c);
+                *c = r;
+            }
+        }
+}
+
+// In-place cyclic Jacobi diagonalization of a small Hermitian matrix K (c x c,
+// row-major in registers of one thread) accumulating the unitary U (K <- U^H K U).
+// Each rotation first makes the (p, q) entry real with a phase, then applies the
+// real symmetric Jacobi rotation t = sign(tau)/(|tau| + sqrt(1 + tau^2)).
+constexpr int kClusterMax = 8;
+
+// Cluster slots kept in LDS by k_oa_correct, and its dynamic LDS size.
+__host__ __device__ inline int oa_max_clusters(int N) { return N / 2 + 1 < 64 ? N / 2 + 1 : 64; }
+inline size_t oa_lds_bytes(int N) {
+    const int maxcl = oa_max_clusters(N);
+    const size_t o_red = (24 * (size_t)N + 8 * (size_t)maxcl + 15) & ~(size_t)15;
+    return o_red + 16 * 8 + (size_t)maxcl * kClusterMax * kClusterMax * 16;
+}
+
+__device__ void jacobi_herm(int c, cx<double>* K, cx<double>* U) {
+    for (int i = 0; i < c; ++i)
+        for (int j = 0; j < c; ++j) U[i * kClusterMax + j] = {i == j ? 1.0 : 0.0, 0.0};
+    for (int sweep = 0; sweep < 12; ++sweep) {
+        double off = 0.0, dia = 0.0;
+        for (int i = 0; i < c; ++i)
+            for (int j = 0; j < c; ++j) {
+                const double a = abs2(K[i * kClusterMax + j]);
+                if (i == j) dia += a; else off += a;
+            }
+        if (off <= 1e-34 * dia || off == 0.0) break;
+        for (int p = 0; p < c - 1; ++p)
+            for (int q = p + 1; q < c; ++q) {
+                const cx<double> kpq = K[p * kClusterMax + q];
+                const double b = sqrt(abs2(kpq));
+                if (b == 0.0) continue;
+                const cx<double> w = {kpq.re / b, kpq.im / b};  // e^{i phi}
+                const double a = K[p * kClusterMax + p].re, d = K[q * kClusterMax + q].re;
+                const double tau = (d - a) / (2.0 * b);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
+                // J = [[cs, sn], [-sn conj(w), cs conj(w)]] on columns (p, q)
+                const cx<double> wc = conj(w);
+                for (int r = 0; r < c; ++r) {  // columns: K <- K J, U <- U J
+                    const cx<double> kp = K[r * kClusterMax + p], kq = K[r * kClusterMax + q];
+                    K[r * kClusterMax + p] = cs * kp - sn * (wc * kq);
+                    K[r * kClusterMax + q] = sn * kp + cs * (wc * kq);
+                    const cx<double> up = U[r * kClusterMax + p], uq = U[r * kClusterMax + q];
+                    U[r * kClusterMax + p] = cs * up - sn * (wc * uq);
+                    U[r * kClusterMax + q] = sn * up + cs * (wc * uq);
+                }
+                for (int col = 0; col < c; ++col) {  // rows: K <- J^H K
+                    const cx<double> kp = K[p * kClusterMax + col], kq = K[q * kClusterMax + col];
+                    K[p * kClusterMax + col] = cs * kp - sn * (w * kq);
+                    K[q * kClusterMax + col] = sn * kp + cs * (w * kq);
+                }
+                K[p * kClusterMax + q] = {0.0, 0.0};
+                K[q * kClusterMax + p] = {0.0, 0.0};
+            }
+    }
+}
+
+// Ogita-Aishima correction with Rayleigh-Ritz on near-degenerate clusters.
+// In: S = X^H A X, G = X^H X.  Out: E[kk] (X <- X + X E), lambda[kk*N + i], and
+// stat[kk*4 + {0: ||R||_F, 1: delta, 2: flags, 3: off-cluster ||S - D||_F}] where
+// flags bit 0 = some cluster was rotated, bit 1 = a cluster wider than kClusterMax
+// that is not degenerate to working precision (left unresolved).
+// Clusters are maximal runs of consecutive columns whose lambda gaps are <= delta
+// (columns arrive in ascending FP32 eigenvalue order).  For a cluster C the
+// orthonormalised block K = (I + R/2)^H S_CC (I + R/2) is diagonalised, K = U L U^H,
+// E_CC = (I + R_CC/2) U - I, lambda_C = L, and E_iC (i outside C) = E^std_iC U.
+__global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __restrict__ S,
+                                                    const cx<double>* __restrict__ G,
+                                                    cx<double>* __restrict__ E,
+                                                    double* __restrict__ lambda,
+                                                    double* __restrict__ stat) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int maxcl = oa_max_clusters(N);
+    const size_t o_lnew = 8 * (size_t)N, o_cid = o_lnew + 8 * (size_t)N, o_cpos = o_cid + 4 * (size_t)N;
+    const size_t o_cst = o_cpos + 4 * (size_t)N, o_csz = o_cst + 4 * (size_t)maxcl;
+    const size_t o_red = (o_csz + 4 * (size_t)maxcl + 15) & ~(size_t)15;
+    const size_t o_u = o_red + 16 * 8;
+    double* lam = reinterpret_cast<double*>(smem);               // [N]  Rayleigh quotients
+    double* lnew = reinterpret_cast<double*>(smem + o_lnew);     // [N]  final lambda
+    int* cid = reinterpret_cast<int*>(smem + o_cid);             // [N]  cluster ordinal or -1
+    int* cpos = reinterpret_cast<int*>(smem + o_cpos);           // [N]  position in the cluster
+    int* cstart = reinterpret_cast<int*>(smem + o_cst);          // [maxcl]
+    int* csize = reinterpret_cast<int*>(smem + o_csz);           // [maxcl]
+    double* red = reinterpret_cast<double*>(smem + o_red);       // [16]
+    cx<double>* Ubuf = reinterpret_cast<cx<double>*>(smem + o_u);  // [maxcl][kClusterMax^2]
+    __shared__ int s_ncl, s_flags;
+    const int kk = blockIdx.x, tid = threadIdx.x;
+    const size_t NN = (size_t)N * N;
+    const cx<double>* Sk = S + (size_t)kk * NN;
+    const cx<double>* Gk = G + (size_t)kk * NN;
+    cx<double>* Ek = E + (size_t)kk * NN;
+    for (int i = tid; i < N; i += 256) {
+        lam[i] = Sk[(size_t)i * N + i].re / Gk[(size_t)i * N + i].re;
+        lnew[i] = lam[i];
+        cid[i] = -1;
+    }
+    __syncthreads();
+    double r2 = 0.0, amax = 0.0;
+    for (int e = tid; e < N * N; e += 256) {
+        const int i = e % N, j = e / N;
+        const cx<double> g = Gk[e];
+        const double rre = (i == j ? 1.0 : 0.0) - g.re, rim = -g.im;
+        r2 += rre * rre + rim * rim;
+    }
+    for (int i = tid; i < N; i += 256) amax = fmax(amax, fabs(lam[i]));
+    double s2all = 0.0;
+    for (int e = tid; e < N * N; e += 256) {
+        const int i = e % N, j = e / N;
+        cx<double> sv = Sk[e];
+        if (i == j) sv.re -= lam[i];
+        s2all += abs2(sv);
+    }
+    r2 = wave_sum(r2);
+    s2all = wave_sum(s2all);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_xor(amax, o));
+    if ((tid & 63) == 0) {
+        red[tid / 64] = r2;
+        red[4 + tid / 64] = s2all;
+        red[8 + tid / 64] = amax;
+    }
+    __syncthreads();
+    const double rn = sqrt(red[0] + red[1] + red[2] + red[3]);
+    const double sn_all = sqrt(red[4] + red[5] + red[6] + red[7]);
+    const double an = fmax(fmax(red[8], red[9]), fmax(red[10], red[11]));
+    const double delta = 2.0 * (sn_all + an * rn);
+    const double tiny = 64.0 * 2.220446049250313e-16 * (an > 0.0 ? an : 1.0);
+    // cluster detection and Rayleigh-Ritz (one thread: clusters are rare and small)
+    if (tid == 0) {
+        int ncl = 0, flags = 0;
+        int i = 0;
+        while (i < N) {
+            int j = i;
+            while (j + 1 < N && fabs(lam[j + 1] - lam[j]) <= delta) ++j;
+            const int c = j - i + 1;
+            if (c >= 2) {
+                double spread = 0.0;
+                for (int q = i + 1; q <= j; ++q) spread = fmax(spread, fabs(lam[q] - lam[i]));
+                if (c > kClusterMax || ncl >= maxcl) {
+                    if (spread > tiny) flags |= 2;
+                } else {
+                    cx<double> K[kClusterMax * kClusterMax], B[kClusterMax * kClusterMax];
+                    // B = I + R_CC / 2 (R = I - G), K = B^H S_CC B
+                    for (int a = 0; a < c; ++a)
+                        for (int b = 0; b < c; ++b) {
+                            const cx<double> g = Gk[(size_t)(i + b) * N + (i + a)];
+                            const double rre = (a == b ? 1.0 : 0.0) - g.re, rim = -g.im;
+                            B[a * kClusterMax + b] = {(a == b ? 1.0 : 0.0) + 0.5 * rre, 0.5 * rim};
+                        }
+                    for (int a = 0; a < c; ++a)
+                        for (int b = 0; b < c; ++b) {
+                            cx<double> acc = {0.0, 0.0};
+                            for (int x = 0; x < c; ++x)
+                                for (int y = 0; y < c; ++y) {
+                                    const cx<double> sxy = Sk[(size_t)(i + y) * N + (i + x)];
+                                    acc = cmad(acc, conj(B[x * kClusterMax + a]) * sxy, B[y * kClusterMax + b]);
+                                }
+                            K[a * kClusterMax + b] = acc;
+                        }
+                    cx<double>* U = Ubuf + (size_t)ncl * kClusterMax * kClusterMax;
+                    jacobi_herm(c, K, U);
+                    for (int a = 0; a < c; ++a) {
+                        lnew[i + a] = K[a * kClusterMax + a].re;
+                        cid[i + a] = ncl;
+                        cpos[i + a] = a;
+                    }
+                    // store BU in place of U (the new coefficients of the cluster columns)
+                    cx<double> BU[kClusterMax * kClusterMax];
+                    for (int a = 0; a < c; ++a)
+                        for (int b = 0; b < c; ++b) {
+                            cx<double> acc = {0.0, 0.0};
+                            for (int x = 0; x < c; ++x) acc = cmad(acc, B[a * kClusterMax + x], U[x * kClusterMax + b]);
+                            BU[a * kClusterMax + b] = acc;
+                        }
+                    for (int q = 0; q < c * kClusterMax; ++q) U[q] = BU[q];
+                    cstart[ncl] = i;
+                    csize[ncl] = c;
+                    ++ncl;
+                    if (spread > tiny) flags |= 1;
+                }
+            }
+            i = j + 1;
+        }
+        s_ncl = ncl;
+        s_flags = flags;
+    }
+    __syncthreads();
+    // E entries
+    double s2off = 0.0;
+    for (int e = tid; e < N * N; e += 256) {
+        const int i = e % N, j = e / N;
+        const int ci = cid[i], cj = cid[j];
+        cx<double> out;
+        if (cj >= 0 && ci == cj) {
+            const cx<double> bu = Ubuf[(size_t)cj * kClusterMax * kClusterMax + cpos[i] * kClusterMax + cpos[j]];
+            out = {bu.re - (i == j ? 1.0 : 0.0), bu.im};
+        } else if (cj >= 0) {
+            // column j of a rotated cluster: E_ij = sum_l Estd_il BU_lj (l in cluster cj)
+            const int c0 = cstart[cj], cs = csize[cj];
+            out = {0.0, 0.0};
+            for (int l = 0; l < cs; ++l) {
+                const int ll = c0 + l;
+                const cx<double> g = Gk[(size_t)ll * N + i];
+                const double rre = -g.re, rim = -g.im;  // i != ll
+                const double gap = lam[ll] - lam[i];
+                cx<double> est;
+                if (fabs(gap) > delta) {
+                    const cx<double> sv = Sk[(size_t)ll * N + i];
+                    est = {(sv.re + lam[ll] * rre) / gap, (sv.im + lam[ll] * rim) / gap};
+                } else {
+                    est = {0.5 * rre, 0.5 * rim};
+                }
+                out = cmad(out, est, Ubuf[(size_t)cj * kClusterMax * kClusterMax + l * kClusterMax + cpos[j]]);
+            }
+        } else {
+            const cx<double> g = Gk[e];
+            const double rre = (i == j ? 1.0 : 0.0) - g.re, rim = -g.im;
+            const double gap = lam[j] - lam[i];
+            if (i != j && fabs(gap) > delta) {
+                const cx<double> sv = Sk[e];
+                out = {(sv.re + lam[j] * rre) / gap, (sv.im + lam[j] * rim) / gap};
+            } else {
+                out = {0.5 * rre, 0.5 * rim};
+            }
+        }
+        Ek[e] = out;
+        if (i != j && !(ci >= 0 && ci == cj)) s2off += abs2(Sk[e]);
+    }
+    s2off = wave_sum(s2off);
+    __syncthreads();
+    if ((tid & 63) == 0) red[12 + tid / 64] = s2off;
+    __syncthreads();
+    for (int i = tid; i < N; i += 256) lambda[(size_t)kk * N + i] = lnew[i];
+    if (tid == 0) {
+        stat[(size_t)kk * 4 + 0] = rn;
+        stat[(size_t)kk * 4 + 1] = delta;
+        stat[(size_t)kk * 4 + 2] = s_flags;
+        stat[(size_t)kk * 4 + 3] = sqrt(red[12] + red[13] + red[14] + red[15]);
+    }
+    (void)s_ncl;
+}
+
+// Y = X diag(f), f_n = fermidirac(lambda_n - mu; T); eb[kk] = w_k sum_n lambda_n f_n.
+__global__ __launch_bounds__(256) void k_scale_occ(int N, const cx<double>* __restrict__ X,
+                                                   const double* __restrict__ lambda,
+                                                   const double* __restrict__ kw, double wuni,
+                                                   double mu, double T, cx<double>* __restrict__ Y,
+                                                   double* __restrict__ eb) {
+    __shared__ double sf[512];
+    __shared__ double sred[4];
+    const int kk = blockIdx.x, tid = threadIdx.x;
+    const double* lk = lambda + (size_t)kk * N;
+    double e = 0.0;
+    for (int n = tid; n < N; n += 256) {
+        const double x = lk[n] - mu;
+        const double f = T > 0.0 ? 1.0 / (exp(x / T) + 1.0) : (x < 0.0 ? 1.0 : (x > 0.0 ? 0.0 : 0.5));
+        if (n < 512) sf[n] = f;
+        e += lk[n] * f;
+    }
+    e = wave_sum(e);
+    if ((tid & 63) == 0) sred[tid / 64] = e;
+    __syncthreads();
+    if (tid == 0) eb[kk] = (kw ? kw[kk] : wuni) * (sred[0] + sred[1] + sred[2] + sred[3]);
+    const size_t NN = (size_t)N * N;
+    for (int q = tid; q < N * N; q += 256) {
+        const int n = q / N;
+        const double f = sf[n];
+        const cx<double> x = X[(size_t)kk * NN + q];
+        Y[(size_t)kk * NN + q] = {f * x.re, f * x.im};
+    }
+}
+
+// Ascending copy of lambda per k-point (rank counting; N <= 1024).
+__global__ __launch_bounds__(256) void k_sort_evals(int N, const double* __restrict__ lambda,
+                                                    double* __restrict__ out) {
+    const int kk = blockIdx.x;
+    const double* l = lambda + (size_t)kk * N;
+    for (int i = threadIdx.x; i < N; i += 256) {
+        const double li = l[i];
+        int rank = 0;
+        for (int j = 0; j < N; ++j) {
+            const double lj = l[j];
+            rank += (lj < li) || (lj == li && j < i);
+        }
+        out[(size_t)kk * N + min(rank, N - 1)] = li;
+    }
+}
+
+}  // namespace lqm

@@ -1,0 +1,86 @@
+"""Mixed precision (FP32 eigensolve + FP64 Ogita-Aishima refinement) vs the FP64 oracle.
+
+Bars: eigenvalues |e - e64| <= 1e-10 * max(|e64|, ||H(k)||_2); rho to 1e-9 absolute;
+E_band to 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mixed(pkg, gpu):
+    return pkg.api.to_executor("gpu", precision="mixed")
+
+
+def _hnorm(h, ks):
+    return np.array([np.linalg.norm(h(k), 2) for k in ks.T])
+
+
+@pytest.mark.parametrize("N,rmax,nk", [(2, 1, 16), (4, 2, 64), (17, 2, 25), (64, 2, 128), (96, 1, 16), (128, 1, 8)])
+def test_mixed_bands(pkg, oracle, mixed, N, rmax, nk):
+    h = pkg.models.random_tb(N, rmax=rmax, seed=300 + N)
+    ks = pkg.regulargrid(nk=nk)
+    R, T = h.pack()
+    ref = oracle.bands(R, T, ks)
+    b, _ = pkg.api.bandmatrix(h, ks, executor=mixed)
+    err = np.abs(b - ref) / np.maximum(np.abs(ref), _hnorm(h, ks)[None, :])
+    assert np.max(err) < 1e-10, np.max(err)
+    rep = mixed.refinement_report()
+    assert rep["sweeps"] >= 1 and rep["max_orth_residual"] < 1e-3
+
+
+@pytest.mark.parametrize("N,nk,mu,T", [(4, 64, 0.2, 0.0), (8, 49, 0.0, 0.05), (64, 64, 0.0, 0.02), (96, 9, 0.1, 0.02)])
+def test_mixed_rho(pkg, oracle, mixed, N, nk, mu, T):
+    h = pkg.models.random_tb(N, rmax=1 if N > 8 else 2, seed=N + 7)
+    ks = pkg.regulargrid(nk=nk)
+    rho = pkg.Hops({R: np.zeros((N, N), complex) for R in h.keys()})
+    eb = pkg.api.getdensitymatrix(rho, h, ks, mu, T=T, executor=mixed)
+    R, Tt = h.pack()
+    ref, eb_ref = oracle.densitymatrix(R, Tt, ks, R, mu=mu, Temp=T)
+    err = max(np.max(np.abs(rho[tuple(R[:, r])] - ref[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+
+
+def test_mixed_near_degenerate_needs_second_sweep(pkg, oracle, mixed):
+    # spin-degenerate levels split by 1e-7: closer than FP32 can resolve
+    h = pkg.models.addspin(pkg.models.random_tb(8, rmax=1, seed=21), "spinhalf")
+    sz = np.diag([1.0, -1.0])
+    h[(0, 0)] = h[(0, 0)] + np.kron(np.eye(8), 1e-7 * sz)
+    ks = pkg.regulargrid(nk=16)
+    R, T = h.pack()
+    ref = oracle.bands(R, T, ks)
+    b, _ = pkg.api.bandmatrix(h, ks, executor=mixed)
+    err = np.abs(b - ref) / np.maximum(np.abs(ref), _hnorm(h, ks)[None, :])
+    assert np.max(err) < 1e-10, np.max(err)
+    assert mixed.refinement_report()["sweeps"] >= 2
+    rho = pkg.Hops({R_: np.zeros((16, 16), complex) for R_ in h.keys()})
+    pkg.api.getdensitymatrix(rho, h, ks, 0.05, T=0.01, executor=mixed)
+    refr, _ = oracle.densitymatrix(R, T, ks, R, mu=0.05, Temp=0.01)
+    err = max(np.max(np.abs(rho[tuple(R[:, r])] - refr[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+
+
+def test_mixed_exactly_degenerate(pkg, oracle, mixed):
+    h = pkg.models.addspin(pkg.models.random_tb(6, rmax=1, seed=5), "spinhalf")
+    ks = pkg.regulargrid(nk=25)
+    R, T = h.pack()
+    rho = pkg.Hops({R_: np.zeros((12, 12), complex) for R_ in h.keys()})
+    pkg.api.getdensitymatrix(rho, h, ks, 0.0, T=0.0, executor=mixed)
+    refr, _ = oracle.densitymatrix(R, T, ks, R, mu=0.0, Temp=0.0)
+    err = max(np.max(np.abs(rho[tuple(R[:, r])] - refr[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+
+
+def test_mixed_chemicalpotential_and_dos(pkg, oracle, mixed):
+    h = pkg.models.random_tb(10, rmax=1, seed=31)
+    ks = pkg.regulargrid(nk=64)
+    R, T = h.pack()
+    bands = oracle.bands(R, T, ks)
+    mu = pkg.api.chemicalpotential(h, ks, 0.4, T=0.01, executor=mixed)
+    assert abs(oracle.getfilling(bands, mu, 0.01) - 0.4) < 1e-12
+    om = np.linspace(-5, 5, 101)
+    dos = pkg.api.getdos(h, om, ks, Gamma=0.1, executor=mixed)
+    assert np.max(np.abs(dos - oracle.dos(R, T, ks, om, 0.1))) < 1e-10 * np.max(dos)
