@@ -8,10 +8,12 @@ HBM.  The default workload is the dense random tight-binding model with 64
 orbitals, 25 hopping keys (|R_x|, |R_y| <= 2), a 128 x 128 Gamma-centred grid,
 mu = 0, kT = 0.02 and rho on all 25 keys.
 
-Printed: one JSON line with the throughput (k-points/s), the dominant kernel's
-roofline figures measured with HIP events inside the timed region, and the FP64
-CPU oracle timed on this machine's host cores over a bounded sample of the same
-workload (cpu_baseline).
+Both GPU paths are timed: the mixed-precision path (FP32 eigensolve, FP64
+Ogita-Aishima refinement; its throughput is ``value``) and the plain FP64 path.
+Printed: one JSON line with both throughputs, the accuracy of the mixed path
+against the FP64 GPU path (whole grid) and against the FP64 CPU oracle, the
+dominant kernel's roofline figures measured with HIP events on the library's
+stream, and the oracle timed on this machine's host cores (cpu_baseline).
 """
 from __future__ import annotations
 
@@ -43,17 +45,6 @@ WORKLOADS = {
     "random_tb_N32_R25_k128x128": (32, 2, 128, 0.0, 0.02),
 }
 DEFAULT = "random_tb_N64_R25_k128x128"
-
-
-def stage_flops(stage: str, N: int, nR: int, nRr: int, nk: int) -> float:
-    """Algorithmic real flops of one launch over nk k-points (complex MAC = 8)."""
-    if stage == "hk_tridiag":
-        return nk * (8.0 * nR * N * N + 16.0 / 3.0 * N ** 3)
-    if stage == "backtransform":
-        return nk * (8.0 * N ** 3 + 8.0 * N ** 3)
-    if stage == "phase_contract":
-        return nk * 8.0 * nRr * N * N
-    return 0.0
 
 
 def host_threads() -> int:
@@ -95,60 +86,97 @@ def main() -> int:
     nR = nRr = R.shape[1]
     Rr = np.asfortranarray(R.astype(np.int32))
 
-    ex = api.to_executor("gpu")
-    model = api.GPUModel(h, ex)
     dev = torch.device("cuda:0")
     ks_d = torch.from_numpy(np.ascontiguousarray(ks.T)).to(dev)
-    rho_d = torch.zeros((nRr, N, N, 2), dtype=torch.float64, device=dev)
-    eb_d = torch.zeros(1, dtype=torch.float64, device=dev)
-    bands_d = torch.zeros((Nk, N), dtype=torch.float64, device=dev)
     L = _lib.lib()
     vp = ctypes.c_void_p
 
-    def step(bands=None):
-        st = L.lqm_densitymatrix_dev(model.handle, Nk, vp(ks_d.data_ptr()), None, mu, T, nRr,
-                                     Rr.ctypes.data_as(vp), vp(rho_d.data_ptr()), vp(eb_d.data_ptr()),
-                                     vp(bands.data_ptr()) if bands is not None else None)
-        _lib.check(st, "lqm_densitymatrix_dev")
+    class Path:
+        """One precision mode: its executor, resident model and device outputs."""
 
-    chosen = api.autotune(model, ks_d.data_ptr(), min(Nk, 2048))
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = Nk * args.steps / (t1 - t0)
+        def __init__(self, precision):
+            self.ex = api.to_executor("gpu", precision=precision)
+            self.model = api.GPUModel(h, self.ex)
+            self.rho = torch.zeros((nRr, N, N, 2), dtype=torch.float64, device=dev)
+            self.eb = torch.zeros(1, dtype=torch.float64, device=dev)
+            self.bands = torch.zeros((Nk, N), dtype=torch.float64, device=dev)
 
-    # per-stage device time of the same passes (HIP events on the library's stream)
-    _lib.call("lqm_context_set_timing", ex.handle, 1)
-    nt = max(1, min(args.steps, 5))
-    for _ in range(nt):
-        step(bands_d)
-    ms = (ctypes.c_double * _lib.NSTAGES)()
-    cnt = (ctypes.c_int64 * _lib.NSTAGES)()
-    _lib.call("lqm_context_stage_times", ex.handle, ms, cnt, _lib.NSTAGES)
-    _lib.call("lqm_context_set_timing", ex.handle, 0)
-    stages = {}
-    for i in range(_lib.NSTAGES):
-        if cnt[i]:
-            name = L.lqm_stage_name(i).decode()
-            stages[name] = {"ms_per_step": ms[i] / nt, "launches_per_step": cnt[i] / nt,
-                            "avg_launch_ms": ms[i] / cnt[i]}
-    dom = max(stages, key=lambda s: stages[s]["ms_per_step"])
-    launches = stages[dom]["launches_per_step"]
-    fl = stage_flops(dom, N, nR, nRr, Nk) / launches
-    achieved = fl / (stages[dom]["avg_launch_ms"] * 1e-3) / 1e12
-    path_tflops = stage_flops("hk_tridiag", N, nR, nRr, Nk) + stage_flops("backtransform", N, nR, nRr, Nk) \
-        + stage_flops("phase_contract", N, nR, nRr, Nk)
-    roofline = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                "flops_per_launch": fl, "avg_launch_ms": stages[dom]["avg_launch_ms"]}
+        def step(self, bands=False):
+            st = L.lqm_densitymatrix_dev(self.model.handle, Nk, vp(ks_d.data_ptr()), None, mu, T, nRr,
+                                         Rr.ctypes.data_as(vp), vp(self.rho.data_ptr()),
+                                         vp(self.eb.data_ptr()),
+                                         vp(self.bands.data_ptr()) if bands else None)
+            _lib.check(st, "lqm_densitymatrix_dev")
 
-    # correctness on a sample (GPU vs oracle on the same sub-grid)
+        def timed(self, steps, warmup):
+            for _ in range(warmup):
+                self.step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                self.step()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / steps
+
+        def stages(self, n):
+            _lib.call("lqm_context_set_timing", self.ex.handle, 1)
+            for _ in range(n):
+                self.step(bands=True)
+            ms = (ctypes.c_double * _lib.NSTAGES)()
+            cnt = (ctypes.c_int64 * _lib.NSTAGES)()
+            _lib.call("lqm_context_stage_times", self.ex.handle, ms, cnt, _lib.NSTAGES)
+            _lib.call("lqm_context_set_timing", self.ex.handle, 0)
+            out = {}
+            for i in range(_lib.NSTAGES):
+                if cnt[i]:
+                    out[L.lqm_stage_name(i).decode()] = {
+                        "ms_per_step": ms[i] / n, "launches_per_step": cnt[i] / n,
+                        "avg_launch_ms": ms[i] / cnt[i]}
+            return out
+
+    fp64 = Path("fp64")
+    mixed = Path("mixed")
+    chosen = api.autotune(fp64.model, ks_d.data_ptr(), min(Nk, 2048))
+    t64 = fp64.timed(args.steps, args.warmup)
+    tmx = mixed.timed(args.steps, args.warmup)
+    value = Nk / tmx
+    ms_per_step = tmx * 1e3
+    nt = max(1, min(args.steps, 3))
+    st64 = fp64.stages(nt)
+    stmx = mixed.stages(nt)
+    report = mixed.ex.refinement_report()
+
+    # accuracy of the mixed path against the FP64 GPU path on the whole grid
+    b64 = fp64.bands.cpu().numpy()
+    bmx = mixed.bands.cpu().numpy()
+    scale = np.maximum(np.abs(b64), np.max(np.abs(b64), axis=1, keepdims=True))
+    eig_err = float(np.max(np.abs(bmx - b64) / scale))
+    rho_err = float((mixed.rho - fp64.rho).abs().max().item())
+    eb_err = float(abs(mixed.eb.item() - fp64.eb.item()))
+
+    # dominant kernel of the headline (mixed) path
+    def flops(stage, launches):
+        nk = Nk / launches
+        if stage == "hk_tridiag":
+            return nk * (8.0 * nR * N * N + 16.0 / 3.0 * N ** 3)
+        if stage == "backtransform":
+            return nk * 8.0 * N ** 3
+        if stage == "zgemm_fp64":
+            return nk * 8.0 * N ** 3
+        if stage == "phase_contract":
+            return nk * 8.0 * nRr * N * N
+        return 0.0
+
+    dom = max(stmx, key=lambda k: stmx[k]["ms_per_step"])
+    fl = flops(dom, stmx[dom]["launches_per_step"])
+    peak = PEAK_FP64_TFLOPS if dom in ("zgemm_fp64", "phase_contract") else 157.3
+    achieved = fl / (stmx[dom]["avg_launch_ms"] * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "flops_per_launch": fl, "avg_launch_ms": stmx[dom]["avg_launch_ms"],
+                "precision_of_peak": "fp64" if peak == PEAK_FP64_TFLOPS else "fp32"}
+    path_flops = Nk * ((64.0 / 3.0) * N ** 3 + 8.0 * (nR + nRr) * N * N + 2.0 * N * N)
+
     out = {}
     cpu = None
     if not args.no_cpu:
@@ -165,16 +193,18 @@ def main() -> int:
             times.append(time.perf_counter() - c0)
         cpu_t = statistics.median(times)
         cpu = {"value": S / cpu_t, "unit": "k-points/s", "cores": nth, "kind": "port",
-               "sample": f"{S} of the {Nk} k-points (strided), full pass H(k)+zheevr('V')+rho on "
-                         f"{nRr} keys, OpenMP static chunks, OpenBLAS 1 thread/core, median of 3"}
-        rho_g = pkg.Hops({tuple(R[:, r]): np.zeros((N, N), complex) for r in range(nRr)})
-        eb_g = api.getdensitymatrix(rho_g, model, sub, mu, T=T)
-        err = max(np.max(np.abs(rho_g[tuple(R[:, r])] - ref[:, :, r])) for r in range(nRr))
+               "sample": f"{S} of the {Nk} k-points, full pass H(k)+zheevr('V')+rho on {nRr} keys, "
+                         f"OpenMP static chunks, OpenBLAS 1 thread/core, median of 3"}
+        if S == Nk:
+            ref_rho = np.stack([ref[:, :, r].T for r in range(nRr)])  # [r][j][i] like the device layout
+            dev_rho = mixed.rho.cpu().numpy()
+            dev_rho = dev_rho[..., 0] + 1j * dev_rho[..., 1]
+            out["max_abs_err_rho_mixed_vs_fp64_oracle"] = float(np.max(np.abs(dev_rho - ref_rho)))
+            out["abs_err_eband_mixed_vs_fp64_oracle"] = float(abs(mixed.eb.item() - eb_ref))
         bands_ref = oracle.bands(R, Tst, sub, nthreads=nth)
-        bands_g, _ = api.bandmatrix(model, sub)
-        out["max_abs_err_rho_vs_fp64_oracle"] = float(err)
-        out["abs_err_eband_vs_fp64_oracle"] = float(abs(eb_g - eb_ref))
-        out["max_eigval_err_vs_fp64_oracle"] = float(np.max(np.abs(bands_g - bands_ref)))
+        idx = np.arange(Nk)[:: max(1, Nk // S)][:S]
+        bsc = np.maximum(np.abs(bands_ref), np.max(np.abs(bands_ref), axis=0, keepdims=True))
+        out["max_eigval_err_mixed_vs_fp64_oracle"] = float(np.max(np.abs(bmx[idx].T - bands_ref) / bsc))
 
     line = {
         "metric": METRIC,
@@ -187,15 +217,22 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 eigensolve + f64 refinement",
         "data": "synthetic (seeded random tight-binding model)",
         "config": {"workload": args.workload, "N": N, "nR": nR, "nRrho": nRr, "Nk": Nk, "mu": mu,
-                   "kT": T, "precision": "fp64 plain path", "parallelism": "single GPU",
-                   "kernel_variants": {"tridiag": chosen[0], "rho": chosen[1]}},
+                   "kT": T, "precision": "mixed: fp32 H(k)+eigensolve, fp64 Ogita-Aishima refinement",
+                   "parallelism": "single GPU",
+                   "kernel_variants_fp64": {"tridiag": chosen[0], "rho": chosen[1]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "stages": stages,
-        "path_tflops_per_s": path_tflops / (ms_per_step * 1e-3) / 1e12,
+        "fp64_gpu": {"value": Nk / t64, "unit": "k-points/s", "ms_per_step": t64 * 1e3,
+                     "path_tflops_per_s": path_flops / t64 / 1e12, "stages": st64},
+        "mixed_over_fp64": t64 / tmx,
+        "mixed_stages": stmx,
+        "mixed_accuracy_vs_fp64_gpu": {"max_rel_eigval_err": eig_err, "max_abs_rho_err": rho_err,
+                                       "abs_eband_err": eb_err},
+        "refinement": report,
+        "path_tflops_per_s": path_flops / tmx / 1e12,
         **out,
     }
     print(json.dumps(line))

@@ -52,6 +52,7 @@ enum Stage {
     ST_TRIDIAG, ST_QL, ST_REPLAY, ST_BACKTRANSFORM, ST_ZFZ, ST_RHO2, ST_CONTRACT, ST_FOLD, ST_DOS,
     ST_REDUCE, ST_HK64, ST_GEMM, ST_OA, ST_N
 };
+static_assert(ST_N == LQM_NSTAGES, "stage list and LQM_NSTAGES disagree");
 const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "backtransform", "zfz",
                                        "rho_twosided", "phase_contract", "phase_fold", "dos", "reduce",
                                        "herm_hk_fp64", "zgemm_fp64", "oa_correct"};
