@@ -66,6 +66,8 @@ def main() -> int:
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-sample", type=int, default=16384, help="k-points of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="after timing, copy this many MiB device-to-device once (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the k-grid would shard "
@@ -161,7 +163,7 @@ def main() -> int:
             return nk * (8.0 * nR * N * N + 16.0 / 3.0 * N ** 3)
         if stage == "backtransform":
             return nk * 8.0 * N ** 3
-        if stage == "zgemm_fp64":
+        if stage.startswith("zgemm_"):
             return nk * 8.0 * N ** 3
         if stage == "phase_contract":
             return nk * 8.0 * nRr * N * N
@@ -169,13 +171,30 @@ def main() -> int:
 
     dom = max(stmx, key=lambda k: stmx[k]["ms_per_step"])
     fl = flops(dom, stmx[dom]["launches_per_step"])
-    peak = PEAK_FP64_TFLOPS if dom in ("zgemm_fp64", "phase_contract") else 157.3
+    peak = PEAK_FP64_TFLOPS if (dom.startswith("zgemm_") or dom == "phase_contract") else 157.3
     achieved = fl / (stmx[dom]["avg_launch_ms"] * 1e-3) / 1e12
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            tr = json.load(fh).get("stages", {}).get(dom)
+        if tr is not None:
+            traffic = tr["bytes_per_launch"]
     roofline = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
+                "traffic_source": os.path.relpath(tfile, ROOT) if traffic is not None else None,
                 "flops_per_launch": fl, "avg_launch_ms": stmx[dom]["avg_launch_ms"],
                 "precision_of_peak": "fp64" if peak == PEAK_FP64_TFLOPS else "fp32"}
     path_flops = Nk * ((64.0 / 3.0) * N ** 3 + 8.0 * (nR + nRr) * N * N + 2.0 * N * N)
+
+    if args.copy_probe > 0:
+        nbytes = args.copy_probe << 20
+        src = torch.empty(nbytes // 8, dtype=torch.float64, device=dev).uniform_()
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     out = {}
     cpu = None

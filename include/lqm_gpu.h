@@ -68,8 +68,9 @@ lqm_status lqm_context_synchronize(lqm_context *ctx);
  * milliseconds and launch counts since the last query, then resets them.
  * Stages: 0 hk_tridiag, 1 tridiag_ql, 2 replay, 3 backtransform, 4 zfz,
  * 5 rho_twosided, 6 phase_contract, 7 phase_fold, 8 dos, 9 reduce, 10 herm_hk_fp64,
- * 11 zgemm_fp64, 12 oa_correct (LQM_NSTAGES = 13); lqm_stage_name gives the names. */
-#define LQM_NSTAGES 13
+ * 11 zgemm_nn_fp64, 12 zgemm_cn_fp64, 13 zgemm_nc_fp64, 14 oa_correct (LQM_NSTAGES = 15);
+ * lqm_stage_name gives the names. */
+#define LQM_NSTAGES 15
 lqm_status lqm_context_set_timing(lqm_context *ctx, int enable);
 lqm_status lqm_context_stage_times(lqm_context *ctx, double *ms, int64_t *launches, int n);
 const char *lqm_stage_name(int stage);

@@ -59,7 +59,7 @@ _PROTOS = {
     "lqm_densitymatrix_dev": [P, I, P, P, D, D, I, P, P, P, P],
 }
 _RESTYPE = {"lqm_last_error": ctypes.c_char_p, "lqm_abi_version": I, "lqm_stage_name": ctypes.c_char_p}
-NSTAGES = 13
+NSTAGES = 15
 
 
 def header_symbols(path: str = HEADER) -> list[str]:

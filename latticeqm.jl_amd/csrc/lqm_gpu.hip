@@ -50,12 +50,13 @@ constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch
 // Kernel stages timed with HIP events when timing is on.
 enum Stage {
     ST_TRIDIAG, ST_QL, ST_REPLAY, ST_BACKTRANSFORM, ST_ZFZ, ST_RHO2, ST_CONTRACT, ST_FOLD, ST_DOS,
-    ST_REDUCE, ST_HK64, ST_GEMM, ST_OA, ST_N
+    ST_REDUCE, ST_HK64, ST_GEMM_NN, ST_GEMM_CN, ST_GEMM_NC, ST_OA, ST_N
 };
 static_assert(ST_N == LQM_NSTAGES, "stage list and LQM_NSTAGES disagree");
 const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "backtransform", "zfz",
                                        "rho_twosided", "phase_contract", "phase_fold", "dos", "reduce",
-                                       "herm_hk_fp64", "zgemm_fp64", "oa_correct"};
+                                       "herm_hk_fp64", "zgemm_nn_fp64", "zgemm_cn_fp64",
+                                       "zgemm_nc_fp64", "oa_correct"};
 
 struct TimedLaunch {
     int stage;
@@ -528,7 +529,7 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
 template <int OPA, int OPB>
 void zgemm(lqm_context* c, int N, int Kc, const cx<double>* A, const cx<double>* B, cx<double>* C,
            cx<double> alpha, cx<double> beta) {
-    StageScope sc(c, ST_GEMM);
+    StageScope sc(c, OPA == 0 ? (OPB == 0 ? ST_GEMM_NN : ST_GEMM_NC) : ST_GEMM_CN);
     const int t = (N + lqm::kGT - 1) / lqm::kGT;
     auto kG = lqm::k_zgemm_batched<OPA, OPB>;
     hipLaunchKernelGGL(kG, dim3(t, t, Kc), dim3(256), 0, c->stream, N, A, B, C, alpha, beta);
