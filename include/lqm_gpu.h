@@ -68,9 +68,9 @@ lqm_status lqm_context_synchronize(lqm_context *ctx);
  * milliseconds and launch counts since the last query, then resets them.
  * Stages: 0 hk_tridiag, 1 tridiag_ql, 2 replay, 3 backtransform, 4 zfz,
  * 5 rho_twosided, 6 phase_contract, 7 phase_fold, 8 dos, 9 reduce, 10 herm_hk_fp64,
- * 11 zgemm_nn_fp64, 12 zgemm_cn_fp64, 13 zgemm_nc_fp64, 14 oa_correct (LQM_NSTAGES = 15);
- * lqm_stage_name gives the names. */
-#define LQM_NSTAGES 15
+ * 11 zgemm_nn_fp64, 12 zgemm_cn_fp64, 13 zgemm_nc_fp64, 14 oa_correct, 15 hf_meanfield,
+ * 16 mix, 17 mu_bisect (LQM_NSTAGES = 18); lqm_stage_name gives the names. */
+#define LQM_NSTAGES 18
 lqm_status lqm_context_set_timing(lqm_context *ctx, int enable);
 lqm_status lqm_context_stage_times(lqm_context *ctx, double *ms, int64_t *launches, int n);
 const char *lqm_stage_name(int stage);
@@ -158,6 +158,55 @@ lqm_status lqm_densitymatrix_dev(lqm_model *m, int Nk, const double *ks_dev,
                                  const double *kweights_dev, double mu, double T, int nRr,
                                  const int32_t *Rr, double *rho_dev, double *eband_dev,
                                  double *bands_dev);
+
+/* ---- Hartree-Fock mean field and self-consistency ----------------------------------- */
+/* Flags of the HF functional: which channels HartreeFock(h, v; hartree, fock) keeps
+ * (Meanfield/hartreefock.jl:158). */
+#define LQM_HF_HARTREE 1
+#define LQM_HF_FOCK 2
+
+/* The HartreeFock functional applied to rho (Meanfield/hartreefock.jl:178-221; the
+ * meanfieldOperator! / meanfieldScalar! pair behind hf(rho)):
+ *   H_MF[L] = h[L] - v[L] .* rho[L]                      (Fock, no conjugate, :247-276)
+ *   H_MF[0][i,i] += (V0 * diag(rho[0]))[i], V0 = sum_L v[L]   (Hartree, :278-289)
+ *   eps_H = 1/2 n^T V0 n, eps_F = -1/2 sum_L sum_ij v_L |rho_L|^2   (:392-410).
+ * h: nRh keys Rh [d x nRh] with blocks Th; v: nRv keys Rv, blocks Tv; rho: nRr keys Rr,
+ * blocks rho.  The key set of H_MF is the keys of h in order, then the keys of v that h
+ * lacks, then the zero key if neither has it: *nK_out (<= nRh + nRv + 1) keys are
+ * written to RK_out [d x nK] and their blocks to hmf_out [N x N x nK] (RK_out and
+ * hmf_out may be NULL).  energies_out[2] = {eps_H, eps_F}.  The Hartree term needs the
+ * zero key in rho (LQM_ERR_ARG otherwise); a complex eps_H or eps_F beyond
+ * sqrt(eps) max(1, N) (a non-Hermitian rho) is LQM_ERR_ARG, the reference's assertion. */
+lqm_status lqm_hartreefock_meanfield(lqm_context *ctx, int N, int d, int nRh, const int32_t *Rh,
+                                     const double *Th, int nRv, const int32_t *Rv,
+                                     const double *Tv, int nRr, const int32_t *Rr,
+                                     const double *rho, int flags, int *nK_out, int32_t *RK_out,
+                                     double *hmf_out, double *energies_out);
+
+/* Self-consistent Hartree-Fock: solvehartreefock(h, v, rho_init, filling, ks; T, beta,
+ * tol, iterations) (Meanfield/selfconsistent.jl:19, :366-410, :256-336 and the linear
+ * branch of fixedpoint!, Meanfield/fixedpoint.jl:39-194).  Every iteration, on the device:
+ *   H_MF[rho0] and its energies; mu = chemicalpotential(H_MF, ks, filling; T) from an
+ *   eigenvalues pass; rho1 = getdensitymatrix(H_MF, ks, mu; T) on the keys Rr, returning
+ *   eps_band; E = eps_band - (eps_H + eps_F); rho1 <- beta rho1 + (1 - beta) rho0;
+ *   residual = ||rho1 - rho0|| / ||rho1|| (Frobenius over all blocks, after mixing;
+ *   absolute when ||rho1|| <= eps); rho0 <- rho1; stop when residual < tol.
+ * Then H_MF and the energies are refreshed at the final rho and
+ *   eps_kin = eps_band - 2 (eps_H + eps_F), E_GS = eps_band - (eps_H + eps_F).
+ * rho [N x N x nRr] holds the initial guess on entry (the caller has made it Hermitian,
+ * sanitize!) and the final density matrix on return.  kweights (may be NULL: uniform
+ * 1/Nk) weight both mu and rho.  result[8] = {E_GS, mu, eps_band, eps_H, eps_F, eps_kin,
+ * residual, iterations}; *converged = residual < tol; history (NULL or 3 x max_iter
+ * doubles) receives (E, residual, mu) per iteration; nK_out / RK_out / hmf_out as in
+ * lqm_hartreefock_meanfield, for the final H_MF.  The context's precision selects the
+ * eigensolver of both k passes; mu, occupations, mixing and energies are FP64. */
+lqm_status lqm_hartreefock_scf(lqm_context *ctx, int N, int d, int nRh, const int32_t *Rh,
+                               const double *Th, int nRv, const int32_t *Rv, const double *Tv,
+                               int nRr, const int32_t *Rr, double *rho, int Nk, const double *ks,
+                               const double *kweights, double filling, double T, double beta,
+                               double tol, int max_iter, int flags, double *result,
+                               int *converged, double *history, int *nK_out, int32_t *RK_out,
+                               double *hmf_out);
 
 #ifdef __cplusplus
 }

@@ -17,9 +17,9 @@ def __getattr__(name):
     # containers stay importable where the library has not been built.
     import importlib
 
+    if name in ("api", "meanfield"):
+        return importlib.import_module(__name__ + "." + name)
     api = importlib.import_module(__name__ + ".api")
-    if name == "api":
-        return api
     if hasattr(api, name):
         return getattr(api, name)
     raise AttributeError(name)

@@ -57,9 +57,14 @@ _PROTOS = {
     "lqm_chemicalpotential_from_bands_dev": [P, I, I, P, P, D, D, P],
     "lqm_densitymatrix": [P, I, P, P, D, D, I, P, P, P],
     "lqm_densitymatrix_dev": [P, I, P, P, D, D, I, P, P, P, P],
+    "lqm_hartreefock_meanfield": [P, I, I, I, P, P, I, P, P, I, P, P, I, P, P, P, P],
+    "lqm_hartreefock_scf": [P, I, I, I, P, P, I, P, P, I, P, P, I, P, P, D, D, D, D, I, I, P, P, P,
+                            P, P, P],
 }
 _RESTYPE = {"lqm_last_error": ctypes.c_char_p, "lqm_abi_version": I, "lqm_stage_name": ctypes.c_char_p}
-NSTAGES = 15
+NSTAGES = 18
+LQM_HF_HARTREE = 1
+LQM_HF_FOCK = 2
 
 
 def header_symbols(path: str = HEADER) -> list[str]:

@@ -18,6 +18,7 @@
 #include "eig_kernels.hpp"
 #include "wave_kernels.hpp"
 #include "refine_kernels.hpp"
+#include "hf_kernels.hpp"
 
 using lqm::cx;
 
@@ -50,13 +51,14 @@ constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch
 // Kernel stages timed with HIP events when timing is on.
 enum Stage {
     ST_TRIDIAG, ST_QL, ST_REPLAY, ST_BACKTRANSFORM, ST_ZFZ, ST_RHO2, ST_CONTRACT, ST_FOLD, ST_DOS,
-    ST_REDUCE, ST_HK64, ST_GEMM_NN, ST_GEMM_CN, ST_GEMM_NC, ST_OA, ST_N
+    ST_REDUCE, ST_HK64, ST_GEMM_NN, ST_GEMM_CN, ST_GEMM_NC, ST_OA, ST_HF, ST_MIX, ST_MU, ST_N
 };
 static_assert(ST_N == LQM_NSTAGES, "stage list and LQM_NSTAGES disagree");
 const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "backtransform", "zfz",
                                        "rho_twosided", "phase_contract", "phase_fold", "dos", "reduce",
                                        "herm_hk_fp64", "zgemm_nn_fp64", "zgemm_cn_fp64",
-                                       "zgemm_nc_fp64", "oa_correct"};
+                                       "zgemm_nc_fp64", "oa_correct", "hf_meanfield", "mix",
+                                       "mu_bisect"};
 
 struct TimedLaunch {
     int stage;
@@ -689,37 +691,6 @@ lqm_status dos_bands_dev(lqm_context* c, int N, int Nk, const double* bands, con
     return LQM_OK;
 }
 
-// Roots.jl-style midpoint: 0 across a sign change, else the midpoint of the
-// bit patterns of |a| and |b| (so bisection ends in <= 64 steps).
-double bit_middle(double a, double b) {
-    if ((a < 0) != (b < 0) && a != 0 && b != 0) return 0.0;
-    uint64_t ia, ib;
-    double aa = std::fabs(a), bb = std::fabs(b);
-    std::memcpy(&ia, &aa, 8);
-    std::memcpy(&ib, &bb, 8);
-    uint64_t im = (ia >> 1) + (ib >> 1) + (ia & ib & 1);
-    double mid;
-    std::memcpy(&mid, &im, 8);
-    return (a + b) < 0 ? -mid : mid;
-}
-
-lqm_status filling_eval(lqm_context* c, int N, int Nk, const double* bands, const double* kw,
-                        double mu, double T, double* out) {
-    const int Ne = N * Nk;
-    const int G = std::max(1, std::min(1024, (Ne + 255) / 256));
-    double* part;
-    lqm_status st = wsT(c, W_SMALL, (size_t)G + 8, &part);
-    if (st) return st;
-    hipLaunchKernelGGL(lqm::k_filling, dim3(G), dim3(256), 0, c->stream, N, Ne, bands, kw, mu, T, part);
-    hipLaunchKernelGGL(lqm::k_sum, dim3(1), dim3(256), 0, c->stream, G, part, part + G, 0);
-    double h = 0;
-    LQM_HIP(hipMemcpyAsync(&h, part + G, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    LQM_HIP(hipStreamSynchronize(c->stream));
-    // getfilling: unweighted mean over N*Nk, or sum_k w_k sum_n f / N
-    *out = kw ? h / (double)N : h / (double)Ne;
-    return LQM_OK;
-}
-
 uint64_t host_dkey(double v) {
     uint64_t u;
     std::memcpy(&u, &v, 8);
@@ -779,44 +750,22 @@ lqm_status mu_from_bands(lqm_context* c, int N, int Nk, const double* bands, con
         *mu = (a + b) / 2;
         return LQM_OK;
     }
-    // T > 0: bracketing bisection of getfilling(mu) - filling on [min, max]
-    const int G = std::max(1, std::min(1024, (int)((L + 255) / 256)));
-    double* part;
-    lqm_status st = wsT(c, W_SMALL, (size_t)2 * G + 8, &part);
+    // T > 0: bracketing bisection of getfilling(mu) - filling on [min, max], on the device
+    double* res;
+    lqm_status st = wsT(c, W_SMALL, 8, &res);
     if (st) return st;
-    hipLaunchKernelGGL(lqm::k_minmax, dim3(G), dim3(256), 0, c->stream, (int)L, bands, part);
-    std::vector<double> mm(2 * G);
-    LQM_HIP(hipMemcpyAsync(mm.data(), part, sizeof(double) * 2 * G, hipMemcpyDeviceToHost, c->stream));
+    {
+        StageScope sc(c, ST_MU);
+        hipLaunchKernelGGL(lqm::k_mu_bisect, dim3(1), dim3(1024), 0, c->stream, N, (int)L, bands, kw,
+                           filling, T, res);
+    }
+    LQM_HIP(hipGetLastError());
+    double h[2];
+    LQM_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     LQM_HIP(hipStreamSynchronize(c->stream));
-    double lo = INFINITY, hi = -INFINITY;
-    for (int g = 0; g < G; ++g) {
-        lo = std::min(lo, mm[2 * g]);
-        hi = std::max(hi, mm[2 * g + 1]);
-    }
-    if (!std::isfinite(lo) || !std::isfinite(hi)) return fail(LQM_ERR_NONFINITE, "non-finite bands");
-    auto F = [&](double x, double* fx) -> lqm_status {
-        double f;
-        lqm_status s2 = filling_eval(c, N, Nk, bands, kw, x, T, &f);
-        *fx = f - filling;
-        return s2;
-    };
-    double fa, fb;
-    if ((st = F(lo, &fa))) return st;
-    if ((st = F(hi, &fb))) return st;
-    if (fa == 0) { *mu = lo; return LQM_OK; }
-    if (fb == 0) { *mu = hi; return LQM_OK; }
-    if ((fa < 0) == (fb < 0)) return fail(LQM_ERR_ARG, "filling not bracketed by the band range");
-    double a = lo, b = hi;
-    for (int it = 0; it < 200; ++it) {
-        double mid = bit_middle(a, b);
-        if (!(mid > a && mid < b)) break;
-        double fm;
-        if ((st = F(mid, &fm))) return st;
-        if (fm == 0) { *mu = mid; return LQM_OK; }
-        if ((fm < 0) == (fa < 0)) { a = mid; fa = fm; }
-        else { b = mid; fb = fm; }
-    }
-    *mu = std::fabs(fa) <= std::fabs(fb) ? a : b;
+    if (h[1] == 2.0) return fail(LQM_ERR_NONFINITE, "non-finite bands");
+    if (h[1] == 1.0) return fail(LQM_ERR_ARG, "filling not bracketed by the band range");
+    *mu = h[0];
     return LQM_OK;
 }
 
@@ -853,6 +802,159 @@ lqm_status model_init(lqm_context* ctx, int N, int d, int nR, const int32_t* R, 
         return fail(LQM_ERR_DEVICE, std::string("model allocation: ") + hipGetErrorString(e));
     }
     *out = m;
+    return LQM_OK;
+}
+
+// ---- Hartree-Fock: key bookkeeping, device state, the self-consistent loop ----------
+constexpr int kMixBlocks = 256;
+
+struct HFState {
+    lqm_context* c = nullptr;
+    int N = 0, d = 0, nRh = 0, nRv = 0, nRr = 0, nK = 0, qz = -1, rz = -1, flags = 3;
+    std::vector<int32_t> RK;  // d x nK mean-field keys
+    std::vector<int> map;     // per mean-field key: index in h, v, rho (-1 absent)
+    std::vector<int> vr;      // per v key: index in rho (-1 absent)
+    void* mem = nullptr;
+    cx<double>*h = nullptr, *v = nullptr, *V0 = nullptr, *rho0 = nullptr, *rho1 = nullptr;
+    int *dmap = nullptr, *dvr = nullptr;
+    double* scal = nullptr;   // 16 doubles of per-iteration scalars
+    double* part = nullptr;   // 2 x kMixBlocks mixing partials
+    double* bands = nullptr;  // N x Nk eigenvalues of the mu pass
+    lqm_model* hmf = nullptr;
+    ~HFState() {
+        if (c) (void)hipStreamSynchronize(c->stream);
+        if (hmf) lqm_model_destroy(hmf);
+        if (mem) (void)hipFree(mem);
+    }
+};
+
+lqm_status hf_keys(int d, int n, const int32_t* R, const char* what, std::map<std::vector<int>, int>& out) {
+    for (int r = 0; r < n; ++r) {
+        std::vector<int> k(R + (size_t)r * d, R + (size_t)(r + 1) * d);
+        if (!out.emplace(k, r).second) return fail(LQM_ERR_ARG, std::string("duplicate key in ") + what);
+    }
+    return LQM_OK;
+}
+
+lqm_status hf_setup(HFState& S, lqm_context* c, int N, int d, int nRh, const int32_t* Rh,
+                    const double* Th, int nRv, const int32_t* Rv, const double* Tv, int nRr,
+                    const int32_t* Rr, const double* rho, int flags, int Nk) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    if (N <= 0 || d <= 0) return fail(LQM_ERR_ARG, "N and d must be positive");
+    if (nRh < 1 || !Rh || !Th) return fail(LQM_ERR_ARG, "h needs at least one key and its blocks");
+    if (nRv < 0 || (nRv > 0 && (!Rv || !Tv))) return fail(LQM_ERR_ARG, "bad interaction v");
+    if (nRr < 1 || !Rr || !rho) return fail(LQM_ERR_ARG, "rho needs at least one key and its blocks");
+    if (flags < 0 || flags > 3) return fail(LQM_ERR_ARG, "flags must combine LQM_HF_HARTREE and LQM_HF_FOCK");
+    S.c = c;
+    S.N = N;
+    S.d = d;
+    S.nRh = nRh;
+    S.nRv = nRv;
+    S.nRr = nRr;
+    S.flags = flags;
+    std::map<std::vector<int>, int> kh, kv, kr;
+    lqm_status st;
+    if ((st = hf_keys(d, nRh, Rh, "h", kh)) || (st = hf_keys(d, nRv, Rv, "v", kv)) ||
+        (st = hf_keys(d, nRr, Rr, "rho", kr)))
+        return st;
+    // H_MF keys: keys(h), then keys(v) \ keys(h) (initialize_hMF!, hartreefock.jl:186-201),
+    // then the zero key for the Hartree shift
+    std::vector<std::vector<int>> K;
+    for (int r = 0; r < nRh; ++r) K.emplace_back(Rh + (size_t)r * d, Rh + (size_t)(r + 1) * d);
+    for (int r = 0; r < nRv; ++r) {
+        std::vector<int> k(Rv + (size_t)r * d, Rv + (size_t)(r + 1) * d);
+        if (!kh.count(k)) K.push_back(k);
+    }
+    const std::vector<int> zero(d, 0);
+    if (!kh.count(zero) && !kv.count(zero)) K.push_back(zero);
+    S.nK = (int)K.size();
+    S.RK.clear();
+    S.map.assign(3 * S.nK, -1);
+    for (int q = 0; q < S.nK; ++q) {
+        S.RK.insert(S.RK.end(), K[q].begin(), K[q].end());
+        auto a = kh.find(K[q]), b = kv.find(K[q]), e = kr.find(K[q]);
+        if (a != kh.end()) S.map[3 * q] = a->second;
+        if (b != kv.end()) S.map[3 * q + 1] = b->second;
+        if (e != kr.end()) S.map[3 * q + 2] = e->second;
+        if (K[q] == zero) S.qz = q;
+    }
+    auto z = kr.find(zero);
+    S.rz = z == kr.end() ? -1 : z->second;
+    if ((flags & 1) && S.rz < 0) return fail(LQM_ERR_ARG, "rho has no zero key (needed by the Hartree term)");
+    S.vr.assign(std::max(nRv, 1), -1);
+    for (int r = 0; r < nRv; ++r) {
+        auto e = kr.find(std::vector<int>(Rv + (size_t)r * d, Rv + (size_t)(r + 1) * d));
+        if (e != kr.end()) S.vr[r] = e->second;
+    }
+    // one device allocation: h, v, V0, rho0, rho1 | map, vr | scalars, partials, bands
+    const size_t NN = (size_t)N * N;
+    const size_t ncx = NN * ((size_t)nRh + std::max(nRv, 1) + 1 + 2 * (size_t)nRr);
+    const size_t nint = 3 * (size_t)S.nK + S.vr.size();
+    const size_t ndbl = 16 + 2 * kMixBlocks + (size_t)N * std::max(Nk, 0);
+    const size_t bytes = ncx * sizeof(cx<double>) + ((nint * sizeof(int) + 255) / 256) * 256 + ndbl * sizeof(double);
+    LQM_HIP(hipSetDevice(c->device));
+    {
+        hipError_t e = hipMalloc(&S.mem, bytes);
+        if (e != hipSuccess) {
+            S.mem = nullptr;
+            return fail(LQM_ERR_DEVICE, std::string("Hartree-Fock state allocation: ") + hipGetErrorString(e));
+        }
+    }
+    char* p = static_cast<char*>(S.mem);
+    S.h = reinterpret_cast<cx<double>*>(p);
+    S.v = S.h + NN * nRh;
+    S.V0 = S.v + NN * std::max(nRv, 1);
+    S.rho0 = S.V0 + NN;
+    S.rho1 = S.rho0 + NN * nRr;
+    p += ncx * sizeof(cx<double>);
+    S.dmap = reinterpret_cast<int*>(p);
+    S.dvr = S.dmap + 3 * S.nK;
+    p += ((nint * sizeof(int) + 255) / 256) * 256;
+    S.scal = reinterpret_cast<double*>(p);
+    S.part = S.scal + 16;
+    S.bands = S.part + 2 * kMixBlocks;
+    hipStream_t s = c->stream;
+    LQM_HIP(hipMemcpyAsync(S.h, Th, sizeof(cx<double>) * NN * nRh, hipMemcpyHostToDevice, s));
+    if (nRv > 0) LQM_HIP(hipMemcpyAsync(S.v, Tv, sizeof(cx<double>) * NN * nRv, hipMemcpyHostToDevice, s));
+    LQM_HIP(hipMemcpyAsync(S.rho0, rho, sizeof(cx<double>) * NN * nRr, hipMemcpyHostToDevice, s));
+    LQM_HIP(hipMemcpyAsync(S.dmap, S.map.data(), sizeof(int) * S.map.size(), hipMemcpyHostToDevice, s));
+    LQM_HIP(hipMemcpyAsync(S.dvr, S.vr.data(), sizeof(int) * S.vr.size(), hipMemcpyHostToDevice, s));
+    LQM_HIP(hipMemsetAsync(S.scal, 0, sizeof(double) * 16, s));
+    hipLaunchKernelGGL(lqm::k_blocksum, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, s, (int)NN,
+                       nRv, S.v, S.V0);
+    LQM_HIP(hipGetLastError());
+    if ((st = model_init(c, N, d, S.nK, S.RK.data(), &S.hmf))) return st;
+    LQM_HIP(hipMemcpyAsync(S.hmf->dR, S.RK.data(), sizeof(int) * S.RK.size(), hipMemcpyHostToDevice, s));
+    return LQM_OK;
+}
+
+// H_MF[rho] into the mean-field model, eps_H / eps_F (+ imaginary parts) into scal[0..3].
+lqm_status hf_apply(HFState& S, const cx<double>* rho) {
+    lqm_context* c = S.c;
+    const size_t tot = (size_t)S.N * S.N * S.nK;
+    StageScope sc(c, ST_HF);
+    hipLaunchKernelGGL(lqm::k_hf_build, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+                       S.N, S.nK, S.dmap, S.qz, S.rz, S.flags, S.h, S.v, S.V0, rho, S.hmf->dT);
+    hipLaunchKernelGGL(lqm::k_hf_energy, dim3(1), dim3(256), 0, c->stream, S.N, S.nRv, S.dvr, S.rz,
+                       S.flags, S.v, S.V0, rho, S.scal);
+    LQM_HIP(hipGetLastError());
+    return LQM_OK;
+}
+
+lqm_status hf_check_energy(const HFState& S, const double* sc) {
+    const double tol = std::sqrt(2.220446049250313e-16) * std::max(1, S.N);
+    if (!std::isfinite(sc[0]) || !std::isfinite(sc[1]))
+        return fail(LQM_ERR_NONFINITE, "non-finite Hartree/Fock energy");
+    if (std::fabs(sc[2]) > tol || std::fabs(sc[3]) > tol)
+        return fail(LQM_ERR_ARG, "complex-valued Hartree/Fock energy: rho is not Hermitian");
+    return LQM_OK;
+}
+
+lqm_status hf_outputs(HFState& S, int* nK_out, int32_t* RK_out, double* hmf_out) {
+    if (nK_out) *nK_out = S.nK;
+    if (RK_out) std::memcpy(RK_out, S.RK.data(), sizeof(int32_t) * S.RK.size());
+    if (hmf_out)
+        return d2h(S.c, hmf_out, S.hmf->dT, sizeof(cx<double>) * (size_t)S.N * S.N * S.nK);
     return LQM_OK;
 }
 
@@ -1196,6 +1298,102 @@ lqm_status lqm_densitymatrix(lqm_model* m, int Nk, const double* ks, const doubl
     if ((st = d2h(c, &h, deb, sizeof(double)))) return st;
     if (eband) *eband = h;
     return LQM_OK;
+}
+
+lqm_status lqm_hartreefock_meanfield(lqm_context* ctx, int N, int d, int nRh, const int32_t* Rh,
+                                     const double* Th, int nRv, const int32_t* Rv, const double* Tv,
+                                     int nRr, const int32_t* Rr, const double* rho, int flags,
+                                     int* nK_out, int32_t* RK_out, double* hmf_out,
+                                     double* energies_out) {
+    HFState S;
+    lqm_status st = hf_setup(S, ctx, N, d, nRh, Rh, Th, nRv, Rv, Tv, nRr, Rr, rho, flags, 0);
+    if (st) return st;
+    if ((st = hf_apply(S, S.rho0))) return st;
+    double sc[4];
+    if ((st = d2h(ctx, sc, S.scal, sizeof(sc)))) return st;
+    if ((st = hf_check_energy(S, sc))) return st;
+    if (energies_out) {
+        energies_out[0] = sc[0];
+        energies_out[1] = sc[1];
+    }
+    return hf_outputs(S, nK_out, RK_out, hmf_out);
+}
+
+lqm_status lqm_hartreefock_scf(lqm_context* ctx, int N, int d, int nRh, const int32_t* Rh,
+                               const double* Th, int nRv, const int32_t* Rv, const double* Tv,
+                               int nRr, const int32_t* Rr, double* rho, int Nk, const double* ks,
+                               const double* kweights, double filling, double T, double beta,
+                               double tol, int max_iter, int flags, double* result, int* converged,
+                               double* history, int* nK_out, int32_t* RK_out, double* hmf_out) {
+    if (Nk <= 0 || !ks) return fail(LQM_ERR_ARG, "the SCF needs a non-empty k-grid");
+    if (max_iter < 1) return fail(LQM_ERR_ARG, "max_iter must be positive");
+    if (!(T >= 0.0)) return fail(LQM_ERR_ARG, "negative temperature");
+    if (!std::isfinite(beta) || !std::isfinite(tol)) return fail(LQM_ERR_ARG, "non-finite beta or tol");
+    if (!result || !converged) return fail(LQM_ERR_ARG, "null result or converged pointer");
+    HFState S;
+    lqm_status st = hf_setup(S, ctx, N, d, nRh, Rh, Th, nRv, Rv, Tv, nRr, Rr, rho, flags, Nk);
+    if (st) return st;
+    lqm_context* c = ctx;
+    void *dks, *dkw = nullptr;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    const double* ksd = static_cast<const double*>(dks);
+    const double* kwd = static_cast<const double*>(dkw);
+    const size_t nrho = (size_t)N * N * nRr;
+    double sc[8] = {};
+    double mu = 0.0, eband = 0.0, residual = INFINITY;
+    int it = 0;
+    bool conv = false;
+    while (it < max_iter) {
+        ++it;
+        // f!(x1, x0): H_MF[rho0]; mu; rho1 and eps_band  (selfconsistent.jl:281-295, :374-381)
+        if ((st = hf_apply(S, S.rho0))) return st;
+        if ((st = run_bands(S.hmf, Nk, ksd, S.bands, nullptr))) return st;
+        if ((st = mu_from_bands(c, N, Nk, S.bands, kwd, filling, T, &mu))) return st;
+        if ((st = run_rho(S.hmf, Nk, ksd, kwd, mu, T, nRr, Rr, reinterpret_cast<double*>(S.rho1),
+                          S.scal + 4, nullptr)))
+            return st;
+        {
+            StageScope scm(c, ST_MIX);
+            hipLaunchKernelGGL(lqm::k_mix, dim3(kMixBlocks), dim3(256), 0, c->stream, nrho, beta, S.rho1,
+                               S.rho0, S.part);
+            hipLaunchKernelGGL(lqm::k_fold2, dim3(1), dim3(256), 0, c->stream, kMixBlocks, S.part,
+                               S.scal + 5);
+        }
+        LQM_HIP(hipGetLastError());
+        if ((st = d2h(c, sc, S.scal, sizeof(sc)))) return st;
+        if ((st = hf_check_energy(S, sc))) return st;
+        eband = sc[4];
+        const double E = eband - (sc[0] + sc[1]);
+        const double dn = std::sqrt(sc[5]), xn = std::sqrt(sc[6]);
+        residual = xn > 2.220446049250313e-16 ? dn / xn : dn;
+        if (history) {
+            history[3 * (it - 1)] = E;
+            history[3 * (it - 1) + 1] = residual;
+            history[3 * (it - 1) + 2] = mu;
+        }
+        if (!std::isfinite(residual)) return fail(LQM_ERR_NONFINITE, "non-finite SCF residual");
+        if (residual < tol) {
+            conv = true;
+            break;
+        }
+    }
+    // refresh H_MF and the energies at the final rho (selfconsistent.jl:319-328)
+    if ((st = hf_apply(S, S.rho0))) return st;
+    if ((st = d2h(c, sc, S.scal, sizeof(sc)))) return st;
+    if ((st = hf_check_energy(S, sc))) return st;
+    const double dc = sc[0] + sc[1];
+    result[0] = eband - dc;
+    result[1] = mu;
+    result[2] = eband;
+    result[3] = sc[0];
+    result[4] = sc[1];
+    result[5] = eband - 2.0 * dc;
+    result[6] = residual;
+    result[7] = (double)it;
+    *converged = conv ? 1 : 0;
+    if ((st = d2h(c, rho, S.rho0, sizeof(cx<double>) * nrho))) return st;
+    return hf_outputs(S, nK_out, RK_out, hmf_out);
 }
 
 }  // extern "C"

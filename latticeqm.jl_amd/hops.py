@@ -120,6 +120,18 @@ class Hops:
                 return False
         return True
 
+    def hermitianize(self) -> "Hops":
+        """In place: t_R <- (t_R + t_{-R}^H)/2, t_{-R} <- t_R^H; a missing partner is
+        added as the adjoint (types.jl:174-188)."""
+        for R in list(self.data.keys()):
+            mR = tuple(-x for x in R)
+            if mR in self.data:
+                self.data[R] = (self.data[R] + self.data[mR].conj().T) / 2
+                self.data[mR] = self.data[R].conj().T.copy()
+            else:
+                self.data[mR] = self.data[R].conj().T.copy()
+        return self
+
 
 def regulargrid(nk: int = 100, dim: int = 2) -> np.ndarray:
     """Gamma-centred unshifted grid, first index fastest (Structure/grid.jl:42-54).

@@ -1,0 +1,180 @@
+"""Lattice geometry and real-space hopping generators (host set-up, run once per model).
+
+Restates the parts of LatticeQM's ``Structure`` and ``Operators`` modules that build
+the exact ``Hops`` the band path is fed:
+
+* ``Lattice``: Bravais basis A (3 x d, spatial rows padded to 3) and cartesian orbital
+  positions (3 x n) plus extra labels such as ``sublattice`` (Structure/Lattices/type.jl);
+* ``honeycomb`` (Structure/Geometries/2d.jl:50-52, :88-94);
+* ``getneighborcells`` (Structure/Lattices/neighbors.jl:56-150): cells up to the k-th
+  shell found in a Lagrange-reduced basis, half-space representatives;
+* ``gethops`` (Operators/gethops.jl:33-140): block (i, j) of key L is
+  t(r_i + A L, r_j), entries below sqrt(eps) dropped, the -L block is the adjoint,
+  all-zero blocks removed (trim!, TightBinding/types.jl:278-294);
+* ``nearestneighbor`` (Operators/nearestneighbor.jl:22-62), ``t_graphene`` /
+  ``graphene`` (Operators/graphene.jl:17-28, :228-273), ``gethubbard``
+  (Operators/interactions.jl:8-28).
+"""
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .hops import Hops
+from .models import addspin
+
+DEFAULT_PRECISION = float(np.sqrt(np.finfo(float).eps))
+
+_A_TRI = np.array([[np.cos(np.pi / 6), np.cos(np.pi / 6)],
+                   [-np.sin(np.pi / 6), np.sin(np.pi / 6)],
+                   [0.0, 0.0]])
+
+
+@dataclass
+class Lattice:
+    """A: 3 x d lattice vectors (columns); positions: 3 x n cartesian orbital positions."""
+    A: np.ndarray
+    positions: np.ndarray
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def d(self) -> int:
+        return self.A.shape[1]
+
+    @property
+    def n(self) -> int:
+        return self.positions.shape[1]
+
+    def label(self, name: str) -> np.ndarray:
+        return np.asarray(self.extra[name])
+
+
+def honeycomb(a: float = 1.0) -> Lattice:
+    """Two-site honeycomb, nearest-neighbour distance a: A = a sqrt(3) A_tri, sites at
+    0 and (a1 + a2)/3, sublattice labels 0 (A) and 1 (B)."""
+    A = a * np.sqrt(3.0) * _A_TRI
+    frac = np.array([[0.0, 1.0 / 3.0], [0.0, 1.0 / 3.0]])
+    return Lattice(A, A @ frac, {"sublattice": np.array([0, 1])})
+
+
+def _lattice_reduce(A: np.ndarray):
+    """Lagrange reduction of a 2D basis; returns (reduced basis, integer transform U)."""
+    d = A.shape[1]
+    U = np.eye(d, dtype=np.int64)
+    if d != 2:
+        return A.astype(float), U
+    b1, b2 = A[:, 0].astype(float), A[:, 1].astype(float)
+    for _ in range(1000):
+        if np.linalg.norm(b1) > np.linalg.norm(b2):
+            b1, b2 = b2, b1
+            U = U[:, ::-1].copy()
+        m = int(np.round(np.dot(b1, b2) / np.dot(b1, b1)))
+        if m == 0:
+            break
+        b2 = b2 - m * b1
+        U[:, 1] -= m * U[:, 0]
+    return np.stack([b1, b2], axis=1), U
+
+
+def getneighborcells(A: np.ndarray, k: int = 1, halfspace: bool = True, innerpoints: bool = False,
+                     excludeorigin: bool = True):
+    """Integer cell offsets of the k-th neighbour shell (all shells up to k with
+    innerpoints), one of each +-pair when halfspace."""
+    d = A.shape[1]
+    Ared, U = _lattice_reduce(A)
+    n = int(np.ceil(np.sqrt(3.0) * (k + 1)))
+    IJ = np.array(list(itertools.product(range(-n, n + 1), repeat=d))).T  # first index fastest?
+    # Iterators.product varies the first coordinate fastest
+    IJ = IJ[::-1, :] if d > 1 else IJ
+    D = np.round(np.linalg.norm(Ared @ IJ, axis=0), 9)
+    dist = np.unique(D)
+    sel = D <= dist[k] if innerpoints else D == dist[k]
+    cells = [tuple(int(x) for x in U @ IJ[:, c]) for c in np.nonzero(sel)[0]]
+    if halfspace:
+        seen, keep = set(), []
+        for v in cells:
+            if all(x == 0 for x in v):
+                if not excludeorigin:
+                    keep.append(v)
+            elif v not in seen and tuple(-x for x in v) not in seen:
+                keep.append(v)
+                seen.add(v)
+        return keep
+    if excludeorigin:
+        cells = [v for v in cells if any(cells_x != 0 for cells_x in v)]
+    return cells
+
+
+def _hoppingblock(lat: Lattice, L, t, precision: float) -> np.ndarray:
+    shift = lat.A @ np.asarray(L, dtype=float)
+    R = lat.positions
+    M = np.zeros((lat.n, lat.n), dtype=np.complex128)
+    for i in range(lat.n):
+        ri = R[:, i] + shift
+        for j in range(lat.n):
+            val = t(ri, R[:, j])
+            if abs(val) >= precision:
+                M[i, j] = val
+    return M
+
+
+def gethops(lat: Lattice, t, neighbors=None, cellrange: int = 2,
+            precision: float = DEFAULT_PRECISION) -> Hops:
+    """Hops of the pair function t over the neighbour cells (all shells up to ``cellrange``,
+    half space + origin) or over an explicit list ``neighbors``."""
+    if neighbors is None:
+        neighbors = getneighborcells(lat.A, cellrange, halfspace=True, innerpoints=True,
+                                     excludeorigin=False)
+    h = Hops()
+    for L in neighbors:
+        L = tuple(int(x) for x in L)
+        M = _hoppingblock(lat, L, t, precision)
+        h[L] = M
+        if any(x != 0 for x in L):
+            h[tuple(-x for x in L)] = M.conj().T
+    for L in list(h.keys()):  # trim!
+        if not np.any(h[L] != 0):
+            del h.data[L]
+    return h
+
+
+def nearestneighbor(lat: Lattice, t0: float = -1.0, a: float = 1.0, halfwidth: float = 0.01,
+                    cellrange: int = 2) -> Hops:
+    """t0 between orbitals at distance within (a - halfwidth, a + halfwidth)."""
+    def t(r1, r2):
+        r = np.linalg.norm(r1[:3] - r2[:3])
+        return t0 if (a + halfwidth > r > a - halfwidth) else 0.0
+    return gethops(lat, t, cellrange=cellrange)
+
+
+def t_graphene(r1, r2, tmin: float = 1e-5, tz: float = 0.46, t0: float = 1.0, l_inter: float = 0.125,
+               l_intra: float = 0.08, z: float = 3.0, a: float = 1.0, dmin: float = 0.1,
+               dmax: float = 5.0) -> float:
+    """-t0 (1 - chi) e^{-|D - a|/l_intra} - tz chi e^{-|D - z|/l_inter}, chi = dz^2/D^2,
+    zero outside [dmin, dmax] or below tmin in magnitude."""
+    dr = np.asarray(r1[:3]) - np.asarray(r2[:3])
+    D = float(np.sqrt(np.sum(dr * dr)))
+    if dmax < D or D < dmin:
+        return 0.0
+    chi = dr[2] ** 2 / D ** 2
+    v = -t0 * (1 - chi) * np.exp(-abs(D - a) / l_intra) - tz * chi * np.exp(-abs(D - z) / l_inter)
+    return 0.0 if abs(v) < tmin else float(v)
+
+
+def graphene(lat: Lattice, mode: str = "nospin", cellrange: int = 2, **kw) -> Hops:
+    """Graphene Hamiltonian with the t_graphene hopping (keyword overrides in kw)."""
+    h = gethops(lat, lambda r1, r2: t_graphene(r1, r2, **kw), cellrange=cellrange, precision=0.0)
+    return addspin(h, "spinhalf") if mode == "spinhalf" else h
+
+
+def gethubbard(lat: Lattice, neighbors=((0, 0),), mode: str = "nospin", a: float = 0.5,
+               U: float = 1.0) -> Hops:
+    """U heaviside(a - |r_i + A L - r_j|) on the listed cells, then addspin(mode)."""
+    def t(r1, r2):
+        x = a - np.linalg.norm(r1[:3] - r2[:3])
+        return U * (0.0 if x < 0 else (1.0 if x > 0 else 0.5))
+    v = gethops(lat, t, neighbors=[tuple(L) for L in neighbors])
+    spin = {"σx": "sigmax", "σ0": "spinhalf"}.get(mode, mode)
+    return addspin(v, spin)

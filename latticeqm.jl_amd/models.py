@@ -95,3 +95,14 @@ def random_tb(N: int, rmax: int = 2, seed: int = 2, dim: int = 2) -> Hops:
             mR = tuple(-x for x in R)
             h[R] = blocks[mR].conj().T
     return h
+
+
+def honeycomb_hubbard(U: float = 3.0, t: float = -1.0):
+    """Spinful honeycomb Hubbard model: h = nearest-neighbour t on the three bonds (x) sigma_0
+    (N = 4: sites A, B x spin, spin fast), v = U kron(I_2, sigma_x) on the zero key
+    (gethubbard(lat; mode=:sigmax, a=0.5, U), Operators/interactions.jl:18-28).  Returns
+    (h, v, sublattice labels of the two sites)."""
+    h = addspin(honeycomb_nn(t), "spinhalf")
+    sx = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+    v = Hops({(0, 0): U * np.kron(np.eye(2), sx)})
+    return h, v, np.array([0, 1])

@@ -207,16 +207,29 @@ def getfilling(bands, mu, T=0.0, kweights=None):
     return float(np.sum(np.asarray(kweights)[None, :] * f) / f.shape[0])
 
 
+def _middle(a, b):
+    """Roots.jl's bracketing midpoint for Float64: 0 across a sign change, else the
+    midpoint of the IEEE bit patterns of |a| and |b| with the sign of a + b, so the
+    bisection reaches adjacent doubles in at most 64 steps."""
+    if (a < 0) != (b < 0) and a != 0 and b != 0:
+        return 0.0
+    ia = int(np.array(abs(a), dtype=np.float64).view(np.uint64))
+    ib = int(np.array(abs(b), dtype=np.float64).view(np.uint64))
+    mid = float(np.array((ia + ib) >> 1, dtype=np.uint64).view(np.float64))
+    return -mid if (a + b) < 0 else mid
+
+
 def _bisect_doubles(fn, a, b):
-    """Bracketing bisection down to adjacent doubles (fn(a) < 0 < fn(b))."""
+    """Bracketing bisection with Roots.jl's midpoint down to adjacent doubles; a midpoint
+    with fn == 0 is returned at once, else the end with the smaller |fn|."""
     fa, fb = fn(a), fn(b)
     if fa == 0:
         return a
     if fb == 0:
         return b
-    for _ in range(2000):
-        m = 0.5 * (a + b)
-        if m <= a or m >= b:
+    for _ in range(200):
+        m = _middle(a, b)
+        if not (a < m < b):
             break
         fm = fn(m)
         if fm == 0:

@@ -168,3 +168,26 @@ def test_random_model_hermitian(pkg):
     assert len(h) == 25 and h.ishermitian(1e-15)
     k = np.array([0.13, 0.71])
     assert np.max(np.abs(h(k) - h(k).conj().T)) < 1e-14
+
+
+def test_roots_midpoint_and_bisection(oracle):
+    # Roots.jl's Float64 midpoint: bit-pattern midpoint, 0 across a sign change
+    assert oracle._middle(1.0, 2.0) == 1.5
+    assert oracle._middle(1.0, 4.0) == 2.0
+    assert oracle._middle(-4.0, -1.0) == -2.0
+    assert oracle._middle(-1.0, 2.0) == 0.0
+    # reaches adjacent doubles around an irrational root
+    r = oracle._bisect_doubles(lambda x: x * x - 2.0, 0.0, 2.0)
+    assert abs(r - np.sqrt(2.0)) <= np.spacing(np.sqrt(2.0))
+    # on a plateau of exact zeros the first midpoint inside it is returned
+    f = lambda x: 0.0 if 0.3 <= x <= 0.9 else (x - 0.6)
+    r = oracle._bisect_doubles(f, -1.0, 2.0)
+    assert 0.3 <= r <= 0.9
+    # bit midpoints from 0 walk through the exponents: the first point tried above 0.3
+    a, b = 0.0, 2.0
+    while True:
+        m = oracle._middle(a, b)
+        if 0.3 <= m <= 0.9:
+            break
+        a, b = (m, b) if m < 0.3 else (a, m)
+    assert r == m
