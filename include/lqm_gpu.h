@@ -159,6 +159,33 @@ lqm_status lqm_densitymatrix_dev(lqm_model *m, int Nk, const double *ks_dev,
                                  const int32_t *Rr, double *rho_dev, double *eband_dev,
                                  double *bands_dev);
 
+/* ---- spectral adjoints: grand potential and weighted density matrices -------------- */
+/* No reference counterpart (the reference has no grand potential or gradients); these are
+ * the pullbacks a ChainRulesCore rrule for the band path binds.
+ * Grand potential Omega = sum_k w_k sum_n omega(e_nk), omega(e) = -T ln(1 + e^{-(e - mu)/T})
+ * (T > 0) or min(e - mu, 0) (T = 0); w_k = 1/Nk when kweights == NULL.  grad (may be NULL)
+ * receives dOmega/dt_R for every model key R, in model key order [N x N x nR]:
+ * [grad_R]_ij = dOmega/d(t_R)_ij = rho(-R)_ji (Hellmann-Feynman: dOmega = sum_k w_k
+ * tr(f(H_k) dH_k)), with rho(-R) the density matrix of getdensitymatrix! at (mu, T).
+ * For a perturbation that keeps H Hermitian (t_R += h D, t_-R += h D^H) the directional
+ * derivative is 2 Re tr(rho(-R) D) (R != 0) or tr(rho(0) D) (R = 0, D Hermitian). */
+lqm_status lqm_grandpotential(lqm_model *m, int Nk, const double *ks, const double *kweights,
+                              double mu, double T, double *omega, double *grad);
+/* Device variant: ks_dev, kweights_dev, grad_dev on the device; *omega a host scalar. */
+lqm_status lqm_grandpotential_dev(lqm_model *m, int Nk, const double *ks_dev,
+                                  const double *kweights_dev, double mu, double T, double *omega,
+                                  double *grad_dev);
+/* Weighted density matrix rho_g(R) = sum_k w_k e^{-2 pi i k.R} U_k diag(g_k) U_k^H on the
+ * keys Rr, where occ [N x Nk] holds g_k for the ascending eigenvalues at k_j.  The
+ * pullback of any spectral loss L = sum_k w_k sum_n l(e_nk) is dL/d(t_R)_ij =
+ * rho_{l'}(-R)_ji (occ = l'(bands)).  rho is overwritten. */
+lqm_status lqm_densitymatrix_occ(lqm_model *m, int Nk, const double *ks, const double *kweights,
+                                 const double *occ, int nRr, const int32_t *Rr, double *rho);
+/* Device variant (Rr stays a host array; bands_dev may be NULL). */
+lqm_status lqm_densitymatrix_occ_dev(lqm_model *m, int Nk, const double *ks_dev,
+                                     const double *kweights_dev, const double *occ_dev, int nRr,
+                                     const int32_t *Rr, double *rho_dev, double *bands_dev);
+
 /* ---- Hartree-Fock mean field and self-consistency ----------------------------------- */
 /* Flags of the HF functional: which channels HartreeFock(h, v; hartree, fock) keeps
  * (Meanfield/hartreefock.jl:158). */

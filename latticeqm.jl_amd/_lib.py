@@ -57,6 +57,10 @@ _PROTOS = {
     "lqm_chemicalpotential_from_bands_dev": [P, I, I, P, P, D, D, P],
     "lqm_densitymatrix": [P, I, P, P, D, D, I, P, P, P],
     "lqm_densitymatrix_dev": [P, I, P, P, D, D, I, P, P, P, P],
+    "lqm_grandpotential": [P, I, P, P, D, D, P, P],
+    "lqm_grandpotential_dev": [P, I, P, P, D, D, P, P],
+    "lqm_densitymatrix_occ": [P, I, P, P, P, I, P, P],
+    "lqm_densitymatrix_occ_dev": [P, I, P, P, P, I, P, P, P],
     "lqm_hartreefock_meanfield": [P, I, I, I, P, P, I, P, P, I, P, P, I, P, P, P, P],
     "lqm_hartreefock_scf": [P, I, I, I, P, P, I, P, P, I, P, P, I, P, P, D, D, D, D, I, I, P, P, P,
                             P, P, P],

@@ -298,3 +298,72 @@ def setfilling(H: Hops, filling: float, nk: int = 100, T: float = 0.0, executor=
     mu = chemicalpotential(H, regulargrid(nk=nk, dim=H.d), filling, T=T, executor=executor)
     addchemicalpotential(H, -mu)
     return mu
+
+
+# ---- spectral adjoints: grand potential and band-loss gradients --------------------
+def grandpotential(H, ks, mu: float = 0.0, T: float = 0.05, kweights=None, executor=None,
+                   gradient: bool = False):
+    """Omega = sum_k w_k sum_n -T ln(1 + exp(-(e_nk - mu)/T)) (min(e - mu, 0) at T = 0).
+
+    With ``gradient=True`` returns ``(Omega, dOmega)``, dOmega a Hops on the keys of H with
+    dOmega[R][i, j] = dOmega/d(t_R)_ij = rho(-R)[j, i] (Hellmann-Feynman)."""
+    m = _model(H, executor)
+    k = _kpts(ks, m.d)
+    w = _weights(ks, kweights)
+    om = _c.c_double()
+    g = np.zeros((m.N, m.N, m.nR), dtype=np.complex128, order="F") if gradient else None
+    _lib.call("lqm_grandpotential", m.handle, k.shape[1], _vp(k), _vp(w), float(mu), float(T),
+              _c.byref(om), _vp(g))
+    if not gradient:
+        return om.value
+    return om.value, Hops({key: np.array(g[:, :, r]) for r, key in enumerate(m.keys)})
+
+
+def grandpotential_rrule(H: Hops, ks, mu: float = 0.0, T: float = 0.05, kweights=None, executor=None):
+    """(Omega, pullback): pullback(dOmega_bar) returns the cotangent of every block t_R,
+    dOmega_bar * conj(dOmega/dt_R) = dOmega_bar * rho(R) -- the gradient of the real Omega
+    in the real inner product <X, D> = Re sum conj(X) D, the convention of ChainRulesCore
+    for complex inputs."""
+    om, g = grandpotential(H, ks, mu, T, kweights, executor, gradient=True)
+
+    def pullback(dbar: float = 1.0) -> Hops:
+        return Hops({R: dbar * np.conj(b) for R, b in g.items()})
+
+    return om, pullback
+
+
+def densitymatrix_occ(rhos: Hops, H, ks, occ, kweights=None, executor=None) -> Hops:
+    """Overwrite rhos with sum_k w_k e^{-2 pi i k.R} U_k diag(occ[:, k]) U_k^H, occ (N, Nk)
+    weighting the ascending eigenvalues."""
+    m = _model(H, executor)
+    k = _kpts(ks, m.d)
+    w = _weights(ks, kweights)
+    g = np.asfortranarray(np.asarray(occ, dtype=np.float64))
+    if g.shape != (m.N, k.shape[1]):
+        raise ValueError(f"occupations must have shape (N, Nk) = {(m.N, k.shape[1])}")
+    keys = list(rhos.keys())
+    nRr = len(keys)
+    Rr = np.asfortranarray(np.array(keys, dtype=np.int32).T.reshape(m.d, nRr))
+    rho = np.zeros((m.N, m.N, max(nRr, 1)), dtype=np.complex128, order="F")
+    _lib.call("lqm_densitymatrix_occ", m.handle, k.shape[1], _vp(k), _vp(w), _vp(g), nRr, _vp(Rr),
+              _vp(rho))
+    for r, key in enumerate(keys):
+        rhos.data[key] = np.array(rho[:, :, r])
+    return rhos
+
+
+def bandloss(H: Hops, ks, target, kweights=None, executor=None):
+    """L = sum_k w_k sum_n (e_nk - target_nk)^2 and its cotangent per block t_R,
+    rho_g(R) with g = 2 (e - target): two device passes (eigenvalues, then the weighted
+    density matrix on the keys of H)."""
+    ex = to_executor(executor or "gpu")
+    m = _model(H, ex)
+    bands, _ = bandmatrix(m, ks)
+    Nk = bands.shape[1]
+    w = _weights(ks, kweights)
+    wk = np.full(Nk, 1.0 / Nk) if w is None else w
+    r = bands - np.asarray(target, dtype=np.float64)
+    L = float(np.sum(wk[None, :] * r * r))
+    grad = densitymatrix_occ(Hops({R: np.zeros((m.N, m.N), complex) for R in m.keys}), m, ks, 2.0 * r,
+                             kweights=kweights)
+    return L, grad

@@ -224,4 +224,40 @@ __global__ __launch_bounds__(256) void k_fouriersum(int N, int d, int nR, const 
     }
 }
 
+// Grand-potential partials: part[b] = block b's share of sum_k w_k sum_n omega(e_nk),
+// omega(e) = -T ln(1 + e^{-(e - mu)/T}) in the overflow-free form
+// -T (max(-x, 0) + log1p(e^{-|x|})), x = (e - mu)/T; at T = 0 omega = min(e - mu, 0).
+__global__ __launch_bounds__(256) void k_omega(int N, int Ne, const double* __restrict__ bands,
+                                               const double* __restrict__ kw, double mu, double T,
+                                               double* __restrict__ part) {
+    __shared__ double sred[4];
+    double a = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < Ne; i += gridDim.x * 256) {
+        const double e = bands[i] - mu;
+        double w;
+        if (T > 0.0) {
+            const double x = e / T;
+            w = -T * (fmax(-x, 0.0) + log1p(exp(-fabs(x))));
+        } else {
+            w = fmin(e, 0.0);
+        }
+        a += kw ? kw[i / N] * w : w;
+    }
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x / 64] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sred[0] + sred[1] + sred[2] + sred[3];
+}
+
+// out[b] = transpose(in[b]) for nb stacked N x N complex blocks.
+__global__ __launch_bounds__(256) void k_block_transpose(int N, int nb, const cx<double>* __restrict__ in,
+                                                         cx<double>* __restrict__ out) {
+    const size_t NN = (size_t)N * N;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= NN * nb) return;
+    const size_t b = idx / NN, e = idx - b * NN;
+    const size_t i = e % N, j = e / N;
+    out[b * NN + i * N + j] = in[idx];
+}
+
 }  // namespace lqm

@@ -291,7 +291,8 @@ __device__ __forceinline__ double fermidirac(double e, double mu, double T) {
 //      read and one write per rotation);
 //   2. V = H_0 H_1 ... H_{N-2} Z, reflectors applied from the last one;
 //   3. MODE 0: write evecs (ascending order); MODE 1: rho_k = V diag(f) V^H with
-//      f = fermidirac(e - mu; T), written as FP64, and eb[kk] = w_k sum_n e_n f_n.
+//      f = fermidirac(e - mu; T) (or f_n = occ[kk*N + n] for the n-th smallest
+//      eigenvalue when occ is given), written as FP64, and eb[kk] = w_k sum_n e_n f_n.
 // ---------------------------------------------------------------------------------
 constexpr int kRotChunk = 512;  // default rotation-staging capacity (>= longest sweep)
 constexpr int kHdrChunk = 256;  // sweep headers staged per LDS load
@@ -430,8 +431,9 @@ __global__ __launch_bounds__(kTPB) void k_backtransform(
     int N, int Nkc, const cx<R>* __restrict__ refl, const cx<R>* __restrict__ tau,
     const R* __restrict__ rot, const int* __restrict__ sweeps, const int* __restrict__ nsweeps,
     int cap_rot, int cap_sw, int rch, const R* __restrict__ evals, const int* __restrict__ perm,
-    const double* __restrict__ kw, double wuni, double mu, double Temp, R* __restrict__ gV,
-    const R* __restrict__ zin, cx<double>* __restrict__ out, double* __restrict__ eb) {
+    const double* __restrict__ kw, double wuni, double mu, double Temp,
+    const double* __restrict__ occ, R* __restrict__ gV, const R* __restrict__ zin,
+    cx<double>* __restrict__ out, double* __restrict__ eb) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int kk = blockIdx.x;
     const int tid = threadIdx.x;
@@ -582,7 +584,7 @@ __global__ __launch_bounds__(kTPB) void k_backtransform(
         double ebp = 0.0;
         for (int n = tid; n < N; n += kTPB) {
             double en = (double)ev[n];
-            double f = fermidirac(en, mu, Temp);
+            double f = occ ? occ[(size_t)kk * N + n] : fermidirac(en, mu, Temp);
             sf[min(max(pm[n], 0), N - 1)] = f;
             ebp += en * f;
         }

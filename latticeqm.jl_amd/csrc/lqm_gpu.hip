@@ -41,7 +41,7 @@ lqm_status fail(lqm_status st, const std::string& msg) {
 enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
-    W_S, W_G, W_LAM, W_STAT, W_NSLOTS
+    W_S, W_G, W_LAM, W_STAT, W_GRADT, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -85,6 +85,7 @@ struct lqm_context {
     int* d_status = nullptr;
     int refine_max = 3;      // Ogita-Aishima sweeps allowed per call (mixed precision)
     double report[4] = {};   // last mixed call: sweeps, max ||R||_F, max delta, near-cluster k
+    const double* occ = nullptr;  // given occupations of the chunk in flight (device), or null
 };
 
 struct lqm_model {
@@ -303,14 +304,14 @@ void launch_kc(lqm_context* c, const Plan& P, int N, int Kc, const cx<R>* refl, 
         auto kC = lqm::k_backtransform<R, true, MODE, ZIN>;
         allow_lds(kC, P.shC);
         hipLaunchKernelGGL(kC, dim3(Kc), dim3(lqm::kTPB), P.shC, c->stream, N, Kc, refl, tau, rot, sw,
-                           nsw, P.cap_rot, P.cap_sw, P.rch, evals, perm, kw, wuni, mu, T, gV, zin, out,
-                           eb);
+                           nsw, P.cap_rot, P.cap_sw, P.rch, evals, perm, kw, wuni, mu, T, c->occ, gV, zin,
+                           out, eb);
     } else {
         auto kC = lqm::k_backtransform<R, false, MODE, ZIN>;
         allow_lds(kC, P.shC);
         hipLaunchKernelGGL(kC, dim3(Kc), dim3(lqm::kTPB), P.shC, c->stream, N, Kc, refl, tau, rot, sw,
-                           nsw, P.cap_rot, P.cap_sw, P.rch, evals, perm, kw, wuni, mu, T, gV, zin, out,
-                           eb);
+                           nsw, P.cap_rot, P.cap_sw, P.rch, evals, perm, kw, wuni, mu, T, c->occ, gV, zin,
+                           out, eb);
     }
 }
 
@@ -379,7 +380,7 @@ lqm_status launch_rho2(lqm_model* m, const Plan& P, int Kc, const double* evals,
         auto kM = lqm::k_zfz<double>;
         allow_lds(kM, P.shM);
         hipLaunchKernelGGL(kM, dim3(Kc), dim3(lqm::kTPB), P.shM, c->stream, N, Z, evals, perm, kw, wuni,
-                           mu, T, M, eb);
+                           mu, T, c->occ, M, eb);
     }
     LQM_HIP(hipGetLastError());
     {
@@ -432,7 +433,14 @@ lqm_status check_model(lqm_model* m) {
 
 // bands (+ evecs) over the whole grid, device pointers.
 lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
-                     int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands);
+                     int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands,
+                     const double* occ = nullptr);
+
+// Clears the context's occupation pointer when a pass ends, on every return path.
+struct OccScope {
+    lqm_context* c;
+    ~OccScope() { c->occ = nullptr; }
+};
 
 lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, double* evecs) {
     const int N = m->N;
@@ -459,8 +467,11 @@ lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, doub
     return check_status(m->ctx, "bandmatrix");
 }
 
+// occ (device, N x Nk, may be null): occupations of the ascending eigenvalues used in
+// place of fermidirac(e - mu; T) -- rho then is sum_k w_k e^{-2 pi i k.R} U diag(occ_k) U^H.
 lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
-                   int nRr, const int32_t* Rr, double* rho, double* eband, double* bands_out) {
+                   int nRr, const int32_t* Rr, double* rho, double* eband, double* bands_out,
+                   const double* occ = nullptr) {
     lqm_context* c = m->ctx;
     const int N = m->N, d = m->d;
     if (Nk <= 0) return fail(LQM_ERR_ARG, "Nk must be positive");
@@ -476,7 +487,7 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
         LQM_HIP(hipMemcpyAsync(dRr, Rr, sizeof(int) * nRr * d, hipMemcpyHostToDevice, c->stream));
     if (c->precision == LQM_PREC_FP32_REFINED)
         return run_mixed(m, Nk, ks, kw, mu, T, nRr, dRr, reinterpret_cast<cx<double>*>(rho), eband,
-                         bands_out);
+                         bands_out, occ);
     Plan P = make_plan(c, N, m->nR, Nk, true, true);
     const int RG = nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : 32;
     const int groups = std::max(1, (nRr + RG - 1) / RG);
@@ -485,7 +496,9 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
     cx<double>* part = nullptr;
     if ((st = wsT(c, W_PART, (size_t)groups * S * RG * NN, &part))) return st;
     const double wuni = 1.0 / (double)Nk;
+    OccScope occ_scope{c};
     for (int k0 = 0; k0 < Nk; k0 += P.Kc) {
+        c->occ = occ ? occ + (size_t)k0 * N : nullptr;
         const int Kc = std::min(P.Kc, Nk - k0);
         const double* ksc = ks + (size_t)k0 * d;
         const double* kwc = kw ? kw + k0 : nullptr;
@@ -540,7 +553,8 @@ void zgemm(lqm_context* c, int N, int Kc, const cx<double>* A, const cx<double>*
 // One pass over the grid.  Outputs (any may be null): sorted refined eigenvalues
 // (bands), rho on keys dRr (rho: accumulated into, zeroed by the caller) and eband.
 lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
-                     int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands) {
+                     int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands,
+                     const double* occ) {
     lqm_context* c = m->ctx;
     const int N = m->N, d = m->d;
     if (N > 512) return fail(LQM_ERR_UNSUPPORTED, "mixed precision path supports N <= 512");
@@ -643,8 +657,8 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
             if ((st = wsT(c, W_EB, (size_t)P.Kc, &eb))) return st;
             {
                 StageScope sc(c, ST_REDUCE);
-                hipLaunchKernelGGL(lqm::k_scale_occ, dim3(Kc), dim3(256), 0, c->stream, N, X, lam, kwc, wuni,
-                                   mu, T, G, eb);
+                hipLaunchKernelGGL(lqm::k_scale_occ, dim3(Kc), dim3(256), sizeof(double) * N, c->stream, N,
+                                   X, lam, kwc, wuni, mu, T, occ ? occ + (size_t)k0 * N : nullptr, G, eb);
                 hipLaunchKernelGGL(lqm::k_sum, dim3(1), dim3(256), 0, c->stream, Kc, eb, eband, 1);
             }
             zgemm<0, 1>(c, N, Kc, G, X, Sm, one, zero);  // rho_k = X f X^H
@@ -1394,6 +1408,101 @@ lqm_status lqm_hartreefock_scf(lqm_context* ctx, int N, int d, int nRh, const in
     *converged = conv ? 1 : 0;
     if ((st = d2h(c, rho, S.rho0, sizeof(cx<double>) * nrho))) return st;
     return hf_outputs(S, nK_out, RK_out, hmf_out);
+}
+
+lqm_status lqm_densitymatrix_occ_dev(lqm_model* m, int Nk, const double* ks_dev,
+                                     const double* kweights_dev, const double* occ_dev, int nRr,
+                                     const int32_t* Rr, double* rho_dev, double* bands_dev) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (!occ_dev) return fail(LQM_ERR_ARG, "null occupations");
+    double* eb;
+    if ((st = wsT(m->ctx, W_SMALL, 8, &eb))) return st;
+    return run_rho(m, Nk, ks_dev, kweights_dev, 0.0, 0.0, nRr, Rr, rho_dev, eb, bands_dev, occ_dev);
+}
+
+lqm_status lqm_densitymatrix_occ(lqm_model* m, int Nk, const double* ks, const double* kweights,
+                                 const double* occ, int nRr, const int32_t* Rr, double* rho) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0) return fail(LQM_ERR_ARG, "Nk must be positive");
+    if (!occ || (!rho && nRr > 0)) return fail(LQM_ERR_ARG, "null occupations or rho");
+    lqm_context* c = m->ctx;
+    const size_t NN = (size_t)m->N * m->N;
+    void *dks, *dkw = nullptr, *docc, *drho;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    if ((st = stage_in(c, W_OM, occ, sizeof(double) * m->N * Nk, &docc))) return st;
+    if ((st = ws(c, W_OUT, sizeof(cx<double>) * NN * std::max(nRr, 1), &drho))) return st;
+    if ((st = lqm_densitymatrix_occ_dev(m, Nk, (const double*)dks, (const double*)dkw,
+                                        (const double*)docc, nRr, Rr, (double*)drho, nullptr)))
+        return st;
+    if (nRr > 0) return d2h(c, rho, drho, sizeof(cx<double>) * NN * nRr);
+    return LQM_OK;
+}
+
+lqm_status lqm_grandpotential_dev(lqm_model* m, int Nk, const double* ks_dev,
+                                  const double* kweights_dev, double mu, double T, double* omega,
+                                  double* grad_dev) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0 || !ks_dev) return fail(LQM_ERR_ARG, "Nk must be positive");
+    if (!(T >= 0.0)) return fail(LQM_ERR_ARG, "negative temperature");
+    if (!omega) return fail(LQM_ERR_ARG, "null omega");
+    lqm_context* c = m->ctx;
+    const int N = m->N, nR = m->nR;
+    const size_t NN = (size_t)N * N;
+    // the adjoint is rho on the reflected keys, transposed block by block
+    std::vector<int32_t> negR(m->R.size());
+    for (size_t i = 0; i < negR.size(); ++i) negR[i] = -m->R[i];
+    cx<double>* tmp;
+    double *bands, *small;
+    if ((st = wsT(c, W_GRADT, grad_dev ? NN * nR : 1, &tmp))) return st;
+    if ((st = wsT(c, W_BANDS, (size_t)N * Nk, &bands))) return st;
+    const int Ne = N * Nk;
+    const int G = std::max(1, std::min(1024, (Ne + 255) / 256));
+    if ((st = wsT(c, W_SMALL, (size_t)G + 8, &small))) return st;
+    if (grad_dev) {
+        if ((st = run_rho(m, Nk, ks_dev, kweights_dev, mu, T, nR, negR.data(), reinterpret_cast<double*>(tmp),
+                          small + G + 1, bands)))
+            return st;
+    } else if ((st = run_bands(m, Nk, ks_dev, bands, nullptr))) {
+        return st;
+    }
+    {
+        StageScope sc(c, ST_REDUCE);
+        hipLaunchKernelGGL(lqm::k_omega, dim3(G), dim3(256), 0, c->stream, N, Ne, bands, kweights_dev, mu,
+                           T, small);
+        hipLaunchKernelGGL(lqm::k_sum, dim3(1), dim3(256), 0, c->stream, G, small, small + G, 0);
+        if (grad_dev) {
+            const size_t tot = NN * nR;
+            hipLaunchKernelGGL(lqm::k_block_transpose, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                               c->stream, N, nR, tmp, reinterpret_cast<cx<double>*>(grad_dev));
+        }
+    }
+    LQM_HIP(hipGetLastError());
+    double h = 0.0;
+    if ((st = d2h(c, &h, small + G, sizeof(double)))) return st;
+    *omega = kweights_dev ? h : h / (double)Nk;
+    return LQM_OK;
+}
+
+lqm_status lqm_grandpotential(lqm_model* m, int Nk, const double* ks, const double* kweights,
+                              double mu, double T, double* omega, double* grad) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0 || !ks) return fail(LQM_ERR_ARG, "Nk must be positive");
+    lqm_context* c = m->ctx;
+    const size_t NN = (size_t)m->N * m->N;
+    void *dks, *dkw = nullptr, *dg = nullptr;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    if (grad && (st = ws(c, W_OUT, sizeof(cx<double>) * NN * m->nR, &dg))) return st;
+    if ((st = lqm_grandpotential_dev(m, Nk, (const double*)dks, (const double*)dkw, mu, T, omega,
+                                     (double*)dg)))
+        return st;
+    if (grad) return d2h(c, grad, dg, sizeof(cx<double>) * NN * m->nR);
+    return LQM_OK;
 }
 
 }  // extern "C"

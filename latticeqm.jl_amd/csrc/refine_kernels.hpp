@@ -7,7 +7,8 @@
 //   k_oa_correct     lambda_i = s_ii / g_ii, delta = 2(||S - D|| + ||A|| ||R||),
 //                    E_ij = (s_ij + lambda_j r_ij)/(lambda_j - lambda_i) or r_ij / 2
 //                    with R = I - G, G = X^H X, S = X^H A X
-//   k_scale_occ      Y = X diag(f(lambda)), f = fermidirac(lambda - mu; T) in FP64,
+//   k_scale_occ      Y = X diag(f(lambda)), f = fermidirac(lambda - mu; T) in FP64
+//                    (or given occupations),
 //                    and eb[k] = w_k sum_n lambda_n f_n
 //   k_sort_evals     ascending copy of the refined eigenvalues
 #pragma once
@@ -367,22 +368,38 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
     (void)s_ncl;
 }
 
-// Y = X diag(f), f_n = fermidirac(lambda_n - mu; T); eb[kk] = w_k sum_n lambda_n f_n.
+// Y = X diag(f), f_n = fermidirac(lambda_n - mu; T) -- or, when occ is given, the
+// occupation occ[kk*N + r] of the rank-r eigenvalue (lambda is in X's column order,
+// ties ranked by column); eb[kk] = w_k sum_n lambda_n f_n.  Dynamic LDS: N doubles.
 __global__ __launch_bounds__(256) void k_scale_occ(int N, const cx<double>* __restrict__ X,
                                                    const double* __restrict__ lambda,
                                                    const double* __restrict__ kw, double wuni,
-                                                   double mu, double T, cx<double>* __restrict__ Y,
+                                                   double mu, double T,
+                                                   const double* __restrict__ occ,
+                                                   cx<double>* __restrict__ Y,
                                                    double* __restrict__ eb) {
-    __shared__ double sf[512];
+    extern __shared__ __align__(16) unsigned char smem[];
+    double* sf = reinterpret_cast<double*>(smem);
     __shared__ double sred[4];
     const int kk = blockIdx.x, tid = threadIdx.x;
     const double* lk = lambda + (size_t)kk * N;
     double e = 0.0;
     for (int n = tid; n < N; n += 256) {
-        const double x = lk[n] - mu;
-        const double f = T > 0.0 ? 1.0 / (exp(x / T) + 1.0) : (x < 0.0 ? 1.0 : (x > 0.0 ? 0.0 : 0.5));
-        if (n < 512) sf[n] = f;
-        e += lk[n] * f;
+        const double ln = lk[n];
+        double f;
+        if (occ) {
+            int rank = 0;
+            for (int j = 0; j < N; ++j) {
+                const double lj = lk[j];
+                rank += (lj < ln) || (lj == ln && j < n);
+            }
+            f = occ[(size_t)kk * N + min(rank, N - 1)];
+        } else {
+            const double x = ln - mu;
+            f = T > 0.0 ? 1.0 / (exp(x / T) + 1.0) : (x < 0.0 ? 1.0 : (x > 0.0 ? 0.0 : 0.5));
+        }
+        sf[n] = f;
+        e += ln * f;
     }
     e = wave_sum(e);
     if ((tid & 63) == 0) sred[tid / 64] = e;

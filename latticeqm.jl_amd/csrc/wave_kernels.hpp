@@ -241,7 +241,8 @@ __global__ __launch_bounds__(kWave) void k_hk_tridiag_w(
 }
 
 // ---------------------------------------------------------------------------------
-// M = Z diag(f) Z^T (real), f_c = fermidirac(e_c - mu; T) for Z column c, and the
+// M = Z diag(f) Z^T (real), f_c = fermidirac(e_c - mu; T) for Z column c (or the given
+// occupation occ[kk*N + n] of the n-th smallest eigenvalue), and the
 // band-energy term eb[kk] = w_k sum_n e_n f_n.  One 256-thread block per k-point,
 // Z staged in LDS, 4x4 register tiles.
 // ---------------------------------------------------------------------------------
@@ -250,7 +251,8 @@ __global__ __launch_bounds__(kTPB) void k_zfz(int N, const R* __restrict__ Z,
                                               const R* __restrict__ evals,
                                               const int* __restrict__ perm,
                                               const double* __restrict__ kw, double wuni,
-                                              double mu, double Temp, R* __restrict__ M,
+                                              double mu, double Temp,
+                                              const double* __restrict__ occ, R* __restrict__ M,
                                               double* __restrict__ eb) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int kk = blockIdx.x;
@@ -268,7 +270,8 @@ __global__ __launch_bounds__(kTPB) void k_zfz(int N, const R* __restrict__ Z,
     for (int n = tid; n < N; n += kTPB) {
         const double en = (double)ev[n];
         const double x = en - mu;
-        const double f = Temp > 0.0 ? 1.0 / (exp(x / Temp) + 1.0) : (x < 0.0 ? 1.0 : (x > 0.0 ? 0.0 : 0.5));
+        const double f = occ ? occ[(size_t)kk * N + n]
+                             : (Temp > 0.0 ? 1.0 / (exp(x / Temp) + 1.0) : (x < 0.0 ? 1.0 : (x > 0.0 ? 0.0 : 0.5)));
         sf[min(max(pm[n], 0), N - 1)] = f;
         ebp += en * f;
     }
