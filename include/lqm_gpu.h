@@ -120,6 +120,17 @@ lqm_status lqm_bandmatrix(lqm_model *m, int Nk, const double *ks, double *bands,
 lqm_status lqm_bandmatrix_dev(lqm_model *m, int Nk, const double *ks_dev, double *bands_dev,
                               double *evecs_dev);
 
+/* bandmatrix(H, ks, projectors) (Spectrum/bands.jl:52-66, :107-112, :222-224): bands and
+ * the expectation values obs[n, j, p] = real(psi_nj^H O_p(k_j) psi_nj) [N x Nk x nP] of
+ * nP operators, each a model handle on the same context (a k-independent matrix is a
+ * model with the single zero key; a Hops operator gives O_p(k) = sum_R O_R e^{2 pi i k.R},
+ * the k-dependent ExpvalMap). */
+lqm_status lqm_bandmatrix_expval(lqm_model *m, int Nk, const double *ks, int nP,
+                                 lqm_model *const *projectors, double *bands, double *obs);
+lqm_status lqm_bandmatrix_expval_dev(lqm_model *m, int Nk, const double *ks_dev, int nP,
+                                     lqm_model *const *projectors, double *bands_dev,
+                                     double *obs_dev);
+
 /* ---- DOS -------------------------------------------------------------------------- */
 /* dos_compute! (Spectrum/dos.jl:202-232): DOS(w) = sum_k wk sum_n Im[1/(w - i G - e_nk)],
  * divided by Nk when kweights == NULL (the unweighted branch, dos.jl:221). */

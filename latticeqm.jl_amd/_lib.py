@@ -57,6 +57,8 @@ _PROTOS = {
     "lqm_chemicalpotential_from_bands_dev": [P, I, I, P, P, D, D, P],
     "lqm_densitymatrix": [P, I, P, P, D, D, I, P, P, P],
     "lqm_densitymatrix_dev": [P, I, P, P, D, D, I, P, P, P, P],
+    "lqm_bandmatrix_expval": [P, I, P, I, P, P, P],
+    "lqm_bandmatrix_expval_dev": [P, I, P, I, P, P, P],
     "lqm_grandpotential": [P, I, P, P, D, D, P, P],
     "lqm_grandpotential_dev": [P, I, P, P, D, D, P, P],
     "lqm_densitymatrix_occ": [P, I, P, P, P, I, P, P],

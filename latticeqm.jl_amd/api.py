@@ -176,16 +176,36 @@ def fouriersum(H, ks, executor=None) -> np.ndarray:
     return out
 
 
+def _projector_models(projectors, m: GPUModel):
+    """handleprojector (bands.jl:62-66): a matrix, a Hops operator or a list of them."""
+    if len(projectors) == 1 and isinstance(projectors[0], (list, tuple)):
+        projectors = tuple(projectors[0])
+    out = []
+    for P in projectors:
+        if isinstance(P, Hops):
+            out.append(GPUModel(P, m.exec))
+        elif isinstance(P, np.ndarray) and P.ndim == 2:
+            out.append(GPUModel(Hops({(0,) * m.d: P}), m.exec))
+        else:
+            raise NotImplementedError("projectors must be matrices or Hops operators")
+    return out
+
+
 def bandmatrix(H, ks, *projectors, executor=None, multimode="gpu"):
-    """Eigenvalues of Hermitian(H(k)) per k: returns (bands (N, Nk), obs (N, Nk, 0))."""
-    if projectors:
-        raise NotImplementedError("projector expectation values are not part of this build")
+    """Eigenvalues of Hermitian(H(k)) per k: returns (bands (N, Nk), obs (N, Nk, nP)) with
+    obs[n, j, p] = real(psi^H O_p(k_j) psi) for the eigenvector of bands[n, j]."""
     m = _model(H, executor or multimode)
     k = _kpts(ks, m.d)
     Nk = k.shape[1]
     bands = np.zeros((m.N, Nk), order="F")
-    _lib.call("lqm_bandmatrix", m.handle, Nk, _vp(k), _vp(bands), None)
-    return bands, np.zeros((m.N, Nk, 0))
+    if not projectors:
+        _lib.call("lqm_bandmatrix", m.handle, Nk, _vp(k), _vp(bands), None)
+        return bands, np.zeros((m.N, Nk, 0))
+    pm = _projector_models(projectors, m)
+    obs = np.zeros((m.N, Nk, len(pm)), order="F")
+    handles = (_c.c_void_p * len(pm))(*[p.handle.value for p in pm])
+    _lib.call("lqm_bandmatrix_expval", m.handle, Nk, _vp(k), len(pm), handles, _vp(bands), _vp(obs))
+    return bands, obs
 
 
 def eigen(H, ks, executor=None):

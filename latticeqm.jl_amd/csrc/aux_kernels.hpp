@@ -260,4 +260,24 @@ __global__ __launch_bounds__(256) void k_block_transpose(int N, int nb, const cx
     out[b * NN + i * N + j] = in[idx];
 }
 
+// Band expectation values (Spectrum/bands.jl:52-58, :107-112): obs[n] of k-point kk =
+// Re(psi_n^H W_n) with W = O(k) V, V the eigenvectors (column n = psi_n).  One block per
+// k-point; out points at obs[0, k0, p] of the N x Nk x nP array (column stride N).
+__global__ __launch_bounds__(256) void k_expval_cols(int N, const cx<double>* __restrict__ V,
+                                                     const cx<double>* __restrict__ W,
+                                                     double* __restrict__ out) {
+    const int kk = blockIdx.x;
+    const size_t NN = (size_t)N * N;
+    const cx<double>* v = V + (size_t)kk * NN;
+    const cx<double>* w = W + (size_t)kk * NN;
+    for (int n = threadIdx.x; n < N; n += 256) {
+        double a = 0.0;
+        for (int i = 0; i < N; ++i) {
+            const cx<double> x = v[(size_t)n * N + i], y = w[(size_t)n * N + i];
+            a += x.re * y.re + x.im * y.im;
+        }
+        out[(size_t)kk * N + n] = a;
+    }
+}
+
 }  // namespace lqm

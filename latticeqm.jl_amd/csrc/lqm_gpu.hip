@@ -41,7 +41,7 @@ lqm_status fail(lqm_status st, const std::string& msg) {
 enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
-    W_S, W_G, W_LAM, W_STAT, W_GRADT, W_NSLOTS
+    W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -1502,6 +1502,72 @@ lqm_status lqm_grandpotential(lqm_model* m, int Nk, const double* ks, const doub
                                      (double*)dg)))
         return st;
     if (grad) return d2h(c, grad, dg, sizeof(cx<double>) * NN * m->nR);
+    return LQM_OK;
+}
+
+lqm_status lqm_bandmatrix_expval_dev(lqm_model* m, int Nk, const double* ks_dev, int nP,
+                                     lqm_model* const* projectors, double* bands_dev,
+                                     double* obs_dev) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk < 0 || nP < 0) return fail(LQM_ERR_ARG, "Nk and nP must be non-negative");
+    if (Nk == 0) return LQM_OK;
+    if (!ks_dev || !bands_dev || (nP > 0 && (!projectors || !obs_dev)))
+        return fail(LQM_ERR_ARG, "null buffer");
+    const int N = m->N;
+    for (int p = 0; p < nP; ++p) {
+        if (!projectors[p] || projectors[p]->ctx != m->ctx) return fail(LQM_ERR_ARG, "bad projector handle");
+        if (projectors[p]->N != N || projectors[p]->d != m->d)
+            return fail(LQM_ERR_ARG, "projector and Hamiltonian differ in N or d");
+    }
+    if (nP == 0) return run_bands(m, Nk, ks_dev, bands_dev, nullptr);
+    lqm_context* c = m->ctx;
+    const size_t NN = (size_t)N * N;
+    const size_t budget = size_t(4) << 30;
+    const int Kx = (int)std::max<size_t>(1, std::min<size_t>((size_t)Nk, budget / (3 * NN * sizeof(cx<double>))));
+    cx<double> *V, *O, *W;
+    if ((st = wsT(c, W_PV, NN * Kx, &V))) return st;
+    if ((st = wsT(c, W_PO, NN * Kx, &O))) return st;
+    if ((st = wsT(c, W_PW, NN * Kx, &W))) return st;
+    const cx<double> one = {1.0, 0.0}, zero = {0.0, 0.0};
+    for (int k0 = 0; k0 < Nk; k0 += Kx) {
+        const int Kc = std::min(Kx, Nk - k0);
+        const double* ksc = ks_dev + (size_t)k0 * m->d;
+        if ((st = run_bands(m, Kc, ksc, bands_dev + (size_t)k0 * N, reinterpret_cast<double*>(V)))) return st;
+        for (int p = 0; p < nP; ++p) {
+            lqm_model* q = projectors[p];
+            const size_t sh = sizeof(cx<double>) * q->nR;
+            auto kH = lqm::k_fouriersum;
+            allow_lds(kH, sh);
+            hipLaunchKernelGGL(kH, dim3(Kc), dim3(256), sh, c->stream, N, q->d, q->nR, q->dR, q->dT, ksc, O);
+            zgemm<0, 0>(c, N, Kc, O, V, W, one, zero);
+            hipLaunchKernelGGL(lqm::k_expval_cols, dim3(Kc), dim3(256), 0, c->stream, N, V, W,
+                               obs_dev + ((size_t)p * Nk + k0) * N);
+            LQM_HIP(hipGetLastError());
+        }
+    }
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    return LQM_OK;
+}
+
+lqm_status lqm_bandmatrix_expval(lqm_model* m, int Nk, const double* ks, int nP,
+                                 lqm_model* const* projectors, double* bands, double* obs) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk < 0 || nP < 0) return fail(LQM_ERR_ARG, "Nk and nP must be non-negative");
+    if (Nk == 0) return LQM_OK;
+    if (!ks || !bands || (nP > 0 && !obs)) return fail(LQM_ERR_ARG, "null buffer");
+    lqm_context* c = m->ctx;
+    const size_t N = m->N;
+    void *dks, *db, *dobs = nullptr;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if ((st = ws(c, W_BANDS, sizeof(double) * N * Nk, &db))) return st;
+    if (nP > 0 && (st = ws(c, W_OUT, sizeof(double) * N * Nk * nP, &dobs))) return st;
+    if ((st = lqm_bandmatrix_expval_dev(m, Nk, (const double*)dks, nP, projectors, (double*)db,
+                                        (double*)dobs)))
+        return st;
+    if ((st = d2h(c, bands, db, sizeof(double) * N * Nk))) return st;
+    if (nP > 0) return d2h(c, obs, dobs, sizeof(double) * N * Nk * nP);
     return LQM_OK;
 }
 

@@ -11,6 +11,11 @@ the exact ``Hops`` the band path is fed:
 * ``gethops`` (Operators/gethops.jl:33-140): block (i, j) of key L is
   t(r_i + A L, r_j), entries below sqrt(eps) dropped, the -L block is the adjoint,
   all-zero blocks removed (trim!, TightBinding/types.jl:278-294);
+* ``twist`` / ``honeycomb_twisted`` (Structure/Lattices/twist.jl:45-173,
+  Structure/Geometries/2d.jl:180-227): commensurate twisted bilayer on the moire
+  supercell S = [n, -n-m; n+m, 2n+m], static layer at +z/2, the other layer parity
+  flipped to -z/2, tiled 3 x 3, rotated by the twist angle and cropped to the cell
+  (supercell.jl:32-61, :171-268);
 * ``nearestneighbor`` (Operators/nearestneighbor.jl:22-62), ``t_graphene`` /
   ``graphene`` (Operators/graphene.jl:17-28, :228-273), ``gethubbard``
   (Operators/interactions.jl:8-28).
@@ -110,6 +115,12 @@ def getneighborcells(A: np.ndarray, k: int = 1, halfspace: bool = True, innerpoi
 def _hoppingblock(lat: Lattice, L, t, precision: float) -> np.ndarray:
     shift = lat.A @ np.asarray(L, dtype=float)
     R = lat.positions
+    if getattr(t, "vectorized", False):
+        # t(dr) on the whole (3, n, n) array of r_i + A L - r_j
+        dr = (R + shift[:, None])[:, :, None] - R[:, None, :]
+        M = np.asarray(t(dr), dtype=np.complex128)
+        M[np.abs(M) < precision] = 0.0
+        return M
     M = np.zeros((lat.n, lat.n), dtype=np.complex128)
     for i in range(lat.n):
         ri = R[:, i] + shift
@@ -163,10 +174,107 @@ def t_graphene(r1, r2, tmin: float = 1e-5, tz: float = 0.46, t0: float = 1.0, l_
     return 0.0 if abs(v) < tmin else float(v)
 
 
+def _t_graphene_vec(tmin: float = 1e-5, tz: float = 0.46, t0: float = 1.0, l_inter: float = 0.125,
+                    l_intra: float = 0.08, z: float = 3.0, a: float = 1.0, dmin: float = 0.1,
+                    dmax: float = 5.0):
+    """t_graphene on an array of separation vectors dr (3, ...)."""
+    def t(dr):
+        D = np.sqrt(np.sum(dr * dr, axis=0))
+        ok = (D >= dmin) & (D <= dmax)
+        Ds = np.where(ok, D, 1.0)
+        chi = dr[2] ** 2 / Ds ** 2
+        v = -t0 * (1 - chi) * np.exp(-np.abs(Ds - a) / l_intra) - tz * chi * np.exp(-np.abs(Ds - z) / l_inter)
+        return np.where(ok & (np.abs(v) >= tmin), v, 0.0)
+    t.vectorized = True
+    return t
+
+
 def graphene(lat: Lattice, mode: str = "nospin", cellrange: int = 2, **kw) -> Hops:
     """Graphene Hamiltonian with the t_graphene hopping (keyword overrides in kw)."""
-    h = gethops(lat, lambda r1, r2: t_graphene(r1, r2, **kw), cellrange=cellrange, precision=0.0)
+    h = gethops(lat, _t_graphene_vec(**kw), cellrange=cellrange, precision=0.0)
     return addspin(h, "spinhalf") if mode == "spinhalf" else h
+
+
+# ---- commensurate twisted bilayers ------------------------------------------------
+def twistangle(n: int, m: int = 1, degrees: bool = False) -> float:
+    """cos a = (3n^2 + 3nm + m^2/2) / (3n^2 + 3nm + m^2)  (twist.jl:44-47)."""
+    a = float(np.arccos((3.0 * n ** 2 + 3 * n * m + m ** 2 / 2.0) / (3.0 * n ** 2 + 3 * n * m + m ** 2)))
+    return a * 180 / np.pi if degrees else a
+
+
+def moire_supercell(n: int, m: int = 1) -> np.ndarray:
+    """Columns: the moire vectors in the primitive basis, [n, -n-m; n+m, 2n+m]."""
+    if np.gcd(n, m) != 1:
+        raise ValueError("twist indices must be coprime")
+    return np.array([[n, -n - m], [n + m, 2 * n + m]], dtype=np.int64)
+
+
+def supercellpoints(S: np.ndarray, offset: float = float(np.sqrt(np.finfo(float).eps))) -> np.ndarray:
+    """Integer points x with inv(S) x in [0, 1)^d (up to offset), (d, |det S|)."""
+    d = S.shape[0]
+    corners = S @ np.array(list(itertools.product([0, 1], repeat=d))).T
+    lo, hi = corners.min(axis=1), corners.max(axis=1)
+    Phi = np.linalg.inv(S.astype(float))
+    pts = []
+    for x in itertools.product(*[range(int(a), int(b) + 1) for a, b in zip(lo, hi)]):
+        f = Phi @ np.array(x, dtype=float)
+        if np.all((f < 1 - offset) & (f >= -offset)):
+            pts.append(x)
+    n = int(round(abs(np.linalg.det(S))))
+    if len(pts) != n:
+        raise RuntimeError("supercell enumeration failed")
+    return np.array(pts, dtype=np.int64).T
+
+
+def twist(lat: Lattice, n: int, z: float = 3.0, m: int = 1) -> Lattice:
+    """Twisted bilayer of a 2D triangular-Bravais lattice (positions in the returned
+    lattice are cartesian, folded into the moire cell; extra labels: layer, sublattice)."""
+    A2 = lat.A[:2, :]
+    alpha = twistangle(n, m)
+    S = moire_supercell(n, m)
+    Am = A2 @ S
+    frac = np.linalg.solve(A2, lat.positions[:2])  # primitive fractional coordinates
+    zs = lat.positions[2]
+    sub = lat.extra.get("sublattice", np.zeros(lat.n))
+    P = supercellpoints(S)
+    off = float(np.sqrt(np.finfo(float).eps))
+    xs, zz, layer, sl = [], [], [], []
+    # static layer: translated to +z/2
+    for p in P.T:
+        for s in range(lat.n):
+            xs.append(A2 @ (frac[:, s] + p))
+            zz.append(zs[s] + z / 2)
+            layer.append(0.0)
+            sl.append(sub[s])
+    nstatic = len(xs)
+    # rotated layer: +z/2, full parity flip, fold, supercell, 3x3 tile, rotate, crop
+    fflip = np.mod(-frac, 1.0)
+    c, sn = np.cos(alpha), np.sin(alpha)
+    Rot = np.array([[c, -sn], [sn, c]])
+    for i, j in itertools.product((-1, 0, 1), repeat=2):
+        tile = Am @ np.array([i, j], dtype=float)
+        for p in P.T:
+            for s in range(lat.n):
+                r = Rot @ (A2 @ (fflip[:, s] + p) + tile)
+                f = np.linalg.solve(Am, r)
+                if np.all((f < 1 - off) & (f >= -off)):
+                    xs.append(r)
+                    zz.append(-(zs[s] + z / 2))
+                    layer.append(1.0)
+                    sl.append(sub[s])
+    if len(xs) - nstatic != nstatic:
+        raise RuntimeError("twisted layer lost or duplicated atoms at the cell boundary")
+    pos = np.zeros((3, len(xs)))
+    pos[:2] = np.array(xs).T
+    pos[2] = zz
+    A = np.zeros((3, 2))
+    A[:2] = Am
+    return Lattice(A, pos, {"layer": np.array(layer), "sublattice": np.array(sl)})
+
+
+def honeycomb_twisted(N: int, a: float = 1.0, z: float = 3.0) -> Lattice:
+    """Twisted bilayer graphene with twist index N: 4(3N^2 + 3N + 1) orbitals."""
+    return twist(honeycomb(a), N, z=z, m=1)
 
 
 def gethubbard(lat: Lattice, neighbors=((0, 0),), mode: str = "nospin", a: float = 0.5,

@@ -123,3 +123,38 @@ def test_dimension_mismatch_raises(pkg, gpu):
     h = pkg.models.random_tb(4, rmax=1, seed=1)
     with pytest.raises(ValueError):
         pkg.api.bandmatrix(h, np.zeros((3, 2)))
+
+
+def test_band_expectation_values(pkg, oracle, gpu):
+    # projector expectation values real(psi^H O psi) (Spectrum/bands.jl:52-58, :107-112):
+    # a k-independent matrix and a k-dependent Hops operator, against the oracle's eigenvectors
+    N = 12
+    h = pkg.models.random_tb(N, rmax=1, seed=31)
+    ks = pkg.regulargrid(nk=25)
+    rng = np.random.default_rng(8)
+    M = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    O = pkg.models.random_tb(N, rmax=1, seed=32)
+    b, obs = pkg.api.bandmatrix(h, ks, [M, O])
+    R, T = h.pack()
+    rb, V = oracle.bands(R, T, ks, vectors=True)
+    _check_evals(b, rb, _hnorm(h, ks))
+    assert obs.shape == (N, ks.shape[1], 2)
+    for j in range(ks.shape[1]):
+        v = V[:, :, j]
+        e0 = np.real(np.einsum("in,ij,jn->n", v.conj(), M, v))
+        e1 = np.real(np.einsum("in,ij,jn->n", v.conj(), O(ks[:, j]), v))
+        assert np.max(np.abs(obs[:, j, 0] - e0)) < 1e-11
+        assert np.max(np.abs(obs[:, j, 1] - e1)) < 1e-11
+    bd = pkg.api.getbands(h, ks, M)
+    assert bd.obs.shape == (N, ks.shape[1], 1)
+
+
+def test_band_expectation_spin_zeeman(pkg, gpu):
+    # s_z of spinful graphene in a z Zeeman field: every band has <s_z> = +-1
+    h = pkg.models.addspin(pkg.models.honeycomb_nn(-1.0), "spinhalf")
+    sz = np.kron(np.eye(2), np.diag([1.0, -1.0]))
+    h[(0, 0)] = h[(0, 0)] + 0.3 * sz
+    ks = np.random.default_rng(4).random((2, 50))
+    b, obs = pkg.api.bandmatrix(h, ks, sz)
+    assert np.max(np.abs(np.abs(obs[:, :, 0]) - 1.0)) < 1e-12
+    assert np.max(np.abs(np.sum(obs[:, :, 0], axis=0))) < 1e-12
