@@ -58,6 +58,99 @@ def host_threads() -> int:
     return max(1, n)
 
 
+def scf_leg(no_cpu: bool) -> dict:
+    """Spinful honeycomb Hubbard model (4 orbitals), collinear Hartree-Fock at U = 3, half
+    filling, kT = 0.01, 60 x 60 grid, linear mixing 0.5, tol 1e-8, staggered m = 0.1 start
+    with 1e-3 Hermitian noise (seed 1): the whole SCF on the device, FP64."""
+    import importlib
+
+    import torch
+
+    mf = importlib.import_module("latticeqm_amd.meanfield")
+    h, v, sub = pkg.models.honeycomb_hubbard(3.0)
+    rho0 = mf.staggered_guess(v, sub, m=0.1, noise=1e-3, seed=1)
+    ks = pkg.regulargrid(nk=60 * 60)
+    kw = dict(T=0.01, beta=0.5, tol=1e-8, iterations=500)
+    ex = api.to_executor("gpu")
+    mf.solvehartreefock(h, v, rho0, 0.5, ks, executor=ex, **kw)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        rho, E, m, conv, res = mf.solvehartreefock(h, v, rho0, 0.5, ks, executor=ex, **kw)
+    t = (time.perf_counter() - t0) / reps
+    nit = m.history.shape[0]
+    A = mf.magnetization(rho, mf.sublatticeprojection(sub, 0))
+    out = {"workload": "honeycomb_hubbard_U3_k60x60_beta0.5_tol1e-8", "N": 4, "Nk": ks.shape[1],
+           "iterations": nit, "converged": conv, "residual": res, "E_GS": E, "mu": m.mu,
+           "sublattice_A_magnetization": A.tolist(), "seconds_per_scf": t,
+           "iterations_per_s": nit / t}
+    if not no_cpu:
+        from oracle import hartreefock
+
+        c0 = time.perf_counter()
+        ref = hartreefock.scf(h.data, v.data, rho0.data, 0.5, ks, nthreads=host_threads(), **kw)
+        tc = time.perf_counter() - c0
+        out["cpu_baseline"] = {"iterations_per_s": ref["iterations"] / tc, "cores": host_threads(),
+                               "kind": "port", "sample": "the same SCF, oracle passes (zheevr per k, "
+                                                         "OpenMP) + numpy mean field"}
+        out["max_abs_err_rho_vs_oracle"] = float(max(np.max(np.abs(rho[R] - ref["rho"][R])) for R in ref["rho"]))
+        out["abs_err_E_vs_oracle"] = float(abs(E - ref["E"]))
+    return out
+
+
+def adjoint_leg(no_cpu: bool) -> dict:
+    """Grand potential and dOmega/dt_R of a 128-orbital random model (|R_x|, |R_y| <= 1,
+    9 keys, seed 5) on a 64 x 64 grid at mu = 0, kT = 0.05, FP64 path, device-resident."""
+    import torch
+
+    N, side, mu, T = 128, 64, 0.0, 0.05
+    h = pkg.models.random_tb(N, rmax=1, seed=5)
+    ks = pkg.regulargrid(nk=side * side)
+    Nk = ks.shape[1]
+    dev = torch.device("cuda:0")
+    ks_d = torch.from_numpy(np.ascontiguousarray(ks.T)).to(dev)
+    m = api.GPUModel(h, api.to_executor("gpu"))
+    grad = torch.zeros((m.nR, N, N, 2), dtype=torch.float64, device=dev)
+    om = ctypes.c_double()
+    L = _lib.lib()
+
+    def run():
+        _lib.check(L.lqm_grandpotential_dev(m.handle, Nk, ctypes.c_void_p(ks_d.data_ptr()), None, mu, T,
+                                            ctypes.byref(om), ctypes.c_void_p(grad.data_ptr())),
+                   "lqm_grandpotential_dev")
+
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        run()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    out = {"workload": "random_tb_N128_R9_k64x64_grandpotential", "N": N, "Nk": Nk, "omega": om.value,
+           "seconds_per_evaluation": t, "kpoints_per_s": Nk / t,
+           "path_tflops_per_s": Nk * ((64.0 / 3.0) * N ** 3 + 8.0 * 18 * N * N) / t / 1e12}
+    if not no_cpu:
+        from oracle import oracle
+
+        R, Tt = h.pack()
+        S = 512
+        sub = np.asfortranarray(ks[:, :: Nk // S][:, :S])
+        c0 = time.perf_counter()
+        oracle.densitymatrix(R, Tt, sub, -R, mu=mu, Temp=T, nthreads=host_threads())
+        tc = time.perf_counter() - c0
+        out["cpu_baseline"] = {"kpoints_per_s": S / tc, "cores": host_threads(), "kind": "port",
+                               "sample": f"{S} of the {Nk} k-points, rho on the 9 reflected keys"}
+        # gradient against the oracle's rho(-R)^T on the full grid
+        rho, _ = oracle.densitymatrix(R, Tt, ks, -R, mu=mu, Temp=T, nthreads=host_threads())
+        g = grad.cpu().numpy()
+        g = g[..., 0] + 1j * g[..., 1]  # [r][j][i]: column-major blocks read row-major
+        out["max_abs_err_grad_vs_oracle"] = float(max(np.max(np.abs(g[r].T - rho[:, :, r].T))
+                                                      for r in range(R.shape[1])))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +159,8 @@ def main() -> int:
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-sample", type=int, default=16384, help="k-points of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-scf", action="store_true", help="skip the Hartree-Fock SCF leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the grand-potential adjoint leg")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after timing, copy this many MiB device-to-device once (HBM counter calibration)")
     args = ap.parse_args()
@@ -126,14 +221,15 @@ def main() -> int:
                 self.step(bands=True)
             ms = (ctypes.c_double * _lib.NSTAGES)()
             cnt = (ctypes.c_int64 * _lib.NSTAGES)()
-            _lib.call("lqm_context_stage_times", self.ex.handle, ms, cnt, _lib.NSTAGES)
+            kpt = (ctypes.c_int64 * _lib.NSTAGES)()
+            _lib.call("lqm_context_stage_times", self.ex.handle, ms, cnt, kpt, _lib.NSTAGES)
             _lib.call("lqm_context_set_timing", self.ex.handle, 0)
             out = {}
             for i in range(_lib.NSTAGES):
                 if cnt[i]:
                     out[L.lqm_stage_name(i).decode()] = {
                         "ms_per_step": ms[i] / n, "launches_per_step": cnt[i] / n,
-                        "avg_launch_ms": ms[i] / cnt[i]}
+                        "avg_launch_ms": ms[i] / cnt[i], "kpoints_per_launch": kpt[i] / cnt[i]}
             return out
 
     fp64 = Path("fp64")
@@ -157,8 +253,8 @@ def main() -> int:
     eb_err = float(abs(mixed.eb.item() - fp64.eb.item()))
 
     # dominant kernel of the headline (mixed) path
-    def flops(stage, launches):
-        nk = Nk / launches
+    def flops(stage, nk):
+        """Algorithmic FP flops of one launch over nk k-points (complex multiply-add = 8)."""
         if stage == "hk_tridiag":
             return nk * (8.0 * nR * N * N + 16.0 / 3.0 * N ** 3)
         if stage == "backtransform":
@@ -170,7 +266,7 @@ def main() -> int:
         return 0.0
 
     dom = max(stmx, key=lambda k: stmx[k]["ms_per_step"])
-    fl = flops(dom, stmx[dom]["launches_per_step"])
+    fl = flops(dom, stmx[dom]["kpoints_per_launch"])
     peak = PEAK_FP64_TFLOPS if (dom.startswith("zgemm_") or dom == "phase_contract") else 157.3
     achieved = fl / (stmx[dom]["avg_launch_ms"] * 1e-3) / 1e12
     traffic = None
@@ -225,6 +321,9 @@ def main() -> int:
         bsc = np.maximum(np.abs(bands_ref), np.max(np.abs(bands_ref), axis=0, keepdims=True))
         out["max_eigval_err_mixed_vs_fp64_oracle"] = float(np.max(np.abs(bmx[idx].T - bands_ref) / bsc))
 
+    scf = None if args.no_scf else scf_leg(args.no_cpu)
+    adj = None if args.no_adjoint else adjoint_leg(args.no_cpu)
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -252,6 +351,8 @@ def main() -> int:
                                        "abs_eband_err": eb_err},
         "refinement": report,
         "path_tflops_per_s": path_flops / tmx / 1e12,
+        "scf": scf,
+        "adjoint": adj,
         **out,
     }
     print(json.dumps(line))

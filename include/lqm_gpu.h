@@ -65,14 +65,16 @@ lqm_status lqm_context_synchronize(lqm_context *ctx);
 /* Kernel-stage timing (no reference counterpart: the reference has no tracer).
  * When enabled every kernel launch is bracketed by HIP events on the context
  * stream.  lqm_context_stage_times returns, per stage, the accumulated device
- * milliseconds and launch counts since the last query, then resets them.
+ * milliseconds, launch counts and k-points processed (0 for stages that do not work
+ * per k-point) since the last query, then resets them; any output may be NULL.
  * Stages: 0 hk_tridiag, 1 tridiag_ql, 2 replay, 3 backtransform, 4 zfz,
  * 5 rho_twosided, 6 phase_contract, 7 phase_fold, 8 dos, 9 reduce, 10 herm_hk_fp64,
  * 11 zgemm_nn_fp64, 12 zgemm_cn_fp64, 13 zgemm_nc_fp64, 14 oa_correct, 15 hf_meanfield,
  * 16 mix, 17 mu_bisect (LQM_NSTAGES = 18); lqm_stage_name gives the names. */
 #define LQM_NSTAGES 18
 lqm_status lqm_context_set_timing(lqm_context *ctx, int enable);
-lqm_status lqm_context_stage_times(lqm_context *ctx, double *ms, int64_t *launches, int n);
+lqm_status lqm_context_stage_times(lqm_context *ctx, double *ms, int64_t *launches,
+                                   int64_t *kpoints, int n);
 const char *lqm_stage_name(int stage);
 
 /* Kernel variants (no reference counterpart; the reference's only tunable is
@@ -86,13 +88,19 @@ lqm_status lqm_context_set_variant(lqm_context *ctx, int kind, int variant);
 
 /* Mixed precision (LQM_PREC_FP32_REFINED; no reference counterpart).  Eigenpairs
  * from the FP32 eigensolver are refined in FP64 by Ogita-Aishima sweeps
- * (R = I - X^H X, S = X^H A X, lambda_i = s_ii/(1-r_ii), X <- X + X E); a further
- * sweep runs only while some k-point has a near-degenerate pair the last sweep
- * could not separate, up to max_sweeps (default 3).  The report of the last call:
+ * (R = I - X^H X, S = X^H A X, X <- X + X E) with a Rayleigh-Ritz step on
+ * near-degenerate clusters; eigenvalues are the Rayleigh quotients s_ii/g_ii with their
+ * second-order correction from the off-diagonal of (S, G).  The first sweep refines every
+ * k-point; a k-point takes a further sweep only while its last correction rotated or
+ * left a cluster, or was so large that its square -- the error the sweep leaves --
+ * exceeds the threshold (default 1e-8; <= 0 gives every k-point at least two sweeps),
+ * up to max(2, max_sweeps) sweeps (max_sweeps default 3).  The report of the last call:
  * out[0] sweeps used, out[1] max_k ||I - X^H X||_F, out[2] max_k delta (cluster
- * threshold), out[3] k-points still holding near-degenerate pairs.  Eigenvector
- * output (bandmatrix with evecs) always uses the FP64 path. */
+ * threshold), out[3] k-points still holding near-degenerate pairs, out[4] k-point
+ * sweeps run (the sum over sweeps of the k-points refined).  Eigenvector output
+ * (bandmatrix with evecs) always uses the FP64 path. */
 lqm_status lqm_context_set_refinement(lqm_context *ctx, int max_sweeps);
+lqm_status lqm_context_set_refinement_threshold(lqm_context *ctx, double threshold);
 lqm_status lqm_context_refinement_report(lqm_context *ctx, double *out);
 lqm_status lqm_context_get_variant(lqm_context *ctx, int kind, int N, int *variant);
 lqm_status lqm_autotune(lqm_model *m, int Nk, const double *ks_dev);

@@ -37,11 +37,12 @@ _PROTOS = {
     "lqm_context_set_stream": [P, P],
     "lqm_context_synchronize": [P],
     "lqm_context_set_timing": [P, I],
-    "lqm_context_stage_times": [P, P, P, I],
+    "lqm_context_stage_times": [P, P, P, P, I],
     "lqm_stage_name": [I],
     "lqm_context_set_variant": [P, I, I],
     "lqm_context_set_refinement": [P, I],
     "lqm_context_refinement_report": [P, P],
+    "lqm_context_set_refinement_threshold": [P, D],
     "lqm_context_get_variant": [P, I, I, P],
     "lqm_autotune": [P, I, P],
     "lqm_model_create": [P, I, I, I, P, P, PP],

@@ -65,12 +65,17 @@ class GPUExec:
         """Largest number of Ogita-Aishima sweeps of the mixed-precision path."""
         _lib.call("lqm_context_set_refinement", self.handle, int(max_sweeps))
 
+    def set_refinement_threshold(self, thr: float) -> None:
+        """Squared-correction level below which a k-point stops refining (default 1e-8;
+        <= 0: two sweeps for every k-point)."""
+        _lib.call("lqm_context_set_refinement_threshold", self.handle, float(thr))
+
     def refinement_report(self) -> dict:
         """Diagnostics of the last mixed-precision call."""
-        out = (_c.c_double * 4)()
+        out = (_c.c_double * 5)()
         _lib.call("lqm_context_refinement_report", self.handle, out)
         return {"sweeps": int(out[0]), "max_orth_residual": out[1], "max_cluster_delta": out[2],
-                "kpoints_near_degenerate": int(out[3])}
+                "kpoints_near_degenerate": int(out[3]), "kpoint_sweeps": int(out[4])}
 
     def set_stream(self, stream_ptr: int | None) -> None:
         _lib.call("lqm_context_set_stream", self.handle, _c.c_void_p(stream_ptr or 0))

@@ -41,7 +41,7 @@ lqm_status fail(lqm_status st, const std::string& msg) {
 enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
-    W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_NSLOTS
+    W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -62,6 +62,7 @@ const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "ba
 
 struct TimedLaunch {
     int stage;
+    long long kpts;
     hipEvent_t a, b;
 };
 
@@ -75,6 +76,7 @@ struct lqm_context {
     std::vector<TimedLaunch> pending;
     double stage_ms[ST_N] = {};
     long long stage_n[ST_N] = {};
+    long long stage_k[ST_N] = {};
     int device = 0;
     int precision = 0;
     int lds_max = 65536;
@@ -84,7 +86,9 @@ struct lqm_context {
     size_t cap[W_NSLOTS] = {};
     int* d_status = nullptr;
     int refine_max = 3;      // Ogita-Aishima sweeps allowed per call (mixed precision)
-    double report[4] = {};   // last mixed call: sweeps, max ||R||_F, max delta, near-cluster k
+    double refine_thr = 1e-8;  // a k-point stops refining once its squared correction is below
+    double report[5] = {};   // last mixed call: sweeps, max ||R||_F, max delta, near-cluster k,
+                             // k-point sweeps run (sum over sweeps of the k-points refined)
     const double* occ = nullptr;  // given occupations of the chunk in flight (device), or null
 };
 
@@ -194,9 +198,11 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
 struct StageScope {
     lqm_context* c;
     TimedLaunch t{};
-    StageScope(lqm_context* ctx, int stage) : c(ctx) {
+    // kpts: k-points the bracketed launch processes (0 for stages not per k-point)
+    StageScope(lqm_context* ctx, int stage, long long kpts = 0) : c(ctx) {
         if (!c->timing) return;
         t.stage = stage;
+        t.kpts = kpts;
         if (hipEventCreate(&t.a) != hipSuccess || hipEventCreate(&t.b) != hipSuccess) {
             c = nullptr;
             return;
@@ -242,7 +248,7 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
         if ((st = wsT(c, W_NSW, (size_t)Kc, &nsw))) return st;
     }
     {
-        StageScope sc(c, ST_TRIDIAG);
+        StageScope sc(c, ST_TRIDIAG, Kc);
         if (P.nqA == 1) {
             auto kA = lqm::k_hk_tridiag_w<R, 1>;
             allow_lds(kA, P.shAw);
@@ -269,7 +275,7 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
     const int L = P.laneB;
     const int gB = (Kc + L - 1) / L;
     {
-        StageScope sc(c, ST_QL);
+        StageScope sc(c, ST_QL, Kc);
         if (logq) {
             auto kB = lqm::k_tridiag_ql<R, true>;
             allow_lds(kB, P.shB);
@@ -289,7 +295,7 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
 template <typename R, int NM>
 void launch_replay(lqm_context* c, int N, int Kc, const R* rot, const int* sw, const int* nsw,
                    int cap_rot, int cap_sw, R* Z) {
-    StageScope sc(c, ST_REPLAY);
+    StageScope sc(c, ST_REPLAY, Kc);
     auto kR = lqm::k_replay<R, NM>;
     hipLaunchKernelGGL(kR, dim3(Kc), dim3(lqm::kWave), 0, c->stream, N, rot, sw, nsw, cap_rot, cap_sw, Z);
 }
@@ -299,7 +305,7 @@ void launch_kc(lqm_context* c, const Plan& P, int N, int Kc, const cx<R>* refl, 
                const R* rot, const int* sw, const int* nsw, const R* evals, const int* perm,
                const double* kw, double wuni, double mu, double T, R* gV, const R* zin,
                cx<double>* out, double* eb) {
-    StageScope sc(c, ST_BACKTRANSFORM);
+    StageScope sc(c, ST_BACKTRANSFORM, Kc);
     if (P.ldsC) {
         auto kC = lqm::k_backtransform<R, true, MODE, ZIN>;
         allow_lds(kC, P.shC);
@@ -376,7 +382,7 @@ lqm_status launch_rho2(lqm_model* m, const Plan& P, int Kc, const double* evals,
     }
     LQM_HIP(hipGetLastError());
     {
-        StageScope sc(c, ST_ZFZ);
+        StageScope sc(c, ST_ZFZ, Kc);
         auto kM = lqm::k_zfz<double>;
         allow_lds(kM, P.shM);
         hipLaunchKernelGGL(kM, dim3(Kc), dim3(lqm::kTPB), P.shM, c->stream, N, Z, evals, perm, kw, wuni,
@@ -384,7 +390,7 @@ lqm_status launch_rho2(lqm_model* m, const Plan& P, int Kc, const double* evals,
     }
     LQM_HIP(hipGetLastError());
     {
-        StageScope sc(c, ST_RHO2);
+        StageScope sc(c, ST_RHO2, Kc);
         auto kR2 = lqm::k_rho_w<double, 1>;
         allow_lds(kR2, P.shR2);
         hipLaunchKernelGGL(kR2, dim3(Kc), dim3(lqm::kWave), P.shR2, c->stream, N, refl, tau, M, rhok);
@@ -400,7 +406,7 @@ void launch_contract(lqm_context* c, int NN, int Kc, int d, const double* ks, co
     hipStream_t s = c->stream;
     dim3 grid((NN + 255) / 256, S, groups);
     {
-        StageScope sc(c, ST_CONTRACT);
+        StageScope sc(c, ST_CONTRACT, Kc);
         hipLaunchKernelGGL(lqm::k_phase_contract<RG>, grid, dim3(256), 0, s, NN, Kc, d, ks, kw, wuni,
                            nRr, Rr, rhok, ksplit, part);
     }
@@ -541,13 +547,15 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
 }
 
 // ---- mixed precision: FP32 eigensolve + FP64 Ogita-Aishima refinement ---------------
+// C = alpha op(A) op(B) + beta D over nb k-points (kidx: their indices, null = 0..nb-1).
 template <int OPA, int OPB>
-void zgemm(lqm_context* c, int N, int Kc, const cx<double>* A, const cx<double>* B, cx<double>* C,
-           cx<double> alpha, cx<double> beta) {
-    StageScope sc(c, OPA == 0 ? (OPB == 0 ? ST_GEMM_NN : ST_GEMM_NC) : ST_GEMM_CN);
+void zgemm(lqm_context* c, int N, int nb, const cx<double>* A, const cx<double>* B, cx<double>* C,
+           cx<double> alpha, cx<double> beta, const cx<double>* D = nullptr,
+           const int* kidx = nullptr) {
+    StageScope sc(c, OPA == 0 ? (OPB == 0 ? ST_GEMM_NN : ST_GEMM_NC) : ST_GEMM_CN, nb);
     const int t = (N + lqm::kGT - 1) / lqm::kGT;
     auto kG = lqm::k_zgemm_batched<OPA, OPB>;
-    hipLaunchKernelGGL(kG, dim3(t, t, Kc), dim3(256), 0, c->stream, N, A, B, C, alpha, beta);
+    hipLaunchKernelGGL(kG, dim3(t, t, nb), dim3(256), 0, c->stream, N, A, B, C, D, alpha, beta, kidx);
 }
 
 // One pass over the grid.  Outputs (any may be null): sorted refined eigenvalues
@@ -558,6 +566,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
     lqm_context* c = m->ctx;
     const int N = m->N, d = m->d;
     if (N > 512) return fail(LQM_ERR_UNSUPPORTED, "mixed precision path supports N <= 512");
+    c->report[4] = 0.0;
     Plan P = make_plan(c, N, m->nR, Nk, true, false, sizeof(float));
     const size_t NN = (size_t)N * N;
     // refinement buffers: X, A, W, S, G (+ rho_k) in FP64 complex per k-point
@@ -595,56 +604,78 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         if ((st = launch_backtransform<float, 0>(m, P, Kc, ev32, nullptr, 0.0, 0.0, 0.0, X, nullptr)))
             return st;
         {
-            StageScope sc(c, ST_HK64);
+            StageScope sc(c, ST_HK64, Kc);
             const size_t sh = sizeof(cx<double>) * m->nR;
             allow_lds(lqm::k_herm_hk, sh);
             hipLaunchKernelGGL(lqm::k_herm_hk, dim3(Kc), dim3(256), sh, c->stream, N, d, m->nR, m->dR,
                                m->dT, ksc, A);
         }
         LQM_HIP(hipGetLastError());
+        // Ogita-Aishima sweeps.  The first runs on every k-point; later ones only on the
+        // k-points whose last correction flagged another sweep (a rotated or unresolved
+        // cluster, or a correction whose square -- the error it leaves -- is above the
+        // threshold), addressed through a device index list.
         int sweeps = 0;
+        int nsub = Kc;
+        const int* kidx = nullptr;
+        std::vector<int> cur, next;
+        const int max_sw = std::max(2, c->refine_max);
+        hstat.resize((size_t)4 * Kc);
         for (;;) {
-            zgemm<0, 0>(c, N, Kc, A, X, W, one, zero);   // W = A X
-            zgemm<1, 0>(c, N, Kc, X, W, Sm, one, zero);  // S = X^H A X
-            zgemm<1, 0>(c, N, Kc, X, X, G, one, zero);   // G = X^H X
+            zgemm<0, 0>(c, N, nsub, A, X, W, one, zero, nullptr, kidx);   // W = A X
+            zgemm<1, 0>(c, N, nsub, X, W, Sm, one, zero, nullptr, kidx);  // S = X^H A X
+            zgemm<1, 0>(c, N, nsub, X, X, G, one, zero, nullptr, kidx);   // G = X^H X
             {
-                StageScope sc(c, ST_OA);
+                StageScope sc(c, ST_OA, nsub);
                 const size_t sh = lqm::oa_lds_bytes(N);
                 allow_lds(lqm::k_oa_correct, sh);
-                hipLaunchKernelGGL(lqm::k_oa_correct, dim3(Kc), dim3(256), sh, c->stream, N, Sm, G, W, lam,
-                                   stat);
+                hipLaunchKernelGGL(lqm::k_oa_correct, dim3(nsub), dim3(256), sh, c->stream, N, Sm, G, W, lam,
+                                   stat, c->refine_thr, kidx);
             }
             LQM_HIP(hipGetLastError());
-            // X <- X + X E  (E in W; G is free after the correction)
-            LQM_HIP(hipMemcpyAsync(G, X, sizeof(cx<double>) * NN * Kc, hipMemcpyDeviceToDevice, c->stream));
-            zgemm<0, 0>(c, N, Kc, X, W, G, one, one);
-            std::swap(X, G);
+            // X <- X + X E  (E in W), written to G; then G holds the refined vectors
+            zgemm<0, 0>(c, N, nsub, X, W, G, one, one, X, kidx);
+            if (kidx) {
+                const int bx = (int)std::min<size_t>(64, (NN + 255) / 256);
+                hipLaunchKernelGGL(lqm::k_copy_sub, dim3(bx, nsub), dim3(256), 0, c->stream, (int)NN, nsub, kidx,
+                                   G, X);
+                LQM_HIP(hipGetLastError());
+            } else {
+                std::swap(X, G);
+            }
             ++sweeps;
-            hstat.resize((size_t)4 * Kc);
             LQM_HIP(hipMemcpyAsync(hstat.data(), stat, sizeof(double) * 4 * Kc, hipMemcpyDeviceToHost, c->stream));
             LQM_HIP(hipStreamSynchronize(c->stream));
-            int rotated = 0, unresolved = 0;
-            double offmax = 0.0;
-            for (int k = 0; k < Kc; ++k) {
+            if (!kidx) {
+                cur.resize(Kc);
+                for (int k = 0; k < Kc; ++k) cur[k] = k;
+            }
+            next.clear();
+            int unresolved = 0;
+            for (int k : cur) {
                 rmax = std::max(rmax, hstat[4 * k]);
                 dmax = std::max(dmax, hstat[4 * k + 1]);
                 const int fl = (int)hstat[4 * k + 2];
-                rotated += (fl & 1) != 0;
                 unresolved += (fl & 2) != 0;
-                offmax = std::max(offmax, hstat[4 * k + 3]);
+                if (fl & 4) next.push_back(k);
                 if (!std::isfinite(hstat[4 * k]) || !std::isfinite(hstat[4 * k + 3]))
                     return fail(LQM_ERR_NONFINITE, "mixed precision: non-finite refinement residual");
             }
-            // Two sweeps take FP32 eigenvectors to FP64 accuracy; more run while a
-            // cluster was rotated from a still inaccurate subspace.
-            const bool done = sweeps >= 2 && rotated == 0 && unresolved == 0;
-            if (done || sweeps >= std::max(2, c->refine_max)) {
+            c->report[4] += (double)cur.size();
+            if (next.empty()) break;
+            if (sweeps >= max_sw) {
                 nnear += unresolved;
                 if (unresolved)
                     return fail(LQM_ERR_NOCONV, "mixed precision: a near-degenerate cluster wider than 8 "
                                                 "eigenvalues could not be resolved; use the FP64 path");
                 break;
             }
+            int* dk;
+            if ((st = wsT(c, W_KIDX, (size_t)Kc, &dk))) return st;
+            LQM_HIP(hipMemcpyAsync(dk, next.data(), sizeof(int) * next.size(), hipMemcpyHostToDevice, c->stream));
+            kidx = dk;
+            nsub = (int)next.size();
+            cur.swap(next);
         }
         sweeps_max = std::max(sweeps_max, sweeps);
         if (bands) {
@@ -1030,7 +1061,8 @@ lqm_status lqm_context_set_timing(lqm_context* c, int enable) {
     return LQM_OK;
 }
 
-lqm_status lqm_context_stage_times(lqm_context* c, double* ms, int64_t* launches, int n) {
+lqm_status lqm_context_stage_times(lqm_context* c, double* ms, int64_t* launches, int64_t* kpoints,
+                                   int n) {
     if (!c) return fail(LQM_ERR_ARG, "null context");
     LQM_HIP(hipStreamSynchronize(c->stream));
     for (auto& t : c->pending) {
@@ -1038,6 +1070,7 @@ lqm_status lqm_context_stage_times(lqm_context* c, double* ms, int64_t* launches
         if (hipEventElapsedTime(&e, t.a, t.b) == hipSuccess) {
             c->stage_ms[t.stage] += e;
             c->stage_n[t.stage] += 1;
+            c->stage_k[t.stage] += t.kpts;
         }
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
@@ -1046,8 +1079,10 @@ lqm_status lqm_context_stage_times(lqm_context* c, double* ms, int64_t* launches
     for (int i = 0; i < n && i < ST_N; ++i) {
         if (ms) ms[i] = c->stage_ms[i];
         if (launches) launches[i] = c->stage_n[i];
+        if (kpoints) kpoints[i] = c->stage_k[i];
         c->stage_ms[i] = 0;
         c->stage_n[i] = 0;
+        c->stage_k[i] = 0;
     }
     return LQM_OK;
 }
@@ -1059,9 +1094,16 @@ lqm_status lqm_context_set_refinement(lqm_context* c, int max_sweeps) {
     return LQM_OK;
 }
 
+lqm_status lqm_context_set_refinement_threshold(lqm_context* c, double thr) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    if (!(thr < 1.0)) return fail(LQM_ERR_ARG, "threshold must be below 1");
+    c->refine_thr = thr;
+    return LQM_OK;
+}
+
 lqm_status lqm_context_refinement_report(lqm_context* c, double* out) {
     if (!c || !out) return fail(LQM_ERR_ARG, "null argument");
-    for (int i = 0; i < 4; ++i) out[i] = c->report[i];
+    for (int i = 0; i < 5; ++i) out[i] = c->report[i];
     return LQM_OK;
 }
 

@@ -47,26 +47,30 @@ __global__ __launch_bounds__(256) void k_herm_hk(int N, int d, int nR, const int
     }
 }
 
-// Batched C = alpha op(A) op(B) + beta C, square N x N column-major FP64 complex,
-// batch stride N*N.  op 0 = as is, 1 = conjugate transpose.  64 x 64 output tile
-// per block, K-steps of 16 staged in LDS, 4 x 4 complex register block per thread
-// (rows ti + 16u, columns tj + 16v).  grid = (tiles, tiles, batch).
+// Batched C = alpha op(A) op(B) + beta D, square N x N column-major FP64 complex,
+// batch stride N*N (D may be C, or null when beta = 0).  op 0 = as is, 1 = conjugate
+// transpose.  The z-th batch entry is k-point kidx[z] (z itself when kidx is null).
+// 64 x 64 output tile per block, K-steps of 16 staged in LDS, 4 x 4 complex register
+// block per thread (rows ti + 16u, columns tj + 16v).  grid = (tiles, tiles, batch).
 constexpr int kGT = 64, kGK = 16;
 
 template <int OPA, int OPB>
 __global__ __launch_bounds__(256) void k_zgemm_batched(int N, const cx<double>* __restrict__ A,
                                                        const cx<double>* __restrict__ B,
-                                                       cx<double>* __restrict__ C,
-                                                       cx<double> alpha, cx<double> beta) {
+                                                       cx<double>* C, const cx<double>* D,
+                                                       cx<double> alpha, cx<double> beta,
+                                                       const int* __restrict__ kidx) {
     __shared__ cx<double> As[kGK][kGT + 1];  // As[k][i] = op(A)(i0 + i, k0 + k)
     __shared__ cx<double> Bs[kGK][kGT + 1];  // Bs[k][j] = op(B)(k0 + k, j0 + j)
     const int tid = threadIdx.x;
     const int ti = tid & 15, tj = tid >> 4;
     const int i0 = blockIdx.x * kGT, j0 = blockIdx.y * kGT;
     const size_t NN = (size_t)N * N;
-    const cx<double>* Ak = A + (size_t)blockIdx.z * NN;
-    const cx<double>* Bk = B + (size_t)blockIdx.z * NN;
-    cx<double>* Ck = C + (size_t)blockIdx.z * NN;
+    const size_t kk = kidx ? (size_t)kidx[blockIdx.z] : (size_t)blockIdx.z;
+    const cx<double>* Ak = A + kk * NN;
+    const cx<double>* Bk = B + kk * NN;
+    cx<double>* Ck = C + kk * NN;
+    const cx<double>* Dk = D ? D + kk * NN : nullptr;
     cx<double> acc[4][4];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
@@ -112,17 +116,17 @@ __global__ __launch_bounds__(256) void k_zgemm_batched(int N, const cx<double>* 
         }
         __syncthreads();
     }
-    const bool zero_beta = (beta.re == 0.0 && beta.im == 0.0);
+    const bool zero_beta = (beta.re == 0.0 && beta.im == 0.0) || !Dk;
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int gi = i0 + ti + 16 * u, gj = j0 + tj + 16 * v;
             if (gi < N && gj < N) {
-                cx<double>* c = Ck + (size_t)gj * N + gi;
+                const size_t off = (size_t)gj * N + gi;
                 cx<double> r = alpha * acc[u][v];
-                if (!zero_beta) r = cmad(r, beta, *c);
-                *c = r;
+                if (!zero_beta) r = cmad(r, beta, Dk[off]);
+                Ck[off] = r;
             }
         }
 }
@@ -185,9 +189,12 @@ __device__ void jacobi_herm(int c, cx<double>* K, cx<double>* U) {
 
 // Ogita-Aishima correction with Rayleigh-Ritz on near-degenerate clusters.
 // In: S = X^H A X, G = X^H X.  Out: E[kk] (X <- X + X E), lambda[kk*N + i], and
-// stat[kk*4 + {0: ||R||_F, 1: delta, 2: flags, 3: off-cluster ||S - D||_F}] where
+// stat[kk*4 + {0: ||R||_F, 1: delta, 2: flags, 3: max |E_ij| outside the clusters}] where
 // flags bit 0 = some cluster was rotated, bit 1 = a cluster wider than kClusterMax
-// that is not degenerate to working precision (left unresolved).
+// that is not degenerate to working precision (left unresolved), bit 2 = another sweep
+// is needed: bit 0 or 1, or the first-order correction is so large that its square,
+// the error left after this sweep, exceeds thr (thr <= 0: always).
+// Block z works on k-point kidx[z] (z when kidx is null).
 // Clusters are maximal runs of consecutive columns whose lambda gaps are <= delta
 // (columns arrive in ascending FP32 eigenvalue order).  For a cluster C the
 // orthonormalised block K = (I + R/2)^H S_CC (I + R/2) is diagonalised, K = U L U^H,
@@ -196,7 +203,8 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
                                                     const cx<double>* __restrict__ G,
                                                     cx<double>* __restrict__ E,
                                                     double* __restrict__ lambda,
-                                                    double* __restrict__ stat) {
+                                                    double* __restrict__ stat, double thr,
+                                                    const int* __restrict__ kidx) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int maxcl = oa_max_clusters(N);
     const size_t o_lnew = 8 * (size_t)N, o_cid = o_lnew + 8 * (size_t)N, o_cpos = o_cid + 4 * (size_t)N;
@@ -212,7 +220,8 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
     double* red = reinterpret_cast<double*>(smem + o_red);       // [16]
     cx<double>* Ubuf = reinterpret_cast<cx<double>*>(smem + o_u);  // [maxcl][kClusterMax^2]
     __shared__ int s_ncl, s_flags;
-    const int kk = blockIdx.x, tid = threadIdx.x;
+    const int kk = kidx ? kidx[blockIdx.x] : (int)blockIdx.x;
+    const int tid = threadIdx.x;
     const size_t NN = (size_t)N * N;
     const cx<double>* Sk = S + (size_t)kk * NN;
     const cx<double>* Gk = G + (size_t)kk * NN;
@@ -314,7 +323,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
     }
     __syncthreads();
     // E entries
-    double s2off = 0.0;
+    double emax = 0.0;
     for (int e = tid; e < N * N; e += 256) {
         const int i = e % N, j = e / N;
         const int ci = cid[i], cj = cid[j];
@@ -352,18 +361,40 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
             }
         }
         Ek[e] = out;
-        if (i != j && !(ci >= 0 && ci == cj)) s2off += abs2(Sk[e]);
+        if (i != j && !(ci >= 0 && ci == cj)) emax = fmax(emax, abs2(out));
     }
-    s2off = wave_sum(s2off);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o));
     __syncthreads();
-    if ((tid & 63) == 0) red[12 + tid / 64] = s2off;
+    if ((tid & 63) == 0) red[12 + tid / 64] = emax;
+    // Second-order (Rayleigh-Schroedinger) correction of the Rayleigh quotient of every
+    // column outside a cluster, from the pencil (S, G):
+    //   lambda_j += sum_{i != j, |lambda_j - lambda_i| > delta} |s_ij - lambda_j g_ij|^2 / (lambda_j - lambda_i),
+    // so a single sweep leaves an eigenvalue error of third order in the FP32 error.
+    for (int j = tid; j < N; j += 256) {
+        if (cid[j] >= 0) continue;
+        const double lj = lam[j];
+        double corr = 0.0;
+        for (int i = 0; i < N; ++i) {
+            const double gap = lj - lam[i];
+            if (i == j || fabs(gap) <= delta) continue;
+            const cx<double> sv = Sk[(size_t)j * N + i], g = Gk[(size_t)j * N + i];
+            const double tre = sv.re - lj * g.re, tim = sv.im - lj * g.im;
+            corr += (tre * tre + tim * tim) / gap;
+        }
+        lnew[j] = lj + corr;
+    }
     __syncthreads();
     for (int i = tid; i < N; i += 256) lambda[(size_t)kk * N + i] = lnew[i];
     if (tid == 0) {
+        const double em = sqrt(fmax(fmax(red[12], red[13]), fmax(red[14], red[15])));
+        const double lead = fmax(em, rn);
+        int flags = s_flags;
+        if ((flags & 3) || thr <= 0.0 || lead * lead > thr) flags |= 4;
         stat[(size_t)kk * 4 + 0] = rn;
         stat[(size_t)kk * 4 + 1] = delta;
-        stat[(size_t)kk * 4 + 2] = s_flags;
-        stat[(size_t)kk * 4 + 3] = sqrt(red[12] + red[13] + red[14] + red[15]);
+        stat[(size_t)kk * 4 + 2] = flags;
+        stat[(size_t)kk * 4 + 3] = em;
     }
     (void)s_ncl;
 }
@@ -430,4 +461,17 @@ __global__ __launch_bounds__(256) void k_sort_evals(int N, const double* __restr
     }
 }
 
+}  // namespace lqm
+
+namespace lqm {
+// dst[kidx[z]] = src[kidx[z]] for n k-points of N x N complex blocks.
+__global__ __launch_bounds__(256) void k_copy_sub(int NN, int n, const int* __restrict__ kidx,
+                                                  const cx<double>* __restrict__ src,
+                                                  cx<double>* __restrict__ dst) {
+    const size_t z = blockIdx.y;
+    const size_t kk = (size_t)kidx[z];
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < NN; e += gridDim.x * 256)
+        dst[kk * NN + e] = src[kk * NN + e];
+    (void)n;
+}
 }  // namespace lqm

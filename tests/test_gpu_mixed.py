@@ -84,3 +84,33 @@ def test_mixed_chemicalpotential_and_dos(pkg, oracle, mixed):
     om = np.linspace(-5, 5, 101)
     dos = pkg.api.getdos(h, om, ks, Gamma=0.1, executor=mixed)
     assert np.max(np.abs(dos - oracle.dos(R, T, ks, om, 0.1))) < 1e-10 * np.max(dos)
+
+
+def test_adaptive_refinement_sweeps(pkg, oracle, mixed):
+    # the second sweep runs only where the first left a correction above the threshold;
+    # forcing it everywhere (threshold 0) must not change the results beyond the bars
+    N = 64
+    h = pkg.models.random_tb(N, rmax=2, seed=2)
+    ks = pkg.regulargrid(nk=32 ** 2)
+    Nk = ks.shape[1]
+    R, Tt = h.pack()
+    ref, eb_ref = oracle.densitymatrix(R, Tt, ks, R, mu=0.0, Temp=0.02)
+    refb = oracle.bands(R, Tt, ks)
+    out = {}
+    try:
+        for thr in (1e-8, 0.0):
+            mixed.set_refinement_threshold(thr)
+            rho = pkg.Hops({key: np.zeros((N, N), complex) for key in h.keys()})
+            eb = pkg.api.getdensitymatrix(rho, h, ks, 0.0, T=0.02, executor=mixed)
+            rep = mixed.refinement_report()
+            err = max(np.max(np.abs(rho[tuple(R[:, r])] - ref[:, :, r])) for r in range(R.shape[1]))
+            assert err < 1e-9, (thr, err)
+            assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+            b, _ = pkg.api.bandmatrix(h, ks, executor=mixed)
+            e = np.abs(b - refb) / np.maximum(np.abs(refb), _hnorm(h, ks)[None, :])
+            assert np.max(e) < 1e-10, (thr, np.max(e))
+            out[thr] = rep["kpoint_sweeps"]
+    finally:
+        mixed.set_refinement_threshold(1e-8)
+    assert Nk <= out[1e-8] < 2 * Nk
+    assert out[0.0] >= 2 * Nk
