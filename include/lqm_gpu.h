@@ -79,7 +79,8 @@ const char *lqm_stage_name(int stage);
 
 /* Kernel variants (no reference counterpart; the reference's only tunable is
  * the executor).  kind 0: tridiagonalization (0 = 256-thread full tile, 1 = one
- * wave with the packed Hermitian tile); kind 1: rho_k (0 = V = QZ then V f V^H,
+ * wave with the packed Hermitian tile, 2 = register tile on a materialised H(k), N <= 64);
+ * kind 1: rho_k (0 = V = QZ then V f V^H,
  * 1 = two-sided reflectors on Z f Z^T).  variant -1 returns to the tuned/default
  * choice.  lqm_autotune times every variant of every kind on Nk device k-points of
  * the model and records the fastest per orbital count N in the context;

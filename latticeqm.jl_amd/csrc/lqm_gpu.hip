@@ -19,6 +19,7 @@
 #include "wave_kernels.hpp"
 #include "refine_kernels.hpp"
 #include "hf_kernels.hpp"
+#include "reg_kernels.hpp"
 
 using lqm::cx;
 
@@ -41,7 +42,7 @@ lqm_status fail(lqm_status st, const std::string& msg) {
 enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
-    W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_NSLOTS
+    W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -139,6 +140,7 @@ struct Plan {
     int rch = 0;  // rotation-staging capacity of K_C
     int nm = 0;   // register row length of k_replay (0: replay inside K_C)
     int nqA = 0;  // rows per lane of the one-wave tridiagonalization (0: 256-thread kernel)
+    bool regA = false;  // register-tile tridiagonalization on a materialised H(k) (N <= 64)
     size_t shAw = 0;
     bool rho2 = false;  // rho_k by two-sided reflectors on the packed Z f Z^T
     size_t shM = 0, shR2 = 0;
@@ -180,8 +182,14 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
         auto it = c->tuned.find((kind << 20) | N);
         return it == c->tuned.end() ? 0 : it->second;
     };
-    // tridiagonalization: 0 = 256-thread full-tile kernel, 1 = one wave, packed tile
-    if (variant(VK_TRIDIAG) == 1 && N <= 128 && p.shAw <= (size_t)c->lds_max) p.nqA = N <= 64 ? 1 : 2;
+    // tridiagonalization: 0 = 256-thread LDS-tile kernel, 1 = one wave, packed tile,
+    // 2 = register tile on a materialised H(k) (N <= 64)
+    const int vt = variant(VK_TRIDIAG);
+    if (vt == 1 && N <= 128 && p.shAw <= (size_t)c->lds_max) p.nqA = N <= 64 ? 1 : 2;
+    if (vt == 2 && N <= 64) {
+        p.regA = true;
+        perk += 16 * NN;
+    }
     p.shM = sizeof(double) * ((size_t)N * (N + 1) + 2 + N + 4);
     p.shR2 = sizeof(cx<double>) * (np + 2 * N);
     // rho_k: 0 = V = QZ then V f V^H, 1 = two-sided reflectors on Z f Z^T
@@ -225,9 +233,10 @@ void allow_lds(K kernel, size_t bytes) {
 
 // Eigenvalues (and optionally the QL log) of one chunk: K_A then K_B, in the
 // working precision R (double: the plain path; float: the bulk of the mixed path).
+// Hpre: H(k) of the chunk already materialised in FP64 (full Hermitian), or null.
 template <typename R>
 lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks, R* evals,
-                          bool logq) {
+                          bool logq, const cx<double>* Hpre = nullptr) {
     lqm_context* c = m->ctx;
     const int N = m->N;
     R *dd, *ee;
@@ -247,9 +256,22 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
         if ((st = wsT(c, W_SW, (size_t)P.cap_sw * Kc, &sw))) return st;
         if ((st = wsT(c, W_NSW, (size_t)Kc, &nsw))) return st;
     }
+    if (P.regA && !Hpre) {
+        cx<double>* Hk;
+        if ((st = wsT(c, W_HK, (size_t)N * N * Kc, &Hk))) return st;
+        StageScope sc(c, ST_HK64, Kc);
+        const size_t sh = sizeof(cx<double>) * m->nR;
+        allow_lds(lqm::k_herm_hk, sh);
+        hipLaunchKernelGGL(lqm::k_herm_hk, dim3(Kc), dim3(256), sh, c->stream, N, m->d, m->nR, m->dR, m->dT,
+                           ks, Hk);
+        Hpre = Hk;
+    }
     {
         StageScope sc(c, ST_TRIDIAG, Kc);
-        if (P.nqA == 1) {
+        if (P.regA) {
+            auto kA = lqm::k_tridiag_r<R>;
+            hipLaunchKernelGGL(kA, dim3(Kc), dim3(256), 0, c->stream, N, Kc, Hpre, dd, ee, refl, tau);
+        } else if (P.nqA == 1) {
             auto kA = lqm::k_hk_tridiag_w<R, 1>;
             allow_lds(kA, P.shAw);
             hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kWave), P.shAw, c->stream, N, m->d, m->nR, m->dR,
@@ -554,8 +576,15 @@ void zgemm(lqm_context* c, int N, int nb, const cx<double>* A, const cx<double>*
            const int* kidx = nullptr) {
     StageScope sc(c, OPA == 0 ? (OPB == 0 ? ST_GEMM_NN : ST_GEMM_NC) : ST_GEMM_CN, nb);
     const int t = (N + lqm::kGT - 1) / lqm::kGT;
-    auto kG = lqm::k_zgemm_batched<OPA, OPB>;
-    hipLaunchKernelGGL(kG, dim3(t, t, nb), dim3(256), 0, c->stream, N, A, B, C, D, alpha, beta, kidx);
+    // op(A) = A: matrix-core kernel (fastest measured); products with a conjugate
+    // transpose: the LDS-tiled VALU kernel, whose staging reads them coalesced
+    if (OPA == 0 && OPB == 0) {
+        auto kG = lqm::k_zgemm_mfma<OPA, OPB>;
+        hipLaunchKernelGGL(kG, dim3(t, t, nb), dim3(256), 0, c->stream, N, A, B, C, D, alpha, beta, kidx);
+    } else {
+        auto kG = lqm::k_zgemm_batched<OPA, OPB>;
+        hipLaunchKernelGGL(kG, dim3(t, t, nb), dim3(256), 0, c->stream, N, A, B, C, D, alpha, beta, kidx);
+    }
 }
 
 // One pass over the grid.  Outputs (any may be null): sorted refined eigenvalues
@@ -599,10 +628,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         if ((st = wsT(c, W_G, NN * P.Kc, &G))) return st;
         if ((st = wsT(c, W_LAM, (size_t)N * P.Kc, &lam))) return st;
         if ((st = wsT(c, W_STAT, (size_t)4 * P.Kc, &stat))) return st;
-        // FP32 eigenvectors, ascending, written as FP64 complex
-        if ((st = launch_eigvals<float>(m, P, Kc, ksc, ev32, true))) return st;
-        if ((st = launch_backtransform<float, 0>(m, P, Kc, ev32, nullptr, 0.0, 0.0, 0.0, X, nullptr)))
-            return st;
+        // H(k) in FP64 (the refinement's A), then FP32 eigenvectors, ascending, as FP64 complex
         {
             StageScope sc(c, ST_HK64, Kc);
             const size_t sh = sizeof(cx<double>) * m->nR;
@@ -611,6 +637,9 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
                                m->dT, ksc, A);
         }
         LQM_HIP(hipGetLastError());
+        if ((st = launch_eigvals<float>(m, P, Kc, ksc, ev32, true, A))) return st;
+        if ((st = launch_backtransform<float, 0>(m, P, Kc, ev32, nullptr, 0.0, 0.0, 0.0, X, nullptr)))
+            return st;
         // Ogita-Aishima sweeps.  The first runs on every k-point; later ones only on the
         // k-points whose last correction flagged another sweep (a rotated or unresolved
         // cluster, or a correction whose square -- the error it leaves -- is above the
@@ -1109,7 +1138,8 @@ lqm_status lqm_context_refinement_report(lqm_context* c, double* out) {
 
 lqm_status lqm_context_set_variant(lqm_context* c, int kind, int variant) {
     if (!c) return fail(LQM_ERR_ARG, "null context");
-    if (kind < 0 || kind >= VK_N || variant < -1 || variant > 1) return fail(LQM_ERR_ARG, "bad variant");
+    if (kind < 0 || kind >= VK_N || variant < -1 || variant > (kind == VK_TRIDIAG ? 2 : 1))
+        return fail(LQM_ERR_ARG, "bad variant");
     c->forced[kind] = variant;
     return LQM_OK;
 }
@@ -1293,7 +1323,8 @@ lqm_status lqm_autotune(lqm_model* m, int Nk, const double* ks_dev) {
     for (int kind = 0; kind < VK_N; ++kind) {
         float best = 1e30f;
         int bestv = 0;
-        for (int v = 0; v <= 1; ++v) {
+        const int vmax = kind == VK_TRIDIAG && N <= 64 ? 2 : 1;
+        for (int v = 0; v <= vmax; ++v) {
             c->forced[kind] = v;
             float t = 0.f;
             for (int rep = 0; rep < 3 && !st; ++rep) {

@@ -131,6 +131,110 @@ __global__ __launch_bounds__(256) void k_zgemm_batched(int N, const cx<double>* 
         }
 }
 
+// The same batched C = alpha op(A) op(B) + beta D on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64).  A 256-thread block computes a 64 x 64 tile of C, each
+// wave a 32 x 32 quadrant as 2 x 2 MFMA tiles of 16 x 16; a complex product is four
+// real products, C_re += A_re B_re - A_im B_im, C_im += A_re B_im + A_im B_re, issued
+// so that consecutive instructions never share an accumulator.
+// Instruction layouts (16 x 16 x 4, 64 lanes): lane l supplies A[l%16][k] and
+// B[k][l%16] and holds D[l/16 + 4i][l%16], i = 0..3.  The contraction index is
+// permuted so that a lane's four consecutive instructions use four consecutive k
+// (k = k0 + 4 (l/16) + s, s = 0..3): the operand walking k is then read as one
+// contiguous 64-byte run straight from global memory (L1/L2-resident).
+typedef double lqm_d4 __attribute__((ext_vector_type(4)));
+
+template <int OP>
+__device__ __forceinline__ cx<double> op_at(const cx<double>* __restrict__ M, int N, int r, int c) {
+    // element (r, c) of op(M): M itself (column-major) or its conjugate transpose
+    if (OP == 0) return M[(size_t)c * N + r];
+    const cx<double> x = M[(size_t)r * N + c];
+    return {x.re, -x.im};
+}
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(256) void k_zgemm_mfma(int N, const cx<double>* __restrict__ A,
+                                                    const cx<double>* __restrict__ B, cx<double>* C,
+                                                    const cx<double>* D, cx<double> alpha,
+                                                    cx<double> beta, const int* __restrict__ kidx) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
+    const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
+    const size_t NN = (size_t)N * N;
+    const size_t kk = kidx ? (size_t)kidx[blockIdx.z] : (size_t)blockIdx.z;
+    const cx<double>* Ak = A + kk * NN;
+    const cx<double>* Bk = B + kk * NN;
+    lqm_d4 cr[2][2], ci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            cr[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+            ci[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+        }
+    for (int k0 = 0; k0 < N; k0 += 16) {
+        const int kb = k0 + 4 * g;
+        double ar[2][4], ai[2][4], br[2][4], bi[2][4];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int m = m0 + 16 * t + li, n = n0 + 16 * t + li;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int k = kb + s;
+                cx<double> x = {0.0, 0.0}, y = {0.0, 0.0};
+                if (k < N) {
+                    if (m < N) x = op_at<OPA>(Ak, N, m, k);
+                    if (n < N) y = op_at<OPB>(Bk, N, k, n);
+                }
+                ar[t][s] = x.re;
+                ai[t][s] = x.im;
+                br[t][s] = y.re;
+                bi[t][s] = y.im;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a][s], br[b][s], cr[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a][s], bi[b][s], ci[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a][s], bi[b][s], cr[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a][s], br[b][s], ci[a][b], 0, 0, 0);
+        }
+    }
+    cx<double>* Ck = C + kk * NN;
+    const cx<double>* Dk = D ? D + kk * NN : nullptr;
+    const bool zero_beta = (beta.re == 0.0 && beta.im == 0.0) || !Dk;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = m0 + 16 * a + g + 4 * i, n = n0 + 16 * b + li;
+                if (m < N && n < N) {
+                    const size_t off = (size_t)n * N + m;
+                    cx<double> r = alpha * cx<double>{cr[a][b][i], ci[a][b][i]};
+                    if (!zero_beta) r = cmad(r, beta, Dk[off]);
+                    Ck[off] = r;
+                }
+            }
+}
+
 // In-place cyclic Jacobi diagonalization of a small Hermitian matrix K (c x c,
 // row-major in registers of one thread) accumulating the unitary U (K <- U^H K U).
 // Each rotation first makes the (p, q) entry real with a phase, then applies the

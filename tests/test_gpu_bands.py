@@ -39,7 +39,7 @@ def test_square_triangular_analytic(pkg, gpu):
     assert np.max(np.abs(b[0] - tri)) < 1e-12
 
 
-@pytest.fixture(params=[0, 1], ids=["tri-block", "tri-wave"])
+@pytest.fixture(params=[0, 1, 2], ids=["tri-block", "tri-wave", "tri-reg"])
 def tri_variant(request, gpu):
     gpu.set_variant(0, request.param)
     yield request.param

@@ -9,9 +9,12 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def mixed(pkg, gpu):
-    return pkg.api.to_executor("gpu", precision="mixed")
+@pytest.fixture(scope="module", params=[-1, 2], ids=["tri-default", "tri-reg"])
+def mixed(pkg, gpu, request):
+    ex = pkg.api.to_executor("gpu", precision="mixed")
+    ex.set_variant(0, request.param)
+    yield ex
+    ex.set_variant(0, -1)
 
 
 def _hnorm(h, ks):
