@@ -81,7 +81,9 @@ const char *lqm_stage_name(int stage);
  * the executor).  kind 0: tridiagonalization (0 = 256-thread full tile, 1 = one
  * wave with the packed Hermitian tile, 2 = register tile on a materialised H(k), N <= 64);
  * kind 1: rho_k (0 = V = QZ then V f V^H,
- * 1 = two-sided reflectors on Z f Z^T).  variant -1 returns to the tuned/default
+ * 1 = two-sided reflectors on Z f Z^T); kind 2: Ogita-Aishima sweep of the mixed path
+ * (0 = batched GEMM chain, 1 = one fused kernel per k-point, N <= 64, the default).
+ * variant -1 returns to the tuned/default
  * choice.  lqm_autotune times every variant of every kind on Nk device k-points of
  * the model and records the fastest per orbital count N in the context;
  * lqm_context_get_variant reports the choice in force (-1: default, not tuned). */

@@ -42,7 +42,7 @@ lqm_status fail(lqm_status st, const std::string& msg) {
 enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
-    W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_NSLOTS
+    W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -68,11 +68,11 @@ struct TimedLaunch {
 };
 
 // Kernel variants the autotuner chooses between (per N).
-enum VariantKind { VK_TRIDIAG = 0, VK_RHO = 1, VK_N };
+enum VariantKind { VK_TRIDIAG = 0, VK_RHO = 1, VK_REFINE = 2, VK_N };
 
 struct lqm_context {
     std::map<int, int> tuned;  // (kind << 20 | N) -> variant
-    int forced[VK_N] = {-1, -1};
+    int forced[VK_N] = {-1, -1, -1};
     int timing = 0;
     std::vector<TimedLaunch> pending;
     double stage_ms[ST_N] = {};
@@ -650,7 +650,52 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         std::vector<int> cur, next;
         const int max_sw = std::max(2, c->refine_max);
         hstat.resize((size_t)4 * Kc);
+        // refinement kind: 1 = one fused kernel per k-point and sweep (N <= 64, default),
+        // 0 = the GEMM chain below
+        const bool fused = N <= 64 && c->forced[VK_REFINE] != 0;
+        std::vector<int> clus;
         for (;;) {
+            bool have_stat = false;
+            if (fused) {
+                StageScope sc(c, ST_OA, nsub);
+                const size_t sh = lqm::oa_fused_lds_bytes();
+                allow_lds(lqm::k_oa_fused, sh);
+                hipLaunchKernelGGL(lqm::k_oa_fused, dim3(nsub), dim3(256), sh, c->stream, N, A, X, Sm, G, lam,
+                                   stat, c->refine_thr, kidx);
+                LQM_HIP(hipGetLastError());
+                // k-points with a cluster: general correction (Rayleigh-Ritz) on their S, G
+                LQM_HIP(hipMemcpyAsync(hstat.data(), stat, sizeof(double) * 4 * Kc, hipMemcpyDeviceToHost,
+                                       c->stream));
+                LQM_HIP(hipStreamSynchronize(c->stream));
+                clus.clear();
+                if (kidx) {
+                    for (int k : cur)
+                        if ((int)hstat[4 * k + 2] & 8) clus.push_back(k);
+                } else {
+                    for (int k = 0; k < Kc; ++k)
+                        if ((int)hstat[4 * k + 2] & 8) clus.push_back(k);
+                }
+                have_stat = clus.empty();
+                if (!clus.empty()) {
+                    int* dk2;
+                    if ((st = wsT(c, W_KIDX2, (size_t)Kc, &dk2))) return st;
+                    LQM_HIP(hipMemcpyAsync(dk2, clus.data(), sizeof(int) * clus.size(), hipMemcpyHostToDevice,
+                                           c->stream));
+                    const int nc = (int)clus.size();
+                    {
+                        StageScope sc(c, ST_OA, nc);
+                        const size_t sh = lqm::oa_lds_bytes(N);
+                        allow_lds(lqm::k_oa_correct, sh);
+                        hipLaunchKernelGGL(lqm::k_oa_correct, dim3(nc), dim3(256), sh, c->stream, N, Sm, G, W,
+                                           lam, stat, c->refine_thr, dk2);
+                    }
+                    LQM_HIP(hipGetLastError());
+                    zgemm<0, 0>(c, N, nc, X, W, G, one, one, X, dk2);
+                    const int bx = (int)std::min<size_t>(64, (NN + 255) / 256);
+                    hipLaunchKernelGGL(lqm::k_copy_sub, dim3(bx, nc), dim3(256), 0, c->stream, (int)NN, nc, dk2, G, X);
+                    LQM_HIP(hipGetLastError());
+                }
+            } else {
             zgemm<0, 0>(c, N, nsub, A, X, W, one, zero, nullptr, kidx);   // W = A X
             zgemm<1, 0>(c, N, nsub, X, W, Sm, one, zero, nullptr, kidx);  // S = X^H A X
             zgemm<1, 0>(c, N, nsub, X, X, G, one, zero, nullptr, kidx);   // G = X^H X
@@ -672,9 +717,13 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
             } else {
                 std::swap(X, G);
             }
+            }
             ++sweeps;
-            LQM_HIP(hipMemcpyAsync(hstat.data(), stat, sizeof(double) * 4 * Kc, hipMemcpyDeviceToHost, c->stream));
-            LQM_HIP(hipStreamSynchronize(c->stream));
+            if (!have_stat) {
+                LQM_HIP(hipMemcpyAsync(hstat.data(), stat, sizeof(double) * 4 * Kc, hipMemcpyDeviceToHost,
+                                       c->stream));
+                LQM_HIP(hipStreamSynchronize(c->stream));
+            }
             if (!kidx) {
                 cur.resize(Kc);
                 for (int k = 0; k < Kc; ++k) cur[k] = k;
@@ -1309,7 +1358,7 @@ lqm_status lqm_autotune(lqm_model* m, int Nk, const double* ks_dev) {
     if (Nk <= 0 || !ks_dev) return fail(LQM_ERR_ARG, "autotune needs a non-empty device k sample");
     lqm_context* c = m->ctx;
     const int N = m->N;
-    const int saved[VK_N] = {c->forced[0], c->forced[1]};
+    const int saved[VK_N] = {c->forced[0], c->forced[1], c->forced[2]};
     const int saved_timing = c->timing;
     c->timing = 0;
     double* rho;
@@ -1320,7 +1369,7 @@ lqm_status lqm_autotune(lqm_model* m, int Nk, const double* ks_dev) {
     LQM_HIP(hipEventCreate(&a));
     LQM_HIP(hipEventCreate(&b));
     // one kind at a time, the other left at its current choice
-    for (int kind = 0; kind < VK_N; ++kind) {
+    for (int kind = 0; kind <= VK_RHO; ++kind) {
         float best = 1e30f;
         int bestv = 0;
         const int vmax = kind == VK_TRIDIAG && N <= 64 ? 2 : 1;

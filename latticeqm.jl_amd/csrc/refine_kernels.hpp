@@ -579,3 +579,278 @@ __global__ __launch_bounds__(256) void k_copy_sub(int NN, int n, const int* __re
     (void)n;
 }
 }  // namespace lqm
+
+namespace lqm {
+
+// ---------------------------------------------------------------------------------
+// One Ogita-Aishima sweep of a k-point in a single kernel (N <= 64): X and W = A X
+// live in LDS (two padded 64 x 65 complex tiles, 130 KB), S = X^H W and G = X^H X in
+// the matrix-core accumulators, and the corrected X is written once.  A k-point whose
+// Rayleigh quotients hold a cluster (consecutive gap <= delta) is left to the general
+// correction: its S and G are written to global memory and flags bit 3 is set.
+// Out: X[kk] <- X + X E, lambda[kk*N + i] (Rayleigh quotient + second-order
+// correction), stat[kk*4 + {||R||_F, delta, flags, max |E_ij|}], flags as k_oa_correct
+// (bit 2: another sweep needed) plus bit 3.
+// ---------------------------------------------------------------------------------
+constexpr int kFusedLd = 65;
+inline size_t oa_fused_lds_bytes() { return sizeof(cx<double>) * 2 * 64 * kFusedLd + 8 * (64 + 64 + 2 * 64 + 32); }
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// acc[a][b][i] <-> element (m0 + 16a + g + 4i, n0 + 16b + li) of a 32 x 32 wave quadrant
+template <int OPA>
+__device__ __forceinline__ void fused_gemm(const cx<double>* Asrc, int ldA, bool global_a, int N,
+                                           const cx<double>* Bs, int m0, int n0, int li, int g,
+                                           lqm_d4 (&cr)[2][2], lqm_d4 (&ci)[2][2]) {
+    // C += op(A) B with op(A)(m, k) = A(m, k) (OPA 0) or conj(A(k, m)) (OPA 1);
+    // A column-major with leading dimension ldA (global: N, rows/cols >= N read as 0),
+    // B = Bs column-major in LDS with leading dimension kFusedLd (zero padded to 64).
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            cr[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+            ci[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+        }
+    for (int k0 = 0; k0 < 64; k0 += 16) {
+        const int kb = k0 + 4 * g;
+        double ar[2][4], ai[2][4], br[2][4], bi[2][4];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int m = m0 + 16 * t + li, n = n0 + 16 * t + li;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int k = kb + s;
+                cx<double> x;
+                if (OPA == 0) {
+                    if (global_a) x = (m < N && k < N) ? Asrc[(size_t)k * ldA + m] : cx<double>{0.0, 0.0};
+                    else x = Asrc[(size_t)k * ldA + m];
+                } else {
+                    x = Asrc[(size_t)m * ldA + k];
+                    x.im = -x.im;
+                }
+                const cx<double> y = Bs[(size_t)n * kFusedLd + k];
+                ar[t][s] = x.re;
+                ai[t][s] = x.im;
+                br[t][s] = y.re;
+                bi[t][s] = y.im;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a][s], br[b][s], cr[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a][s], bi[b][s], ci[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a][s], bi[b][s], cr[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a][s], br[b][s], ci[a][b], 0, 0, 0);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __restrict__ Ag,
+                                                  cx<double>* __restrict__ Xg,
+                                                  cx<double>* __restrict__ Sg,
+                                                  cx<double>* __restrict__ Gg,
+                                                  double* __restrict__ lambda,
+                                                  double* __restrict__ stat, double thr,
+                                                  const int* __restrict__ kidx) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<double>* Xs = reinterpret_cast<cx<double>*>(smem);  // [64][kFusedLd] column-major
+    cx<double>* Ws = Xs + 64 * kFusedLd;                     // W = A X, later E
+    double* lam = reinterpret_cast<double*>(Ws + 64 * kFusedLd);  // [64]
+    double* corr = lam + 64;                                       // [2][64]
+    double* red = corr + 128;                                      // [32]
+    __shared__ int s_cluster;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    const int wr = wave & 1, wc = wave >> 1;
+    const int m0 = 32 * wr, n0 = 32 * wc;
+    const size_t kk = kidx ? (size_t)kidx[blockIdx.x] : (size_t)blockIdx.x;
+    const size_t NN = (size_t)N * N;
+    const cx<double>* A = Ag + kk * NN;
+    cx<double>* X = Xg + kk * NN;
+
+    // X into LDS, zero padded to 64 x 64
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int r = e & 63, c = e >> 6;
+        Xs[c * kFusedLd + r] = (r < N && c < N) ? X[(size_t)c * N + r] : cx<double>{0.0, 0.0};
+    }
+    if (tid == 0) s_cluster = 0;
+    __syncthreads();
+
+    lqm_d4 ar_[2][2], ai_[2][2];
+    // W = A X
+    fused_gemm<0>(A, N, true, N, Xs, m0, n0, li, g, ar_, ai_);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = m0 + 16 * a + g + 4 * i, n = n0 + 16 * b + li;
+                Ws[n * kFusedLd + m] = cx<double>{ar_[a][b][i], ai_[a][b][i]};
+            }
+    __syncthreads();
+    // S = X^H W, G = X^H X
+    lqm_d4 sr[2][2], si[2][2], gr[2][2], gi[2][2];
+    fused_gemm<1>(Xs, kFusedLd, false, N, Ws, m0, n0, li, g, sr, si);
+    fused_gemm<1>(Xs, kFusedLd, false, N, Xs, m0, n0, li, g, gr, gi);
+
+    // Rayleigh quotients from the diagonal
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = m0 + 16 * a + g + 4 * i, n = n0 + 16 * b + li;
+                if (m == n && m < N) lam[m] = sr[a][b][i] / gr[a][b][i];
+            }
+    __syncthreads();  // lam ready; every wave is done reading Ws
+    double r2 = 0.0, s2 = 0.0, amax = 0.0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = m0 + 16 * a + g + 4 * i, n = n0 + 16 * b + li;
+                if (m < N && n < N) {
+                    const double rre = (m == n ? 1.0 : 0.0) - gr[a][b][i], rim = -gi[a][b][i];
+                    r2 += rre * rre + rim * rim;
+                    const double sre = sr[a][b][i] - (m == n ? lam[m] : 0.0), sim = si[a][b][i];
+                    s2 += sre * sre + sim * sim;
+                    if (m == n) amax = fmax(amax, fabs(lam[m]));
+                }
+            }
+    r2 = wave_sum(r2);
+    s2 = wave_sum(s2);
+    amax = wave_max(amax);
+    if (lane == 0) {
+        red[wave] = r2;
+        red[4 + wave] = s2;
+        red[8 + wave] = amax;
+    }
+    __syncthreads();
+    const double rn = sqrt(red[0] + red[1] + red[2] + red[3]);
+    const double an = fmax(fmax(red[8], red[9]), fmax(red[10], red[11]));
+    const double delta = 2.0 * (sqrt(red[4] + red[5] + red[6] + red[7]) + an * rn);
+    if (tid == 0) {
+        int cl = 0;
+        for (int i = 0; i + 1 < N; ++i) cl |= fabs(lam[i + 1] - lam[i]) <= delta;
+        s_cluster = cl;
+    }
+    __syncthreads();
+    if (s_cluster) {
+        // hand S and G to the general correction (Rayleigh-Ritz on the clusters)
+        cx<double>* Sk = Sg + kk * NN;
+        cx<double>* Gk = Gg + kk * NN;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int m = m0 + 16 * a + g + 4 * i, n = n0 + 16 * b + li;
+                    if (m < N && n < N) {
+                        Sk[(size_t)n * N + m] = cx<double>{sr[a][b][i], si[a][b][i]};
+                        Gk[(size_t)n * N + m] = cx<double>{gr[a][b][i], gi[a][b][i]};
+                    }
+                }
+        if (tid == 0) {
+            stat[kk * 4 + 0] = rn;
+            stat[kk * 4 + 1] = delta;
+            stat[kk * 4 + 2] = 8;
+            stat[kk * 4 + 3] = 0.0;
+        }
+        return;
+    }
+    // E and the second-order eigenvalue correction (column sums over m)
+    double emax = 0.0;
+    double cpart[2] = {0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = m0 + 16 * a + g + 4 * i, n = n0 + 16 * b + li;
+                cx<double> e = {0.0, 0.0};
+                if (m < N && n < N) {
+                    const double rre = (m == n ? 1.0 : 0.0) - gr[a][b][i], rim = -gi[a][b][i];
+                    if (m == n) {
+                        e = {0.5 * rre, 0.5 * rim};
+                    } else {
+                        const double ln = lam[n];
+                        const double gap = ln - lam[m];
+                        if (fabs(gap) > delta) {
+                            const double tre = sr[a][b][i] + ln * rre, tim = si[a][b][i] + ln * rim;
+                            e = {tre / gap, tim / gap};
+                            cpart[b] += (tre * tre + tim * tim) / gap;
+                        } else {
+                            e = {0.5 * rre, 0.5 * rim};
+                        }
+                        emax = fmax(emax, abs2(e));
+                    }
+                }
+                Ws[n * kFusedLd + m] = e;
+            }
+    // column partial sums: the four lane rows of the wave share a column, then the two
+    // waves of a column quadrant meet in LDS
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        double v = cpart[b];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        if (g == 0) corr[wr * 64 + n0 + 16 * b + li] = v;
+    }
+    emax = wave_max(emax);
+    if (lane == 0) red[12 + wave] = emax;
+    __syncthreads();
+    const double em = sqrt(fmax(fmax(red[12], red[13]), fmax(red[14], red[15])));
+    for (int n = tid; n < N; n += 256) lambda[kk * N + n] = lam[n] + (corr[n] + corr[64 + n]);
+    if (tid == 0) {
+        const double lead = fmax(em, rn);
+        int flags = 0;
+        if (thr <= 0.0 || lead * lead > thr) flags |= 4;
+        stat[kk * 4 + 0] = rn;
+        stat[kk * 4 + 1] = delta;
+        stat[kk * 4 + 2] = flags;
+        stat[kk * 4 + 3] = em;
+    }
+    // X <- X + X E
+    fused_gemm<0>(Xs, kFusedLd, false, N, Ws, m0, n0, li, g, ar_, ai_);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = m0 + 16 * a + g + 4 * i, n = n0 + 16 * b + li;
+                if (m < N && n < N) {
+                    const cx<double> x = Xs[n * kFusedLd + m];
+                    X[(size_t)n * N + m] = cx<double>{x.re + ar_[a][b][i], x.im + ai_[a][b][i]};
+                }
+            }
+}
+
+}  // namespace lqm

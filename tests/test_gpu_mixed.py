@@ -9,12 +9,16 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module", params=[-1, 2], ids=["tri-default", "tri-reg"])
+@pytest.fixture(scope="module", params=[(-1, -1), (2, -1), (-1, 0)],
+                ids=["default", "tri-reg", "refine-gemm-chain"])
 def mixed(pkg, gpu, request):
+    """Mixed-precision executor with (tridiagonalization, refinement sweep) variants forced."""
     ex = pkg.api.to_executor("gpu", precision="mixed")
-    ex.set_variant(0, request.param)
+    ex.set_variant(0, request.param[0])
+    ex.set_variant(2, request.param[1])
     yield ex
     ex.set_variant(0, -1)
+    ex.set_variant(2, -1)
 
 
 def _hnorm(h, ks):
