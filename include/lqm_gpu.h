@@ -82,7 +82,8 @@ const char *lqm_stage_name(int stage);
  * wave with the packed Hermitian tile, 2 = register tile on a materialised H(k), N <= 64);
  * kind 1: rho_k (0 = V = QZ then V f V^H,
  * 1 = two-sided reflectors on Z f Z^T); kind 2: Ogita-Aishima sweep of the mixed path
- * (0 = batched GEMM chain, 1 = one fused kernel per k-point, N <= 64, the default).
+ * (0 = batched GEMM chain, 1 = one fused kernel per k-point, N <= 64, the default);
+ * kind 3: k-points per block of the lane-per-k QL (64 >> variant, 0..3).
  * variant -1 returns to the tuned/default
  * choice.  lqm_autotune times every variant of every kind on Nk device k-points of
  * the model and records the fastest per orbital count N in the context;
@@ -256,6 +257,36 @@ lqm_status lqm_hartreefock_scf(lqm_context *ctx, int N, int d, int nRh, const in
                                double tol, int max_iter, int flags, double *result,
                                int *converged, double *history, int *nK_out, int32_t *RK_out,
                                double *hmf_out);
+
+/* Many independent Hartree-Fock problems of uneven size (no reference counterpart: the
+ * reference runs models one at a time).  Each problem is lqm_hartreefock_scf on its own
+ * model, k-grid and filling; T, beta, tol, max_iter, flags and d are shared.  The batch
+ * runs on nstreams (1..16) worker contexts, each with its own stream, fed from one queue
+ * in decreasing N^3 Nk order, so that small models run beside large ones.  Per problem:
+ * rho is the initial guess on entry and the result on return; result / converged as in
+ * lqm_hartreefock_scf; status = that problem's lqm_status.  Returns the first failure. */
+typedef struct {
+    int N;
+    int nRh;
+    const int32_t *Rh;
+    const double *Th;
+    int nRv;
+    const int32_t *Rv;
+    const double *Tv;
+    int nRr;
+    const int32_t *Rr;
+    double *rho;
+    int Nk;
+    const double *ks;
+    const double *kweights;
+    double filling;
+    double result[8];
+    int converged;
+    int status;
+} lqm_hf_problem;
+lqm_status lqm_hartreefock_scf_batch(lqm_context *ctx, int d, int M, lqm_hf_problem *problems,
+                                     double T, double beta, double tol, int max_iter, int flags,
+                                     int nstreams);
 
 #ifdef __cplusplus
 }

@@ -64,12 +64,20 @@ _PROTOS = {
     "lqm_grandpotential_dev": [P, I, P, P, D, D, P, P],
     "lqm_densitymatrix_occ": [P, I, P, P, P, I, P, P],
     "lqm_densitymatrix_occ_dev": [P, I, P, P, P, I, P, P, P],
+    "lqm_hartreefock_scf_batch": [P, I, I, P, D, D, D, I, I, I],
     "lqm_hartreefock_meanfield": [P, I, I, I, P, P, I, P, P, I, P, P, I, P, P, P, P],
     "lqm_hartreefock_scf": [P, I, I, I, P, P, I, P, P, I, P, P, I, P, P, D, D, D, D, I, I, P, P, P,
                             P, P, P],
 }
 _RESTYPE = {"lqm_last_error": ctypes.c_char_p, "lqm_abi_version": I, "lqm_stage_name": ctypes.c_char_p}
 NSTAGES = 18
+
+
+class HFProblem(ctypes.Structure):
+    """lqm_hf_problem of include/lqm_gpu.h."""
+    _fields_ = [("N", I), ("nRh", I), ("Rh", P), ("Th", P), ("nRv", I), ("Rv", P), ("Tv", P),
+                ("nRr", I), ("Rr", P), ("rho", P), ("Nk", I), ("ks", P), ("kweights", P),
+                ("filling", D), ("result", D * 8), ("converged", I), ("status", I)]
 LQM_HF_HARTREE = 1
 LQM_HF_FOCK = 2
 

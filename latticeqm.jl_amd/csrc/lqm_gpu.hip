@@ -6,11 +6,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lqm_gpu.h"
@@ -68,11 +70,11 @@ struct TimedLaunch {
 };
 
 // Kernel variants the autotuner chooses between (per N).
-enum VariantKind { VK_TRIDIAG = 0, VK_RHO = 1, VK_REFINE = 2, VK_N };
+enum VariantKind { VK_TRIDIAG = 0, VK_RHO = 1, VK_REFINE = 2, VK_QL = 3, VK_N };
 
 struct lqm_context {
     std::map<int, int> tuned;  // (kind << 20 | N) -> variant
-    int forced[VK_N] = {-1, -1, -1};
+    int forced[VK_N] = {-1, -1, -1, -1};
     int timing = 0;
     std::vector<TimedLaunch> pending;
     double stage_ms[ST_N] = {};
@@ -161,10 +163,6 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     p.shC = 2 * cR * (size_t)N * (N + 1) + smallC;
     p.ldsC = p.shC <= (size_t)c->lds_max;
     if (!p.ldsC) p.shC = smallC;
-    int L = 64;
-    while (L > 1 && 2 * cR * (size_t)N * L > (size_t)c->lds_max) L >>= 1;
-    p.laneB = L;
-    p.shB = 2 * cR * (size_t)N * L;
     p.cap_rot = 3 * N * N + 64 * N;
     p.cap_sw = 30 * N + 8;
     const size_t NN = (size_t)N * N;
@@ -182,6 +180,15 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
         auto it = c->tuned.find((kind << 20) | N);
         return it == c->tuned.end() ? 0 : it->second;
     };
+    // QL: k-points (lanes) per block, 64 >> variant; fewer lanes per wave put more
+    // independent recurrences on each SIMD
+    {
+        const int vq = variant(VK_QL);
+        int L = 64 >> std::max(0, std::min(3, vq < 0 ? 0 : vq));
+        while (L > 1 && 2 * cR * (size_t)N * L > (size_t)c->lds_max) L >>= 1;
+        p.laneB = L;
+        p.shB = 2 * cR * (size_t)N * L;
+    }
     // tridiagonalization: 0 = 256-thread LDS-tile kernel, 1 = one wave, packed tile,
     // 2 = register tile on a materialised H(k) (N <= 64)
     const int vt = variant(VK_TRIDIAG);
@@ -1187,7 +1194,8 @@ lqm_status lqm_context_refinement_report(lqm_context* c, double* out) {
 
 lqm_status lqm_context_set_variant(lqm_context* c, int kind, int variant) {
     if (!c) return fail(LQM_ERR_ARG, "null context");
-    if (kind < 0 || kind >= VK_N || variant < -1 || variant > (kind == VK_TRIDIAG ? 2 : 1))
+    if (kind < 0 || kind >= VK_N || variant < -1 ||
+        variant > (kind == VK_TRIDIAG ? 2 : kind == VK_QL ? 3 : 1))
         return fail(LQM_ERR_ARG, "bad variant");
     c->forced[kind] = variant;
     return LQM_OK;
@@ -1358,7 +1366,7 @@ lqm_status lqm_autotune(lqm_model* m, int Nk, const double* ks_dev) {
     if (Nk <= 0 || !ks_dev) return fail(LQM_ERR_ARG, "autotune needs a non-empty device k sample");
     lqm_context* c = m->ctx;
     const int N = m->N;
-    const int saved[VK_N] = {c->forced[0], c->forced[1], c->forced[2]};
+    const int saved[VK_N] = {c->forced[0], c->forced[1], c->forced[2], c->forced[3]};
     const int saved_timing = c->timing;
     c->timing = 0;
     double* rho;
@@ -1369,10 +1377,11 @@ lqm_status lqm_autotune(lqm_model* m, int Nk, const double* ks_dev) {
     LQM_HIP(hipEventCreate(&a));
     LQM_HIP(hipEventCreate(&b));
     // one kind at a time, the other left at its current choice
-    for (int kind = 0; kind <= VK_RHO; ++kind) {
+    for (int kind = 0; kind < VK_N; ++kind) {
+        if (kind == VK_REFINE) continue;  // a mixed-path choice with a fixed default
         float best = 1e30f;
         int bestv = 0;
-        const int vmax = kind == VK_TRIDIAG && N <= 64 ? 2 : 1;
+        const int vmax = kind == VK_TRIDIAG ? (N <= 64 ? 2 : 1) : kind == VK_QL ? 3 : 1;
         for (int v = 0; v <= vmax; ++v) {
             c->forced[kind] = v;
             float t = 0.f;
@@ -1397,8 +1406,7 @@ lqm_status lqm_autotune(lqm_model* m, int Nk, const double* ks_dev) {
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    c->forced[0] = saved[0];
-    c->forced[1] = saved[1];
+    for (int kind = 0; kind < VK_N; ++kind) c->forced[kind] = saved[kind];
     c->timing = saved_timing;
     return st;
 }
@@ -1690,6 +1698,59 @@ lqm_status lqm_bandmatrix_expval(lqm_model* m, int Nk, const double* ks, int nP,
         return st;
     if ((st = d2h(c, bands, db, sizeof(double) * N * Nk))) return st;
     if (nP > 0) return d2h(c, obs, dobs, sizeof(double) * N * Nk * nP);
+    return LQM_OK;
+}
+
+lqm_status lqm_hartreefock_scf_batch(lqm_context* ctx, int d, int M, lqm_hf_problem* problems,
+                                     double T, double beta, double tol, int max_iter, int flags,
+                                     int nstreams) {
+    if (!ctx) return fail(LQM_ERR_ARG, "null context");
+    if (M < 0 || (M > 0 && !problems)) return fail(LQM_ERR_ARG, "bad problem list");
+    if (nstreams < 1 || nstreams > 16) return fail(LQM_ERR_ARG, "nstreams must be in 1..16");
+    if (M == 0) return LQM_OK;
+    // longest first (N^3 Nk), so the large models start early and small ones fill in
+    std::vector<int> order(M);
+    for (int i = 0; i < M; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const double ca = std::pow((double)problems[a].N, 3) * problems[a].Nk;
+        const double cb = std::pow((double)problems[b].N, 3) * problems[b].Nk;
+        return ca > cb;
+    });
+    const int W = std::min(nstreams, M);
+    std::atomic<int> next{0};
+    std::vector<std::string> errs(W);
+    std::vector<int> errst(W, LQM_OK);
+    auto worker = [&](int w) {
+        lqm_context* wc = nullptr;
+        lqm_status st = lqm_context_create(ctx->device, ctx->precision, &wc);
+        if (st) {
+            errst[w] = st;
+            errs[w] = g_err;
+            return;
+        }
+        wc->refine_max = ctx->refine_max;
+        wc->refine_thr = ctx->refine_thr;
+        wc->tuned = ctx->tuned;
+        for (int k = 0; k < VK_N; ++k) wc->forced[k] = ctx->forced[k];
+        for (int i = next.fetch_add(1); i < M; i = next.fetch_add(1)) {
+            lqm_hf_problem& p = problems[order[i]];
+            p.converged = 0;
+            p.status = lqm_hartreefock_scf(wc, p.N, d, p.nRh, p.Rh, p.Th, p.nRv, p.Rv, p.Tv, p.nRr, p.Rr, p.rho,
+                                           p.Nk, p.ks, p.kweights, p.filling, T, beta, tol, max_iter, flags,
+                                           p.result, &p.converged, nullptr, nullptr, nullptr, nullptr);
+            if (p.status && errst[w] == LQM_OK) {
+                errst[w] = p.status;
+                errs[w] = g_err;
+            }
+        }
+        lqm_context_destroy(wc);
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < W; ++w) pool.emplace_back(worker, w);
+    worker(0);
+    for (auto& t : pool) t.join();
+    for (int w = 0; w < W; ++w)
+        if (errst[w] != LQM_OK) return fail((lqm_status)errst[w], "batch: " + errs[w]);
     return LQM_OK;
 }
 

@@ -291,3 +291,53 @@ def magnetization(rho: Hops, A=None) -> np.ndarray:
     if A is not None:
         ops = [A @ s for s in ops]
     return np.real(np.array([expval(rho, s) for s in ops]))
+
+
+def solvehartreefock_batch(problems, T: float = 0.0, beta: float = 1.0, tol: float = 1e-7,
+                           iterations: int = 500, hartree: bool = True, fock: bool = True,
+                           nstreams: int = 4, executor=None):
+    """Many independent Hartree-Fock SCFs of uneven size in one call.
+
+    ``problems``: sequence of (h, v, rho_init, filling, ks) tuples (ks: points, Mesh or an
+    int klin).  They run concurrently on ``nstreams`` device streams, largest first.
+    Returns, per problem, ``(rho, E_GS, info, converged, residual)`` with info holding mu,
+    epsband, epsH, epsF, epskin and the iteration count.
+    """
+    ex = to_executor(executor or "gpu")
+    keep = []
+    arr = (_lib.HFProblem * len(problems))()
+    d = None
+    for i, (h, v, rho_init, filling, ks) in enumerate(problems):
+        d = h.d if d is None else d
+        if h.d != d:
+            raise ValueError("all problems must have the same lattice dimension")
+        if isinstance(ks, int):
+            ks = regulargrid(nk=ks ** d, dim=d)
+        N = h.N
+        rho0 = sanitize(rho_init.copy())
+        Rh, Th = _pack_keys(h, N, d)
+        if len(v):
+            Rv, Tv = _pack_keys(v, N, d)
+        else:
+            Rv, Tv = np.zeros((d, 1), np.int32), np.zeros((N, N, 1), np.complex128, order="F")
+        Rr, Tr = _pack_keys(rho0, N, d)
+        k = _kpts(ks, d)
+        w = _weights(ks, None)
+        keep.append((Rh, Th, Rv, Tv, Rr, Tr, k, w))
+        p = arr[i]
+        p.N, p.nRh, p.Rh, p.Th = N, Rh.shape[1], Rh.ctypes.data, Th.ctypes.data
+        p.nRv, p.Rv, p.Tv = len(v), Rv.ctypes.data, Tv.ctypes.data
+        p.nRr, p.Rr, p.rho = Rr.shape[1], Rr.ctypes.data, Tr.ctypes.data
+        p.Nk, p.ks, p.kweights = k.shape[1], k.ctypes.data, (w.ctypes.data if w is not None else None)
+        p.filling = float(filling)
+    flags = (_lib.LQM_HF_HARTREE if hartree else 0) | (_lib.LQM_HF_FOCK if fock else 0)
+    _lib.call("lqm_hartreefock_scf_batch", ex.handle, int(d or 2), len(problems), arr, float(T), float(beta),
+              float(tol), int(iterations), flags, int(nstreams))
+    out = []
+    for i, p in enumerate(arr):
+        Rr, Tr = keep[i][4], keep[i][5]
+        rho = sanitize(Hops({tuple(int(x) for x in Rr[:, r]): Tr[:, :, r] for r in range(Rr.shape[1])}))
+        E, mu, eb, eH, eF, ek, res, nit = list(p.result)
+        info = {"mu": mu, "epsband": eb, "epsH": eH, "epsF": eF, "epskin": ek, "iterations": int(nit)}
+        out.append((rho, E, info, bool(p.converged), res))
+    return out

@@ -210,3 +210,35 @@ def test_scf_errors(pkg, gpu, mfm):
         out = mfm.solvehartreefock(h, v, rho0, 0.5, klin=12, T=0.01, beta=0.1, tol=1e-12, iterations=2,
                                    executor=gpu)
     assert out[2].history.shape == (2, 3) and not out[3]
+
+
+def _spinful_random(pkg, n_sites, seed, U):
+    """n_sites x 2 spins: a random spinless model (x) sigma_0 plus onsite Hubbard (sigma_x form)."""
+    h = pkg.models.addspin(pkg.models.random_tb(n_sites, rmax=1, seed=seed), "spinhalf")
+    sx = np.array([[0, 1], [1, 0]], dtype=complex)
+    v = pkg.Hops({(0, 0): U * np.kron(np.eye(n_sites), sx)})
+    return h, v
+
+
+def test_scf_batch_uneven_models(pkg, gpu, mfm, hf):
+    # uneven models (N = 4 .. 40) in one batched call on several streams: every result equals
+    # the single-model SCF bit for bit (same kernels, same order) and the oracle to 1e-8
+    rng = np.random.default_rng(44)
+    probs = []
+    for i, n in enumerate([2, 20, 6, 12, 3, 9]):
+        h, v = _spinful_random(pkg, n, 60 + i, U=float(rng.uniform(1, 4)))
+        sub = np.arange(n) % 2
+        rho0 = mfm.staggered_guess(v, sub, m=0.2, noise=1e-3, seed=i)
+        probs.append((h, v, rho0, 0.5, pkg.regulargrid(nk=8 ** 2)))
+    kw = dict(T=0.02, beta=0.5, tol=1e-9, iterations=300)
+    out = mfm.solvehartreefock_batch(probs, nstreams=4, executor=gpu, **kw)
+    for (h, v, rho0, fill, ks), (rho, E, info, conv, res) in zip(probs, out):
+        single = mfm.solvehartreefock(h, v, rho0, fill, ks, executor=gpu, **kw)
+        assert E == single[1] and conv == single[3] and info["iterations"] == single[2].history.shape[0]
+        for R in rho.keys():
+            assert np.array_equal(rho[R], single[0][R])
+    h, v, rho0, fill, ks = probs[1]
+    ref = hf.scf(h.data, v.data, rho0.data, fill, ks, **kw)
+    assert abs(out[1][1] - ref["E"]) < 1e-8
+    for R in ref["rho"]:
+        assert np.max(np.abs(out[1][0][R] - ref["rho"][R])) < 1e-8
