@@ -57,7 +57,7 @@ def test_autotune_picks_valid_variants(pkg, oracle, gpu):
     m = pkg.api.GPUModel(h, gpu)
     ks_d = torch.from_numpy(np.ascontiguousarray(ks.T)).cuda()
     chosen = pkg.api.autotune(m, ks_d.data_ptr(), ks.shape[1])
-    assert chosen[0] in (0, 1, 2) and chosen[1] in (0, 1)
+    assert chosen[0] in (0, 1, 2) and chosen[1] in (0, 1) and chosen[3] in (0, 1, 2, 3)
     _rho_vs_oracle(pkg, oracle, h, ks, 0.0, 0.05)
 
 
