@@ -22,6 +22,7 @@
 #include "refine_kernels.hpp"
 #include "hf_kernels.hpp"
 #include "reg_kernels.hpp"
+#include "wy_kernels.hpp"
 
 using lqm::cx;
 
@@ -70,11 +71,11 @@ struct TimedLaunch {
 };
 
 // Kernel variants the autotuner chooses between (per N).
-enum VariantKind { VK_TRIDIAG = 0, VK_RHO = 1, VK_REFINE = 2, VK_QL = 3, VK_N };
+enum VariantKind { VK_TRIDIAG = 0, VK_RHO = 1, VK_REFINE = 2, VK_QL = 3, VK_BT = 4, VK_N };
 
 struct lqm_context {
     std::map<int, int> tuned;  // (kind << 20 | N) -> variant
-    int forced[VK_N] = {-1, -1, -1, -1};
+    int forced[VK_N] = {-1, -1, -1, -1, -1};
     int timing = 0;
     std::vector<TimedLaunch> pending;
     double stage_ms[ST_N] = {};
@@ -143,6 +144,8 @@ struct Plan {
     int nm = 0;   // register row length of k_replay (0: replay inside K_C)
     int nqA = 0;  // rows per lane of the one-wave tridiagonalization (0: 256-thread kernel)
     bool regA = false;  // register-tile tridiagonalization on a materialised H(k) (N <= 64)
+    bool wyC = false;   // compact-WY back-transformation possible (N <= 64, Z from the replay)
+    int btv = -1;       // back-transformation variant: 1 WY, 0 reflector loop, -1 WY for rho_k only
     size_t shAw = 0;
     bool rho2 = false;  // rho_k by two-sided reflectors on the packed Z f Z^T
     size_t shM = 0, shR2 = 0;
@@ -175,10 +178,12 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     if ((vectors || rho) && p.nm) perk += cR * NN;
     const size_t np = (size_t)N * (N + 1) / 2;
     p.shAw = cC * (np + 2 * N) + sizeof(cx<double>) * nR;
+    // forced variant, else the tuned one, else the default (-1 "automatic" for the
+    // back-transformation, 0 otherwise)
     auto variant = [&](int kind) {
         if (c->forced[kind] >= 0) return c->forced[kind];
         auto it = c->tuned.find((kind << 20) | N);
-        return it == c->tuned.end() ? 0 : it->second;
+        return it == c->tuned.end() ? (kind == VK_BT ? -1 : 0) : it->second;
     };
     // QL: k-points (lanes) per block, 64 >> variant; fewer lanes per wave put more
     // independent recurrences on each SIMD
@@ -197,6 +202,10 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
         p.regA = true;
         perk += 16 * NN;
     }
+    // back-transformation: 0 = reflector by reflector (LDS), 1 = compact WY blocks of 16 on
+    // the matrix cores (default for N <= 64, needs Z from the replay kernel)
+    p.wyC = p.nm && N <= 64 && lqm::wy_lds_bytes() <= (size_t)c->lds_max;
+    p.btv = variant(VK_BT);
     p.shM = sizeof(double) * ((size_t)N * (N + 1) + 2 + N + 4);
     p.shR2 = sizeof(cx<double>) * (np + 2 * N);
     // rho_k: 0 = V = QZ then V f V^H, 1 = two-sided reflectors on Z f Z^T
@@ -371,6 +380,24 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const R* ev
         R* Z;
         lqm_status st = wsT(c, W_Z, (size_t)N * N * Kc, &Z);
         if (st) return st;
+        // measured at N = 64: WY wins for rho_k (MODE 1), the reflector loop for plain vectors
+        if (P.wyC && (P.btv == 1 || (P.btv < 0 && MODE == 1))) {
+            switch (P.nm) {
+                case 8: launch_replay<R, 8>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+                case 16: launch_replay<R, 16>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+                case 32: launch_replay<R, 32>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+                default: launch_replay<R, 64>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+            }
+            LQM_HIP(hipGetLastError());
+            StageScope sc(c, ST_BACKTRANSFORM, Kc);
+            auto kW = lqm::k_backtransform_wy<R, MODE>;
+            const size_t sh = lqm::wy_lds_bytes();
+            allow_lds(kW, sh);
+            hipLaunchKernelGGL(kW, dim3(Kc), dim3(256), sh, c->stream, N, refl, tau, Z, evals, perm, kw, wuni, mu,
+                               T, c->occ, out, eb);
+            LQM_HIP(hipGetLastError());
+            return LQM_OK;
+        }
         switch (P.nm) {
             case 8: launch_replay<R, 8>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
             case 16: launch_replay<R, 16>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
@@ -1366,7 +1393,8 @@ lqm_status lqm_autotune(lqm_model* m, int Nk, const double* ks_dev) {
     if (Nk <= 0 || !ks_dev) return fail(LQM_ERR_ARG, "autotune needs a non-empty device k sample");
     lqm_context* c = m->ctx;
     const int N = m->N;
-    const int saved[VK_N] = {c->forced[0], c->forced[1], c->forced[2], c->forced[3]};
+    int saved[VK_N];
+    for (int kind = 0; kind < VK_N; ++kind) saved[kind] = c->forced[kind];
     const int saved_timing = c->timing;
     c->timing = 0;
     double* rho;
@@ -1382,6 +1410,7 @@ lqm_status lqm_autotune(lqm_model* m, int Nk, const double* ks_dev) {
         float best = 1e30f;
         int bestv = 0;
         const int vmax = kind == VK_TRIDIAG ? (N <= 64 ? 2 : 1) : kind == VK_QL ? 3 : 1;
+        if (kind == VK_BT && N > 64) continue;
         for (int v = 0; v <= vmax; ++v) {
             c->forced[kind] = v;
             float t = 0.f;

@@ -39,11 +39,14 @@ def test_square_triangular_analytic(pkg, gpu):
     assert np.max(np.abs(b[0] - tri)) < 1e-12
 
 
-@pytest.fixture(params=[0, 1, 2], ids=["tri-block", "tri-wave", "tri-reg"])
+@pytest.fixture(params=[(0, -1), (1, -1), (2, -1), (0, 1)], ids=["tri-block", "tri-wave", "tri-reg", "bt-wy"])
 def tri_variant(request, gpu):
-    gpu.set_variant(0, request.param)
+    """Tridiagonalization variant and back-transformation variant (-1: default, WY)."""
+    gpu.set_variant(0, request.param[0])
+    gpu.set_variant(4, request.param[1])
     yield request.param
     gpu.set_variant(0, -1)
+    gpu.set_variant(4, -1)
 
 
 @pytest.mark.parametrize("N,rmax,nk", [(1, 1, 9), (2, 1, 16), (3, 2, 16), (8, 1, 64), (17, 2, 25),

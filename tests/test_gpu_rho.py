@@ -27,14 +27,15 @@ def _rho_vs_oracle(pkg, oracle, h, ks, mu, T, kweights=None, keys=None, tol=1e-9
     return rho, eb
 
 
-@pytest.fixture(params=[(0, 0), (1, 1), (0, 1), (1, 0), (2, 0), (2, 1)], ids=lambda v: f"tri{v[0]}-rho{v[1]}")
+@pytest.fixture(params=[(0, 0, -1), (1, 1, -1), (0, 1, -1), (1, 0, -1), (2, 0, -1), (2, 1, -1), (0, 0, 0),
+                        (2, 0, 1)], ids=lambda v: f"tri{v[0]}-rho{v[1]}-bt{v[2]}")
 def variants(request, gpu):
-    """Force each (tridiagonalization, rho_k) kernel variant pair, then restore."""
-    gpu.set_variant(0, request.param[0])
-    gpu.set_variant(1, request.param[1])
+    """Force each (tridiagonalization, rho_k, back-transformation) kernel variant, then restore."""
+    for kind, v in zip((0, 1, 4), request.param):
+        gpu.set_variant(kind, v)
     yield request.param
-    gpu.set_variant(0, -1)
-    gpu.set_variant(1, -1)
+    for kind in (0, 1, 4):
+        gpu.set_variant(kind, -1)
 
 
 @pytest.mark.parametrize("N,nk,mu,T", [(2, 36, 0.0, 0.01), (4, 64, 0.3, 0.0), (8, 49, -0.2, 0.05),
