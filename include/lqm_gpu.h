@@ -85,7 +85,7 @@ const char *lqm_stage_name(int stage);
  * (0 = batched GEMM chain, 1 = one fused kernel per k-point, N <= 64, the default);
  * kind 3: k-points per block of the lane-per-k QL (64 >> variant, 0..3); kind 4:
  * back-transformation (0 = reflector by reflector, 1 = compact WY blocks on the matrix
- * cores, N <= 64; by default WY when forming rho_k and the loop for eigenvectors).
+ * cores, N <= 64, the default).
  * variant -1 returns to the tuned/default
  * choice.  lqm_autotune times every variant of every kind on Nk device k-points of
  * the model and records the fastest per orbital count N in the context;

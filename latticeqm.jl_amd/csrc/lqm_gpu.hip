@@ -145,7 +145,7 @@ struct Plan {
     int nqA = 0;  // rows per lane of the one-wave tridiagonalization (0: 256-thread kernel)
     bool regA = false;  // register-tile tridiagonalization on a materialised H(k) (N <= 64)
     bool wyC = false;   // compact-WY back-transformation possible (N <= 64, Z from the replay)
-    int btv = -1;       // back-transformation variant: 1 WY, 0 reflector loop, -1 WY for rho_k only
+    int btv = 1;        // back-transformation variant: 1 compact WY, 0 reflector loop
     size_t shAw = 0;
     bool rho2 = false;  // rho_k by two-sided reflectors on the packed Z f Z^T
     size_t shM = 0, shR2 = 0;
@@ -178,12 +178,12 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     if ((vectors || rho) && p.nm) perk += cR * NN;
     const size_t np = (size_t)N * (N + 1) / 2;
     p.shAw = cC * (np + 2 * N) + sizeof(cx<double>) * nR;
-    // forced variant, else the tuned one, else the default (-1 "automatic" for the
+    // forced variant, else the tuned one, else the default (1 = compact WY for the
     // back-transformation, 0 otherwise)
     auto variant = [&](int kind) {
         if (c->forced[kind] >= 0) return c->forced[kind];
         auto it = c->tuned.find((kind << 20) | N);
-        return it == c->tuned.end() ? (kind == VK_BT ? -1 : 0) : it->second;
+        return it == c->tuned.end() ? (kind == VK_BT ? 1 : 0) : it->second;
     };
     // QL: k-points (lanes) per block, 64 >> variant; fewer lanes per wave put more
     // independent recurrences on each SIMD
@@ -380,8 +380,7 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const R* ev
         R* Z;
         lqm_status st = wsT(c, W_Z, (size_t)N * N * Kc, &Z);
         if (st) return st;
-        // measured at N = 64: WY wins for rho_k (MODE 1), the reflector loop for plain vectors
-        if (P.wyC && (P.btv == 1 || (P.btv < 0 && MODE == 1))) {
+        if (P.wyC && P.btv == 1) {
             switch (P.nm) {
                 case 8: launch_replay<R, 8>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
                 case 16: launch_replay<R, 16>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;

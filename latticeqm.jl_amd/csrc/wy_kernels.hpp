@@ -3,10 +3,17 @@
 // The N - 1 Householder reflectors H_j = I - tau_j v_j v_j^H of the tridiagonalization are
 // applied to the QL rotation product Z in blocks of 16 in compact WY form (LAPACK zlarft,
 // forward, columnwise): Q_b = H_{16b} ... H_{16b+15} = I - Y T Y^H with T upper triangular,
-// T(i,i) = tau_i, T(0:i, i) = -tau_i T(0:i, 0:i) Y(:, 0:i)^H v_i.  The last block is applied
-// first: V <- V - (Y T)(Y^H V), two matrix-core products per block instead of a rank-1
-// update (and two barriers) per reflector.  V stays in LDS (FP64 complex, 64 x 65) for the
-// whole k-point; the reflectors and Z may be FP32 (mixed path) or FP64.
+// T(i,i) = tau_i, T(0:i, i) = -tau_i T(0:i, 0:i) P(0:i, i), P = Y^H Y.  The last block is
+// applied first: V <- V - (Y T)(Y^H V).
+//
+// V never leaves the registers: wave (wr, wc) of the 256-thread block holds the 32 x 32
+// quadrant V(32 wr + ., 32 wc + .) as matrix-core accumulators (lane l, element e of tile
+// (a, b): row 32 wr + 16 a + l/16 + 4 e, column 32 wc + 16 b + l%16).  Y^H V is formed from
+// those registers directly (contraction row r = 16 a + l/16 + 4 s in instruction s, which is
+// the row the lane already holds), the two row-halves meeting in LDS; P, U = Y T and the
+// update V -= U W are matrix-core products too.  LDS holds only the block's Y, U, W and P
+// (80 KB), so two k-points share a CU.  The reflectors and Z may be FP32 (mixed path) or
+// FP64; all arithmetic is FP64.
 //   MODE 0: write the eigenvectors, column n = V(:, perm[n]) (ascending eigenvalues);
 //   MODE 1: rho_k = V diag(f) V^H with f = fermidirac(e - mu; T) (or the given occupations)
 //           and eb[kk] = w_k sum_n e_n f_n, as k_backtransform MODE 1.
@@ -16,47 +23,61 @@
 
 namespace lqm {
 
-constexpr int kWyLd = 65;  // leading dimension of V in LDS
 constexpr int kWyB = 16;   // reflectors per block
+constexpr int kWyLd = 65;  // leading dimension of V when it is staged for rho_k (MODE 1)
 
+struct WySmem {
+    cx<double> Y[64 * kWyB];           // Y(r, i) at r * 16 + i
+    cx<double> U[64 * kWyB];           // U = Y T, same layout
+    cx<double> W[2][64 * (kWyB + 1)];  // row-half partials of W = Y^H V: W(i, c) at c * 17 + i
+    cx<double> P[2][kWyB * kWyB];      // row-half partials of P = Y^H Y: P(p, q) at q * 16 + p
+    cx<double> T[kWyB * kWyB];         // T(row, col) at row * 16 + col
+    cx<double> tau[kWyB];
+    int invp[64];                      // output column of each Z column (MODE 0)
+    double sf[64];                     // occupation of each Z column (MODE 1)
+    double red[8];
+};
 inline size_t wy_lds_bytes() {
-    return sizeof(cx<double>) * (64 * kWyLd + 64 * kWyB * 2 + 64 * (kWyB + 1) + kWyB * kWyB + kWyB) +
-           sizeof(double) * (64 + 8);
+    const size_t v = sizeof(cx<double>) * 64 * kWyLd;  // staged V of MODE 1 overlays Y, U, W
+    return sizeof(WySmem) > v ? sizeof(WySmem) : v;
 }
 
 template <typename R, int MODE>
-__global__ __launch_bounds__(256) void k_backtransform_wy(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_backtransform_wy(
     int N, const cx<R>* __restrict__ refl, const cx<R>* __restrict__ tau, const R* __restrict__ Zg,
     const R* __restrict__ evals, const int* __restrict__ perm, const double* __restrict__ kw,
     double wuni, double mu, double Temp, const double* __restrict__ occ, cx<double>* __restrict__ out,
     double* __restrict__ eb) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<double>* Vs = reinterpret_cast<cx<double>*>(smem);  // V(r, c) at c * kWyLd + r
-    cx<double>* Ys = Vs + 64 * kWyLd;                        // Y(r, i) at r * 16 + i
-    cx<double>* Us = Ys + 64 * kWyB;                         // U = Y T, same layout
-    cx<double>* Ws = Us + 64 * kWyB;                         // W = Y^H V: W(i, c) at c * 17 + i
-    cx<double>* Ps = Ws + 64 * (kWyB + 1);                   // P = Y^H Y: P(p, q) at q * 16 + p
-    cx<double>* Ts = Ps + kWyB * kWyB;                       // tau of the block (16)
-    double* sf = reinterpret_cast<double*>(Ts + kWyB);       // occupations per Z column
-    double* sred = sf + 64;
+    WySmem& S = *reinterpret_cast<WySmem*>(smem);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, g = lane >> 4;
+    const int wr = wave & 1, wc = wave >> 1;
+    const int m0 = 32 * wr, n0 = 32 * wc;
     const int kk = blockIdx.x;
     const size_t NN = (size_t)N * N;
     const cx<R>* rf = refl + (size_t)kk * NN;
 
-    // V = Z (real), zero padded to 64 x 64
-    const R* Z = Zg + (size_t)kk * NN;
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int r = e & 63, c = e >> 6;
-        Vs[c * kWyLd + r] = cx<double>{(r < N && c < N) ? (double)Z[(size_t)c * N + r] : 0.0, 0.0};
+    // V = Z (real) in the accumulator layout, zero padded to 64 x 64
+    lqm_d4 vr[2][2], vi[2][2];
+    {
+        const R* Z = Zg + (size_t)kk * NN;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int r = m0 + 16 * a + g + 4 * e, c = n0 + 16 * b + li;
+                    vr[a][b][e] = (r < N && c < N) ? (double)Z[(size_t)c * N + r] : 0.0;
+                    vi[a][b][e] = 0.0;
+                }
     }
     const int nrefl = N - 1;
     const int nb = (nrefl + kWyB - 1) / kWyB;
-    for (int b = nb - 1; b >= 0; --b) {
-        const int j0 = b * kWyB;
-        __syncthreads();  // V complete (load or previous block), Ys free
-        // (1) reflectors of the block: Y(r, i) = v_{j0+i}(r), zero above row j+1
+    for (int b0 = nb - 1; b0 >= 0; --b0) {
+        const int j0 = b0 * kWyB;
+        // (1) the block's reflectors: Y(r, i) = v_{j0+i}(r), zero above row j+1
         for (int e = tid; e < 64 * kWyB; e += 256) {
             const int r = e >> 4, i = e & 15;
             const int j = j0 + i;
@@ -65,7 +86,7 @@ __global__ __launch_bounds__(256) void k_backtransform_wy(
                 const cx<R> v = rf[(size_t)j * N + r];
                 y = {(double)v.re, (double)v.im};
             }
-            Ys[r * kWyB + i] = y;
+            S.Y[e] = y;
         }
         if (tid < kWyB) {
             const int j = j0 + tid;
@@ -74,154 +95,175 @@ __global__ __launch_bounds__(256) void k_backtransform_wy(
                 const cx<R> x = tau[(size_t)kk * N + j];
                 t = {(double)x.re, (double)x.im};
             }
-            Ts[tid] = t;
+            S.tau[tid] = t;
         }
         __syncthreads();
-        // (2) P = Y^H Y (VALU, one entry per thread) and W = Y^H V (matrix cores, wave w
-        //     owns columns 16w .. 16w + 15)
+        // (2) row-half partials W_wr = Y(R_wr, :)^H V(R_wr, C_wc) (B operand = the lane's own
+        //     accumulators) and, by the waves of column half 0, P_wr = Y(R_wr, :)^H Y(R_wr, :)
         {
-            const int p = tid & 15, q = tid >> 4;
-            cx<double> acc = {0.0, 0.0};
-            for (int r = 0; r < 64; ++r) acc = cmadc(acc, Ys[r * kWyB + p], Ys[r * kWyB + q]);
-            Ps[q * kWyB + p] = acc;
-        }
-        {
-            lqm_d4 wr = {0.0, 0.0, 0.0, 0.0}, wi = {0.0, 0.0, 0.0, 0.0};
-            const int c = 16 * wave + li;
-            for (int k0 = 0; k0 < 64; k0 += 16) {
+            lqm_d4 wr_[2], wi_[2];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                wr_[b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+                wi_[b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+            }
+            lqm_d4 pr = {0.0, 0.0, 0.0, 0.0}, pi = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
-                    const int r = k0 + 4 * g + s;
-                    const cx<double> y = Ys[r * kWyB + li];  // A(m = li, k = r) = conj(Y(r, li))
-                    const cx<double> v = Vs[c * kWyLd + r];  // B(k = r, n = c)
-                    wr = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, v.re, wr, 0, 0, 0);
-                    wi = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, v.im, wi, 0, 0, 0);
-                    wr = __builtin_amdgcn_mfma_f64_16x16x4f64(y.im, v.im, wr, 0, 0, 0);
-                    wi = __builtin_amdgcn_mfma_f64_16x16x4f64(-y.im, v.re, wi, 0, 0, 0);
-                }
-            }
+                    const int r = m0 + 16 * a + g + 4 * s;
+                    const cx<double> y = S.Y[r * kWyB + li];  // A(m = li, k = r) = conj(Y(r, li))
 #pragma unroll
-            for (int e = 0; e < 4; ++e) Ws[c * (kWyB + 1) + g + 4 * e] = cx<double>{wr[e], wi[e]};
+                    for (int b = 0; b < 2; ++b) {
+                        const double br = vr[a][b][s], bi = vi[a][b][s];
+                        wr_[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, br, wr_[b], 0, 0, 0);
+                        wi_[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, bi, wi_[b], 0, 0, 0);
+                        wr_[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(y.im, bi, wr_[b], 0, 0, 0);
+                        wi_[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-y.im, br, wi_[b], 0, 0, 0);
+                    }
+                    if (wc == 0) {
+                        const cx<double> q = S.Y[r * kWyB + li];  // B(k = r, n = li) = Y(r, li)
+                        pr = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, q.re, pr, 0, 0, 0);
+                        pi = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, q.im, pi, 0, 0, 0);
+                        pr = __builtin_amdgcn_mfma_f64_16x16x4f64(y.im, q.im, pr, 0, 0, 0);
+                        pi = __builtin_amdgcn_mfma_f64_16x16x4f64(-y.im, q.re, pi, 0, 0, 0);
+                    }
+                }
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    S.W[wr][(n0 + 16 * b + li) * (kWyB + 1) + g + 4 * e] = cx<double>{wr_[b][e], wi_[b][e]};
+            if (wc == 0) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) S.P[wr][li * kWyB + g + 4 * e] = cx<double>{pr[e], pi[e]};
+            }
         }
         __syncthreads();
-        // (3) T, row t held by lane t of wave 0: T(t, i) = -tau_i sum_{l=t}^{i-1} T(t, l) P(l, i)
+        // (3) W = W_0 + W_1 (in place in W[0]); T by lane t of wave 0 holding row t:
+        //     T(t, i) = -tau_i sum_{l=t}^{i-1} T(t, l) P(l, i)
+        for (int e = tid; e < 64 * (kWyB + 1); e += 256) S.W[0][e] = S.W[0][e] + S.W[1][e];
         if (wave == 0 && lane < kWyB) {
-            cx<double> trow[kWyB];
-#pragma unroll
+            // the lane's own row of T is built in LDS (a lane reads back only what it wrote)
+            cx<double>* trow = S.T + lane * kWyB;
+#pragma unroll 1
             for (int i = 0; i < kWyB; ++i) {
                 cx<double> acc = {0.0, 0.0};
-#pragma unroll
-                for (int l = 0; l < i; ++l)
-                    if (l >= lane) acc = cmad(acc, trow[l], Ps[i * kWyB + l]);
-                const cx<double> ti = Ts[i];
+                for (int l = lane; l < i; ++l) {
+                    const cx<double> p = S.P[0][i * kWyB + l] + S.P[1][i * kWyB + l];
+                    acc = cmad(acc, trow[l], p);
+                }
+                const cx<double> ti = S.tau[i];
                 trow[i] = lane < i ? cx<double>{0.0, 0.0} - ti * acc : (lane == i ? ti : cx<double>{0.0, 0.0});
             }
-            // U = Y T is formed from these rows below; park T in Ps's transpose slot (P is done)
-#pragma unroll
-            for (int i = 0; i < kWyB; ++i) Ps[lane * kWyB + i] = trow[i];  // T(lane, i)
         }
         __syncthreads();
-        // (4) U = Y T (64 x 16), four entries per thread
-        for (int e = tid; e < 64 * kWyB; e += 256) {
-            const int r = e >> 4, i = e & 15;
-            cx<double> acc = {0.0, 0.0};
-            for (int l = 0; l <= i; ++l) acc = cmad(acc, Ys[r * kWyB + l], Ps[l * kWyB + i]);
-            Us[r * kWyB + i] = acc;
-        }
-        __syncthreads();
-        // (5) V <- V - U W on the wave's 32 x 32 quadrant
+        // (4) U = Y T: wave w forms rows 16w .. 16w + 15 (one 16 x 16 tile, K = 16)
         {
-            const int m0 = 32 * (wave & 1), n0 = 32 * (wave >> 1);
-            lqm_d4 vr[2][2], vi[2][2];
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const cx<double> x = Vs[(n0 + 16 * bb + li) * kWyLd + m0 + 16 * a + g + 4 * e];
-                        vr[a][bb][e] = x.re;
-                        vi[a][bb][e] = x.im;
-                    }
+            lqm_d4 ur = {0.0, 0.0, 0.0, 0.0}, ui = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const int i = 4 * g + s;
-                double ur[2], ui[2], wr[2], wi[2];
+                const int l = 4 * g + s;
+                const cx<double> y = S.Y[(16 * wave + li) * kWyB + l];  // A(m = li, k = l)
+                const cx<double> t = S.T[l * kWyB + li];                // B(k = l, n = li) = T(l, li)
+                ur = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, t.re, ur, 0, 0, 0);
+                ui = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, t.im, ui, 0, 0, 0);
+                ur = __builtin_amdgcn_mfma_f64_16x16x4f64(-y.im, t.im, ur, 0, 0, 0);
+                ui = __builtin_amdgcn_mfma_f64_16x16x4f64(y.im, t.re, ui, 0, 0, 0);
+            }
 #pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    const cx<double> u = Us[(m0 + 16 * t + li) * kWyB + i];  // A(m, k = i) = -U
-                    const cx<double> w = Ws[(n0 + 16 * t + li) * (kWyB + 1) + i];
-                    ur[t] = -u.re;
-                    ui[t] = -u.im;
-                    wr[t] = w.re;
-                    wi[t] = w.im;
-                }
+            for (int e = 0; e < 4; ++e) S.U[(16 * wave + g + 4 * e) * kWyB + li] = cx<double>{ur[e], ui[e]};
+        }
+        __syncthreads();
+        // (5) V <- V - U W on the wave's quadrant (accumulators updated in place)
 #pragma unroll
-                for (int a = 0; a < 2; ++a)
+        for (int s = 0; s < 4; ++s) {
+            const int i = 4 * g + s;
+            double ur[2], ui[2], wre[2], wim[2];
 #pragma unroll
-                    for (int bb = 0; bb < 2; ++bb) {
-                        vr[a][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[a], wr[bb], vr[a][bb], 0, 0, 0);
-                        vi[a][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[a], wi[bb], vi[a][bb], 0, 0, 0);
-                    }
-#pragma unroll
-                for (int a = 0; a < 2; ++a)
-#pragma unroll
-                    for (int bb = 0; bb < 2; ++bb) {
-                        vr[a][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ui[a], wi[bb], vr[a][bb], 0, 0, 0);
-                        vi[a][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[a], wr[bb], vi[a][bb], 0, 0, 0);
-                    }
+            for (int t = 0; t < 2; ++t) {
+                const cx<double> u = S.U[(m0 + 16 * t + li) * kWyB + i];  // A(m, k = i) = -U
+                const cx<double> w = S.W[0][(n0 + 16 * t + li) * (kWyB + 1) + i];
+                ur[t] = -u.re;
+                ui[t] = -u.im;
+                wre[t] = w.re;
+                wim[t] = w.im;
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int bb = 0; bb < 2; ++bb)
+                for (int b = 0; b < 2; ++b) {
+                    vr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[a], wre[b], vr[a][b], 0, 0, 0);
+                    vi[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[a], wim[b], vi[a][b], 0, 0, 0);
+                }
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        Vs[(n0 + 16 * bb + li) * kWyLd + m0 + 16 * a + g + 4 * e] =
-                            cx<double>{vr[a][bb][e], vi[a][bb][e]};
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    vr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ui[a], wim[b], vr[a][b], 0, 0, 0);
+                    vi[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[a], wre[b], vi[a][b], 0, 0, 0);
+                }
         }
+        __syncthreads();  // Y, U, W are rewritten by the next block
     }
-    __syncthreads();
 
     const R* ev = evals + (size_t)kk * N;
     const int* pm = perm + (size_t)kk * N;
     if constexpr (MODE == 0) {
+        for (int n = tid; n < 64; n += 256) S.invp[n] = -1;
+        __syncthreads();
+        for (int n = tid; n < N; n += 256) S.invp[min(max(pm[n], 0), N - 1)] = n;  // perm is a permutation
+        __syncthreads();
         cx<double>* o = out + (size_t)kk * NN;
-        for (int e = tid; e < N * N; e += 256) {
-            const int i = e % N, n = e / N;
-            const int c = min(max(pm[n], 0), N - 1);  // perm is only garbage after a flagged NaN
-            o[e] = Vs[c * kWyLd + i];
-        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int r = m0 + 16 * a + g + 4 * e, c = n0 + 16 * b + li;
+                    if (r < N && c < N) {
+                        const int n = S.invp[c];
+                        if (n >= 0) o[(size_t)n * N + r] = cx<double>{vr[a][b][e], vi[a][b][e]};
+                    }
+                }
     } else {
-        // occupations per Z column and the band-energy term
-        for (int c = tid; c < 64; c += 256) sf[c] = 0.0;
+        // stage V for the product V f V^H (overlays Y, U, W), occupations per Z column
+        cx<double>* Vs = reinterpret_cast<cx<double>*>(smem);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    Vs[(n0 + 16 * b + li) * kWyLd + m0 + 16 * a + g + 4 * e] = cx<double>{vr[a][b][e], vi[a][b][e]};
+        for (int c = tid; c < 64; c += 256) S.sf[c] = 0.0;
         __syncthreads();
         double ebp = 0.0;
         for (int n = tid; n < N; n += 256) {
             const double en = (double)ev[n];
             const double f = occ ? occ[(size_t)kk * N + n] : fermidirac(en, mu, Temp);
-            sf[min(max(pm[n], 0), N - 1)] = f;
+            S.sf[min(max(pm[n], 0), N - 1)] = f;
             ebp += en * f;
         }
         ebp = wave_sum(ebp);
-        if (lane == 0) sred[wave] = ebp;
+        if (lane == 0) S.red[wave] = ebp;
         __syncthreads();
-        if (tid == 0) eb[kk] = (kw ? kw[kk] : wuni) * (sred[0] + sred[1] + sred[2] + sred[3]);
-        // rho_k = (V f) V^H on the wave's quadrant
-        const int m0 = 32 * (wave & 1), n0 = 32 * (wave >> 1);
+        if (tid == 0) eb[kk] = (kw ? kw[kk] : wuni) * (S.red[0] + S.red[1] + S.red[2] + S.red[3]);
         lqm_d4 cr[2][2], ci[2][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int bb = 0; bb < 2; ++bb) {
-                cr[a][bb] = lqm_d4{0.0, 0.0, 0.0, 0.0};
-                ci[a][bb] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+            for (int b = 0; b < 2; ++b) {
+                cr[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+                ci[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
             }
         for (int k0 = 0; k0 < 64; k0 += 16) {
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int c = k0 + 4 * g + s;
-                const double f = sf[c];
+                const double f = S.sf[c];
                 double ar[2], ai[2], br[2], bi[2];
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
@@ -235,16 +277,16 @@ __global__ __launch_bounds__(256) void k_backtransform_wy(
 #pragma unroll
                 for (int a = 0; a < 2; ++a)
 #pragma unroll
-                    for (int bb = 0; bb < 2; ++bb) {
-                        cr[a][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[bb], cr[a][bb], 0, 0, 0);
-                        ci[a][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[bb], ci[a][bb], 0, 0, 0);
+                    for (int b = 0; b < 2; ++b) {
+                        cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
+                        ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
                     }
 #pragma unroll
                 for (int a = 0; a < 2; ++a)
 #pragma unroll
-                    for (int bb = 0; bb < 2; ++bb) {
-                        cr[a][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[bb], cr[a][bb], 0, 0, 0);
-                        ci[a][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[bb], ci[a][bb], 0, 0, 0);
+                    for (int b = 0; b < 2; ++b) {
+                        cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[b], cr[a][b], 0, 0, 0);
+                        ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
                     }
             }
         }
@@ -252,11 +294,11 @@ __global__ __launch_bounds__(256) void k_backtransform_wy(
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int bb = 0; bb < 2; ++bb)
+            for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int m = m0 + 16 * a + g + 4 * e, n = n0 + 16 * bb + li;
-                    if (m < N && n < N) o[(size_t)n * N + m] = cx<double>{cr[a][bb][e], ci[a][bb][e]};
+                    const int m = m0 + 16 * a + g + 4 * e, n = n0 + 16 * b + li;
+                    if (m < N && n < N) o[(size_t)n * N + m] = cx<double>{cr[a][b][e], ci[a][b][e]};
                 }
     }
 }
