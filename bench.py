@@ -298,13 +298,21 @@ def main() -> int:
             return nk * 8.0 * N ** 3
         if stage.startswith("zgemm_"):
             return nk * 8.0 * N ** 3
+        if stage == "oa_correct":
+            # one Ogita-Aishima sweep: A X, X^H (A X), X^H X, X E per k-point refined, plus
+            # X f X^H for each k-point whose rho_k the fused sweep formed
+            fused_rho = report.get("kpoints_rho_fused", 0) / max(1.0, stmx[stage]["launches_per_step"])
+            return nk * 32.0 * N ** 3 + fused_rho * 8.0 * N ** 3
+        if stage == "herm_hk_fp64":
+            return nk * 8.0 * nR * N * N
         if stage == "phase_contract":
             return nk * 8.0 * nRr * N * N
         return 0.0
 
     dom = max(stmx, key=lambda k: stmx[k]["ms_per_step"])
     fl = flops(dom, stmx[dom]["kpoints_per_launch"])
-    peak = PEAK_FP64_TFLOPS if (dom.startswith("zgemm_") or dom == "phase_contract") else 157.3
+    # FP32 VALU kernels of the mixed eigensolver vs FP64 (matrix cores or VALU) elsewhere
+    peak = 157.3 if dom in ("hk_tridiag", "tridiag_ql", "replay") else PEAK_FP64_TFLOPS
     achieved = fl / (stmx[dom]["avg_launch_ms"] * 1e-3) / 1e12
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")

@@ -82,10 +82,11 @@ const char *lqm_stage_name(int stage);
  * wave with the packed Hermitian tile, 2 = register tile on a materialised H(k), N <= 64);
  * kind 1: rho_k (0 = V = QZ then V f V^H,
  * 1 = two-sided reflectors on Z f Z^T); kind 2: Ogita-Aishima sweep of the mixed path
- * (0 = batched GEMM chain, 1 = one fused kernel per k-point, N <= 64, the default);
+ * (0 = batched GEMM chain, 1 = one fused kernel per k-point, N <= 64, the default for
+ * N > 16);
  * kind 3: k-points per block of the lane-per-k QL (64 >> variant, 0..3); kind 4:
  * back-transformation (0 = reflector by reflector, 1 = compact WY blocks on the matrix
- * cores, N <= 64, the default).
+ * cores, N <= 64, the default for N > 16).
  * variant -1 returns to the tuned/default
  * choice.  lqm_autotune times every variant of every kind on Nk device k-points of
  * the model and records the fastest per orbital count N in the context;
@@ -103,7 +104,8 @@ lqm_status lqm_context_set_variant(lqm_context *ctx, int kind, int variant);
  * up to max(2, max_sweeps) sweeps (max_sweeps default 3).  The report of the last call:
  * out[0] sweeps used, out[1] max_k ||I - X^H X||_F, out[2] max_k delta (cluster
  * threshold), out[3] k-points still holding near-degenerate pairs, out[4] k-point
- * sweeps run (the sum over sweeps of the k-points refined).  Eigenvector output
+ * sweeps run (the sum over sweeps of the k-points refined), out[5] k-points whose rho_k
+ * the fused sweep formed in its epilogue (out must hold 6 doubles).  Eigenvector output
  * (bandmatrix with evecs) always uses the FP64 path. */
 lqm_status lqm_context_set_refinement(lqm_context *ctx, int max_sweeps);
 lqm_status lqm_context_set_refinement_threshold(lqm_context *ctx, double threshold);

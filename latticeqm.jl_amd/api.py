@@ -72,10 +72,11 @@ class GPUExec:
 
     def refinement_report(self) -> dict:
         """Diagnostics of the last mixed-precision call."""
-        out = (_c.c_double * 5)()
+        out = (_c.c_double * 6)()
         _lib.call("lqm_context_refinement_report", self.handle, out)
         return {"sweeps": int(out[0]), "max_orth_residual": out[1], "max_cluster_delta": out[2],
-                "kpoints_near_degenerate": int(out[3]), "kpoint_sweeps": int(out[4])}
+                "kpoints_near_degenerate": int(out[3]), "kpoint_sweeps": int(out[4]),
+                "kpoints_rho_fused": int(out[5])}
 
     def set_stream(self, stream_ptr: int | None) -> None:
         _lib.call("lqm_context_set_stream", self.handle, _c.c_void_p(stream_ptr or 0))

@@ -91,8 +91,9 @@ struct lqm_context {
     int* d_status = nullptr;
     int refine_max = 3;      // Ogita-Aishima sweeps allowed per call (mixed precision)
     double refine_thr = 1e-8;  // a k-point stops refining once its squared correction is below
-    double report[5] = {};   // last mixed call: sweeps, max ||R||_F, max delta, near-cluster k,
-                             // k-point sweeps run (sum over sweeps of the k-points refined)
+    double report[6] = {};   // last mixed call: sweeps, max ||R||_F, max delta, near-cluster k,
+                             // k-point sweeps run (sum over sweeps of the k-points refined),
+                             // k-points whose rho_k the fused sweep formed
     const double* occ = nullptr;  // given occupations of the chunk in flight (device), or null
 };
 
@@ -179,11 +180,11 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     const size_t np = (size_t)N * (N + 1) / 2;
     p.shAw = cC * (np + 2 * N) + sizeof(cx<double>) * nR;
     // forced variant, else the tuned one, else the default (1 = compact WY for the
-    // back-transformation, 0 otherwise)
+    // back-transformation when N > 16, 0 otherwise: the WY kernel works on a 64 x 64 tile)
     auto variant = [&](int kind) {
         if (c->forced[kind] >= 0) return c->forced[kind];
         auto it = c->tuned.find((kind << 20) | N);
-        return it == c->tuned.end() ? (kind == VK_BT ? 1 : 0) : it->second;
+        return it == c->tuned.end() ? (kind == VK_BT && N > 16 ? 1 : 0) : it->second;
     };
     // QL: k-points (lanes) per block, 64 >> variant; fewer lanes per wave put more
     // independent recurrences on each SIMD
@@ -629,6 +630,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
     const int N = m->N, d = m->d;
     if (N > 512) return fail(LQM_ERR_UNSUPPORTED, "mixed precision path supports N <= 512");
     c->report[4] = 0.0;
+    c->report[5] = 0.0;
     Plan P = make_plan(c, N, m->nR, Nk, true, false, sizeof(float));
     const size_t NN = (size_t)N * N;
     // refinement buffers: X, A, W, S, G (+ rho_k) in FP64 complex per k-point
@@ -683,10 +685,16 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         std::vector<int> cur, next;
         const int max_sw = std::max(2, c->refine_max);
         hstat.resize((size_t)4 * Kc);
-        // refinement kind: 1 = one fused kernel per k-point and sweep (N <= 64, default),
-        // 0 = the GEMM chain below
-        const bool fused = N <= 64 && c->forced[VK_REFINE] != 0;
+        // refinement kind: 1 = one fused kernel per k-point and sweep (N <= 64; the default
+        // for 16 < N <= 64, as it works on a 64 x 64 tile), 0 = the GEMM chain below
+        const bool fused = N <= 64 && (c->forced[VK_REFINE] == 1 || (c->forced[VK_REFINE] < 0 && N > 16));
         std::vector<int> clus;
+        // with rho or eband requested, a fused sweep forms rho_k (into Sm) and eb of every
+        // k-point it finishes; the rest go through k_scale_occ and the NC product below
+        double* eb = nullptr;
+        if ((rho || eband) && (st = wsT(c, W_EB, (size_t)P.Kc, &eb))) return st;
+        const double* occk = occ ? occ + (size_t)k0 * N : nullptr;
+        const lqm::OaRho rq{(rho || eband) ? 1 : 0, kwc, wuni, mu, T, occk, Sm, eb};
         for (;;) {
             bool have_stat = false;
             if (fused) {
@@ -694,7 +702,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
                 const size_t sh = lqm::oa_fused_lds_bytes();
                 allow_lds(lqm::k_oa_fused, sh);
                 hipLaunchKernelGGL(lqm::k_oa_fused, dim3(nsub), dim3(256), sh, c->stream, N, A, X, Sm, G, lam,
-                                   stat, c->refine_thr, kidx);
+                                   stat, c->refine_thr, kidx, rq);
                 LQM_HIP(hipGetLastError());
                 // k-points with a cluster: general correction (Rayleigh-Ritz) on their S, G
                 LQM_HIP(hipMemcpyAsync(hstat.data(), stat, sizeof(double) * 4 * Kc, hipMemcpyDeviceToHost,
@@ -795,15 +803,32 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
                                bands + (size_t)k0 * N);
         }
         if (rho || eband) {
-            double* eb;
-            if ((st = wsT(c, W_EB, (size_t)P.Kc, &eb))) return st;
+            // k-points whose rho_k the fused sweep did not form (all of them on the GEMM
+            // chain; clusters and k-points stopped at the sweep limit on the fused path)
+            int nrest = Kc;
+            const int* krest = nullptr;
+            if (fused) {
+                clus.clear();
+                for (int k = 0; k < Kc; ++k)
+                    if (!((int)hstat[4 * k + 2] & 16)) clus.push_back(k);
+                nrest = (int)clus.size();
+                c->report[5] += (double)(Kc - nrest);
+                if (nrest > 0 && nrest < Kc) {
+                    int* dk2;
+                    if ((st = wsT(c, W_KIDX2, (size_t)Kc, &dk2))) return st;
+                    LQM_HIP(hipMemcpyAsync(dk2, clus.data(), sizeof(int) * nrest, hipMemcpyHostToDevice,
+                                           c->stream));
+                    krest = dk2;
+                }
+            }
             {
                 StageScope sc(c, ST_REDUCE);
-                hipLaunchKernelGGL(lqm::k_scale_occ, dim3(Kc), dim3(256), sizeof(double) * N, c->stream, N,
-                                   X, lam, kwc, wuni, mu, T, occ ? occ + (size_t)k0 * N : nullptr, G, eb);
+                if (nrest > 0)
+                    hipLaunchKernelGGL(lqm::k_scale_occ, dim3(nrest), dim3(256), sizeof(double) * N, c->stream,
+                                       N, X, lam, kwc, wuni, mu, T, occk, G, eb, krest);
                 hipLaunchKernelGGL(lqm::k_sum, dim3(1), dim3(256), 0, c->stream, Kc, eb, eband, 1);
             }
-            zgemm<0, 1>(c, N, Kc, G, X, Sm, one, zero);  // rho_k = X f X^H
+            if (nrest > 0) zgemm<0, 1>(c, N, nrest, G, X, Sm, one, zero, nullptr, krest);  // rho_k = X f X^H
             if (rho && nRr > 0) {
                 const int Sc = std::min(S, Kc);
                 const int ksplit = (Kc + Sc - 1) / Sc;
@@ -1214,7 +1239,7 @@ lqm_status lqm_context_set_refinement_threshold(lqm_context* c, double thr) {
 
 lqm_status lqm_context_refinement_report(lqm_context* c, double* out) {
     if (!c || !out) return fail(LQM_ERR_ARG, "null argument");
-    for (int i = 0; i < 5; ++i) out[i] = c->report[i];
+    for (int i = 0; i < 6; ++i) out[i] = c->report[i];
     return LQM_OK;
 }
 

@@ -506,17 +506,24 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
 // Y = X diag(f), f_n = fermidirac(lambda_n - mu; T) -- or, when occ is given, the
 // occupation occ[kk*N + r] of the rank-r eigenvalue (lambda is in X's column order,
 // ties ranked by column); eb[kk] = w_k sum_n lambda_n f_n.  Dynamic LDS: N doubles.
+// Fermi-Dirac occupation of an eigenvalue (T = 0: step with 1/2 at lambda = mu).
+__device__ __forceinline__ double fermi_occ(double l, double mu, double T) {
+    const double x = l - mu;
+    return T > 0.0 ? 1.0 / (exp(x / T) + 1.0) : (x < 0.0 ? 1.0 : (x > 0.0 ? 0.0 : 0.5));
+}
+
 __global__ __launch_bounds__(256) void k_scale_occ(int N, const cx<double>* __restrict__ X,
                                                    const double* __restrict__ lambda,
                                                    const double* __restrict__ kw, double wuni,
                                                    double mu, double T,
                                                    const double* __restrict__ occ,
                                                    cx<double>* __restrict__ Y,
-                                                   double* __restrict__ eb) {
+                                                   double* __restrict__ eb,
+                                                   const int* __restrict__ kidx) {
     extern __shared__ __align__(16) unsigned char smem[];
     double* sf = reinterpret_cast<double*>(smem);
     __shared__ double sred[4];
-    const int kk = blockIdx.x, tid = threadIdx.x;
+    const int kk = kidx ? kidx[blockIdx.x] : (int)blockIdx.x, tid = threadIdx.x;
     const double* lk = lambda + (size_t)kk * N;
     double e = 0.0;
     for (int n = tid; n < N; n += 256) {
@@ -530,8 +537,7 @@ __global__ __launch_bounds__(256) void k_scale_occ(int N, const cx<double>* __re
             }
             f = occ[(size_t)kk * N + min(rank, N - 1)];
         } else {
-            const double x = ln - mu;
-            f = T > 0.0 ? 1.0 / (exp(x / T) + 1.0) : (x < 0.0 ? 1.0 : (x > 0.0 ? 0.0 : 0.5));
+            f = fermi_occ(ln, mu, T);
         }
         sf[n] = f;
         e += ln * f;
@@ -590,10 +596,23 @@ namespace lqm {
 // correction: its S and G are written to global memory and flags bit 3 is set.
 // Out: X[kk] <- X + X E, lambda[kk*N + i] (Rayleigh quotient + second-order
 // correction), stat[kk*4 + {||R||_F, delta, flags, max |E_ij|}], flags as k_oa_correct
-// (bit 2: another sweep needed) plus bit 3.
+// (bit 2: another sweep needed) plus bit 3 and bit 4 (rho_k formed: with rq.want, a
+// k-point that needs no further sweep writes rho_k[kk] and eb[kk], not X).
 // ---------------------------------------------------------------------------------
 constexpr int kFusedLd = 65;
-inline size_t oa_fused_lds_bytes() { return sizeof(cx<double>) * 2 * 64 * kFusedLd + 8 * (64 + 64 + 2 * 64 + 32); }
+inline size_t oa_fused_lds_bytes() { return sizeof(cx<double>) * 2 * 64 * kFusedLd + 8 * (64 + 2 * 64 + 32 + 64); }
+
+// The density-matrix request of a fused sweep: a k-point that needs no further sweep
+// forms rho_k = X f(lambda) X^H and eb[kk] = w_k sum_n lambda_n f_n right away (flags
+// bit 4) instead of writing X back; occupations as k_scale_occ.
+struct OaRho {
+    int want;
+    const double* kw;
+    double wuni, mu, T;
+    const double* occ;
+    cx<double>* rhok;
+    double* eb;
+};
 
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
@@ -672,13 +691,14 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
                                                   cx<double>* __restrict__ Gg,
                                                   double* __restrict__ lambda,
                                                   double* __restrict__ stat, double thr,
-                                                  const int* __restrict__ kidx) {
+                                                  const int* __restrict__ kidx, OaRho rq) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<double>* Xs = reinterpret_cast<cx<double>*>(smem);  // [64][kFusedLd] column-major
     cx<double>* Ws = Xs + 64 * kFusedLd;                     // W = A X, later E
     double* lam = reinterpret_cast<double*>(Ws + 64 * kFusedLd);  // [64]
     double* corr = lam + 64;                                       // [2][64]
     double* red = corr + 128;                                      // [32]
+    double* sf = red + 32;                                         // [64]
     __shared__ int s_cluster;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, g = lane >> 4;
@@ -827,18 +847,55 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
     if (lane == 0) red[12 + wave] = emax;
     __syncthreads();
     const double em = sqrt(fmax(fmax(red[12], red[13]), fmax(red[14], red[15])));
-    for (int n = tid; n < N; n += 256) lambda[kk * N + n] = lam[n] + (corr[n] + corr[64 + n]);
+    const double lead = fmax(em, rn);
+    const bool more = thr <= 0.0 || lead * lead > thr;
+    const bool form_rho = rq.want && !more;
+    __syncthreads();  // every wave has read lam[] and corr[]
+    for (int n = tid; n < 64; n += 256) {
+        const double ln = n < N ? lam[n] + (corr[n] + corr[64 + n]) : 0.0;
+        if (n < N) lambda[kk * N + n] = ln;
+        sf[n] = ln;  // the refined eigenvalue, read back for the occupations
+    }
     if (tid == 0) {
-        const double lead = fmax(em, rn);
-        int flags = 0;
-        if (thr <= 0.0 || lead * lead > thr) flags |= 4;
         stat[kk * 4 + 0] = rn;
         stat[kk * 4 + 1] = delta;
-        stat[kk * 4 + 2] = flags;
+        stat[kk * 4 + 2] = (more ? 4 : 0) | (form_rho ? 16 : 0);
         stat[kk * 4 + 3] = em;
     }
     // X <- X + X E
     fused_gemm<0>(Xs, kFusedLd, false, N, Ws, m0, n0, li, g, ar_, ai_);
+    if (!form_rho) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int m = m0 + 16 * a + g + 4 * i, n = n0 + 16 * b + li;
+                    if (m < N && n < N) {
+                        const cx<double> x = Xs[n * kFusedLd + m];
+                        X[(size_t)n * N + m] = cx<double>{x.re + ar_[a][b][i], x.im + ai_[a][b][i]};
+                    }
+                }
+        return;
+    }
+    // rho_k = X f X^H from the refined X, staged back into LDS (rows/columns >= N stay 0)
+    __syncthreads();  // every wave is done reading the old X; sf[] holds lambda
+    double fn = 0.0, ebp = 0.0;
+    if (tid < N) {
+        const double ln = sf[tid];
+        if (rq.occ) {
+            int rank = 0;
+            for (int j = 0; j < N; ++j) {
+                const double lj = sf[j];
+                rank += (lj < ln) || (lj == ln && j < tid);
+            }
+            fn = rq.occ[kk * N + min(rank, N - 1)];
+        } else {
+            fn = fermi_occ(ln, rq.mu, rq.T);
+        }
+        ebp = ln * fn;
+    }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -846,10 +903,64 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int m = m0 + 16 * a + g + 4 * i, n = n0 + 16 * b + li;
-                if (m < N && n < N) {
-                    const cx<double> x = Xs[n * kFusedLd + m];
-                    X[(size_t)n * N + m] = cx<double>{x.re + ar_[a][b][i], x.im + ai_[a][b][i]};
+                const cx<double> x = Xs[n * kFusedLd + m];
+                Xs[n * kFusedLd + m] = cx<double>{x.re + ar_[a][b][i], x.im + ai_[a][b][i]};
+            }
+    ebp = wave_sum(ebp);
+    if (lane == 0) red[16 + wave] = ebp;
+    __syncthreads();  // the new X is in LDS; every lambda has been read from sf[]
+    if (tid < 64) sf[tid] = fn;
+    if (tid == 0) rq.eb[kk] = (rq.kw ? rq.kw[kk] : rq.wuni) * (red[16] + red[17] + red[18] + red[19]);
+    __syncthreads();
+    lqm_d4 cr[2][2], ci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            cr[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+            ci[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+        }
+    // A(m, c) = X(m, c) f_c and B(c, n) = conj X(n, c); lane (li, g) supplies c = k0 + 4g + s
+    for (int k0 = 0; k0 < 64; k0 += 16) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int c = k0 + 4 * g + s;
+            const double f = sf[c];
+            double xr[2], xi[2], yr[2], yi[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const cx<double> x = Xs[c * kFusedLd + m0 + 16 * t + li];
+                const cx<double> y = Xs[c * kFusedLd + n0 + 16 * t + li];
+                xr[t] = f * x.re;
+                xi[t] = f * x.im;
+                yr[t] = y.re;
+                yi[t] = -y.im;
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[a], yr[b], cr[a][b], 0, 0, 0);
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[a], yi[b], ci[a][b], 0, 0, 0);
                 }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-xi[a], yi[b], cr[a][b], 0, 0, 0);
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[a], yr[b], ci[a][b], 0, 0, 0);
+                }
+        }
+    }
+    cx<double>* Rk = rq.rhok + kk * NN;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = m0 + 16 * a + g + 4 * i, n = n0 + 16 * b + li;
+                if (m < N && n < N) Rk[(size_t)n * N + m] = cx<double>{cr[a][b][i], ci[a][b][i]};
             }
 }
 

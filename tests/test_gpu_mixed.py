@@ -9,8 +9,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module", params=[(-1, -1), (2, -1), (-1, 0)],
-                ids=["default", "tri-reg", "refine-gemm-chain"])
+@pytest.fixture(scope="module", params=[(-1, -1), (2, -1), (-1, 0), (-1, 1)],
+                ids=["default", "tri-reg", "refine-gemm-chain", "refine-fused"])
 def mixed(pkg, gpu, request):
     """Mixed-precision executor with (tridiagonalization, refinement sweep) variants forced."""
     ex = pkg.api.to_executor("gpu", precision="mixed")
@@ -121,3 +121,19 @@ def test_adaptive_refinement_sweeps(pkg, oracle, mixed):
         mixed.set_refinement_threshold(1e-8)
     assert Nk <= out[1e-8] < 2 * Nk
     assert out[0.0] >= 2 * Nk
+
+
+@pytest.mark.parametrize("N,nk", [(10, 25), (40, 36)])
+def test_mixed_densitymatrix_occ(pkg, oracle, mixed, N, nk):
+    # given occupations by eigenvalue rank (the fused sweep ranks its refined eigenvalues)
+    h = pkg.models.random_tb(N, rmax=1, seed=40 + N)
+    ks = pkg.regulargrid(nk=nk)
+    R, Tt = h.pack()
+    b, V = oracle.bands(R, Tt, ks, vectors=True)
+    occ = np.random.default_rng(N).standard_normal(b.shape)
+    rhos = pkg.Hops({key: np.zeros((N, N), complex) for key in [(0, 0), (1, 0), (2, -1)]})
+    pkg.api.densitymatrix_occ(rhos, h, ks, occ, executor=mixed)
+    for key in rhos.keys():
+        ref = sum(np.exp(-2j * np.pi * np.dot(ks[:, j], key)) * (V[:, :, j] * occ[:, j]) @ V[:, :, j].conj().T
+                  for j in range(ks.shape[1])) / ks.shape[1]
+        assert np.max(np.abs(rhos[key] - ref)) < 1e-9
