@@ -70,8 +70,10 @@ lqm_status lqm_context_synchronize(lqm_context *ctx);
  * Stages: 0 hk_tridiag, 1 tridiag_ql, 2 replay, 3 backtransform, 4 zfz,
  * 5 rho_twosided, 6 phase_contract, 7 phase_fold, 8 dos, 9 reduce, 10 herm_hk_fp64,
  * 11 zgemm_nn_fp64, 12 zgemm_cn_fp64, 13 zgemm_nc_fp64, 14 oa_correct, 15 hf_meanfield,
- * 16 mix, 17 mu_bisect (LQM_NSTAGES = 18); lqm_stage_name gives the names. */
-#define LQM_NSTAGES 18
+ * 16 mix, 17 mu_bisect, 18 blocked_hk, 19 blocked_panel, 20 blocked_update,
+ * 21 blocked_bisect, 22 blocked_invit, 23 blocked_backtransform, 24 blocked_rho
+ * (LQM_NSTAGES = 25); lqm_stage_name gives the names. */
+#define LQM_NSTAGES 25
 lqm_status lqm_context_set_timing(lqm_context *ctx, int enable);
 lqm_status lqm_context_stage_times(lqm_context *ctx, double *ms, int64_t *launches,
                                    int64_t *kpoints, int n);
@@ -86,7 +88,12 @@ const char *lqm_stage_name(int stage);
  * N > 16);
  * kind 3: k-points per block of the lane-per-k QL (64 >> variant, 0..3); kind 4:
  * back-transformation (0 = reflector by reflector, 1 = compact WY blocks on the matrix
- * cores, N <= 64, the default for N > 16).
+ * cores, N <= 64, the default for N > 16); kind 5: eigensolver layout (0 = per-k-point
+ * kernels, 1 = the large-N path, the default for N > 512: blocked Householder
+ * tridiagonalization with zlatrd panels of 32 and matrix-core trailing updates, Sturm
+ * bisection, inverse iteration with clusters closer than 1e-5 ||T|| orthogonalised,
+ * compact-WY back-transformation and rho_k on the matrix cores; FP64 in both precision
+ * modes, N <= 4096; occupations below 1e-18 leave rho_k out).
  * variant -1 returns to the tuned/default
  * choice.  lqm_autotune times every variant of every kind on Nk device k-points of
  * the model and records the fastest per orbital count N in the context;

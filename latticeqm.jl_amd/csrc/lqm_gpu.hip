@@ -23,6 +23,7 @@
 #include "hf_kernels.hpp"
 #include "reg_kernels.hpp"
 #include "wy_kernels.hpp"
+#include "blocked_kernels.hpp"
 
 using lqm::cx;
 
@@ -45,7 +46,8 @@ lqm_status fail(lqm_status st, const std::string& msg) {
 enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
-    W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_NSLOTS
+    W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -55,14 +57,17 @@ constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch
 // Kernel stages timed with HIP events when timing is on.
 enum Stage {
     ST_TRIDIAG, ST_QL, ST_REPLAY, ST_BACKTRANSFORM, ST_ZFZ, ST_RHO2, ST_CONTRACT, ST_FOLD, ST_DOS,
-    ST_REDUCE, ST_HK64, ST_GEMM_NN, ST_GEMM_CN, ST_GEMM_NC, ST_OA, ST_HF, ST_MIX, ST_MU, ST_N
+    ST_REDUCE, ST_HK64, ST_GEMM_NN, ST_GEMM_CN, ST_GEMM_NC, ST_OA, ST_HF, ST_MIX, ST_MU, ST_BHK,
+    ST_BTRD, ST_BUPD, ST_BEIG, ST_BVEC, ST_BBT, ST_BRHO, ST_N
 };
 static_assert(ST_N == LQM_NSTAGES, "stage list and LQM_NSTAGES disagree");
 const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "backtransform", "zfz",
                                        "rho_twosided", "phase_contract", "phase_fold", "dos", "reduce",
                                        "herm_hk_fp64", "zgemm_nn_fp64", "zgemm_cn_fp64",
                                        "zgemm_nc_fp64", "oa_correct", "hf_meanfield", "mix",
-                                       "mu_bisect"};
+                                       "mu_bisect", "blocked_hk", "blocked_panel", "blocked_update",
+                                       "blocked_bisect", "blocked_invit", "blocked_backtransform",
+                                       "blocked_rho"};
 
 struct TimedLaunch {
     int stage;
@@ -71,11 +76,11 @@ struct TimedLaunch {
 };
 
 // Kernel variants the autotuner chooses between (per N).
-enum VariantKind { VK_TRIDIAG = 0, VK_RHO = 1, VK_REFINE = 2, VK_QL = 3, VK_BT = 4, VK_N };
+enum VariantKind { VK_TRIDIAG = 0, VK_RHO = 1, VK_REFINE = 2, VK_QL = 3, VK_BT = 4, VK_BLOCKED = 5, VK_N };
 
 struct lqm_context {
     std::map<int, int> tuned;  // (kind << 20 | N) -> variant
-    int forced[VK_N] = {-1, -1, -1, -1, -1};
+    int forced[VK_N] = {-1, -1, -1, -1, -1, -1};
     int timing = 0;
     std::vector<TimedLaunch> pending;
     double stage_ms[ST_N] = {};
@@ -498,6 +503,11 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
                      int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands,
                      const double* occ = nullptr);
 
+lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
+                       int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands,
+                       double* evecs, const double* occ);
+bool use_blocked(const lqm_context* c, int N);
+
 // Clears the context's occupation pointer when a pass ends, on every return path.
 struct OccScope {
     lqm_context* c;
@@ -509,6 +519,8 @@ lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, doub
     if (Nk < 0) return fail(LQM_ERR_ARG, "Nk < 0");
     if (Nk == 0) return LQM_OK;
     if (!ks || !bands) return fail(LQM_ERR_ARG, "null ks or bands");
+    if (use_blocked(m->ctx, N))
+        return run_blocked(m, Nk, ks, nullptr, 0.0, 0.0, 0, nullptr, nullptr, nullptr, bands, evecs, nullptr);
     // eigenvector output always comes from the FP64 path
     if (m->ctx->precision == LQM_PREC_FP32_REFINED && !evecs)
         return run_mixed(m, Nk, ks, nullptr, 0.0, 0.0, 0, nullptr, nullptr, nullptr, bands);
@@ -547,6 +559,9 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
     if ((st = wsT(c, W_RR, (size_t)std::max(1, nRr * d), &dRr))) return st;
     if (nRr > 0)
         LQM_HIP(hipMemcpyAsync(dRr, Rr, sizeof(int) * nRr * d, hipMemcpyHostToDevice, c->stream));
+    if (use_blocked(c, N))
+        return run_blocked(m, Nk, ks, kw, mu, T, nRr, dRr, reinterpret_cast<cx<double>*>(rho), eband,
+                           bands_out, nullptr, occ);
     if (c->precision == LQM_PREC_FP32_REFINED)
         return run_mixed(m, Nk, ks, kw, mu, T, nRr, dRr, reinterpret_cast<cx<double>*>(rho), eband,
                          bands_out, occ);
@@ -849,6 +864,269 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
     c->report[2] = dmax;
     c->report[3] = nnear;
     return check_status(c, "mixed-precision pass");
+}
+
+// ---- large N: blocked tridiagonalization, bisection, inverse iteration, compact WY ------
+constexpr size_t kBlockedBudget = size_t(40) << 30;  // bytes of per-k scratch per chunk
+constexpr int kBlockedMinN = 512;    // default: the blocked path above this N
+constexpr double kClusterRel = 1e-5;  // eigenvalues closer than this x ||T|| share a cluster
+constexpr double kOccTiny = 1e-18;    // occupations below this leave rho_k out
+
+bool use_blocked(const lqm_context* c, int N) {
+    const int f = c->forced[VK_BLOCKED];
+    if (f >= 0) return f == 1;
+    auto it = c->tuned.find((VK_BLOCKED << 20) | N);
+    if (it != c->tuned.end()) return it->second == 1;
+    return N > kBlockedMinN;
+}
+
+template <int OPA, int OPB>
+void zgemm_gen(lqm_context* c, int stage, long long kpts, int nb, const lqm::ZGemmArgs& g) {
+    if (g.M <= 0 || g.N <= 0 || nb <= 0) return;
+    StageScope sc(c, stage, kpts);
+    const dim3 grid((unsigned)((g.M + 63) / 64), (unsigned)((g.N + 63) / 64), (unsigned)nb);
+    hipLaunchKernelGGL((lqm::k_zgemm_gen<OPA, OPB>), grid, dim3(256), 0, c->stream, g);
+}
+
+lqm::ZGemmArgs gemm_args(int M, int N, int K, const cx<double>* A, int lda, long long sA,
+                         const cx<double>* B, int ldb, long long sB, cx<double>* C, int ldc, long long sC,
+                         cx<double> alpha, cx<double> beta) {
+    lqm::ZGemmArgs g{};
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.npair = 1;
+    g.A[0] = A;
+    g.B[0] = B;
+    g.A[1] = nullptr;
+    g.B[1] = nullptr;
+    g.lda = lda;
+    g.ldb = ldb;
+    g.ldc = ldc;
+    g.sA = sA;
+    g.sB = sB;
+    g.sC = sC;
+    g.sS = 0;
+    g.C = C;
+    g.alpha = alpha;
+    g.beta = beta;
+    g.ascale = nullptr;
+    return g;
+}
+
+// zhetrd (lower) of Kc full Hermitian matrices A with zlatrd panels: d, e, tau out, the
+// reflectors left in A below the diagonal.
+lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<double>* A, cx<double>* Vp, cx<double>* Wp,
+                           cx<double>* P, int nchmax, double* dd, double* ee, cx<double>* tau) {
+    hipStream_t s = c->stream;
+    const size_t NN = (size_t)N * N;
+    for (int p = 0; p < N - 1; p += lqm::kBnb) {
+        const int pw = std::min(lqm::kBnb, N - 1 - p);
+        {
+            StageScope sc(c, ST_BTRD, Kc);
+            for (int i = 0; i < pw; ++i) {
+                const int j = p + i, n = N - j - 1;
+                const int nch = (n + lqm::kHvCols - 1) / lqm::kHvCols;
+                hipLaunchKernelGGL(lqm::k_trd_col, dim3(Kc), dim3(256), 0, s, N, j, i, 0, A, Vp, Wp, dd, ee,
+                                   tau);
+                hipLaunchKernelGGL(lqm::k_trd_hemv, dim3((n + 255) / 256, nch, Kc), dim3(256), 0, s, N, j, i,
+                                   nchmax, A, Vp, P);
+                hipLaunchKernelGGL(lqm::k_trd_w, dim3(Kc), dim3(256), 0, s, N, j, i, nch, nchmax, P, Vp, Wp,
+                                   tau);
+            }
+        }
+        LQM_HIP(hipGetLastError());
+        // A22 -= V W^H + W V^H on rows/columns >= p + pw
+        const int q = p + pw, n = N - q;
+        lqm::ZGemmArgs g = gemm_args(n, n, pw, Vp + q, N, (long long)N * lqm::kBnb, Wp + q, N,
+                                     (long long)N * lqm::kBnb, A + (size_t)q * N + q, N, (long long)NN,
+                                     {-1.0, 0.0}, {1.0, 0.0});
+        g.npair = 2;
+        g.A[1] = Wp + q;
+        g.B[1] = Vp + q;
+        zgemm_gen<0, 1>(c, ST_BUPD, Kc, Kc, g);
+        LQM_HIP(hipGetLastError());
+    }
+    {
+        StageScope sc(c, ST_BTRD, Kc);
+        hipLaunchKernelGGL(lqm::k_trd_col, dim3(Kc), dim3(256), 0, s, N, N - 1, 0, 1, A, Vp, Wp, dd, ee, tau);
+    }
+    LQM_HIP(hipGetLastError());
+    return LQM_OK;
+}
+
+// One pass of the large-N FP64 path over the grid.  Outputs (any may be null): bands
+// (ascending), evecs (N x N per k-point), rho on keys dRr (accumulated; zeroed by the
+// caller) and eband.  occ: given occupations of the ascending eigenvalues (device).
+lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
+                       int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands,
+                       double* evecs, const double* occ) {
+    lqm_context* c = m->ctx;
+    const int N = m->N, d = m->d;
+    if (N > lqm::kBMaxN) return fail(LQM_ERR_UNSUPPORTED, "blocked eigensolver supports N <= 4096");
+    hipStream_t s = c->stream;
+    const size_t NN = (size_t)N * N;
+    const bool want_rho = rho != nullptr || eband != nullptr;
+    const bool vectors = want_rho || evecs != nullptr;
+    const int nchmax = std::max(1, (N - 1 + lqm::kHvCols - 1) / lqm::kHvCols);
+    const int nblk = std::max(1, (N - 1 + lqm::kBwy - 1) / lqm::kBwy);
+    size_t perk = 16 * NN + 2 * 16 * (size_t)N * lqm::kBnb + 16 * (size_t)nchmax * N + 64 * (size_t)N + 64;
+    if (vectors) perk += 8 * NN + 24 * NN + 16 * NN + 16 * (size_t)nblk * lqm::kBwy * lqm::kBwy + 2 * 16 * 64 * (size_t)N;
+    if (want_rho) perk += 16 * NN;
+    const int Kmax = (int)std::max<size_t>(1, std::min<size_t>({kBlockedBudget / perk, (size_t)Nk, (size_t)65535}));
+    const double wuni = 1.0 / (double)Nk;
+    lqm_status st;
+    const int RG = nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : 32;
+    const int groups = std::max(1, (nRr + RG - 1) / RG);
+    const int tilesE = (int)((NN + 255) / 256);
+    const int S = std::max(1, std::min(Kmax, 1024 / std::max(1, tilesE * groups)));
+    cx<double>* part = nullptr;
+    if (rho && nRr > 0 && (st = wsT(c, W_PART, (size_t)groups * S * RG * NN, &part))) return st;
+    std::vector<double> hlam, hnorm;
+    for (int k0 = 0; k0 < Nk; k0 += Kmax) {
+        const int Kc = std::min(Kmax, Nk - k0);
+        const double* ksc = ks + (size_t)k0 * d;
+        const double* kwc = kw ? kw + k0 : nullptr;
+        cx<double> *A, *Vp, *Wp, *P, *tau;
+        double* vec;
+        if ((st = wsT(c, W_BA, NN * Kmax, &A))) return st;
+        if ((st = wsT(c, W_BVW, (size_t)2 * N * lqm::kBnb * Kmax, &Vp))) return st;
+        Wp = Vp + (size_t)N * lqm::kBnb * Kmax;
+        if ((st = wsT(c, W_BP, (size_t)nchmax * N * Kmax, &P))) return st;
+        if ((st = wsT(c, W_BVEC, (size_t)(6 * N + 2) * Kmax, &vec))) return st;
+        double* dd = vec;
+        double* ee = dd + (size_t)N * Kmax;
+        double* lamw = ee + (size_t)N * Kmax;
+        double* fo = lamw + (size_t)N * Kmax;
+        tau = reinterpret_cast<cx<double>*>(fo + (size_t)N * Kmax);
+        double* tn = reinterpret_cast<double*>(tau + (size_t)N * Kmax);
+        double* lam = bands ? bands + (size_t)k0 * N : lamw;
+        // H(k), full Hermitian storage
+        {
+            StageScope sc(c, ST_BHK, Kc);
+            const int nt = (N + 31) / 32;
+            const size_t sh = sizeof(cx<double>) * lqm::kHkK * std::max(1, m->nR);
+            allow_lds(lqm::k_hk_full, sh);
+            hipLaunchKernelGGL(lqm::k_hk_full, dim3(nt * (nt + 1) / 2, (Kc + lqm::kHkK - 1) / lqm::kHkK), dim3(256),
+                               sh, s, N, d, m->nR, m->dR, m->dT, Kc, ksc, A);
+        }
+        LQM_HIP(hipGetLastError());
+        if ((st = blocked_tridiag(c, N, Kc, A, Vp, Wp, P, nchmax, dd, ee, tau))) return st;
+        {
+            StageScope sc(c, ST_BEIG, Kc);
+            hipLaunchKernelGGL(lqm::k_sturm_bisect, dim3((N + 255) / 256, Kc), dim3(256), 0, s, N, dd, ee, lam, tn);
+        }
+        LQM_HIP(hipGetLastError());
+        if (!vectors) continue;
+        // columns needed: all for eigenvectors or given occupations, else the eigenvalues
+        // whose occupation reaches kOccTiny (a prefix, f falling with the eigenvalue),
+        // extended to the end of any cluster the cut would split
+        int ncol = N;
+        if (!evecs && !occ) {
+            hlam.resize((size_t)Kc * N);
+            hnorm.resize(Kc);
+            LQM_HIP(hipMemcpyAsync(hlam.data(), lam, sizeof(double) * Kc * N, hipMemcpyDeviceToHost, s));
+            LQM_HIP(hipMemcpyAsync(hnorm.data(), tn, sizeof(double) * Kc, hipMemcpyDeviceToHost, s));
+            LQM_HIP(hipStreamSynchronize(s));
+            ncol = 0;
+            for (int k = 0; k < Kc; ++k) {
+                int nk = 0;
+                while (nk < N) {
+                    const double x = hlam[(size_t)k * N + nk] - mu;
+                    const double f = T > 0.0 ? 1.0 / (std::exp(x / T) + 1.0) : (x < 0.0 ? 1.0 : (x > 0.0 ? 0.0 : 0.5));
+                    if (!(f >= kOccTiny)) break;
+                    ++nk;
+                }
+                ncol = std::max(ncol, nk);
+            }
+            for (bool ext = ncol > 0; ext && ncol < N;) {
+                ext = false;
+                for (int k = 0; k < Kc && !ext; ++k)
+                    ext = hlam[(size_t)k * N + ncol] - hlam[(size_t)k * N + ncol - 1] <= kClusterRel * hnorm[k];
+                if (ext) ++ncol;
+            }
+        }
+        double* eb = nullptr;
+        if (want_rho) {
+            if ((st = wsT(c, W_EB, (size_t)Kmax, &eb))) return st;
+            StageScope sc(c, ST_REDUCE);
+            hipLaunchKernelGGL(lqm::k_occ_eb, dim3(Kc), dim3(256), 0, s, N, lam, kwc, wuni, mu, T,
+                               occ ? occ + (size_t)k0 * N : nullptr, fo, eb);
+            if (eband) hipLaunchKernelGGL(lqm::k_sum, dim3(1), dim3(256), 0, s, Kc, eb, eband, 1);
+            LQM_HIP(hipGetLastError());
+        }
+        if (ncol == 0) continue;  // every occupation below kOccTiny: rho_k = 0
+        const int ldz = ncol;
+        double *Zt, *scr;
+        if ((st = wsT(c, W_BZ, (size_t)N * ldz * Kmax, &Zt))) return st;
+        if ((st = wsT(c, W_BSC, (size_t)3 * N * ldz * Kmax, &scr))) return st;
+        {
+            StageScope sc(c, ST_BVEC, Kc);
+            hipLaunchKernelGGL(lqm::k_invit, dim3((ncol + 63) / 64, Kc), dim3(64), 0, s, N, ncol, ldz, dd, ee, lam,
+                               tn, kClusterRel, Zt, scr, k0, c->d_status);
+        }
+        LQM_HIP(hipGetLastError());
+        // V = Q Z: Y explicit in A, T per block of kBwy reflectors, blocks applied last first
+        cx<double>* V;
+        if (evecs) {
+            V = reinterpret_cast<cx<double>*>(evecs) + (size_t)k0 * NN;
+        } else if ((st = wsT(c, W_BV, (size_t)N * ncol * Kmax, &V))) {
+            return st;
+        }
+        cx<double> *Tw, *Wt;
+        if ((st = wsT(c, W_BT, (size_t)nblk * lqm::kBwy * lqm::kBwy * Kmax, &Tw))) return st;
+        if ((st = wsT(c, W_BWT, (size_t)2 * lqm::kBwy * ncol * Kmax, &Wt))) return st;
+        cx<double>* TW = Wt + (size_t)lqm::kBwy * ncol * Kmax;
+        {
+            StageScope sc(c, ST_BBT, Kc);
+            hipLaunchKernelGGL(lqm::k_make_y, dim3((unsigned)((NN + 255) / 256), Kc), dim3(256), 0, s, N, A);
+            if (N > 1)
+                hipLaunchKernelGGL(lqm::k_wy_T, dim3(nblk, Kc), dim3(256), 0, s, N, nblk, A, tau, Tw);
+            hipLaunchKernelGGL(lqm::k_z_to_v, dim3((N + 31) / 32, (ncol + 31) / 32, Kc), dim3(256), 0, s, N, ncol,
+                               ldz, Zt, V);
+        }
+        LQM_HIP(hipGetLastError());
+        const long long sV = (long long)N * ncol, sW = (long long)lqm::kBwy * ncol;
+        for (int b = (N > 1 ? nblk - 1 : -1); b >= 0; --b) {
+            const int jb = lqm::kBwy * b, nbb = std::min(lqm::kBwy, N - 1 - jb), r0 = jb + 1, Mr = N - r0;
+            const cx<double>* Yb = A + (size_t)jb * N + r0;
+            // Wt = Y^H V(r0:, :); TW = T Wt; V(r0:, :) -= Y TW
+            zgemm_gen<1, 0>(c, ST_BBT, Kc, Kc,
+                            gemm_args(nbb, ncol, Mr, Yb, N, (long long)NN, V + r0, N, sV, Wt, lqm::kBwy, sW,
+                                      {1.0, 0.0}, {0.0, 0.0}));
+            zgemm_gen<0, 0>(c, ST_BBT, Kc, Kc,
+                            gemm_args(nbb, ncol, nbb, Tw + (size_t)b * lqm::kBwy * lqm::kBwy, lqm::kBwy,
+                                      (long long)nblk * lqm::kBwy * lqm::kBwy, Wt, lqm::kBwy, sW, TW, lqm::kBwy, sW,
+                                      {1.0, 0.0}, {0.0, 0.0}));
+            zgemm_gen<0, 0>(c, ST_BBT, Kc, Kc,
+                            gemm_args(Mr, ncol, nbb, Yb, N, (long long)NN, TW, lqm::kBwy, sW, V + r0, N, sV,
+                                      {-1.0, 0.0}, {1.0, 0.0}));
+        }
+        LQM_HIP(hipGetLastError());
+        if (!rho || nRr <= 0) continue;
+        // rho_k = V diag(f) V^H, then the phase contraction onto the keys
+        cx<double>* rhok;
+        if ((st = wsT(c, W_RHOK, NN * Kmax, &rhok))) return st;
+        {
+            lqm::ZGemmArgs g = gemm_args(N, N, ncol, V, N, sV, V, N, sV, rhok, N, (long long)NN, {1.0, 0.0},
+                                         {0.0, 0.0});
+            g.ascale = fo;
+            g.sS = N;
+            zgemm_gen<0, 1>(c, ST_BRHO, Kc, Kc, g);
+        }
+        const int Sc = std::min(S, Kc);
+        const int ksplit = (Kc + Sc - 1) / Sc;
+        const int Sused = (Kc + ksplit - 1) / ksplit;
+        switch (RG) {
+            case 1: launch_contract<1>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho); break;
+            case 4: launch_contract<4>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho); break;
+            case 8: launch_contract<8>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho); break;
+            case 16: launch_contract<16>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho); break;
+            default: launch_contract<32>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho); break;
+        }
+        LQM_HIP(hipGetLastError());
+    }
+    return check_status(c, "blocked eigensolver");
 }
 
 lqm_status dos_bands_dev(lqm_context* c, int N, int Nk, const double* bands, const double* kw,

@@ -106,3 +106,31 @@ def honeycomb_hubbard(U: float = 3.0, t: float = -1.0):
     sx = np.array([[0, 1], [1, 0]], dtype=np.complex128)
     v = Hops({(0, 0): U * np.kron(np.eye(2), sx)})
     return h, v, np.array([0, 1])
+
+
+def tbg_slater_koster(m: int = 11, a: float = 1.42, z: float = 3.35, vpppi: float = -2.7,
+                      vppsigma: float = 0.48, decay: float = 0.184, cutoff: float = 3.0) -> Hops:
+    """Twisted bilayer graphene of twist index m (4(3m^2 + 3m + 1) orbitals; m = 11: 1588)
+    from honeycomb_twisted(m; a, z) (Structure/Geometries/2d.jl:222-227,
+    Structure/Lattices/twist.jl:45-173) with the two-centre Slater-Koster hopping
+    t(d) = vpppi e^{-(|d| - a)/delta} (1 - c^2) + vppsigma e^{-(|d| - z)/delta} c^2,
+    c = d_z/|d|, delta = decay * a, zero beyond cutoff * a and on site, over the images
+    R in {-1, 0, 1}^2 (empty blocks trimmed).  a is the C-C distance, as the lattice
+    generator takes it."""
+    from . import lattice
+
+    lat = lattice.honeycomb_twisted(m, a=a, z=z)
+    delta = decay * a
+    dmax = cutoff * a
+
+    def t(dr):
+        D = np.sqrt(np.sum(dr * dr, axis=0))
+        ok = (D > 1e-6) & (D <= dmax)
+        Ds = np.where(ok, D, 1.0)
+        c2 = dr[2] ** 2 / Ds ** 2
+        v = vpppi * np.exp(-(Ds - a) / delta) * (1.0 - c2) + vppsigma * np.exp(-(Ds - z) / delta) * c2
+        return np.where(ok, v, 0.0)
+
+    t.vectorized = True
+    half = [(0, 0), (1, 0), (0, 1), (1, 1), (1, -1)]
+    return lattice.gethops(lat, t, neighbors=half, precision=0.0)

@@ -1,0 +1,161 @@
+"""Large-N eigensolver (blocked Householder tridiagonalization, Sturm bisection, inverse
+iteration, compact-WY back-transformation) vs the FP64 oracle (LAPACK zheevr).
+
+Bars: eigenvalues |e - e64| <= 1e-10 max(|e64|, ||H(k)||_2); rho to 1e-9 absolute;
+E_band to 1e-10 relative; eigenvectors: residual ||H v - e v|| <= 1e-10 ||H||,
+||V^H V - I|| <= 1e-10, and equal to the oracle's up to a phase where the eigenvalue is
+isolated.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def blocked(pkg, gpu):
+    """FP64 executor with the large-N path forced for every N."""
+    ex = pkg.api.to_executor("gpu")
+    ex.set_variant(5, 1)
+    yield ex
+    ex.set_variant(5, -1)
+
+
+def _hnorm(h, ks):
+    return np.array([np.linalg.norm(h(k), 2) for k in ks.T])
+
+
+def _degenerate_model(pkg, n, seed):
+    return pkg.models.addspin(pkg.models.random_tb(n, rmax=1, seed=seed), "spinhalf")
+
+
+@pytest.mark.parametrize("N,nk", [(1, 4), (2, 9), (5, 16), (33, 9), (70, 9), (97, 4), (200, 4)])
+def test_blocked_bands(pkg, oracle, blocked, N, nk):
+    h = pkg.models.random_tb(N, rmax=1, seed=500 + N)
+    ks = pkg.regulargrid(nk=nk)
+    R, T = h.pack()
+    ref = oracle.bands(R, T, ks)
+    b, _ = pkg.api.bandmatrix(h, ks, executor=blocked)
+    err = np.abs(b - ref) / np.maximum(np.abs(ref), _hnorm(h, ks)[None, :])
+    assert np.max(err) < 1e-10, np.max(err)
+
+
+@pytest.mark.parametrize("N,nk,mu,T", [(5, 16, 0.1, 0.05), (70, 9, 0.0, 0.02), (130, 4, 0.2, 0.0)])
+def test_blocked_rho(pkg, oracle, blocked, N, nk, mu, T):
+    h = pkg.models.random_tb(N, rmax=1, seed=600 + N)
+    ks = pkg.regulargrid(nk=nk)
+    rho = pkg.Hops({R: np.zeros((N, N), complex) for R in h.keys()})
+    eb = pkg.api.getdensitymatrix(rho, h, ks, mu, T=T, executor=blocked)
+    R, Tt = h.pack()
+    ref, eb_ref = oracle.densitymatrix(R, Tt, ks, R, mu=mu, Temp=T)
+    err = max(np.max(np.abs(rho[tuple(R[:, r])] - ref[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+
+
+def test_blocked_degenerate_spectrum(pkg, oracle, blocked):
+    # exactly doubly degenerate levels (spin copies): clusters orthogonalised together
+    h = _degenerate_model(pkg, 40, 9)
+    ks = pkg.regulargrid(nk=4)
+    R, Tt = h.pack()
+    ref = oracle.bands(R, Tt, ks)
+    b, _ = pkg.api.bandmatrix(h, ks, executor=blocked)
+    assert np.max(np.abs(b - ref) / np.maximum(np.abs(ref), _hnorm(h, ks)[None, :])) < 1e-10
+    rho = pkg.Hops({key: np.zeros((80, 80), complex) for key in h.keys()})
+    eb = pkg.api.getdensitymatrix(rho, h, ks, 0.05, T=0.0, executor=blocked)
+    refr, eb_ref = oracle.densitymatrix(R, Tt, ks, R, mu=0.05, Temp=0.0)
+    err = max(np.max(np.abs(rho[tuple(R[:, r])] - refr[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+
+
+@pytest.mark.parametrize("N", [70, 150])
+def test_blocked_eigenvectors(pkg, oracle, blocked, N):
+    h = pkg.models.random_tb(N, rmax=1, seed=700 + N)
+    ks = pkg.regulargrid(nk=4)
+    b, V = pkg.api.eigen(h, ks, executor=blocked)
+    R, Tt = h.pack()
+    eb, Vr = oracle.bands(R, Tt, ks, vectors=True)
+    for j in range(ks.shape[1]):
+        H = h(ks[:, j])
+        Hn = np.linalg.norm(H, 2)
+        Vj = V[:, :, j]
+        assert np.max(np.abs(H @ Vj - Vj * b[:, j])) < 1e-10 * Hn
+        assert np.max(np.abs(Vj.conj().T @ Vj - np.eye(N))) < 1e-10
+        # isolated eigenvalues: the same vector up to a phase
+        gaps = np.diff(eb[:, j])
+        iso = np.ones(N, bool)
+        iso[:-1] &= gaps > 1e-6 * Hn
+        iso[1:] &= gaps > 1e-6 * Hn
+        ov = np.abs(np.sum(Vr[:, iso, j].conj() * Vj[:, iso], axis=0))
+        assert np.max(np.abs(ov - 1.0)) < 1e-9
+
+
+def test_blocked_densitymatrix_occ(pkg, oracle, blocked):
+    N = 60
+    h = pkg.models.random_tb(N, rmax=1, seed=88)
+    ks = pkg.regulargrid(nk=9)
+    R, Tt = h.pack()
+    b, V = oracle.bands(R, Tt, ks, vectors=True)
+    occ = np.random.default_rng(5).standard_normal(b.shape)
+    rhos = pkg.Hops({key: np.zeros((N, N), complex) for key in [(0, 0), (1, 0), (1, -1)]})
+    pkg.api.densitymatrix_occ(rhos, h, ks, occ, executor=blocked)
+    for key in rhos.keys():
+        ref = sum(np.exp(-2j * np.pi * np.dot(ks[:, j], key)) * (V[:, :, j] * occ[:, j]) @ V[:, :, j].conj().T
+                  for j in range(ks.shape[1])) / ks.shape[1]
+        assert np.max(np.abs(rhos[key] - ref)) < 1e-9
+
+
+def test_default_path_above_512(pkg, oracle, gpu):
+    # N > 512 takes the large-N path by default, in both precision modes
+    N = 600
+    h = pkg.models.random_tb(N, rmax=1, seed=3)
+    ks = pkg.regulargrid(nk=4)
+    R, Tt = h.pack()
+    ref, eb_ref = oracle.densitymatrix(R, Tt, ks, R, mu=0.0, Temp=0.02)
+    refb = oracle.bands(R, Tt, ks)
+    for ex in (gpu, pkg.api.to_executor("gpu", precision="mixed")):
+        b, _ = pkg.api.bandmatrix(h, ks, executor=ex)
+        assert np.max(np.abs(b - refb) / np.maximum(np.abs(refb), _hnorm(h, ks)[None, :])) < 1e-10
+        rho = pkg.Hops({key: np.zeros((N, N), complex) for key in h.keys()})
+        eb = pkg.api.getdensitymatrix(rho, h, ks, 0.0, T=0.02, executor=ex)
+        err = max(np.max(np.abs(rho[tuple(R[:, r])] - ref[:, :, r])) for r in range(R.shape[1]))
+        assert err < 1e-9, err
+        assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+
+
+def test_tbg_m11_slater_koster(pkg, oracle, gpu):
+    # the 1588-orbital twisted bilayer (twist index 11) at Gamma and a generic k-point
+    h = pkg.models.tbg_slater_koster(11)
+    N = h[(0, 0)].shape[0]
+    assert N == 1588
+    ks = np.array([[0.0, 0.0], [0.137, 0.291]]).T
+    R, Tt = h.pack()
+    refb = oracle.bands(R, Tt, ks)
+    b, _ = pkg.api.bandmatrix(h, ks, executor=gpu)
+    hn = _hnorm(h, ks)
+    assert np.max(np.abs(b - refb) / np.maximum(np.abs(refb), hn[None, :])) < 1e-10
+    keys = [(0, 0), (1, 0), (0, 1)]
+    Rr = np.array(keys).T
+    ref, eb_ref = oracle.densitymatrix(R, Tt, ks, Rr, mu=0.0, Temp=0.01)
+    rho = pkg.Hops({key: np.zeros((N, N), complex) for key in keys})
+    eb = pkg.api.getdensitymatrix(rho, h, ks, 0.0, T=0.01, executor=gpu)
+    err = max(np.max(np.abs(rho[key] - ref[:, :, r])) for r, key in enumerate(keys))
+    assert err < 1e-9, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+
+
+def test_blocked_expectation_values(pkg, oracle, blocked):
+    # projector expectation values on the large-N eigenvectors
+    N = 70
+    h = pkg.models.random_tb(N, rmax=1, seed=71)
+    ks = pkg.regulargrid(nk=4)
+    M = np.diag(np.linspace(-1.0, 1.0, N)).astype(complex)
+    b, obs = pkg.api.bandmatrix(h, ks, M, executor=blocked)
+    R, T = h.pack()
+    rb, V = oracle.bands(R, T, ks, vectors=True)
+    assert np.max(np.abs(b - rb)) < 1e-10 * np.max(_hnorm(h, ks))
+    for j in range(ks.shape[1]):
+        v = V[:, :, j]
+        e0 = np.real(np.einsum("in,ij,jn->n", v.conj(), M, v))
+        assert np.max(np.abs(obs[:, j, 0] - e0)) < 1e-10
