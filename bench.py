@@ -151,6 +151,86 @@ def adjoint_leg(no_cpu: bool) -> dict:
     return out
 
 
+def tbg_leg(no_cpu: bool) -> dict:
+    """The 1588-orbital twisted bilayer (twist index 11, Slater-Koster hopping, 9 keys) on
+    the 12 x 12 grid with the FP64 large-N path, device-resident: an eigenvalue pass
+    (H(k) + eigvals) and a full pass (H(k) + eigh + rho on the 9 keys, mu = 0, kT = 0.01)."""
+    import torch
+
+    h = pkg.models.tbg_slater_koster(11)
+    N = h[(0, 0)].shape[0]
+    ks = pkg.regulargrid(nk=144)
+    Nk = ks.shape[1]
+    mu, T = 0.0, 0.01
+    R, Tt = h.pack()
+    nRr = R.shape[1]
+    Rr = np.asfortranarray(R.astype(np.int32))
+    dev = torch.device("cuda:0")
+    ks_d = torch.from_numpy(np.ascontiguousarray(ks.T)).to(dev)
+    ex = api.to_executor("gpu")
+    m = api.GPUModel(h, ex)
+    rho = torch.zeros((nRr, N, N, 2), dtype=torch.float64, device=dev)
+    eb = torch.zeros(1, dtype=torch.float64, device=dev)
+    bands = torch.zeros((Nk, N), dtype=torch.float64, device=dev)
+    L = _lib.lib()
+    vp = ctypes.c_void_p
+
+    def full():
+        _lib.check(L.lqm_densitymatrix_dev(m.handle, Nk, vp(ks_d.data_ptr()), None, mu, T, nRr,
+                                           Rr.ctypes.data_as(vp), vp(rho.data_ptr()), vp(eb.data_ptr()),
+                                           vp(bands.data_ptr())), "lqm_densitymatrix_dev")
+
+    def evals():
+        _lib.check(L.lqm_bandmatrix_dev(m.handle, Nk, vp(ks_d.data_ptr()), vp(bands.data_ptr()), None),
+                   "lqm_bandmatrix_dev")
+
+    def clock(fn, reps=2):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    te = clock(evals)
+    tf = clock(full)
+    _lib.call("lqm_context_set_timing", ex.handle, 1)
+    full()
+    ms = (ctypes.c_double * _lib.NSTAGES)()
+    cnt = (ctypes.c_int64 * _lib.NSTAGES)()
+    _lib.call("lqm_context_stage_times", ex.handle, ms, cnt, None, _lib.NSTAGES)
+    _lib.call("lqm_context_set_timing", ex.handle, 0)
+    stages = {L.lqm_stage_name(i).decode(): {"ms": ms[i], "launches": cnt[i]}
+              for i in range(_lib.NSTAGES) if cnt[i]}
+    fl_full = (64.0 / 3.0) * N ** 3 + 8.0 * (R.shape[1] + nRr) * N * N
+    fl_val = (16.0 / 3.0) * N ** 3 + 8.0 * R.shape[1] * N * N
+    out = {"workload": "tbg_m11_slater_koster_N1588_R9_k12x12", "N": N, "nR": R.shape[1], "nRrho": nRr,
+           "Nk": Nk, "mu": mu, "kT": T, "precision": "fp64",
+           "eigvals_pass": {"seconds": te, "kpoints_per_s": Nk / te,
+                            "path_tflops_per_s": Nk * fl_val / te / 1e12},
+           "full_pass": {"seconds": tf, "kpoints_per_s": Nk / tf, "path_tflops_per_s": Nk * fl_full / tf / 1e12},
+           "stages_full_pass_ms": stages}
+    if not no_cpu:
+        from oracle import oracle
+
+        nth = host_threads()
+        S = min(16, nth)
+        sub = np.asfortranarray(ks[:, :: max(1, Nk // S)][:, :S])
+        c0 = time.perf_counter()
+        oracle.densitymatrix(R, Tt, sub, Rr, mu=mu, Temp=T, nthreads=nth)
+        tc = time.perf_counter() - c0
+        out["cpu_baseline"] = {"kpoints_per_s": S / tc, "cores": nth, "kind": "port",
+                               "sample": f"{S} of the {Nk} k-points (one per core), full pass "
+                                         f"H(k)+zheevr('V')+rho on {nRr} keys"}
+        bref = oracle.bands(R, Tt, sub, nthreads=nth)
+        idx = np.arange(Nk)[:: max(1, Nk // S)][:S]
+        bg = bands.cpu().numpy()[idx].T
+        hn = np.array([np.linalg.norm(h(k), 2) for k in sub.T])
+        out["max_eigval_err_vs_oracle"] = float(np.max(np.abs(bg - bref) / np.maximum(np.abs(bref), hn[None, :])))
+    return out
+
+
 def batch_leg() -> dict:
     """Uneven models, scaled down from 256 x N in [16, 512]: 48 spinful random models with
     N = 2 x sites drawn log-uniform in [8, 96] (seed 4), onsite Hubbard U ~ U(1, 4) in the
@@ -198,6 +278,7 @@ def main() -> int:
     ap.add_argument("--no-scf", action="store_true", help="skip the Hartree-Fock SCF leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the grand-potential adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-SCF leg")
+    ap.add_argument("--no-tbg", action="store_true", help="skip the 1588-orbital twisted-bilayer leg")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after timing, copy this many MiB device-to-device once (HBM counter calibration)")
     args = ap.parse_args()
@@ -369,6 +450,7 @@ def main() -> int:
     scf = None if args.no_scf else scf_leg(args.no_cpu)
     adj = None if args.no_adjoint else adjoint_leg(args.no_cpu)
     bat = None if args.no_batch else batch_leg()
+    tbg = None if args.no_tbg else tbg_leg(args.no_cpu)
 
     line = {
         "metric": METRIC,
@@ -401,6 +483,7 @@ def main() -> int:
         "scf": scf,
         "adjoint": adj,
         "batch_scf": bat,
+        "tbg_large_n": tbg,
         **out,
     }
     print(json.dumps(line))
