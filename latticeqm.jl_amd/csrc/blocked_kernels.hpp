@@ -2,7 +2,7 @@
 // entry, all matrices column-major with leading dimension N:
 //   k_hk_full        A = Hermitian(H(k)) (upper triangle of H(k)), full storage
 //   k_trd_col        zlatrd column step j: pending panel update of column j, zlarfg
-//   k_trd_hemv       partial products of y = A22 v over column chunks
+//   k_trd_hemv       y = A22 v from the lower triangle, W^H v, V^H v, v^H A22 v partials
 //   k_trd_w          w = tau (A22 v - V (W^H v) - W (V^H v)) + alpha v (zlatrd)
 //   k_zgemm_gen      batched C = alpha sum_p op(A_p) op(B_p) + beta C on the FP64 matrix
 //                    cores: the trailing her2k update, compact-WY products, rho_k
@@ -14,7 +14,8 @@
 // The tridiagonalization is the one-stage blocked Householder reduction (LAPACK zhetrd,
 // lower, with zlatrd panels of kBnb columns): inside a panel the trailing matrix is left
 // as it was at the panel start and the pending rank-2i update enters every column step
-// through V and W; after the panel A22 -= V W^H + W V^H on the matrix cores.  The
+// through V and W; after the panel A22 -= V W^H + W V^H on the matrix cores (tiles on or
+// below the diagonal: only the lower triangle of A is kept up to date and read).  The
 // reflectors H_j = I - tau_j v_j v_j^H (v_j(j+1) = 1) give A = Q T Q^H, Q = H_0 ... H_{N-2}.
 #pragma once
 #include "lqm_common.hpp"
@@ -192,107 +193,180 @@ __global__ __launch_bounds__(256) void k_trd_col(int N, int j, int i, int last, 
     }
 }
 
-// Partial products of y = A22 v, A22 = A(j+1:N, j+1:N) as stored (the trailing matrix of
-// the panel start, both triangles): block (x, y, k) sums columns j+1 + 64y ... of rows
-// j+1 + 256x + t into P[k][y][row] (P stride per k-point: nchmax * N).
-__global__ __launch_bounds__(256) void k_trd_hemv(int N, int j, int i, int nchmax, const cx<double>* __restrict__ A,
+// Products of the column step j over the lower triangle of A22 = A(j+1:N, j+1:N) (only
+// elements on or below the diagonal are read; 64 x 64 tiles from row/column j+1).  Tile
+// column J is cut into parts of at most kHvPart tiles (I = J + kHvPart p ...), one
+// 512-thread block per (J, p, k); lane = row of the tile, wave w = columns w, w + 8, ...:
+//   P[k][J][r]       = sum_{c in J, c <= r} A(r, c) v_c           (r in the part's tiles)
+//   P2[k][p][c]      = sum_{r > c, r in the part} conj(A(r, c)) v_r   (c in J)
+//   SS[k][b]         = the block's share of s = v^H A22 v (real)
+//   Q[k][J][1 + t], Q[k][J][1 + kBnb + t] (part 0): W(:, t)^H v and V(:, t)^H v over the
+//   rows of tile J (t < i).
+constexpr int kQStride = 1 + 2 * kBnb;
+constexpr int kHvPart = 4;
+constexpr int kHvWaves = 8;
+__host__ __device__ __forceinline__ int hv_parts(int nt, int J) { return (nt - J + kHvPart - 1) / kHvPart; }
+// number of (J, p) blocks of a column step with nt tile columns
+__host__ __device__ __forceinline__ int hv_blocks(int nt) {
+    int b = 0;
+    for (int J = 0; J < nt; ++J) b += hv_parts(nt, J);
+    return b;
+}
+__global__ __launch_bounds__(512) void k_trd_hemv(int N, int j, int i, int nchmax, int nbmax,
+                                                  const cx<double>* __restrict__ A,
                                                   const cx<double>* __restrict__ V,
-                                                  cx<double>* __restrict__ P) {
+                                                  const cx<double>* __restrict__ W,
+                                                  cx<double>* __restrict__ P, cx<double>* __restrict__ P2,
+                                                  cx<double>* __restrict__ Q, double* __restrict__ SS) {
     __shared__ cx<double> sv[kHvCols];
-    const size_t kk = blockIdx.z;
-    const int c0 = j + 1 + blockIdx.y * kHvCols;
-    const int nc = min(kHvCols, N - c0);
-    const cx<double>* v = V + kk * (size_t)N * kBnb + (size_t)i * N;
-    if ((int)threadIdx.x < nc) sv[threadIdx.x] = v[c0 + threadIdx.x];
-    __syncthreads();
-    const int r = j + 1 + blockIdx.x * 256 + threadIdx.x;
-    if (r >= N) return;
-    const cx<double>* Ak = A + kk * (size_t)N * N + (size_t)c0 * N + r;
-    cx<double> a0 = {0.0, 0.0}, a1 = {0.0, 0.0};
-    int c = 0;
-#pragma unroll 4
-    for (; c + 1 < nc; c += 2) {
-        a0 = cmad(a0, Ak[(size_t)c * N], sv[c]);
-        a1 = cmad(a1, Ak[(size_t)(c + 1) * N], sv[c + 1]);
+    __shared__ cx<double> red[2][kHvWaves][64];
+    __shared__ double sred[kHvWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t kk = blockIdx.y;
+    const int n = N - j - 1;
+    const int nt = (n + kHvCols - 1) / kHvCols;
+    // block -> (J, p)
+    int J = 0, p = blockIdx.x;
+    while (p >= hv_parts(nt, J)) {
+        p -= hv_parts(nt, J);
+        ++J;
     }
-    if (c < nc) a0 = cmad(a0, Ak[(size_t)c * N], sv[c]);
-    P[kk * (size_t)nchmax * N + (size_t)blockIdx.y * N + r] = a0 + a1;
+    const int I0 = J + kHvPart * p, I1 = min(nt, I0 + kHvPart);
+    const int c0 = j + 1 + J * kHvCols;
+    const size_t NP = (size_t)N * kBnb;
+    const cx<double>* Vk = V + kk * NP;
+    const cx<double>* Wk = W + kk * NP;
+    const cx<double>* v = Vk + (size_t)i * N;
+    const cx<double>* Ak = A + kk * (size_t)N * N;
+    if (tid < kHvCols) sv[tid] = c0 + tid < N ? v[c0 + tid] : cx<double>{0.0, 0.0};
+    __syncthreads();
+    constexpr int CPW = kHvCols / kHvWaves;  // columns per wave
+    cx<double> ycol[CPW];
+#pragma unroll
+    for (int u = 0; u < CPW; ++u) ycol[u] = {0.0, 0.0};
+    double spart = 0.0;
+    for (int I = I0; I < I1; ++I) {
+        const int r = j + 1 + I * kHvCols + lane;
+        const bool ok = r < N;
+        const int rr = ok ? r : N - 1;  // loads stay in bounds and unconditional (batched)
+        const cx<double> vr = ok ? v[r] : cx<double>{0.0, 0.0};
+        cx<double> a[CPW];
+#pragma unroll
+        for (int u = 0; u < CPW; ++u) {
+            const int c = min(c0 + wave + kHvWaves * u, N - 1);
+            a[u] = Ak[(size_t)c * N + rr];
+        }
+        cx<double> yrow = {0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < CPW; ++u) {
+            const int cl = wave + kHvWaves * u, c = c0 + cl;
+            const bool in = ok && c < N && r >= c;
+            const cx<double> x = in ? a[u] : cx<double>{0.0, 0.0};
+            yrow = cmad(yrow, x, sv[cl]);
+            const cx<double> xt = (in && r > c) ? x : cx<double>{0.0, 0.0};
+            ycol[u] = cmadc(ycol[u], xt, vr);
+        }
+        const int buf = (I - I0) & 1;
+        red[buf][wave][lane] = yrow;
+        __syncthreads();
+        if (wave == 0 && ok) {
+            cx<double> y = red[buf][0][lane];
+#pragma unroll
+            for (int w = 1; w < kHvWaves; ++w) y = y + red[buf][w][lane];
+            P[kk * (size_t)nchmax * N + (size_t)J * N + r] = y;
+            spart += vr.re * y.re + vr.im * y.im;  // Re(conj(v_r) y_r)
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < CPW; ++u) {
+        const int cl = wave + kHvWaves * u, c = c0 + cl;
+        const cx<double> y = wave_sum(ycol[u]);
+        if (lane == 0 && c < N) {
+            P2[kk * (size_t)nchmax * N + (size_t)p * N + c] = y;
+            spart += sv[cl].re * y.re + sv[cl].im * y.im;
+        }
+    }
+    if (p == 0) {
+        // W^H v (even products) and V^H v (odd) over the rows of tile J
+        cx<double>* Qk = Q + (kk * nchmax + J) * kQStride;
+        const int r = c0 + lane;
+        const cx<double> vr = r < N ? v[r] : cx<double>{0.0, 0.0};
+        const int rr = r < N ? r : N - 1;
+        for (int idx = wave; idx < 2 * i; idx += kHvWaves) {
+            const cx<double>* X = (idx & 1) ? Vk : Wk;
+            const cx<double> x = X[(size_t)(idx >> 1) * N + rr];
+            cx<double> acc = {0.0, 0.0};
+            acc = cmadc(acc, r < N ? x : cx<double>{0.0, 0.0}, vr);
+            acc = wave_sum(acc);
+            if (lane == 0) Qk[1 + (idx & 1) * kBnb + (idx >> 1)] = acc;
+        }
+    }
+    const double st = blk_sum<kHvWaves>(spart, sred);
+    if (tid == 0) SS[kk * nbmax + blockIdx.x] = st;
 }
 
-// w = tau (y - V (W^H v) - W (V^H v)), y = sum_c P[k][c][.], alpha = -1/2 tau (w^H v),
-// w += alpha v, into W(:, i) rows j+1..N-1 (zlatrd).  One block per k-point.
-__global__ __launch_bounds__(256) void k_trd_w(int N, int j, int i, int nch, int nchmax,
+// w = tau (y - V t1 - W t2) + alpha v into W(:, i), rows j+1..N-1 (zlatrd), from the
+// products of k_trd_hemv: y_r = sum_{J <= tile(r)} P[k][J][r] + sum_p P2[k][p][r],
+// t1 = W^H v, t2 = V^H v and s = v^H A22 v summed over the blocks, and
+// alpha = -1/2 tau (w^H v) = -1/2 |tau|^2 (s - 2 Re(t1^H t2)) (w^H v = conj(tau) v^H
+// (A22 - V W^H - W V^H) v).  Block (x, k): rows j + 1 + 256x + t.
+__global__ __launch_bounds__(256) void k_trd_w(int N, int j, int i, int nchmax, int nbmax,
                                                const cx<double>* __restrict__ P,
+                                               const cx<double>* __restrict__ P2,
+                                               const cx<double>* __restrict__ Q,
+                                               const double* __restrict__ SS,
                                                const cx<double>* __restrict__ V,
                                                cx<double>* __restrict__ W,
                                                const cx<double>* __restrict__ tau) {
-    __shared__ cx<double> t1[kBnb], t2[kBnb];
-    __shared__ double red[8];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const size_t kk = blockIdx.x;
+    __shared__ cx<double> t12[kQStride];
+    __shared__ double s_alpha;
+    const int tid = threadIdx.x;
+    const size_t kk = blockIdx.y;
+    const int n = N - j - 1;
+    const int nt = (n + kHvCols - 1) / kHvCols;
+    const int nb = hv_blocks(nt);
     const size_t NP = (size_t)N * kBnb;
     const cx<double>* Vk = V + kk * NP;
     cx<double>* Wk = W + kk * NP;
     const cx<double>* v = Vk + (size_t)i * N;
-    // t1 = W(:, 0:i)^H v (even products), t2 = V(:, 0:i)^H v (odd); wave w takes the
-    // products w, w + 4, ...; lanes stride the rows
-    {
-        cx<double> acc[2 * kBnb / 4];
-#pragma unroll
-        for (int q = 0; q < 2 * kBnb / 4; ++q) acc[q] = {0.0, 0.0};
-        for (int r = j + 1 + lane; r < N; r += 64) {
-            const cx<double> vr = v[r];
-#pragma unroll
-            for (int q = 0; q < 2 * kBnb / 4; ++q) {
-                const int idx = wave + 4 * q;
-                if (idx < 2 * i) {
-                    const cx<double>* X = (idx & 1) ? Vk : Wk;
-                    acc[q] = cmadc(acc[q], X[(size_t)(idx >> 1) * N + r], vr);
-                }
-            }
+    const cx<double>* Qk = Q + kk * nchmax * kQStride;
+    const cx<double> tj = tau[kk * N + j];
+    if (tid < kQStride) {
+        cx<double> acc = {0.0, 0.0};
+        if (tid == 0) {
+            for (int b = 0; b < nb; ++b) acc.re += SS[kk * nbmax + b];
+        } else if ((tid - 1) % kBnb < i) {
+            for (int J = 0; J < nt; ++J) acc = acc + Qk[(size_t)J * kQStride + tid];
         }
-#pragma unroll
-        for (int q = 0; q < 2 * kBnb / 4; ++q) {
-            const int idx = wave + 4 * q;
-            if (idx < 2 * i) {
-                const cx<double> s = wave_sum(acc[q]);
-                if (lane == 0) ((idx & 1) ? t2 : t1)[idx >> 1] = s;
-            }
-        }
+        t12[tid] = acc;
     }
     __syncthreads();
-    const cx<double> tj = tau[kk * N + j];
-    const cx<double>* Pk = P + kk * (size_t)nchmax * N;
-    cx<double> w[kBRows];
-    double dre = 0.0, dim = 0.0;
-#pragma unroll
-    for (int q = 0; q < kBRows; ++q) {
-        const int r = j + 1 + tid + 256 * q;
-        cx<double> x = {0.0, 0.0};
-        if (r < N) {
-            cx<double> y = {0.0, 0.0};
-            for (int c = 0; c < nch; ++c) y = y + Pk[(size_t)c * N + r];
-            cx<double> u = {0.0, 0.0};
-            for (int t = 0; t < i; ++t) {
-                u = cmad(u, Vk[(size_t)t * N + r], t1[t]);
-                u = cmad(u, Wk[(size_t)t * N + r], t2[t]);
-            }
-            x = tj * (y - u);
-            const cx<double> vr = v[r];
-            dre += x.re * vr.re + x.im * vr.im;  // conj(w) v
-            dim += x.re * vr.im - x.im * vr.re;
+    if (tid == 0) {
+        double re = 0.0;
+        for (int t = 0; t < i; ++t) {
+            const cx<double> a = t12[1 + t], b = t12[1 + kBnb + t];
+            re += a.re * b.re + a.im * b.im;  // Re(conj(t1) t2)
         }
-        w[q] = x;
+        s_alpha = -0.5 * (tj.re * tj.re + tj.im * tj.im) * (t12[0].re - 2.0 * re);
     }
-    const double sre = blk_sum<4>(dre, red);
-    const double sim = blk_sum<4>(dim, red + 4);
-    // alpha = -1/2 tau (w^H v)
-    const cx<double> al = {-0.5 * (tj.re * sre - tj.im * sim), -0.5 * (tj.re * sim + tj.im * sre)};
-#pragma unroll
-    for (int q = 0; q < kBRows; ++q) {
-        const int r = j + 1 + tid + 256 * q;
-        if (r < N) Wk[(size_t)i * N + r] = cmad(w[q], al, v[r]);
+    __syncthreads();
+    const int r = j + 1 + blockIdx.x * 256 + tid;
+    if (r >= N) return;
+    const cx<double>* Pk = P + kk * (size_t)nchmax * N;
+    const cx<double>* P2k = P2 + kk * (size_t)nchmax * N;
+    const int I = (r - j - 1) / kHvCols;
+    cx<double> y = {0.0, 0.0};
+    for (int J = 0; J <= I; ++J) y = y + Pk[(size_t)J * N + r];
+    const int np = hv_parts(nt, I);
+    for (int q = 0; q < np; ++q) y = y + P2k[(size_t)q * N + r];
+    cx<double> u = {0.0, 0.0};
+    for (int t = 0; t < i; ++t) {
+        u = cmad(u, Vk[(size_t)t * N + r], t12[1 + t]);
+        u = cmad(u, Wk[(size_t)t * N + r], t12[1 + kBnb + t]);
     }
+    const cx<double> w = tj * (y - u);
+    const cx<double> vr = v[r];
+    Wk[(size_t)i * N + r] = {w.re + s_alpha * vr.re, w.im + s_alpha * vr.im};
 }
 
 // ---------------------------------------------------------------------------------
@@ -305,6 +379,7 @@ __global__ __launch_bounds__(256) void k_trd_w(int N, int j, int i, int nch, int
 // ---------------------------------------------------------------------------------
 struct ZGemmArgs {
     int M, N, K, npair;
+    int lower;  // M == N: only the 64 x 64 tiles on or below the diagonal (grid.x = nt(nt+1)/2)
     const cx<double>* A[2];
     const cx<double>* B[2];
     int lda, ldb, ldc;
@@ -339,22 +414,17 @@ __device__ __forceinline__ void gemm_acc(const cx<double>* Ab, int lda, const cx
             const int m = m0 + 16 * t + li, n = n0 + 16 * t + li;
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const int k = kb + s;
-                cx<double> x = {0.0, 0.0}, y = {0.0, 0.0};
-                if (k < K) {
-                    if (m < M) {
-                        x = opA_at<OPA>(Ab, lda, m, k);
-                        if (sc) {
-                            const double f = sc[k];
-                            x = {f * x.re, f * x.im};
-                        }
-                    }
-                    if (n < N) y = opB_at<OPB>(Bb, ldb, k, n);
-                }
-                ar[t][s] = x.re;
-                ai[t][s] = x.im;
-                br[t][s] = y.re;
-                bi[t][s] = y.im;
+                // clamped indices: every load is in bounds and unconditional, so the
+                // compiler can issue them together; out-of-range values are zeroed after
+                const int k = kb + s, kc = min(k, K - 1);
+                cx<double> x = opA_at<OPA>(Ab, lda, min(m, M - 1), kc);
+                cx<double> y = opB_at<OPB>(Bb, ldb, kc, min(n, N - 1));
+                const bool ina = k < K && m < M, inb = k < K && n < N;
+                const double f = sc ? sc[kc] : 1.0;
+                ar[t][s] = ina ? f * x.re : 0.0;
+                ai[t][s] = ina ? f * x.im : 0.0;
+                br[t][s] = inb ? y.re : 0.0;
+                bi[t][s] = inb ? y.im : 0.0;
             }
         }
 #pragma unroll
@@ -387,8 +457,16 @@ template <int OPA, int OPB>
 __global__ __launch_bounds__(256) void k_zgemm_gen(ZGemmArgs g) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, g4 = lane >> 4;
-    const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
-    const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
+    int ti = blockIdx.x, tj = blockIdx.y;
+    if (g.lower) {
+        const int t = blockIdx.x;
+        ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+        while (ti * (ti + 1) / 2 > t) --ti;
+        tj = t - ti * (ti + 1) / 2;
+    }
+    const int m0 = ti * 64 + (wave & 1) * 32;
+    const int n0 = tj * 64 + (wave >> 1) * 32;
     const size_t b = blockIdx.z;
     lqm_d4 cr[2][2], ci[2][2];
 #pragma unroll

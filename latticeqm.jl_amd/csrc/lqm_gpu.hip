@@ -884,7 +884,9 @@ template <int OPA, int OPB>
 void zgemm_gen(lqm_context* c, int stage, long long kpts, int nb, const lqm::ZGemmArgs& g) {
     if (g.M <= 0 || g.N <= 0 || nb <= 0) return;
     StageScope sc(c, stage, kpts);
-    const dim3 grid((unsigned)((g.M + 63) / 64), (unsigned)((g.N + 63) / 64), (unsigned)nb);
+    const unsigned nt = (unsigned)((g.M + 63) / 64);
+    const dim3 grid = g.lower ? dim3(nt * (nt + 1) / 2, 1, (unsigned)nb)
+                              : dim3(nt, (unsigned)((g.N + 63) / 64), (unsigned)nb);
     hipLaunchKernelGGL((lqm::k_zgemm_gen<OPA, OPB>), grid, dim3(256), 0, c->stream, g);
 }
 
@@ -917,7 +919,8 @@ lqm::ZGemmArgs gemm_args(int M, int N, int K, const cx<double>* A, int lda, long
 // zhetrd (lower) of Kc full Hermitian matrices A with zlatrd panels: d, e, tau out, the
 // reflectors left in A below the diagonal.
 lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<double>* A, cx<double>* Vp, cx<double>* Wp,
-                           cx<double>* P, int nchmax, double* dd, double* ee, cx<double>* tau) {
+                           cx<double>* P, cx<double>* P2, cx<double>* Q, double* SS, int nchmax, int nbmax,
+                           double* dd, double* ee, cx<double>* tau) {
     hipStream_t s = c->stream;
     const size_t NN = (size_t)N * N;
     for (int p = 0; p < N - 1; p += lqm::kBnb) {
@@ -929,14 +932,14 @@ lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<double>* A, cx<doub
                 const int nch = (n + lqm::kHvCols - 1) / lqm::kHvCols;
                 hipLaunchKernelGGL(lqm::k_trd_col, dim3(Kc), dim3(256), 0, s, N, j, i, 0, A, Vp, Wp, dd, ee,
                                    tau);
-                hipLaunchKernelGGL(lqm::k_trd_hemv, dim3((n + 255) / 256, nch, Kc), dim3(256), 0, s, N, j, i,
-                                   nchmax, A, Vp, P);
-                hipLaunchKernelGGL(lqm::k_trd_w, dim3(Kc), dim3(256), 0, s, N, j, i, nch, nchmax, P, Vp, Wp,
-                                   tau);
+                hipLaunchKernelGGL(lqm::k_trd_hemv, dim3(lqm::hv_blocks(nch), Kc), dim3(512), 0, s, N, j, i, nchmax,
+                                   nbmax, A, Vp, Wp, P, P2, Q, SS);
+                hipLaunchKernelGGL(lqm::k_trd_w, dim3((n + 255) / 256, Kc), dim3(256), 0, s, N, j, i, nchmax, nbmax,
+                                   P, P2, Q, SS, Vp, Wp, tau);
             }
         }
         LQM_HIP(hipGetLastError());
-        // A22 -= V W^H + W V^H on rows/columns >= p + pw
+        // A22 -= V W^H + W V^H on rows/columns >= p + pw, tiles on or below the diagonal
         const int q = p + pw, n = N - q;
         lqm::ZGemmArgs g = gemm_args(n, n, pw, Vp + q, N, (long long)N * lqm::kBnb, Wp + q, N,
                                      (long long)N * lqm::kBnb, A + (size_t)q * N + q, N, (long long)NN,
@@ -944,6 +947,7 @@ lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<double>* A, cx<doub
         g.npair = 2;
         g.A[1] = Wp + q;
         g.B[1] = Vp + q;
+        g.lower = 1;
         zgemm_gen<0, 1>(c, ST_BUPD, Kc, Kc, g);
         LQM_HIP(hipGetLastError());
     }
@@ -969,8 +973,10 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
     const bool want_rho = rho != nullptr || eband != nullptr;
     const bool vectors = want_rho || evecs != nullptr;
     const int nchmax = std::max(1, (N - 1 + lqm::kHvCols - 1) / lqm::kHvCols);
+    const int nbmax = lqm::hv_blocks(nchmax);
     const int nblk = std::max(1, (N - 1 + lqm::kBwy - 1) / lqm::kBwy);
-    size_t perk = 16 * NN + 2 * 16 * (size_t)N * lqm::kBnb + 16 * (size_t)nchmax * N + 64 * (size_t)N + 64;
+    size_t perk = 16 * NN + 2 * 16 * (size_t)N * lqm::kBnb + 2 * 16 * (size_t)nchmax * N +
+                  16 * (size_t)nchmax * lqm::kQStride + 8 * (size_t)nbmax + 64 * (size_t)N + 64;
     if (vectors) perk += 8 * NN + 24 * NN + 16 * NN + 16 * (size_t)nblk * lqm::kBwy * lqm::kBwy + 2 * 16 * 64 * (size_t)N;
     if (want_rho) perk += 16 * NN;
     const int Kmax = (int)std::max<size_t>(1, std::min<size_t>({kBlockedBudget / perk, (size_t)Nk, (size_t)65535}));
@@ -992,7 +998,12 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
         if ((st = wsT(c, W_BA, NN * Kmax, &A))) return st;
         if ((st = wsT(c, W_BVW, (size_t)2 * N * lqm::kBnb * Kmax, &Vp))) return st;
         Wp = Vp + (size_t)N * lqm::kBnb * Kmax;
-        if ((st = wsT(c, W_BP, (size_t)nchmax * N * Kmax, &P))) return st;
+        if ((st = wsT(c, W_BP, (size_t)2 * nchmax * N * Kmax + (size_t)nchmax * lqm::kQStride * Kmax + nbmax * Kmax,
+                      &P)))
+            return st;
+        cx<double>* P2 = P + (size_t)nchmax * N * Kmax;
+        cx<double>* Q = P2 + (size_t)nchmax * N * Kmax;
+        double* SS = reinterpret_cast<double*>(Q + (size_t)nchmax * lqm::kQStride * Kmax);
         if ((st = wsT(c, W_BVEC, (size_t)(6 * N + 2) * Kmax, &vec))) return st;
         double* dd = vec;
         double* ee = dd + (size_t)N * Kmax;
@@ -1011,7 +1022,7 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
                                sh, s, N, d, m->nR, m->dR, m->dT, Kc, ksc, A);
         }
         LQM_HIP(hipGetLastError());
-        if ((st = blocked_tridiag(c, N, Kc, A, Vp, Wp, P, nchmax, dd, ee, tau))) return st;
+        if ((st = blocked_tridiag(c, N, Kc, A, Vp, Wp, P, P2, Q, SS, nchmax, nbmax, dd, ee, tau))) return st;
         {
             StageScope sc(c, ST_BEIG, Kc);
             hipLaunchKernelGGL(lqm::k_sturm_bisect, dim3((N + 255) / 256, Kc), dim3(256), 0, s, N, dd, ee, lam, tn);
