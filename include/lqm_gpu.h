@@ -89,7 +89,7 @@ const char *lqm_stage_name(int stage);
  * kind 3: k-points per block of the lane-per-k QL (64 >> variant, 0..3); kind 4:
  * back-transformation (0 = reflector by reflector, 1 = compact WY blocks on the matrix
  * cores, N <= 64, the default for N > 16); kind 5: eigensolver layout (0 = per-k-point
- * kernels, 1 = the large-N path, the default for N > 512: blocked Householder
+ * kernels, 1 = the large-N path, the default for N > 112: blocked Householder
  * tridiagonalization with zlatrd panels of 32 and matrix-core trailing updates, Sturm
  * bisection, inverse iteration with clusters closer than 1e-5 ||T|| orthogonalised,
  * compact-WY back-transformation and rho_k on the matrix cores; FP64 in both precision

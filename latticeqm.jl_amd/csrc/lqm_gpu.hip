@@ -868,7 +868,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
 
 // ---- large N: blocked tridiagonalization, bisection, inverse iteration, compact WY ------
 constexpr size_t kBlockedBudget = size_t(40) << 30;  // bytes of per-k scratch per chunk
-constexpr int kBlockedMinN = 512;    // default: the blocked path above this N
+constexpr int kBlockedMinN = 112;    // default: the blocked path above this N (measured crossover)
 constexpr double kClusterRel = 1e-5;  // eigenvalues closer than this x ||T|| share a cluster
 constexpr double kOccTiny = 1e-18;    // occupations below this leave rho_k out
 
@@ -1724,6 +1724,7 @@ lqm_status lqm_autotune(lqm_model* m, int Nk, const double* ks_dev) {
         int bestv = 0;
         const int vmax = kind == VK_TRIDIAG ? (N <= 64 ? 2 : 1) : kind == VK_QL ? 3 : 1;
         if (kind == VK_BT && N > 64) continue;
+        if (kind == VK_BLOCKED && (N <= 64 || N > 512)) continue;  // per-k kernels / large-N path only
         for (int v = 0; v <= vmax; ++v) {
             c->forced[kind] = v;
             float t = 0.f;

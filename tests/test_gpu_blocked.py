@@ -159,3 +159,25 @@ def test_blocked_expectation_values(pkg, oracle, blocked):
         v = V[:, :, j]
         e0 = np.real(np.einsum("in,ij,jn->n", v.conj(), M, v))
         assert np.max(np.abs(obs[:, j, 0] - e0)) < 1e-10
+
+
+def test_autotune_chooses_eigensolver_layout(pkg, oracle, gpu):
+    # between the per-k kernels and the large-N path (kind 5) for 64 < N <= 512, on a
+    # private context so the tuned choice does not leak into other tests
+    import torch
+
+    ex = pkg.api.GPUExec(0, pkg.api._PRECISIONS["fp64"])
+    N = 96
+    h = pkg.models.random_tb(N, rmax=1, seed=13)
+    ks = pkg.regulargrid(nk=64)
+    m = pkg.api.GPUModel(h, ex)
+    ks_d = torch.from_numpy(np.ascontiguousarray(ks.T)).cuda()
+    chosen = pkg.api.autotune(m, ks_d.data_ptr(), ks.shape[1])
+    assert chosen[5] in (0, 1)
+    rho = pkg.Hops({key: np.zeros((N, N), complex) for key in h.keys()})
+    eb = pkg.api.getdensitymatrix(rho, h, ks, 0.0, T=0.05, executor=ex)
+    R, Tt = h.pack()
+    ref, eb_ref = oracle.densitymatrix(R, Tt, ks, R, mu=0.0, Temp=0.05)
+    err = max(np.max(np.abs(rho[tuple(R[:, r])] - ref[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
