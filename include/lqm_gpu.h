@@ -84,8 +84,9 @@ const char *lqm_stage_name(int stage);
  * wave with the packed Hermitian tile, 2 = register tile on a materialised H(k), N <= 64);
  * kind 1: rho_k (0 = V = QZ then V f V^H,
  * 1 = two-sided reflectors on Z f Z^T); kind 2: Ogita-Aishima sweep of the mixed path
- * (0 = batched GEMM chain, 1 = one fused kernel per k-point, N <= 64, the default for
- * N > 16);
+ * (0 = batched GEMM chain, 1 = one fused kernel per k-point with X and W in LDS, 2 = the
+ * fused kernel in column-panel form with the products in matrix-core accumulators, two
+ * blocks per CU; 1 and 2 need N <= 64, 2 is the default for N > 16);
  * kind 3: k-points per block of the lane-per-k QL (64 >> variant, 0..3); kind 4:
  * back-transformation (0 = reflector by reflector, 1 = compact WY blocks on the matrix
  * cores, N <= 64, the default for N > 16); kind 5: eigensolver layout (0 = per-k-point

@@ -700,9 +700,11 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         std::vector<int> cur, next;
         const int max_sw = std::max(2, c->refine_max);
         hstat.resize((size_t)4 * Kc);
-        // refinement kind: 1 = one fused kernel per k-point and sweep (N <= 64; the default
-        // for 16 < N <= 64, as it works on a 64 x 64 tile), 0 = the GEMM chain below
-        const bool fused = N <= 64 && (c->forced[VK_REFINE] == 1 || (c->forced[VK_REFINE] < 0 && N > 16));
+        // refinement kind: 2 = one fused kernel per k-point and sweep in column-panel form
+        // (two blocks per CU; the default for 16 < N <= 64, as it works on a 64 x 64 tile),
+        // 1 = the same with X and W in LDS (one block per CU), 0 = the GEMM chain below
+        const int fk = c->forced[VK_REFINE] >= 0 ? c->forced[VK_REFINE] : (N > 16 ? 2 : 0);
+        const bool fused = N <= 64 && fk >= 1;
         std::vector<int> clus;
         // with rho or eband requested, a fused sweep forms rho_k (into Sm) and eb of every
         // k-point it finishes; the rest go through k_scale_occ and the NC product below
@@ -714,10 +716,17 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
             bool have_stat = false;
             if (fused) {
                 StageScope sc(c, ST_OA, nsub);
-                const size_t sh = lqm::oa_fused_lds_bytes();
-                allow_lds(lqm::k_oa_fused, sh);
-                hipLaunchKernelGGL(lqm::k_oa_fused, dim3(nsub), dim3(256), sh, c->stream, N, A, X, Sm, G, lam,
-                                   stat, c->refine_thr, kidx, rq);
+                if (fk == 2) {
+                    const size_t sh = lqm::oa_panel_lds_bytes();
+                    allow_lds(lqm::k_oa_panel, sh);
+                    hipLaunchKernelGGL(lqm::k_oa_panel, dim3(nsub), dim3(256), sh, c->stream, N, A, X, Sm, G, lam,
+                                       stat, c->refine_thr, kidx, rq);
+                } else {
+                    const size_t sh = lqm::oa_fused_lds_bytes();
+                    allow_lds(lqm::k_oa_fused, sh);
+                    hipLaunchKernelGGL(lqm::k_oa_fused, dim3(nsub), dim3(256), sh, c->stream, N, A, X, Sm, G, lam,
+                                       stat, c->refine_thr, kidx, rq);
+                }
                 LQM_HIP(hipGetLastError());
                 // k-points with a cluster: general correction (Rayleigh-Ritz) on their S, G
                 LQM_HIP(hipMemcpyAsync(hstat.data(), stat, sizeof(double) * 4 * Kc, hipMemcpyDeviceToHost,
@@ -1535,7 +1544,7 @@ lqm_status lqm_context_refinement_report(lqm_context* c, double* out) {
 lqm_status lqm_context_set_variant(lqm_context* c, int kind, int variant) {
     if (!c) return fail(LQM_ERR_ARG, "null context");
     if (kind < 0 || kind >= VK_N || variant < -1 ||
-        variant > (kind == VK_TRIDIAG ? 2 : kind == VK_QL ? 3 : 1))
+        variant > (kind == VK_TRIDIAG || kind == VK_REFINE ? 2 : kind == VK_QL ? 3 : 1))
         return fail(LQM_ERR_ARG, "bad variant");
     c->forced[kind] = variant;
     return LQM_OK;

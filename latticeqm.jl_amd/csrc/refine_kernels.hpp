@@ -964,4 +964,329 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
             }
 }
 
+// ---------------------------------------------------------------------------------
+// Fused Ogita-Aishima sweep, column-panel form (N <= 64): the computation of k_oa_fused
+// with wave w owning columns 16w..16w+15 of every product, all held in matrix-core
+// accumulators -- W = A X, G = X^H X, S = X^H W (W straight from the accumulators: the
+// MFMA's K index k = 16a + g + 4s is the accumulator layout's row g + 4s of tile a), E and
+// the second-order correction (a column never leaves its wave), X E (E likewise) and,
+// with rq.want on a finished k-point, rho_k = X f X^H.  Only X lives in LDS (68 KB), so
+// two blocks share a CU.  Outputs and flags as k_oa_fused.
+// ---------------------------------------------------------------------------------
+inline size_t oa_panel_lds_bytes() { return sizeof(cx<double>) * 64 * kFusedLd + 8 * (64 + 32 + 64 + 64); }
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_oa_panel(
+    int N, const cx<double>* __restrict__ Ag, cx<double>* __restrict__ Xg, cx<double>* __restrict__ Sg,
+    cx<double>* __restrict__ Gg, double* __restrict__ lambda, double* __restrict__ stat, double thr,
+    const int* __restrict__ kidx, OaRho rq) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<double>* Xs = reinterpret_cast<cx<double>*>(smem);  // [64][kFusedLd] column-major
+    double* lam = reinterpret_cast<double*>(Xs + 64 * kFusedLd);  // [64] Rayleigh quotients
+    double* red = lam + 64;                                        // [32]
+    double* sf = red + 32;                                         // [64] occupations
+    double* lnew = sf + 64;                                        // [64] refined eigenvalues
+    __shared__ int s_cluster;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    const int n = 16 * wave + li;  // this lane's column in every product
+    const size_t kk = kidx ? (size_t)kidx[blockIdx.x] : (size_t)blockIdx.x;
+    const size_t NN = (size_t)N * N;
+    const cx<double>* A = Ag + kk * NN;
+    cx<double>* X = Xg + kk * NN;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int r = e & 63, c = e >> 6;
+        Xs[c * kFusedLd + r] = (r < N && c < N) ? X[(size_t)c * N + r] : cx<double>{0.0, 0.0};
+    }
+    if (tid == 0) s_cluster = 0;
+    __syncthreads();
+
+    lqm_d4 wr[4], wi[4], gr[4], gi[4], sr[4], si[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        wr[t] = wi[t] = gr[t] = gi[t] = sr[t] = si[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+    }
+    // W = A X and G = X^H X over k = k0 + 4g + s
+    for (int k0 = 0; k0 < 64; k0 += 16) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int k = k0 + 4 * g + s;
+            const cx<double> b = Xs[n * kFusedLd + k];
+            double ar[4], ai[4], xr[4], xi[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int m = 16 * t + li;
+                const cx<double> a = Ag[kk * NN + (size_t)min(k, N - 1) * N + min(m, N - 1)];
+                const bool in = m < N && k < N;
+                ar[t] = in ? a.re : 0.0;
+                ai[t] = in ? a.im : 0.0;
+                const cx<double> x = Xs[m * kFusedLd + k];  // conj(X(k, m)) = X^H(m, k)
+                xr[t] = x.re;
+                xi[t] = -x.im;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                wr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], b.re, wr[t], 0, 0, 0);
+                wi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], b.im, wi[t], 0, 0, 0);
+                gr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], b.re, gr[t], 0, 0, 0);
+                gi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], b.im, gi[t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                wr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[t], b.im, wr[t], 0, 0, 0);
+                wi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[t], b.re, wi[t], 0, 0, 0);
+                gr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-xi[t], b.im, gr[t], 0, 0, 0);
+                gi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], b.re, gi[t], 0, 0, 0);
+            }
+        }
+    }
+    // S = X^H W: B from W's accumulators, k = 16a + g + 4s
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int k = 16 * a + g + 4 * s;
+            const double br = wr[a][s], bi = wi[a][s];
+            double xr[4], xi[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const cx<double> x = Xs[(16 * t + li) * kFusedLd + k];
+                xr[t] = x.re;
+                xi[t] = -x.im;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                sr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], br, sr[t], 0, 0, 0);
+                si[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], bi, si[t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                sr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-xi[t], bi, sr[t], 0, 0, 0);
+                si[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], br, si[t], 0, 0, 0);
+            }
+        }
+    }
+    // Rayleigh quotients from the diagonal: (n, n) sits in tile `wave`, element i = li / 4 of
+    // the lanes with g = li % 4
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int m = 16 * t + g + 4 * i;
+            if (m == n && n < N) lam[n] = sr[t][i] / gr[t][i];
+        }
+    __syncthreads();
+    double r2 = 0.0, s2 = 0.0, amax = 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int m = 16 * t + g + 4 * i;
+            if (m < N && n < N) {
+                const double rre = (m == n ? 1.0 : 0.0) - gr[t][i], rim = -gi[t][i];
+                r2 += rre * rre + rim * rim;
+                const double sre = sr[t][i] - (m == n ? lam[m] : 0.0), sim = si[t][i];
+                s2 += sre * sre + sim * sim;
+                if (m == n) amax = fmax(amax, fabs(lam[m]));
+            }
+        }
+    r2 = wave_sum(r2);
+    s2 = wave_sum(s2);
+    amax = wave_max(amax);
+    if (lane == 0) {
+        red[wave] = r2;
+        red[4 + wave] = s2;
+        red[8 + wave] = amax;
+    }
+    __syncthreads();
+    const double rn = sqrt(red[0] + red[1] + red[2] + red[3]);
+    const double an = fmax(fmax(red[8], red[9]), fmax(red[10], red[11]));
+    const double delta = 2.0 * (sqrt(red[4] + red[5] + red[6] + red[7]) + an * rn);
+    if (tid == 0) {
+        int cl = 0;
+        for (int i = 0; i + 1 < N; ++i) cl |= fabs(lam[i + 1] - lam[i]) <= delta;
+        s_cluster = cl;
+    }
+    __syncthreads();
+    if (s_cluster) {
+        cx<double>* Sk = Sg + kk * NN;
+        cx<double>* Gk = Gg + kk * NN;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = 16 * t + g + 4 * i;
+                if (m < N && n < N) {
+                    Sk[(size_t)n * N + m] = cx<double>{sr[t][i], si[t][i]};
+                    Gk[(size_t)n * N + m] = cx<double>{gr[t][i], gi[t][i]};
+                }
+            }
+        if (tid == 0) {
+            stat[kk * 4 + 0] = rn;
+            stat[kk * 4 + 1] = delta;
+            stat[kk * 4 + 2] = 8;
+            stat[kk * 4 + 3] = 0.0;
+        }
+        return;
+    }
+    // E (into S's accumulators) and the column sums of the second-order correction
+    double emax = 0.0, cpart = 0.0;
+    const double ln = n < N ? lam[n] : 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int m = 16 * t + g + 4 * i;
+            double ere = 0.0, eim = 0.0;
+            if (m < N && n < N) {
+                const double rre = (m == n ? 1.0 : 0.0) - gr[t][i], rim = -gi[t][i];
+                if (m == n) {
+                    ere = 0.5 * rre;
+                    eim = 0.5 * rim;
+                } else {
+                    const double gap = ln - lam[m];
+                    if (fabs(gap) > delta) {
+                        const double tre = sr[t][i] + ln * rre, tim = si[t][i] + ln * rim;
+                        ere = tre / gap;
+                        eim = tim / gap;
+                        cpart += (tre * tre + tim * tim) / gap;
+                    } else {
+                        ere = 0.5 * rre;
+                        eim = 0.5 * rim;
+                    }
+                    emax = fmax(emax, ere * ere + eim * eim);
+                }
+            }
+            sr[t][i] = ere;
+            si[t][i] = eim;
+        }
+    cpart += __shfl_xor(cpart, 16);
+    cpart += __shfl_xor(cpart, 32);
+    emax = wave_max(emax);
+    if (lane == 0) red[12 + wave] = emax;
+    if (g == 0) lnew[n] = ln + cpart;  // rows n >= N hold ln = cpart = 0
+    __syncthreads();
+    const double em = sqrt(fmax(fmax(red[12], red[13]), fmax(red[14], red[15])));
+    const double lead = fmax(em, rn);
+    const bool more = thr <= 0.0 || lead * lead > thr;
+    const bool form_rho = rq.want && !more;
+    if (g == 0 && n < N) lambda[kk * N + n] = lnew[n];
+    if (tid == 0) {
+        stat[kk * 4 + 0] = rn;
+        stat[kk * 4 + 1] = delta;
+        stat[kk * 4 + 2] = (more ? 4 : 0) | (form_rho ? 16 : 0);
+        stat[kk * 4 + 3] = em;
+    }
+    // X E: B from E's accumulators (k = 16a + g + 4s), A = X from LDS
+    lqm_d4 xr_[4], xi_[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) xr_[t] = xi_[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int k = 16 * a + g + 4 * s;
+            const double br = sr[a][s], bi = si[a][s];
+            double xr[4], xi[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const cx<double> x = Xs[k * kFusedLd + 16 * t + li];
+                xr[t] = x.re;
+                xi[t] = x.im;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                xr_[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], br, xr_[t], 0, 0, 0);
+                xi_[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], bi, xi_[t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                xr_[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-xi[t], bi, xr_[t], 0, 0, 0);
+                xi_[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], br, xi_[t], 0, 0, 0);
+            }
+        }
+    }
+    // X + X E
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const cx<double> x = Xs[n * kFusedLd + 16 * t + g + 4 * i];
+            xr_[t][i] += x.re;
+            xi_[t][i] += x.im;
+        }
+    if (!form_rho) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = 16 * t + g + 4 * i;
+                if (m < N && n < N) X[(size_t)n * N + m] = cx<double>{xr_[t][i], xi_[t][i]};
+            }
+        return;
+    }
+    // rho_k = X f X^H from the refined X, staged back into LDS
+    __syncthreads();  // every wave is done reading the old X
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Xs[n * kFusedLd + 16 * t + g + 4 * i] = cx<double>{xr_[t][i], xi_[t][i]};
+    double fn = 0.0, ebp = 0.0;
+    if (tid < N) {
+        const double l = lnew[tid];
+        if (rq.occ) {
+            int rank = 0;
+            for (int j = 0; j < N; ++j) {
+                const double lj = lnew[j];
+                rank += (lj < l) || (lj == l && j < tid);
+            }
+            fn = rq.occ[kk * N + min(rank, N - 1)];
+        } else {
+            fn = fermi_occ(l, rq.mu, rq.T);
+        }
+        ebp = l * fn;
+    }
+    if (tid < 64) sf[tid] = fn;
+    ebp = wave_sum(ebp);
+    if (lane == 0) red[16 + wave] = ebp;
+    __syncthreads();
+    if (tid == 0) rq.eb[kk] = (rq.kw ? rq.kw[kk] : rq.wuni) * (red[16] + red[17] + red[18] + red[19]);
+    lqm_d4 cr[4], ci[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) cr[t] = ci[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < 64; k0 += 16) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int c = k0 + 4 * g + s;
+            const double f = sf[c];
+            const cx<double> y = Xs[c * kFusedLd + n];  // B(c, n) = conj X(n, c)
+            const double yr = y.re, yi = -y.im;
+            double xr[4], xi[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const cx<double> x = Xs[c * kFusedLd + 16 * t + li];
+                xr[t] = f * x.re;
+                xi[t] = f * x.im;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], yr, cr[t], 0, 0, 0);
+                ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], yi, ci[t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-xi[t], yi, cr[t], 0, 0, 0);
+                ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], yr, ci[t], 0, 0, 0);
+            }
+        }
+    }
+    cx<double>* Rk = rq.rhok + kk * NN;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int m = 16 * t + g + 4 * i;
+            if (m < N && n < N) Rk[(size_t)n * N + m] = cx<double>{cr[t][i], ci[t][i]};
+        }
+}
+
 }  // namespace lqm
