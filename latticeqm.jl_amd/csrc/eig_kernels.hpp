@@ -227,7 +227,23 @@ __global__ void k_tridiag_ql(int N, int Nkc, const R* __restrict__ dd, const R* 
             for (i = m - 1; i >= l; --i) {
                 R f = s * E(i);
                 R b = c * E(i);
-                r = hypot(f, g);
+                // r = |(f, g)|, s = f / r, c = g / r from one reciprocal square root (FP32)
+                // or one square root and one reciprocal (FP64); hypot and divisions only
+                // where f^2 + g^2 leaves the safe normal range
+                const R q = f * f + g * g;
+                R inv;
+                if (q > num<R>::rot_lo && q < num<R>::rot_hi) {
+                    if (sizeof(R) == 4) {
+                        inv = (R)__frsqrt_rn((float)q);
+                        r = q * inv;
+                    } else {
+                        r = sqrt(q);
+                        inv = R(1) / r;
+                    }
+                } else {
+                    r = hypot(f, g);
+                    inv = r == R(0) ? R(0) : R(1) / r;
+                }
                 E(i + 1) = r;
                 if (r == R(0)) {
                     D(i + 1) -= pacc;
@@ -235,8 +251,8 @@ __global__ void k_tridiag_ql(int N, int Nkc, const R* __restrict__ dd, const R* 
                     broke = true;
                     break;
                 }
-                s = f / r;
-                c = g / r;
+                s = f * inv;
+                c = g * inv;
                 g = D(i + 1) - pacc;
                 r = (D(i) - g) * s + R(2) * c * b;
                 pacc = s * r;

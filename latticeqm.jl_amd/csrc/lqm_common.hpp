@@ -89,11 +89,15 @@ __device__ __forceinline__ cx<double> cis2pi(double x) {
 
 // Numerical-type constants used by the eigensolver.
 template <typename R> struct num;
+// rot_lo, rot_hi: the range of f^2 + g^2 in which a Givens rotation is formed from the
+// sum of squares directly (no overflow, no loss to subnormals).
 template <> struct num<double> {
     static constexpr double half_eps = 1.1102230246251565e-16;  // 2^-53
+    static constexpr double rot_lo = 1e-290, rot_hi = 1e290;
 };
 template <> struct num<float> {
     static constexpr float half_eps = 5.9604645e-08f;  // 2^-24
+    static constexpr float rot_lo = 1e-30f, rot_hi = 1e30f;
 };
 
 // Device-side status bits, OR-ed into a per-call flag word.
