@@ -201,79 +201,100 @@ __global__ void k_tridiag_ql(int N, int Nkc, const R* __restrict__ dd, const R* 
     int* lsw = LOG ? sweeps + (size_t)kk * cap_sw : nullptr;
     int pos = 0, nsw = 0;
     const R heps = num<R>::half_eps;
-    for (int l = 0; l < N; ++l) {
-        int iter = 0;
-        int m;
-        for (;;) {
-            for (m = l; m < N - 1; ++m) {
-                R dsum = fabs(D(m)) + fabs(D(m + 1));
-                if (fabs(E(m)) <= heps * dsum) break;
+    // E(j) is negligible when |E(j)| <= heps (|D(j)| + |D(j+1)|); E(N-1) is the sentinel.
+    // m is always the first negligible index at or after l.  A full sweep over [l, m]
+    // produces the final D, E of every index it touches, so it records the first negligible
+    // index in (l, m] (f1) on the way and the next m needs no scan; a scan runs only after
+    // an interrupted sweep or when l moves past what the last sweep saw.  Each lane runs its
+    // own event loop (advance l, or one sweep), so lanes at different l do not wait for
+    // each other at every l.  The rotations and their order are those of the scanning form.
+    auto negl = [&](int j) -> bool { return fabs(E(j)) <= heps * (fabs(D(j)) + fabs(D(j + 1))); };
+    auto scan = [&](int from) -> int {
+        int mm = from;
+        while (mm < N - 1 && !negl(mm)) ++mm;
+        return mm;
+    };
+    int l = 0, iter = 0, f1 = -1;
+    int m = scan(0);
+    while (l < N) {
+        if (m == l) {
+            ++l;
+            iter = 0;
+            if (l < N) m = f1 > l ? f1 : scan(l);
+            continue;
+        }
+        if (++iter > 30) {
+            flag |= kStatNoConv;
+            break;
+        }
+        // Wilkinson shift from the leading 2x2 block
+        R g = (D(l + 1) - D(l)) / (R(2) * E(l));
+        R r = hypot(g, R(1));
+        g = D(m) - D(l) + E(l) / (g + copysign(r, g));
+        R s = 1, c = 1, pacc = 0;
+        const bool logit = LOG && (pos + (m - l) <= cap_rot) && (nsw < cap_sw);
+        if (LOG && !logit) flag |= kStatLogFull;
+        int cnt = 0;
+        bool broke = false;
+        f1 = m;
+        R dnext = D(m);  // D(i + 2) of the iteration below (final for this sweep)
+        for (int i = m - 1; i >= l; --i) {
+            R f = s * E(i);
+            R b = c * E(i);
+            // r = |(f, g)|, s = f / r, c = g / r from one reciprocal square root (FP32)
+            // or one square root and one reciprocal (FP64); hypot and divisions only
+            // where f^2 + g^2 leaves the safe normal range
+            const R q = f * f + g * g;
+            R inv;
+            if (q > num<R>::rot_lo && q < num<R>::rot_hi) {
+                if (sizeof(R) == 4) {
+                    inv = (R)__frsqrt_rn((float)q);
+                    r = q * inv;
+                } else {
+                    r = sqrt(q);
+                    inv = R(1) / r;
+                }
+            } else {
+                r = hypot(f, g);
+                inv = r == R(0) ? R(0) : R(1) / r;
             }
-            if (m == l) break;
-            if (++iter > 30) {
-                flag |= kStatNoConv;
+            E(i + 1) = r;
+            if (r == R(0)) {
+                D(i + 1) -= pacc;
+                E(m) = 0;
+                broke = true;
                 break;
             }
-            // Wilkinson shift from the leading 2x2 block
-            R g = (D(l + 1) - D(l)) / (R(2) * E(l));
-            R r = hypot(g, R(1));
-            g = D(m) - D(l) + E(l) / (g + copysign(r, g));
-            R s = 1, c = 1, pacc = 0;
-            int i;
-            const bool logit = LOG && (pos + (m - l) <= cap_rot) && (nsw < cap_sw);
-            if (LOG && !logit) flag |= kStatLogFull;
-            int cnt = 0;
-            bool broke = false;
-            for (i = m - 1; i >= l; --i) {
-                R f = s * E(i);
-                R b = c * E(i);
-                // r = |(f, g)|, s = f / r, c = g / r from one reciprocal square root (FP32)
-                // or one square root and one reciprocal (FP64); hypot and divisions only
-                // where f^2 + g^2 leaves the safe normal range
-                const R q = f * f + g * g;
-                R inv;
-                if (q > num<R>::rot_lo && q < num<R>::rot_hi) {
-                    if (sizeof(R) == 4) {
-                        inv = (R)__frsqrt_rn((float)q);
-                        r = q * inv;
-                    } else {
-                        r = sqrt(q);
-                        inv = R(1) / r;
-                    }
-                } else {
-                    r = hypot(f, g);
-                    inv = r == R(0) ? R(0) : R(1) / r;
-                }
-                E(i + 1) = r;
-                if (r == R(0)) {
-                    D(i + 1) -= pacc;
-                    E(m) = 0;
-                    broke = true;
-                    break;
-                }
-                s = f * inv;
-                c = g * inv;
-                g = D(i + 1) - pacc;
-                r = (D(i) - g) * s + R(2) * c * b;
-                pacc = s * r;
-                D(i + 1) = g + pacc;
-                g = c * r - b;
-                if (logit) {
-                    lrot[2 * (pos + cnt)] = c;
-                    lrot[2 * (pos + cnt) + 1] = s;
-                }
-                ++cnt;
+            s = f * inv;
+            c = g * inv;
+            g = D(i + 1) - pacc;
+            r = (D(i) - g) * s + R(2) * c * b;
+            pacc = s * r;
+            const R dn = g + pacc;
+            D(i + 1) = dn;
+            g = c * r - b;
+            // E(i+1), D(i+1), D(i+2) are final for this sweep (E(m) is zeroed below)
+            if (i + 1 < m && fabs(E(i + 1)) <= heps * (fabs(dn) + fabs(dnext))) f1 = i + 1;
+            dnext = dn;
+            if (logit) {
+                lrot[2 * (pos + cnt)] = c;
+                lrot[2 * (pos + cnt) + 1] = s;
             }
-            if (logit && cnt > 0) {
-                lsw[nsw++] = ((m - 1) << 16) | cnt;
-                pos += cnt;
-            }
-            if (broke) continue;
-            D(l) -= pacc;
-            E(l) = g;
-            E(m) = 0;
+            ++cnt;
         }
-        if (flag & kStatNoConv) break;
+        if (logit && cnt > 0) {
+            lsw[nsw++] = ((m - 1) << 16) | cnt;
+            pos += cnt;
+        }
+        if (broke) {
+            f1 = -1;
+            m = scan(l);
+            continue;
+        }
+        D(l) -= pacc;
+        E(l) = g;
+        E(m) = 0;
+        m = negl(l) ? l : f1;
     }
     // ascending order by rank counting (ties broken by index)
     for (int i = 0; i < N; ++i) {
