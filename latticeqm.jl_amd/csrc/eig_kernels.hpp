@@ -30,7 +30,8 @@ template <typename R, bool LDSMAT>
 __global__ __launch_bounds__(kTPB) void k_hk_tridiag(
     int N, int d, int nR, const int* __restrict__ Rk, const cx<double>* __restrict__ T,
     const double* __restrict__ ks, int Nkc, cx<R>* __restrict__ gA, R* __restrict__ dd,
-    R* __restrict__ ee, cx<R>* __restrict__ refl, cx<R>* __restrict__ tau) {
+    R* __restrict__ ee, cx<R>* __restrict__ refl, cx<R>* __restrict__ tau,
+    const cx<double>* __restrict__ Hpre) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int kk = blockIdx.x;
     const int tid = threadIdx.x;
@@ -64,9 +65,18 @@ __global__ __launch_bounds__(kTPB) void k_hk_tridiag(
     }
     __syncthreads();
 
-    // H(k), upper triangle summed in FP64, mirrored into the lower triangle
+    // H(k), upper triangle summed in FP64, mirrored into the lower triangle (or, with
+    // Hpre, read from the FP64 Hermitian H(k) the pass has already built)
     const size_t NN = (size_t)N * N;
-    for (int e = tid; e < N * N; e += kTPB) {
+    if (Hpre) {
+        const cx<double>* H = Hpre + (size_t)kk * NN;
+        for (int e = tid; e < N * N; e += kTPB) {
+            const int i = e % N, j = e / N;
+            const cx<double> h = H[e];
+            At(i, j) = mkc<R>((R)h.re, i == j ? (R)0 : (R)h.im);
+        }
+    }
+    for (int e = tid; e < N * N && !Hpre; e += kTPB) {
         int i = e % N, j = e / N;
         if (i > j) continue;
         cx<double> h = {0.0, 0.0};

@@ -255,6 +255,17 @@ void allow_lds(K kernel, size_t bytes) {
 
 // Eigenvalues (and optionally the QL log) of one chunk: K_A then K_B, in the
 // working precision R (double: the plain path; float: the bulk of the mixed path).
+// A = Hermitian(H(k)) in FP64, full storage, for Kc k-points: 32 x 32 tile pairs held
+// across 8 k-points per block, so the hopping stack is read from L2 once per tile and
+// k-group rather than once per k-point.
+void launch_hk_full(lqm_context* c, lqm_model* m, int Kc, const double* ks, cx<double>* A) {
+    const int nt = (m->N + 31) / 32;
+    const size_t sh = sizeof(cx<double>) * lqm::kHkK * std::max(1, m->nR);
+    allow_lds(lqm::k_hk_full, sh);
+    hipLaunchKernelGGL(lqm::k_hk_full, dim3(nt * (nt + 1) / 2, (Kc + lqm::kHkK - 1) / lqm::kHkK), dim3(256), sh,
+                       c->stream, m->N, m->d, m->nR, m->dR, m->dT, Kc, ks, A);
+}
+
 // Hpre: H(k) of the chunk already materialised in FP64 (full Hermitian), or null.
 template <typename R>
 lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks, R* evals,
@@ -282,10 +293,7 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
         cx<double>* Hk;
         if ((st = wsT(c, W_HK, (size_t)N * N * Kc, &Hk))) return st;
         StageScope sc(c, ST_HK64, Kc);
-        const size_t sh = sizeof(cx<double>) * m->nR;
-        allow_lds(lqm::k_herm_hk, sh);
-        hipLaunchKernelGGL(lqm::k_herm_hk, dim3(Kc), dim3(256), sh, c->stream, N, m->d, m->nR, m->dR, m->dT,
-                           ks, Hk);
+        launch_hk_full(c, m, Kc, ks, Hk);
         Hpre = Hk;
     }
     {
@@ -307,12 +315,12 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
             auto kA = lqm::k_hk_tridiag<R, true>;
             allow_lds(kA, P.shA);
             hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kTPB), P.shA, c->stream, N, m->d, m->nR, m->dR,
-                               m->dT, ks, Kc, gA, dd, ee, refl, tau);
+                               m->dT, ks, Kc, gA, dd, ee, refl, tau, Hpre);
         } else {
             auto kA = lqm::k_hk_tridiag<R, false>;
             allow_lds(kA, P.shA);
             hipLaunchKernelGGL(kA, dim3(Kc), dim3(lqm::kTPB), P.shA, c->stream, N, m->d, m->nR, m->dR,
-                               m->dT, ks, Kc, gA, dd, ee, refl, tau);
+                               m->dT, ks, Kc, gA, dd, ee, refl, tau, Hpre);
         }
     }
     LQM_HIP(hipGetLastError());
@@ -681,10 +689,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         // H(k) in FP64 (the refinement's A), then FP32 eigenvectors, ascending, as FP64 complex
         {
             StageScope sc(c, ST_HK64, Kc);
-            const size_t sh = sizeof(cx<double>) * m->nR;
-            allow_lds(lqm::k_herm_hk, sh);
-            hipLaunchKernelGGL(lqm::k_herm_hk, dim3(Kc), dim3(256), sh, c->stream, N, d, m->nR, m->dR,
-                               m->dT, ksc, A);
+            launch_hk_full(c, m, Kc, ksc, A);
         }
         LQM_HIP(hipGetLastError());
         if ((st = launch_eigvals<float>(m, P, Kc, ksc, ev32, true, A))) return st;

@@ -2,7 +2,6 @@
 // symmetric eigenvalue decomposition", JJIAM 35 (2018), Algorithm 1, in its Hermitian
 // form) and the FP64 pieces around it:
 //
-//   k_herm_hk        A = Hermitian(H(k)) in FP64 from the upper triangle
 //   k_zgemm_batched  C = alpha op(A) op(B) + beta C per k-point (N <= 128 tiles)
 //   k_oa_correct     lambda_i = s_ii / g_ii, delta = 2(||S - D|| + ||A|| ||R||),
 //                    E_ij = (s_ij + lambda_j r_ij)/(lambda_j - lambda_i) or r_ij / 2
@@ -15,37 +14,6 @@
 #include "lqm_common.hpp"
 
 namespace lqm {
-
-// Hermitian(H(k)) with H(k) = sum_R T_R e^{2 pi i k.R}; one block per k-point.
-__global__ __launch_bounds__(256) void k_herm_hk(int N, int d, int nR, const int* __restrict__ Rk,
-                                                 const cx<double>* __restrict__ T,
-                                                 const double* __restrict__ ks,
-                                                 cx<double>* __restrict__ A) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    cx<double>* sph = reinterpret_cast<cx<double>*>(smem);
-    const int kk = blockIdx.x;
-    const double* k = ks + (size_t)kk * d;
-    for (int r = threadIdx.x; r < nR; r += 256) {
-        double x = 0.0;
-        for (int a = 0; a < d; ++a) x += k[a] * (double)Rk[r * d + a];
-        sph[r] = cis2pi(x);
-    }
-    __syncthreads();
-    const size_t NN = (size_t)N * N;
-    cx<double>* Ak = A + (size_t)kk * NN;
-    for (int e = threadIdx.x; e < N * N; e += 256) {
-        const int i = e % N, j = e / N;
-        if (i > j) continue;
-        cx<double> h = {0.0, 0.0};
-        for (int r = 0; r < nR; ++r) h = cmad(h, T[r * NN + (size_t)j * N + i], sph[r]);
-        if (i == j) {
-            Ak[e] = {h.re, 0.0};
-        } else {
-            Ak[e] = h;
-            Ak[(size_t)i * N + j] = {h.re, -h.im};
-        }
-    }
-}
 
 // Batched C = alpha op(A) op(B) + beta D, square N x N column-major FP64 complex,
 // batch stride N*N (D may be C, or null when beta = 0).  op 0 = as is, 1 = conjugate
