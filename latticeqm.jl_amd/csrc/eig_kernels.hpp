@@ -188,6 +188,10 @@ __global__ __launch_bounds__(kTPB) void k_hk_tridiag(
 // Eigenvalues are written ascending (evals[kk*N + n]) with perm[kk*N + n] = the Z
 // column that belongs to them.
 // ---------------------------------------------------------------------------------
+template <typename R> struct RotPair;
+template <> struct RotPair<float> { using type = float2; };
+template <> struct RotPair<double> { using type = double2; };
+
 template <typename R, bool LOG>
 __global__ void k_tridiag_ql(int N, int Nkc, const R* __restrict__ dd, const R* __restrict__ ee,
                              R* __restrict__ evals, int* __restrict__ perm, R* __restrict__ rot,
@@ -248,9 +252,18 @@ __global__ void k_tridiag_ql(int N, int Nkc, const R* __restrict__ dd, const R* 
         bool broke = false;
         f1 = m;
         R dnext = D(m);  // D(i + 2) of the iteration below (final for this sweep)
+        // E(i) and D(i) do not depend on the recurrence: they are read one iteration ahead,
+        // and D(i + 1) is the previous iteration's D(i) (unchanged since), so no LDS read
+        // sits on the critical path of a rotation
+        R ei = E(m - 1), di = D(m - 1), dip1 = D(m);
         for (int i = m - 1; i >= l; --i) {
-            R f = s * E(i);
-            R b = c * E(i);
+            R en = R(0), dnx = R(0);
+            if (i > l) {
+                en = E(i - 1);
+                dnx = D(i - 1);
+            }
+            R f = s * ei;
+            R b = c * ei;
             // r = |(f, g)|, s = f / r, c = g / r from one reciprocal square root (FP32)
             // or one square root and one reciprocal (FP64); hypot and divisions only
             // where f^2 + g^2 leaves the safe normal range
@@ -270,27 +283,32 @@ __global__ void k_tridiag_ql(int N, int Nkc, const R* __restrict__ dd, const R* 
             }
             E(i + 1) = r;
             if (r == R(0)) {
-                D(i + 1) -= pacc;
+                D(i + 1) = dip1 - pacc;
                 E(m) = 0;
                 broke = true;
                 break;
             }
+            const R er = r;
             s = f * inv;
             c = g * inv;
-            g = D(i + 1) - pacc;
-            r = (D(i) - g) * s + R(2) * c * b;
+            g = dip1 - pacc;
+            r = (di - g) * s + R(2) * c * b;
             pacc = s * r;
             const R dn = g + pacc;
             D(i + 1) = dn;
             g = c * r - b;
             // E(i+1), D(i+1), D(i+2) are final for this sweep (E(m) is zeroed below)
-            if (i + 1 < m && fabs(E(i + 1)) <= heps * (fabs(dn) + fabs(dnext))) f1 = i + 1;
+            if (i + 1 < m && fabs(er) <= heps * (fabs(dn) + fabs(dnext))) f1 = i + 1;
             dnext = dn;
             if (logit) {
-                lrot[2 * (pos + cnt)] = c;
-                lrot[2 * (pos + cnt) + 1] = s;
+                // (c, s) as one vector store (the pair is 2-element aligned)
+                using pair_t = typename RotPair<R>::type;
+                reinterpret_cast<pair_t*>(lrot)[pos + cnt] = pair_t{c, s};
             }
             ++cnt;
+            dip1 = di;
+            ei = en;
+            di = dnx;
         }
         if (logit && cnt > 0) {
             lsw[nsw++] = ((m - 1) << 16) | cnt;

@@ -242,3 +242,18 @@ def test_scf_batch_uneven_models(pkg, gpu, mfm, hf):
     assert abs(out[1][1] - ref["E"]) < 1e-8
     for R in ref["rho"]:
         assert np.max(np.abs(out[1][0][R] - ref["rho"][R])) < 1e-8
+
+
+def test_scf_large_n_path_vs_oracle(pkg, gpu, mfm, hf):
+    # 130 orbitals (65 sites x spin): above the large-N threshold, so every k pass of the SCF
+    # (mu from the eigenvalue pass, rho) runs the blocked eigensolver; oracle to 1e-8
+    h, v = _spinful_random(pkg, 65, 90, U=2.5)
+    rho0 = mfm.staggered_guess(v, np.arange(65) % 2, m=0.2, noise=1e-3, seed=3)
+    ks = pkg.regulargrid(nk=3 ** 2)
+    kw = dict(T=0.05, beta=0.5, tol=1e-9, iterations=200)
+    rho, E, info, conv, res = mfm.solvehartreefock(h, v, rho0, 0.5, ks, executor=gpu, **kw)
+    ref = hf.scf(h.data, v.data, rho0.data, 0.5, ks, **kw)
+    assert conv and ref["converged"]
+    assert abs(E - ref["E"]) < 1e-8
+    for R in ref["rho"]:
+        assert np.max(np.abs(rho[R] - ref["rho"][R])) < 1e-8
