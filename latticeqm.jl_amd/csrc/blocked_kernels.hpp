@@ -7,8 +7,8 @@
 //   k_zgemm_gen      batched C = alpha sum_p op(A_p) op(B_p) + beta C on the FP64 matrix
 //                    cores: the trailing her2k update, compact-WY products, rho_k
 //   k_sturm_bisect   eigenvalue n of the tridiagonal by Sturm-count bisection
-//   k_invit          eigenvectors of the tridiagonal by inverse iteration, clusters
-//                    orthogonalised within every iteration
+//   k_invit          eigenvectors of the tridiagonal by inverse iteration, one thread each
+//   k_invit_orth     orthonormalisation of the vectors of each eigenvalue cluster
 //   k_make_y, k_wy_T, k_z_to_v, k_occ_eb   compact-WY and rho_k helpers
 //
 // The tridiagonalization is the one-stage blocked Householder reduction (LAPACK zhetrd,
@@ -575,22 +575,19 @@ __device__ __forceinline__ double invit_start(unsigned long long seed, int i) {
 
 // ---------------------------------------------------------------------------------
 // Eigenvectors of the symmetric tridiagonal (d, e) by inverse iteration (LAPACK dstein's
-// scheme).  For eigenvalue lambda_m: three solves of (T - lambda_m I) x = b, b the last x
-// scaled to unit max-norm, first b a seeded uniform(-1, 1) vector; the solve factors
+// solve).  Thread m of block (x, k): three solves of (T - lambda_m I) x = b, b the last x
+// scaled to unit max-norm, first b a seeded uniform(-1, 1) vector; each solve factors
 // T - lambda I = P L U with partial pivoting (dlagtf) fused with the forward
-// substitution, and replaces U pivots below eps ||T|| by +-eps ||T|| (dlagts).
-// Eigenvalues within ctol of their predecessor form a cluster; the thread of a cluster's
-// first member computes all members in order, each orthogonalised against the earlier
-// ones after every solve (modified Gram-Schmidt) with its shift raised to at least
-// 10 eps |lambda| above the previous member's (dstein's PERTOL).  Thread m of block
-// (x, k): eigenvector m of k-point k into Zt[k][i * ldz + m] (unit 2-norm), m < ncol
-// (no cluster crosses ncol).  scr: U's diagonals, 3 * N * ldz doubles per k-point,
-// interleaved by m.
+// substitution and replaces U pivots below eps ||T|| by +-eps ||T|| (dlagts).  Every
+// eigenvector iterates on its own; the members of a cluster (eigenvalues within ctol of
+// their predecessor) are orthonormalised afterwards by k_invit_orth.  Out:
+// Zt[k][i * ldz + m] = x_m[i] (unit 2-norm), m < ncol.  scr: U's three diagonals,
+// 3 * N * ldz doubles per k-point, interleaved by m so a wave's accesses are coalesced.
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_invit(int N, int ncol, int ldz, const double* __restrict__ dd,
                                               const double* __restrict__ ee,
                                               const double* __restrict__ lam,
-                                              const double* __restrict__ tnorm, double ctol_rel,
+                                              const double* __restrict__ tnorm,
                                               double* __restrict__ Zt, double* __restrict__ scr,
                                               int kbase, int* __restrict__ status) {
     const int m = blockIdx.x * 64 + threadIdx.x;
@@ -598,113 +595,142 @@ __global__ __launch_bounds__(64) void k_invit(int N, int ncol, int ldz, const do
     if (m >= ncol) return;
     const double* d = dd + kk * N;
     const double* e = ee + kk * N;
+    const double x = lam[kk * N + m];
+    const double eps = 2.220446049250313e-16;
+    const double tol = fmax(eps * tnorm[kk], 2.2250738585072014e-308);
+    const size_t NZ = (size_t)N * ldz;
+    double* Z = Zt + kk * NZ + m;
+    double* U0 = scr + (kk * 3 + 0) * NZ + m;
+    double* U1 = scr + (kk * 3 + 1) * NZ + m;
+    double* U2 = scr + (kk * 3 + 2) * NZ + m;
+    const unsigned long long seed =
+        0x51A7E5EEDull ^ ((unsigned long long)(kbase + (long long)kk) << 24) ^ (unsigned long long)m;
+    bool bad = false;
+    for (int i = 0; i < N; ++i) Z[(size_t)i * ldz] = invit_start(seed, i);
+    for (int it = 0; it < 3; ++it) {
+        // factor + forward substitution (the next right-hand side entry is read one step ahead)
+        double acur = d[0] - x, bcur = N > 1 ? e[0] : 0.0;
+        double ycur = Z[0];
+        double ynext = N > 1 ? Z[ldz] : 0.0;
+        for (int k = 0; k + 1 < N; ++k) {
+            const double c = e[k];
+            const double anext = d[k + 1] - x;
+            const double bnext = k + 2 < N ? e[k + 1] : 0.0;
+            const double yfar = k + 2 < N ? Z[(size_t)(k + 2) * ldz] : 0.0;
+            const size_t o = (size_t)k * ldz;
+            if (fabs(acur) >= fabs(c)) {
+                const double mult = acur != 0.0 ? c / acur : 0.0;
+                U0[o] = acur;
+                U1[o] = bcur;
+                U2[o] = 0.0;
+                Z[o] = ycur;
+                acur = anext - mult * bcur;
+                bcur = bnext;
+                ycur = ynext - mult * ycur;
+            } else {
+                const double mult = acur / c;
+                U0[o] = c;
+                U1[o] = anext;
+                U2[o] = bnext;
+                Z[o] = ynext;
+                const double na = bcur - mult * anext;
+                bcur = -mult * bnext;
+                acur = na;
+                ycur = ycur - mult * ynext;
+            }
+            ynext = yfar;
+        }
+        U0[(size_t)(N - 1) * ldz] = acur;
+        Z[(size_t)(N - 1) * ldz] = ycur;
+        // back substitution (row k - 1 read one step ahead)
+        double x1 = 0.0, x2 = 0.0, amax = 0.0;
+        size_t o = (size_t)(N - 1) * ldz;
+        double p0 = U0[o], p1 = U1[o], p2 = U2[o], zz = Z[o];
+        for (int k = N - 1; k >= 0; --k) {
+            double q0 = 0.0, q1 = 0.0, q2 = 0.0, qz = 0.0;
+            if (k > 0) {
+                const size_t on = o - ldz;
+                q0 = U0[on];
+                q1 = U1[on];
+                q2 = U2[on];
+                qz = Z[on];
+            }
+            double p = p0;
+            if (fabs(p) < tol) p = p >= 0.0 ? tol : -tol;
+            double v = zz;
+            if (k + 1 < N) v -= p1 * x1;
+            if (k + 2 < N) v -= p2 * x2;
+            v /= p;
+            Z[o] = v;
+            x2 = x1;
+            x1 = v;
+            amax = fmax(amax, fabs(v));
+            o -= ldz;
+            p0 = q0;
+            p1 = q1;
+            p2 = q2;
+            zz = qz;
+        }
+        if (!(amax > 0.0) || !isfinite(amax)) {
+            bad = true;
+            amax = 1.0;
+        }
+        if (it < 2) {
+            const double sc = 1.0 / amax;
+            for (int i = 0; i < N; ++i) Z[(size_t)i * ldz] *= sc;
+        }
+    }
+    double nrm = 0.0;
+    for (int i = 0; i < N; ++i) {
+        const double v = Z[(size_t)i * ldz];
+        nrm += v * v;
+    }
+    nrm = sqrt(nrm);
+    if (!(nrm > 0.0) || !isfinite(nrm)) {
+        bad = true;
+        nrm = 1.0;
+    }
+    const double sc = 1.0 / nrm;
+    for (int i = 0; i < N; ++i) Z[(size_t)i * ldz] *= sc;
+    if (bad) atomicOr(status, kStatNonFinite);
+}
+
+// Clusters of consecutive eigenvalues within ctol = ctol_rel ||T|| of their predecessor:
+// the members' independently iterated vectors are orthonormalised in eigenvalue order by
+// modified Gram-Schmidt, applied twice.  One wave per candidate first member (wave w of
+// block x: m = 4x + w), lanes over the rows; waves whose m does not start a cluster of two
+// or more leave at once.
+__global__ __launch_bounds__(256) void k_invit_orth(int N, int ncol, int ldz, const double* __restrict__ lam,
+                                                    const double* __restrict__ tnorm, double ctol_rel,
+                                                    double* __restrict__ Zt) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = blockIdx.x * 4 + wave;
+    const size_t kk = blockIdx.y;
+    if (m >= ncol) return;
     const double* l = lam + kk * N;
-    const double tn = tnorm[kk];
-    const double ctol = ctol_rel * tn;
+    const double ctol = ctol_rel * tnorm[kk];
     if (m > 0 && l[m] - l[m - 1] <= ctol) return;
     int mend = m + 1;
     while (mend < ncol && l[mend] - l[mend - 1] <= ctol) ++mend;
-    const double eps = 2.220446049250313e-16;
-    const double tol = fmax(eps * tn, 2.2250738585072014e-308);
-    const size_t NZ = (size_t)N * ldz;
-    double* Z = Zt + kk * NZ;
-    double* U0 = scr + (kk * 3 + 0) * NZ;
-    double* U1 = scr + (kk * 3 + 1) * NZ;
-    double* U2 = scr + (kk * 3 + 2) * NZ;
-    double xprev = 0.0;
-    bool bad = false;
-    for (int mm = m; mm < mend; ++mm) {
-        double x = l[mm];
-        if (mm > m) {
-            const double pert = 10.0 * eps * fabs(x);
-            if (x - xprev < pert) x = xprev + pert;
-        }
-        xprev = x;
-        const unsigned long long seed =
-            0x51A7E5EEDull ^ ((unsigned long long)(kbase + (long long)kk) << 24) ^ (unsigned long long)mm;
-        for (int i = 0; i < N; ++i) Z[(size_t)i * ldz + mm] = invit_start(seed, i);
-        for (int it = 0; it < 3; ++it) {
-            // factor + forward substitution
-            double acur = d[0] - x, bcur = N > 1 ? e[0] : 0.0;
-            double ycur = Z[mm];
-            for (int k = 0; k + 1 < N; ++k) {
-                const double c = e[k];
-                const double anext = d[k + 1] - x;
-                const double bnext = k + 2 < N ? e[k + 1] : 0.0;
-                const double ynext = Z[(size_t)(k + 1) * ldz + mm];
-                const size_t o = (size_t)k * ldz + mm;
-                if (fabs(acur) >= fabs(c)) {
-                    const double mult = acur != 0.0 ? c / acur : 0.0;
-                    U0[o] = acur;
-                    U1[o] = bcur;
-                    U2[o] = 0.0;
-                    Z[o] = ycur;
-                    acur = anext - mult * bcur;
-                    bcur = bnext;
-                    ycur = ynext - mult * ycur;
-                } else {
-                    const double mult = acur / c;
-                    U0[o] = c;
-                    U1[o] = anext;
-                    U2[o] = bnext;
-                    Z[o] = ynext;
-                    const double na = bcur - mult * anext;
-                    bcur = -mult * bnext;
-                    acur = na;
-                    ycur = ycur - mult * ynext;
-                }
-            }
-            U0[(size_t)(N - 1) * ldz + mm] = acur;
-            Z[(size_t)(N - 1) * ldz + mm] = ycur;
-            // back substitution
-            double x1 = 0.0, x2 = 0.0, amax = 0.0;
-            for (int k = N - 1; k >= 0; --k) {
-                const size_t o = (size_t)k * ldz + mm;
-                double p = U0[o];
-                if (fabs(p) < tol) p = p >= 0.0 ? tol : -tol;
-                double v = Z[o];
-                if (k + 1 < N) v -= U1[o] * x1;
-                if (k + 2 < N) v -= U2[o] * x2;
-                v /= p;
-                Z[o] = v;
-                x2 = x1;
-                x1 = v;
-                amax = fmax(amax, fabs(v));
-            }
-            // orthogonalise against the earlier members of the cluster
-            for (int mp = m; mp < mm; ++mp) {
+    if (mend - m < 2) return;
+    double* Z = Zt + kk * (size_t)N * ldz;
+    for (int q = m; q < mend; ++q) {
+        for (int pass = 0; pass < 2; ++pass)
+            for (int p = m; p < q; ++p) {
                 double dot = 0.0;
-                for (int i = 0; i < N; ++i) dot += Z[(size_t)i * ldz + mp] * Z[(size_t)i * ldz + mm];
-                amax = 0.0;
-                for (int i = 0; i < N; ++i) {
-                    const double v = Z[(size_t)i * ldz + mm] - dot * Z[(size_t)i * ldz + mp];
-                    Z[(size_t)i * ldz + mm] = v;
-                    amax = fmax(amax, fabs(v));
-                }
+                for (int i = lane; i < N; i += 64) dot += Z[(size_t)i * ldz + p] * Z[(size_t)i * ldz + q];
+                dot = wave_sum(dot);
+                for (int i = lane; i < N; i += 64) Z[(size_t)i * ldz + q] -= dot * Z[(size_t)i * ldz + p];
             }
-            if (!(amax > 0.0) || !isfinite(amax)) {
-                bad = true;
-                amax = 1.0;
-            }
-            if (it < 2) {
-                const double s = 1.0 / amax;
-                for (int i = 0; i < N; ++i) Z[(size_t)i * ldz + mm] *= s;
-            }
-        }
         double nrm = 0.0;
-        for (int i = 0; i < N; ++i) {
-            const double v = Z[(size_t)i * ldz + mm];
+        for (int i = lane; i < N; i += 64) {
+            const double v = Z[(size_t)i * ldz + q];
             nrm += v * v;
         }
-        nrm = sqrt(nrm);
-        if (!(nrm > 0.0) || !isfinite(nrm)) {
-            bad = true;
-            nrm = 1.0;
-        }
-        const double s = 1.0 / nrm;
-        for (int i = 0; i < N; ++i) Z[(size_t)i * ldz + mm] *= s;
+        nrm = wave_sum(nrm);
+        const double sc = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+        for (int i = lane; i < N; i += 64) Z[(size_t)i * ldz + q] *= sc;
     }
-    if (bad) atomicOr(status, kStatNonFinite);
 }
 
 // A (reflectors below the subdiagonal, beta on it) -> explicit Y in place: column c < N-1

@@ -1088,7 +1088,9 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
         {
             StageScope sc(c, ST_BVEC, Kc);
             hipLaunchKernelGGL(lqm::k_invit, dim3((ncol + 63) / 64, Kc), dim3(64), 0, s, N, ncol, ldz, dd, ee, lam,
-                               tn, kClusterRel, Zt, scr, k0, c->d_status);
+                               tn, Zt, scr, k0, c->d_status);
+            hipLaunchKernelGGL(lqm::k_invit_orth, dim3((ncol + 3) / 4, Kc), dim3(256), 0, s, N, ncol, ldz, lam, tn,
+                               kClusterRel, Zt);
         }
         LQM_HIP(hipGetLastError());
         // V = Q Z: Y explicit in A, T per block of kBwy reflectors, blocks applied last first
