@@ -47,7 +47,7 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -403,11 +403,17 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const R* ev
             }
             LQM_HIP(hipGetLastError());
             StageScope sc(c, ST_BACKTRANSFORM, Kc);
+            const int nbw = (N - 1 + lqm::kWyB - 1) / lqm::kWyB;
+            cx<double>* Tg = nullptr;
+            lqm_status st;
+            if ((st = wsT(c, W_WYT, (size_t)std::max(1, nbw) * lqm::kWyB * lqm::kWyB * Kc, &Tg))) return st;
+            if (nbw > 0)
+                hipLaunchKernelGGL(lqm::k_wy_tfac<R>, dim3(nbw, Kc), dim3(64), 0, c->stream, N, refl, tau, Tg);
             auto kW = lqm::k_backtransform_wy<R, MODE>;
             const size_t sh = lqm::wy_lds_bytes();
             allow_lds(kW, sh);
             hipLaunchKernelGGL(kW, dim3(Kc), dim3(256), sh, c->stream, N, refl, tau, Z, evals, perm, kw, wuni, mu,
-                               T, c->occ, out, eb);
+                               T, c->occ, out, eb, Tg);
             LQM_HIP(hipGetLastError());
             return LQM_OK;
         }

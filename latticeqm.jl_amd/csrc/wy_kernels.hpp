@@ -10,10 +10,11 @@
 // quadrant V(32 wr + ., 32 wc + .) as matrix-core accumulators (lane l, element e of tile
 // (a, b): row 32 wr + 16 a + l/16 + 4 e, column 32 wc + 16 b + l%16).  Y^H V is formed from
 // those registers directly (contraction row r = 16 a + l/16 + 4 s in instruction s, which is
-// the row the lane already holds), the two row-halves meeting in LDS; P, U = Y T and the
-// update V -= U W are matrix-core products too.  LDS holds only the block's Y, U, W and P
-// (80 KB), so two k-points share a CU.  The reflectors and Z may be FP32 (mixed path) or
-// FP64; all arithmetic is FP64.
+// the row the lane already holds), the two row-halves meeting in LDS; U = Y T and the update
+// V -= U W are matrix-core products too.  The T factors come from k_wy_tfac (they depend on
+// the reflectors only), so a block of reflectors costs three barriers and no serial section.
+// LDS holds the block's Y, U, W and T (72 KB), so two k-points share a CU.  The reflectors
+// and Z may be FP32 (mixed path) or FP64; all arithmetic is FP64.
 //   MODE 0: write the eigenvectors, column n = V(:, perm[n]) (ascending eigenvalues);
 //   MODE 1: rho_k = V diag(f) V^H with f = fermidirac(e - mu; T) (or the given occupations)
 //           and eb[kk] = w_k sum_n e_n f_n, as k_backtransform MODE 1.
@@ -30,13 +31,65 @@ struct WySmem {
     cx<double> Y[64 * kWyB];           // Y(r, i) at r * 16 + i
     cx<double> U[64 * kWyB];           // U = Y T, same layout
     cx<double> W[2][64 * (kWyB + 1)];  // row-half partials of W = Y^H V: W(i, c) at c * 17 + i
-    cx<double> P[2][kWyB * kWyB];      // row-half partials of P = Y^H Y: P(p, q) at q * 16 + p
     cx<double> T[kWyB * kWyB];         // T(row, col) at row * 16 + col
-    cx<double> tau[kWyB];
     int invp[64];                      // output column of each Z column (MODE 0)
     double sf[64];                     // occupation of each Z column (MODE 1)
     double red[8];
 };
+
+// T factors of every block of 16 reflectors, ahead of the back-transformation (T depends
+// on the reflectors only): one wave per (block, k-point) forms P = Y^H Y on the matrix
+// cores (K = the 64 rows, lane (li, g) supplying row 4 step + g of column li as both
+// operands) and runs the zlarft recurrence T(t, i) = -tau_i sum_{l=t}^{i-1} T(t, l) P(l, i)
+// with lane t holding row t.  Out: Tg[(k * nb + b) * 256 + row * 16 + col] in FP64.
+template <typename R>
+__global__ __launch_bounds__(64) void k_wy_tfac(int N, const cx<R>* __restrict__ refl,
+                                                const cx<R>* __restrict__ tau, cx<double>* __restrict__ Tg) {
+    __shared__ cx<double> P[kWyB * kWyB];  // P(p, q) at q * 16 + p
+    __shared__ cx<double> Ts[kWyB * kWyB];
+    const int lane = threadIdx.x, li = lane & 15, g = lane >> 4;
+    const int b = blockIdx.x, nb = gridDim.x;
+    const size_t kk = blockIdx.y;
+    const int j0 = b * kWyB, nrefl = N - 1;
+    const size_t NN = (size_t)N * N;
+    const cx<R>* rf = refl + kk * NN;
+    lqm_d4 pr = {0.0, 0.0, 0.0, 0.0}, pi = {0.0, 0.0, 0.0, 0.0};
+    const int j = j0 + li;
+#pragma unroll 4
+    for (int st = 0; st < 16; ++st) {
+        const int r = 4 * st + g;
+        cx<double> y = {0.0, 0.0};
+        if (j < nrefl && r > j && r < N) {
+            const cx<R> v = rf[(size_t)j * N + r];
+            y = {(double)v.re, (double)v.im};
+        }
+        // A(m = li, k = r) = conj(Y(r, li)), B(k = r, n = li) = Y(r, li)
+        pr = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, y.re, pr, 0, 0, 0);
+        pi = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, y.im, pi, 0, 0, 0);
+        pr = __builtin_amdgcn_mfma_f64_16x16x4f64(y.im, y.im, pr, 0, 0, 0);
+        pi = __builtin_amdgcn_mfma_f64_16x16x4f64(-y.im, y.re, pi, 0, 0, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) P[li * kWyB + g + 4 * e] = cx<double>{pr[e], pi[e]};
+    __syncthreads();
+    if (lane < kWyB) {
+        cx<double>* trow = Ts + lane * kWyB;
+#pragma unroll 1
+        for (int i = 0; i < kWyB; ++i) {
+            cx<double> acc = {0.0, 0.0};
+            for (int l = lane; l < i; ++l) acc = cmad(acc, trow[l], P[i * kWyB + l]);
+            cx<double> ti = {0.0, 0.0};
+            if (j0 + i < nrefl) {
+                const cx<R> x = tau[kk * N + j0 + i];
+                ti = {(double)x.re, (double)x.im};
+            }
+            trow[i] = lane < i ? cx<double>{0.0, 0.0} - ti * acc : (lane == i ? ti : cx<double>{0.0, 0.0});
+        }
+    }
+    __syncthreads();
+    cx<double>* To = Tg + (kk * nb + b) * (size_t)(kWyB * kWyB);
+    for (int e = lane; e < kWyB * kWyB; e += 64) To[e] = Ts[e];
+}
 inline size_t wy_lds_bytes() {
     const size_t v = sizeof(cx<double>) * 64 * kWyLd;  // staged V of MODE 1 overlays Y, U, W
     return sizeof(WySmem) > v ? sizeof(WySmem) : v;
@@ -47,7 +100,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     int N, const cx<R>* __restrict__ refl, const cx<R>* __restrict__ tau, const R* __restrict__ Zg,
     const R* __restrict__ evals, const int* __restrict__ perm, const double* __restrict__ kw,
     double wuni, double mu, double Temp, const double* __restrict__ occ, cx<double>* __restrict__ out,
-    double* __restrict__ eb) {
+    double* __restrict__ eb, const cx<double>* __restrict__ Tg) {
     extern __shared__ __align__(16) unsigned char smem[];
     WySmem& S = *reinterpret_cast<WySmem*>(smem);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -88,18 +141,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             }
             S.Y[e] = y;
         }
-        if (tid < kWyB) {
-            const int j = j0 + tid;
-            cx<double> t = {0.0, 0.0};
-            if (j < nrefl) {
-                const cx<R> x = tau[(size_t)kk * N + j];
-                t = {(double)x.re, (double)x.im};
-            }
-            S.tau[tid] = t;
+        {
+            const cx<double>* Tb = Tg + ((size_t)kk * nb + b0) * (kWyB * kWyB);
+            S.T[tid] = Tb[tid];  // 256 threads, 256 entries
         }
         __syncthreads();
         // (2) row-half partials W_wr = Y(R_wr, :)^H V(R_wr, C_wc) (B operand = the lane's own
-        //     accumulators) and, by the waves of column half 0, P_wr = Y(R_wr, :)^H Y(R_wr, :)
+        //     accumulators), and U = Y T (wave w: rows 16w .. 16w + 15, one 16 x 16 tile, K = 16)
         {
             lqm_d4 wr_[2], wi_[2];
 #pragma unroll
@@ -107,7 +155,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 wr_[b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
                 wi_[b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
             }
-            lqm_d4 pr = {0.0, 0.0, 0.0, 0.0}, pi = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -122,45 +169,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                         wr_[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(y.im, bi, wr_[b], 0, 0, 0);
                         wi_[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-y.im, br, wi_[b], 0, 0, 0);
                     }
-                    if (wc == 0) {
-                        const cx<double> q = S.Y[r * kWyB + li];  // B(k = r, n = li) = Y(r, li)
-                        pr = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, q.re, pr, 0, 0, 0);
-                        pi = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, q.im, pi, 0, 0, 0);
-                        pr = __builtin_amdgcn_mfma_f64_16x16x4f64(y.im, q.im, pr, 0, 0, 0);
-                        pi = __builtin_amdgcn_mfma_f64_16x16x4f64(-y.im, q.re, pi, 0, 0, 0);
-                    }
                 }
 #pragma unroll
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
                     S.W[wr][(n0 + 16 * b + li) * (kWyB + 1) + g + 4 * e] = cx<double>{wr_[b][e], wi_[b][e]};
-            if (wc == 0) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) S.P[wr][li * kWyB + g + 4 * e] = cx<double>{pr[e], pi[e]};
-            }
-        }
-        __syncthreads();
-        // (3) W = W_0 + W_1 (in place in W[0]); T by lane t of wave 0 holding row t:
-        //     T(t, i) = -tau_i sum_{l=t}^{i-1} T(t, l) P(l, i)
-        for (int e = tid; e < 64 * (kWyB + 1); e += 256) S.W[0][e] = S.W[0][e] + S.W[1][e];
-        if (wave == 0 && lane < kWyB) {
-            // the lane's own row of T is built in LDS (a lane reads back only what it wrote)
-            cx<double>* trow = S.T + lane * kWyB;
-#pragma unroll 1
-            for (int i = 0; i < kWyB; ++i) {
-                cx<double> acc = {0.0, 0.0};
-                for (int l = lane; l < i; ++l) {
-                    const cx<double> p = S.P[0][i * kWyB + l] + S.P[1][i * kWyB + l];
-                    acc = cmad(acc, trow[l], p);
-                }
-                const cx<double> ti = S.tau[i];
-                trow[i] = lane < i ? cx<double>{0.0, 0.0} - ti * acc : (lane == i ? ti : cx<double>{0.0, 0.0});
-            }
-        }
-        __syncthreads();
-        // (4) U = Y T: wave w forms rows 16w .. 16w + 15 (one 16 x 16 tile, K = 16)
-        {
             lqm_d4 ur = {0.0, 0.0, 0.0, 0.0}, ui = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
@@ -176,7 +190,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             for (int e = 0; e < 4; ++e) S.U[(16 * wave + g + 4 * e) * kWyB + li] = cx<double>{ur[e], ui[e]};
         }
         __syncthreads();
-        // (5) V <- V - U W on the wave's quadrant (accumulators updated in place)
+        // (3) V <- V - U W, W = W_0 + W_1, on the wave's quadrant (accumulators updated in place)
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int i = 4 * g + s;
@@ -184,7 +198,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
                 const cx<double> u = S.U[(m0 + 16 * t + li) * kWyB + i];  // A(m, k = i) = -U
-                const cx<double> w = S.W[0][(n0 + 16 * t + li) * (kWyB + 1) + i];
+                const cx<double> w = S.W[0][(n0 + 16 * t + li) * (kWyB + 1) + i] +
+                                      S.W[1][(n0 + 16 * t + li) * (kWyB + 1) + i];
                 ur[t] = -u.re;
                 ui[t] = -u.im;
                 wre[t] = w.re;
