@@ -47,8 +47,8 @@ __global__ __launch_bounds__(kTPB) void k_hk_tridiag(
     } else {
         A = gA + (size_t)kk * N * lda;
     }
-    cx<R>* sv = reinterpret_cast<cx<R>*>(p);
-    p += sizeof(cx<R>) * N;
+    cx<R>* sv = reinterpret_cast<cx<R>*>(p);  // [2][N]: v of step j in sv[(j & 1) * N]
+    p += sizeof(cx<R>) * 2 * N;
     cx<R>* sp = reinterpret_cast<cx<R>*>(p);
     p += sizeof(cx<R>) * N;
     cx<double>* sph = reinterpret_cast<cx<double>*>(p);
@@ -92,41 +92,50 @@ __global__ __launch_bounds__(kTPB) void k_hk_tridiag(
     __syncthreads();
 
     cx<R>* rf = refl + (size_t)kk * NN;
+    // Householder vector of column j (zlarfg semantics: H^H x = beta e_1 with beta real),
+    // formed by wave 0 into sv[(j & 1) N] from A as it stands.  Column j+1 is relative
+    // column 0 of step j's trailing matrix, which wave 0 updates in full (columns go to
+    // waves by residue mod 4, all rows of a column to one wave): so wave 0 forms v_{j+1}
+    // right after its own part of the step-j update, without a barrier, while the other
+    // waves finish theirs; v and tau are double-buffered by step parity.
+    auto householder = [&](int j) {
+        const int m = N - 1 - j;
+        cx<R>* svb = sv + (j & 1) * N;
+        R xn2 = 0;
+        for (int i = j + 2 + lane; i < N; i += kWave) xn2 += abs2(At(i, j));
+        xn2 = wave_sum(xn2);
+        cx<R> alpha = At(j + 1, j);
+        cx<R> tj = {0, 0}, scale = {0, 0};
+        R beta = alpha.re;
+        if (!(xn2 == (R)0 && alpha.im == (R)0)) {
+            R nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+            beta = alpha.re >= (R)0 ? -nrm : nrm;
+            tj = mkc<R>((beta - alpha.re) / beta, -alpha.im / beta);
+            scale = crecip(mkc<R>(alpha.re - beta, alpha.im));
+        }
+        const bool triv = (tj.re == (R)0 && tj.im == (R)0);
+        for (int ip = lane; ip < m; ip += kWave) {
+            cx<R> v = (ip == 0) ? mkc<R>(1, 0) : (triv ? mkc<R>(0, 0) : At(j + 1 + ip, j) * scale);
+            svb[ip] = v;
+            rf[(size_t)j * N + j + 1 + ip] = v;
+        }
+        if (lane == 0) {
+            sred[4 + (j & 1)] = tj;
+            tau[(size_t)kk * N + j] = tj;
+            dd[(size_t)j * Nkc + kk] = At(j, j).re;
+            ee[(size_t)j * Nkc + kk] = beta;
+        }
+    };
+    if (N > 1 && wave == 0) householder(0);
     for (int j = 0; j + 1 < N; ++j) {
         const int m = N - 1 - j;  // length of the reflected column x = A[j+1:N, j]
-        // (a) Householder vector of x, computed by wave 0 (zlarfg semantics:
-        //     H^H x = beta e_1 with beta real).
-        if (wave == 0) {
-            R xn2 = 0;
-            for (int i = j + 2 + lane; i < N; i += kWave) xn2 += abs2(At(i, j));
-            xn2 = wave_sum(xn2);
-            cx<R> alpha = At(j + 1, j);
-            cx<R> tj = {0, 0}, scale = {0, 0};
-            R beta = alpha.re;
-            if (!(xn2 == (R)0 && alpha.im == (R)0)) {
-                R nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
-                beta = alpha.re >= (R)0 ? -nrm : nrm;
-                tj = mkc<R>((beta - alpha.re) / beta, -alpha.im / beta);
-                scale = crecip(mkc<R>(alpha.re - beta, alpha.im));
-            }
-            const bool triv = (tj.re == (R)0 && tj.im == (R)0);
-            for (int ip = lane; ip < m; ip += kWave) {
-                cx<R> v = (ip == 0) ? mkc<R>(1, 0) : (triv ? mkc<R>(0, 0) : At(j + 1 + ip, j) * scale);
-                sv[ip] = v;
-                rf[(size_t)j * N + j + 1 + ip] = v;
-            }
-            if (lane == 0) {
-                sred[4 + (j & 1)] = tj;
-                tau[(size_t)kk * N + j] = tj;
-                dd[(size_t)j * Nkc + kk] = At(j, j).re;
-                ee[(size_t)j * Nkc + kk] = beta;
-            }
-        }
-        __syncthreads();
-        // tau slot alternates so a wave still reading step j never races wave 0's
-        // write for step j+1 on the skip path (no barrier there)
+        __syncthreads();          // v_j, tau_j formed; the step j-1 update is complete
         const cx<R> tj = sred[4 + (j & 1)];
-        if (tj.re == (R)0 && tj.im == (R)0) continue;  // H_j = I (uniform branch)
+        const cx<R>* svj = sv + (j & 1) * N;
+        if (tj.re == (R)0 && tj.im == (R)0) {  // H_j = I (uniform branch)
+            if (wave == 0 && j + 2 < N) householder(j + 1);
+            continue;
+        }
 
         // (b) p = tau * A22 v  (4 lanes per row, shuffle-reduced) and partial p^H v
         cx<R> dotp = {0, 0};
@@ -135,7 +144,7 @@ __global__ __launch_bounds__(kTPB) void k_hk_tridiag(
             for (int r = tid >> 2; r < m; r += kTPB / 4) {
                 cx<R> acc = {0, 0};
                 const cx<R>* arow = &At(j + 1 + r, j + 1);
-                for (int c = q; c < m; c += 4) acc = cmad(acc, arow[(size_t)c * lda], sv[c]);
+                for (int c = q; c < m; c += 4) acc = cmad(acc, arow[(size_t)c * lda], svj[c]);
                 acc.re += __shfl_xor(acc.re, 1);
                 acc.im += __shfl_xor(acc.im, 1);
                 acc.re += __shfl_xor(acc.re, 2);
@@ -143,7 +152,7 @@ __global__ __launch_bounds__(kTPB) void k_hk_tridiag(
                 if (q == 0) {
                     cx<R> pr = tj * acc;
                     sp[r] = pr;
-                    dotp = cmadc(dotp, pr, sv[r]);
+                    dotp = cmadc(dotp, pr, svj[r]);
                 }
             }
         }
@@ -158,10 +167,10 @@ __global__ __launch_bounds__(kTPB) void k_hk_tridiag(
             const int rr = tid & (kWave - 1);
             const int cq = tid / kWave;
             for (int r = rr; r < m; r += kWave) {
-                const cx<R> vr = sv[r];
+                const cx<R> vr = svj[r];
                 const cx<R> wr = cmad(sp[r], alpha2, vr);
                 for (int c = cq; c < m; c += kTPB / kWave) {
-                    const cx<R> vc = sv[c];
+                    const cx<R> vc = svj[c];
                     const cx<R> wc = cmad(sp[c], alpha2, vc);
                     cx<R> a = At(j + 1 + r, j + 1 + c);
                     cx<R> u = {0, 0};
@@ -171,7 +180,7 @@ __global__ __launch_bounds__(kTPB) void k_hk_tridiag(
                 }
             }
         }
-        __syncthreads();
+        if (wave == 0 && j + 2 < N) householder(j + 1);
     }
     if (tid == 0) {
         dd[(size_t)(N - 1) * Nkc + kk] = At(N - 1, N - 1).re;

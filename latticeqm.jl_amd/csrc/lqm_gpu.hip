@@ -164,9 +164,9 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     p.N = N;
     p.nR = nR;
     const size_t cC = 2 * cR;  // complex element of the eigensolver's working precision
-    p.shA = cC * ((size_t)N * (N + 1) + 2 * N + 6) + sizeof(cx<double>) * nR;
+    p.shA = cC * ((size_t)N * (N + 1) + 3 * N + 6) + sizeof(cx<double>) * nR;  // A, v[2], p, scalars
     p.ldsA = p.shA <= (size_t)c->lds_max;
-    if (!p.ldsA) p.shA = cC * (2 * N + 6) + sizeof(cx<double>) * nR;
+    if (!p.ldsA) p.shA = cC * (3 * N + 6) + sizeof(cx<double>) * nR;
     p.rch = std::max(lqm::kRotChunk, N);
     const size_t smallC = cR * 2 * p.rch + cC * 4 * N + 8 * N + 32 + sizeof(int) * lqm::kHdrChunk + 16;
     p.shC = 2 * cR * (size_t)N * (N + 1) + smallC;
