@@ -267,6 +267,50 @@ def batch_leg() -> dict:
             "scf_iterations_per_s": its / out[4], "speedup_4_vs_1_stream": out[1] / out[4]}
 
 
+def cfg4_leg() -> dict:
+    """Config 4 at full size: 256 models with N_m = even round(exp U(ln 16, ln 512)) (seed 4),
+    each a random tight-binding model built like the headline one (25 keys, sub-seed 4000 + m),
+    onsite Hubbard U_m ~ U(1, 4) in the sigma_x form on the N/2 site pairs, half filling,
+    kT = 0.02, beta = 0.5, a 32 x 32 grid each (Nk = 1024), FP64: ONE mean-field iteration of
+    the whole batch on 4 streams; the large-N path serves N > 112.  Each model's iteration is
+    one pass over its grid (H(k), eigenvalues, mu, eigenvectors, rho): the flop count is that
+    single factorization, 64/3 N^3 + 8 (nR + 1) N^2 per k-point."""
+    import importlib
+
+    import torch
+
+    mf = importlib.import_module("latticeqm_amd.meanfield")
+    rng = np.random.default_rng(4)
+    ks = pkg.regulargrid(nk=32 * 32)
+    Nk = ks.shape[1]
+    sx = np.array([[0, 1], [1, 0]], dtype=complex)
+    probs, Ns = [], []
+    for m in range(256):
+        N = int(2 * round(np.exp(rng.uniform(np.log(16), np.log(512))) / 2))
+        U = float(rng.uniform(1, 4))
+        h = pkg.models.random_tb(N, rmax=2, seed=4000 + m)
+        v = pkg.Hops({(0, 0): U * np.kron(np.eye(N // 2), sx)})
+        rho0 = mf.staggered_guess(v, np.arange(N // 2) % 2, m=0.2, noise=1e-3, seed=m)
+        probs.append((h, v, rho0, 0.5, ks))
+        Ns.append(N)
+    kw = dict(T=0.02, beta=0.5, tol=1e-12, iterations=1)
+    ex = api.to_executor("gpu")
+    order = np.argsort(Ns)
+    mf.solvehartreefock_batch([probs[i] for i in order[-4:]] + [probs[i] for i in order[:4]], nstreams=4,
+                              executor=ex, **kw)  # warm-up: workspaces of the largest models
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = mf.solvehartreefock_batch(probs, nstreams=4, executor=ex, **kw)
+    t = time.perf_counter() - t0
+    nR = 25
+    fl = sum((64.0 / 3.0) * N ** 3 + 8.0 * (nR + 1) * N * N for N in Ns) * Nk
+    return {"workload": "cfg4_256_random_models_N16-512_R25_k32x32_hubbard_one_iteration", "models": len(probs),
+            "N_min": int(min(Ns)), "N_max": int(max(Ns)), "sum_N3": float(sum(float(N) ** 3 for N in Ns)),
+            "Nk": Nk, "seconds_per_iteration": t, "batch_iterations_per_s": 1.0 / t,
+            "model_iterations_per_s": len(probs) / t, "tflop_per_iteration": fl / 1e12,
+            "path_tflops_per_s": fl / t / 1e12, "all_finite": bool(all(np.isfinite(r[1]) for r in res))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +323,7 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the grand-potential adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-SCF leg")
     ap.add_argument("--no-tbg", action="store_true", help="skip the 1588-orbital twisted-bilayer leg")
+    ap.add_argument("--no-cfg4", action="store_true", help="skip the 256-model batch iteration leg")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after timing, copy this many MiB device-to-device once (HBM counter calibration)")
     args = ap.parse_args()
@@ -451,6 +496,7 @@ def main() -> int:
     adj = None if args.no_adjoint else adjoint_leg(args.no_cpu)
     bat = None if args.no_batch else batch_leg()
     tbg = None if args.no_tbg else tbg_leg(args.no_cpu)
+    cfg4 = None if args.no_cfg4 else cfg4_leg()
 
     line = {
         "metric": METRIC,
@@ -484,6 +530,7 @@ def main() -> int:
         "adjoint": adj,
         "batch_scf": bat,
         "tbg_large_n": tbg,
+        "cfg4_batch_iteration": cfg4,
         **out,
     }
     print(json.dumps(line))

@@ -517,9 +517,21 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
                      int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands,
                      const double* occ = nullptr);
 
+// Chemical potential requested from the pass itself: when the whole grid is one chunk, mu
+// is solved (filling, the pass's own eigenvalues) between the eigenvalues and the
+// occupations, and written to *mu; otherwise the pass returns kFallback untouched.
+struct MuReq {
+    double filling;
+    double* mu;
+};
+// internal status, never returned through the C ABI: take the two-pass route (7 lies in the
+// enumeration's value range, above every public code)
+constexpr lqm_status kFallback = static_cast<lqm_status>(7);
+lqm_status mu_from_bands(lqm_context* c, int N, int Nk, const double* bands, const double* kw,
+                         double filling, double T, double* mu);
 lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
                        int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands,
-                       double* evecs, const double* occ);
+                       double* evecs, const double* occ, const MuReq* mq = nullptr);
 bool use_blocked(const lqm_context* c, int N);
 
 // Clears the context's occupation pointer when a pass ends, on every return path.
@@ -559,8 +571,10 @@ lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, doub
 // place of fermidirac(e - mu; T) -- rho then is sum_k w_k e^{-2 pi i k.R} U diag(occ_k) U^H.
 lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
                    int nRr, const int32_t* Rr, double* rho, double* eband, double* bands_out,
-                   const double* occ = nullptr) {
+                   const double* occ = nullptr, const MuReq* mq = nullptr) {
     lqm_context* c = m->ctx;
+    if (mq && !use_blocked(c, m->N) && c->precision == LQM_PREC_FP32_REFINED)
+        return kFallback;  // the refined eigenvalues come with the occupations in one kernel
     const int N = m->N, d = m->d;
     if (Nk <= 0) return fail(LQM_ERR_ARG, "Nk must be positive");
     if (nRr < 0 || (nRr > 0 && !Rr)) return fail(LQM_ERR_ARG, "bad rho key list");
@@ -575,11 +589,12 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
         LQM_HIP(hipMemcpyAsync(dRr, Rr, sizeof(int) * nRr * d, hipMemcpyHostToDevice, c->stream));
     if (use_blocked(c, N))
         return run_blocked(m, Nk, ks, kw, mu, T, nRr, dRr, reinterpret_cast<cx<double>*>(rho), eband,
-                           bands_out, nullptr, occ);
+                           bands_out, nullptr, occ, mq);
     if (c->precision == LQM_PREC_FP32_REFINED)
         return run_mixed(m, Nk, ks, kw, mu, T, nRr, dRr, reinterpret_cast<cx<double>*>(rho), eband,
                          bands_out, occ);
     Plan P = make_plan(c, N, m->nR, Nk, true, true);
+    if (mq && P.Kc < Nk) return kFallback;  // mu needs every eigenvalue before the first occupation
     const int RG = nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : 32;
     const int groups = std::max(1, (nRr + RG - 1) / RG);
     const int tilesE = (int)((NN + 255) / 256);
@@ -600,6 +615,10 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
             return st;
         }
         if ((st = launch_eigvals(m, P, Kc, ksc, ev, true))) return st;
+        if (mq) {
+            if ((st = mu_from_bands(c, N, Nk, ev, kw, mq->filling, T, &mu))) return st;
+            *mq->mu = mu;
+        }
         cx<double>* rhok;
         double* eb;
         if ((st = wsT(c, W_RHOK, NN * P.Kc, &rhok))) return st;
@@ -984,7 +1003,7 @@ lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<double>* A, cx<doub
 // caller) and eband.  occ: given occupations of the ascending eigenvalues (device).
 lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
                        int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands,
-                       double* evecs, const double* occ) {
+                       double* evecs, const double* occ, const MuReq* mq) {
     lqm_context* c = m->ctx;
     const int N = m->N, d = m->d;
     if (N > lqm::kBMaxN) return fail(LQM_ERR_UNSUPPORTED, "blocked eigensolver supports N <= 4096");
@@ -1000,6 +1019,7 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
     if (vectors) perk += 8 * NN + 24 * NN + 16 * NN + 16 * (size_t)nblk * lqm::kBwy * lqm::kBwy + 2 * 16 * 64 * (size_t)N;
     if (want_rho) perk += 16 * NN;
     const int Kmax = (int)std::max<size_t>(1, std::min<size_t>({kBlockedBudget / perk, (size_t)Nk, (size_t)65535}));
+    if (mq && Kmax < Nk) return kFallback;  // mu needs every eigenvalue before the first occupation
     const double wuni = 1.0 / (double)Nk;
     lqm_status st;
     const int RG = nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : 32;
@@ -1048,6 +1068,10 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
             hipLaunchKernelGGL(lqm::k_sturm_bisect, dim3((N + 255) / 256, Kc), dim3(256), 0, s, N, dd, ee, lam, tn);
         }
         LQM_HIP(hipGetLastError());
+        if (mq) {
+            if ((st = mu_from_bands(c, N, Nk, lam, kw, mq->filling, T, &mu))) return st;
+            *mq->mu = mu;
+        }
         if (!vectors) continue;
         // columns needed: all for eigenvectors or given occupations, else the eigenvalues
         // whose occupation reaches kOccTiny (a prefix, f falling with the eigenvalue),
@@ -1857,11 +1881,18 @@ lqm_status lqm_hartreefock_scf(lqm_context* ctx, int N, int d, int nRh, const in
         ++it;
         // f!(x1, x0): H_MF[rho0]; mu; rho1 and eps_band  (selfconsistent.jl:281-295, :374-381)
         if ((st = hf_apply(S, S.rho0))) return st;
-        if ((st = run_bands(S.hmf, Nk, ksd, S.bands, nullptr))) return st;
-        if ((st = mu_from_bands(c, N, Nk, S.bands, kwd, filling, T, &mu))) return st;
-        if ((st = run_rho(S.hmf, Nk, ksd, kwd, mu, T, nRr, Rr, reinterpret_cast<double*>(S.rho1),
-                          S.scal + 4, nullptr)))
-            return st;
+        // one factorization for mu and rho when the grid is one chunk (FP64 eigenvalues);
+        // else bands, mu, then rho in a second pass
+        const MuReq mq{filling, &mu};
+        st = run_rho(S.hmf, Nk, ksd, kwd, 0.0, T, nRr, Rr, reinterpret_cast<double*>(S.rho1), S.scal + 4,
+                     S.bands, nullptr, &mq);
+        if (st == kFallback) {
+            if ((st = run_bands(S.hmf, Nk, ksd, S.bands, nullptr))) return st;
+            if ((st = mu_from_bands(c, N, Nk, S.bands, kwd, filling, T, &mu))) return st;
+            st = run_rho(S.hmf, Nk, ksd, kwd, mu, T, nRr, Rr, reinterpret_cast<double*>(S.rho1), S.scal + 4,
+                         nullptr);
+        }
+        if (st) return st;
         {
             StageScope scm(c, ST_MIX);
             hipLaunchKernelGGL(lqm::k_mix, dim3(kMixBlocks), dim3(256), 0, c->stream, nrho, beta, S.rho1,
