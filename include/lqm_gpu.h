@@ -94,7 +94,11 @@ const char *lqm_stage_name(int stage);
  * tridiagonalization with zlatrd panels of 32 and matrix-core trailing updates, Sturm
  * bisection, inverse iteration with clusters closer than 1e-5 ||T|| orthogonalised,
  * compact-WY back-transformation and rho_k on the matrix cores; FP64 in both precision
- * modes, N <= 4096; occupations below 1e-18 leave rho_k out).
+ * modes, N <= 4096; occupations below 1e-18 leave rho_k out); kind 6: eigenpairs of the
+ * tridiagonal ahead of the compact-WY back-transformation (0 = QL with a rotation log, then
+ * its replay; 1 = the default: bisection with Newton steps and twisted-factorisation
+ * eigenvectors, O(N^2) per k-point, with the QL path for the k-points whose eigenvalues
+ * come closer than 1e-3 ||T||).
  * variant -1 returns to the tuned/default
  * choice.  lqm_autotune times every variant of every kind on Nk device k-points of
  * the model and records the fastest per orbital count N in the context;

@@ -146,7 +146,7 @@ class GPUModel:
 def autotune(model: GPUModel, ks_dev_ptr: int, Nk: int) -> dict:
     """Time every kernel variant on Nk device-resident k-points and keep the fastest."""
     _lib.call("lqm_autotune", model.handle, int(Nk), _c.c_void_p(ks_dev_ptr))
-    return {kind: model.exec.get_variant(kind, model.N) for kind in (0, 1, 3, 5)}
+    return {kind: model.exec.get_variant(kind, model.N) for kind in (0, 1, 3, 5, 6)}
 
 
 def _model(H, executor) -> GPUModel:

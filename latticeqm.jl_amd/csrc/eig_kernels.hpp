@@ -205,13 +205,18 @@ template <typename R, bool LOG>
 __global__ void k_tridiag_ql(int N, int Nkc, const R* __restrict__ dd, const R* __restrict__ ee,
                              R* __restrict__ evals, int* __restrict__ perm, R* __restrict__ rot,
                              int* __restrict__ sweeps, int* __restrict__ nsweeps, int cap_rot,
-                             int cap_sw, int* __restrict__ status) {
+                             int cap_sw, int* __restrict__ status, const int* __restrict__ klist) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int L = blockDim.x;
     const int lane = threadIdx.x;
-    const int kk = blockIdx.x * L + lane;
+    int kk = blockIdx.x * L + lane;
     R* sd = reinterpret_cast<R*>(smem);
     R* se = sd + (size_t)N * L;
+    // klist (optional): klist[0] k-points, listed at klist[1 ...]
+    if (klist) {
+        if (kk >= klist[0]) return;
+        kk = klist[1 + kk];
+    }
     if (kk >= Nkc) return;  // no barriers below: early exit is safe
     auto D = [&](int i) -> R& { return sd[(size_t)i * L + lane]; };
     auto E = [&](int i) -> R& { return se[(size_t)i * L + lane]; };
@@ -452,9 +457,11 @@ template <typename R, int NM>
 __global__ __launch_bounds__(kWave) void k_replay(int N, const R* __restrict__ rot,
                                                    const int* __restrict__ sweeps,
                                                    const int* __restrict__ nsweeps, int cap_rot,
-                                                   int cap_sw, R* __restrict__ Z) {
+                                                   int cap_sw, R* __restrict__ Z,
+                                                   const int* __restrict__ zfmt) {
     const int kk = blockIdx.x;
     const int lane = threadIdx.x;
+    if (zfmt && zfmt[kk]) return;  // eigenvectors already formed by k_tri_twist
     R z[NM];
 #pragma unroll
     for (int c = 0; c < NM; ++c) z[c] = (c == lane) ? R(1) : R(0);

@@ -93,10 +93,12 @@ template <typename R> struct num;
 // sum of squares directly (no overflow, no loss to subnormals).
 template <> struct num<double> {
     static constexpr double half_eps = 1.1102230246251565e-16;  // 2^-53
+    static constexpr double tiny = 2.2250738585072014e-308;     // smallest normal
     static constexpr double rot_lo = 1e-290, rot_hi = 1e290;
 };
 template <> struct num<float> {
     static constexpr float half_eps = 5.9604645e-08f;  // 2^-24
+    static constexpr float tiny = 1.17549435e-38f;     // smallest normal
     static constexpr float rot_lo = 1e-30f, rot_hi = 1e30f;
 };
 

@@ -24,6 +24,7 @@
 #include "reg_kernels.hpp"
 #include "wy_kernels.hpp"
 #include "blocked_kernels.hpp"
+#include "tri_kernels.hpp"
 
 using lqm::cx;
 
@@ -47,10 +48,13 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
+// per-k tridiagonal eigenpairs: a k-point whose eigenvalues come closer than this x ||T||
+// (Gershgorin bound) takes the QL path, whose vectors stay orthogonal in clusters
+constexpr double kTriClusterRel = 1e-3;
 
 }  // namespace
 
@@ -76,11 +80,13 @@ struct TimedLaunch {
 };
 
 // Kernel variants the autotuner chooses between (per N).
-enum VariantKind { VK_TRIDIAG = 0, VK_RHO = 1, VK_REFINE = 2, VK_QL = 3, VK_BT = 4, VK_BLOCKED = 5, VK_N };
+enum VariantKind {
+    VK_TRIDIAG = 0, VK_RHO = 1, VK_REFINE = 2, VK_QL = 3, VK_BT = 4, VK_BLOCKED = 5, VK_TRIEIG = 6, VK_N
+};
 
 struct lqm_context {
     std::map<int, int> tuned;  // (kind << 20 | N) -> variant
-    int forced[VK_N] = {-1, -1, -1, -1, -1, -1};
+    int forced[VK_N] = {-1, -1, -1, -1, -1, -1, -1};
     int timing = 0;
     std::vector<TimedLaunch> pending;
     double stage_ms[ST_N] = {};
@@ -154,6 +160,7 @@ struct Plan {
     int btv = 1;        // back-transformation variant: 1 compact WY, 0 reflector loop
     size_t shAw = 0;
     bool rho2 = false;  // rho_k by two-sided reflectors on the packed Z f Z^T
+    bool tri2 = false;  // tridiagonal eigenpairs by bisection + twisted factorisation (QL for clusters)
     size_t shM = 0, shR2 = 0;
     int Kc = 0;   // k-points per chunk
 };
@@ -189,7 +196,7 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     auto variant = [&](int kind) {
         if (c->forced[kind] >= 0) return c->forced[kind];
         auto it = c->tuned.find((kind << 20) | N);
-        return it == c->tuned.end() ? (kind == VK_BT && N > 16 ? 1 : 0) : it->second;
+        return it == c->tuned.end() ? ((kind == VK_BT && N > 16) || kind == VK_TRIEIG ? 1 : 0) : it->second;
     };
     // QL: k-points (lanes) per block, 64 >> variant; fewer lanes per wave put more
     // independent recurrences on each SIMD
@@ -218,6 +225,11 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     p.rho2 = variant(VK_RHO) == 1 && rho && p.nm && p.shM <= (size_t)c->lds_max &&
              p.shR2 <= (size_t)c->lds_max;
     if (p.rho2) perk += 8 * NN;
+    // tridiagonal eigenpairs: 0 = QL + rotation replay, 1 = bisection + twisted
+    // factorisation with the QL path for k-points holding an eigenvalue cluster (feeds the
+    // compact-WY back-transformation only)
+    p.tri2 = (vectors || rho) && variant(VK_TRIEIG) == 1 && p.wyC && p.btv == 1 && !p.rho2;
+    perk += 8;
     size_t kc = kWorkspaceBudget / perk;
     if (kc < 1) kc = 1;
     p.Kc = (int)std::min<size_t>(kc, (size_t)Nk);
@@ -326,30 +338,52 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
     LQM_HIP(hipGetLastError());
     const int L = P.laneB;
     const int gB = (Kc + L - 1) / L;
+    int* klist = nullptr;
+    if (logq && P.tri2) {
+        // eigenpairs by bisection and twisted factorisations; the k-points whose cluster
+        // vectors came out dependent are listed for the QL
+        int* zfmt;
+        R *Z, *tn;
+        if ((st = wsT(c, W_ZFMT, (size_t)Kc, &zfmt))) return st;
+        if ((st = wsT(c, W_KLIST, (size_t)Kc + 1, &klist))) return st;
+        if ((st = wsT(c, W_Z, (size_t)N * N * Kc, &Z))) return st;
+        if ((st = wsT(c, W_TNORM, (size_t)Kc, &tn))) return st;
+        LQM_HIP(hipMemsetAsync(klist, 0, sizeof(int), c->stream));
+        StageScope sc(c, ST_QL, Kc);
+        const dim3 g4((Kc + 3) / 4);
+        hipLaunchKernelGGL(lqm::k_tri_bisect<R>, g4, dim3(256), 0, c->stream, N, Kc, dd, ee, evals, perm, tn);
+        switch (P.nm) {
+            case 8: hipLaunchKernelGGL((lqm::k_tri_twist<R, 8>), g4, dim3(256), 0, c->stream, N, Kc, dd, ee, evals, tn, (R)kTriClusterRel, Z, zfmt, klist); break;
+            case 16: hipLaunchKernelGGL((lqm::k_tri_twist<R, 16>), g4, dim3(256), 0, c->stream, N, Kc, dd, ee, evals, tn, (R)kTriClusterRel, Z, zfmt, klist); break;
+            case 32: hipLaunchKernelGGL((lqm::k_tri_twist<R, 32>), g4, dim3(256), 0, c->stream, N, Kc, dd, ee, evals, tn, (R)kTriClusterRel, Z, zfmt, klist); break;
+            default: hipLaunchKernelGGL((lqm::k_tri_twist<R, 64>), g4, dim3(256), 0, c->stream, N, Kc, dd, ee, evals, tn, (R)kTriClusterRel, Z, zfmt, klist); break;
+        }
+    }
     {
         StageScope sc(c, ST_QL, Kc);
         if (logq) {
             auto kB = lqm::k_tridiag_ql<R, true>;
             allow_lds(kB, P.shB);
             hipLaunchKernelGGL(kB, dim3(gB), dim3(L), P.shB, c->stream, N, Kc, dd, ee, evals, perm, rot,
-                               sw, nsw, P.cap_rot, P.cap_sw, c->d_status);
+                               sw, nsw, P.cap_rot, P.cap_sw, c->d_status, klist);
         } else {
             auto kB = lqm::k_tridiag_ql<R, false>;
             allow_lds(kB, P.shB);
             hipLaunchKernelGGL(kB, dim3(gB), dim3(L), P.shB, c->stream, N, Kc, dd, ee, evals, perm, rot,
-                               sw, nsw, P.cap_rot, P.cap_sw, c->d_status);
+                               sw, nsw, P.cap_rot, P.cap_sw, c->d_status, nullptr);
         }
     }
     LQM_HIP(hipGetLastError());
     return LQM_OK;
 }
 
+// zfmt (optional): k-points with zfmt[k] = 1 already hold their eigenvectors (k_tri_twist)
 template <typename R, int NM>
 void launch_replay(lqm_context* c, int N, int Kc, const R* rot, const int* sw, const int* nsw,
-                   int cap_rot, int cap_sw, R* Z) {
+                   int cap_rot, int cap_sw, R* Z, const int* zfmt = nullptr) {
     StageScope sc(c, ST_REPLAY, Kc);
     auto kR = lqm::k_replay<R, NM>;
-    hipLaunchKernelGGL(kR, dim3(Kc), dim3(lqm::kWave), 0, c->stream, N, rot, sw, nsw, cap_rot, cap_sw, Z);
+    hipLaunchKernelGGL(kR, dim3(Kc), dim3(lqm::kWave), 0, c->stream, N, rot, sw, nsw, cap_rot, cap_sw, Z, zfmt);
 }
 
 template <typename R, int MODE, bool ZIN>
@@ -395,11 +429,12 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const R* ev
         lqm_status st = wsT(c, W_Z, (size_t)N * N * Kc, &Z);
         if (st) return st;
         if (P.wyC && P.btv == 1) {
+            const int* zfmt = P.tri2 ? static_cast<const int*>(c->buf[W_ZFMT]) : nullptr;
             switch (P.nm) {
-                case 8: launch_replay<R, 8>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
-                case 16: launch_replay<R, 16>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
-                case 32: launch_replay<R, 32>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
-                default: launch_replay<R, 64>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z); break;
+                case 8: launch_replay<R, 8>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z, zfmt); break;
+                case 16: launch_replay<R, 16>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z, zfmt); break;
+                case 32: launch_replay<R, 32>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z, zfmt); break;
+                default: launch_replay<R, 64>(c, N, Kc, rot, sw, nsw, P.cap_rot, P.cap_sw, Z, zfmt); break;
             }
             LQM_HIP(hipGetLastError());
             StageScope sc(c, ST_BACKTRANSFORM, Kc);
@@ -413,7 +448,7 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const R* ev
             const size_t sh = lqm::wy_lds_bytes();
             allow_lds(kW, sh);
             hipLaunchKernelGGL(kW, dim3(Kc), dim3(256), sh, c->stream, N, refl, tau, Z, evals, perm, kw, wuni, mu,
-                               T, c->occ, out, eb, Tg);
+                               T, c->occ, out, eb, Tg, zfmt);
             LQM_HIP(hipGetLastError());
             return LQM_OK;
         }
@@ -1771,6 +1806,7 @@ lqm_status lqm_autotune(lqm_model* m, int Nk, const double* ks_dev) {
         const int vmax = kind == VK_TRIDIAG ? (N <= 64 ? 2 : 1) : kind == VK_QL ? 3 : 1;
         if (kind == VK_BT && N > 64) continue;
         if (kind == VK_BLOCKED && (N <= 64 || N > 512)) continue;  // per-k kernels / large-N path only
+        if (kind == VK_TRIEIG && (N <= 16 || N > 64)) continue;    // feeds the compact-WY kernel only
         for (int v = 0; v <= vmax; ++v) {
             c->forced[kind] = v;
             float t = 0.f;

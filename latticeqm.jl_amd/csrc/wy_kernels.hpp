@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     int N, const cx<R>* __restrict__ refl, const cx<R>* __restrict__ tau, const R* __restrict__ Zg,
     const R* __restrict__ evals, const int* __restrict__ perm, const double* __restrict__ kw,
     double wuni, double mu, double Temp, const double* __restrict__ occ, cx<double>* __restrict__ out,
-    double* __restrict__ eb, const cx<double>* __restrict__ Tg) {
+    double* __restrict__ eb, const cx<double>* __restrict__ Tg, const int* __restrict__ zfmt) {
     extern __shared__ __align__(16) unsigned char smem[];
     WySmem& S = *reinterpret_cast<WySmem*>(smem);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -111,10 +111,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const size_t NN = (size_t)N * N;
     const cx<R>* rf = refl + (size_t)kk * NN;
 
-    // V = Z (real) in the accumulator layout, zero padded to 64 x 64
+    // V = Z (real) in the accumulator layout, zero padded to 64 x 64; Z column-major, or
+    // row-major where zfmt[kk] = 1
     lqm_d4 vr[2][2], vi[2][2];
     {
         const R* Z = Zg + (size_t)kk * NN;
+        const bool zrow = zfmt && zfmt[kk];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int r = m0 + 16 * a + g + 4 * e, c = n0 + 16 * b + li;
-                    vr[a][b][e] = (r < N && c < N) ? (double)Z[(size_t)c * N + r] : 0.0;
+                    vr[a][b][e] = (r < N && c < N) ? (double)Z[zrow ? (size_t)r * N + c : (size_t)c * N + r] : 0.0;
                     vi[a][b][e] = 0.0;
                 }
     }
