@@ -8,54 +8,121 @@
 namespace lqm {
 
 // rho(R)[e] = sum_k w_k e^{-2 pi i k.R} rho_k[e]   (Operators/densitymatrix.jl:141-143)
-// grid: x = element tiles of 256, y = k-splits, z = key groups of RG keys.
+// grid of the contraction: x = element tiles of 256, y = k-splits, z = key groups of RG keys;
 // part[(z*S + y)*RG + r][e] receives the split's partial sum.
-constexpr int kPhaseK = 32;  // k-points whose phases are staged in LDS at a time
 
+// Phase table of the k-grid sums: ph[k * ldp + r] = w_k e^{sgn 2 pi i k.R_r} for r < nr,
+// zero for nr <= r < ldp (w_k = kw[k], or wuni where kw is null).  The sums below read it
+// at wave-uniform addresses, so the phases arrive through the scalar cache into scalar
+// registers and feed the FMAs directly.
+__global__ __launch_bounds__(256) void k_phase_table(int Kc, int d, const double* __restrict__ ks,
+                                                     const double* __restrict__ kw, double wuni, int nr,
+                                                     int ldp, const int* __restrict__ Rr, double sgn,
+                                                     cx<double>* __restrict__ ph) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)Kc * ldp) return;
+    const int k = (int)(idx / ldp), r = (int)(idx % ldp);
+    cx<double> p = {0.0, 0.0};
+    if (r < nr) {
+        double x = 0.0;
+        for (int a = 0; a < d; ++a) x += ks[(size_t)k * d + a] * (double)Rr[r * d + a];
+        const double w = kw ? kw[k] : wuni;
+        p = w * cis2pi(sgn * x);
+    }
+    ph[idx] = p;
+}
+
+// The phase contraction with the phases from the table ph (row stride ldp; key group z
+// starts at column z * RG), read at wave-uniform addresses (scalar loads, the phases feed
+// the FMAs from scalar registers): thread = element e, RG accumulators, the k-points of
+// split y in order, four rho_k elements in flight.
 template <int RG>
-__global__ __launch_bounds__(256) void k_phase_contract(
-    int NN, int Nkc, int d, const double* __restrict__ ks, const double* __restrict__ kw,
-    double wuni, int nRr, const int* __restrict__ Rr, const cx<double>* __restrict__ rhok,
-    int ksplit, cx<double>* __restrict__ part) {
-    __shared__ cx<double> sph[kPhaseK][RG];
-    const int tid = threadIdx.x;
-    const int e = blockIdx.x * 256 + tid;
+__global__ __launch_bounds__(256) void k_phase_contract_s(int NN, int Nkc, const cx<double>* __restrict__ ph,
+                                                          int ldp, const cx<double>* __restrict__ rhok,
+                                                          int ksplit, int nRr, cx<double>* __restrict__ part) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
     const int s = blockIdx.y;
     const int S = gridDim.y;
     const int rg0 = blockIdx.z * RG;
     const int nr = min(RG, nRr - rg0);
     const int k0 = s * ksplit;
     const int k1 = min(k0 + ksplit, Nkc);
+    const cx<double>* phg = ph + rg0;
     cx<double> acc[RG];
 #pragma unroll
     for (int r = 0; r < RG; ++r) acc[r] = {0.0, 0.0};
-    for (int kb = k0; kb < k1; kb += kPhaseK) {
-        __syncthreads();
-        for (int t = tid; t < kPhaseK * RG; t += 256) {
-            const int kl = t / RG, r = t % RG, k = kb + kl;
-            cx<double> ph = {0.0, 0.0};
-            if (k < k1 && r < nr) {
-                double x = 0.0;
-                for (int a = 0; a < d; ++a) x += ks[(size_t)k * d + a] * (double)Rr[(rg0 + r) * d + a];
-                const double w = kw ? kw[k] : wuni;
-                ph = w * cis2pi(-x);
-            }
-            sph[kl][r] = ph;
-        }
-        __syncthreads();
-        if (e < NN) {
-            const int kn = min(kPhaseK, k1 - kb);
-            for (int kl = 0; kl < kn; ++kl) {
-                const cx<double> v = rhok[(size_t)(kb + kl) * NN + e];
-#pragma unroll
-                for (int r = 0; r < RG; ++r) acc[r] = cmad(acc[r], sph[kl][r], v);
-            }
-        }
-    }
     if (e < NN) {
+        // slot j holds k-point min(k + j, k1 - 1); the loads are unconditional (clamped
+        // index) so the waits can count them
+        constexpr int D = 4;
+        if (k0 < k1) {
+            cx<double> buf[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) buf[j] = rhok[(size_t)min(k0 + j, k1 - 1) * NN + e];
+            int k = k0;
+            for (; k + D <= k1; k += D) {
+#pragma unroll
+                for (int j = 0; j < D; ++j) {
+                    const cx<double> v = buf[j];
+                    buf[j] = rhok[(size_t)min(k + j + D, k1 - 1) * NN + e];
+                    const cx<double>* p = phg + (size_t)(k + j) * ldp;
+#pragma unroll
+                    for (int r = 0; r < RG; ++r) acc[r] = cmad(acc[r], p[r], v);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+                if (k + j < k1) {
+                    const cx<double>* p = phg + (size_t)(k + j) * ldp;
+#pragma unroll
+                    for (int r = 0; r < RG; ++r) acc[r] = cmad(acc[r], p[r], buf[j]);
+                }
+        }
 #pragma unroll
         for (int r = 0; r < RG; ++r)
             if (r < nr) part[((size_t)(blockIdx.z * S + s) * RG + r) * NN + e] = acc[r];
+    }
+}
+
+// A = Hermitian(H(k)), H(k) = sum_R T_R e^{2 pi i k.R} (full storage, as k_hk_full): thread
+// = element e = (r, c) of the N x N matrix, which keeps the nR <= NRM hopping entries of
+// its upper-triangle source (r, c) or (c, r) in registers across the block's kb k-points.
+// The phases come from the table ph (row stride ldp = NRM, zero beyond nR), staged through
+// LDS 32 k-points at a time and read as broadcasts.  Every term up to NRM is summed: the
+// padded entries are zero and fma(0, t, h) = h, so the sums are those over the nR keys.  A
+// lower element is the conjugate of the same sum its upper mirror forms, so A is
+// Hermitian bitwise.
+constexpr int kHkChunk = 32;
+template <int NRM>
+__global__ __launch_bounds__(256) void k_hk_stream(int N, int nR, const cx<double>* __restrict__ T, int Kc,
+                                                   int kb, const cx<double>* __restrict__ ph,
+                                                   cx<double>* __restrict__ A) {
+    __shared__ cx<double> sph[kHkChunk * NRM];
+    const size_t NN = (size_t)N * N;
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int k0 = blockIdx.y * kb;
+    const int k1 = min(k0 + kb, Kc);
+    const bool live = e < NN;
+    const int c = live ? (int)(e / N) : 0;
+    const int r = live ? (int)(e - (size_t)c * N) : 0;
+    const bool lower = r > c;
+    const size_t src = lower ? (size_t)r * N + c : e;
+    cx<double> t[NRM];
+#pragma unroll
+    for (int q = 0; q < NRM; ++q) t[q] = (live && q < nR) ? T[(size_t)q * NN + src] : cx<double>{0.0, 0.0};
+    for (int kc = k0; kc < k1; kc += kHkChunk) {
+        const int nk = min(kHkChunk, k1 - kc);
+        __syncthreads();
+        for (int i = threadIdx.x; i < nk * NRM; i += 256) sph[i] = ph[(size_t)kc * NRM + i];
+        __syncthreads();
+        if (!live) continue;
+        for (int kq = 0; kq < nk; ++kq) {
+            const cx<double>* p = sph + kq * NRM;
+            cx<double> h = {0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < NRM; ++q) h = cmad(h, t[q], p[q]);
+            A[(size_t)(kc + kq) * NN + e] = r == c ? cx<double>{h.re, 0.0} : (lower ? cx<double>{h.re, -h.im} : h);
+        }
     }
 }
 

@@ -48,13 +48,16 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
 // per-k tridiagonal eigenpairs: a k-point whose eigenvalues come closer than this x ||T||
 // (Gershgorin bound) takes the QL path, whose vectors stay orthogonal in clusters
 constexpr double kTriClusterRel = 1e-3;
+// blocks of one phase contraction launch (element tiles x k-splits x key groups): one full
+// wave of three 256-thread blocks per CU on 256 CUs
+constexpr int kContractBlocks = 768;
 
 }  // namespace
 
@@ -270,12 +273,38 @@ void allow_lds(K kernel, size_t bytes) {
 // A = Hermitian(H(k)) in FP64, full storage, for Kc k-points: 32 x 32 tile pairs held
 // across 8 k-points per block, so the hopping stack is read from L2 once per tile and
 // k-group rather than once per k-point.
-void launch_hk_full(lqm_context* c, lqm_model* m, int Kc, const double* ks, cx<double>* A) {
-    const int nt = (m->N + 31) / 32;
-    const size_t sh = sizeof(cx<double>) * lqm::kHkK * std::max(1, m->nR);
+// For nR <= 32 a phase table and one thread per matrix element that keeps its hopping
+// entries in registers across up to 128 k-points (k_hk_stream); else the tiled kernel.
+lqm_status launch_hk_full(lqm_context* c, lqm_model* m, int Kc, const double* ks, cx<double>* A) {
+    const int N = m->N, nR = m->nR;
+    if (nR > 0 && nR <= 32) {
+        const int nrm = nR <= 8 ? 8 : nR <= 16 ? 16 : nR <= 24 ? 24 : nR <= 28 ? 28 : 32;
+        cx<double>* ph;
+        lqm_status st = wsT(c, W_PHT, (size_t)Kc * nrm, &ph);
+        if (st) return st;
+        const size_t nph = (size_t)Kc * nrm;
+        hipLaunchKernelGGL(lqm::k_phase_table, dim3((unsigned)((nph + 255) / 256)), dim3(256), 0, c->stream, Kc,
+                           m->d, ks, nullptr, 1.0, nR, nrm, m->dR, 1.0, ph);
+        const size_t NN = (size_t)N * N;
+        const int be = (int)((NN + 255) / 256);
+        // k-points per block: up to 128 while the grid keeps about 2048 blocks
+        const int kb = (int)std::max<long long>(8, std::min<long long>(128, (long long)Kc * be / 2048));
+        const dim3 g(be, (Kc + kb - 1) / kb);
+        switch (nrm) {
+            case 8: hipLaunchKernelGGL(lqm::k_hk_stream<8>, g, dim3(256), 0, c->stream, N, nR, m->dT, Kc, kb, ph, A); break;
+            case 16: hipLaunchKernelGGL(lqm::k_hk_stream<16>, g, dim3(256), 0, c->stream, N, nR, m->dT, Kc, kb, ph, A); break;
+            case 24: hipLaunchKernelGGL(lqm::k_hk_stream<24>, g, dim3(256), 0, c->stream, N, nR, m->dT, Kc, kb, ph, A); break;
+            case 28: hipLaunchKernelGGL(lqm::k_hk_stream<28>, g, dim3(256), 0, c->stream, N, nR, m->dT, Kc, kb, ph, A); break;
+            default: hipLaunchKernelGGL(lqm::k_hk_stream<32>, g, dim3(256), 0, c->stream, N, nR, m->dT, Kc, kb, ph, A); break;
+        }
+        return LQM_OK;
+    }
+    const int nt = (N + 31) / 32;
+    const size_t sh = sizeof(cx<double>) * lqm::kHkK * std::max(1, nR);
     allow_lds(lqm::k_hk_full, sh);
     hipLaunchKernelGGL(lqm::k_hk_full, dim3(nt * (nt + 1) / 2, (Kc + lqm::kHkK - 1) / lqm::kHkK), dim3(256), sh,
-                       c->stream, m->N, m->d, m->nR, m->dR, m->dT, Kc, ks, A);
+                       c->stream, N, m->d, nR, m->dR, m->dT, Kc, ks, A);
+    return LQM_OK;
 }
 
 // Hpre: H(k) of the chunk already materialised in FP64 (full Hermitian), or null.
@@ -305,7 +334,7 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
         cx<double>* Hk;
         if ((st = wsT(c, W_HK, (size_t)N * N * Kc, &Hk))) return st;
         StageScope sc(c, ST_HK64, Kc);
-        launch_hk_full(c, m, Kc, ks, Hk);
+        if ((st = launch_hk_full(c, m, Kc, ks, Hk))) return st;
         Hpre = Hk;
     }
     {
@@ -509,21 +538,55 @@ lqm_status launch_rho2(lqm_model* m, const Plan& P, int Kc, const double* evals,
     return LQM_OK;
 }
 
-template <int RG>
-void launch_contract(lqm_context* c, int NN, int Kc, int d, const double* ks, const double* kw,
-                     double wuni, int nRr, const int* Rr, const cx<double>* rhok, int ksplit,
-                     int S, int groups, cx<double>* part, cx<double>* rho) {
+// Key groups of the phase contraction: RG keys per group, the smallest of these holding nRr.
+int contract_group(int nRr) {
+    return nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : nRr <= 24 ? 24 : nRr <= 28 ? 28 : 32;
+}
+
+// rho(R) += sum_k w_k e^{-2 pi i k.R} rho_k over a chunk: the phase table of the chunk's
+// k-points and keys (groups x RG columns), the split-K contraction, the fixed-order fold.
+lqm_status launch_contract(int RG, lqm_context* c, int NN, int Kc, int d, const double* ks, const double* kw,
+                           double wuni, int nRr, const int* Rr, const cx<double>* rhok, int ksplit, int S,
+                           int groups, cx<double>* part, cx<double>* rho) {
     hipStream_t s = c->stream;
+    const int ldp = groups * RG;
+    cx<double>* ph;
+    lqm_status st = wsT(c, W_PHC, (size_t)Kc * ldp, &ph);
+    if (st) return st;
     dim3 grid((NN + 255) / 256, S, groups);
     {
         StageScope sc(c, ST_CONTRACT, Kc);
-        hipLaunchKernelGGL(lqm::k_phase_contract<RG>, grid, dim3(256), 0, s, NN, Kc, d, ks, kw, wuni,
-                           nRr, Rr, rhok, ksplit, part);
+        const size_t nph = (size_t)Kc * ldp;
+        hipLaunchKernelGGL(lqm::k_phase_table, dim3((unsigned)((nph + 255) / 256)), dim3(256), 0, s, Kc, d, ks, kw,
+                           wuni, nRr, ldp, Rr, -1.0, ph);
+#define LQM_CONTRACT(G)                                                                                   \
+    hipLaunchKernelGGL(lqm::k_phase_contract_s<G>, grid, dim3(256), 0, s, NN, Kc, ph, ldp, rhok, ksplit, nRr, \
+                       part)
+        switch (RG) {
+            case 1: LQM_CONTRACT(1); break;
+            case 4: LQM_CONTRACT(4); break;
+            case 8: LQM_CONTRACT(8); break;
+            case 16: LQM_CONTRACT(16); break;
+            case 24: LQM_CONTRACT(24); break;
+            case 28: LQM_CONTRACT(28); break;
+            default: LQM_CONTRACT(32); break;
+        }
+#undef LQM_CONTRACT
     }
     const size_t total = (size_t)nRr * NN;
     StageScope sc(c, ST_FOLD);
-    hipLaunchKernelGGL(lqm::k_phase_fold<RG>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                       NN, nRr, S, part, rho);
+    const dim3 gf((unsigned)((total + 255) / 256));
+    switch (RG) {
+        case 1: hipLaunchKernelGGL(lqm::k_phase_fold<1>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
+        case 4: hipLaunchKernelGGL(lqm::k_phase_fold<4>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
+        case 8: hipLaunchKernelGGL(lqm::k_phase_fold<8>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
+        case 16: hipLaunchKernelGGL(lqm::k_phase_fold<16>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
+        case 24: hipLaunchKernelGGL(lqm::k_phase_fold<24>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
+        case 28: hipLaunchKernelGGL(lqm::k_phase_fold<28>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
+        default: hipLaunchKernelGGL(lqm::k_phase_fold<32>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
+    }
+    LQM_HIP(hipGetLastError());
+    return LQM_OK;
 }
 
 lqm_status check_status(lqm_context* c, const char* what) {
@@ -630,10 +693,10 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
                          bands_out, occ);
     Plan P = make_plan(c, N, m->nR, Nk, true, true);
     if (mq && P.Kc < Nk) return kFallback;  // mu needs every eigenvalue before the first occupation
-    const int RG = nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : 32;
+    const int RG = contract_group(nRr);
     const int groups = std::max(1, (nRr + RG - 1) / RG);
     const int tilesE = (int)((NN + 255) / 256);
-    int S = std::max(1, std::min(P.Kc, 1024 / std::max(1, tilesE * groups)));
+    int S = std::max(1, std::min(P.Kc, kContractBlocks / std::max(1, tilesE * groups)));
     cx<double>* part = nullptr;
     if ((st = wsT(c, W_PART, (size_t)groups * S * RG * NN, &part))) return st;
     const double wuni = 1.0 / (double)Nk;
@@ -672,13 +735,7 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
             const int ksplit = (Kc + Sc - 1) / Sc;
             const int Sused = (Kc + ksplit - 1) / ksplit;
             auto* rh = reinterpret_cast<cx<double>*>(rho);
-            switch (RG) {
-                case 1: launch_contract<1>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
-                case 4: launch_contract<4>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
-                case 8: launch_contract<8>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
-                case 16: launch_contract<16>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
-                default: launch_contract<32>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh); break;
-            }
+            if ((st = launch_contract(RG, c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh))) return st;
             LQM_HIP(hipGetLastError());
         }
     }
@@ -724,10 +781,10 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
     lqm_status st;
     int sweeps_max = 0;
     double rmax = 0.0, dmax = 0.0, nnear = 0.0;
-    const int RG = nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : 32;
+    const int RG = contract_group(nRr);
     const int groups = std::max(1, (nRr + RG - 1) / RG);
     const int tilesE = (int)((NN + 255) / 256);
-    const int S = std::max(1, std::min(P.Kc, 1024 / std::max(1, tilesE * groups)));
+    const int S = std::max(1, std::min(P.Kc, kContractBlocks / std::max(1, tilesE * groups)));
     cx<double>* part = nullptr;
     if (rho && nRr > 0 && (st = wsT(c, W_PART, (size_t)groups * S * RG * NN, &part))) return st;
     std::vector<double> hstat;
@@ -749,7 +806,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         // H(k) in FP64 (the refinement's A), then FP32 eigenvectors, ascending, as FP64 complex
         {
             StageScope sc(c, ST_HK64, Kc);
-            launch_hk_full(c, m, Kc, ksc, A);
+            if ((st = launch_hk_full(c, m, Kc, ksc, A))) return st;
         }
         LQM_HIP(hipGetLastError());
         if ((st = launch_eigvals<float>(m, P, Kc, ksc, ev32, true, A))) return st;
@@ -922,13 +979,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
                 const int Sc = std::min(S, Kc);
                 const int ksplit = (Kc + Sc - 1) / Sc;
                 const int Sused = (Kc + ksplit - 1) / ksplit;
-                switch (RG) {
-                    case 1: launch_contract<1>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho); break;
-                    case 4: launch_contract<4>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho); break;
-                    case 8: launch_contract<8>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho); break;
-                    case 16: launch_contract<16>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho); break;
-                    default: launch_contract<32>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho); break;
-                }
+                if ((st = launch_contract(RG, c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho))) return st;
             }
             LQM_HIP(hipGetLastError());
         }
@@ -1057,10 +1108,10 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
     if (mq && Kmax < Nk) return kFallback;  // mu needs every eigenvalue before the first occupation
     const double wuni = 1.0 / (double)Nk;
     lqm_status st;
-    const int RG = nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : 32;
+    const int RG = contract_group(nRr);
     const int groups = std::max(1, (nRr + RG - 1) / RG);
     const int tilesE = (int)((NN + 255) / 256);
-    const int S = std::max(1, std::min(Kmax, 1024 / std::max(1, tilesE * groups)));
+    const int S = std::max(1, std::min(Kmax, kContractBlocks / std::max(1, tilesE * groups)));
     cx<double>* part = nullptr;
     if (rho && nRr > 0 && (st = wsT(c, W_PART, (size_t)groups * S * RG * NN, &part))) return st;
     std::vector<double> hlam, hnorm;
@@ -1090,11 +1141,7 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
         // H(k), full Hermitian storage
         {
             StageScope sc(c, ST_BHK, Kc);
-            const int nt = (N + 31) / 32;
-            const size_t sh = sizeof(cx<double>) * lqm::kHkK * std::max(1, m->nR);
-            allow_lds(lqm::k_hk_full, sh);
-            hipLaunchKernelGGL(lqm::k_hk_full, dim3(nt * (nt + 1) / 2, (Kc + lqm::kHkK - 1) / lqm::kHkK), dim3(256),
-                               sh, s, N, d, m->nR, m->dR, m->dT, Kc, ksc, A);
+            if ((st = launch_hk_full(c, m, Kc, ksc, A))) return st;
         }
         LQM_HIP(hipGetLastError());
         if ((st = blocked_tridiag(c, N, Kc, A, Vp, Wp, P, P2, Q, SS, nchmax, nbmax, dd, ee, tau))) return st;
@@ -1209,13 +1256,7 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
         const int Sc = std::min(S, Kc);
         const int ksplit = (Kc + Sc - 1) / Sc;
         const int Sused = (Kc + ksplit - 1) / ksplit;
-        switch (RG) {
-            case 1: launch_contract<1>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho); break;
-            case 4: launch_contract<4>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho); break;
-            case 8: launch_contract<8>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho); break;
-            case 16: launch_contract<16>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho); break;
-            default: launch_contract<32>(c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho); break;
-        }
+        if ((st = launch_contract(RG, c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho))) return st;
         LQM_HIP(hipGetLastError());
     }
     return check_status(c, "blocked eigensolver");
