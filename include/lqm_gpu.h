@@ -97,8 +97,9 @@ const char *lqm_stage_name(int stage);
  * modes, N <= 4096; occupations below 1e-18 leave rho_k out); kind 6: eigenpairs of the
  * tridiagonal ahead of the compact-WY back-transformation (0 = QL with a rotation log, then
  * its replay; 1 = the default: bisection with Newton steps and twisted-factorisation
- * eigenvectors, O(N^2) per k-point, with the QL path for the k-points whose eigenvalues
- * come closer than 1e-3 ||T||).
+ * eigenvectors, O(N^2) per k-point, Gram-Schmidt inside clusters of eigenvalues closer
+ * than 1e-3 ||T||, and the QL path for the k-points whose cluster vectors come out
+ * dependent; eigenvalue-only passes for N <= 64 use the bisection alone).
  * variant -1 returns to the tuned/default
  * choice.  lqm_autotune times every variant of every kind on Nk device k-points of
  * the model and records the fastest per orbital count N in the context;

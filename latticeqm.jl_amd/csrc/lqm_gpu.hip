@@ -164,6 +164,7 @@ struct Plan {
     size_t shAw = 0;
     bool rho2 = false;  // rho_k by two-sided reflectors on the packed Z f Z^T
     bool tri2 = false;  // tridiagonal eigenpairs by bisection + twisted factorisation (QL for clusters)
+    bool trib = false;  // eigenvalues only, by bisection (N <= 64)
     size_t shM = 0, shR2 = 0;
     int Kc = 0;   // k-points per chunk
 };
@@ -232,6 +233,7 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     // factorisation with the QL path for k-points holding an eigenvalue cluster (feeds the
     // compact-WY back-transformation only)
     p.tri2 = (vectors || rho) && variant(VK_TRIEIG) == 1 && p.wyC && p.btv == 1 && !p.rho2;
+    p.trib = !vectors && !rho && variant(VK_TRIEIG) == 1 && N <= 64;
     perk += 8;
     size_t kc = kWorkspaceBudget / perk;
     if (kc < 1) kc = 1;
@@ -380,13 +382,23 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
         LQM_HIP(hipMemsetAsync(klist, 0, sizeof(int), c->stream));
         StageScope sc(c, ST_QL, Kc);
         const dim3 g4((Kc + 3) / 4);
-        hipLaunchKernelGGL(lqm::k_tri_bisect<R>, g4, dim3(256), 0, c->stream, N, Kc, dd, ee, evals, perm, tn);
+        hipLaunchKernelGGL(lqm::k_tri_bisect<R>, g4, dim3(256), 0, c->stream, N, Kc, dd, ee, evals, perm, tn, c->d_status);
         switch (P.nm) {
             case 8: hipLaunchKernelGGL((lqm::k_tri_twist<R, 8>), g4, dim3(256), 0, c->stream, N, Kc, dd, ee, evals, tn, (R)kTriClusterRel, Z, zfmt, klist); break;
             case 16: hipLaunchKernelGGL((lqm::k_tri_twist<R, 16>), g4, dim3(256), 0, c->stream, N, Kc, dd, ee, evals, tn, (R)kTriClusterRel, Z, zfmt, klist); break;
             case 32: hipLaunchKernelGGL((lqm::k_tri_twist<R, 32>), g4, dim3(256), 0, c->stream, N, Kc, dd, ee, evals, tn, (R)kTriClusterRel, Z, zfmt, klist); break;
             default: hipLaunchKernelGGL((lqm::k_tri_twist<R, 64>), g4, dim3(256), 0, c->stream, N, Kc, dd, ee, evals, tn, (R)kTriClusterRel, Z, zfmt, klist); break;
         }
+    }
+    if (!logq && P.trib) {
+        // eigenvalues only: bisection, no QL
+        R* tn;
+        if ((st = wsT(c, W_TNORM, (size_t)Kc, &tn))) return st;
+        StageScope sc(c, ST_QL, Kc);
+        hipLaunchKernelGGL(lqm::k_tri_bisect<R>, dim3((Kc + 3) / 4), dim3(256), 0, c->stream, N, Kc, dd, ee, evals,
+                           perm, tn, c->d_status);
+        LQM_HIP(hipGetLastError());
+        return LQM_OK;
     }
     {
         StageScope sc(c, ST_QL, Kc);

@@ -44,11 +44,13 @@ __device__ __forceinline__ bool tri_finite(R x) {
 // inside the bracket.  Converged when the Newton step is at most 4 eps ||T|| and points
 // into the bracket, or the bracket is at most max(2 eps max(|a|, |b|), eps ||T||) (||T|| =
 // the Gershgorin bound).
-// Out: evals, perm[k * N + n] = n, tnorm[k] = ||T||.
+// Out: evals, perm[k * N + n] = n, tnorm[k] = ||T||; a non-finite eigenvalue sets
+// kStatNonFinite in *status.
 template <typename R>
 __global__ __launch_bounds__(256) void k_tri_bisect(int N, int Nkc, const R* __restrict__ dd,
                                                     const R* __restrict__ ee, R* __restrict__ evals,
-                                                    int* __restrict__ perm, R* __restrict__ tnorm) {
+                                                    int* __restrict__ perm, R* __restrict__ tnorm,
+                                                    int* __restrict__ status) {
     __shared__ R sd[4][kWave], se2[4][kWave];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
     const int kk = blockIdx.x * 4 + w;
@@ -61,6 +63,18 @@ __global__ __launch_bounds__(256) void k_tri_bisect(int N, int Nkc, const R* __r
     }
     __syncthreads();
     if (!act) return;  // no barrier below
+    // a non-finite entry: fmin/fmax below would pass over it, so it is caught here
+    if (__ballot(lane < N && !(tri_finite(sd[w][lane]) && tri_finite(ev))) != 0) {
+        if (lane < N) {
+            evals[(size_t)kk * N + lane] = sd[w][lane] - sd[w][lane] + ev - ev;
+            perm[(size_t)kk * N + lane] = lane;
+        }
+        if (lane == 0) {
+            tnorm[kk] = R(0);
+            atomicOr(status, kStatNonFinite);
+        }
+        return;
+    }
     // Gershgorin interval and the largest e^2
     const R eabs = fabs(ev);
     const R eleft = __shfl(eabs, lane > 0 ? lane - 1 : 0);
@@ -136,6 +150,7 @@ __global__ __launch_bounds__(256) void k_tri_bisect(int N, int Nkc, const R* __r
     }
     evals[(size_t)kk * N + n] = x;
     perm[(size_t)kk * N + n] = n;
+    if (!tri_finite(x)) atomicOr(status, kStatNonFinite);
 }
 
 // Eigenvector of lambda_n (lane n): twisted factorisation
