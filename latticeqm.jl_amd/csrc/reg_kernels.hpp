@@ -55,11 +55,14 @@ __device__ __forceinline__ R row16_sum(R v) {
     return v;
 }
 
-// Shared memory of k_tridiag_r (double-buffered per step parity).
+// Shared memory of k_tridiag_r (double-buffered per step parity).  The vectors are read
+// four entries per lane (rows 4 ti .. 4 ti + 3), so entry r sits at rpad(r) = r + r/4: one
+// spare slot per four entries spreads the 16 lanes of a row over distinct banks.
+__device__ __forceinline__ int rpad(int r) { return r + (r >> 2); }
 template <typename R>
 struct RegTriSmem {
-    cx<R> sv[2][64];  // Householder vector v (zero above row j+1)
-    cx<R> sp[2][64];  // p = tau A22 v
+    cx<R> sv[2][80];  // Householder vector v (zero above row j+1), entry r at rpad(r)
+    cx<R> sp[2][80];  // p = tau A22 v, entry r at rpad(r)
     R ss[2][16];      // per tile column partial of v^H A22 v
     cx<R> stau[2];
 };
@@ -104,7 +107,7 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
             cx<R> vr = mkc<R>(0, 0);
             if (r == j + 1) vr = mkc<R>(1, 0);
             else if (r > j + 1 && r < N && !triv) vr = a[u][JV] * scale;
-            S.sv[buf][r] = vr;
+            S.sv[buf][rpad(r)] = vr;
             if (r > j && r < N) rf[(size_t)j * N + r] = vr;
         }
         if (ti == jt) dd[(size_t)j * Nkc + kk] = a[JV][JV].re;  // A[j][j]
@@ -121,9 +124,9 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
     // (b) y_c = sum_r conj(A_rc) v_r for this thread's columns, summed over the lane row
     cx<R> vr[4], vc[4], p[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) vr[u] = S.sv[buf][4 * ti + u];
+    for (int u = 0; u < 4; ++u) vr[u] = S.sv[buf][5 * ti + u];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) vc[v] = S.sv[buf][4 * tj + v];
+    for (int v = 0; v < 4; ++v) vc[v] = S.sv[buf][5 * tj + v];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         cx<R> y = mkc<R>(0, 0);
@@ -142,7 +145,7 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
     }
     if (ti == 0) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) S.sp[buf][4 * tj + v] = p[v];
+        for (int v = 0; v < 4; ++v) S.sp[buf][5 * tj + v] = p[v];
         S.ss[buf][tj] = spart;
     }
     __syncthreads();
@@ -157,7 +160,7 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int r = 4 * ti + u;
-            const cx<R> pr = S.sp[buf][r];
+            const cx<R> pr = S.sp[buf][5 * ti + u];
             wr[u] = (r > j && r < N) ? cmad(pr, mkc<R>(alpha2, 0), vr[u]) : mkc<R>(0, 0);
         }
 #pragma unroll
