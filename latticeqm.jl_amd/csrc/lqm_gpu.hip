@@ -13,6 +13,7 @@
 #include <map>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "lqm_gpu.h"
@@ -485,11 +486,19 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const R* ev
             if ((st = wsT(c, W_WYT, (size_t)std::max(1, nbw) * lqm::kWyB * lqm::kWyB * Kc, &Tg))) return st;
             if (nbw > 0)
                 hipLaunchKernelGGL(lqm::k_wy_tfac<R>, dim3(nbw, Kc), dim3(64), 0, c->stream, N, refl, tau, Tg);
-            auto kW = lqm::k_backtransform_wy<R, MODE>;
-            const size_t sh = lqm::wy_lds_bytes();
-            allow_lds(kW, sh);
-            hipLaunchKernelGGL(kW, dim3(Kc), dim3(256), sh, c->stream, N, refl, tau, Z, evals, perm, kw, wuni, mu,
-                               T, c->occ, out, eb, Tg, zfmt);
+            if constexpr (std::is_same<R, float>::value && MODE == 0) {
+                // mixed path: the FP32 vectors on the FP32 matrix cores
+                const size_t sh = lqm::wy32_lds_bytes();
+                allow_lds(lqm::k_backtransform_wy32, sh);
+                hipLaunchKernelGGL(lqm::k_backtransform_wy32, dim3(Kc), dim3(256), sh, c->stream, N, refl, Z, perm,
+                                   out, Tg, zfmt);
+            } else {
+                auto kW = lqm::k_backtransform_wy<R, MODE>;
+                const size_t sh = lqm::wy_lds_bytes();
+                allow_lds(kW, sh);
+                hipLaunchKernelGGL(kW, dim3(Kc), dim3(256), sh, c->stream, N, refl, tau, Z, evals, perm, kw, wuni,
+                                   mu, T, c->occ, out, eb, Tg, zfmt);
+            }
             LQM_HIP(hipGetLastError());
             return LQM_OK;
         }

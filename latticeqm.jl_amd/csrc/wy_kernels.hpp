@@ -320,4 +320,188 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 }
 
+
+// ---------------------------------------------------------------------------------
+// FP32 form of the back-transformation for the mixed path (MODE 0: eigenvectors only):
+// the same blocks of 16 reflectors with V, Y, U, W and T in FP32 on v_mfma_f32_16x16x4f32
+// (twice the FP64 matrix-core rate; the Ogita-Aishima refinement that follows makes the
+// vectors FP64-accurate).  Lane layout of that instruction: A(m, k) from lane
+// (m, k-slot) = (l % 16, l / 16), B(k, n) from lane (n, k-slot), and element e of the
+// accumulator holds row 4 (l / 16) + e, column l % 16 -- so in K-step s lane (li, g)
+// supplies contraction index 4 g + s, the row its accumulator element s already holds,
+// and V feeds Y^H V straight from the registers.  T (computed in FP64 by k_wy_tfac) is
+// rounded to FP32.  The result is staged through LDS 32 columns at a time and written as
+// FP64 with whole columns contiguous.
+// ---------------------------------------------------------------------------------
+typedef float lqm_f4 __attribute__((ext_vector_type(4)));
+constexpr int kWy32Ld = 65;  // leading dimension of a staged output half (FP64 complex)
+__host__ __device__ constexpr size_t wy32_region() {
+    // Y, U (64 x 16 each), W[2] (64 x 17), T (16 x 16) in FP32 complex; the staged half
+    // of X (32 x 65 FP64 complex) overlays them
+    return (sizeof(cx<float>) * (2 * 64 * kWyB + 2 * 64 * (kWyB + 1) + kWyB * kWyB)) >
+                   sizeof(cx<double>) * 32 * kWy32Ld
+               ? sizeof(cx<float>) * (2 * 64 * kWyB + 2 * 64 * (kWyB + 1) + kWyB * kWyB)
+               : sizeof(cx<double>) * 32 * kWy32Ld;
+}
+inline size_t wy32_lds_bytes() { return wy32_region() + 64 * sizeof(int); }
+
+__global__ __launch_bounds__(256) void k_backtransform_wy32(int N, const cx<float>* __restrict__ refl,
+                                                            const float* __restrict__ Zg,
+                                                            const int* __restrict__ perm,
+                                                            cx<double>* __restrict__ out,
+                                                            const cx<double>* __restrict__ Tg,
+                                                            const int* __restrict__ zfmt) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<float>* Ys = reinterpret_cast<cx<float>*>(smem);  // Y(r, i) at r * 16 + i
+    cx<float>* Us = Ys + 64 * kWyB;                       // U = Y T, same layout
+    cx<float>* Ws = Us + 64 * kWyB;                       // [2][c * 17 + i]: row-half partials of W
+    cx<float>* Ts = Ws + 2 * 64 * (kWyB + 1);             // T(row, col) at row * 16 + col
+    int* invp = reinterpret_cast<int*>(smem + wy32_region());
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    const int wr = wave & 1, wc = wave >> 1;
+    const int m0 = 32 * wr, n0 = 32 * wc;
+    const int kk = blockIdx.x;
+    const size_t NN = (size_t)N * N;
+    const cx<float>* rf = refl + (size_t)kk * NN;
+
+    // V = Z in the accumulator layout (row m0 + 16 a + 4 g + e, column n0 + 16 b + li)
+    lqm_f4 vr[2][2], vi[2][2];
+    {
+        const float* Z = Zg + (size_t)kk * NN;
+        const bool zrow = zfmt && zfmt[kk];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int r = m0 + 16 * a + 4 * g + e, c = n0 + 16 * b + li;
+                    vr[a][b][e] = (r < N && c < N) ? Z[zrow ? (size_t)r * N + c : (size_t)c * N + r] : 0.f;
+                    vi[a][b][e] = 0.f;
+                }
+    }
+    const int nrefl = N - 1;
+    const int nb = (nrefl + kWyB - 1) / kWyB;
+    for (int b0 = nb - 1; b0 >= 0; --b0) {
+        const int j0 = b0 * kWyB;
+        // (1) the block's reflectors (zero above row j + 1) and its T factor
+        for (int e = tid; e < 64 * kWyB; e += 256) {
+            const int r = e >> 4, i = e & 15;
+            const int j = j0 + i;
+            cx<float> y = {0.f, 0.f};
+            if (j < nrefl && r > j && r < N) y = rf[(size_t)j * N + r];
+            Ys[e] = y;
+        }
+        {
+            const cx<double> t = Tg[((size_t)kk * nb + b0) * (kWyB * kWyB) + tid];
+            Ts[tid] = cx<float>{(float)t.re, (float)t.im};
+        }
+        __syncthreads();
+        // (2) W_wr = Y(R_wr, :)^H V(R_wr, C_wc) and U = Y T (wave w: rows 16 w .. 16 w + 15)
+        {
+            lqm_f4 wr_[2], wi_[2];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                wr_[b] = lqm_f4{0.f, 0.f, 0.f, 0.f};
+                wi_[b] = lqm_f4{0.f, 0.f, 0.f, 0.f};
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int r = m0 + 16 * a + 4 * g + s;
+                    const cx<float> y = Ys[r * kWyB + li];  // A(m = li, k) = conj(Y(r, li))
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        const float br = vr[a][b][s], bi = vi[a][b][s];
+                        wr_[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(y.re, br, wr_[b], 0, 0, 0);
+                        wi_[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(y.re, bi, wi_[b], 0, 0, 0);
+                        wr_[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(y.im, bi, wr_[b], 0, 0, 0);
+                        wi_[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-y.im, br, wi_[b], 0, 0, 0);
+                    }
+                }
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    Ws[wr * 64 * (kWyB + 1) + (n0 + 16 * b + li) * (kWyB + 1) + 4 * g + e] =
+                        cx<float>{wr_[b][e], wi_[b][e]};
+            lqm_f4 ur = {0.f, 0.f, 0.f, 0.f}, ui = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int l = 4 * g + s;
+                const cx<float> y = Ys[(16 * wave + li) * kWyB + l];  // A(m = li, k = l)
+                const cx<float> t = Ts[l * kWyB + li];                // B(k = l, n = li)
+                ur = __builtin_amdgcn_mfma_f32_16x16x4f32(y.re, t.re, ur, 0, 0, 0);
+                ui = __builtin_amdgcn_mfma_f32_16x16x4f32(y.re, t.im, ui, 0, 0, 0);
+                ur = __builtin_amdgcn_mfma_f32_16x16x4f32(-y.im, t.im, ur, 0, 0, 0);
+                ui = __builtin_amdgcn_mfma_f32_16x16x4f32(y.im, t.re, ui, 0, 0, 0);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Us[(16 * wave + 4 * g + e) * kWyB + li] = cx<float>{ur[e], ui[e]};
+        }
+        __syncthreads();
+        // (3) V <- V - U W, W = W_0 + W_1
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int i = 4 * g + s;
+            float ur[2], ui[2], wre[2], wim[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const cx<float> u = Us[(m0 + 16 * t + li) * kWyB + i];  // A(m, k = i) = -U
+                const cx<float> w0 = Ws[(n0 + 16 * t + li) * (kWyB + 1) + i];
+                const cx<float> w1 = Ws[64 * (kWyB + 1) + (n0 + 16 * t + li) * (kWyB + 1) + i];
+                ur[t] = -u.re;
+                ui[t] = -u.im;
+                wre[t] = w0.re + w1.re;
+                wim[t] = w0.im + w1.im;
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    vr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[a], wre[b], vr[a][b], 0, 0, 0);
+                    vi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[a], wim[b], vi[a][b], 0, 0, 0);
+                }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    vr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui[a], wim[b], vr[a][b], 0, 0, 0);
+                    vi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[a], wre[b], vi[a][b], 0, 0, 0);
+                }
+        }
+        __syncthreads();  // Y, U, W, T are rewritten by the next block
+    }
+    // eigenvector n = V(:, perm[n]): output column of each V column
+    if (tid < 64) invp[tid] = -1;
+    __syncthreads();
+    for (int n = tid; n < N; n += 256) invp[min(max(perm[(size_t)kk * N + n], 0), N - 1)] = n;
+    __syncthreads();
+    cx<double>* Xs = reinterpret_cast<cx<double>*>(smem);  // overlays Y, U, W, T
+    cx<double>* o = out + (size_t)kk * NN;
+    for (int h = 0; h < 2; ++h) {
+        if (wc == h) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        Xs[(16 * b + li) * kWy32Ld + m0 + 16 * a + 4 * g + e] =
+                            cx<double>{(double)vr[a][b][e], (double)vi[a][b][e]};
+        }
+        __syncthreads();
+        for (int idx = tid; idx < 32 * 64; idx += 256) {
+            const int cl = idx >> 6, r = idx & 63, c = 32 * h + cl;
+            if (c < N && r < N) {
+                const int n = invp[c];
+                if (n >= 0) o[(size_t)n * N + r] = Xs[cl * kWy32Ld + r];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace lqm
