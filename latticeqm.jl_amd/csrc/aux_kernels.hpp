@@ -137,8 +137,22 @@ __global__ void k_phase_fold(int NN, int nRr, int S, const cx<double>* __restric
     const int e = (int)(idx % NN);
     const int g = key / RG, r = key % RG;
     cx<double> acc = rho[idx];
-    for (int s = 0; s < S; ++s) {
-        const cx<double> v = part[((size_t)(g * S + s) * RG + r) * NN + e];
+    const cx<double>* pp = part + ((size_t)g * S * RG + r) * NN + e;  // split s at pp[s * RG * NN]
+    const size_t step = (size_t)RG * NN;
+    // eight partials in flight, added in split order (the sum is the same as one at a time)
+    int s = 0;
+    for (; s + 8 <= S; s += 8) {
+        cx<double> v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = pp[(size_t)(s + q) * step];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            acc.re += v[q].re;
+            acc.im += v[q].im;
+        }
+    }
+    for (; s < S; ++s) {
+        const cx<double> v = pp[(size_t)s * step];
         acc.re += v.re;
         acc.im += v.im;
     }

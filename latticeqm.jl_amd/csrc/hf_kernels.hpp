@@ -202,12 +202,66 @@ __global__ __launch_bounds__(1024) void k_mu_bisect(int N, int L, const double* 
         return;
     }
     const double norm = kw ? (double)N : (double)L;
+    const double invT = 1.0 / T;
+    // the k-point of value i = threadIdx.x + 1024 q advances by 1024 / N per step: tracked
+    // incrementally instead of an integer division per value
+    const int kq = 1024 / N, rq = 1024 - kq * N;
+    // Up to kMuReg values per thread stay in registers across the bisection (with their
+    // weights); when the band range is below 1000 T they are kept as e_i = exp(e_i / T - c),
+    // c = the range's centre / T (exponents within +-500), and a step costs one exp in all:
+    // f_i(mu) = 1 / (e_i exp(c - mu / T) + 1).  Otherwise the values are read from memory
+    // and each f_i is formed directly.
+    constexpr int kMuReg = 16;
+    const bool inreg = L <= kMuReg * 1024;
+    const double cexp = 0.5 * (lo + hi) * invT;
+    const bool factor = inreg && (hi - lo) * invT <= 1000.0;
+    double xv[kMuReg], wv[kMuReg];
+    if (inreg) {
+        int k = threadIdx.x / N, n = threadIdx.x - k * N;
+#pragma unroll
+        for (int q = 0; q < kMuReg; ++q) {
+            const int i = threadIdx.x + 1024 * q;
+            const double x = i < L ? bands[i] : 0.0;
+            xv[q] = factor ? exp(fmin(fmax(x * invT - cexp, -700.0), 700.0)) : x;
+            wv[q] = i < L && kw ? kw[k] : 0.0;
+            k += kq;
+            n += rq;
+            if (n >= N) {
+                n -= N;
+                ++k;
+            }
+        }
+    }
     auto F = [&](double mu) -> double {
         double a = 0.0;
-        for (int i = threadIdx.x; i < L; i += 1024) {
-            const double x = bands[i] - mu;
-            const double f = 1.0 / (exp(x / T) + 1.0);
-            a += kw ? kw[i / N] * f : f;
+        if (factor) {
+            const double em = exp(cexp - mu * invT);
+#pragma unroll
+            for (int q = 0; q < kMuReg; ++q)
+                if (threadIdx.x + 1024 * q < L) {
+                    const double f = 1.0 / (xv[q] * em + 1.0);
+                    a += kw ? wv[q] * f : f;
+                }
+        } else if (inreg) {
+#pragma unroll
+            for (int q = 0; q < kMuReg; ++q)
+                if (threadIdx.x + 1024 * q < L) {
+                    const double f = 1.0 / (exp((xv[q] - mu) * invT) + 1.0);
+                    a += kw ? wv[q] * f : f;
+                }
+        } else {
+            int k = threadIdx.x / N, n = threadIdx.x - k * N;
+            for (int i = threadIdx.x; i < L; i += 1024) {
+                const double x = bands[i] - mu;
+                const double f = 1.0 / (exp(x * invT) + 1.0);
+                a += kw ? kw[k] * f : f;
+                k += kq;
+                n += rq;
+                if (n >= N) {
+                    n -= N;
+                    ++k;
+                }
+            }
         }
         return block_sum<4>(a, sred) / norm - filling;
     };

@@ -559,6 +559,13 @@ lqm_status launch_rho2(lqm_model* m, const Plan& P, int Kc, const double* evals,
     return LQM_OK;
 }
 
+// k-splits of a contraction launch with `tiles` element tiles x key groups: kContractBlocks
+// blocks in all (callers cap it by the chunk's k-points / kContractMinK, so that a small
+// matrix is not cut into splits of a few k-points each, which the fold would then sum
+// one partial at a time)
+constexpr int kContractMinK = 32;
+int contract_splits(int tiles) { return kContractBlocks / std::max(1, tiles); }
+
 // Key groups of the phase contraction: RG keys per group, the smallest of these holding nRr.
 int contract_group(int nRr) {
     return nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : nRr <= 24 ? 24 : nRr <= 28 ? 28 : 32;
@@ -717,7 +724,7 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
     const int RG = contract_group(nRr);
     const int groups = std::max(1, (nRr + RG - 1) / RG);
     const int tilesE = (int)((NN + 255) / 256);
-    int S = std::max(1, std::min(P.Kc, kContractBlocks / std::max(1, tilesE * groups)));
+    int S = std::max(1, std::min(std::max(1, P.Kc / kContractMinK), contract_splits(tilesE * groups)));
     cx<double>* part = nullptr;
     if ((st = wsT(c, W_PART, (size_t)groups * S * RG * NN, &part))) return st;
     const double wuni = 1.0 / (double)Nk;
@@ -805,7 +812,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
     const int RG = contract_group(nRr);
     const int groups = std::max(1, (nRr + RG - 1) / RG);
     const int tilesE = (int)((NN + 255) / 256);
-    const int S = std::max(1, std::min(P.Kc, kContractBlocks / std::max(1, tilesE * groups)));
+    const int S = std::max(1, std::min(std::max(1, P.Kc / kContractMinK), contract_splits(tilesE * groups)));
     cx<double>* part = nullptr;
     if (rho && nRr > 0 && (st = wsT(c, W_PART, (size_t)groups * S * RG * NN, &part))) return st;
     std::vector<double> hstat;
@@ -1132,7 +1139,7 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
     const int RG = contract_group(nRr);
     const int groups = std::max(1, (nRr + RG - 1) / RG);
     const int tilesE = (int)((NN + 255) / 256);
-    const int S = std::max(1, std::min(Kmax, kContractBlocks / std::max(1, tilesE * groups)));
+    const int S = std::max(1, std::min(std::max(1, Kmax / kContractMinK), contract_splits(tilesE * groups)));
     cx<double>* part = nullptr;
     if (rho && nRr > 0 && (st = wsT(c, W_PART, (size_t)groups * S * RG * NN, &part))) return st;
     std::vector<double> hlam, hnorm;
