@@ -152,6 +152,15 @@ __global__ __launch_bounds__(256) void k_fold2(int G, const double* __restrict__
     }
 }
 
+// occ[i] = fermidirac(e_i - mu; T) with mu read from the device (the chemical potential the
+// bisection below has just written), so no host round trip sits between mu and rho.
+__global__ __launch_bounds__(256) void k_fermi_occ(int L, const double* __restrict__ ev,
+                                                   const double* __restrict__ mu, double T,
+                                                   double* __restrict__ occ) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < L) occ[i] = fermidirac(ev[i], mu[0], T);
+}
+
 // Roots.jl-style bracketing midpoint: 0 across a sign change, else the midpoint of the
 // bit patterns of |a| and |b|, so a bisection reaches adjacent doubles in <= 64 steps.
 __device__ __forceinline__ double bit_middle_dev(double a, double b) {

@@ -49,7 +49,7 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -645,10 +645,16 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
 
 // Chemical potential requested from the pass itself: when the whole grid is one chunk, mu
 // is solved (filling, the pass's own eigenvalues) between the eigenvalues and the
-// occupations, and written to *mu; otherwise the pass returns kFallback untouched.
+// occupations, and written to *mu; otherwise the pass returns kFallback untouched.  With
+// dres (T > 0, per-k kernels) mu stays on the device: the bisection writes dres[0] = mu,
+// dres[1] = its status, the occupations are formed from dres[0], and the pass neither
+// writes *mu nor reads the status word -- the caller reads dres and the status with its
+// own next device-to-host copy.
 struct MuReq {
     double filling;
     double* mu;
+    double* dres = nullptr;
+    mutable bool on_device = false;  // set by the pass when it took the dres route
 };
 // internal status, never returned through the C ABI: take the two-pass route (7 lies in the
 // enumeration's value range, above every public code)
@@ -741,7 +747,21 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
             return st;
         }
         if ((st = launch_eigvals(m, P, Kc, ksc, ev, true))) return st;
-        if (mq) {
+        if (mq && mq->dres && T > 0.0) {
+            double* focc;
+            if ((st = wsT(c, W_FOCC, (size_t)N * Kc, &focc))) return st;
+            {
+                StageScope sc(c, ST_MU);
+                hipLaunchKernelGGL(lqm::k_mu_bisect, dim3(1), dim3(1024), 0, c->stream, N, N * Nk, ev, kw,
+                                   mq->filling, T, mq->dres);
+                const int L = N * Kc;
+                hipLaunchKernelGGL(lqm::k_fermi_occ, dim3((L + 255) / 256), dim3(256), 0, c->stream, L, ev,
+                                   mq->dres, T, focc);
+            }
+            LQM_HIP(hipGetLastError());
+            c->occ = focc;
+            mq->on_device = true;
+        } else if (mq) {
             if ((st = mu_from_bands(c, N, Nk, ev, kw, mq->filling, T, &mu))) return st;
             *mq->mu = mu;
         }
@@ -767,6 +787,7 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
             LQM_HIP(hipGetLastError());
         }
     }
+    if (mq && mq->on_device) return LQM_OK;  // the caller reads the status word with dres
     return check_status(c, "densitymatrix");
 }
 
@@ -1978,17 +1999,19 @@ lqm_status lqm_hartreefock_scf(lqm_context* ctx, int N, int d, int nRh, const in
     const double* ksd = static_cast<const double*>(dks);
     const double* kwd = static_cast<const double*>(dkw);
     const size_t nrho = (size_t)N * N * nRr;
-    double sc[8] = {};
+    double sc[10] = {};
     double mu = 0.0, eband = 0.0, residual = INFINITY;
     int it = 0;
     bool conv = false;
+    int hstat = 0;
     while (it < max_iter) {
         ++it;
         // f!(x1, x0): H_MF[rho0]; mu; rho1 and eps_band  (selfconsistent.jl:281-295, :374-381)
         if ((st = hf_apply(S, S.rho0))) return st;
         // one factorization for mu and rho when the grid is one chunk (FP64 eigenvalues);
-        // else bands, mu, then rho in a second pass
-        const MuReq mq{filling, &mu};
+        // else bands, mu, then rho in a second pass.  For T > 0 on the per-k kernels mu stays
+        // on the device (scal[8], scal[9]) and comes back with the iteration's scalars.
+        const MuReq mq{filling, &mu, S.scal + 8};
         st = run_rho(S.hmf, Nk, ksd, kwd, 0.0, T, nRr, Rr, reinterpret_cast<double*>(S.rho1), S.scal + 4,
                      S.bands, nullptr, &mq);
         if (st == kFallback) {
@@ -2006,7 +2029,20 @@ lqm_status lqm_hartreefock_scf(lqm_context* ctx, int N, int d, int nRh, const in
                                S.scal + 5);
         }
         LQM_HIP(hipGetLastError());
-        if ((st = d2h(c, sc, S.scal, sizeof(sc)))) return st;
+        if (mq.on_device) {
+            // one synchronisation for the scalars, mu and the kernels' status word
+            LQM_HIP(hipMemcpyAsync(sc, S.scal, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+            LQM_HIP(hipMemcpyAsync(&hstat, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            LQM_HIP(hipStreamSynchronize(c->stream));
+            if (hstat) {
+                if ((st = check_status(c, "densitymatrix"))) return st;
+            }
+            if (sc[9] == 2.0) return fail(LQM_ERR_NONFINITE, "non-finite bands");
+            if (sc[9] == 1.0) return fail(LQM_ERR_ARG, "filling not bracketed by the band range");
+            mu = sc[8];
+        } else if ((st = d2h(c, sc, S.scal, sizeof(sc)))) {
+            return st;
+        }
         if ((st = hf_check_energy(S, sc))) return st;
         eband = sc[4];
         const double E = eband - (sc[0] + sc[1]);
