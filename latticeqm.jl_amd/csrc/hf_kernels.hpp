@@ -177,9 +177,12 @@ __device__ __forceinline__ double bit_middle_dev(double a, double b) {
 // device in one block of 1024 threads (no host round trip per step).
 // getfilling = mean of fermidirac over all L = N*Nk values, or sum_k w_k sum_n f / N.
 // out[0] = mu, out[1] = 0 ok / 1 filling not bracketed / 2 non-finite bands.
+// ework (L doubles, or null): scratch for exp(e_i / T - c) when the values do not fit in
+// registers.
 __global__ __launch_bounds__(1024) void k_mu_bisect(int N, int L, const double* __restrict__ bands,
                                                     const double* __restrict__ kw, double filling,
-                                                    double T, double* __restrict__ out) {
+                                                    double T, double* __restrict__ out,
+                                                    double* __restrict__ ework) {
     __shared__ double sred[16];
     __shared__ double smin[16], smax[16];
     double lo = INFINITY, hi = -INFINITY;
@@ -216,22 +219,23 @@ __global__ __launch_bounds__(1024) void k_mu_bisect(int N, int L, const double* 
     // incrementally instead of an integer division per value
     const int kq = 1024 / N, rq = 1024 - kq * N;
     // Up to kMuReg values per thread stay in registers across the bisection (with their
-    // weights); when the band range is below 1000 T they are kept as e_i = exp(e_i / T - c),
-    // c = the range's centre / T (exponents within +-500), and a step costs one exp in all:
-    // f_i(mu) = 1 / (e_i exp(c - mu / T) + 1).  Otherwise the values are read from memory
-    // and each f_i is formed directly.
+    // weights), else they are read from memory.  Once the bracket [a, b] is within 1000 T,
+    // the values are turned into e_i = exp(e_i / T - c), c = the bracket's centre / T
+    // (exponents clamped to +-700; for mu in the bracket exp(c - mu / T) stays within
+    // e^{+-500}), in the registers or in ework, and a step costs one exp in all:
+    // f_i(mu) = 1 / (e_i exp(c - mu / T) + 1).  A clamped e_i gives f below e^-200 or above
+    // 1 - e^-200, as the direct form does to double precision.
     constexpr int kMuReg = 16;
     const bool inreg = L <= kMuReg * 1024;
-    const double cexp = 0.5 * (lo + hi) * invT;
-    const bool factor = inreg && (hi - lo) * invT <= 1000.0;
+    double cexp = 0.0;
+    bool factored = false;
     double xv[kMuReg], wv[kMuReg];
     if (inreg) {
         int k = threadIdx.x / N, n = threadIdx.x - k * N;
 #pragma unroll
         for (int q = 0; q < kMuReg; ++q) {
             const int i = threadIdx.x + 1024 * q;
-            const double x = i < L ? bands[i] : 0.0;
-            xv[q] = factor ? exp(fmin(fmax(x * invT - cexp, -700.0), 700.0)) : x;
+            xv[q] = i < L ? bands[i] : 0.0;
             wv[q] = i < L && kw ? kw[k] : 0.0;
             k += kq;
             n += rq;
@@ -241,9 +245,21 @@ __global__ __launch_bounds__(1024) void k_mu_bisect(int N, int L, const double* 
             }
         }
     }
+    auto try_factor = [&](double ba, double bb) {
+        if (factored || (bb - ba) * invT > 1000.0 || (!inreg && !ework)) return;
+        cexp = 0.5 * (ba + bb) * invT;
+        if (inreg) {
+#pragma unroll
+            for (int q = 0; q < kMuReg; ++q) xv[q] = exp(fmin(fmax(xv[q] * invT - cexp, -700.0), 700.0));
+        } else {
+            for (int i = threadIdx.x; i < L; i += 1024)
+                ework[i] = exp(fmin(fmax(bands[i] * invT - cexp, -700.0), 700.0));
+        }
+        factored = true;  // each thread reads back only the values it wrote: no barrier
+    };
     auto F = [&](double mu) -> double {
         double a = 0.0;
-        if (factor) {
+        if (factored && inreg) {
             const double em = exp(cexp - mu * invT);
 #pragma unroll
             for (int q = 0; q < kMuReg; ++q)
@@ -258,6 +274,19 @@ __global__ __launch_bounds__(1024) void k_mu_bisect(int N, int L, const double* 
                     const double f = 1.0 / (exp((xv[q] - mu) * invT) + 1.0);
                     a += kw ? wv[q] * f : f;
                 }
+        } else if (factored) {
+            const double em = exp(cexp - mu * invT);
+            int k = threadIdx.x / N, n = threadIdx.x - k * N;
+            for (int i = threadIdx.x; i < L; i += 1024) {
+                const double f = 1.0 / (ework[i] * em + 1.0);
+                a += kw ? kw[k] * f : f;
+                k += kq;
+                n += rq;
+                if (n >= N) {
+                    n -= N;
+                    ++k;
+                }
+            }
         } else {
             int k = threadIdx.x / N, n = threadIdx.x - k * N;
             for (int i = threadIdx.x; i < L; i += 1024) {
@@ -275,6 +304,7 @@ __global__ __launch_bounds__(1024) void k_mu_bisect(int N, int L, const double* 
         return block_sum<4>(a, sred) / norm - filling;
     };
     double a = lo, b = hi;
+    try_factor(a, b);
     double fa = F(a), fb = F(b);
     double mu = 0.0;
     int status = 0;
@@ -290,6 +320,7 @@ __global__ __launch_bounds__(1024) void k_mu_bisect(int N, int L, const double* 
         for (int it = 0; it < 200; ++it) {
             const double mid = bit_middle_dev(a, b);
             if (!(mid > a && mid < b)) break;
+            try_factor(a, b);
             const double fm = F(mid);
             if (fm == 0.0) {
                 mu = mid;

@@ -49,7 +49,7 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -748,12 +748,13 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
         }
         if ((st = launch_eigvals(m, P, Kc, ksc, ev, true))) return st;
         if (mq && mq->dres && T > 0.0) {
-            double* focc;
+            double *focc, *ework = nullptr;
             if ((st = wsT(c, W_FOCC, (size_t)N * Kc, &focc))) return st;
+            if ((size_t)N * Nk > 16 * 1024 && (st = wsT(c, W_MUE, (size_t)N * Nk, &ework))) return st;
             {
                 StageScope sc(c, ST_MU);
                 hipLaunchKernelGGL(lqm::k_mu_bisect, dim3(1), dim3(1024), 0, c->stream, N, N * Nk, ev, kw,
-                                   mq->filling, T, mq->dres);
+                                   mq->filling, T, mq->dres, ework);
                 const int L = N * Kc;
                 hipLaunchKernelGGL(lqm::k_fermi_occ, dim3((L + 255) / 256), dim3(256), 0, c->stream, L, ev,
                                    mq->dres, T, focc);
@@ -1392,13 +1393,14 @@ lqm_status mu_from_bands(lqm_context* c, int N, int Nk, const double* bands, con
         return LQM_OK;
     }
     // T > 0: bracketing bisection of getfilling(mu) - filling on [min, max], on the device
-    double* res;
+    double *res, *ework = nullptr;
     lqm_status st = wsT(c, W_SMALL, 8, &res);
     if (st) return st;
+    if (L > 16 * 1024 && (st = wsT(c, W_MUE, (size_t)L, &ework))) return st;
     {
         StageScope sc(c, ST_MU);
         hipLaunchKernelGGL(lqm::k_mu_bisect, dim3(1), dim3(1024), 0, c->stream, N, (int)L, bands, kw,
-                           filling, T, res);
+                           filling, T, res, ework);
     }
     LQM_HIP(hipGetLastError());
     double h[2];
