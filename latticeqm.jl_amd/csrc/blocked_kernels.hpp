@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void k_trd_col(int N, int j, int i, int last, 
 //   Q[k][J][1 + t], Q[k][J][1 + kBnb + t] (part 0): W(:, t)^H v and V(:, t)^H v over the
 //   rows of tile J (t < i).
 constexpr int kQStride = 1 + 2 * kBnb;
-constexpr int kHvPart = 4;
+constexpr int kHvPart = 8;
 constexpr int kHvWaves = 8;
 __host__ __device__ __forceinline__ int hv_parts(int nt, int J) { return (nt - J + kHvPart - 1) / kHvPart; }
 // number of (J, p) blocks of a column step with nt tile columns
@@ -212,7 +212,7 @@ __host__ __device__ __forceinline__ int hv_blocks(int nt) {
     for (int J = 0; J < nt; ++J) b += hv_parts(nt, J);
     return b;
 }
-__global__ __launch_bounds__(512) void k_trd_hemv(int N, int j, int i, int nchmax, int nbmax,
+__global__ __launch_bounds__(kHvWaves * 64) void k_trd_hemv(int N, int j, int i, int nchmax, int nbmax,
                                                   const cx<double>* __restrict__ A,
                                                   const cx<double>* __restrict__ V,
                                                   const cx<double>* __restrict__ W,

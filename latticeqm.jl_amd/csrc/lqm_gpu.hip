@@ -1107,7 +1107,7 @@ lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<double>* A, cx<doub
                 const int nch = (n + lqm::kHvCols - 1) / lqm::kHvCols;
                 hipLaunchKernelGGL(lqm::k_trd_col, dim3(Kc), dim3(256), 0, s, N, j, i, 0, A, Vp, Wp, dd, ee,
                                    tau);
-                hipLaunchKernelGGL(lqm::k_trd_hemv, dim3(lqm::hv_blocks(nch), Kc), dim3(512), 0, s, N, j, i, nchmax,
+                hipLaunchKernelGGL(lqm::k_trd_hemv, dim3(lqm::hv_blocks(nch), Kc), dim3(lqm::kHvWaves * 64), 0, s, N, j, i, nchmax,
                                    nbmax, A, Vp, Wp, P, P2, Q, SS);
                 hipLaunchKernelGGL(lqm::k_trd_w, dim3((n + 255) / 256, Kc), dim3(256), 0, s, N, j, i, nchmax, nbmax,
                                    P, P2, Q, SS, Vp, Wp, tau);
