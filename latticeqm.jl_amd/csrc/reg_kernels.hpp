@@ -181,7 +181,7 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_tridiag_r(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) void k_tridiag_r(
     int N, int Nkc, const cx<double>* __restrict__ Hk, R* __restrict__ dd, R* __restrict__ ee,
     cx<R>* __restrict__ refl, cx<R>* __restrict__ tau) {
     __shared__ RegTriSmem<R> S;
