@@ -197,11 +197,14 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     const size_t np = (size_t)N * (N + 1) / 2;
     p.shAw = cC * (np + 2 * N) + sizeof(cx<double>) * nR;
     // forced variant, else the tuned one, else the default (1 = compact WY for the
-    // back-transformation when N > 16, 0 otherwise: the WY kernel works on a 64 x 64 tile)
+    // back-transformation when N > 16, 0 otherwise: the WY kernel works on a 64 x 64 tile;
+    // the register-tile tridiagonalization for 16 < N <= 64)
     auto variant = [&](int kind) {
         if (c->forced[kind] >= 0) return c->forced[kind];
         auto it = c->tuned.find((kind << 20) | N);
-        return it == c->tuned.end() ? ((kind == VK_BT && N > 16) || kind == VK_TRIEIG ? 1 : 0) : it->second;
+        if (it != c->tuned.end()) return it->second;
+        if (kind == VK_TRIDIAG) return N > 16 && N <= 64 ? 2 : 0;  // register tile (measured fastest)
+        return (kind == VK_BT && N > 16) || kind == VK_TRIEIG ? 1 : 0;
     };
     // QL: k-points (lanes) per block, 64 >> variant; fewer lanes per wave put more
     // independent recurrences on each SIMD
