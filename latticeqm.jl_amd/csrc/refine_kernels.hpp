@@ -475,6 +475,11 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
 // occupation occ[kk*N + r] of the rank-r eigenvalue (lambda is in X's column order,
 // ties ranked by column); eb[kk] = w_k sum_n lambda_n f_n.  Dynamic LDS: N doubles.
 // Fermi-Dirac occupation of an eigenvalue (T = 0: step with 1/2 at lambda = mu).
+// Occupations of magnitude at or below 2^-70 are dropped from rho_k = X f X^H, in chunks of
+// 16 bands: each element of rho_k then moves by at most 64 * 2^-70 < 6e-20, far below the
+// rounding of its FP64 sum (occupations given by the caller may be negative; NaN is kept).
+constexpr double kOccNegligible = 0x1p-70;
+
 __device__ __forceinline__ double fermi_occ(double l, double mu, double T) {
     const double x = l - mu;
     return T > 0.0 ? 1.0 / (exp(x / T) + 1.0) : (x < 0.0 ? 1.0 : (x > 0.0 ? 0.0 : 0.5));
@@ -588,6 +593,32 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// X (N x N column-major) into the zero-padded 64 x 64 LDS tile Xs (leading dimension
+// kFusedLd): all 16 loads of a thread are issued before the first store (clamped addresses,
+// so every load is in bounds and unconditional).
+__device__ __forceinline__ void load_x_tile(const cx<double>* __restrict__ X, int N, cx<double>* Xs, int tid) {
+    const int r = tid & 63, c0 = tid >> 6;
+    cx<double> v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = X[(size_t)min(c0 + 4 * u, N - 1) * N + min(r, N - 1)];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int c = c0 + 4 * u;
+        Xs[c * kFusedLd + r] = (r < N && c < N) ? v[u] : cx<double>{0.0, 0.0};
+    }
+}
+
+// A cluster among the sorted Rayleigh quotients lam[0..N): some consecutive gap <= delta.
+// Wave 0 tests the N - 1 gaps at once and thread 0 records the answer.
+__device__ __forceinline__ void cluster_flag(const double* lam, int N, double delta, int wave, int lane,
+                                             int* flag) {
+    if (wave == 0) {
+        const bool close = lane + 1 < N && fabs(lam[min(lane + 1, 63)] - lam[lane]) <= delta;
+        const unsigned long long m = __ballot(close);
+        if (lane == 0) *flag = m != 0ull;
+    }
+}
+
 // acc[a][b][i] <-> element (m0 + 16a + g + 4i, n0 + 16b + li) of a 32 x 32 wave quadrant
 template <int OPA>
 __device__ __forceinline__ void fused_gemm(const cx<double>* Asrc, int ldA, bool global_a, int N,
@@ -603,7 +634,8 @@ __device__ __forceinline__ void fused_gemm(const cx<double>* Asrc, int ldA, bool
             cr[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
             ci[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
         }
-    for (int k0 = 0; k0 < 64; k0 += 16) {
+    const int kend = (N + 15) & ~15;  // the inner index runs over N (zero padded)
+    for (int k0 = 0; k0 < kend; k0 += 16) {
         const int kb = k0 + 4 * g;
         double ar[2][4], ai[2][4], br[2][4], bi[2][4];
 #pragma unroll
@@ -668,6 +700,7 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
     double* red = corr + 128;                                      // [32]
     double* sf = red + 32;                                         // [64]
     __shared__ int s_cluster;
+    __shared__ unsigned long long s_live;  // bands whose occupation enters rho_k
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, g = lane >> 4;
     const int wr = wave & 1, wc = wave >> 1;
@@ -678,10 +711,7 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
     cx<double>* X = Xg + kk * NN;
 
     // X into LDS, zero padded to 64 x 64
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int r = e & 63, c = e >> 6;
-        Xs[c * kFusedLd + r] = (r < N && c < N) ? X[(size_t)c * N + r] : cx<double>{0.0, 0.0};
-    }
+    load_x_tile(X, N, Xs, tid);
     if (tid == 0) s_cluster = 0;
     __syncthreads();
 
@@ -742,11 +772,7 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
     const double rn = sqrt(red[0] + red[1] + red[2] + red[3]);
     const double an = fmax(fmax(red[8], red[9]), fmax(red[10], red[11]));
     const double delta = 2.0 * (sqrt(red[4] + red[5] + red[6] + red[7]) + an * rn);
-    if (tid == 0) {
-        int cl = 0;
-        for (int i = 0; i + 1 < N; ++i) cl |= fabs(lam[i + 1] - lam[i]) <= delta;
-        s_cluster = cl;
-    }
+    cluster_flag(lam, N, delta, wave, lane, &s_cluster);
     __syncthreads();
     if (s_cluster) {
         // hand S and G to the general correction (Rayleigh-Ritz on the clusters)
@@ -792,8 +818,9 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
                         const double gap = ln - lam[m];
                         if (fabs(gap) > delta) {
                             const double tre = sr[a][b][i] + ln * rre, tim = si[a][b][i] + ln * rim;
-                            e = {tre / gap, tim / gap};
-                            cpart[b] += (tre * tre + tim * tim) / gap;
+                            const double rg = 1.0 / gap;
+                            e = {tre * rg, tim * rg};
+                            cpart[b] += (tre * tre + tim * tim) * rg;
                         } else {
                             e = {0.5 * rre, 0.5 * rim};
                         }
@@ -877,9 +904,14 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
     ebp = wave_sum(ebp);
     if (lane == 0) red[16 + wave] = ebp;
     __syncthreads();  // the new X is in LDS; every lambda has been read from sf[]
-    if (tid < 64) sf[tid] = fn;
+    if (tid < 64) {
+        sf[tid] = fn;
+        const unsigned long long live = __ballot(!(fabs(fn) <= kOccNegligible));
+        if (tid == 0) s_live = live;
+    }
     if (tid == 0) rq.eb[kk] = (rq.kw ? rq.kw[kk] : rq.wuni) * (red[16] + red[17] + red[18] + red[19]);
     __syncthreads();
+    const unsigned long long live = s_live;
     lqm_d4 cr[2][2], ci[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -890,6 +922,7 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
         }
     // A(m, c) = X(m, c) f_c and B(c, n) = conj X(n, c); lane (li, g) supplies c = k0 + 4g + s
     for (int k0 = 0; k0 < 64; k0 += 16) {
+        if (((live >> k0) & 0xFFFFull) == 0) continue;  // 16 bands of negligible occupation
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int c = k0 + 4 * g + s;
@@ -954,6 +987,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     double* sf = red + 32;                                         // [64] occupations
     double* lnew = sf + 64;                                        // [64] refined eigenvalues
     __shared__ int s_cluster;
+    __shared__ unsigned long long s_live;  // bands whose occupation enters rho_k
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, g = lane >> 4;
     const int n = 16 * wave + li;  // this lane's column in every product
@@ -961,10 +995,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const size_t NN = (size_t)N * N;
     const cx<double>* A = Ag + kk * NN;
     cx<double>* X = Xg + kk * NN;
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int r = e & 63, c = e >> 6;
-        Xs[c * kFusedLd + r] = (r < N && c < N) ? X[(size_t)c * N + r] : cx<double>{0.0, 0.0};
-    }
+    load_x_tile(X, N, Xs, tid);
     if (tid == 0) s_cluster = 0;
     __syncthreads();
 
@@ -973,8 +1004,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     for (int t = 0; t < 4; ++t) {
         wr[t] = wi[t] = gr[t] = gi[t] = sr[t] = si[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
     }
-    // W = A X and G = X^H X over k = k0 + 4g + s
-    for (int k0 = 0; k0 < 64; k0 += 16) {
+    // W = A X and G = X^H X over k = k0 + 4g + s (rows k >= N of X are zero)
+    const int kend = (N + 15) & ~15;
+    for (int k0 = 0; k0 < kend; k0 += 16) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int k = k0 + 4 * g + s;
@@ -1069,11 +1101,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const double rn = sqrt(red[0] + red[1] + red[2] + red[3]);
     const double an = fmax(fmax(red[8], red[9]), fmax(red[10], red[11]));
     const double delta = 2.0 * (sqrt(red[4] + red[5] + red[6] + red[7]) + an * rn);
-    if (tid == 0) {
-        int cl = 0;
-        for (int i = 0; i + 1 < N; ++i) cl |= fabs(lam[i + 1] - lam[i]) <= delta;
-        s_cluster = cl;
-    }
+    cluster_flag(lam, N, delta, wave, lane, &s_cluster);
     __syncthreads();
     if (s_cluster) {
         cx<double>* Sk = Sg + kk * NN;
@@ -1114,9 +1142,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                     const double gap = ln - lam[m];
                     if (fabs(gap) > delta) {
                         const double tre = sr[t][i] + ln * rre, tim = si[t][i] + ln * rim;
-                        ere = tre / gap;
-                        eim = tim / gap;
-                        cpart += (tre * tre + tim * tim) / gap;
+                        const double rg = 1.0 / gap;
+                        ere = tre * rg;
+                        eim = tim * rg;
+                        cpart += (tre * tre + tim * tim) * rg;
                     } else {
                         ere = 0.5 * rre;
                         eim = 0.5 * rim;
@@ -1213,15 +1242,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
         ebp = l * fn;
     }
-    if (tid < 64) sf[tid] = fn;
+    if (tid < 64) {
+        sf[tid] = fn;
+        const unsigned long long live = __ballot(!(fabs(fn) <= kOccNegligible));
+        if (tid == 0) s_live = live;
+    }
     ebp = wave_sum(ebp);
     if (lane == 0) red[16 + wave] = ebp;
     __syncthreads();
     if (tid == 0) rq.eb[kk] = (rq.kw ? rq.kw[kk] : rq.wuni) * (red[16] + red[17] + red[18] + red[19]);
+    const unsigned long long live = s_live;
     lqm_d4 cr[4], ci[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) cr[t] = ci[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
     for (int k0 = 0; k0 < 64; k0 += 16) {
+        if (((live >> k0) & 0xFFFFull) == 0) continue;  // 16 bands of negligible occupation
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int c = k0 + 4 * g + s;

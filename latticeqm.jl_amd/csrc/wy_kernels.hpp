@@ -268,6 +268,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         if (lane == 0) S.red[wave] = ebp;
         __syncthreads();
         if (tid == 0) eb[kk] = (kw ? kw[kk] : wuni) * (S.red[0] + S.red[1] + S.red[2] + S.red[3]);
+        // columns whose occupation enters the product (the rest are below kOccNegligible)
+        const unsigned long long live = __ballot(!(fabs(S.sf[lane]) <= kOccNegligible));
         lqm_d4 cr[2][2], ci[2][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -277,6 +279,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 ci[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
             }
         for (int k0 = 0; k0 < 64; k0 += 16) {
+            if (((live >> k0) & 0xFFFFull) == 0) continue;  // 16 columns of negligible occupation
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int c = k0 + 4 * g + s;

@@ -51,6 +51,22 @@ def test_mixed_rho(pkg, oracle, mixed, N, nk, mu, T):
     assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
 
 
+@pytest.mark.parametrize("mu", [-1e3, 1e3, -3.0])
+def test_mixed_rho_empty_full_and_partly_occupied_chunks(pkg, oracle, mixed, mu):
+    # the fused sweep drops chunks of 16 bands whose occupations are all below 2^-70:
+    # no band occupied, every band occupied (rho_k = I), and empty chunks beside partly
+    # filled ones
+    h = pkg.models.random_tb(64, rmax=1, seed=62)
+    ks = pkg.regulargrid(nk=16)
+    rho = pkg.Hops({R: np.zeros((64, 64), complex) for R in h.keys()})
+    eb = pkg.api.getdensitymatrix(rho, h, ks, mu, T=0.02, executor=mixed)
+    R, Tt = h.pack()
+    ref, eb_ref = oracle.densitymatrix(R, Tt, ks, R, mu=mu, Temp=0.02)
+    err = max(np.max(np.abs(rho[tuple(R[:, r])] - ref[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+
+
 def test_mixed_near_degenerate_needs_second_sweep(pkg, oracle, mixed):
     # spin-degenerate levels split by 1e-7: closer than FP32 can resolve
     h = pkg.models.addspin(pkg.models.random_tb(8, rmax=1, seed=21), "spinhalf")

@@ -45,6 +45,15 @@ def test_rho_random_models(pkg, oracle, gpu, variants, N, nk, mu, T):
     _rho_vs_oracle(pkg, oracle, h, pkg.regulargrid(nk=nk), mu, T)
 
 
+@pytest.mark.parametrize("mu", [-1e3, 1e3, -1.5])
+def test_rho_empty_full_and_partly_occupied_chunks(pkg, oracle, gpu, variants, mu):
+    # occupations enter rho_k in chunks of 16 bands, and a chunk whose occupations are all
+    # below 2^-70 is dropped: mu below every band (rho = 0), above every band (rho_k = I)
+    # and a mu that leaves whole chunks empty beside partly filled ones
+    h = pkg.models.random_tb(48, rmax=1, seed=61)
+    _rho_vs_oracle(pkg, oracle, h, pkg.regulargrid(nk=25), mu, 0.02)
+
+
 def test_rho_degenerate_spinful_all_variants(pkg, oracle, gpu, variants):
     h = pkg.models.addspin(pkg.models.random_tb(5, rmax=1, seed=3), "spinhalf")
     _rho_vs_oracle(pkg, oracle, h, pkg.regulargrid(nk=36), 0.1, 0.0)
