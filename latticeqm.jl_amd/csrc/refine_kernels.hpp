@@ -619,6 +619,19 @@ __device__ __forceinline__ void cluster_flag(const double* lam, int N, double de
     }
 }
 
+// Combines three real products into a complex result, re = P1 - P2 and im = P3 - P1 - P2,
+// and keeps the result there: the empty asm statements stop the compiler from sinking the
+// combination past the next product, which would keep all three accumulator sets live.
+__device__ __forceinline__ void combine3(const lqm_d4 (&p1)[4], const lqm_d4 (&p2)[4], const lqm_d4 (&p3)[4],
+                                         lqm_d4 (&re)[4], lqm_d4 (&im)[4]) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        re[t] = p1[t] - p2[t];
+        im[t] = p3[t] - p1[t] - p2[t];
+        asm volatile("" : "+v"(re[t]), "+v"(im[t]));
+    }
+}
+
 // acc[a][b][i] <-> element (m0 + 16a + g + 4i, n0 + 16b + li) of a 32 x 32 wave quadrant
 template <int OPA>
 __device__ __forceinline__ void fused_gemm(const cx<double>* Asrc, int ldA, bool global_a, int N,
@@ -999,71 +1012,100 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (tid == 0) s_cluster = 0;
     __syncthreads();
 
+    // Complex products as three real ones (P1 = ar br, P2 = ai bi, P3 = (ar + ai)(br + bi);
+    // re = P1 - P2, im = P3 - P1 - P2): three accumulator sets live only inside a product.
     lqm_d4 wr[4], wi[4], gr[4], gi[4], sr[4], si[4];
+    {
+        // W = A X over k = k0 + 4g + s
+        lqm_d4 p1[4], p2[4], p3[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        wr[t] = wi[t] = gr[t] = gi[t] = sr[t] = si[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
-    }
-    // W = A X and G = X^H X over k = k0 + 4g + s (rows k >= N of X are zero)
-    const int kend = (N + 15) & ~15;
-    for (int k0 = 0; k0 < kend; k0 += 16) {
+        for (int t = 0; t < 4; ++t) p1[t] = p2[t] = p3[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+        const int kend = (N + 15) & ~15;  // rows k >= N of X are zero
+        for (int k0 = 0; k0 < kend; k0 += 16) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int k = k0 + 4 * g + s;
-            const cx<double> b = Xs[n * kFusedLd + k];
-            double ar[4], ai[4], xr[4], xi[4];
+            for (int s = 0; s < 4; ++s) {
+                const int k = k0 + 4 * g + s;
+                const cx<double> b = Xs[n * kFusedLd + k];
+                const double bs = b.re + b.im;
+                double ar[4], ai[4], as[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int m = 16 * t + li;
-                const cx<double> a = Ag[kk * NN + (size_t)min(k, N - 1) * N + min(m, N - 1)];
-                const bool in = m < N && k < N;
-                ar[t] = in ? a.re : 0.0;
-                ai[t] = in ? a.im : 0.0;
-                const cx<double> x = Xs[m * kFusedLd + k];  // conj(X(k, m)) = X^H(m, k)
-                xr[t] = x.re;
-                xi[t] = -x.im;
-            }
+                for (int t = 0; t < 4; ++t) {
+                    const int m = 16 * t + li;
+                    const cx<double> a = Ag[kk * NN + (size_t)min(k, N - 1) * N + min(m, N - 1)];
+                    const bool in = m < N && k < N;
+                    ar[t] = in ? a.re : 0.0;
+                    ai[t] = in ? a.im : 0.0;
+                    as[t] = ar[t] + ai[t];
+                }
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                wr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], b.re, wr[t], 0, 0, 0);
-                wi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], b.im, wi[t], 0, 0, 0);
-                gr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], b.re, gr[t], 0, 0, 0);
-                gi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], b.im, gi[t], 0, 0, 0);
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                wr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[t], b.im, wr[t], 0, 0, 0);
-                wi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[t], b.re, wi[t], 0, 0, 0);
-                gr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-xi[t], b.im, gr[t], 0, 0, 0);
-                gi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], b.re, gi[t], 0, 0, 0);
+                for (int t = 0; t < 4; ++t) {
+                    p1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], b.re, p1[t], 0, 0, 0);
+                    p2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[t], b.im, p2[t], 0, 0, 0);
+                    p3[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[t], bs, p3[t], 0, 0, 0);
+                }
             }
         }
+        combine3(p1, p2, p3, wr, wi);
     }
-    // S = X^H W: B from W's accumulators, k = 16a + g + 4s
+    {
+        // S = X^H W: A(m, k) = conj X(k, m) from LDS, B from W's accumulators (k = 16a + g + 4s)
+        lqm_d4 p1[4], p2[4], p3[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+        for (int t = 0; t < 4; ++t) p1[t] = p2[t] = p3[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int k = 16 * a + g + 4 * s;
-            const double br = wr[a][s], bi = wi[a][s];
-            double xr[4], xi[4];
+        for (int a = 0; a < 4; ++a) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const cx<double> x = Xs[(16 * t + li) * kFusedLd + k];
-                xr[t] = x.re;
-                xi[t] = -x.im;
+            for (int s = 0; s < 4; ++s) {
+                const int k = 16 * a + g + 4 * s;
+                const double br = wr[a][s], bi = wi[a][s], bs = br + bi;
+                double xr[4], xi[4], xs[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const cx<double> x = Xs[(16 * t + li) * kFusedLd + k];
+                    xr[t] = x.re;
+                    xi[t] = -x.im;
+                    xs[t] = x.re - x.im;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    p1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], br, p1[t], 0, 0, 0);
+                    p2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], bi, p2[t], 0, 0, 0);
+                    p3[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[t], bs, p3[t], 0, 0, 0);
+                }
             }
+            __builtin_amdgcn_sched_barrier(0);  // bounds the LDS loads hoisted ahead (registers)
+        }
+        combine3(p1, p2, p3, sr, si);
+    }
+    {
+        // G = X^H X over k = k0 + 4g + s, both operands from LDS
+        lqm_d4 p1[4], p2[4], p3[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                sr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], br, sr[t], 0, 0, 0);
-                si[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], bi, si[t], 0, 0, 0);
-            }
+        for (int t = 0; t < 4; ++t) p1[t] = p2[t] = p3[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+        const int kend = (N + 15) & ~15;
+        for (int k0 = 0; k0 < kend; k0 += 16) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                sr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-xi[t], bi, sr[t], 0, 0, 0);
-                si[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], br, si[t], 0, 0, 0);
+            for (int s = 0; s < 4; ++s) {
+                const int k = k0 + 4 * g + s;
+                const cx<double> b = Xs[n * kFusedLd + k];
+                const double bs = b.re + b.im;
+                double xr[4], xi[4], xs[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const cx<double> x = Xs[(16 * t + li) * kFusedLd + k];  // conj(X(k, m)) = X^H(m, k)
+                    xr[t] = x.re;
+                    xi[t] = -x.im;
+                    xs[t] = x.re - x.im;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    p1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], b.re, p1[t], 0, 0, 0);
+                    p2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], b.im, p2[t], 0, 0, 0);
+                    p3[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[t], bs, p3[t], 0, 0, 0);
+                }
             }
         }
+        combine3(p1, p2, p3, gr, gi);
     }
     // Rayleigh quotients from the diagonal: (n, n) sits in tile `wave`, element i = li / 4 of
     // the lanes with g = li % 4
@@ -1173,34 +1215,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         stat[kk * 4 + 2] = (more ? 4 : 0) | (form_rho ? 16 : 0);
         stat[kk * 4 + 3] = em;
     }
-    // X E: B from E's accumulators (k = 16a + g + 4s), A = X from LDS
+    // X E: B from E's accumulators (k = 16a + g + 4s), A = X from LDS (three real products)
     lqm_d4 xr_[4], xi_[4];
+    {
+        lqm_d4 p1[4], p2[4], p3[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) xr_[t] = xi_[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+        for (int t = 0; t < 4; ++t) p1[t] = p2[t] = p3[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+        for (int a = 0; a < 4; ++a) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int k = 16 * a + g + 4 * s;
-            const double br = sr[a][s], bi = si[a][s];
-            double xr[4], xi[4];
+            for (int s = 0; s < 4; ++s) {
+                const int k = 16 * a + g + 4 * s;
+                const double br = sr[a][s], bi = si[a][s], bs = br + bi;
+                double xr[4], xi[4], xs[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const cx<double> x = Xs[k * kFusedLd + 16 * t + li];
-                xr[t] = x.re;
-                xi[t] = x.im;
+                for (int t = 0; t < 4; ++t) {
+                    const cx<double> x = Xs[k * kFusedLd + 16 * t + li];
+                    xr[t] = x.re;
+                    xi[t] = x.im;
+                    xs[t] = x.re + x.im;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    p1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], br, p1[t], 0, 0, 0);
+                    p2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], bi, p2[t], 0, 0, 0);
+                    p3[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[t], bs, p3[t], 0, 0, 0);
+                }
             }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                xr_[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], br, xr_[t], 0, 0, 0);
-                xi_[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], bi, xi_[t], 0, 0, 0);
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                xr_[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-xi[t], bi, xr_[t], 0, 0, 0);
-                xi_[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], br, xi_[t], 0, 0, 0);
-            }
+            __builtin_amdgcn_sched_barrier(0);  // bounds the LDS loads hoisted ahead (registers)
         }
+        combine3(p1, p2, p3, xr_, xi_);
     }
     // X + X E
 #pragma unroll
@@ -1253,34 +1297,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (tid == 0) rq.eb[kk] = (rq.kw ? rq.kw[kk] : rq.wuni) * (red[16] + red[17] + red[18] + red[19]);
     const unsigned long long live = s_live;
     lqm_d4 cr[4], ci[4];
+    {
+        lqm_d4 p1[4], p2[4], p3[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) cr[t] = ci[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < 64; k0 += 16) {
-        if (((live >> k0) & 0xFFFFull) == 0) continue;  // 16 bands of negligible occupation
+        for (int t = 0; t < 4; ++t) p1[t] = p2[t] = p3[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < 64; k0 += 16) {
+            if (((live >> k0) & 0xFFFFull) == 0) continue;  // 16 bands of negligible occupation
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int c = k0 + 4 * g + s;
-            const double f = sf[c];
-            const cx<double> y = Xs[c * kFusedLd + n];  // B(c, n) = conj X(n, c)
-            const double yr = y.re, yi = -y.im;
-            double xr[4], xi[4];
+            for (int s = 0; s < 4; ++s) {
+                const int c = k0 + 4 * g + s;
+                const double f = sf[c];
+                const cx<double> y = Xs[c * kFusedLd + n];  // B(c, n) = conj X(n, c)
+                const double yr = y.re, yi = -y.im, ys = y.re - y.im;
+                double xr[4], xi[4], xs[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const cx<double> x = Xs[c * kFusedLd + 16 * t + li];
-                xr[t] = f * x.re;
-                xi[t] = f * x.im;
-            }
+                for (int t = 0; t < 4; ++t) {
+                    const cx<double> x = Xs[c * kFusedLd + 16 * t + li];
+                    xr[t] = f * x.re;
+                    xi[t] = f * x.im;
+                    xs[t] = xr[t] + xi[t];
+                }
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], yr, cr[t], 0, 0, 0);
-                ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], yi, ci[t], 0, 0, 0);
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-xi[t], yi, cr[t], 0, 0, 0);
-                ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], yr, ci[t], 0, 0, 0);
+                for (int t = 0; t < 4; ++t) {
+                    p1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], yr, p1[t], 0, 0, 0);
+                    p2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], yi, p2[t], 0, 0, 0);
+                    p3[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[t], ys, p3[t], 0, 0, 0);
+                }
             }
         }
+        combine3(p1, p2, p3, cr, ci);
     }
     cx<double>* Rk = rq.rhok + kk * NN;
 #pragma unroll
