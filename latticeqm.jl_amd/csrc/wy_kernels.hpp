@@ -386,42 +386,49 @@ __global__ __launch_bounds__(256) void k_backtransform_wy32(int N, const cx<floa
     }
     const int nrefl = N - 1;
     const int nb = (nrefl + kWyB - 1) / kWyB;
-    for (int b0 = nb - 1; b0 >= 0; --b0) {
-        const int j0 = b0 * kWyB;
-        // (1) the block's reflectors (zero above row j + 1) and its T factor
-        for (int e = tid; e < 64 * kWyB; e += 256) {
-            const int r = e >> 4, i = e & 15;
-            const int j = j0 + i;
-            cx<float> y = {0.f, 0.f};
-            if (j < nrefl && r > j && r < N) y = rf[(size_t)j * N + r];
-            Ys[e] = y;
-        }
-        {
-            const cx<double> t = Tg[((size_t)kk * nb + b0) * (kWyB * kWyB) + tid];
-            Ts[tid] = cx<float>{(float)t.re, (float)t.im};
-        }
-        __syncthreads();
-        // (2) W_wr = Y(R_wr, :)^H V(R_wr, C_wc) and U = Y T (wave w: rows 16 w .. 16 w + 15)
-        {
-            lqm_f4 wr_[2], wi_[2];
+    // the reflectors (zero above row j + 1) and the T factor of block b, held in registers one
+    // block ahead: thread tid holds Y entries e = tid + 256 u (row e / 16, column e % 16)
+    cx<float> ypre[4];
+    cx<double> tpre;
+    auto fetch = [&](int b) {
+        const int j0 = b * kWyB;
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                wr_[b] = lqm_f4{0.f, 0.f, 0.f, 0.f};
-                wi_[b] = lqm_f4{0.f, 0.f, 0.f, 0.f};
-            }
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            const int r = e >> 4, j = j0 + (e & 15);
+            const cx<float> y = rf[(size_t)min(j, N - 1) * N + min(r, N - 1)];
+            ypre[u] = (j < nrefl && r > j && r < N) ? y : cx<float>{0.f, 0.f};
+        }
+        tpre = Tg[((size_t)kk * nb + b) * (kWyB * kWyB) + tid];
+    };
+    if (nb > 0) fetch(nb - 1);
+    for (int b0 = nb - 1; b0 >= 0; --b0) {
+        // (1) the block's reflectors and its T factor into LDS; the next block's are fetched
+#pragma unroll
+        for (int u = 0; u < 4; ++u) Ys[tid + 256 * u] = ypre[u];
+        Ts[tid] = cx<float>{(float)tpre.re, (float)tpre.im};
+        __syncthreads();
+        if (b0 > 0) fetch(b0 - 1);
+        // (2) W_wr = Y(R_wr, :)^H V(R_wr, C_wc) and U = Y T (wave w: rows 16 w .. 16 w + 15);
+        //     complex products as three real ones (P1 = ar br, P2 = ai bi,
+        //     P3 = (ar + ai)(br + bi); re = P1 - P2, im = P3 - P1 - P2)
+        {
+            lqm_f4 p1[2], p2[2], p3[2];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) p1[b] = p2[b] = p3[b] = lqm_f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     const int r = m0 + 16 * a + 4 * g + s;
                     const cx<float> y = Ys[r * kWyB + li];  // A(m = li, k) = conj(Y(r, li))
+                    const float ar = y.re, ai = -y.im, as = y.re - y.im;
 #pragma unroll
                     for (int b = 0; b < 2; ++b) {
                         const float br = vr[a][b][s], bi = vi[a][b][s];
-                        wr_[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(y.re, br, wr_[b], 0, 0, 0);
-                        wi_[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(y.re, bi, wi_[b], 0, 0, 0);
-                        wr_[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(y.im, bi, wr_[b], 0, 0, 0);
-                        wi_[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-y.im, br, wi_[b], 0, 0, 0);
+                        p1[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, p1[b], 0, 0, 0);
+                        p2[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, bi, p2[b], 0, 0, 0);
+                        p3[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(as, br + bi, p3[b], 0, 0, 0);
                     }
                 }
 #pragma unroll
@@ -429,50 +436,60 @@ __global__ __launch_bounds__(256) void k_backtransform_wy32(int N, const cx<floa
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
                     Ws[wr * 64 * (kWyB + 1) + (n0 + 16 * b + li) * (kWyB + 1) + 4 * g + e] =
-                        cx<float>{wr_[b][e], wi_[b][e]};
-            lqm_f4 ur = {0.f, 0.f, 0.f, 0.f}, ui = {0.f, 0.f, 0.f, 0.f};
+                        cx<float>{p1[b][e] - p2[b][e], p3[b][e] - p1[b][e] - p2[b][e]};
+            lqm_f4 q1 = {0.f, 0.f, 0.f, 0.f}, q2 = q1, q3 = q1;
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int l = 4 * g + s;
                 const cx<float> y = Ys[(16 * wave + li) * kWyB + l];  // A(m = li, k = l)
                 const cx<float> t = Ts[l * kWyB + li];                // B(k = l, n = li)
-                ur = __builtin_amdgcn_mfma_f32_16x16x4f32(y.re, t.re, ur, 0, 0, 0);
-                ui = __builtin_amdgcn_mfma_f32_16x16x4f32(y.re, t.im, ui, 0, 0, 0);
-                ur = __builtin_amdgcn_mfma_f32_16x16x4f32(-y.im, t.im, ur, 0, 0, 0);
-                ui = __builtin_amdgcn_mfma_f32_16x16x4f32(y.im, t.re, ui, 0, 0, 0);
+                q1 = __builtin_amdgcn_mfma_f32_16x16x4f32(y.re, t.re, q1, 0, 0, 0);
+                q2 = __builtin_amdgcn_mfma_f32_16x16x4f32(y.im, t.im, q2, 0, 0, 0);
+                q3 = __builtin_amdgcn_mfma_f32_16x16x4f32(y.re + y.im, t.re + t.im, q3, 0, 0, 0);
             }
 #pragma unroll
-            for (int e = 0; e < 4; ++e) Us[(16 * wave + 4 * g + e) * kWyB + li] = cx<float>{ur[e], ui[e]};
+            for (int e = 0; e < 4; ++e)
+                Us[(16 * wave + 4 * g + e) * kWyB + li] = cx<float>{q1[e] - q2[e], q3[e] - q1[e] - q2[e]};
         }
         __syncthreads();
         // (3) V <- V - U W, W = W_0 + W_1
+        {
+            lqm_f4 p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int i = 4 * g + s;
-            float ur[2], ui[2], wre[2], wim[2];
+            for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const cx<float> u = Us[(m0 + 16 * t + li) * kWyB + i];  // A(m, k = i) = -U
-                const cx<float> w0 = Ws[(n0 + 16 * t + li) * (kWyB + 1) + i];
-                const cx<float> w1 = Ws[64 * (kWyB + 1) + (n0 + 16 * t + li) * (kWyB + 1) + i];
-                ur[t] = -u.re;
-                ui[t] = -u.im;
-                wre[t] = w0.re + w1.re;
-                wim[t] = w0.im + w1.im;
+                for (int b = 0; b < 2; ++b) p1[a][b] = p2[a][b] = p3[a][b] = lqm_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int i = 4 * g + s;
+                float ur[2], ui[2], us[2], wre[2], wim[2], ws[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const cx<float> u = Us[(m0 + 16 * t + li) * kWyB + i];  // A(m, k = i) = -U
+                    const cx<float> w0 = Ws[(n0 + 16 * t + li) * (kWyB + 1) + i];
+                    const cx<float> w1 = Ws[64 * (kWyB + 1) + (n0 + 16 * t + li) * (kWyB + 1) + i];
+                    ur[t] = -u.re;
+                    ui[t] = -u.im;
+                    us[t] = ur[t] + ui[t];
+                    wre[t] = w0.re + w1.re;
+                    wim[t] = w0.im + w1.im;
+                    ws[t] = wre[t] + wim[t];
+                }
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        p1[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[a], wre[b], p1[a][b], 0, 0, 0);
+                        p2[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[a], wim[b], p2[a][b], 0, 0, 0);
+                        p3[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(us[a], ws[b], p3[a][b], 0, 0, 0);
+                    }
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    vr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[a], wre[b], vr[a][b], 0, 0, 0);
-                    vi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[a], wim[b], vi[a][b], 0, 0, 0);
-                }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    vr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui[a], wim[b], vr[a][b], 0, 0, 0);
-                    vi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[a], wre[b], vi[a][b], 0, 0, 0);
+                    vr[a][b] += p1[a][b] - p2[a][b];
+                    vi[a][b] += p3[a][b] - p1[a][b] - p2[a][b];
                 }
         }
         __syncthreads();  // Y, U, W, T are rewritten by the next block
