@@ -130,96 +130,113 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
     const int nrefl = N - 1;
     const int nb = (nrefl + kWyB - 1) / kWyB;
-    for (int b0 = nb - 1; b0 >= 0; --b0) {
-        const int j0 = b0 * kWyB;
-        // (1) the block's reflectors: Y(r, i) = v_{j0+i}(r), zero above row j+1
-        for (int e = tid; e < 64 * kWyB; e += 256) {
-            const int r = e >> 4, i = e & 15;
-            const int j = j0 + i;
-            cx<double> y = {0.0, 0.0};
-            if (j < nrefl && r > j && r < N) {
-                const cx<R> v = rf[(size_t)j * N + r];
-                y = {(double)v.re, (double)v.im};
-            }
-            S.Y[e] = y;
-        }
-        {
-            const cx<double>* Tb = Tg + ((size_t)kk * nb + b0) * (kWyB * kWyB);
-            S.T[tid] = Tb[tid];  // 256 threads, 256 entries
-        }
-        __syncthreads();
-        // (2) row-half partials W_wr = Y(R_wr, :)^H V(R_wr, C_wc) (B operand = the lane's own
-        //     accumulators), and U = Y T (wave w: rows 16w .. 16w + 15, one 16 x 16 tile, K = 16)
-        {
-            lqm_d4 wr_[2], wi_[2];
+    // the reflectors Y(r, i) = v_{j0+i}(r) (zero above row j + 1) and the T factor of block b,
+    // held in registers one block ahead: thread tid holds Y entries e = tid + 256 u
+    cx<double> ypre[4], tpre;
+    auto fetch = [&](int b) {
+        const int j0 = b * kWyB;
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                wr_[b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
-                wi_[b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
-            }
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            const int r = e >> 4, j = j0 + (e & 15);
+            const cx<R> v = rf[(size_t)min(j, N - 1) * N + min(r, N - 1)];
+            ypre[u] = (j < nrefl && r > j && r < N) ? cx<double>{(double)v.re, (double)v.im} : cx<double>{0.0, 0.0};
+        }
+        tpre = Tg[((size_t)kk * nb + b) * (kWyB * kWyB) + tid];  // 256 threads, 256 entries
+    };
+    if (nb > 0) fetch(nb - 1);
+    for (int b0 = nb - 1; b0 >= 0; --b0) {
+        // (1) the block's reflectors and T factor into LDS; the next block's are fetched
+#pragma unroll
+        for (int u = 0; u < 4; ++u) S.Y[tid + 256 * u] = ypre[u];
+        S.T[tid] = tpre;
+        __syncthreads();
+        if (b0 > 0) fetch(b0 - 1);
+        // (2) row-half partials W_wr = Y(R_wr, :)^H V(R_wr, C_wc) (B operand = the lane's own
+        //     accumulators), and U = Y T (wave w: rows 16w .. 16w + 15, one 16 x 16 tile, K = 16);
+        //     complex products as three real ones (P1 = ar br, P2 = ai bi,
+        //     P3 = (ar + ai)(br + bi); re = P1 - P2, im = P3 - P1 - P2)
+        {
+            lqm_d4 p1[2], p2[2], p3[2];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) p1[b] = p2[b] = p3[b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     const int r = m0 + 16 * a + g + 4 * s;
                     const cx<double> y = S.Y[r * kWyB + li];  // A(m = li, k = r) = conj(Y(r, li))
+                    const double ar = y.re, ai = -y.im, as = y.re - y.im;
 #pragma unroll
                     for (int b = 0; b < 2; ++b) {
                         const double br = vr[a][b][s], bi = vi[a][b][s];
-                        wr_[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, br, wr_[b], 0, 0, 0);
-                        wi_[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, bi, wi_[b], 0, 0, 0);
-                        wr_[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(y.im, bi, wr_[b], 0, 0, 0);
-                        wi_[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-y.im, br, wi_[b], 0, 0, 0);
+                        p1[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, p1[b], 0, 0, 0);
+                        p2[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, p2[b], 0, 0, 0);
+                        p3[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(as, br + bi, p3[b], 0, 0, 0);
                     }
                 }
 #pragma unroll
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
-                    S.W[wr][(n0 + 16 * b + li) * (kWyB + 1) + g + 4 * e] = cx<double>{wr_[b][e], wi_[b][e]};
-            lqm_d4 ur = {0.0, 0.0, 0.0, 0.0}, ui = {0.0, 0.0, 0.0, 0.0};
+                    S.W[wr][(n0 + 16 * b + li) * (kWyB + 1) + g + 4 * e] =
+                        cx<double>{p1[b][e] - p2[b][e], p3[b][e] - p1[b][e] - p2[b][e]};
+            lqm_d4 q1 = {0.0, 0.0, 0.0, 0.0}, q2 = q1, q3 = q1;
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int l = 4 * g + s;
                 const cx<double> y = S.Y[(16 * wave + li) * kWyB + l];  // A(m = li, k = l)
                 const cx<double> t = S.T[l * kWyB + li];                // B(k = l, n = li) = T(l, li)
-                ur = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, t.re, ur, 0, 0, 0);
-                ui = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, t.im, ui, 0, 0, 0);
-                ur = __builtin_amdgcn_mfma_f64_16x16x4f64(-y.im, t.im, ur, 0, 0, 0);
-                ui = __builtin_amdgcn_mfma_f64_16x16x4f64(y.im, t.re, ui, 0, 0, 0);
+                q1 = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, t.re, q1, 0, 0, 0);
+                q2 = __builtin_amdgcn_mfma_f64_16x16x4f64(y.im, t.im, q2, 0, 0, 0);
+                q3 = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re + y.im, t.re + t.im, q3, 0, 0, 0);
             }
 #pragma unroll
-            for (int e = 0; e < 4; ++e) S.U[(16 * wave + g + 4 * e) * kWyB + li] = cx<double>{ur[e], ui[e]};
+            for (int e = 0; e < 4; ++e)
+                S.U[(16 * wave + g + 4 * e) * kWyB + li] = cx<double>{q1[e] - q2[e], q3[e] - q1[e] - q2[e]};
         }
         __syncthreads();
-        // (3) V <- V - U W, W = W_0 + W_1, on the wave's quadrant (accumulators updated in place)
+        // (3) V <- V - U W, W = W_0 + W_1, on the wave's quadrant
+        {
+            lqm_d4 p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int i = 4 * g + s;
-            double ur[2], ui[2], wre[2], wim[2];
+            for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const cx<double> u = S.U[(m0 + 16 * t + li) * kWyB + i];  // A(m, k = i) = -U
-                const cx<double> w = S.W[0][(n0 + 16 * t + li) * (kWyB + 1) + i] +
-                                      S.W[1][(n0 + 16 * t + li) * (kWyB + 1) + i];
-                ur[t] = -u.re;
-                ui[t] = -u.im;
-                wre[t] = w.re;
-                wim[t] = w.im;
+                for (int b = 0; b < 2; ++b) p1[a][b] = p2[a][b] = p3[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int i = 4 * g + s;
+                double ur[2], ui[2], us[2], wre[2], wim[2], ws[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const cx<double> u = S.U[(m0 + 16 * t + li) * kWyB + i];  // A(m, k = i) = -U
+                    const cx<double> w = S.W[0][(n0 + 16 * t + li) * (kWyB + 1) + i] +
+                                          S.W[1][(n0 + 16 * t + li) * (kWyB + 1) + i];
+                    ur[t] = -u.re;
+                    ui[t] = -u.im;
+                    us[t] = ur[t] + ui[t];
+                    wre[t] = w.re;
+                    wim[t] = w.im;
+                    ws[t] = w.re + w.im;
+                }
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        p1[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[a], wre[b], p1[a][b], 0, 0, 0);
+                        p2[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[a], wim[b], p2[a][b], 0, 0, 0);
+                        p3[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(us[a], ws[b], p3[a][b], 0, 0, 0);
+                    }
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    vr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[a], wre[b], vr[a][b], 0, 0, 0);
-                    vi[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[a], wim[b], vi[a][b], 0, 0, 0);
-                }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    vr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ui[a], wim[b], vr[a][b], 0, 0, 0);
-                    vi[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[a], wre[b], vi[a][b], 0, 0, 0);
+                    vr[a][b] += p1[a][b] - p2[a][b];
+                    vi[a][b] += p3[a][b] - p1[a][b] - p2[a][b];
+                    // keeps the update here rather than sunk into the next block, where the
+                    // three products would stay live beside V
+                    asm volatile("" : "+v"(vr[a][b]), "+v"(vi[a][b]));
                 }
         }
         __syncthreads();  // Y, U, W are rewritten by the next block
@@ -270,46 +287,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         if (tid == 0) eb[kk] = (kw ? kw[kk] : wuni) * (S.red[0] + S.red[1] + S.red[2] + S.red[3]);
         // columns whose occupation enters the product (the rest are below kOccNegligible)
         const unsigned long long live = __ballot(!(fabs(S.sf[lane]) <= kOccNegligible));
-        lqm_d4 cr[2][2], ci[2][2];
+        lqm_d4 p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                cr[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
-                ci[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
-            }
+            for (int b = 0; b < 2; ++b) p1[a][b] = p2[a][b] = p3[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
         for (int k0 = 0; k0 < 64; k0 += 16) {
             if (((live >> k0) & 0xFFFFull) == 0) continue;  // 16 columns of negligible occupation
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int c = k0 + 4 * g + s;
                 const double f = S.sf[c];
-                double ar[2], ai[2], br[2], bi[2];
+                double ar[2], ai[2], as[2], br[2], bi[2], bs[2];
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
                     const cx<double> x = Vs[c * kWyLd + m0 + 16 * t + li];  // A(m, k = c) = V f
                     const cx<double> y = Vs[c * kWyLd + n0 + 16 * t + li];  // B(k = c, n) = conj V(n, c)
                     ar[t] = f * x.re;
                     ai[t] = f * x.im;
+                    as[t] = ar[t] + ai[t];
                     br[t] = y.re;
                     bi[t] = -y.im;
+                    bs[t] = y.re - y.im;
                 }
 #pragma unroll
                 for (int a = 0; a < 2; ++a)
 #pragma unroll
                     for (int b = 0; b < 2; ++b) {
-                        cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
-                        ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
-                    }
-#pragma unroll
-                for (int a = 0; a < 2; ++a)
-#pragma unroll
-                    for (int b = 0; b < 2; ++b) {
-                        cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[b], cr[a][b], 0, 0, 0);
-                        ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+                        p1[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], p1[a][b], 0, 0, 0);
+                        p2[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bi[b], p2[a][b], 0, 0, 0);
+                        p3[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[a], bs[b], p3[a][b], 0, 0, 0);
                     }
             }
         }
+        lqm_d4 cr[2][2], ci[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                cr[a][b] = p1[a][b] - p2[a][b];
+                ci[a][b] = p3[a][b] - p1[a][b] - p2[a][b];
+            }
         cx<double>* o = out + (size_t)kk * NN;
 #pragma unroll
         for (int a = 0; a < 2; ++a)
