@@ -57,6 +57,20 @@ __device__ __forceinline__ cx<R> cmadcb(cx<R> acc, cx<R> a, cx<R> b) {
     acc.im = fma(-a.re, b.im, acc.im);
     return acc;
 }
+// Reciprocal from the hardware approximation: FP32 as is (1 ulp), FP64 with one Newton step
+// (for normal, finite x).
+template <typename R>
+__device__ __forceinline__ R rcp_nr(R x);
+template <>
+__device__ __forceinline__ float rcp_nr(float x) {
+    return __builtin_amdgcn_rcpf(x);
+}
+template <>
+__device__ __forceinline__ double rcp_nr(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
 // 1/z
 template <typename R>
 __device__ __forceinline__ cx<R> crecip(cx<R> z) {

@@ -77,8 +77,11 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
     const int j = 4 * jt + JV;
     if (j + 1 >= N) return;
     const int buf = j & 1;
-    // (a) Householder vector of column j, by the 16 lanes of tile column jt
+    // (a) Householder vector of column j, by the 16 lanes of tile column jt.  Every other
+    //     wave waits for it at the barrier, so its wave is issued ahead of the waves of other
+    //     blocks on the same SIMD while it runs (s_setprio).
     if (tj == jt) {
+        __builtin_amdgcn_s_setprio(3);
         R xn2 = 0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -97,8 +100,19 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
         if (!(xn2 == (R)0 && alpha.im == (R)0)) {
             const R nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
             beta = alpha.re >= (R)0 ? -nrm : nrm;
-            tj_ = mkc<R>((beta - alpha.re) / beta, -alpha.im / beta);
-            scale = crecip(mkc<R>(alpha.re - beta, alpha.im));
+            // reciprocals from the hardware approximation while |alpha - beta|^2 (<= 4 beta^2,
+            // >= beta^2) sits in the normal range of R, IEEE division below it (wave-uniform)
+            const R zr = alpha.re - beta, den = zr * zr + alpha.im * alpha.im;
+            R rb, rz;
+            if (den >= num<R>::rot_lo) {
+                rb = rcp_nr(beta);
+                rz = rcp_nr(den);
+            } else {
+                rb = (R)1 / beta;
+                rz = (R)1 / den;
+            }
+            tj_ = mkc<R>((beta - alpha.re) * rb, -alpha.im * rb);
+            scale = mkc<R>(zr * rz, -alpha.im * rz);
         }
         const bool triv = (tj_.re == (R)0 && tj_.im == (R)0);
 #pragma unroll
@@ -116,38 +130,44 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
             tau[(size_t)kk * N + j] = tj_;
             ee[(size_t)j * Nkc + kk] = beta;
         }
+        __builtin_amdgcn_s_setprio(0);
     }
     __syncthreads();
     const cx<R> t = S.stau[buf];
     if (t.re == (R)0 && t.im == (R)0) return;  // H_j = I (uniform branch)
 
-    // (b) y_c = sum_r conj(A_rc) v_r for this thread's columns, summed over the lane row
+    // (b) y_c = sum_r conj(A_rc) v_r for this thread's columns, summed over the lane row.
+    //     A wave whose four tile columns all lie at or left of column j has no part in the
+    //     rest of the step (wave-uniform test): it only posts a zero partial of v^H A22 v.
+    const bool wact = 16 * (tj >> 2) + 15 > j;
     cx<R> vr[4], vc[4], p[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) vr[u] = S.sv[buf][5 * ti + u];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) vc[v] = S.sv[buf][5 * tj + v];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        cx<R> y = mkc<R>(0, 0);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) y = cmadc(y, a[u][v], vr[u]);
-        y.re = row16_sum(y.re);
-        y.im = row16_sum(y.im);
-        p[v] = y;
-    }
     R spart = 0;
+    if (wact) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int c = 4 * tj + v;
-        if (c > j && c < N) spart += p[v].re * vc[v].re + p[v].im * vc[v].im;  // Re(conj(y_c) v_c)
-        p[v] = t * p[v];
-    }
-    if (ti == 0) {
+        for (int u = 0; u < 4; ++u) vr[u] = S.sv[buf][5 * ti + u];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) S.sp[buf][5 * tj + v] = p[v];
-        S.ss[buf][tj] = spart;
+        for (int v = 0; v < 4; ++v) vc[v] = S.sv[buf][5 * tj + v];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            cx<R> y = mkc<R>(0, 0);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) y = cmadc(y, a[u][v], vr[u]);
+            y.re = row16_sum(y.re);
+            y.im = row16_sum(y.im);
+            p[v] = y;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = 4 * tj + v;
+            if (c > j && c < N) spart += p[v].re * vc[v].re + p[v].im * vc[v].im;  // Re(conj(y_c) v_c)
+            p[v] = t * p[v];
+        }
+        if (ti == 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) S.sp[buf][5 * tj + v] = p[v];
+        }
     }
+    if (ti == 0) S.ss[buf][tj] = spart;
     __syncthreads();
 
     // (c) A22 -= v w^H + w v^H, w = p + alpha2 v, alpha2 = -1/2 |tau|^2 v^H A22 v
@@ -155,7 +175,7 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
 #pragma unroll
     for (int q = 0; q < 16; ++q) sum += S.ss[buf][q];
     const R alpha2 = (R)-0.5 * (t.re * t.re + t.im * t.im) * sum;
-    if (4 * ti + 3 > j && 4 * tj + 3 > j) {
+    if (wact && 4 * ti + 3 > j && 4 * tj + 3 > j) {
         cx<R> wr[4], wc[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {

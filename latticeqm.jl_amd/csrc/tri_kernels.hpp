@@ -18,19 +18,6 @@
 
 namespace lqm {
 
-// Reciprocal from the hardware approximation: FP32 as is (1 ulp), FP64 with one Newton step.
-template <typename R>
-__device__ __forceinline__ R tri_rcp(R x);
-template <>
-__device__ __forceinline__ float tri_rcp(float x) {
-    return __builtin_amdgcn_rcpf(x);
-}
-template <>
-__device__ __forceinline__ double tri_rcp(double x) {
-    const double r = __builtin_amdgcn_rcp(x);
-    return fma(r, fma(-x, r, 1.0), r);
-}
-
 template <typename R>
 __device__ __forceinline__ bool tri_finite(R x) {
     return x - x == R(0);
@@ -109,7 +96,7 @@ __global__ __launch_bounds__(256) void k_tri_bisect(int N, int Nkc, const R* __r
             bad = true;
         }
         int cnt = q < R(0);
-        R r = tri_rcp(q);
+        R r = rcp_nr(q);
         R qp = R(-1), S = -r;
         for (int i = 1; i < N; ++i) {
             const R t = se2[w][i - 1] * r;
@@ -119,7 +106,7 @@ __global__ __launch_bounds__(256) void k_tri_bisect(int N, int Nkc, const R* __r
                 bad = true;
             }
             qp = fma(t * r, qp, R(-1));
-            r = tri_rcp(qn);
+            r = rcp_nr(qn);
             S = fma(qp, r, S);
             cnt += qn < R(0);
         }
@@ -192,13 +179,13 @@ __global__ __launch_bounds__(256) void k_tri_twist(int N, int Nkc, const R* __re
     const R lam = mine ? evals[(size_t)kk * N + lane] : R(0);
     R z[NM];
     // forward pivots: z[i] = 1 / D+_i
-    R rp = tri_rcp(guard(sd[w][0] - lam));
+    R rp = rcp_nr(guard(sd[w][0] - lam));
     z[0] = rp;
 #pragma unroll
     for (int i = 1; i < NM; ++i) {
         z[i] = R(0);
         if (i < N) {
-            rp = tri_rcp(guard((sd[w][i] - lam) - se2[w][i - 1] * rp));
+            rp = rcp_nr(guard((sd[w][i] - lam) - se2[w][i - 1] * rp));
             z[i] = rp;
         }
     }
@@ -216,7 +203,7 @@ __global__ __launch_bounds__(256) void k_tri_twist(int N, int Nkc, const R* __re
                 gbest = fabs(gam);
                 rr = i;
             }
-            rm = tri_rcp(guard(i + 1 < N ? dl - se2[w][i] * rm : dl));
+            rm = rcp_nr(guard(i + 1 < N ? dl - se2[w][i] * rm : dl));
         }
     // 1 / D-_i above the twist
     rm = R(0);
@@ -224,7 +211,7 @@ __global__ __launch_bounds__(256) void k_tri_twist(int N, int Nkc, const R* __re
     for (int i = NM - 1; i >= 1; --i)
         if (i < N) {
             const R dl = sd[w][i] - lam;
-            rm = tri_rcp(guard(i + 1 < N ? dl - se2[w][i] * rm : dl));
+            rm = rcp_nr(guard(i + 1 < N ? dl - se2[w][i] * rm : dl));
             if (i > rr) z[i] = rm;
         }
     // the vector, outward from the twist
