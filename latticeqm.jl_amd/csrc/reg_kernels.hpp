@@ -2,7 +2,10 @@
 // k-point).
 //
 // The N x N Hermitian matrix is held entirely in registers: thread (ti, tj) (ti =
-// tid & 15, tj = tid >> 4) owns the 4 x 4 tile A[4ti + u][4tj + v], both triangles.
+// tid & 15, tj = tid >> 4) owns the T x T tile A[off + T ti + u][off + T tj + v] of the
+// trailing matrix from row/column off, both triangles.  T starts at the smallest size with
+// 16 T >= N and drops by one whenever the trailing matrix fits the next smaller size
+// (N - j <= 48, 32, 16), so the lanes stay busy with live entries as the matrix shrinks.
 // The 16 threads of one tile column tj are one 16-lane row of a wave, so every
 // reduction over rows is a lane butterfly inside that row; LDS only carries the
 // Householder vector, p = tau A v and 16 partial dot products per step (double
@@ -56,26 +59,30 @@ __device__ __forceinline__ R row16_sum(R v) {
 }
 
 // Shared memory of k_tridiag_r (double-buffered per step parity).  The vectors are read
-// four entries per lane (rows 4 ti .. 4 ti + 3), so entry r sits at rpad(r) = r + r/4: one
-// spare slot per four entries spreads the 16 lanes of a row over distinct banks.
+// T entries per lane (rows off + T ti .. off + T ti + T - 1), so entry r sits at
+// rpad(r) = r + r/4: one spare slot per four entries spreads the 16 lanes of a row over
+// distinct banks.  L carries the trailing matrix from one tile size to the next.
 __device__ __forceinline__ int rpad(int r) { return r + (r >> 2); }
+constexpr int kRegLd = 49;
 template <typename R>
 struct RegTriSmem {
     cx<R> sv[2][80];  // Householder vector v (zero above row j+1), entry r at rpad(r)
     cx<R> sp[2][80];  // p = tau A22 v, entry r at rpad(r)
     R ss[2][16];      // per tile column partial of v^H A22 v
     cx<R> stau[2];
+    cx<R> L[48 * kRegLd];  // trailing matrix (<= 48 x 48) between tile sizes, column-major
 };
 
-// One Householder step j = 4 jt + JV.  JV is a compile-time constant so that every
-// register of the tile is addressed statically (no private-memory array).
-template <typename R, int JV>
-__device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, int N, int Nkc, int kk,
-                                            int jt, int ti, int tj, cx<R>* __restrict__ rf,
-                                            R* __restrict__ dd, R* __restrict__ ee,
+// One Householder step j = off + T jt + JV on T x T tiles (thread (ti, tj) owns rows
+// off + T ti + u and columns off + T tj + v of the trailing matrix).  T and JV are
+// compile-time constants so that every register of the tile is addressed statically.
+template <typename R, int T, int JV>
+__device__ __forceinline__ void regtri_step(cx<R> (&a)[T][T], RegTriSmem<R>& S, int N, int Nkc, int kk,
+                                            int off, int jend, int jt, int ti, int tj,
+                                            cx<R>* __restrict__ rf, R* __restrict__ dd, R* __restrict__ ee,
                                             cx<R>* __restrict__ tau) {
-    const int j = 4 * jt + JV;
-    if (j + 1 >= N) return;
+    const int j = off + T * jt + JV;
+    if (j + 1 >= N || j >= jend) return;
     const int buf = j & 1;
     // (a) Householder vector of column j, by the 16 lanes of tile column jt.  Every other
     //     wave waits for it at the barrier, so its wave is issued ahead of the waves of other
@@ -84,16 +91,16 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
         __builtin_amdgcn_s_setprio(3);
         R xn2 = 0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int r = 4 * ti + u;
+        for (int u = 0; u < T; ++u) {
+            const int r = off + T * ti + u;
             if (r >= j + 2 && r < N) xn2 += abs2(a[u][JV]);
         }
         // the row of lanes is wave-uniform here (tile column jt), so one lane's sum and
         // the owner of A[j+1][j] are read back with readlane: bitwise the same everywhere
         const int base = 16 * (jt & 3);
         xn2 = rl(row16_sum(xn2), base);
-        constexpr int UA = (JV + 1) & 3;  // row j+1 is row UA of tile row (j+1)>>2
-        const int src = base + ((j + 1) >> 2);
+        constexpr int UA = (JV + 1) % T;  // row j+1 is row UA of tile row (j+1-off)/T
+        const int src = base + (j + 1 - off) / T;
         const cx<R> alpha = mkc<R>(rl(a[UA][JV].re, src), rl(a[UA][JV].im, src));
         cx<R> tj_ = mkc<R>(0, 0), scale = mkc<R>(0, 0);
         R beta = alpha.re;
@@ -116,8 +123,8 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
         }
         const bool triv = (tj_.re == (R)0 && tj_.im == (R)0);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int r = 4 * ti + u;
+        for (int u = 0; u < T; ++u) {
+            const int r = off + T * ti + u;
             cx<R> vr = mkc<R>(0, 0);
             if (r == j + 1) vr = mkc<R>(1, 0);
             else if (r > j + 1 && r < N && !triv) vr = a[u][JV] * scale;
@@ -139,32 +146,32 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
     // (b) y_c = sum_r conj(A_rc) v_r for this thread's columns, summed over the lane row.
     //     A wave whose four tile columns all lie at or left of column j has no part in the
     //     rest of the step (wave-uniform test): it only posts a zero partial of v^H A22 v.
-    const bool wact = 16 * (tj >> 2) + 15 > j;
-    cx<R> vr[4], vc[4], p[4];
+    const bool wact = off + T * (4 * (tj >> 2) + 4) - 1 > j;
+    cx<R> vr[T], vc[T], p[T];
     R spart = 0;
     if (wact) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) vr[u] = S.sv[buf][5 * ti + u];
+        for (int u = 0; u < T; ++u) vr[u] = S.sv[buf][rpad(off + T * ti + u)];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) vc[v] = S.sv[buf][5 * tj + v];
+        for (int v = 0; v < T; ++v) vc[v] = S.sv[buf][rpad(off + T * tj + v)];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
+        for (int v = 0; v < T; ++v) {
             cx<R> y = mkc<R>(0, 0);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) y = cmadc(y, a[u][v], vr[u]);
+            for (int u = 0; u < T; ++u) y = cmadc(y, a[u][v], vr[u]);
             y.re = row16_sum(y.re);
             y.im = row16_sum(y.im);
             p[v] = y;
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int c = 4 * tj + v;
+        for (int v = 0; v < T; ++v) {
+            const int c = off + T * tj + v;
             if (c > j && c < N) spart += p[v].re * vc[v].re + p[v].im * vc[v].im;  // Re(conj(y_c) v_c)
             p[v] = t * p[v];
         }
         if (ti == 0) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) S.sp[buf][5 * tj + v] = p[v];
+            for (int v = 0; v < T; ++v) S.sp[buf][rpad(off + T * tj + v)] = p[v];
         }
     }
     if (ti == 0) S.ss[buf][tj] = spart;
@@ -175,23 +182,23 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
 #pragma unroll
     for (int q = 0; q < 16; ++q) sum += S.ss[buf][q];
     const R alpha2 = (R)-0.5 * (t.re * t.re + t.im * t.im) * sum;
-    if (wact && 4 * ti + 3 > j && 4 * tj + 3 > j) {
-        cx<R> wr[4], wc[4];
+    if (wact && off + T * ti + T - 1 > j && off + T * tj + T - 1 > j) {
+        cx<R> wr[T], wc[T];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int r = 4 * ti + u;
-            const cx<R> pr = S.sp[buf][5 * ti + u];
+        for (int u = 0; u < T; ++u) {
+            const int r = off + T * ti + u;
+            const cx<R> pr = S.sp[buf][rpad(r)];
             wr[u] = (r > j && r < N) ? cmad(pr, mkc<R>(alpha2, 0), vr[u]) : mkc<R>(0, 0);
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int c = 4 * tj + v;
+        for (int v = 0; v < T; ++v) {
+            const int c = off + T * tj + v;
             wc[v] = (c > j && c < N) ? cmad(p[v], mkc<R>(alpha2, 0), vc[v]) : mkc<R>(0, 0);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < T; ++u)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
+            for (int v = 0; v < T; ++v) {
                 cx<R> d = mkc<R>(0, 0);
                 d = cmadcb(d, vr[u], wc[v]);
                 d = cmadcb(d, wr[u], vc[v]);
@@ -200,8 +207,73 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[4][4], RegTriSmem<R>& S, 
     }
 }
 
+// The trailing matrix from row/column off on T x T tiles: steps j in [off, jend), then the
+// part from row/column off' = max(0, N - 16 (T - 1)) moves through LDS onto (T-1) x (T-1)
+// tiles for the steps that follow (the active part never needs more tiles than it has rows).
+template <typename R, int T>
+__device__ __forceinline__ void regtri_run(cx<R> (&a)[T][T], RegTriSmem<R>& S, int N, int Nkc, int kk,
+                                           int off, int ti, int tj, cx<R>* __restrict__ rf,
+                                           R* __restrict__ dd, R* __restrict__ ee, cx<R>* __restrict__ tau) {
+    const int offn = T > 1 ? max(0, N - 16 * (T - 1)) : N;
+    const int jend = T > 1 ? offn : N - 1;
+    for (int jt = 0; off + T * jt < jend; ++jt) {
+        regtri_step<R, T, 0>(a, S, N, Nkc, kk, off, jend, jt, ti, tj, rf, dd, ee, tau);
+        if constexpr (T > 1) regtri_step<R, T, 1>(a, S, N, Nkc, kk, off, jend, jt, ti, tj, rf, dd, ee, tau);
+        if constexpr (T > 2) regtri_step<R, T, 2>(a, S, N, Nkc, kk, off, jend, jt, ti, tj, rf, dd, ee, tau);
+        if constexpr (T > 3) regtri_step<R, T, 3>(a, S, N, Nkc, kk, off, jend, jt, ti, tj, rf, dd, ee, tau);
+    }
+    if constexpr (T > 1) {
+        // rows/columns [offn, N) of the trailing matrix into L (column-major, offset offn)
+#pragma unroll
+        for (int u = 0; u < T; ++u)
+#pragma unroll
+            for (int v = 0; v < T; ++v) {
+                const int r = off + T * ti + u, c = off + T * tj + v;
+                if (r >= offn && r < N && c >= offn && c < N) S.L[(c - offn) * kRegLd + (r - offn)] = a[u][v];
+            }
+        __syncthreads();
+        constexpr int T1 = T - 1;
+        cx<R> b[T1][T1];
+#pragma unroll
+        for (int u = 0; u < T1; ++u)
+#pragma unroll
+            for (int v = 0; v < T1; ++v) {
+                const int r = offn + T1 * ti + u, c = offn + T1 * tj + v;
+                b[u][v] = (r < N && c < N) ? S.L[(c - offn) * kRegLd + (r - offn)] : mkc<R>(0, 0);
+            }
+        regtri_run<R, T1>(b, S, N, Nkc, kk, offn, ti, tj, rf, dd, ee, tau);
+    } else {
+        // last diagonal element
+        const int n1 = N - 1;
+        if (off + ti == n1 && off + tj == n1) {
+            dd[(size_t)n1 * Nkc + kk] = a[0][0].re;
+            ee[(size_t)n1 * Nkc + kk] = (R)0;
+        }
+    }
+}
+
+template <typename R, int T>
+__device__ __forceinline__ void regtri_start(const cx<double>* __restrict__ H, RegTriSmem<R>& S, int N, int Nkc,
+                                             int kk, int ti, int tj, cx<R>* __restrict__ rf,
+                                             R* __restrict__ dd, R* __restrict__ ee, cx<R>* __restrict__ tau) {
+    cx<R> a[T][T];
+#pragma unroll
+    for (int u = 0; u < T; ++u)
+#pragma unroll
+        for (int v = 0; v < T; ++v) {
+            const int r = T * ti + u, c = T * tj + v;
+            cx<R> x = mkc<R>(0, 0);
+            if (r < N && c < N) {
+                const cx<double> h = H[(size_t)c * N + r];
+                x = mkc<R>((R)h.re, (R)h.im);
+            }
+            a[u][v] = x;
+        }
+    regtri_run<R, T>(a, S, N, Nkc, kk, 0, ti, tj, rf, dd, ee, tau);
+}
+
 template <typename R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) void k_tridiag_r(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, sizeof(R) == 4 ? 4 : 3))) void k_tridiag_r(
     int N, int Nkc, const cx<double>* __restrict__ Hk, R* __restrict__ dd, R* __restrict__ ee,
     cx<R>* __restrict__ refl, cx<R>* __restrict__ tau) {
     __shared__ RegTriSmem<R> S;
@@ -210,39 +282,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
     const int kk = blockIdx.x;
     const size_t NN = (size_t)N * N;
     const cx<double>* H = Hk + (size_t)kk * NN;
-
-    cx<R> a[4][4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int r = 4 * ti + u, c = 4 * tj + v;
-            cx<R> x = mkc<R>(0, 0);
-            if (r < N && c < N) {
-                const cx<double> h = H[(size_t)c * N + r];
-                x = mkc<R>((R)h.re, (R)h.im);
-            }
-            a[u][v] = x;
-        }
-
     cx<R>* rf = refl + (size_t)kk * NN;
-    const int ntiles = (N + 2) / 4;  // steps j = 0 .. N-2 in groups of four
-    for (int jt = 0; jt < ntiles; ++jt) {
-        regtri_step<R, 0>(a, S, N, Nkc, kk, jt, ti, tj, rf, dd, ee, tau);
-        regtri_step<R, 1>(a, S, N, Nkc, kk, jt, ti, tj, rf, dd, ee, tau);
-        regtri_step<R, 2>(a, S, N, Nkc, kk, jt, ti, tj, rf, dd, ee, tau);
-        regtri_step<R, 3>(a, S, N, Nkc, kk, jt, ti, tj, rf, dd, ee, tau);
-    }
-    // last diagonal element
-    const int n1 = N - 1;
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-            if (4 * ti + u == n1 && 4 * tj + v == n1) {
-                dd[(size_t)n1 * Nkc + kk] = a[u][v].re;
-                ee[(size_t)n1 * Nkc + kk] = (R)0;
-            }
+    // the smallest tile holding the whole matrix: 16 T >= N (N <= 64)
+    if (N > 48) regtri_start<R, 4>(H, S, N, Nkc, kk, ti, tj, rf, dd, ee, tau);
+    else if (N > 32) regtri_start<R, 3>(H, S, N, Nkc, kk, ti, tj, rf, dd, ee, tau);
+    else if (N > 16) regtri_start<R, 2>(H, S, N, Nkc, kk, ti, tj, rf, dd, ee, tau);
+    else regtri_start<R, 1>(H, S, N, Nkc, kk, ti, tj, rf, dd, ee, tau);
 }
 
 }  // namespace lqm
