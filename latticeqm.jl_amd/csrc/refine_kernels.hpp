@@ -478,6 +478,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
 // Occupations of magnitude at or below 2^-70 are dropped from rho_k = X f X^H, in chunks of
 // 16 bands: each element of rho_k then moves by at most 64 * 2^-70 < 6e-20, far below the
 // rounding of its FP64 sum (occupations given by the caller may be negative; NaN is kept).
+// The reference forms U Diagonal(f) U' over every band (Operators/densitymatrix.jl:139-140).
 constexpr double kOccNegligible = 0x1p-70;
 
 __device__ __forceinline__ double fermi_occ(double l, double mu, double T) {
@@ -1184,7 +1185,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                     const double gap = ln - lam[m];
                     if (fabs(gap) > delta) {
                         const double tre = sr[t][i] + ln * rre, tim = si[t][i] + ln * rim;
-                        const double rg = 1.0 / gap;
+                        const double rg = fabs(gap) >= num<double>::rot_lo ? rcp_nr(gap) : 1.0 / gap;
                         ere = tre * rg;
                         eim = tim * rg;
                         cpart += (tre * tre + tim * tim) * rg;
