@@ -41,28 +41,30 @@ struct WySmem {
 // on the reflectors only): one wave per (block, k-point) forms P = Y^H Y on the matrix
 // cores (K = the 64 rows, lane (li, g) supplying row 4 step + g of column li as both
 // operands) and runs the zlarft recurrence T(t, i) = -tau_i sum_{l=t}^{i-1} T(t, l) P(l, i)
-// with lane t holding row t.  Out: Tg[(k * nb + b) * 256 + row * 16 + col] in FP64.
+// with lane t holding row t in registers.  Out: Tg[(k * nb + b) * 256 + row * 16 + col] in
+// FP64.
 template <typename R>
-__global__ __launch_bounds__(64) void k_wy_tfac(int N, const cx<R>* __restrict__ refl,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_wy_tfac(int N, const cx<R>* __restrict__ refl,
                                                 const cx<R>* __restrict__ tau, cx<double>* __restrict__ Tg) {
     __shared__ cx<double> P[kWyB * kWyB];  // P(p, q) at q * 16 + p
-    __shared__ cx<double> Ts[kWyB * kWyB];
     const int lane = threadIdx.x, li = lane & 15, g = lane >> 4;
     const int b = blockIdx.x, nb = gridDim.x;
     const size_t kk = blockIdx.y;
     const int j0 = b * kWyB, nrefl = N - 1;
     const size_t NN = (size_t)N * N;
     const cx<R>* rf = refl + kk * NN;
-    lqm_d4 pr = {0.0, 0.0, 0.0, 0.0}, pi = {0.0, 0.0, 0.0, 0.0};
     const int j = j0 + li;
-#pragma unroll 4
+    // the lane's 16 entries of Y, all loads in flight together (clamped addresses: every
+    // load is in bounds; entries outside the reflector are zeroed below)
+    cx<R> yv[16];
+#pragma unroll
+    for (int st = 0; st < 16; ++st) yv[st] = rf[(size_t)min(j, N - 1) * N + min(4 * st + g, N - 1)];
+    lqm_d4 pr = {0.0, 0.0, 0.0, 0.0}, pi = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
     for (int st = 0; st < 16; ++st) {
         const int r = 4 * st + g;
         cx<double> y = {0.0, 0.0};
-        if (j < nrefl && r > j && r < N) {
-            const cx<R> v = rf[(size_t)j * N + r];
-            y = {(double)v.re, (double)v.im};
-        }
+        if (j < nrefl && r > j && r < N) y = {(double)yv[st].re, (double)yv[st].im};
         // A(m = li, k = r) = conj(Y(r, li)), B(k = r, n = li) = Y(r, li)
         pr = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, y.re, pr, 0, 0, 0);
         pi = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, y.im, pi, 0, 0, 0);
@@ -73,11 +75,17 @@ __global__ __launch_bounds__(64) void k_wy_tfac(int N, const cx<R>* __restrict__
     for (int e = 0; e < 4; ++e) P[li * kWyB + g + 4 * e] = cx<double>{pr[e], pi[e]};
     __syncthreads();
     if (lane < kWyB) {
-        cx<double>* trow = Ts + lane * kWyB;
-#pragma unroll 1
+        // row `lane` of T in registers; T(t, l) = 0 for l < t, so every sum can start at l = 0
+        cx<double> trow[kWyB];
+#pragma unroll
         for (int i = 0; i < kWyB; ++i) {
-            cx<double> acc = {0.0, 0.0};
-            for (int l = lane; l < i; ++l) acc = cmad(acc, trow[l], P[i * kWyB + l]);
+            cx<double> a0 = {0.0, 0.0}, a1 = {0.0, 0.0};
+#pragma unroll
+            for (int l = 0; l < i; ++l) {
+                if (l & 1) a1 = cmad(a1, trow[l], P[i * kWyB + l]);
+                else a0 = cmad(a0, trow[l], P[i * kWyB + l]);
+            }
+            const cx<double> acc = a0 + a1;
             cx<double> ti = {0.0, 0.0};
             if (j0 + i < nrefl) {
                 const cx<R> x = tau[kk * N + j0 + i];
@@ -85,10 +93,10 @@ __global__ __launch_bounds__(64) void k_wy_tfac(int N, const cx<R>* __restrict__
             }
             trow[i] = lane < i ? cx<double>{0.0, 0.0} - ti * acc : (lane == i ? ti : cx<double>{0.0, 0.0});
         }
+        cx<double>* To = Tg + (kk * nb + b) * (size_t)(kWyB * kWyB) + lane * kWyB;
+#pragma unroll
+        for (int c = 0; c < kWyB; ++c) To[c] = trow[c];
     }
-    __syncthreads();
-    cx<double>* To = Tg + (kk * nb + b) * (size_t)(kWyB * kWyB);
-    for (int e = lane; e < kWyB * kWyB; e += 64) To[e] = Ts[e];
 }
 inline size_t wy_lds_bytes() {
     const size_t v = sizeof(cx<double>) * 64 * kWyLd;  // staged V of MODE 1 overlays Y, U, W
