@@ -256,18 +256,20 @@ template <typename R, int T>
 __device__ __forceinline__ void regtri_start(const cx<double>* __restrict__ H, RegTriSmem<R>& S, int N, int Nkc,
                                              int kk, int ti, int tj, cx<R>* __restrict__ rf,
                                              R* __restrict__ dd, R* __restrict__ ee, cx<R>* __restrict__ tau) {
+    // all T x T loads in flight together (clamped addresses: every load is in bounds; the
+    // padding rows and columns are zeroed after)
+    cx<double> h[T][T];
+#pragma unroll
+    for (int u = 0; u < T; ++u)
+#pragma unroll
+        for (int v = 0; v < T; ++v) h[u][v] = H[(size_t)min(T * tj + v, N - 1) * N + min(T * ti + u, N - 1)];
     cx<R> a[T][T];
 #pragma unroll
     for (int u = 0; u < T; ++u)
 #pragma unroll
         for (int v = 0; v < T; ++v) {
             const int r = T * ti + u, c = T * tj + v;
-            cx<R> x = mkc<R>(0, 0);
-            if (r < N && c < N) {
-                const cx<double> h = H[(size_t)c * N + r];
-                x = mkc<R>((R)h.re, (R)h.im);
-            }
-            a[u][v] = x;
+            a[u][v] = (r < N && c < N) ? mkc<R>((R)h[u][v].re, (R)h[u][v].im) : mkc<R>(0, 0);
         }
     regtri_run<R, T>(a, S, N, Nkc, kk, 0, ti, tj, rf, dd, ee, tau);
 }

@@ -125,6 +125,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     {
         const R* Z = Zg + (size_t)kk * NN;
         const bool zrow = zfmt && zfmt[kk];
+        // all 16 loads first (clamped addresses: every load is in bounds), then the padding zeroed
+        R zv[2][2][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int rc = min(m0 + 16 * a + g + 4 * e, N - 1), cc = min(n0 + 16 * b + li, N - 1);
+                    zv[a][b][e] = Z[zrow ? (size_t)rc * N + cc : (size_t)cc * N + rc];
+                }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -132,7 +143,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int r = m0 + 16 * a + g + 4 * e, c = n0 + 16 * b + li;
-                    vr[a][b][e] = (r < N && c < N) ? (double)Z[zrow ? (size_t)r * N + c : (size_t)c * N + r] : 0.0;
+                    vr[a][b][e] = (r < N && c < N) ? (double)zv[a][b][e] : 0.0;
                     vi[a][b][e] = 0.0;
                 }
     }
@@ -399,6 +410,17 @@ __global__ __launch_bounds__(256) void k_backtransform_wy32(int N, const cx<floa
     {
         const float* Z = Zg + (size_t)kk * NN;
         const bool zrow = zfmt && zfmt[kk];
+        // all 16 loads first (clamped addresses: every load is in bounds), then the padding zeroed
+        float zv[2][2][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int rc = min(m0 + 16 * a + 4 * g + e, N - 1), cc = min(n0 + 16 * b + li, N - 1);
+                    zv[a][b][e] = Z[zrow ? (size_t)rc * N + cc : (size_t)cc * N + rc];
+                }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -406,7 +428,7 @@ __global__ __launch_bounds__(256) void k_backtransform_wy32(int N, const cx<floa
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int r = m0 + 16 * a + 4 * g + e, c = n0 + 16 * b + li;
-                    vr[a][b][e] = (r < N && c < N) ? Z[zrow ? (size_t)r * N + c : (size_t)c * N + r] : 0.f;
+                    vr[a][b][e] = (r < N && c < N) ? zv[a][b][e] : 0.f;
                     vi[a][b][e] = 0.f;
                 }
     }
