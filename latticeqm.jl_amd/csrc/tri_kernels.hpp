@@ -83,10 +83,53 @@ __global__ __launch_bounds__(256) void k_tri_bisect(int N, int Nkc, const R* __r
     const R pivmin = num<R>::tiny * fmax(R(1), e2m);
     const R atol = eps * bnorm, ntol = R(4) * eps * bnorm;
     if (lane == 0) tnorm[kk] = bnorm;
-    if (lane >= N) return;
-    const int n = lane;
     R a = lo - R(2) * eps * bnorm - R(2) * pivmin, b = hi + R(2) * eps * bnorm + R(2) * pivmin;
     int ca = 0, cb = N;
+    {
+        // one Sturm pass at 64 points x_l = a + (l + 1) h, h = (b - a) / 65, one per lane (every
+        // lane of the wave, including lanes >= N): each eigenvalue's bracket narrows 65-fold
+        // at the cost of one bisection step
+        const R h = (b - a) / R(65);
+        const R xg = a + R(lane + 1) * h;
+        R q = sd[w][0] - xg;
+        if (fabs(q) <= pivmin) q = -pivmin;
+        int cg = q < R(0);
+        R r = rcp_nr(q);
+        for (int i = 1; i < N; ++i) {
+            R qn = (sd[w][i] - xg) - se2[w][i - 1] * r;
+            if (fabs(qn) <= pivmin) qn = -pivmin;
+            r = rcp_nr(qn);
+            cg += qn < R(0);
+        }
+        // lane n: the first grid point whose count exceeds n (lower-bound search; 64 = none)
+        int l0 = 0, l1 = 64;
+#pragma unroll
+        for (int s = 0; s < 7; ++s) {
+            const int mid = (l0 + l1) >> 1;
+            const int cm = __shfl(cg, min(mid, 63));
+            if (l0 < l1) {
+                if (cm > lane) l1 = mid;
+                else l0 = mid + 1;
+            }
+        }
+        const int c1 = __shfl(cg, max(l0 - 1, 0)), c2 = __shfl(cg, min(l0, 63));
+        const int na = l0 == 0 ? 0 : c1, nb = l0 == 64 ? N : c2;
+        // kept only if it is a consistent bracket (ca <= n < cb) of eigenvalue n; its ends are
+        // the grid points, recomputed with the same expression as their lanes used
+        if (na <= lane && lane < nb) {
+            const R a0 = a;
+            if (l0 > 0) {
+                ca = na;
+                a = a0 + R(l0) * h;
+            }
+            if (l0 < 64) {
+                cb = nb;
+                b = a0 + R(l0 + 1) * h;
+            }
+        }
+    }
+    if (lane >= N) return;
+    const int n = lane;
     R x = R(0.5) * (a + b);
     for (int it = 0; it < 256; ++it) {
         R q = sd[w][0] - x;
