@@ -59,20 +59,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     cx<R> yv[16];
 #pragma unroll
     for (int st = 0; st < 16; ++st) yv[st] = rf[(size_t)min(j, N - 1) * N + min(4 * st + g, N - 1)];
-    lqm_d4 pr = {0.0, 0.0, 0.0, 0.0}, pi = {0.0, 0.0, 0.0, 0.0};
+    // P = Y^H Y as three real products (P1 = ar br, P2 = ai bi, P3 = (ar + ai)(br + bi) with
+    // a = conj y, b = y; re = P1 - P2, im = P3 - P1 - P2); rows below j0 + 1 are zero in every
+    // column of the block, so the K-steps st < j0 / 4 are skipped (block-uniform)
+    lqm_d4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
+    const int st0 = j0 >> 2;
 #pragma unroll
     for (int st = 0; st < 16; ++st) {
+        if (st < st0) continue;
         const int r = 4 * st + g;
         cx<double> y = {0.0, 0.0};
         if (j < nrefl && r > j && r < N) y = {(double)yv[st].re, (double)yv[st].im};
         // A(m = li, k = r) = conj(Y(r, li)), B(k = r, n = li) = Y(r, li)
-        pr = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, y.re, pr, 0, 0, 0);
-        pi = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, y.im, pi, 0, 0, 0);
-        pr = __builtin_amdgcn_mfma_f64_16x16x4f64(y.im, y.im, pr, 0, 0, 0);
-        pi = __builtin_amdgcn_mfma_f64_16x16x4f64(-y.im, y.re, pi, 0, 0, 0);
+        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re, y.re, p1, 0, 0, 0);
+        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(-y.im, y.im, p2, 0, 0, 0);
+        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(y.re - y.im, y.re + y.im, p3, 0, 0, 0);
     }
 #pragma unroll
-    for (int e = 0; e < 4; ++e) P[li * kWyB + g + 4 * e] = cx<double>{pr[e], pi[e]};
+    for (int e = 0; e < 4; ++e) P[li * kWyB + g + 4 * e] = cx<double>{p1[e] - p2[e], p3[e] - p1[e] - p2[e]};
     __syncthreads();
     if (lane < kWyB) {
         // row `lane` of T in registers; T(t, l) = 0 for l < t, so every sum can start at l = 0
