@@ -25,7 +25,8 @@ def _hnorm(h, ks):
     return np.array([np.linalg.norm(h(k), 2) for k in ks.T])
 
 
-@pytest.mark.parametrize("N,rmax,nk", [(2, 1, 16), (4, 2, 64), (17, 2, 25), (64, 2, 128), (96, 1, 16), (128, 1, 8)])
+@pytest.mark.parametrize("N,rmax,nk", [(2, 1, 16), (4, 2, 64), (17, 2, 25), (49, 1, 25), (64, 2, 128), (96, 1, 16),
+                                        (128, 1, 8)])
 def test_mixed_bands(pkg, oracle, mixed, N, rmax, nk):
     h = pkg.models.random_tb(N, rmax=rmax, seed=300 + N)
     ks = pkg.regulargrid(nk=nk)

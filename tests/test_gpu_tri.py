@@ -85,10 +85,12 @@ def test_tri_rho_vs_oracle(pkg, oracle, gpu, trieig, name, precision):
     assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
 
 
-@pytest.mark.parametrize("N", [5, 12, 17, 31, 40])
+@pytest.mark.parametrize("N", [5, 12, 17, 31, 33, 40, 48, 49, 63])
 def test_tri_small_register_tiles(pkg, oracle, gpu, N):
     # N <= 16 reaches the twisted kernel's 8- and 16-entry instantiations only with the
-    # compact-WY back-transformation forced on
+    # compact-WY back-transformation forced on; 33, 48, 49 and 63 start the register-tile
+    # tridiagonalization on 3x3 / 4x4 tiles and hand the trailing matrix to the next smaller
+    # tiles after 1, 16, 1 and 15 columns
     gpu.set_variant(4, 1)
     try:
         h = pkg.models.random_tb(N, rmax=1, seed=40 + N)
