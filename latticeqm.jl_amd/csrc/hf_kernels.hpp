@@ -161,6 +161,10 @@ __global__ __launch_bounds__(256) void k_fermi_occ(int L, const double* __restri
     if (i < L) occ[i] = fermidirac(ev[i], mu[0], T);
 }
 
+// 1/d for d >= 1 (a Fermi denominator): hardware reciprocal with one Newton step, 0 beyond
+// 1e300 (where 1/d is below 1e-300, and the Newton step would meet inf * 0).
+__device__ __forceinline__ double recip_ge1(double d) { return d < 1e300 ? rcp_nr(d) : 0.0; }
+
 // Roots.jl-style bracketing midpoint: 0 across a sign change, else the midpoint of the
 // bit patterns of |a| and |b|, so a bisection reaches adjacent doubles in <= 64 steps.
 __device__ __forceinline__ double bit_middle_dev(double a, double b) {
@@ -264,21 +268,21 @@ __global__ __launch_bounds__(1024) void k_mu_bisect(int N, int L, const double* 
 #pragma unroll
             for (int q = 0; q < kMuReg; ++q)
                 if (threadIdx.x + 1024 * q < L) {
-                    const double f = 1.0 / (xv[q] * em + 1.0);
+                    const double f = recip_ge1(xv[q] * em + 1.0);
                     a += kw ? wv[q] * f : f;
                 }
         } else if (inreg) {
 #pragma unroll
             for (int q = 0; q < kMuReg; ++q)
                 if (threadIdx.x + 1024 * q < L) {
-                    const double f = 1.0 / (exp((xv[q] - mu) * invT) + 1.0);
+                    const double f = recip_ge1(exp((xv[q] - mu) * invT) + 1.0);
                     a += kw ? wv[q] * f : f;
                 }
         } else if (factored) {
             const double em = exp(cexp - mu * invT);
             int k = threadIdx.x / N, n = threadIdx.x - k * N;
             for (int i = threadIdx.x; i < L; i += 1024) {
-                const double f = 1.0 / (ework[i] * em + 1.0);
+                const double f = recip_ge1(ework[i] * em + 1.0);
                 a += kw ? kw[k] * f : f;
                 k += kq;
                 n += rq;
@@ -291,7 +295,7 @@ __global__ __launch_bounds__(1024) void k_mu_bisect(int N, int L, const double* 
             int k = threadIdx.x / N, n = threadIdx.x - k * N;
             for (int i = threadIdx.x; i < L; i += 1024) {
                 const double x = bands[i] - mu;
-                const double f = 1.0 / (exp(x * invT) + 1.0);
+                const double f = recip_ge1(exp(x * invT) + 1.0);
                 a += kw ? kw[k] * f : f;
                 k += kq;
                 n += rq;
@@ -317,7 +321,40 @@ __global__ __launch_bounds__(1024) void k_mu_bisect(int N, int L, const double* 
         status = 1;
     } else {
         bool done = false;
-        for (int it = 0; it < 200; ++it) {
+        // Illinois (modified regula falsi) shrinks the bracket from both sides until it spans a
+        // few dozen doubles (at most 40 steps); the bit-midpoint bisection below then ends it
+        // at adjacent doubles as Roots.jl's Bisection does.  Inside a gap, where the filling is
+        // flat to rounding, the root it settles on may differ from a pure bisection's by up to
+        // the gap, with occupations equal to exp(-gap / 2T).
+        {
+            double wa = fa, wb = fb;
+            int side = 0;
+            for (int it = 0; it < 40; ++it) {
+                if (b - a <= 32.0 * 2.220446049250313e-16 * fmax(fmax(fabs(a), fabs(b)), T)) break;
+                double c = (a * wb - b * wa) / (wb - wa);
+                if (!(c > a && c < b)) c = 0.5 * (a + b);
+                if (!(c > a && c < b)) break;
+                try_factor(a, b);
+                const double fc = F(c);
+                if (fc == 0.0) {
+                    mu = c;
+                    done = true;
+                    break;
+                }
+                if ((fc < 0) == (fa < 0)) {
+                    a = c;
+                    fa = wa = fc;
+                    if (side == -1) wb *= 0.5;
+                    side = -1;
+                } else {
+                    b = c;
+                    fb = wb = fc;
+                    if (side == 1) wa *= 0.5;
+                    side = 1;
+                }
+            }
+        }
+        for (int it = 0; it < 200 && !done; ++it) {
             const double mid = bit_middle_dev(a, b);
             if (!(mid > a && mid < b)) break;
             try_factor(a, b);
