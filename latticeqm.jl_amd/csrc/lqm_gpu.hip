@@ -198,12 +198,15 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     p.shAw = cC * (np + 2 * N) + sizeof(cx<double>) * nR;
     // forced variant, else the tuned one, else the default (1 = compact WY for the
     // back-transformation when N > 16, 0 otherwise: the WY kernel works on a 64 x 64 tile;
-    // the register-tile tridiagonalization for 16 < N <= 64)
+    // the register-tile tridiagonalization for 16 < N <= 64; for N <= 8 the one-wave
+    // tridiagonalization and the two-sided rho_k, whose tiles fit such matrices -- measured
+    // fastest there: 18.5 vs 21.1 ms for the 105-iteration SCF at 4 orbitals / 3600 k)
     auto variant = [&](int kind) {
         if (c->forced[kind] >= 0) return c->forced[kind];
         auto it = c->tuned.find((kind << 20) | N);
         if (it != c->tuned.end()) return it->second;
-        if (kind == VK_TRIDIAG) return N > 16 && N <= 64 ? 2 : 0;  // register tile (measured fastest)
+        if (kind == VK_TRIDIAG) return N > 16 && N <= 64 ? 2 : (N <= 8 ? 1 : 0);
+        if (kind == VK_RHO) return N <= 8 ? 1 : 0;
         return (kind == VK_BT && N > 16) || kind == VK_TRIEIG ? 1 : 0;
     };
     // QL: k-points (lanes) per block, 64 >> variant; fewer lanes per wave put more
