@@ -81,9 +81,10 @@ const char *lqm_stage_name(int stage);
 
 /* Kernel variants (no reference counterpart; the reference's only tunable is
  * the executor).  kind 0: tridiagonalization (0 = 256-thread full tile, 1 = one
- * wave with the packed Hermitian tile, 2 = register tile on a materialised H(k), N <= 64);
+ * wave with the packed Hermitian tile, the default for N <= 8, 2 = register tile on a
+ * materialised H(k), N <= 64, the default for 16 < N <= 64);
  * kind 1: rho_k (0 = V = QZ then V f V^H,
- * 1 = two-sided reflectors on Z f Z^T); kind 2: Ogita-Aishima sweep of the mixed path
+ * 1 = two-sided reflectors on Z f Z^T, the default for N <= 8); kind 2: Ogita-Aishima sweep of the mixed path
  * (0 = batched GEMM chain, 1 = one fused kernel per k-point with X and W in LDS, 2 = the
  * fused kernel in column-panel form with the products in matrix-core accumulators, two
  * blocks per CU; 1 and 2 need N <= 64, 2 is the default for N > 16);
