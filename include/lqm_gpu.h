@@ -59,7 +59,9 @@ int lqm_abi_version(void);
  * a context is the GPUExec a multimode=:gpu call resolves to. */
 lqm_status lqm_context_create(int device, int precision, lqm_context **out);
 lqm_status lqm_context_destroy(lqm_context *ctx);
-/* Run on an existing hipStream_t (NULL restores the context's own stream). */
+/* Run on an existing hipStream_t (NULL restores the context's own stream).  Work the
+ * context places on its internal side stream is forked from and joined back to this stream
+ * with events, so every call is ordered on it as a whole. */
 lqm_status lqm_context_set_stream(lqm_context *ctx, void *hip_stream);
 lqm_status lqm_context_synchronize(lqm_context *ctx);
 /* Kernel-stage timing (no reference counterpart: the reference has no tracer).

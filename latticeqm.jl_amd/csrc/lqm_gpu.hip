@@ -101,6 +101,12 @@ struct lqm_context {
     int lds_max = 65536;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
+    // side stream for the compact-WY T factors, which need only the reflectors and so run
+    // beside the tridiagonal eigensolver (fork after the tridiagonalization, join before the
+    // back-transformation); tfac_pending: a fork not joined yet
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool tfac_pending = false;
     void* buf[W_NSLOTS] = {};
     size_t cap[W_NSLOTS] = {};
     int* d_status = nullptr;
@@ -127,6 +133,7 @@ lqm_status ws(lqm_context* c, Slot s, size_t bytes, void** out) {
     if (c->cap[s] < bytes) {
         if (c->buf[s]) {
             hipError_t e = hipStreamSynchronize(c->stream);
+            if (e == hipSuccess && c->side) e = hipStreamSynchronize(c->side);
             if (e != hipSuccess) return fail(LQM_ERR_DEVICE, hipGetErrorString(e));
             (void)hipFree(c->buf[s]);
             c->buf[s] = nullptr;
@@ -322,6 +329,12 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
                           bool logq, const cx<double>* Hpre = nullptr) {
     lqm_context* c = m->ctx;
     const int N = m->N;
+    if (c->tfac_pending) {
+        // a fork whose back-transformation never ran (an error return in between): the
+        // reflectors are about to be overwritten, so the main stream waits for it here
+        LQM_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        c->tfac_pending = false;
+    }
     R *dd, *ee;
     cx<R>*refl, *tau, *gA = nullptr;
     int* perm = nullptr;
@@ -386,6 +399,19 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
         if ((st = wsT(c, W_KLIST, (size_t)Kc + 1, &klist))) return st;
         if ((st = wsT(c, W_Z, (size_t)N * N * Kc, &Z))) return st;
         if ((st = wsT(c, W_TNORM, (size_t)Kc, &tn))) return st;
+        // the compact-WY T factors (reflectors only) on the side stream, beside the eigensolver
+        const int nbw = (N - 1 + lqm::kWyB - 1) / lqm::kWyB;
+        if (nbw > 0) {
+            cx<double>* Tg;
+            if ((st = wsT(c, W_WYT, (size_t)nbw * lqm::kWyB * lqm::kWyB * Kc, &Tg))) return st;
+            LQM_HIP(hipEventRecord(c->ev_fork, c->stream));
+            LQM_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+            hipLaunchKernelGGL(lqm::k_wy_tfac<R>, dim3(nbw, Kc), dim3(64), 0, c->side, N,
+                               static_cast<const cx<R>*>(c->buf[W_REFL]), static_cast<const cx<R>*>(c->buf[W_TAU]), Tg);
+            LQM_HIP(hipGetLastError());
+            LQM_HIP(hipEventRecord(c->ev_join, c->side));
+            c->tfac_pending = true;
+        }
         LQM_HIP(hipMemsetAsync(klist, 0, sizeof(int), c->stream));
         StageScope sc(c, ST_QL, Kc);
         const dim3 g4((Kc + 3) / 4);
@@ -490,8 +516,13 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const R* ev
             cx<double>* Tg = nullptr;
             lqm_status st;
             if ((st = wsT(c, W_WYT, (size_t)std::max(1, nbw) * lqm::kWyB * lqm::kWyB * Kc, &Tg))) return st;
-            if (nbw > 0)
+            if (c->tfac_pending) {
+                // launched beside the eigensolver for this chunk: join the side stream
+                LQM_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+                c->tfac_pending = false;
+            } else if (nbw > 0) {
                 hipLaunchKernelGGL(lqm::k_wy_tfac<R>, dim3(nbw, Kc), dim3(64), 0, c->stream, N, refl, tau, Tg);
+            }
             if constexpr (std::is_same<R, float>::value && MODE == 0) {
                 // mixed path: the FP32 vectors on the FP32 matrix cores
                 const size_t sh = lqm::wy32_lds_bytes();
@@ -1629,10 +1660,16 @@ lqm_status lqm_context_create(int device, int precision, lqm_context** out) {
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess)
         c->lds_max = lds;
     hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&c->d_status, sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->d_status, 0, sizeof(int));
     if (e != hipSuccess) {
         if (c->own) (void)hipStreamDestroy(c->own);
+        if (c->side) (void)hipStreamDestroy(c->side);
+        if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+        if (c->ev_join) (void)hipEventDestroy(c->ev_join);
         delete c;
         return fail(LQM_ERR_DEVICE, std::string("context init: ") + hipGetErrorString(e));
     }
@@ -1645,10 +1682,14 @@ lqm_status lqm_context_destroy(lqm_context* c) {
     if (!c) return LQM_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->side);
     for (int s = 0; s < W_NSLOTS; ++s)
         if (c->buf[s]) (void)hipFree(c->buf[s]);
     (void)hipFree(c->d_status);
     (void)hipStreamDestroy(c->own);
+    (void)hipStreamDestroy(c->side);
+    (void)hipEventDestroy(c->ev_fork);
+    (void)hipEventDestroy(c->ev_join);
     delete c;
     return LQM_OK;
 }
