@@ -125,7 +125,8 @@ def test_dos_vs_oracle(pkg, oracle, gpu):
     assert np.max(np.abs(dw - ref)) < 1e-10 * np.max(ref)
 
 
-@pytest.mark.parametrize("filling,T", [(0.5, 0.0), (0.25, 0.0), (1.0, 0.0), (0.0, 0.0), (0.5, 0.01), (0.37, 0.2)])
+@pytest.mark.parametrize("filling,T", [(0.5, 0.0), (0.25, 0.0), (1.0, 0.0), (0.0, 0.0), (0.5, 0.01), (0.37, 0.2),
+                                       (0.02, 0.05), (0.98, 0.05), (0.5, 0.001)])
 def test_chemicalpotential_vs_oracle(pkg, oracle, gpu, filling, T):
     h = pkg.models.random_tb(6, rmax=1, seed=12)
     ks = pkg.regulargrid(nk=49)
@@ -137,6 +138,23 @@ def test_chemicalpotential_vs_oracle(pkg, oracle, gpu, filling, T):
     else:
         assert abs(oracle.getfilling(bands, mu, T) - filling) < 1e-12
         assert abs(mu - oracle.chemicalpotential(bands, filling, T)) < 1e-10
+
+
+def test_chemicalpotential_in_a_gap(pkg, oracle, gpu):
+    # half filling inside a gap of width > 2 at kT = 0.01: the filling is flat to rounding
+    # across the gap, so mu must lie in the gap and solve the defining equation
+    base = pkg.models.random_tb(4, rmax=1, seed=3)
+    h = pkg.Hops({key: 0.1 * base[key] for key in base.keys()})
+    h[(0, 0)] = h[(0, 0)] + np.diag([2.0, 2.0, -2.0, -2.0])
+    ks = pkg.regulargrid(nk=36)
+    R, Tt = h.pack()
+    bands = oracle.bands(R, Tt, ks)
+    vals = np.sort(bands.ravel())
+    lo, hi = vals[vals.size // 2 - 1], vals[vals.size // 2]
+    assert hi - lo > 2.0
+    mu = pkg.api.chemicalpotential(h, ks, 0.5, T=0.01)
+    assert lo < mu < hi
+    assert abs(oracle.getfilling(bands, mu, 0.01) - 0.5) < 1e-12
 
 
 def test_setfilling_graphene(pkg, gpu):
