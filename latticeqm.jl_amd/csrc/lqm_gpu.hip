@@ -399,9 +399,10 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
         if ((st = wsT(c, W_KLIST, (size_t)Kc + 1, &klist))) return st;
         if ((st = wsT(c, W_Z, (size_t)N * N * Kc, &Z))) return st;
         if ((st = wsT(c, W_TNORM, (size_t)Kc, &tn))) return st;
-        // the compact-WY T factors (reflectors only) on the side stream, beside the eigensolver
+        // the compact-WY T factors (reflectors only) on the side stream, beside the eigensolver;
+        // below 2048 k-points the fork and join cost more than the overlap saves
         const int nbw = (N - 1 + lqm::kWyB - 1) / lqm::kWyB;
-        if (nbw > 0) {
+        if (nbw > 0 && Kc >= 2048) {
             cx<double>* Tg;
             if ((st = wsT(c, W_WYT, (size_t)nbw * lqm::kWyB * lqm::kWyB * Kc, &Tg))) return st;
             LQM_HIP(hipEventRecord(c->ev_fork, c->stream));
