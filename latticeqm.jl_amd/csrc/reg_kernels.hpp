@@ -68,7 +68,7 @@ template <typename R>
 struct RegTriSmem {
     cx<R> sv[2][80];  // Householder vector v (zero above row j+1), entry r at rpad(r)
     cx<R> sp[2][80];  // p = tau A22 v, entry r at rpad(r)
-    R ss[2][16];      // per tile column partial of v^H A22 v
+    R ss[2][4];       // per wave partial of v^H A22 v
     cx<R> stau[2];
     cx<R> L[48 * kRegLd];  // trailing matrix (<= 48 x 48) between tile sizes, column-major
 };
@@ -174,13 +174,16 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[T][T], RegTriSmem<R>& S, 
             for (int v = 0; v < T; ++v) S.sp[buf][rpad(off + T * tj + v)] = p[v];
         }
     }
-    if (ti == 0) S.ss[buf][tj] = spart;
+    {
+        // the wave's four tile columns (its rows of 16 lanes, lane ti = 0 of each) summed with
+        // readlane: one partial of v^H A22 v per wave
+        const R s4 = ((rl(spart, 0) + rl(spart, 16)) + rl(spart, 32)) + rl(spart, 48);
+        if (ti == 0 && (tj & 3) == 0) S.ss[buf][tj >> 2] = s4;
+    }
     __syncthreads();
 
     // (c) A22 -= v w^H + w v^H, w = p + alpha2 v, alpha2 = -1/2 |tau|^2 v^H A22 v
-    R sum = 0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) sum += S.ss[buf][q];
+    const R sum = (S.ss[buf][0] + S.ss[buf][1]) + (S.ss[buf][2] + S.ss[buf][3]);
     const R alpha2 = (R)-0.5 * (t.re * t.re + t.im * t.im) * sum;
     if (wact && off + T * ti + T - 1 > j && off + T * tj + T - 1 > j) {
         cx<R> wr[T], wc[T];
