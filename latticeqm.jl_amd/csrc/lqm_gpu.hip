@@ -606,7 +606,9 @@ int contract_splits(int tiles) { return kContractBlocks / std::max(1, tiles); }
 
 // Key groups of the phase contraction: RG keys per group, the smallest of these holding nRr.
 int contract_group(int nRr) {
-    return nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 16 ? 16 : nRr <= 24 ? 24 : nRr <= 28 ? 28 : 32;
+    // 9 and 25: the 3x3 and 5x5 key sets of the nearest-image models, taken exactly
+    return nRr <= 1 ? 1 : nRr <= 4 ? 4 : nRr <= 8 ? 8 : nRr <= 9 ? 9 : nRr <= 16 ? 16 : nRr <= 24 ? 24
+         : nRr <= 25 ? 25 : nRr <= 28 ? 28 : 32;
 }
 
 // rho(R) += sum_k w_k e^{-2 pi i k.R} rho_k over a chunk: the phase table of the chunk's
@@ -632,8 +634,10 @@ lqm_status launch_contract(int RG, lqm_context* c, int NN, int Kc, int d, const 
             case 1: LQM_CONTRACT(1); break;
             case 4: LQM_CONTRACT(4); break;
             case 8: LQM_CONTRACT(8); break;
+            case 9: LQM_CONTRACT(9); break;
             case 16: LQM_CONTRACT(16); break;
             case 24: LQM_CONTRACT(24); break;
+            case 25: LQM_CONTRACT(25); break;
             case 28: LQM_CONTRACT(28); break;
             default: LQM_CONTRACT(32); break;
         }
@@ -646,8 +650,10 @@ lqm_status launch_contract(int RG, lqm_context* c, int NN, int Kc, int d, const 
         case 1: hipLaunchKernelGGL(lqm::k_phase_fold<1>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
         case 4: hipLaunchKernelGGL(lqm::k_phase_fold<4>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
         case 8: hipLaunchKernelGGL(lqm::k_phase_fold<8>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
+        case 9: hipLaunchKernelGGL(lqm::k_phase_fold<9>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
         case 16: hipLaunchKernelGGL(lqm::k_phase_fold<16>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
         case 24: hipLaunchKernelGGL(lqm::k_phase_fold<24>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
+        case 25: hipLaunchKernelGGL(lqm::k_phase_fold<25>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
         case 28: hipLaunchKernelGGL(lqm::k_phase_fold<28>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
         default: hipLaunchKernelGGL(lqm::k_phase_fold<32>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
     }
