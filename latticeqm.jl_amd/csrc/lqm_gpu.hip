@@ -294,7 +294,8 @@ void allow_lds(K kernel, size_t bytes) {
 lqm_status launch_hk_full(lqm_context* c, lqm_model* m, int Kc, const double* ks, cx<double>* A) {
     const int N = m->N, nR = m->nR;
     if (nR > 0 && nR <= 32) {
-        const int nrm = nR <= 8 ? 8 : nR <= 16 ? 16 : nR <= 24 ? 24 : nR <= 28 ? 28 : 32;
+        // 9 and 25: the 3x3 and 5x5 key sets of the nearest-image models, taken exactly
+        const int nrm = nR <= 8 ? 8 : nR <= 9 ? 9 : nR <= 16 ? 16 : nR <= 24 ? 24 : nR <= 25 ? 25 : nR <= 28 ? 28 : 32;
         cx<double>* ph;
         lqm_status st = wsT(c, W_PHT, (size_t)Kc * nrm, &ph);
         if (st) return st;
@@ -308,6 +309,8 @@ lqm_status launch_hk_full(lqm_context* c, lqm_model* m, int Kc, const double* ks
         const dim3 g(be, (Kc + kb - 1) / kb);
         switch (nrm) {
             case 8: hipLaunchKernelGGL(lqm::k_hk_stream<8>, g, dim3(256), 0, c->stream, N, nR, m->dT, Kc, kb, ph, A); break;
+            case 9: hipLaunchKernelGGL(lqm::k_hk_stream<9>, g, dim3(256), 0, c->stream, N, nR, m->dT, Kc, kb, ph, A); break;
+            case 25: hipLaunchKernelGGL(lqm::k_hk_stream<25>, g, dim3(256), 0, c->stream, N, nR, m->dT, Kc, kb, ph, A); break;
             case 16: hipLaunchKernelGGL(lqm::k_hk_stream<16>, g, dim3(256), 0, c->stream, N, nR, m->dT, Kc, kb, ph, A); break;
             case 24: hipLaunchKernelGGL(lqm::k_hk_stream<24>, g, dim3(256), 0, c->stream, N, nR, m->dT, Kc, kb, ph, A); break;
             case 28: hipLaunchKernelGGL(lqm::k_hk_stream<28>, g, dim3(256), 0, c->stream, N, nR, m->dT, Kc, kb, ph, A); break;
