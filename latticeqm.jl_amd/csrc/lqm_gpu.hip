@@ -49,7 +49,7 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -488,7 +488,8 @@ void launch_kc(lqm_context* c, const Plan& P, int N, int Kc, const cx<R>* refl, 
 template <typename R, int MODE>
 lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const R* evals,
                                 const double* kw, double wuni, double mu, double T,
-                                cx<double>* out, double* eb) {
+                                cx<double>* out, double* eb, cx<float>* out32 = nullptr,
+                                bool* wrote32 = nullptr) {
     lqm_context* c = m->ctx;
     const int N = m->N;
     R* gV = nullptr;
@@ -528,11 +529,19 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const R* ev
                 hipLaunchKernelGGL(lqm::k_wy_tfac<R>, dim3(nbw, Kc), dim3(64), 0, c->stream, N, refl, tau, Tg);
             }
             if constexpr (std::is_same<R, float>::value && MODE == 0) {
-                // mixed path: the FP32 vectors on the FP32 matrix cores
+                // mixed path: the FP32 vectors on the FP32 matrix cores, written in FP32 when
+                // the caller takes them so (out32), else widened to FP64
                 const size_t sh = lqm::wy32_lds_bytes();
-                allow_lds(lqm::k_backtransform_wy32, sh);
-                hipLaunchKernelGGL(lqm::k_backtransform_wy32, dim3(Kc), dim3(256), sh, c->stream, N, refl, Z, perm,
-                                   out, Tg, zfmt);
+                if (out32) {
+                    allow_lds(lqm::k_backtransform_wy32<cx<float>>, sh);
+                    hipLaunchKernelGGL(lqm::k_backtransform_wy32<cx<float>>, dim3(Kc), dim3(256), sh, c->stream, N,
+                                       refl, Z, perm, out32, Tg, zfmt);
+                    if (wrote32) *wrote32 = true;
+                } else {
+                    allow_lds(lqm::k_backtransform_wy32<cx<double>>, sh);
+                    hipLaunchKernelGGL(lqm::k_backtransform_wy32<cx<double>>, dim3(Kc), dim3(256), sh, c->stream, N,
+                                       refl, Z, perm, out, Tg, zfmt);
+                }
             } else {
                 auto kW = lqm::k_backtransform_wy<R, MODE>;
                 const size_t sh = lqm::wy_lds_bytes();
@@ -870,8 +879,8 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
     c->report[5] = 0.0;
     Plan P = make_plan(c, N, m->nR, Nk, true, false, sizeof(float));
     const size_t NN = (size_t)N * N;
-    // refinement buffers: X, A, W, S, G (+ rho_k) in FP64 complex per k-point
-    const size_t per64 = 16 * NN * (rho ? 6 : 5) + 48 * N;
+    // refinement buffers: X, A, W, S, G (+ rho_k) in FP64 complex per k-point, X in FP32
+    const size_t per64 = 16 * NN * (rho ? 6 : 5) + 8 * NN + 48 * N;
     P.Kc = (int)std::max<size_t>(1, std::min<size_t>((size_t)P.Kc, kWorkspaceBudget / per64));
     const double wuni = 1.0 / (double)Nk;
     const cx<double> one = {1.0, 0.0}, zero = {0.0, 0.0};
@@ -907,7 +916,17 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         }
         LQM_HIP(hipGetLastError());
         if ((st = launch_eigvals<float>(m, P, Kc, ksc, ev32, true, A))) return st;
-        if ((st = launch_backtransform<float, 0>(m, P, Kc, ev32, nullptr, 0.0, 0.0, 0.0, X, nullptr)))
+        // refinement kind: 2 = one fused kernel per k-point and sweep in column-panel form
+        // (two blocks per CU; the default for 16 < N <= 64, as it works on a 64 x 64 tile),
+        // 1 = the same with X and W in LDS (one block per CU), 0 = the GEMM chain below
+        const int fk = c->forced[VK_REFINE] >= 0 ? c->forced[VK_REFINE] : (N > 16 ? 2 : 0);
+        const bool fused = N <= 64 && fk >= 1;
+        // the panel sweep widens FP32 eigenvectors itself: the compact WY then writes them in
+        // FP32 (half the bytes each way), and the first sweep writes FP64 X where needed
+        cx<float>* X32 = nullptr;
+        bool x32 = false;
+        if (fused && fk == 2 && (st = wsT(c, W_X32, NN * P.Kc, &X32))) return st;
+        if ((st = launch_backtransform<float, 0>(m, P, Kc, ev32, nullptr, 0.0, 0.0, 0.0, X, nullptr, X32, &x32)))
             return st;
         // Ogita-Aishima sweeps.  The first runs on every k-point; later ones only on the
         // k-points whose last correction flagged another sweep (a rotated or unresolved
@@ -919,11 +938,6 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         std::vector<int> cur, next;
         const int max_sw = std::max(2, c->refine_max);
         hstat.resize((size_t)4 * Kc);
-        // refinement kind: 2 = one fused kernel per k-point and sweep in column-panel form
-        // (two blocks per CU; the default for 16 < N <= 64, as it works on a 64 x 64 tile),
-        // 1 = the same with X and W in LDS (one block per CU), 0 = the GEMM chain below
-        const int fk = c->forced[VK_REFINE] >= 0 ? c->forced[VK_REFINE] : (N > 16 ? 2 : 0);
-        const bool fused = N <= 64 && fk >= 1;
         std::vector<int> clus;
         // with rho or eband requested, a fused sweep forms rho_k (into Sm) and eb of every
         // k-point it finishes; the rest go through k_scale_occ and the NC product below
@@ -935,11 +949,19 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
             bool have_stat = false;
             if (fused) {
                 StageScope sc(c, ST_OA, nsub);
-                if (fk == 2) {
+                if (fk == 2 && x32) {
+                    // first sweep, every k-point: X read in FP32 from the compact WY
                     const size_t sh = lqm::oa_panel_lds_bytes();
-                    allow_lds(lqm::k_oa_panel, sh);
-                    hipLaunchKernelGGL(lqm::k_oa_panel, dim3(nsub), dim3(256), sh, c->stream, N, A, X, Sm, G, lam,
-                                       stat, c->refine_thr, kidx, rq);
+                    allow_lds(lqm::k_oa_panel<cx<float>>, sh);
+                    hipLaunchKernelGGL(lqm::k_oa_panel<cx<float>>, dim3(nsub), dim3(256), sh, c->stream, N, A,
+                                       static_cast<const cx<float>*>(X32), X, Sm, G, lam, stat, c->refine_thr,
+                                       kidx, rq);
+                    x32 = false;
+                } else if (fk == 2) {
+                    const size_t sh = lqm::oa_panel_lds_bytes();
+                    allow_lds(lqm::k_oa_panel<cx<double>>, sh);
+                    hipLaunchKernelGGL(lqm::k_oa_panel<cx<double>>, dim3(nsub), dim3(256), sh, c->stream, N, A,
+                                       static_cast<const cx<double>*>(X), X, Sm, G, lam, stat, c->refine_thr, kidx, rq);
                 } else {
                     const size_t sh = lqm::oa_fused_lds_bytes();
                     allow_lds(lqm::k_oa_fused, sh);

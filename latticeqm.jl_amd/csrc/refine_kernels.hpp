@@ -594,18 +594,19 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// X (N x N column-major) into the zero-padded 64 x 64 LDS tile Xs (leading dimension
-// kFusedLd): all 16 loads of a thread are issued before the first store (clamped addresses,
-// so every load is in bounds and unconditional).
-__device__ __forceinline__ void load_x_tile(const cx<double>* __restrict__ X, int N, cx<double>* Xs, int tid) {
+// X (N x N column-major, FP64 or FP32 widened exactly) into the zero-padded 64 x 64 LDS
+// tile Xs (leading dimension kFusedLd): all 16 loads of a thread are issued before the first
+// store (clamped addresses, so every load is in bounds and unconditional).
+template <typename XT>
+__device__ __forceinline__ void load_x_tile(const XT* __restrict__ X, int N, cx<double>* Xs, int tid) {
     const int r = tid & 63, c0 = tid >> 6;
-    cx<double> v[16];
+    XT v[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) v[u] = X[(size_t)min(c0 + 4 * u, N - 1) * N + min(r, N - 1)];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
         const int c = c0 + 4 * u;
-        Xs[c * kFusedLd + r] = (r < N && c < N) ? v[u] : cx<double>{0.0, 0.0};
+        Xs[c * kFusedLd + r] = (r < N && c < N) ? cx<double>{(double)v[u].re, (double)v[u].im} : cx<double>{0.0, 0.0};
     }
 }
 
@@ -990,8 +991,12 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
 // ---------------------------------------------------------------------------------
 inline size_t oa_panel_lds_bytes() { return sizeof(cx<double>) * 64 * kFusedLd + 8 * (64 + 32 + 64 + 64); }
 
+// XT: the element type X is read as (Xin); the refined X, and for a k-point handed to the
+// general correction its unrefined X, are written to Xg in FP64 (with XT = cx<double>, Xin and
+// Xg are the same array).
+template <typename XT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_oa_panel(
-    int N, const cx<double>* __restrict__ Ag, cx<double>* __restrict__ Xg, cx<double>* __restrict__ Sg,
+    int N, const cx<double>* __restrict__ Ag, const XT* Xin, cx<double>* Xg, cx<double>* __restrict__ Sg,
     cx<double>* __restrict__ Gg, double* __restrict__ lambda, double* __restrict__ stat, double thr,
     const int* __restrict__ kidx, OaRho rq) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1009,7 +1014,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const size_t NN = (size_t)N * N;
     const cx<double>* A = Ag + kk * NN;
     cx<double>* X = Xg + kk * NN;
-    load_x_tile(X, N, Xs, tid);
+    load_x_tile(Xin + kk * NN, N, Xs, tid);
     if (tid == 0) s_cluster = 0;
     __syncthreads();
 
@@ -1149,6 +1154,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (s_cluster) {
         cx<double>* Sk = Sg + kk * NN;
         cx<double>* Gk = Gg + kk * NN;
+        if constexpr (!std::is_same<XT, cx<double>>::value) {
+            // the general correction reads X from Xg
+            for (int e = tid; e < N * N; e += 256) {
+                const int c = e / N, r = e - c * N;
+                X[e] = Xs[c * kFusedLd + r];
+            }
+        }
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll

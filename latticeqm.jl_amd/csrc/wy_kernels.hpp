@@ -389,10 +389,13 @@ __host__ __device__ constexpr size_t wy32_region() {
 }
 inline size_t wy32_lds_bytes() { return wy32_region() + 64 * sizeof(int); }
 
+// OT: the element type of the eigenvectors written (cx<double>, or cx<float> for a reader
+// that widens them itself -- the values are FP32 either way)
+template <typename OT>
 __global__ __launch_bounds__(256) void k_backtransform_wy32(int N, const cx<float>* __restrict__ refl,
                                                             const float* __restrict__ Zg,
                                                             const int* __restrict__ perm,
-                                                            cx<double>* __restrict__ out,
+                                                            OT* __restrict__ out,
                                                             const cx<double>* __restrict__ Tg,
                                                             const int* __restrict__ zfmt) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -551,8 +554,8 @@ __global__ __launch_bounds__(256) void k_backtransform_wy32(int N, const cx<floa
     __syncthreads();
     for (int n = tid; n < N; n += 256) invp[min(max(perm[(size_t)kk * N + n], 0), N - 1)] = n;
     __syncthreads();
-    cx<double>* Xs = reinterpret_cast<cx<double>*>(smem);  // overlays Y, U, W, T
-    cx<double>* o = out + (size_t)kk * NN;
+    OT* Xs = reinterpret_cast<OT*>(smem);  // overlays Y, U, W, T
+    OT* o = out + (size_t)kk * NN;
     for (int h = 0; h < 2; ++h) {
         if (wc == h) {
 #pragma unroll
@@ -561,8 +564,7 @@ __global__ __launch_bounds__(256) void k_backtransform_wy32(int N, const cx<floa
                 for (int b = 0; b < 2; ++b)
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
-                        Xs[(16 * b + li) * kWy32Ld + m0 + 16 * a + 4 * g + e] =
-                            cx<double>{(double)vr[a][b][e], (double)vi[a][b][e]};
+                        Xs[(16 * b + li) * kWy32Ld + m0 + 16 * a + 4 * g + e] = OT{vr[a][b][e], vi[a][b][e]};
         }
         __syncthreads();
         for (int idx = tid; idx < 32 * 64; idx += 256) {
