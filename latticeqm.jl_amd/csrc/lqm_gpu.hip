@@ -1810,6 +1810,7 @@ const char* lqm_stage_name(int stage) { return (stage >= 0 && stage < ST_N) ? kS
 lqm_status lqm_context_synchronize(lqm_context* c) {
     if (!c) return fail(LQM_ERR_ARG, "null context");
     LQM_HIP(hipStreamSynchronize(c->stream));
+    LQM_HIP(hipStreamSynchronize(c->side));  // joined into the stream already; idle here
     return LQM_OK;
 }
 
