@@ -54,6 +54,34 @@ def test_rho_empty_full_and_partly_occupied_chunks(pkg, oracle, gpu, variants, m
     _rho_vs_oracle(pkg, oracle, h, pkg.regulargrid(nk=25), mu, 0.02)
 
 
+def test_rho_on_a_callers_stream(pkg, gpu):
+    # the context on a caller's stream, with its internal side stream (the compact-WY T
+    # factors, taken for chunks of >= 2048 k-points) forked from and joined back to it: the
+    # same bits as on the context's own stream (every reduction has a fixed order)
+    import torch
+
+    h = pkg.models.random_tb(64, rmax=1, seed=71)
+    ks = pkg.regulargrid(nk=48 * 48)
+
+    def run():
+        rho = _zero_like(pkg, h)
+        eb = pkg.api.getdensitymatrix(rho, h, ks, 0.0, T=0.02, executor=gpu)
+        return rho, eb
+
+    ref_rho, ref_eb = run()
+    s = torch.cuda.Stream()
+    gpu.set_stream(s.cuda_stream)
+    try:
+        with torch.cuda.stream(s):
+            rho, eb = run()
+        s.synchronize()
+    finally:
+        gpu.set_stream(None)
+    assert eb == ref_eb
+    for key in ref_rho.keys():
+        assert np.array_equal(rho[key], ref_rho[key])
+
+
 def test_rho_degenerate_spinful_all_variants(pkg, oracle, gpu, variants):
     h = pkg.models.addspin(pkg.models.random_tb(5, rmax=1, seed=3), "spinhalf")
     _rho_vs_oracle(pkg, oracle, h, pkg.regulargrid(nk=36), 0.1, 0.0)
