@@ -73,9 +73,11 @@ lqm_status lqm_context_synchronize(lqm_context *ctx);
  * 5 rho_twosided, 6 phase_contract, 7 phase_fold, 8 dos, 9 reduce, 10 herm_hk_fp64,
  * 11 zgemm_nn_fp64, 12 zgemm_cn_fp64, 13 zgemm_nc_fp64, 14 oa_correct, 15 hf_meanfield,
  * 16 mix, 17 mu_bisect, 18 blocked_hk, 19 blocked_panel, 20 blocked_update,
- * 21 blocked_bisect, 22 blocked_invit, 23 blocked_backtransform, 24 blocked_rho
- * (LQM_NSTAGES = 25); lqm_stage_name gives the names. */
-#define LQM_NSTAGES 25
+ * 21 blocked_bisect, 22 blocked_invit, 23 blocked_backtransform, 24 blocked_rho,
+ * 25 oa_correct_later (14 = the first refinement sweep, on every k-point; 25 = later sweeps
+ * and cluster corrections, on listed k-points) (LQM_NSTAGES = 26); lqm_stage_name gives the
+ * names. */
+#define LQM_NSTAGES 26
 lqm_status lqm_context_set_timing(lqm_context *ctx, int enable);
 lqm_status lqm_context_stage_times(lqm_context *ctx, double *ms, int64_t *launches,
                                    int64_t *kpoints, int n);

@@ -70,7 +70,7 @@ _PROTOS = {
                             P, P, P],
 }
 _RESTYPE = {"lqm_last_error": ctypes.c_char_p, "lqm_abi_version": I, "lqm_stage_name": ctypes.c_char_p}
-NSTAGES = 25
+NSTAGES = 26
 
 
 class HFProblem(ctypes.Structure):

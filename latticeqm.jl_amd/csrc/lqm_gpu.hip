@@ -66,7 +66,7 @@ constexpr int kContractBlocks = 768;
 enum Stage {
     ST_TRIDIAG, ST_QL, ST_REPLAY, ST_BACKTRANSFORM, ST_ZFZ, ST_RHO2, ST_CONTRACT, ST_FOLD, ST_DOS,
     ST_REDUCE, ST_HK64, ST_GEMM_NN, ST_GEMM_CN, ST_GEMM_NC, ST_OA, ST_HF, ST_MIX, ST_MU, ST_BHK,
-    ST_BTRD, ST_BUPD, ST_BEIG, ST_BVEC, ST_BBT, ST_BRHO, ST_N
+    ST_BTRD, ST_BUPD, ST_BEIG, ST_BVEC, ST_BBT, ST_BRHO, ST_OA2, ST_N
 };
 static_assert(ST_N == LQM_NSTAGES, "stage list and LQM_NSTAGES disagree");
 const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "backtransform", "zfz",
@@ -75,7 +75,7 @@ const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "ba
                                        "zgemm_nc_fp64", "oa_correct", "hf_meanfield", "mix",
                                        "mu_bisect", "blocked_hk", "blocked_panel", "blocked_update",
                                        "blocked_bisect", "blocked_invit", "blocked_backtransform",
-                                       "blocked_rho"};
+                                       "blocked_rho", "oa_correct_later"};
 
 struct TimedLaunch {
     int stage;
@@ -948,7 +948,8 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         for (;;) {
             bool have_stat = false;
             if (fused) {
-                StageScope sc(c, ST_OA, nsub);
+                // the first sweep (every k-point) and the later ones (listed k-points) timed apart
+                StageScope sc(c, kidx ? ST_OA2 : ST_OA, nsub);
                 if (fk == 2 && x32) {
                     // first sweep, every k-point: X read in FP32 from the compact WY
                     const size_t sh = lqm::oa_panel_lds_bytes();
@@ -989,7 +990,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
                                            c->stream));
                     const int nc = (int)clus.size();
                     {
-                        StageScope sc(c, ST_OA, nc);
+                        StageScope sc(c, ST_OA2, nc);
                         const size_t sh = lqm::oa_lds_bytes(N);
                         allow_lds(lqm::k_oa_correct, sh);
                         hipLaunchKernelGGL(lqm::k_oa_correct, dim3(nc), dim3(256), sh, c->stream, N, Sm, G, W,
@@ -1006,7 +1007,8 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
             zgemm<1, 0>(c, N, nsub, X, W, Sm, one, zero, nullptr, kidx);  // S = X^H A X
             zgemm<1, 0>(c, N, nsub, X, X, G, one, zero, nullptr, kidx);   // G = X^H X
             {
-                StageScope sc(c, ST_OA, nsub);
+                // the first sweep (every k-point) and the later ones (listed k-points) timed apart
+                StageScope sc(c, kidx ? ST_OA2 : ST_OA, nsub);
                 const size_t sh = lqm::oa_lds_bytes(N);
                 allow_lds(lqm::k_oa_correct, sh);
                 hipLaunchKernelGGL(lqm::k_oa_correct, dim3(nsub), dim3(256), sh, c->stream, N, Sm, G, W, lam,
