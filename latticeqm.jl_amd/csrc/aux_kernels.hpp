@@ -164,7 +164,16 @@ __global__ __launch_bounds__(256) void k_sum(int n, const double* __restrict__ x
                                              double* __restrict__ out, int accumulate) {
     __shared__ double sred[4];
     double a = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) a += x[i];
+    int i = threadIdx.x;
+    // eight loads in flight, added in index order (the same sum as one at a time)
+    for (; i + 7 * 256 < n; i += 8 * 256) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = x[i + q * 256];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a += v[q];
+    }
+    for (; i < n; i += 256) a += x[i];
     a = wave_sum(a);
     if ((threadIdx.x & 63) == 0) sred[threadIdx.x / 64] = a;
     __syncthreads();

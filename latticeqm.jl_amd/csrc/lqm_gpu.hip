@@ -1066,8 +1066,9 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         sweeps_max = std::max(sweeps_max, sweeps);
         if (bands) {
             StageScope sc(c, ST_REDUCE);
-            hipLaunchKernelGGL(lqm::k_sort_evals, dim3(Kc), dim3(256), 0, c->stream, N, lam,
-                               bands + (size_t)k0 * N);
+            // one wave per k-point up to 64 eigenvalues (the kernel strides by its block size)
+            hipLaunchKernelGGL(lqm::k_sort_evals, dim3(Kc), dim3(std::min(256, (N + 63) / 64 * 64)), 0, c->stream,
+                               N, lam, bands + (size_t)k0 * N);
         }
         if (rho || eband) {
             // k-points whose rho_k the fused sweep did not form (all of them on the GEMM

@@ -534,7 +534,7 @@ __global__ __launch_bounds__(256) void k_sort_evals(int N, const double* __restr
                                                     double* __restrict__ out) {
     const int kk = blockIdx.x;
     const double* l = lambda + (size_t)kk * N;
-    for (int i = threadIdx.x; i < N; i += 256) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
         const double li = l[i];
         int rank = 0;
         for (int j = 0; j < N; ++j) {
