@@ -128,6 +128,12 @@ lqm_status lqm_context_set_variant(lqm_context *ctx, int kind, int variant);
 lqm_status lqm_context_set_refinement(lqm_context *ctx, int max_sweeps);
 lqm_status lqm_context_set_refinement_threshold(lqm_context *ctx, double threshold);
 lqm_status lqm_context_refinement_report(lqm_context *ctx, double *out);
+/* Precision the last pass over a grid actually ran in (LQM_PREC_FP64 or
+ * LQM_PREC_FP32_REFINED; -1 before the first pass).  A LQM_PREC_FP32_REFINED context runs
+ * the FP64 path for eigenvector output and for the large-N layout (kind 5 = 1); the
+ * refinement report is cleared at the start of every pass, so it reads all zeros after a
+ * pass that did no refinement.  No reference counterpart (the reference is FP64 only). */
+lqm_status lqm_context_last_precision(lqm_context *ctx, int *precision);
 lqm_status lqm_context_get_variant(lqm_context *ctx, int kind, int N, int *variant);
 lqm_status lqm_autotune(lqm_model *m, int Nk, const double *ks_dev);
 

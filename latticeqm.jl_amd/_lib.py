@@ -42,6 +42,7 @@ _PROTOS = {
     "lqm_context_set_variant": [P, I, I],
     "lqm_context_set_refinement": [P, I],
     "lqm_context_refinement_report": [P, P],
+    "lqm_context_last_precision": [P, P],
     "lqm_context_set_refinement_threshold": [P, D],
     "lqm_context_get_variant": [P, I, I, P],
     "lqm_autotune": [P, I, P],

@@ -78,6 +78,13 @@ class GPUExec:
                 "kpoints_near_degenerate": int(out[3]), "kpoint_sweeps": int(out[4]),
                 "kpoints_rho_fused": int(out[5])}
 
+    def last_precision(self) -> str | None:
+        """Precision the last pass actually ran in ("fp64" / "mixed"; None before any pass).
+        A mixed executor runs FP64 for eigenvector output and for the large-N layout."""
+        v = _c.c_int()
+        _lib.call("lqm_context_last_precision", self.handle, _c.byref(v))
+        return {0: "fp64", 1: "mixed"}.get(v.value)
+
     def set_stream(self, stream_ptr: int | None) -> None:
         _lib.call("lqm_context_set_stream", self.handle, _c.c_void_p(stream_ptr or 0))
 
@@ -164,12 +171,18 @@ def _kpts(ks, d: int) -> np.ndarray:
     return k
 
 
-def _weights(ks, kweights):
+def _weights(ks, kweights, Nk: int | None = None):
+    """Per-k weights as a contiguous float64 array (None: uniform 1/Nk).  The library copies
+    exactly Nk doubles, so a weight vector of any other length is an error here."""
     if kweights is not None:
-        return np.ascontiguousarray(kweights, dtype=np.float64)
-    if isinstance(ks, Mesh):
-        return np.ascontiguousarray(ks.weights(), dtype=np.float64)
-    return None
+        w = np.ascontiguousarray(kweights, dtype=np.float64)
+    elif isinstance(ks, Mesh):
+        w = np.ascontiguousarray(ks.weights(), dtype=np.float64)
+    else:
+        return None
+    if Nk is not None and w.shape != (Nk,):
+        raise ValueError(f"kweights must have shape ({Nk},), got {w.shape}")
+    return w
 
 
 def fouriersum(H, ks, executor=None) -> np.ndarray:
@@ -265,7 +278,7 @@ def getdos(h, *args, klin: int | None = None, Gamma: float | None = None, kweigh
         ks = regulargrid(nk=klin ** 2)
     m = _model(h, executor)
     k = _kpts(ks, m.d)
-    w = _weights(ks, kweights)
+    w = _weights(ks, kweights, k.shape[1])
     out = np.zeros(len(om))
     _lib.call("lqm_dos", m.handle, k.shape[1], _vp(k), _vp(w), len(om), _vp(om), float(Gamma), _vp(out))
     return out
@@ -275,7 +288,7 @@ def chemicalpotential(H, ks, filling: float, T: float = 0.0, executor=None, kwei
     """mu such that the filling of the bands on ks equals ``filling``."""
     m = _model(H, executor)
     k = _kpts(ks, m.d)
-    w = _weights(ks, kweights)
+    w = _weights(ks, kweights, k.shape[1])
     mu = _c.c_double()
     _lib.call("lqm_chemicalpotential", m.handle, k.shape[1], _vp(k), _vp(w), float(filling), float(T),
               _c.byref(mu))
@@ -291,7 +304,7 @@ def getdensitymatrix(rhos: Hops, H, ks, mu: float = 0.0, T: float = 0.01, kweigh
     """
     m = _model(H, executor)
     k = _kpts(ks, m.d)
-    w = _weights(ks, kweights)
+    w = _weights(ks, kweights, k.shape[1])
     keys = list(rhos.keys())
     if keys and len(keys[0]) != m.d:
         raise ValueError("rho keys and model keys differ in dimension")
@@ -335,7 +348,7 @@ def grandpotential(H, ks, mu: float = 0.0, T: float = 0.05, kweights=None, execu
     dOmega[R][i, j] = dOmega/d(t_R)_ij = rho(-R)[j, i] (Hellmann-Feynman)."""
     m = _model(H, executor)
     k = _kpts(ks, m.d)
-    w = _weights(ks, kweights)
+    w = _weights(ks, kweights, k.shape[1])
     om = _c.c_double()
     g = np.zeros((m.N, m.N, m.nR), dtype=np.complex128, order="F") if gradient else None
     _lib.call("lqm_grandpotential", m.handle, k.shape[1], _vp(k), _vp(w), float(mu), float(T),
@@ -363,7 +376,7 @@ def densitymatrix_occ(rhos: Hops, H, ks, occ, kweights=None, executor=None) -> H
     weighting the ascending eigenvalues."""
     m = _model(H, executor)
     k = _kpts(ks, m.d)
-    w = _weights(ks, kweights)
+    w = _weights(ks, kweights, k.shape[1])
     g = np.asfortranarray(np.asarray(occ, dtype=np.float64))
     if g.shape != (m.N, k.shape[1]):
         raise ValueError(f"occupations must have shape (N, Nk) = {(m.N, k.shape[1])}")
@@ -386,7 +399,7 @@ def bandloss(H: Hops, ks, target, kweights=None, executor=None):
     m = _model(H, ex)
     bands, _ = bandmatrix(m, ks)
     Nk = bands.shape[1]
-    w = _weights(ks, kweights)
+    w = _weights(ks, kweights, Nk)
     wk = np.full(Nk, 1.0 / Nk) if w is None else w
     r = bands - np.asarray(target, dtype=np.float64)
     L = float(np.sum(wk[None, :] * r * r))

@@ -116,6 +116,7 @@ struct lqm_context {
                              // k-point sweeps run (sum over sweeps of the k-points refined),
                              // k-points whose rho_k the fused sweep formed
     const double* occ = nullptr;  // given occupations of the chunk in flight (device), or null
+    int last_prec = -1;      // precision the last pass ran in (-1: none yet)
 };
 
 struct lqm_model {
@@ -722,6 +723,13 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
                        double* evecs, const double* occ, const MuReq* mq = nullptr);
 bool use_blocked(const lqm_context* c, int N);
 
+// Start of a pass over the grid: clears the refinement report of the previous pass and
+// records the precision this pass runs in (reported by lqm_context_last_precision).
+void begin_pass(lqm_context* c, int prec) {
+    for (double& r : c->report) r = 0.0;
+    c->last_prec = prec;
+}
+
 // Clears the context's occupation pointer when a pass ends, on every return path.
 struct OccScope {
     lqm_context* c;
@@ -738,6 +746,7 @@ lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, doub
     // eigenvector output always comes from the FP64 path
     if (m->ctx->precision == LQM_PREC_FP32_REFINED && !evecs)
         return run_mixed(m, Nk, ks, nullptr, 0.0, 0.0, 0, nullptr, nullptr, nullptr, bands);
+    begin_pass(m->ctx, LQM_PREC_FP64);
     Plan P = make_plan(m->ctx, N, m->nR, Nk, evecs != nullptr, false);
     for (int k0 = 0; k0 < Nk; k0 += P.Kc) {
         const int Kc = std::min(P.Kc, Nk - k0);
@@ -783,6 +792,7 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
                          bands_out, occ);
     Plan P = make_plan(c, N, m->nR, Nk, true, true);
     if (mq && P.Kc < Nk) return kFallback;  // mu needs every eigenvalue before the first occupation
+    begin_pass(c, LQM_PREC_FP64);
     const int RG = contract_group(nRr);
     const int groups = std::max(1, (nRr + RG - 1) / RG);
     const int tilesE = (int)((NN + 255) / 256);
@@ -875,8 +885,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
     lqm_context* c = m->ctx;
     const int N = m->N, d = m->d;
     if (N > 512) return fail(LQM_ERR_UNSUPPORTED, "mixed precision path supports N <= 512");
-    c->report[4] = 0.0;
-    c->report[5] = 0.0;
+    begin_pass(c, LQM_PREC_FP32_REFINED);
     Plan P = make_plan(c, N, m->nR, Nk, true, false, sizeof(float));
     const size_t NN = (size_t)N * N;
     // refinement buffers: X, A, W, S, G (+ rho_k) in FP64 complex per k-point, X in FP32
@@ -1228,6 +1237,7 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
     if (want_rho) perk += 16 * NN;
     const int Kmax = (int)std::max<size_t>(1, std::min<size_t>({kBlockedBudget / perk, (size_t)Nk, (size_t)65535}));
     if (mq && Kmax < Nk) return kFallback;  // mu needs every eigenvalue before the first occupation
+    begin_pass(c, LQM_PREC_FP64);
     const double wuni = 1.0 / (double)Nk;
     lqm_status st;
     const int RG = contract_group(nRr);
@@ -1384,23 +1394,40 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
     return check_status(c, "blocked eigensolver");
 }
 
+// Split of the DOS reduction: S slices of esplit eigenvalues each (at most 1024 / tiles of 256
+// frequencies), so the partials S x Nw hold at most max(262144, Nw) doubles.
+struct DosSplit {
+    int tilesO, S, esplit;
+};
+DosSplit dos_split(int Ne, int Nw) {
+    DosSplit p;
+    p.tilesO = (Nw + 255) / 256;
+    p.S = std::max(1, std::min((Ne + 255) / 256, 1024 / p.tilesO));
+    p.esplit = (Ne + p.S - 1) / p.S;
+    p.S = (Ne + p.esplit - 1) / p.esplit;
+    return p;
+}
+
+// k_dos over the slices into part [S x Nw], then the fixed-order fold into dos.
+void dos_launch(lqm_context* c, int N, int Nk, const double* bands, const double* kw, int Nw,
+                const double* om, double gamma, const DosSplit& p, double* part, double* dos) {
+    const int Ne = N * Nk;
+    hipLaunchKernelGGL(lqm::k_dos, dim3(p.tilesO, p.S), dim3(256), 0, c->stream, N, Ne, bands, kw, Nw, om,
+                       gamma, p.esplit, part);
+    const double scale = kw ? 1.0 : 1.0 / (double)Nk;
+    hipLaunchKernelGGL(lqm::k_dos_fold, dim3(p.tilesO), dim3(256), 0, c->stream, Nw, p.S, part, scale,
+                       dos);
+}
+
 lqm_status dos_bands_dev(lqm_context* c, int N, int Nk, const double* bands, const double* kw,
                          int Nw, const double* om, double gamma, double* dos) {
     if (Nw <= 0) return LQM_OK;
-    const int Ne = N * Nk;
-    const int tilesO = (Nw + 255) / 256;
-    int S = std::max(1, std::min((Ne + 255) / 256, 1024 / tilesO));
-    int esplit = (Ne + S - 1) / S;
-    S = (Ne + esplit - 1) / esplit;
+    const DosSplit p = dos_split(N * Nk, Nw);
     double* part;
-    lqm_status st = wsT(c, W_PART, (size_t)S * Nw + 1, &part);
+    lqm_status st = wsT(c, W_PART, (size_t)p.S * Nw + 1, &part);
     if (st) return st;
     StageScope sc(c, ST_DOS);
-    hipLaunchKernelGGL(lqm::k_dos, dim3(tilesO, S), dim3(256), 0, c->stream, N, Ne, bands, kw, Nw, om,
-                       gamma, esplit, part);
-    const double scale = kw ? 1.0 : 1.0 / (double)Nk;
-    hipLaunchKernelGGL(lqm::k_dos_fold, dim3(tilesO), dim3(256), 0, c->stream, Nw, S, part, scale,
-                       dos);
+    dos_launch(c, N, Nk, bands, kw, Nw, om, gamma, p, part, dos);
     LQM_HIP(hipGetLastError());
     return LQM_OK;
 }
@@ -1793,6 +1820,12 @@ lqm_status lqm_context_set_variant(lqm_context* c, int kind, int variant) {
         variant > (kind == VK_TRIDIAG || kind == VK_REFINE ? 2 : kind == VK_QL ? 3 : 1))
         return fail(LQM_ERR_ARG, "bad variant");
     c->forced[kind] = variant;
+    return LQM_OK;
+}
+
+lqm_status lqm_context_last_precision(lqm_context* c, int* precision) {
+    if (!c || !precision) return fail(LQM_ERR_ARG, "null argument");
+    *precision = c->last_prec;
     return LQM_OK;
 }
 

@@ -127,7 +127,7 @@ def solveselfconsistent(rho_init: Hops, mf: HartreeFock, filling: float, ks=None
         ks = regulargrid(nk=klin ** mf.d, dim=mf.d)
     rho0 = sanitize(rho_init.copy())
     k = _kpts(ks, mf.d)
-    w = _weights(ks, kweights)
+    w = _weights(ks, kweights, k.shape[1])
     Rh, Th, Rv, Tv, nRv = mf._hv()
     Rr, Tr = _pack_keys(rho0, mf.N, mf.d)
     nK, RK, hmf = mf._outbufs()
@@ -322,7 +322,7 @@ def solvehartreefock_batch(problems, T: float = 0.0, beta: float = 1.0, tol: flo
             Rv, Tv = np.zeros((d, 1), np.int32), np.zeros((N, N, 1), np.complex128, order="F")
         Rr, Tr = _pack_keys(rho0, N, d)
         k = _kpts(ks, d)
-        w = _weights(ks, None)
+        w = _weights(ks, None, k.shape[1])
         keep.append((Rh, Th, Rv, Tv, Rr, Tr, k, w))
         p = arr[i]
         p.N, p.nRh, p.Rh, p.Th = N, Rh.shape[1], Rh.ctypes.data, Th.ctypes.data

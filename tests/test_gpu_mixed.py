@@ -36,7 +36,12 @@ def test_mixed_bands(pkg, oracle, mixed, N, rmax, nk):
     err = np.abs(b - ref) / np.maximum(np.abs(ref), _hnorm(h, ks)[None, :])
     assert np.max(err) < 1e-10, np.max(err)
     rep = mixed.refinement_report()
-    assert rep["sweeps"] >= 1 and rep["max_orth_residual"] < 1e-3
+    if mixed.last_precision() == "mixed":
+        assert rep["sweeps"] >= 1 and rep["max_orth_residual"] < 1e-3
+    else:
+        # the pass ran FP64 (large-N layout): the report holds no stale refinement figures
+        assert N > 112 and mixed.last_precision() == "fp64"
+        assert all(v == 0 for v in rep.values()), rep
 
 
 @pytest.mark.parametrize("N,nk,mu,T", [(4, 64, 0.2, 0.0), (8, 49, 0.0, 0.05), (64, 64, 0.0, 0.02), (96, 9, 0.1, 0.02)])
