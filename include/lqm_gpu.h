@@ -181,6 +181,17 @@ lqm_status lqm_dos_from_bands_dev(lqm_context *ctx, int N, int Nk, const double 
                                   const double *kweights_dev, int Nw, const double *omegas_dev,
                                   double gamma, double *dos_dev);
 
+/* Allocation-free variant of lqm_dos_from_bands_dev.  No reference counterpart: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same kernels and result; the DOS partials live in
+ * a fixed device array of 1024 x 256 doubles (2 MiB, held by each device the library runs on
+ * until the process ends) instead of the context's workspace, and the call makes no
+ * hipMalloc / hipFree.  Nw above 262144 returns LQM_ERR_UNSUPPORTED.  Every call on one device
+ * shares that array: calls that overlap on one device (several contexts or streams) race. */
+lqm_status lqm_dos_from_bands_dev_noalloc(lqm_context *ctx, int N, int Nk, const double *bands_dev,
+                                          const double *kweights_dev, int Nw,
+                                          const double *omegas_dev, double gamma, double *dos_dev);
+
 /* ---- chemical potential ----------------------------------------------------------- */
 /* chemicalpotential(H, ks, filling; T) (Spectrum/chemicalpotential.jl:6-65):
  * T == 0: mean of the i-th and (i+1)-th smallest of all N*Nk eigenvalues with

@@ -55,6 +55,7 @@ _PROTOS = {
     "lqm_bandmatrix_dev": [P, I, P, P, P],
     "lqm_dos": [P, I, P, P, I, P, D, P],
     "lqm_dos_from_bands_dev": [P, I, I, P, P, I, P, D, P],
+    "lqm_dos_from_bands_dev_noalloc": [P, I, I, P, P, I, P, D, P],
     "lqm_chemicalpotential": [P, I, P, P, D, D, P],
     "lqm_chemicalpotential_from_bands_dev": [P, I, I, P, P, D, D, P],
     "lqm_densitymatrix": [P, I, P, P, D, D, I, P, P, P],

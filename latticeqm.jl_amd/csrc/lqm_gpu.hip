@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -60,6 +61,35 @@ constexpr double kTriClusterRel = 1e-3;
 // wave of three 256-thread blocks per CU on 256 CUs
 constexpr int kContractBlocks = 768;
 
+}  // namespace
+
+// Device scratch of the allocation-free entry point lqm_dos_from_bands_dev_noalloc: the
+// S x Nw DOS partials, at most 1024 x 256 doubles (2 MiB) for any Nw <= kScratchDoubles.
+// Each device holds its own copy for the life of the process.  In host code the name is a
+// host-side stand-in: kernels receive the device address that hipGetSymbolAddress returns,
+// resolved once per device (scratch_address) when a context is created on it.
+constexpr size_t kScratchDoubles = size_t(1024) * 256;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of this device's scratch array; call with `device` current.
+lqm_status scratch_address(int device, double** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(LQM_ERR_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(LQM_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return LQM_OK;
+}
 }  // namespace
 
 // Kernel stages timed with HIP events when timing is on.
@@ -117,6 +147,7 @@ struct lqm_context {
                              // k-points whose rho_k the fused sweep formed
     const double* occ = nullptr;  // given occupations of the chunk in flight (device), or null
     int last_prec = -1;      // precision the last pass ran in (-1: none yet)
+    double* scratch = nullptr;  // this device's scratch array (device address)
 };
 
 struct lqm_model {
@@ -1735,6 +1766,11 @@ lqm_status lqm_context_create(int device, int precision, lqm_context** out) {
         delete c;
         return fail(LQM_ERR_DEVICE, std::string("context init: ") + hipGetErrorString(e));
     }
+    lqm_status sst = scratch_address(device, &c->scratch);
+    if (sst) {
+        lqm_context_destroy(c);
+        return sst;
+    }
     c->stream = c->own;
     *out = c;
     return LQM_OK;
@@ -1942,6 +1978,26 @@ lqm_status lqm_dos_from_bands_dev(lqm_context* c, int N, int Nk, const double* b
     if (N <= 0 || Nk <= 0 || Nw < 0) return fail(LQM_ERR_ARG, "bad sizes");
     lqm_status st = dos_bands_dev(c, N, Nk, bands_dev, kweights_dev, Nw, omegas_dev, gamma, dos_dev);
     if (st) return st;
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    return LQM_OK;
+}
+
+lqm_status lqm_dos_from_bands_dev_noalloc(lqm_context* c, int N, int Nk, const double* bands_dev,
+                                          const double* kweights_dev, int Nw, const double* omegas_dev,
+                                          double gamma, double* dos_dev) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    if (N <= 0 || Nk <= 0 || Nw < 0) return fail(LQM_ERR_ARG, "bad sizes");
+    if ((long long)N * Nk > INT32_MAX) return fail(LQM_ERR_UNSUPPORTED, "N * Nk above 2^31 - 1");
+    if ((size_t)Nw > kScratchDoubles)
+        return fail(LQM_ERR_UNSUPPORTED, "Nw above " + std::to_string(kScratchDoubles) +
+                                             ", the frequencies the scratch partials hold");
+    if (!c->scratch) return fail(LQM_ERR_DEVICE, "scratch address not resolved for this device");
+    if (Nw == 0) return LQM_OK;
+    if (!bands_dev || !omegas_dev || !dos_dev) return fail(LQM_ERR_ARG, "null buffer");
+    const DosSplit p = dos_split(N * Nk, Nw);
+    if ((size_t)p.S * Nw > kScratchDoubles) return fail(LQM_ERR_UNSUPPORTED, "DOS partials exceed the scratch");
+    dos_launch(c, N, Nk, bands_dev, kweights_dev, Nw, omegas_dev, gamma, p, c->scratch, dos_dev);
+    LQM_HIP(hipGetLastError());
     LQM_HIP(hipStreamSynchronize(c->stream));
     return LQM_OK;
 }
