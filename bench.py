@@ -81,10 +81,26 @@ def scf_leg(no_cpu: bool) -> dict:
     t = (time.perf_counter() - t0) / reps
     nit = m.history.shape[0]
     A = mf.magnetization(rho, mf.sublatticeprojection(sub, 0))
+    # outputs of the converged mean field: DOS on 801 frequencies over [-4, 4] from mu
+    # (Lorentzian of width 0.02 without 1/pi, the getdos kernel) and bands on Gamma-K-M-Gamma
+    om = np.linspace(-4.0, 4.0, 801) + m.mu
+    t1 = time.perf_counter()
+    dos = api.getdos(m.hMF, om, ks, Gamma=0.02, executor=ex)
+    path = pkg.kpath(np.array([[0, 0], [1 / 3, -1 / 3], [0.5, 0], [0, 0]]).T, num_points=301)
+    b, _ = api.bandmatrix(m.hMF, path, executor=ex)
+    t_out = time.perf_counter() - t1
     out = {"workload": "honeycomb_hubbard_U3_k60x60_beta0.5_tol1e-8", "N": 4, "Nk": ks.shape[1],
            "iterations": nit, "converged": conv, "residual": res, "E_GS": E, "mu": m.mu,
            "sublattice_A_magnetization": A.tolist(), "seconds_per_scf": t,
-           "iterations_per_s": nit / t}
+           "iterations_per_s": nit / t,
+           "outputs": {"rho_keys": [list(R) for R in rho.keys()],
+                       "dos": {"omega_minus_mu": [-4.0, 4.0, 801], "gamma": 0.02,
+                               "kernel": "Lorentzian without 1/pi (getdos)",
+                               "integral": float(np.trapezoid(dos, om)), "max": float(np.max(dos)),
+                               "at_mu": float(dos[400])},
+                       "bands": {"path": "Gamma-K-M-Gamma (fractional)", "points": int(path.shape[1]),
+                                 "gap_at_half_filling": float(np.min(b[2]) - np.max(b[1]))},
+                       "seconds_dos_and_bands": t_out}}
     if not no_cpu:
         from oracle import hartreefock
 

@@ -7,7 +7,7 @@ kernels in liblqmgpu.so (C ABI: include/lqm_gpu.h).
 The directory name contains a dot, so it is loaded by path; the loaders in this
 repository register it as ``latticeqm_amd``.
 """
-from .hops import Hops, Mesh, points, regulargrid  # noqa: F401
+from .hops import Hops, Mesh, kpath, points, regulargrid  # noqa: F401
 from . import models  # noqa: F401
 from ._lib import header_symbols  # noqa: F401
 

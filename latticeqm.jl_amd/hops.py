@@ -146,6 +146,24 @@ def regulargrid(nk: int = 100, dim: int = 2) -> np.ndarray:
     return np.asfortranarray(pts)
 
 
+def kpath(corners, num_points: int = 60, B=None) -> np.ndarray:
+    """Piecewise-linear path through the fractional k-points ``corners`` (d x m), about
+    ``num_points`` points in all, spread over the segments in proportion to their length
+    measured with the reciprocal basis B (identity: fractional lengths), both ends
+    included (the getpath sampling behind Structure.kpath, Structure/Paths/path.jl:22-28)."""
+    c = np.asarray(corners, dtype=np.float64)
+    Bm = np.eye(c.shape[0]) if B is None else np.asarray(B, dtype=np.float64)
+    seg = [np.linalg.norm(Bm @ (c[:, i + 1] - c[:, i])) for i in range(c.shape[1] - 1)]
+    total = sum(seg) or 1.0
+    pts = []
+    for i, L in enumerate(seg):
+        n = max(1, int(round((num_points - 1) * L / total)))
+        t = np.arange(n) / n
+        pts.append(c[:, i:i + 1] + t[None, :] * (c[:, i + 1] - c[:, i])[:, None])
+    pts.append(c[:, -1:])
+    return np.asfortranarray(np.concatenate(pts, axis=1))
+
+
 class Mesh:
     """k-points with weights (Structure/grid.jl:2-20)."""
 
