@@ -147,6 +147,23 @@ def adjoint_leg(no_cpu: bool) -> dict:
     out = {"workload": "random_tb_N128_R9_k64x64_grandpotential", "N": N, "Nk": Nk, "omega": om.value,
            "seconds_per_evaluation": t, "kpoints_per_s": Nk / t,
            "path_tflops_per_s": Nk * ((64.0 / 3.0) * N ** 3 + 8.0 * 18 * N * N) / t / 1e12}
+    # 200 plain gradient steps (step 0.3) fitting the 9 blocks to the bands of a second seeded
+    # model (seed 6) through the band-loss pullback: per step an eigenvalue pass and a weighted
+    # density-matrix pass on the keys
+    target, _ = api.bandmatrix(pkg.models.random_tb(N, rmax=1, seed=6), ks)
+    hf = pkg.models.random_tb(N, rmax=1, seed=5)
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(200):
+        Lf, cot = api.bandloss(hf, ks, target)
+        losses.append(Lf)
+        for key in hf.keys():
+            hf[key] = hf[key] - 0.3 * cot[key]
+    tfit = time.perf_counter() - t0
+    out["fit_200_steps"] = {"seconds": tfit, "steps_per_s": 200 / tfit, "loss_first": losses[0],
+                            "loss_last": losses[-1],
+                            "monotone": bool(all(b <= a * (1 + 1e-12) for a, b in zip(losses, losses[1:])))}
     if not no_cpu:
         from oracle import oracle
 

@@ -248,6 +248,36 @@ lqm_status lqm_densitymatrix_occ_dev(lqm_model *m, int Nk, const double *ks_dev,
                                      const double *kweights_dev, const double *occ_dev, int nRr,
                                      const int32_t *Rr, double *rho_dev, double *bands_dev);
 
+/* ---- Bogoliubov-de Gennes (Nambu) operators ----------------------------------------- */
+/* m holds the Nambu model H_BdG at mu = 0 (BdGOperator, Superconductivity/types.jl:43-107):
+ * N = 2d, electrons in rows 0..d-1, holes in d..2d-1, and it must have the zero key.  A
+ * chemical potential mu enters as H_BdG - mu tau_z on the zero key (addchemicalpotential!
+ * with -mu, Superconductivity/operators.jl:14-17), written on the device from a saved copy
+ * of the block and undone before the call returns.
+ *
+ * getdensitymatrix!(rho::BdGOperator, H::BdGOperator, ks, [kweights,] mu; T)
+ * (Superconductivity/green.jl:38-54): rho [N x N x nRr] on the keys Rr is overwritten with
+ * the Nambu density matrix of H - mu tau_z; *ekin (may be NULL) receives the physical
+ * <H - mu N> = E_band / 2 + (Re tr h_e(0) - d mu) / 2, h_e(0) the electron block of the zero
+ * key. */
+lqm_status lqm_bdg_densitymatrix(lqm_model *m, int Nk, const double *ks, const double *kweights,
+                                 double mu, double T, int nRr, const int32_t *Rr, double *rho,
+                                 double *ekin);
+/* Spectrum.filling(H::BdGOperator, ks, mu; T) (Superconductivity/operators.jl:107-112):
+ * Re tr(rho_el(0)) / d of the full Nambu density matrix at mu (one density-matrix pass). */
+lqm_status lqm_bdg_filling(lqm_model *m, int Nk, const double *ks, const double *kweights,
+                           double mu, double T, double *filling);
+/* Spectrum.chemicalpotential(H::BdGOperator, ks, target; T) (Superconductivity/operators.jl:
+ * 146-162): the root of filling(mu) - target on the bracket [lo, hi] by Roots.jl's Float64
+ * bisection (bit-pattern midpoints down to adjacent doubles), one full Nambu density-matrix
+ * pass per evaluation, all on the device (one scalar comes back per pass).  The caller
+ * chooses the bracket as the reference does (extrema of the electron-sector bands +- 1,
+ * clipped to [mu_lo, mu_hi]).  *passes (may be NULL) receives the passes run;
+ * LQM_ERR_ARG when the target is not bracketed. */
+lqm_status lqm_bdg_chemicalpotential(lqm_model *m, int Nk, const double *ks, const double *kweights,
+                                     double target, double T, double lo, double hi, double *mu,
+                                     int *passes);
+
 /* ---- Hartree-Fock mean field and self-consistency ----------------------------------- */
 /* Flags of the HF functional: which channels HartreeFock(h, v; hartree, fock) keeps
  * (Meanfield/hartreefock.jl:158). */
@@ -296,6 +326,23 @@ lqm_status lqm_hartreefock_scf(lqm_context *ctx, int N, int d, int nRh, const in
                                double tol, int max_iter, int flags, double *result,
                                int *converged, double *history, int *nK_out, int32_t *RK_out,
                                double *hmf_out);
+
+/* Grand potential of the mean field and its pullback onto the mean-field parameters (no
+ * reference counterpart; the rrule of Omega(h, v) = Omega[H_MF(rho; h, v)] at fixed rho).
+ * H_MF = HartreeFock(h, v)(rho) as in lqm_hartreefock_meanfield; *omega = Omega of H_MF at
+ * (mu, T) as in lqm_grandpotential.  grad_h [N x N x nRh] and grad_v [N x N x nRv] (either
+ * may be NULL) receive dOmega/d(h_R)_ij = G_R,ij and
+ *   dOmega/d(v_L)_ij = -G_L,ij rho_L,ij (Fock, L a key of rho) + G_0,ii n_j (Hartree),
+ * with G_q = rho_MF(-q)^T the grand-potential gradient of H_MF and n = diag(rho(0)): the
+ * derivative along a perturbation that keeps H_MF Hermitian is sum_ij G_ij D_ij over the
+ * perturbed blocks (for the onsite Hubbard form v(0) = U (I (x) sigma_x), dOmega/dU is
+ * sum_ij grad_v(0)_ij (I (x) sigma_x)_ij). */
+lqm_status lqm_hartreefock_grandpotential(lqm_context *ctx, int N, int d, int nRh, const int32_t *Rh,
+                                          const double *Th, int nRv, const int32_t *Rv,
+                                          const double *Tv, int nRr, const int32_t *Rr,
+                                          const double *rho, int flags, int Nk, const double *ks,
+                                          const double *kweights, double mu, double T,
+                                          double *omega, double *grad_h, double *grad_v);
 
 /* Many independent Hartree-Fock problems of uneven size (no reference counterpart: the
  * reference runs models one at a time).  Each problem is lqm_hartreefock_scf on its own

@@ -67,6 +67,10 @@ _PROTOS = {
     "lqm_densitymatrix_occ": [P, I, P, P, P, I, P, P],
     "lqm_densitymatrix_occ_dev": [P, I, P, P, P, I, P, P, P],
     "lqm_hartreefock_scf_batch": [P, I, I, P, D, D, D, I, I, I],
+    "lqm_bdg_densitymatrix": [P, I, P, P, D, D, I, P, P, P],
+    "lqm_hartreefock_grandpotential": [P, I, I, I, P, P, I, P, P, I, P, P, I, I, P, P, D, D, P, P, P],
+    "lqm_bdg_filling": [P, I, P, P, D, D, P],
+    "lqm_bdg_chemicalpotential": [P, I, P, P, D, D, D, D, P, P],
     "lqm_hartreefock_meanfield": [P, I, I, I, P, P, I, P, P, I, P, P, I, P, P, P, P],
     "lqm_hartreefock_scf": [P, I, I, I, P, P, I, P, P, I, P, P, I, P, P, D, D, D, D, I, I, P, P, P,
                             P, P, P],

@@ -370,4 +370,33 @@ __global__ __launch_bounds__(256) void k_expval_cols(int N, const cx<double>* __
     }
 }
 
+// Nambu (BdG) chemical potential shift (Superconductivity/operators.jl:14-17 applied with -mu,
+// as getdensitymatrix!(::BdGOperator) does, green.jl:38-54): the zero-key block of the 2d x 2d
+// Nambu model becomes base - mu tau_z, i.e. -mu on the electron diagonal (i < d) and +mu on
+// the hole diagonal, written from the unshifted base block so repeated shifts never drift.
+__global__ __launch_bounds__(256) void k_bdg_shift(int N, const cx<double>* __restrict__ base, double mu,
+                                                   cx<double>* __restrict__ blk) {
+    const size_t NN = (size_t)N * N;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= NN) return;
+    cx<double> v = base[idx];
+    const size_t i = idx % N, j = idx / N;
+    if (i == j) v.re += (int)i < N / 2 ? -mu : mu;
+    blk[idx] = v;
+}
+
+// Electron filling of a Nambu density matrix (Superconductivity/operators.jl:107-112):
+// out = Re tr(rho[1:d, 1:d]) / d of the zero-key block rho (N = 2d, column-major).  One block.
+__global__ __launch_bounds__(256) void k_bdg_filling(int N, const cx<double>* __restrict__ rho,
+                                                     double* __restrict__ out) {
+    __shared__ double sred[4];
+    const int d = N / 2;
+    double a = 0.0;
+    for (int i = threadIdx.x; i < d; i += 256) a += rho[(size_t)i * N + i].re;
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x / 64] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = (sred[0] + sred[1] + sred[2] + sred[3]) / (double)d;
+}
+
 }  // namespace lqm

@@ -50,6 +50,38 @@ __global__ __launch_bounds__(256) void k_hf_build(int N, int nK, const int* __re
     hmf[idx] = val;
 }
 
+// Pullback of the mean field onto its parameters at fixed rho.  G[q] = dOmega/dH_MF[q] for
+// every mean-field key q (element-wise derivative, [G_q]_ij = rho_MF(-q)_ji); since
+//   H_MF[q] = h[q] - v[q] .* rho[q] (Fock) + delta_{q0} diag(V0 n) (Hartree), V0 = sum_L v_L,
+// the derivatives are gh[r] = G[hq[r]] for the keys of h and
+//   gv[l]_ij = -G[vq[l]]_ij rho[vr[l]]_ij  (Fock, keys of v that rho has)
+//            + G[qz]_ii n_j                (Hartree, every key of v; n = diag(rho[rz])).
+// gh and gv may be null.  One thread per element of the nRh + nRv output blocks.
+__global__ __launch_bounds__(256) void k_hf_adjoint(int N, int nRh, int nRv, const int* __restrict__ hq,
+                                                    const int* __restrict__ vq, const int* __restrict__ vr,
+                                                    int qz, int rz, int flags, const cx<double>* __restrict__ G,
+                                                    const cx<double>* __restrict__ rho,
+                                                    cx<double>* __restrict__ gh, cx<double>* __restrict__ gv) {
+    const size_t NN = (size_t)N * N;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= NN * (size_t)(nRh + nRv)) return;
+    const int b = (int)(idx / NN);
+    const size_t e = idx - (size_t)b * NN;
+    if (b < nRh) {
+        if (gh) gh[idx] = G[(size_t)hq[b] * NN + e];
+        return;
+    }
+    if (!gv) return;
+    const int l = b - nRh;
+    cx<double> g{0.0, 0.0};
+    if ((flags & 2) && vr[l] >= 0) g = g - G[(size_t)vq[l] * NN + e] * rho[(size_t)vr[l] * NN + e];
+    if ((flags & 1) && rz >= 0 && qz >= 0) {
+        const size_t i = e % N, j = e / N;
+        g = g + G[(size_t)qz * NN + i * (N + 1)] * rho[(size_t)rz * NN + j * (N + 1)];
+    }
+    gv[(size_t)l * NN + e] = g;
+}
+
 // Block-wide sum (blockDim = 256 * W), result returned to every thread.
 template <int W>
 __device__ __forceinline__ double block_sum(double a, double* sred) {

@@ -50,7 +50,7 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -1731,6 +1731,87 @@ lqm_status hf_outputs(HFState& S, int* nK_out, int32_t* RK_out, double* hmf_out)
     return LQM_OK;
 }
 
+// ---- BdG (Nambu) operators: the chemical potential enters through the zero key -------
+// The model holds H_BdG at mu = 0 in Nambu order (electrons 0..d-1, holes d..2d-1, N = 2d).
+// A pass at mu rewrites the zero-key block as base - mu tau_z from a saved copy of it (no
+// drift across the many shifts of a root search) and the scope restores the copy on exit.
+struct BdgScope {
+    lqm_model* m = nullptr;
+    int rz = -1;
+    cx<double>* base = nullptr;
+    ~BdgScope() {
+        if (m && base && rz >= 0) {
+            const size_t NN = (size_t)m->N * m->N;
+            (void)hipMemcpyAsync(m->dT + (size_t)rz * NN, base, sizeof(cx<double>) * NN, hipMemcpyDeviceToDevice,
+                                 m->ctx->stream);
+            (void)hipStreamSynchronize(m->ctx->stream);
+        }
+    }
+};
+
+lqm_status bdg_begin(lqm_model* m, BdgScope& S) {
+    if (m->N % 2) return fail(LQM_ERR_ARG, "a Nambu model has even dimension 2d");
+    for (int r = 0; r < m->nR; ++r) {
+        bool z = true;
+        for (int i = 0; i < m->d; ++i) z = z && m->R[(size_t)r * m->d + i] == 0;
+        if (z) {
+            S.rz = r;
+            break;
+        }
+    }
+    if (S.rz < 0) return fail(LQM_ERR_ARG, "the Nambu model needs the zero key (chemical potential)");
+    const size_t NN = (size_t)m->N * m->N;
+    cx<double>* base;
+    lqm_status st = wsT(m->ctx, W_BDGB, NN, &base);
+    if (st) return st;
+    LQM_HIP(hipMemcpyAsync(base, m->dT + (size_t)S.rz * NN, sizeof(cx<double>) * NN, hipMemcpyDeviceToDevice,
+                           m->ctx->stream));
+    S.m = m;
+    S.base = base;
+    return LQM_OK;
+}
+
+lqm_status bdg_shift(const BdgScope& S, double mu) {
+    const size_t NN = (size_t)S.m->N * S.m->N;
+    hipLaunchKernelGGL(lqm::k_bdg_shift, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, S.m->ctx->stream, S.m->N,
+                       S.base, mu, S.m->dT + (size_t)S.rz * NN);
+    LQM_HIP(hipGetLastError());
+    return LQM_OK;
+}
+
+// Spectrum.filling(::BdGOperator) (Superconductivity/operators.jl:107-112): one full
+// density-matrix pass of H - mu tau_z on the zero key, then Re tr(rho_el(0)) / d.
+lqm_status bdg_filling(const BdgScope& S, int Nk, const double* ks, const double* kw, double mu, double T,
+                       double* filling) {
+    lqm_model* m = S.m;
+    lqm_context* c = m->ctx;
+    const size_t NN = (size_t)m->N * m->N;
+    double* buf;
+    lqm_status st = wsT(c, W_BDGR, 2 * NN + 4, &buf);
+    if (st) return st;
+    if ((st = bdg_shift(S, mu))) return st;
+    std::vector<int32_t> R0(m->d, 0);
+    if ((st = run_rho(m, Nk, ks, kw, 0.0, T, 1, R0.data(), buf, buf + 2 * NN, nullptr))) return st;
+    hipLaunchKernelGGL(lqm::k_bdg_filling, dim3(1), dim3(256), 0, c->stream, m->N,
+                       reinterpret_cast<const cx<double>*>(buf), buf + 2 * NN + 1);
+    LQM_HIP(hipGetLastError());
+    return d2h(c, filling, buf + 2 * NN + 1, sizeof(double));
+}
+
+// Roots.jl's bracketing midpoint for Float64: 0 across a sign change, else the midpoint of
+// the IEEE bit patterns of |a| and |b| carrying the sign of a + b (at most 64 halvings).
+double roots_middle(double a, double b) {
+    if ((a < 0.0) != (b < 0.0) && a != 0.0 && b != 0.0) return 0.0;
+    uint64_t ia, ib;
+    const double aa = std::fabs(a), ab = std::fabs(b);
+    std::memcpy(&ia, &aa, 8);
+    std::memcpy(&ib, &ab, 8);
+    const uint64_t im = (ia >> 1) + (ib >> 1) + (ia & ib & 1);
+    double mid;
+    std::memcpy(&mid, &im, 8);
+    return (a + b) < 0.0 ? -mid : mid;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2251,6 +2332,54 @@ lqm_status lqm_hartreefock_scf(lqm_context* ctx, int N, int d, int nRh, const in
     return hf_outputs(S, nK_out, RK_out, hmf_out);
 }
 
+lqm_status lqm_hartreefock_grandpotential(lqm_context* ctx, int N, int d, int nRh, const int32_t* Rh,
+                                          const double* Th, int nRv, const int32_t* Rv, const double* Tv,
+                                          int nRr, const int32_t* Rr, const double* rho, int flags, int Nk,
+                                          const double* ks, const double* kweights, double mu, double T,
+                                          double* omega, double* grad_h, double* grad_v) {
+    if (Nk <= 0 || !ks) return fail(LQM_ERR_ARG, "Nk must be positive");
+    if (!(T >= 0.0)) return fail(LQM_ERR_ARG, "negative temperature");
+    if (!omega) return fail(LQM_ERR_ARG, "null omega");
+    HFState S;
+    lqm_status st = hf_setup(S, ctx, N, d, nRh, Rh, Th, nRv, Rv, Tv, nRr, Rr, rho, flags, 0);
+    if (st) return st;
+    lqm_context* c = ctx;
+    if ((st = hf_apply(S, S.rho0))) return st;
+    void *dks, *dkw = nullptr;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    const size_t NN = (size_t)N * N;
+    const bool want = grad_h || grad_v;
+    cx<double>* G = nullptr;
+    if (want && (st = wsT(c, W_HFG, NN * S.nK, &G))) return st;
+    if ((st = lqm_grandpotential_dev(S.hmf, Nk, (const double*)dks, (const double*)dkw, mu, T, omega,
+                                     reinterpret_cast<double*>(G))))
+        return st;
+    if (!want) return LQM_OK;
+    // key maps: h key r and v key l -> their mean-field key
+    std::vector<int> maps((size_t)nRh + 2 * std::max(nRv, 1), 0);
+    for (int q = 0; q < S.nK; ++q) {
+        if (S.map[3 * q] >= 0) maps[S.map[3 * q]] = q;
+        if (S.map[3 * q + 1] >= 0) maps[nRh + S.map[3 * q + 1]] = q;
+    }
+    for (int l = 0; l < nRv; ++l) maps[(size_t)nRh + std::max(nRv, 1) + l] = S.vr[l];
+    const size_t nout = NN * ((size_t)nRh + std::max(nRv, 0));
+    cx<double>* out;
+    if ((st = wsT(c, W_HFA, nout + (maps.size() * sizeof(int) + 15) / 16 + 1, &out))) return st;
+    int* dmaps = reinterpret_cast<int*>(out + nout);
+    LQM_HIP(hipMemcpyAsync(dmaps, maps.data(), sizeof(int) * maps.size(), hipMemcpyHostToDevice, c->stream));
+    {
+        StageScope sc(c, ST_HF);
+        hipLaunchKernelGGL(lqm::k_hf_adjoint, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, c->stream, N, nRh,
+                           std::max(nRv, 0), dmaps, dmaps + nRh, dmaps + nRh + std::max(nRv, 1), S.qz, S.rz, S.flags,
+                           G, S.rho0, out, out + NN * nRh);
+    }
+    LQM_HIP(hipGetLastError());
+    if (grad_h && (st = d2h(c, grad_h, out, sizeof(cx<double>) * NN * nRh))) return st;
+    if (grad_v && nRv > 0 && (st = d2h(c, grad_v, out + NN * nRh, sizeof(cx<double>) * NN * nRv))) return st;
+    return LQM_OK;
+}
+
 lqm_status lqm_densitymatrix_occ_dev(lqm_model* m, int Nk, const double* ks_dev,
                                      const double* kweights_dev, const double* occ_dev, int nRr,
                                      const int32_t* Rr, double* rho_dev, double* bands_dev) {
@@ -2409,6 +2538,105 @@ lqm_status lqm_bandmatrix_expval(lqm_model* m, int Nk, const double* ks, int nP,
         return st;
     if ((st = d2h(c, bands, db, sizeof(double) * N * Nk))) return st;
     if (nP > 0) return d2h(c, obs, dobs, sizeof(double) * N * Nk * nP);
+    return LQM_OK;
+}
+
+lqm_status lqm_bdg_filling(lqm_model* m, int Nk, const double* ks, const double* kweights, double mu,
+                           double T, double* filling) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0 || !ks || !filling) return fail(LQM_ERR_ARG, "bad k-points or null output");
+    if (!(T >= 0.0)) return fail(LQM_ERR_ARG, "negative temperature");
+    lqm_context* c = m->ctx;
+    void *dks, *dkw = nullptr;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    BdgScope S;
+    if ((st = bdg_begin(m, S))) return st;
+    return bdg_filling(S, Nk, (const double*)dks, (const double*)dkw, mu, T, filling);
+}
+
+lqm_status lqm_bdg_chemicalpotential(lqm_model* m, int Nk, const double* ks, const double* kweights,
+                                     double target, double T, double lo, double hi, double* mu,
+                                     int* passes) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0 || !ks || !mu) return fail(LQM_ERR_ARG, "bad k-points or null output");
+    if (!(T >= 0.0)) return fail(LQM_ERR_ARG, "negative temperature");
+    if (!(lo < hi) || !std::isfinite(lo) || !std::isfinite(hi)) return fail(LQM_ERR_ARG, "bad bracket");
+    lqm_context* c = m->ctx;
+    void *dks, *dkw = nullptr;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    BdgScope S;
+    if ((st = bdg_begin(m, S))) return st;
+    int np = 0;
+    auto f = [&](double x, double* out) {
+        double fl = 0.0;
+        lqm_status s2 = bdg_filling(S, Nk, (const double*)dks, (const double*)dkw, x, T, &fl);
+        ++np;
+        *out = fl - target;
+        if (!s2 && !std::isfinite(*out)) return fail(LQM_ERR_NONFINITE, "non-finite BdG filling");
+        return s2;
+    };
+    double a = lo, b = hi, fa, fb, fm;
+    if ((st = f(a, &fa)) || (st = f(b, &fb))) return st;
+    if (passes) *passes = np;
+    if (fa == 0.0) return (*mu = a, LQM_OK);
+    if (fb == 0.0) return (*mu = b, LQM_OK);
+    if ((fa < 0.0) == (fb < 0.0)) return fail(LQM_ERR_ARG, "the target filling is not bracketed by [lo, hi]");
+    for (int it = 0; it < 200; ++it) {
+        const double x = roots_middle(a, b);
+        if (!(a < x && x < b)) break;
+        if ((st = f(x, &fm))) return st;
+        if (fm == 0.0) {
+            a = b = x;
+            fa = fb = 0.0;
+            break;
+        }
+        if ((fm < 0.0) == (fa < 0.0)) {
+            a = x;
+            fa = fm;
+        } else {
+            b = x;
+            fb = fm;
+        }
+    }
+    *mu = std::fabs(fa) <= std::fabs(fb) ? a : b;
+    if (passes) *passes = np;
+    return LQM_OK;
+}
+
+lqm_status lqm_bdg_densitymatrix(lqm_model* m, int Nk, const double* ks, const double* kweights, double mu,
+                                 double T, int nRr, const int32_t* Rr, double* rho, double* ekin) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0 || !ks) return fail(LQM_ERR_ARG, "Nk must be positive");
+    if (nRr < 0 || (nRr > 0 && (!Rr || !rho))) return fail(LQM_ERR_ARG, "bad rho keys or null rho");
+    lqm_context* c = m->ctx;
+    const size_t NN = (size_t)m->N * m->N;
+    const int d = m->N / 2;
+    void *dks, *dkw = nullptr, *drho;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    if ((st = ws(c, W_OUT, sizeof(cx<double>) * NN * std::max(nRr, 1) + 16, &drho))) return st;
+    double* deb = reinterpret_cast<double*>(static_cast<char*>(drho) + sizeof(cx<double>) * NN * std::max(nRr, 1));
+    double eb = 0.0, trh = 0.0;
+    {
+        BdgScope S;
+        if ((st = bdg_begin(m, S))) return st;
+        // Re tr of the electron block of the unshifted zero key, for the Nambu constant
+        std::vector<cx<double>> diag(d);
+        LQM_HIP(hipMemcpy2DAsync(diag.data(), sizeof(cx<double>), S.base, sizeof(cx<double>) * (m->N + 1),
+                                 sizeof(cx<double>), d, hipMemcpyDeviceToHost, c->stream));
+        if ((st = bdg_shift(S, mu))) return st;
+        if ((st = run_rho(m, Nk, (const double*)dks, (const double*)dkw, 0.0, T, nRr, Rr, (double*)drho, deb, nullptr)))
+            return st;
+        if ((st = d2h(c, &eb, deb, sizeof(double)))) return st;
+        for (int i = 0; i < d; ++i) trh += diag[i].re;
+    }
+    if (nRr > 0 && (st = d2h(c, rho, drho, sizeof(cx<double>) * NN * nRr))) return st;
+    if (ekin) *ekin = eb / 2.0 + (trh - d * mu) / 2.0;
     return LQM_OK;
 }
 

@@ -105,6 +105,39 @@ class HartreeFock:
         """eps_H + eps_F (hartreefock.jl:239-240)."""
         return self.epsH + self.epsF
 
+    def grandpotential(self, rho: Hops, ks, mu: float = 0.0, T: float = 0.05, kweights=None,
+                       gradient: bool = False):
+        """Omega of H_MF = hf(rho) at (mu, T) on ks; with ``gradient=True`` also the
+        derivatives at fixed rho with respect to the mean-field parameters, as Hops on the keys
+        of h and of v: (Omega, dh, dv) with dh[R] = dOmega/dh_R (element-wise, = rho_MF(-R)^T)
+        and dv[L] = dOmega/dv_L (Fock and Hartree chain rule through H_MF)."""
+        Rh, Th, Rv, Tv, nRv = self._hv()
+        Rr, Tr = _pack_keys(rho, self.N, self.d)
+        k = _kpts(ks, self.d)
+        w = _weights(ks, kweights, k.shape[1])
+        om = _c.c_double()
+        gh = np.zeros((self.N, self.N, Rh.shape[1]), dtype=np.complex128, order="F") if gradient else None
+        gv = np.zeros((self.N, self.N, max(nRv, 1)), dtype=np.complex128, order="F") if gradient else None
+        _lib.call("lqm_hartreefock_grandpotential", self.exec.handle, self.N, self.d, Rh.shape[1], _vp(Rh), _vp(Th),
+                  nRv, _vp(Rv), _vp(Tv), Rr.shape[1], _vp(Rr), _vp(Tr), self.flags, k.shape[1], _vp(k), _vp(w),
+                  float(mu), float(T), _c.byref(om), _vp(gh), _vp(gv))
+        if not gradient:
+            return om.value
+        dh = Hops({key: np.array(gh[:, :, r]) for r, key in enumerate(self.h.keys())})
+        dv = Hops({key: np.array(gv[:, :, r]) for r, key in enumerate(self.v.keys())})
+        return om.value, dh, dv
+
+    def grandpotential_rrule(self, rho: Hops, ks, mu: float = 0.0, T: float = 0.05, kweights=None):
+        """(Omega, pullback): pullback(Omega_bar) returns the cotangents (of h, of v) in the
+        convention of ChainRulesCore for complex inputs, Omega_bar * conj(dOmega/dx)."""
+        om, dh, dv = self.grandpotential(rho, ks, mu, T, kweights, gradient=True)
+
+        def pullback(dbar: float = 1.0):
+            return (Hops({R: dbar * np.conj(b) for R, b in dh.items()}),
+                    Hops({R: dbar * np.conj(b) for R, b in dv.items()}))
+
+        return om, pullback
+
 
 def solveselfconsistent(rho_init: Hops, mf: HartreeFock, filling: float, ks=None, *,
                         klin: int | None = None, iterations: int = 500, tol: float = 1e-7,
