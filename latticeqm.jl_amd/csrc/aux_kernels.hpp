@@ -370,6 +370,13 @@ __global__ __launch_bounds__(256) void k_expval_cols(int N, const cx<double>* __
     }
 }
 
+// out[i] = in[i] converted between complex precisions (n elements, grid-stride).
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_convert_cx(size_t n, const cx<TI>* __restrict__ in, cx<TO>* __restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        out[i] = cx<TO>{(TO)in[i].re, (TO)in[i].im};
+}
+
 // Nambu (BdG) chemical potential shift (Superconductivity/operators.jl:14-17 applied with -mu,
 // as getdensitymatrix!(::BdGOperator) does, green.jl:38-54): the zero-key block of the 2d x 2d
 // Nambu model becomes base - mu tau_z, i.e. -mu on the electron diagonal (i < d) and +mu on

@@ -109,86 +109,91 @@ __device__ __forceinline__ double blk_sum(double v, double* red) {
 // j+1..N-1) and into A(j+1:N, j) with A(j+1, j) = beta.  last: column N-1, no reflector.
 // One 256-thread block per k-point; thread t owns rows j + t + 256q.
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_trd_col(int N, int j, int i, int last, cx<double>* __restrict__ A,
-                                                 cx<double>* __restrict__ V,
-                                                 const cx<double>* __restrict__ W,
+template <typename R>
+__global__ __launch_bounds__(256) void k_trd_col(int N, int j, int i, int last, cx<R>* __restrict__ A,
+                                                 cx<R>* __restrict__ V,
+                                                 const cx<R>* __restrict__ W,
                                                  double* __restrict__ dd, double* __restrict__ ee,
-                                                 cx<double>* __restrict__ tau) {
-    __shared__ cx<double> wj[kBnb], vj[kBnb];
+                                                 cx<R>* __restrict__ tau) {
+    __shared__ cx<R> wj[kBnb], vj[kBnb];
     __shared__ double red[4];
-    __shared__ cx<double> s_alpha;
-    __shared__ double s_dj;
+    __shared__ cx<R> s_alpha;
+    __shared__ R s_dj;
     const int tid = threadIdx.x;
     const size_t kk = blockIdx.x;
     const size_t NN = (size_t)N * N, NP = (size_t)N * kBnb;
-    cx<double>* Ak = A + kk * NN;
-    cx<double>* Vk = V + kk * NP;
-    const cx<double>* Wk = W + kk * NP;
+    cx<R>* Ak = A + kk * NN;
+    cx<R>* Vk = V + kk * NP;
+    const cx<R>* Wk = W + kk * NP;
+    const cx<R> czero = {R(0), R(0)};
     if (tid < i) {
         wj[tid] = Wk[(size_t)tid * N + j];
         vj[tid] = Vk[(size_t)tid * N + j];
     }
-    if (tid == 0) s_alpha = {0.0, 0.0};
+    if (tid == 0) s_alpha = czero;
     __syncthreads();
-    cx<double> a[kBRows];
+    cx<R> a[kBRows];
     double xn = 0.0;
 #pragma unroll
     for (int q = 0; q < kBRows; ++q) {
         const int r = j + tid + 256 * q;
-        cx<double> x = {0.0, 0.0};
+        cx<R> x = czero;
         if (r < N) {
             x = Ak[(size_t)j * N + r];
-            cx<double> u = {0.0, 0.0};
+            cx<R> u = czero;
             for (int t = 0; t < i; ++t) {
                 u = cmadcb(u, Vk[(size_t)t * N + r], wj[t]);
                 u = cmadcb(u, Wk[(size_t)t * N + r], vj[t]);
             }
             x = x - u;
-            if (r >= j + 2) xn += abs2(x);
+            if (r >= j + 2) xn += (double)abs2(x);
         }
         a[q] = x;
     }
     if (tid == 0) s_dj = a[0].re;
     if (tid == 1 && j + 1 < N) s_alpha = a[0];
     const double xnorm2 = blk_sum<4>(xn, red);  // (its barriers publish s_dj, s_alpha)
-    const double dj = s_dj;
+    const R dj = s_dj;
     if (last) {
         if (tid == 0) {
-            Ak[(size_t)j * N + j] = {dj, 0.0};
-            dd[kk * N + j] = dj;
+            Ak[(size_t)j * N + j] = {dj, R(0)};
+            dd[kk * N + j] = (double)dj;
             ee[kk * N + j] = 0.0;
-            tau[kk * N + j] = {0.0, 0.0};
+            tau[kk * N + j] = czero;
         }
         return;
     }
-    const cx<double> alpha = s_alpha;
-    cx<double> tj = {0.0, 0.0}, scale = {0.0, 0.0};
-    double beta = alpha.re;
-    if (!(xnorm2 == 0.0 && alpha.im == 0.0)) {
-        const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xnorm2);
-        beta = alpha.re >= 0.0 ? -nrm : nrm;
-        tj = {(beta - alpha.re) / beta, -alpha.im / beta};
-        scale = crecip(cx<double>{alpha.re - beta, alpha.im});
+    const cx<R> alpha = s_alpha;
+    cx<R> tj = czero, scale = czero;
+    double beta = (double)alpha.re;
+    if (!(xnorm2 == 0.0 && alpha.im == R(0))) {
+        const double are = alpha.re, aim = alpha.im;
+        const double nrm = sqrt(are * are + aim * aim + xnorm2);
+        beta = are >= 0.0 ? -nrm : nrm;
+        tj = {(R)((beta - are) / beta), (R)(-aim / beta)};
+        const double dre = are - beta, den = dre * dre + aim * aim;
+        scale = {(R)(dre / den), (R)(-aim / den)};
     }
+    const R rbeta = (R)beta;
 #pragma unroll
     for (int q = 0; q < kBRows; ++q) {
         const int r = j + tid + 256 * q;
         if (r < N) {
             if (r == j) {
-                Ak[(size_t)j * N + j] = {dj, 0.0};
+                Ak[(size_t)j * N + j] = {dj, R(0)};
             } else if (r == j + 1) {
-                Ak[(size_t)j * N + r] = {beta, 0.0};
-                Vk[(size_t)i * N + r] = {1.0, 0.0};
+                Ak[(size_t)j * N + r] = {rbeta, R(0)};
+                Vk[(size_t)i * N + r] = {R(1), R(0)};
             } else {
-                const cx<double> v = a[q] * scale;
+                const cx<R> v = a[q] * scale;
                 Ak[(size_t)j * N + r] = v;
                 Vk[(size_t)i * N + r] = v;
             }
         }
     }
     if (tid == 0) {
-        dd[kk * N + j] = dj;
-        ee[kk * N + j] = beta;
+        dd[kk * N + j] = (double)dj;
+        ee[kk * N + j] = (double)rbeta;
         tau[kk * N + j] = tj;
     }
 }
@@ -212,14 +217,15 @@ __host__ __device__ __forceinline__ int hv_blocks(int nt) {
     for (int J = 0; J < nt; ++J) b += hv_parts(nt, J);
     return b;
 }
+template <typename R>
 __global__ __launch_bounds__(kHvWaves * 64) void k_trd_hemv(int N, int j, int i, int nchmax, int nbmax,
-                                                  const cx<double>* __restrict__ A,
-                                                  const cx<double>* __restrict__ V,
-                                                  const cx<double>* __restrict__ W,
-                                                  cx<double>* __restrict__ P, cx<double>* __restrict__ P2,
-                                                  cx<double>* __restrict__ Q, double* __restrict__ SS) {
-    __shared__ cx<double> sv[kHvCols];
-    __shared__ cx<double> red[2][kHvWaves][64];
+                                                  const cx<R>* __restrict__ A,
+                                                  const cx<R>* __restrict__ V,
+                                                  const cx<R>* __restrict__ W,
+                                                  cx<R>* __restrict__ P, cx<R>* __restrict__ P2,
+                                                  cx<R>* __restrict__ Q, double* __restrict__ SS) {
+    __shared__ cx<R> sv[kHvCols];
+    __shared__ cx<R> red[2][kHvWaves][64];
     __shared__ double sred[kHvWaves];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t kk = blockIdx.y;
@@ -234,69 +240,69 @@ __global__ __launch_bounds__(kHvWaves * 64) void k_trd_hemv(int N, int j, int i,
     const int I0 = J + kHvPart * p, I1 = min(nt, I0 + kHvPart);
     const int c0 = j + 1 + J * kHvCols;
     const size_t NP = (size_t)N * kBnb;
-    const cx<double>* Vk = V + kk * NP;
-    const cx<double>* Wk = W + kk * NP;
-    const cx<double>* v = Vk + (size_t)i * N;
-    const cx<double>* Ak = A + kk * (size_t)N * N;
-    if (tid < kHvCols) sv[tid] = c0 + tid < N ? v[c0 + tid] : cx<double>{0.0, 0.0};
+    const cx<R>* Vk = V + kk * NP;
+    const cx<R>* Wk = W + kk * NP;
+    const cx<R>* v = Vk + (size_t)i * N;
+    const cx<R>* Ak = A + kk * (size_t)N * N;
+    if (tid < kHvCols) sv[tid] = c0 + tid < N ? v[c0 + tid] : cx<R>{R(0), R(0)};
     __syncthreads();
     constexpr int CPW = kHvCols / kHvWaves;  // columns per wave
-    cx<double> ycol[CPW];
+    cx<R> ycol[CPW];
 #pragma unroll
-    for (int u = 0; u < CPW; ++u) ycol[u] = {0.0, 0.0};
+    for (int u = 0; u < CPW; ++u) ycol[u] = {R(0), R(0)};
     double spart = 0.0;
     for (int I = I0; I < I1; ++I) {
         const int r = j + 1 + I * kHvCols + lane;
         const bool ok = r < N;
         const int rr = ok ? r : N - 1;  // loads stay in bounds and unconditional (batched)
-        const cx<double> vr = ok ? v[r] : cx<double>{0.0, 0.0};
-        cx<double> a[CPW];
+        const cx<R> vr = ok ? v[r] : cx<R>{R(0), R(0)};
+        cx<R> a[CPW];
 #pragma unroll
         for (int u = 0; u < CPW; ++u) {
             const int c = min(c0 + wave + kHvWaves * u, N - 1);
             a[u] = Ak[(size_t)c * N + rr];
         }
-        cx<double> yrow = {0.0, 0.0};
+        cx<R> yrow = {0.0, 0.0};
 #pragma unroll
         for (int u = 0; u < CPW; ++u) {
             const int cl = wave + kHvWaves * u, c = c0 + cl;
             const bool in = ok && c < N && r >= c;
-            const cx<double> x = in ? a[u] : cx<double>{0.0, 0.0};
+            const cx<R> x = in ? a[u] : cx<R>{R(0), R(0)};
             yrow = cmad(yrow, x, sv[cl]);
-            const cx<double> xt = (in && r > c) ? x : cx<double>{0.0, 0.0};
+            const cx<R> xt = (in && r > c) ? x : cx<R>{R(0), R(0)};
             ycol[u] = cmadc(ycol[u], xt, vr);
         }
         const int buf = (I - I0) & 1;
         red[buf][wave][lane] = yrow;
         __syncthreads();
         if (wave == 0 && ok) {
-            cx<double> y = red[buf][0][lane];
+            cx<R> y = red[buf][0][lane];
 #pragma unroll
             for (int w = 1; w < kHvWaves; ++w) y = y + red[buf][w][lane];
             P[kk * (size_t)nchmax * N + (size_t)J * N + r] = y;
-            spart += vr.re * y.re + vr.im * y.im;  // Re(conj(v_r) y_r)
+            spart += (double)(vr.re * y.re + vr.im * y.im);  // Re(conj(v_r) y_r)
         }
     }
 #pragma unroll
     for (int u = 0; u < CPW; ++u) {
         const int cl = wave + kHvWaves * u, c = c0 + cl;
-        const cx<double> y = wave_sum(ycol[u]);
+        const cx<R> y = wave_sum(ycol[u]);
         if (lane == 0 && c < N) {
             P2[kk * (size_t)nchmax * N + (size_t)p * N + c] = y;
-            spart += sv[cl].re * y.re + sv[cl].im * y.im;
+            spart += (double)(sv[cl].re * y.re + sv[cl].im * y.im);
         }
     }
     if (p == 0) {
         // W^H v (even products) and V^H v (odd) over the rows of tile J
-        cx<double>* Qk = Q + (kk * nchmax + J) * kQStride;
+        cx<R>* Qk = Q + (kk * nchmax + J) * kQStride;
         const int r = c0 + lane;
-        const cx<double> vr = r < N ? v[r] : cx<double>{0.0, 0.0};
+        const cx<R> vr = r < N ? v[r] : cx<R>{R(0), R(0)};
         const int rr = r < N ? r : N - 1;
         for (int idx = wave; idx < 2 * i; idx += kHvWaves) {
-            const cx<double>* X = (idx & 1) ? Vk : Wk;
-            const cx<double> x = X[(size_t)(idx >> 1) * N + rr];
-            cx<double> acc = {0.0, 0.0};
-            acc = cmadc(acc, r < N ? x : cx<double>{0.0, 0.0}, vr);
+            const cx<R>* X = (idx & 1) ? Vk : Wk;
+            const cx<R> x = X[(size_t)(idx >> 1) * N + rr];
+            cx<R> acc = {0.0, 0.0};
+            acc = cmadc(acc, r < N ? x : cx<R>{R(0), R(0)}, vr);
             acc = wave_sum(acc);
             if (lane == 0) Qk[1 + (idx & 1) * kBnb + (idx >> 1)] = acc;
         }
@@ -310,15 +316,16 @@ __global__ __launch_bounds__(kHvWaves * 64) void k_trd_hemv(int N, int j, int i,
 // t1 = W^H v, t2 = V^H v and s = v^H A22 v summed over the blocks, and
 // alpha = -1/2 tau (w^H v) = -1/2 |tau|^2 (s - 2 Re(t1^H t2)) (w^H v = conj(tau) v^H
 // (A22 - V W^H - W V^H) v).  Block (x, k): rows j + 1 + 256x + t.
+template <typename R>
 __global__ __launch_bounds__(256) void k_trd_w(int N, int j, int i, int nchmax, int nbmax,
-                                               const cx<double>* __restrict__ P,
-                                               const cx<double>* __restrict__ P2,
-                                               const cx<double>* __restrict__ Q,
+                                               const cx<R>* __restrict__ P,
+                                               const cx<R>* __restrict__ P2,
+                                               const cx<R>* __restrict__ Q,
                                                const double* __restrict__ SS,
-                                               const cx<double>* __restrict__ V,
-                                               cx<double>* __restrict__ W,
-                                               const cx<double>* __restrict__ tau) {
-    __shared__ cx<double> t12[kQStride];
+                                               const cx<R>* __restrict__ V,
+                                               cx<R>* __restrict__ W,
+                                               const cx<R>* __restrict__ tau) {
+    __shared__ cx<R> t12[kQStride];
     __shared__ double s_alpha;
     const int tid = threadIdx.x;
     const size_t kk = blockIdx.y;
@@ -326,13 +333,13 @@ __global__ __launch_bounds__(256) void k_trd_w(int N, int j, int i, int nchmax, 
     const int nt = (n + kHvCols - 1) / kHvCols;
     const int nb = hv_blocks(nt);
     const size_t NP = (size_t)N * kBnb;
-    const cx<double>* Vk = V + kk * NP;
-    cx<double>* Wk = W + kk * NP;
-    const cx<double>* v = Vk + (size_t)i * N;
-    const cx<double>* Qk = Q + kk * nchmax * kQStride;
-    const cx<double> tj = tau[kk * N + j];
+    const cx<R>* Vk = V + kk * NP;
+    cx<R>* Wk = W + kk * NP;
+    const cx<R>* v = Vk + (size_t)i * N;
+    const cx<R>* Qk = Q + kk * nchmax * kQStride;
+    const cx<R> tj = tau[kk * N + j];
     if (tid < kQStride) {
-        cx<double> acc = {0.0, 0.0};
+        cx<R> acc = {0.0, 0.0};
         if (tid == 0) {
             for (int b = 0; b < nb; ++b) acc.re += SS[kk * nbmax + b];
         } else if ((tid - 1) % kBnb < i) {
@@ -344,7 +351,7 @@ __global__ __launch_bounds__(256) void k_trd_w(int N, int j, int i, int nchmax, 
     if (tid == 0) {
         double re = 0.0;
         for (int t = 0; t < i; ++t) {
-            const cx<double> a = t12[1 + t], b = t12[1 + kBnb + t];
+            const cx<R> a = t12[1 + t], b = t12[1 + kBnb + t];
             re += a.re * b.re + a.im * b.im;  // Re(conj(t1) t2)
         }
         s_alpha = -0.5 * (tj.re * tj.re + tj.im * tj.im) * (t12[0].re - 2.0 * re);
@@ -352,21 +359,22 @@ __global__ __launch_bounds__(256) void k_trd_w(int N, int j, int i, int nchmax, 
     __syncthreads();
     const int r = j + 1 + blockIdx.x * 256 + tid;
     if (r >= N) return;
-    const cx<double>* Pk = P + kk * (size_t)nchmax * N;
-    const cx<double>* P2k = P2 + kk * (size_t)nchmax * N;
+    const cx<R>* Pk = P + kk * (size_t)nchmax * N;
+    const cx<R>* P2k = P2 + kk * (size_t)nchmax * N;
     const int I = (r - j - 1) / kHvCols;
-    cx<double> y = {0.0, 0.0};
+    cx<R> y = {0.0, 0.0};
     for (int J = 0; J <= I; ++J) y = y + Pk[(size_t)J * N + r];
     const int np = hv_parts(nt, I);
     for (int q = 0; q < np; ++q) y = y + P2k[(size_t)q * N + r];
-    cx<double> u = {0.0, 0.0};
+    cx<R> u = {0.0, 0.0};
     for (int t = 0; t < i; ++t) {
         u = cmad(u, Vk[(size_t)t * N + r], t12[1 + t]);
         u = cmad(u, Wk[(size_t)t * N + r], t12[1 + kBnb + t]);
     }
-    const cx<double> w = tj * (y - u);
-    const cx<double> vr = v[r];
-    Wk[(size_t)i * N + r] = {w.re + s_alpha * vr.re, w.im + s_alpha * vr.im};
+    const cx<R> w = tj * (y - u);
+    const cx<R> vr = v[r];
+    const R sa = (R)s_alpha;
+    Wk[(size_t)i * N + r] = {w.re + sa * vr.re, w.im + sa * vr.im};
 }
 
 // ---------------------------------------------------------------------------------
@@ -377,38 +385,64 @@ __global__ __launch_bounds__(256) void k_trd_w(int N, int j, int i, int nchmax, 
 // 64 x 64 output tile per block (4 waves of 32 x 32, 2 x 2 MFMA 16x16x4 tiles); operands
 // read straight from global memory (L1/L2), rows/columns outside M, N, K read as zero.
 // ---------------------------------------------------------------------------------
-struct ZGemmArgs {
+template <typename R>
+struct GemmArgs {
     int M, N, K, npair;
     int lower;  // M == N: only the 64 x 64 tiles on or below the diagonal (grid.x = nt(nt+1)/2)
-    const cx<double>* A[2];
-    const cx<double>* B[2];
+    const cx<R>* A[2];
+    const cx<R>* B[2];
     int lda, ldb, ldc;
     long long sA, sB, sC, sS;
-    cx<double>* C;
+    cx<R>* C;
     cx<double> alpha, beta;
     const double* ascale;
 };
+using ZGemmArgs = GemmArgs<double>;
 
-template <int OP>
-__device__ __forceinline__ cx<double> opA_at(const cx<double>* A, int ld, int m, int k) {
+// The 16 x 16 x 4 matrix-core instruction of each precision.  Operands: A(m, k) from lane
+// (m, k-slot) = (l % 16, l / 16), B(k, n) from lane (n, k-slot); accumulator element i holds
+// column l % 16 and row g + 4 i (FP64) or 4 g + i (FP32), g = l / 16.
+typedef float lqm_cf4 __attribute__((ext_vector_type(4)));
+template <typename R>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+    typedef lqm_d4 acc;
+    static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int g, int i) { return g + 4 * i; }
+};
+template <>
+struct Mfma16<float> {
+    typedef lqm_cf4 acc;
+    static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int g, int i) { return 4 * g + i; }
+};
+
+template <int OP, typename R>
+__device__ __forceinline__ cx<R> opA_at(const cx<R>* A, int ld, int m, int k) {
     if (OP == 0) return A[(size_t)k * ld + m];
-    const cx<double> x = A[(size_t)m * ld + k];
+    const cx<R> x = A[(size_t)m * ld + k];
     return {x.re, -x.im};
 }
-template <int OP>
-__device__ __forceinline__ cx<double> opB_at(const cx<double>* B, int ld, int k, int n) {
+template <int OP, typename R>
+__device__ __forceinline__ cx<R> opB_at(const cx<R>* B, int ld, int k, int n) {
     if (OP == 0) return B[(size_t)n * ld + k];
-    const cx<double> x = B[(size_t)k * ld + n];
+    const cx<R> x = B[(size_t)k * ld + n];
     return {x.re, -x.im};
 }
 
-template <int OPA, int OPB>
-__device__ __forceinline__ void gemm_acc(const cx<double>* Ab, int lda, const cx<double>* Bb, int ldb,
-                                         const double* sc, int M, int N, int K, int m0, int n0, int li,
-                                         int g, lqm_d4 (&cr)[2][2], lqm_d4 (&ci)[2][2]) {
+template <int OPA, int OPB, typename R>
+__device__ __forceinline__ void gemm_acc(const cx<R>* Ab, int lda, const cx<R>* Bb, int ldb, const double* sc,
+                                         int M, int N, int K, int m0, int n0, int li, int g,
+                                         typename Mfma16<R>::acc (&cr)[2][2], typename Mfma16<R>::acc (&ci)[2][2]) {
+    typedef Mfma16<R> MM;
     for (int k0 = 0; k0 < K; k0 += 16) {
         const int kb = k0 + 4 * g;
-        double ar[2][4], ai[2][4], br[2][4], bi[2][4];
+        R ar[2][4], ai[2][4], br[2][4], bi[2][4];
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             const int m = m0 + 16 * t + li, n = n0 + 16 * t + li;
@@ -417,14 +451,14 @@ __device__ __forceinline__ void gemm_acc(const cx<double>* Ab, int lda, const cx
                 // clamped indices: every load is in bounds and unconditional, so the
                 // compiler can issue them together; out-of-range values are zeroed after
                 const int k = kb + s, kc = min(k, K - 1);
-                cx<double> x = opA_at<OPA>(Ab, lda, min(m, M - 1), kc);
-                cx<double> y = opB_at<OPB>(Bb, ldb, kc, min(n, N - 1));
+                cx<R> x = opA_at<OPA>(Ab, lda, min(m, M - 1), kc);
+                cx<R> y = opB_at<OPB>(Bb, ldb, kc, min(n, N - 1));
                 const bool ina = k < K && m < M, inb = k < K && n < N;
-                const double f = sc ? sc[kc] : 1.0;
-                ar[t][s] = ina ? f * x.re : 0.0;
-                ai[t][s] = ina ? f * x.im : 0.0;
-                br[t][s] = inb ? y.re : 0.0;
-                bi[t][s] = inb ? y.im : 0.0;
+                const R f = sc ? (R)sc[kc] : R(1);
+                ar[t][s] = ina ? f * x.re : R(0);
+                ai[t][s] = ina ? f * x.im : R(0);
+                br[t][s] = inb ? y.re : R(0);
+                bi[t][s] = inb ? y.im : R(0);
             }
         }
 #pragma unroll
@@ -432,29 +466,33 @@ __device__ __forceinline__ void gemm_acc(const cx<double>* Ab, int lda, const cx
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
-                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a][s], br[b][s], cr[a][b], 0, 0, 0);
+                for (int b = 0; b < 2; ++b) cr[a][b] = MM::mma(ar[a][s], br[b][s], cr[a][b]);
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
-                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a][s], bi[b][s], ci[a][b], 0, 0, 0);
+                for (int b = 0; b < 2; ++b) ci[a][b] = MM::mma(ar[a][s], bi[b][s], ci[a][b]);
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
-                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a][s], bi[b][s], cr[a][b], 0, 0, 0);
+                for (int b = 0; b < 2; ++b) cr[a][b] = MM::mma(-ai[a][s], bi[b][s], cr[a][b]);
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
-                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a][s], br[b][s], ci[a][b], 0, 0, 0);
+                for (int b = 0; b < 2; ++b) ci[a][b] = MM::mma(ai[a][s], br[b][s], ci[a][b]);
         }
     }
 }
 
-template <int OPA, int OPB>
-__global__ __launch_bounds__(256) void k_zgemm_gen(ZGemmArgs g) {
+// Batched complex GEMM on the matrix cores of precision R (FP64: v_mfma_f64_16x16x4, FP32:
+// v_mfma_f32_16x16x4 with FP32 accumulation):
+//   C = alpha sum_{p < npair} op(A_p) op(B_p) + beta C
+// op 0: as stored (column-major, leading dimension ld), op 1: conjugate transpose.
+// ascale (optional): column k of op(A_p) scaled by ascale[k] (per batch entry, stride sS).
+// 64 x 64 output tile per block (4 waves of 32 x 32, 2 x 2 MFMA 16x16x4 tiles); operands
+// read straight from global memory (L1/L2), rows/columns outside M, N, K read as zero.
+template <int OPA, int OPB, typename R = double>
+__global__ __launch_bounds__(256) void k_zgemm_gen(GemmArgs<R> g) {
+    typedef Mfma16<R> MM;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, g4 = lane >> 4;
     int ti = blockIdx.x, tj = blockIdx.y;
@@ -468,19 +506,19 @@ __global__ __launch_bounds__(256) void k_zgemm_gen(ZGemmArgs g) {
     const int m0 = ti * 64 + (wave & 1) * 32;
     const int n0 = tj * 64 + (wave >> 1) * 32;
     const size_t b = blockIdx.z;
-    lqm_d4 cr[2][2], ci[2][2];
+    typename MM::acc cr[2][2], ci[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            cr[a][c] = lqm_d4{0.0, 0.0, 0.0, 0.0};
-            ci[a][c] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+            cr[a][c] = typename MM::acc{R(0), R(0), R(0), R(0)};
+            ci[a][c] = typename MM::acc{R(0), R(0), R(0), R(0)};
         }
     const double* sc = g.ascale ? g.ascale + b * g.sS : nullptr;
     for (int p = 0; p < g.npair; ++p)
-        gemm_acc<OPA, OPB>(g.A[p] + b * g.sA, g.lda, g.B[p] + b * g.sB, g.ldb, sc, g.M, g.N, g.K, m0, n0, li,
-                           g4, cr, ci);
-    cx<double>* Cb = g.C + b * g.sC;
+        gemm_acc<OPA, OPB, R>(g.A[p] + b * g.sA, g.lda, g.B[p] + b * g.sB, g.ldb, sc, g.M, g.N, g.K, m0, n0, li,
+                              g4, cr, ci);
+    cx<R>* Cb = g.C + b * g.sC;
     const bool zero_beta = g.beta.re == 0.0 && g.beta.im == 0.0;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -488,13 +526,16 @@ __global__ __launch_bounds__(256) void k_zgemm_gen(ZGemmArgs g) {
         for (int c = 0; c < 2; ++c)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int m = m0 + 16 * a + g4 + 4 * i, n = n0 + 16 * c + li;
+                const int m = m0 + 16 * a + MM::row(g4, i), n = n0 + 16 * c + li;
                 if (m < g.M && n < g.N) {
                     const size_t off = (size_t)n * g.ldc + m;
-                    const cx<double> acc = {cr[a][c][i], ci[a][c][i]};
+                    const cx<double> acc = {(double)cr[a][c][i], (double)ci[a][c][i]};
                     cx<double> out = g.alpha * acc;
-                    if (!zero_beta) out = cmad(out, g.beta, Cb[off]);
-                    Cb[off] = out;
+                    if (!zero_beta) {
+                        const cx<R> cv = Cb[off];
+                        out = cmad(out, g.beta, cx<double>{(double)cv.re, (double)cv.im});
+                    }
+                    Cb[off] = cx<R>{(R)out.re, (R)out.im};
                 }
             }
 }
@@ -735,14 +776,15 @@ __global__ __launch_bounds__(256) void k_invit_orth(int N, int ncol, int ldz, co
 
 // A (reflectors below the subdiagonal, beta on it) -> explicit Y in place: column c < N-1
 // holds v_c (zero above row c+1, 1 at row c+1); column N-1 is zero.
-__global__ __launch_bounds__(256) void k_make_y(int N, cx<double>* __restrict__ A) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_make_y(int N, cx<R>* __restrict__ A) {
     const size_t NN = (size_t)N * N;
     const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= NN) return;
     const int c = (int)(e / N), r = (int)(e - (size_t)c * N);
-    cx<double>* Ak = A + (size_t)blockIdx.y * NN;
-    if (c == N - 1 || r <= c) Ak[e] = {0.0, 0.0};
-    else if (r == c + 1) Ak[e] = {1.0, 0.0};
+    cx<R>* Ak = A + (size_t)blockIdx.y * NN;
+    if (c == N - 1 || r <= c) Ak[e] = {R(0), R(0)};
+    else if (r == c + 1) Ak[e] = {R(1), R(0)};
 }
 
 // Compact-WY factor of reflector block b (reflectors jb = 64b ... jb + nbb - 1): the
@@ -750,11 +792,13 @@ __global__ __launch_bounds__(256) void k_make_y(int N, cx<double>* __restrict__ 
 // columnwise): T(t, t) = tau_t, T(0:t, t) = -tau_t T(0:t, 0:t) P(0:t, t), P = Y^H Y
 // (rows jb+1..N-1, on the matrix cores).  Out: Tw[k][b] (64 x 64 column-major, zero
 // outside 0..nbb-1).  Block (b, k).
-__global__ __launch_bounds__(256) void k_wy_T(int N, int nblk, const cx<double>* __restrict__ Y,
-                                              const cx<double>* __restrict__ tau,
-                                              cx<double>* __restrict__ Tw) {
-    __shared__ cx<double> Ps[64][65];
-    __shared__ cx<double> col[64];
+template <typename R>
+__global__ __launch_bounds__(256) void k_wy_T(int N, int nblk, const cx<R>* __restrict__ Y,
+                                              const cx<R>* __restrict__ tau,
+                                              cx<R>* __restrict__ Tw) {
+    typedef Mfma16<R> MM;
+    __shared__ cx<R> Ps[64][65];
+    __shared__ cx<R> col[64];
     const int b = blockIdx.x;
     const size_t kk = blockIdx.y;
     const int jb = kBwy * b, nbb = min(kBwy, N - 1 - jb);
@@ -762,39 +806,39 @@ __global__ __launch_bounds__(256) void k_wy_T(int N, int nblk, const cx<double>*
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, g4 = lane >> 4;
     const int m0 = (wave & 1) * 32, n0 = (wave >> 1) * 32;
-    const cx<double>* Yb = Y + kk * (size_t)N * N + (size_t)jb * N + r0;
-    lqm_d4 cr[2][2], ci[2][2];
+    const cx<R>* Yb = Y + kk * (size_t)N * N + (size_t)jb * N + r0;
+    typename MM::acc cr[2][2], ci[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            cr[a][c] = lqm_d4{0.0, 0.0, 0.0, 0.0};
-            ci[a][c] = lqm_d4{0.0, 0.0, 0.0, 0.0};
+            cr[a][c] = typename MM::acc{R(0), R(0), R(0), R(0)};
+            ci[a][c] = typename MM::acc{R(0), R(0), R(0), R(0)};
         }
-    gemm_acc<1, 0>(Yb, N, Yb, N, nullptr, nbb, nbb, N - r0, m0, n0, li, g4, cr, ci);
+    gemm_acc<1, 0, R>(Yb, N, Yb, N, nullptr, nbb, nbb, N - r0, m0, n0, li, g4, cr, ci);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int m = m0 + 16 * a + g4 + 4 * i, n = n0 + 16 * c + li;
-                Ps[n][m] = cx<double>{cr[a][c][i], ci[a][c][i]};  // Ps[col][row]
+                const int m = m0 + 16 * a + MM::row(g4, i), n = n0 + 16 * c + li;
+                Ps[n][m] = cx<R>{cr[a][c][i], ci[a][c][i]};  // Ps[col][row]
             }
     __syncthreads();
-    const cx<double>* tk = tau + kk * N + jb;
+    const cx<R>* tk = tau + kk * N + jb;
     // T overwrites P column by column (column t of P copied out first)
     for (int t = 0; t < nbb; ++t) {
         if (threadIdx.x < t) col[threadIdx.x] = Ps[t][threadIdx.x];
         __syncthreads();
-        const cx<double> tt = tk[t];
+        const cx<R> tt = tk[t];
         if (threadIdx.x < 64) {
             const int s = threadIdx.x;
-            cx<double> v = {0.0, 0.0};
+            cx<R> v = {R(0), R(0)};
             if (s < t) {
-                cx<double> acc = {0.0, 0.0};
+                cx<R> acc = {R(0), R(0)};
                 for (int u = s; u < t; ++u) acc = cmad(acc, Ps[u][s], col[u]);
-                v = cx<double>{0.0, 0.0} - tt * acc;
+                v = cx<R>{R(0), R(0)} - tt * acc;
             } else if (s == t) {
                 v = tt;
             }
@@ -802,22 +846,23 @@ __global__ __launch_bounds__(256) void k_wy_T(int N, int nblk, const cx<double>*
         }
         __syncthreads();
     }
-    cx<double>* To = Tw + (kk * nblk + b) * (size_t)(kBwy * kBwy);
+    cx<R>* To = Tw + (kk * nblk + b) * (size_t)(kBwy * kBwy);
     for (int e = threadIdx.x; e < kBwy * kBwy; e += 256) {
         const int c = e >> 6, r = e & 63;
-        To[e] = (c < nbb && r <= c) ? Ps[c][r] : cx<double>{0.0, 0.0};
+        To[e] = (c < nbb && r <= c) ? Ps[c][r] : cx<R>{R(0), R(0)};
     }
 }
 
 // V[k](:, n) = Z[k](:, n) as complex (Zt row-major with row stride ldz), n < ncol.
+template <typename R>
 __global__ __launch_bounds__(256) void k_z_to_v(int N, int ncol, int ldz, const double* __restrict__ Zt,
-                                                cx<double>* __restrict__ V) {
+                                                cx<R>* __restrict__ V) {
     __shared__ double tile[32][33];
     const size_t kk = blockIdx.z;
     const int i0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     const double* Zk = Zt + kk * (size_t)N * ldz;
-    cx<double>* Vk = V + kk * (size_t)N * ncol;
+    cx<R>* Vk = V + kk * (size_t)N * ncol;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int i = i0 + ty + 8 * q, n = n0 + tx;
@@ -827,7 +872,7 @@ __global__ __launch_bounds__(256) void k_z_to_v(int N, int ncol, int ldz, const 
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int n = n0 + ty + 8 * q, i = i0 + tx;
-        if (i < N && n < ncol) Vk[(size_t)n * N + i] = {tile[tx][ty + 8 * q], 0.0};
+        if (i < N && n < ncol) Vk[(size_t)n * N + i] = {(R)tile[tx][ty + 8 * q], R(0)};
     }
 }
 

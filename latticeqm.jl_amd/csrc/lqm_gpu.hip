@@ -50,7 +50,7 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -753,6 +753,9 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
                        int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands,
                        double* evecs, const double* occ, const MuReq* mq = nullptr);
 bool use_blocked(const lqm_context* c, int N);
+lqm_status run_blocked_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
+                             int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands,
+                             const double* occ, const MuReq* mq);
 
 // Start of a pass over the grid: clears the refinement report of the previous pass and
 // records the precision this pass runs in (reported by lqm_context_last_precision).
@@ -772,8 +775,11 @@ lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, doub
     if (Nk < 0) return fail(LQM_ERR_ARG, "Nk < 0");
     if (Nk == 0) return LQM_OK;
     if (!ks || !bands) return fail(LQM_ERR_ARG, "null ks or bands");
-    if (use_blocked(m->ctx, N))
+    if (use_blocked(m->ctx, N)) {
+        if (m->ctx->precision == LQM_PREC_FP32_REFINED && !evecs)
+            return run_blocked_mixed(m, Nk, ks, nullptr, 0.0, 0.0, 0, nullptr, nullptr, nullptr, bands, nullptr, nullptr);
         return run_blocked(m, Nk, ks, nullptr, 0.0, 0.0, 0, nullptr, nullptr, nullptr, bands, evecs, nullptr);
+    }
     // eigenvector output always comes from the FP64 path
     if (m->ctx->precision == LQM_PREC_FP32_REFINED && !evecs)
         return run_mixed(m, Nk, ks, nullptr, 0.0, 0.0, 0, nullptr, nullptr, nullptr, bands);
@@ -815,9 +821,13 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
     if ((st = wsT(c, W_RR, (size_t)std::max(1, nRr * d), &dRr))) return st;
     if (nRr > 0)
         LQM_HIP(hipMemcpyAsync(dRr, Rr, sizeof(int) * nRr * d, hipMemcpyHostToDevice, c->stream));
-    if (use_blocked(c, N))
+    if (use_blocked(c, N)) {
+        if (c->precision == LQM_PREC_FP32_REFINED)
+            return run_blocked_mixed(m, Nk, ks, kw, mu, T, nRr, dRr, reinterpret_cast<cx<double>*>(rho), eband,
+                                     bands_out, occ, mq);
         return run_blocked(m, Nk, ks, kw, mu, T, nRr, dRr, reinterpret_cast<cx<double>*>(rho), eband,
                            bands_out, nullptr, occ, mq);
+    }
     if (c->precision == LQM_PREC_FP32_REFINED)
         return run_mixed(m, Nk, ks, kw, mu, T, nRr, dRr, reinterpret_cast<cx<double>*>(rho), eband,
                          bands_out, occ);
@@ -1167,20 +1177,21 @@ bool use_blocked(const lqm_context* c, int N) {
     return N > kBlockedMinN;
 }
 
-template <int OPA, int OPB>
-void zgemm_gen(lqm_context* c, int stage, long long kpts, int nb, const lqm::ZGemmArgs& g) {
+template <int OPA, int OPB, typename R = double>
+void zgemm_gen(lqm_context* c, int stage, long long kpts, int nb, const lqm::GemmArgs<R>& g) {
     if (g.M <= 0 || g.N <= 0 || nb <= 0) return;
     StageScope sc(c, stage, kpts);
     const unsigned nt = (unsigned)((g.M + 63) / 64);
     const dim3 grid = g.lower ? dim3(nt * (nt + 1) / 2, 1, (unsigned)nb)
                               : dim3(nt, (unsigned)((g.N + 63) / 64), (unsigned)nb);
-    hipLaunchKernelGGL((lqm::k_zgemm_gen<OPA, OPB>), grid, dim3(256), 0, c->stream, g);
+    hipLaunchKernelGGL((lqm::k_zgemm_gen<OPA, OPB, R>), grid, dim3(256), 0, c->stream, g);
 }
 
-lqm::ZGemmArgs gemm_args(int M, int N, int K, const cx<double>* A, int lda, long long sA,
-                         const cx<double>* B, int ldb, long long sB, cx<double>* C, int ldc, long long sC,
-                         cx<double> alpha, cx<double> beta) {
-    lqm::ZGemmArgs g{};
+template <typename R = double>
+lqm::GemmArgs<R> gemm_args(int M, int N, int K, const cx<R>* A, int lda, long long sA,
+                           const cx<R>* B, int ldb, long long sB, cx<R>* C, int ldc, long long sC,
+                           cx<double> alpha, cx<double> beta) {
+    lqm::GemmArgs<R> g{};
     g.M = M;
     g.N = N;
     g.K = K;
@@ -1204,10 +1215,12 @@ lqm::ZGemmArgs gemm_args(int M, int N, int K, const cx<double>* A, int lda, long
 }
 
 // zhetrd (lower) of Kc full Hermitian matrices A with zlatrd panels: d, e, tau out, the
-// reflectors left in A below the diagonal.
-lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<double>* A, cx<double>* Vp, cx<double>* Wp,
-                           cx<double>* P, cx<double>* P2, cx<double>* Q, double* SS, int nchmax, int nbmax,
-                           double* dd, double* ee, cx<double>* tau) {
+// reflectors left in A below the diagonal.  R: working precision (the tridiagonal d, e is
+// written in FP64 either way).
+template <typename R>
+lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<R>* A, cx<R>* Vp, cx<R>* Wp,
+                           cx<R>* P, cx<R>* P2, cx<R>* Q, double* SS, int nchmax, int nbmax,
+                           double* dd, double* ee, cx<R>* tau) {
     hipStream_t s = c->stream;
     const size_t NN = (size_t)N * N;
     for (int p = 0; p < N - 1; p += lqm::kBnb) {
@@ -1217,30 +1230,30 @@ lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<double>* A, cx<doub
             for (int i = 0; i < pw; ++i) {
                 const int j = p + i, n = N - j - 1;
                 const int nch = (n + lqm::kHvCols - 1) / lqm::kHvCols;
-                hipLaunchKernelGGL(lqm::k_trd_col, dim3(Kc), dim3(256), 0, s, N, j, i, 0, A, Vp, Wp, dd, ee,
+                hipLaunchKernelGGL(lqm::k_trd_col<R>, dim3(Kc), dim3(256), 0, s, N, j, i, 0, A, Vp, Wp, dd, ee,
                                    tau);
-                hipLaunchKernelGGL(lqm::k_trd_hemv, dim3(lqm::hv_blocks(nch), Kc), dim3(lqm::kHvWaves * 64), 0, s, N, j, i, nchmax,
+                hipLaunchKernelGGL(lqm::k_trd_hemv<R>, dim3(lqm::hv_blocks(nch), Kc), dim3(lqm::kHvWaves * 64), 0, s, N, j, i, nchmax,
                                    nbmax, A, Vp, Wp, P, P2, Q, SS);
-                hipLaunchKernelGGL(lqm::k_trd_w, dim3((n + 255) / 256, Kc), dim3(256), 0, s, N, j, i, nchmax, nbmax,
+                hipLaunchKernelGGL(lqm::k_trd_w<R>, dim3((n + 255) / 256, Kc), dim3(256), 0, s, N, j, i, nchmax, nbmax,
                                    P, P2, Q, SS, Vp, Wp, tau);
             }
         }
         LQM_HIP(hipGetLastError());
         // A22 -= V W^H + W V^H on rows/columns >= p + pw, tiles on or below the diagonal
         const int q = p + pw, n = N - q;
-        lqm::ZGemmArgs g = gemm_args(n, n, pw, Vp + q, N, (long long)N * lqm::kBnb, Wp + q, N,
-                                     (long long)N * lqm::kBnb, A + (size_t)q * N + q, N, (long long)NN,
-                                     {-1.0, 0.0}, {1.0, 0.0});
+        lqm::GemmArgs<R> g = gemm_args<R>(n, n, pw, Vp + q, N, (long long)N * lqm::kBnb, Wp + q, N,
+                                          (long long)N * lqm::kBnb, A + (size_t)q * N + q, N, (long long)NN,
+                                          {-1.0, 0.0}, {1.0, 0.0});
         g.npair = 2;
         g.A[1] = Wp + q;
         g.B[1] = Vp + q;
         g.lower = 1;
-        zgemm_gen<0, 1>(c, ST_BUPD, Kc, Kc, g);
+        zgemm_gen<0, 1, R>(c, ST_BUPD, Kc, Kc, g);
         LQM_HIP(hipGetLastError());
     }
     {
         StageScope sc(c, ST_BTRD, Kc);
-        hipLaunchKernelGGL(lqm::k_trd_col, dim3(Kc), dim3(256), 0, s, N, N - 1, 0, 1, A, Vp, Wp, dd, ee, tau);
+        hipLaunchKernelGGL(lqm::k_trd_col<R>, dim3(Kc), dim3(256), 0, s, N, N - 1, 0, 1, A, Vp, Wp, dd, ee, tau);
     }
     LQM_HIP(hipGetLastError());
     return LQM_OK;
@@ -1381,10 +1394,10 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
         cx<double>* TW = Wt + (size_t)lqm::kBwy * ncol * Kmax;
         {
             StageScope sc(c, ST_BBT, Kc);
-            hipLaunchKernelGGL(lqm::k_make_y, dim3((unsigned)((NN + 255) / 256), Kc), dim3(256), 0, s, N, A);
+            hipLaunchKernelGGL(lqm::k_make_y<double>, dim3((unsigned)((NN + 255) / 256), Kc), dim3(256), 0, s, N, A);
             if (N > 1)
-                hipLaunchKernelGGL(lqm::k_wy_T, dim3(nblk, Kc), dim3(256), 0, s, N, nblk, A, tau, Tw);
-            hipLaunchKernelGGL(lqm::k_z_to_v, dim3((N + 31) / 32, (ncol + 31) / 32, Kc), dim3(256), 0, s, N, ncol,
+                hipLaunchKernelGGL(lqm::k_wy_T<double>, dim3(nblk, Kc), dim3(256), 0, s, N, nblk, A, tau, Tw);
+            hipLaunchKernelGGL(lqm::k_z_to_v<double>, dim3((N + 31) / 32, (ncol + 31) / 32, Kc), dim3(256), 0, s, N, ncol,
                                ldz, Zt, V);
         }
         LQM_HIP(hipGetLastError());
@@ -1448,6 +1461,274 @@ void dos_launch(lqm_context* c, int N, int Nk, const double* bands, const double
     const double scale = kw ? 1.0 : 1.0 / (double)Nk;
     hipLaunchKernelGGL(lqm::k_dos_fold, dim3(p.tilesO), dim3(256), 0, c->stream, Nw, p.S, part, scale,
                        dos);
+}
+
+void convert_launch_d2f(lqm_context* c, size_t n, const cx<double>* in, cx<float>* out) {
+    const unsigned g = (unsigned)std::min<size_t>(8192, (n + 255) / 256);
+    hipLaunchKernelGGL((lqm::k_convert_cx<double, float>), dim3(g), dim3(256), 0, c->stream, n, in, out);
+}
+void convert_launch_f2d(lqm_context* c, size_t n, const cx<float>* in, cx<double>* out) {
+    const unsigned g = (unsigned)std::min<size_t>(8192, (n + 255) / 256);
+    hipLaunchKernelGGL((lqm::k_convert_cx<float, double>), dim3(g), dim3(256), 0, c->stream, n, in, out);
+}
+
+// One pass of the large-N path in mixed precision (LQM_PREC_FP32_REFINED) over the grid: H(k)
+// in FP64 (kept for the refinement) and rounded to FP32; the blocked Householder reduction,
+// the compact-WY back-transformation and their trailing / panel products in FP32 (FP32
+// matrix cores, FP32 accumulation: half the bytes of the HBM-bound matrix-vector products);
+// the tridiagonal eigenpairs by FP64 bisection and inverse iteration of the FP32-rounded
+// tridiagonal (all N vectors); then Ogita-Aishima sweeps in FP64 on the matrix cores over
+// every k-point of the chunk: W = A X, S = X^H W, G = X^H X, the correction E with Rayleigh-
+// Ritz on near-degenerate clusters (k_oa_correct, cluster rotations in global memory),
+// X <- X + X E, until no k-point asks for another sweep (at most max(2, max_sweeps)).
+// Eigenvalues are the refined Rayleigh quotients with their second-order correction; rho_k =
+// X f X^H in FP64 over the columns whose occupation reaches kOccTiny.  Outputs as run_blocked
+// (eigenvector output stays on the FP64 path).
+constexpr size_t kBlockedMixedBudget = size_t(96) << 30;
+lqm_status run_blocked_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
+                             int nRr, const int* dRr, cx<double>* rho, double* eband, double* bands,
+                             const double* occ, const MuReq* mq) {
+    lqm_context* c = m->ctx;
+    const int N = m->N, d = m->d;
+    if (N > lqm::kBMaxN) return fail(LQM_ERR_UNSUPPORTED, "blocked eigensolver supports N <= 4096");
+    hipStream_t s = c->stream;
+    const size_t NN = (size_t)N * N;
+    const bool want_rho = rho != nullptr || eband != nullptr;
+    const int nchmax = std::max(1, (N - 1 + lqm::kHvCols - 1) / lqm::kHvCols);
+    const int nbmax = lqm::hv_blocks(nchmax);
+    const int nblk = std::max(1, (N - 1 + lqm::kBwy - 1) / lqm::kBwy);
+    const int maxclg = lqm::oa_max_clusters_global(N);
+    size_t perk = 16 * NN + 8 * NN + 2 * 8 * (size_t)N * lqm::kBnb + 2 * 8 * (size_t)nchmax * N +
+                  8 * (size_t)nchmax * lqm::kQStride + 8 * (size_t)nbmax + 64 * (size_t)N + 64 +
+                  8 * NN + 24 * NN + 8 * NN + 8 * (size_t)nblk * lqm::kBwy * lqm::kBwy + 2 * 8 * 64 * (size_t)N +
+                  4 * 16 * NN + 16 * (size_t)maxclg * lqm::kClusterMax * lqm::kClusterMax + 48;
+    if (want_rho) perk += 16 * NN;
+    const int Kmax = (int)std::max<size_t>(1, std::min<size_t>({kBlockedMixedBudget / perk, (size_t)Nk, (size_t)65535}));
+    if (mq && Kmax < Nk) return kFallback;  // mu needs every refined eigenvalue before the first occupation
+    begin_pass(c, LQM_PREC_FP32_REFINED);
+    const double wuni = 1.0 / (double)Nk;
+    const cx<double> one = {1.0, 0.0}, zero = {0.0, 0.0};
+    lqm_status st;
+    const int RG = contract_group(nRr);
+    const int groups = std::max(1, (nRr + RG - 1) / RG);
+    const int tilesE = (int)((NN + 255) / 256);
+    const int S = std::max(1, std::min(std::max(1, Kmax / kContractMinK), contract_splits(tilesE * groups)));
+    cx<double>* part = nullptr;
+    if (rho && nRr > 0 && (st = wsT(c, W_PART, (size_t)groups * S * RG * NN, &part))) return st;
+    const int max_sw = std::max(2, c->refine_max);
+    int sweeps_max = 0;
+    double rmax = 0.0, dmax = 0.0, nnear = 0.0;
+    std::vector<double> hstat, hlam;
+    for (int k0 = 0; k0 < Nk; k0 += Kmax) {
+        const int Kc = std::min(Kmax, Nk - k0);
+        const double* ksc = ks + (size_t)k0 * d;
+        const double* kwc = kw ? kw + k0 : nullptr;
+        cx<double> *A, *X, *Wm, *Sm, *Gm, *Ug;
+        cx<float> *A32, *Vp, *P, *X32, *Tw, *Wt;
+        double *vec, *Zt, *scr, *lam, *stat;
+        if ((st = wsT(c, W_BA, NN * Kmax, &A))) return st;
+        if ((st = wsT(c, W_MA32, NN * Kmax, &A32))) return st;
+        if ((st = wsT(c, W_BVW, (size_t)2 * N * lqm::kBnb * Kmax, &Vp))) return st;
+        cx<float>* Wp = Vp + (size_t)N * lqm::kBnb * Kmax;
+        if ((st = wsT(c, W_BP, (size_t)2 * nchmax * N * Kmax + (size_t)nchmax * lqm::kQStride * Kmax + nbmax * Kmax, &P)))
+            return st;
+        cx<float>* P2 = P + (size_t)nchmax * N * Kmax;
+        cx<float>* Q = P2 + (size_t)nchmax * N * Kmax;
+        double* SS = reinterpret_cast<double*>(Q + (size_t)nchmax * lqm::kQStride * Kmax);
+        if ((st = wsT(c, W_BVEC, (size_t)(6 * N + 2) * Kmax, &vec))) return st;
+        double* dd = vec;
+        double* ee = dd + (size_t)N * Kmax;
+        double* lamw = ee + (size_t)N * Kmax;  // eigenvalues of the FP32 tridiagonal
+        double* fo = lamw + (size_t)N * Kmax;
+        cx<float>* tau = reinterpret_cast<cx<float>*>(fo + (size_t)N * Kmax);
+        double* tn = reinterpret_cast<double*>(tau) + (size_t)2 * N * Kmax;
+        if ((st = wsT(c, W_BZ, NN * Kmax, &Zt))) return st;
+        if ((st = wsT(c, W_BSC, 3 * NN * Kmax, &scr))) return st;
+        if ((st = wsT(c, W_MX32, NN * Kmax, &X32))) return st;
+        if ((st = wsT(c, W_BT, (size_t)nblk * lqm::kBwy * lqm::kBwy * Kmax, &Tw))) return st;
+        if ((st = wsT(c, W_BWT, (size_t)2 * lqm::kBwy * N * Kmax, &Wt))) return st;
+        cx<float>* TW = Wt + (size_t)lqm::kBwy * N * Kmax;
+        if ((st = wsT(c, W_X, NN * Kmax, &X))) return st;
+        if ((st = wsT(c, W_W, NN * Kmax, &Wm))) return st;
+        if ((st = wsT(c, W_S, NN * Kmax, &Sm))) return st;
+        if ((st = wsT(c, W_G, NN * Kmax, &Gm))) return st;
+        if ((st = wsT(c, W_LAM, (size_t)N * Kmax, &lam))) return st;
+        if ((st = wsT(c, W_STAT, (size_t)4 * Kmax, &stat))) return st;
+        if ((st = wsT(c, W_MUG, (size_t)maxclg * lqm::kClusterMax * lqm::kClusterMax * Kmax, &Ug))) return st;
+        // H(k) in FP64 (the refinement's A) and its FP32 rounding (the reduction's)
+        {
+            StageScope sc(c, ST_BHK, Kc);
+            if ((st = launch_hk_full(c, m, Kc, ksc, A))) return st;
+            convert_launch_d2f(c, NN * Kc, A, A32);
+        }
+        LQM_HIP(hipGetLastError());
+        if ((st = blocked_tridiag<float>(c, N, Kc, A32, Vp, Wp, P, P2, Q, SS, nchmax, nbmax, dd, ee, tau))) return st;
+        {
+            StageScope sc(c, ST_BEIG, Kc);
+            hipLaunchKernelGGL(lqm::k_sturm_bisect, dim3((N + 255) / 256, Kc), dim3(256), 0, s, N, dd, ee, lamw, tn);
+        }
+        {
+            StageScope sc(c, ST_BVEC, Kc);
+            hipLaunchKernelGGL(lqm::k_invit, dim3((N + 63) / 64, Kc), dim3(64), 0, s, N, N, N, dd, ee, lamw, tn, Zt,
+                               scr, k0, c->d_status);
+            hipLaunchKernelGGL(lqm::k_invit_orth, dim3((N + 3) / 4, Kc), dim3(256), 0, s, N, N, N, lamw, tn,
+                               kClusterRel, Zt);
+        }
+        LQM_HIP(hipGetLastError());
+        // X32 = Q Z in FP32: Y explicit in A32, T per block of kBwy reflectors, blocks last first
+        {
+            StageScope sc(c, ST_BBT, Kc);
+            hipLaunchKernelGGL(lqm::k_make_y<float>, dim3((unsigned)((NN + 255) / 256), Kc), dim3(256), 0, s, N, A32);
+            if (N > 1)
+                hipLaunchKernelGGL(lqm::k_wy_T<float>, dim3(nblk, Kc), dim3(256), 0, s, N, nblk, A32, tau, Tw);
+            hipLaunchKernelGGL(lqm::k_z_to_v<float>, dim3((N + 31) / 32, (N + 31) / 32, Kc), dim3(256), 0, s, N, N, N,
+                               Zt, X32);
+        }
+        LQM_HIP(hipGetLastError());
+        const long long sV = (long long)NN, sW = (long long)lqm::kBwy * N;
+        for (int b = (N > 1 ? nblk - 1 : -1); b >= 0; --b) {
+            const int jb = lqm::kBwy * b, nbb = std::min(lqm::kBwy, N - 1 - jb), r0 = jb + 1, Mr = N - r0;
+            const cx<float>* Yb = A32 + (size_t)jb * N + r0;
+            zgemm_gen<1, 0, float>(c, ST_BBT, Kc, Kc,
+                                   gemm_args<float>(nbb, N, Mr, Yb, N, (long long)NN, X32 + r0, N, sV, Wt, lqm::kBwy,
+                                                    sW, one, zero));
+            zgemm_gen<0, 0, float>(c, ST_BBT, Kc, Kc,
+                                   gemm_args<float>(nbb, N, nbb, Tw + (size_t)b * lqm::kBwy * lqm::kBwy, lqm::kBwy,
+                                                    (long long)nblk * lqm::kBwy * lqm::kBwy, Wt, lqm::kBwy, sW, TW,
+                                                    lqm::kBwy, sW, one, zero));
+            zgemm_gen<0, 0, float>(c, ST_BBT, Kc, Kc,
+                                   gemm_args<float>(Mr, N, nbb, Yb, N, (long long)NN, TW, lqm::kBwy, sW, X32 + r0, N,
+                                                    sV, {-1.0, 0.0}, one));
+        }
+        LQM_HIP(hipGetLastError());
+        convert_launch_f2d(c, NN * Kc, X32, X);
+        LQM_HIP(hipGetLastError());
+        // Ogita-Aishima sweeps over every k-point of the chunk
+        const size_t shc = lqm::oa_lds_bytes(N, true);
+        allow_lds(lqm::k_oa_correct, shc);
+        int sweeps = 0;
+        hstat.resize((size_t)4 * Kc);
+        for (;;) {
+            const int stg = sweeps == 0 ? ST_OA : ST_OA2;
+            zgemm_gen<0, 0>(c, stg, Kc, Kc, gemm_args(N, N, N, A, N, (long long)NN, X, N, (long long)NN, Wm, N,
+                                                      (long long)NN, one, zero));  // W = A X
+            zgemm_gen<1, 0>(c, stg, 0, Kc, gemm_args(N, N, N, X, N, (long long)NN, Wm, N, (long long)NN, Sm, N,
+                                                     (long long)NN, one, zero));  // S = X^H A X
+            zgemm_gen<1, 0>(c, stg, 0, Kc, gemm_args(N, N, N, X, N, (long long)NN, X, N, (long long)NN, Gm, N,
+                                                     (long long)NN, one, zero));  // G = X^H X
+            {
+                StageScope sc(c, stg);
+                hipLaunchKernelGGL(lqm::k_oa_correct, dim3(Kc), dim3(256), shc, s, N, Sm, Gm, Wm, lam, stat,
+                                   c->refine_thr, nullptr, Ug);
+                // X <- X + X E (E in W), formed in G
+                LQM_HIP(hipMemcpyAsync(Gm, X, sizeof(cx<double>) * NN * Kc, hipMemcpyDeviceToDevice, s));
+            }
+            zgemm_gen<0, 0>(c, stg, 0, Kc, gemm_args(N, N, N, X, N, (long long)NN, Wm, N, (long long)NN, Gm, N,
+                                                     (long long)NN, one, one));
+            LQM_HIP(hipGetLastError());
+            std::swap(X, Gm);
+            ++sweeps;
+            LQM_HIP(hipMemcpyAsync(hstat.data(), stat, sizeof(double) * 4 * Kc, hipMemcpyDeviceToHost, s));
+            LQM_HIP(hipStreamSynchronize(s));
+            bool again = false;
+            int unresolved = 0;
+            for (int k = 0; k < Kc; ++k) {
+                rmax = std::max(rmax, hstat[4 * k]);
+                dmax = std::max(dmax, hstat[4 * k + 1]);
+                const int fl = (int)hstat[4 * k + 2];
+                unresolved += (fl & 2) != 0;
+                again = again || (fl & 4);
+                if (!std::isfinite(hstat[4 * k]) || !std::isfinite(hstat[4 * k + 3]))
+                    return fail(LQM_ERR_NONFINITE, "mixed precision: non-finite refinement residual");
+            }
+            c->report[4] += (double)Kc;
+            if (!again) break;
+            if (sweeps >= max_sw) {
+                nnear += unresolved;
+                if (unresolved)
+                    return fail(LQM_ERR_NOCONV, "mixed precision: a near-degenerate cluster wider than 8 "
+                                                "eigenvalues could not be resolved; use the FP64 path");
+                break;
+            }
+        }
+        sweeps_max = std::max(sweeps_max, sweeps);
+        // the workspace slots keep their pointers: X may now be the G slot's buffer
+        double* bk = bands ? bands + (size_t)k0 * N : lamw;
+        {
+            StageScope sc(c, ST_REDUCE);
+            hipLaunchKernelGGL(lqm::k_sort_evals, dim3(Kc), dim3(256), 0, s, N, lam, bk);
+        }
+        LQM_HIP(hipGetLastError());
+        if (mq) {
+            if ((st = mu_from_bands(c, N, Nk, bk, kw, mq->filling, T, &mu))) return st;
+            *mq->mu = mu;
+        }
+        if (!want_rho) continue;
+        double* eb;
+        if ((st = wsT(c, W_EB, (size_t)Kmax, &eb))) return st;
+        cx<double>* rhok = nullptr;
+        if (rho && nRr > 0 && (st = wsT(c, W_RHOK, NN * Kmax, &rhok))) return st;
+        const double* occk = occ ? occ + (size_t)k0 * N : nullptr;
+        if (occk) {
+            // given occupations by eigenvalue rank: Y = X diag(f) (into S), rho_k = Y X^H
+            {
+                StageScope sc(c, ST_REDUCE);
+                hipLaunchKernelGGL(lqm::k_scale_occ, dim3(Kc), dim3(256), sizeof(double) * N, s, N, X, lam, kwc, wuni,
+                                   mu, T, occk, Sm, eb, nullptr);
+            }
+            if (rhok)
+                zgemm_gen<0, 1>(c, ST_BRHO, Kc, Kc, gemm_args(N, N, N, Sm, N, (long long)NN, X, N, (long long)NN, rhok,
+                                                              N, (long long)NN, one, zero));
+        } else {
+            {
+                StageScope sc(c, ST_REDUCE);
+                hipLaunchKernelGGL(lqm::k_occ_eb, dim3(Kc), dim3(256), 0, s, N, lam, kwc, wuni, mu, T, nullptr, fo, eb);
+            }
+            // columns up to the last whose occupation reaches kOccTiny (columns are in the
+            // ascending order of the FP32 eigenvalues, so this is about the occupied half)
+            int ncol = 0;
+            if (rhok) {
+                hlam.resize((size_t)Kc * N);
+                LQM_HIP(hipMemcpyAsync(hlam.data(), lam, sizeof(double) * Kc * N, hipMemcpyDeviceToHost, s));
+                LQM_HIP(hipStreamSynchronize(s));
+                for (int k = 0; k < Kc; ++k)
+                    for (int n = N - 1; n >= ncol; --n) {
+                        const double x = hlam[(size_t)k * N + n] - mu;
+                        const double f = T > 0.0 ? 1.0 / (std::exp(x / T) + 1.0) : (x < 0.0 ? 1.0 : (x > 0.0 ? 0.0 : 0.5));
+                        if (!(f < kOccTiny)) {
+                            ncol = n + 1;
+                            break;
+                        }
+                    }
+            }
+            if (rhok && ncol > 0) {
+                lqm::ZGemmArgs g = gemm_args(N, N, ncol, X, N, (long long)NN, X, N, (long long)NN, rhok, N,
+                                             (long long)NN, one, zero);
+                g.ascale = fo;
+                g.sS = N;
+                zgemm_gen<0, 1>(c, ST_BRHO, Kc, Kc, g);
+            } else if (rhok) {
+                LQM_HIP(hipMemsetAsync(rhok, 0, sizeof(cx<double>) * NN * Kc, s));
+            }
+        }
+        if (eband) {
+            StageScope sc(c, ST_REDUCE);
+            hipLaunchKernelGGL(lqm::k_sum, dim3(1), dim3(256), 0, s, Kc, eb, eband, 1);
+        }
+        LQM_HIP(hipGetLastError());
+        if (!rhok) continue;
+        const int Sc = std::min(S, Kc);
+        const int ksplit = (Kc + Sc - 1) / Sc;
+        const int Sused = (Kc + ksplit - 1) / ksplit;
+        if ((st = launch_contract(RG, c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho)))
+            return st;
+        LQM_HIP(hipGetLastError());
+    }
+    c->report[0] = sweeps_max;
+    c->report[1] = rmax;
+    c->report[2] = dmax;
+    c->report[3] = nnear;
+    return check_status(c, "mixed-precision blocked eigensolver");
 }
 
 lqm_status dos_bands_dev(lqm_context* c, int N, int Nk, const double* bands, const double* kw,

@@ -209,12 +209,21 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(int N, const cx<double>* __r
 // real symmetric Jacobi rotation t = sign(tau)/(|tau| + sqrt(1 + tau^2)).
 constexpr int kClusterMax = 8;
 
-// Cluster slots kept in LDS by k_oa_correct, and its dynamic LDS size.
+// Cluster slots of k_oa_correct: up to 64 with their rotations in LDS, or (large N, where a
+// symmetric spectrum can hold hundreds of degenerate pairs) N / 2 + 1 with the rotations in a
+// global array of (N / 2 + 1) * kClusterMax^2 complex per k-point; and its dynamic LDS size.
 __host__ __device__ inline int oa_max_clusters(int N) { return N / 2 + 1 < 64 ? N / 2 + 1 : 64; }
-inline size_t oa_lds_bytes(int N) {
-    const int maxcl = oa_max_clusters(N);
-    const size_t o_red = (24 * (size_t)N + 8 * (size_t)maxcl + 15) & ~(size_t)15;
-    return o_red + 16 * 8 + (size_t)maxcl * kClusterMax * kClusterMax * 16;
+__host__ __device__ inline int oa_max_clusters_global(int N) { return N / 2 + 1; }
+// Chains of eigenvalues (consecutive gaps <= delta) wider than kClusterMax are split by their
+// first-order couplings (at most kWideMax such chains per k-point, each at most kWideLen long):
+// a boundary stays where every pair across it has |E_pq| <= kSplitCoupling.
+constexpr int kWideMax = 64;
+constexpr int kWideLen = 1024;
+constexpr double kSplitCoupling = 1.0 / 64.0;
+inline size_t oa_lds_bytes(int N, bool global_u = false) {
+    const int maxcl = global_u ? oa_max_clusters_global(N) : oa_max_clusters(N);
+    const size_t o_red = (32 * (size_t)N + 8 * (size_t)maxcl + 8 * (size_t)kWideMax + 15) & ~(size_t)15;
+    return o_red + 16 * 8 + (global_u ? 0 : (size_t)maxcl * kClusterMax * kClusterMax * 16);
 }
 
 __device__ void jacobi_herm(int c, cx<double>* K, cx<double>* U) {
@@ -259,6 +268,57 @@ __device__ void jacobi_herm(int c, cx<double>* K, cx<double>* U) {
     }
 }
 
+// Rayleigh-Ritz on the cluster of columns i .. i + c - 1 (c <= kClusterMax): K = B^H S_CC B with
+// B = I + R_CC / 2 diagonalised by Jacobi, U <- B U (the new coefficients of the cluster columns).
+// Writes lnew, cid, cpos of the members and U.
+__device__ void oa_rayleigh_ritz(int N, int i, int c, int ncl, const cx<double>* Sk, const cx<double>* Gk,
+                                 double* lnew, int* cid, int* cpos, cx<double>* U) {
+    cx<double> K[kClusterMax * kClusterMax], B[kClusterMax * kClusterMax];
+    for (int a = 0; a < c; ++a)
+        for (int b = 0; b < c; ++b) {
+            const cx<double> g = Gk[(size_t)(i + b) * N + (i + a)];
+            const double rre = (a == b ? 1.0 : 0.0) - g.re, rim = -g.im;
+            B[a * kClusterMax + b] = {(a == b ? 1.0 : 0.0) + 0.5 * rre, 0.5 * rim};
+        }
+    for (int a = 0; a < c; ++a)
+        for (int b = 0; b < c; ++b) {
+            cx<double> acc = {0.0, 0.0};
+            for (int x = 0; x < c; ++x)
+                for (int y = 0; y < c; ++y) {
+                    const cx<double> sxy = Sk[(size_t)(i + y) * N + (i + x)];
+                    acc = cmad(acc, conj(B[x * kClusterMax + a]) * sxy, B[y * kClusterMax + b]);
+                }
+            K[a * kClusterMax + b] = acc;
+        }
+    jacobi_herm(c, K, U);
+    for (int a = 0; a < c; ++a) {
+        lnew[i + a] = K[a * kClusterMax + a].re;
+        cid[i + a] = ncl;
+        cpos[i + a] = a;
+    }
+    cx<double> BU[kClusterMax * kClusterMax];
+    for (int a = 0; a < c; ++a)
+        for (int b = 0; b < c; ++b) {
+            cx<double> acc = {0.0, 0.0};
+            for (int x = 0; x < c; ++x) acc = cmad(acc, B[a * kClusterMax + x], U[x * kClusterMax + b]);
+            BU[a * kClusterMax + b] = acc;
+        }
+    for (int q = 0; q < c * kClusterMax; ++q) U[q] = BU[q];
+}
+
+// First-order coupling of columns p and q: max(|E_pq|, |E_qp|) with E_pq = (s_pq - l_q g_pq) /
+// (l_q - l_p) (off-diagonal r = -g); infinity when the Rayleigh quotients coincide.
+__device__ __forceinline__ double oa_coupling(int N, int p, int q, const double* lam, const cx<double>* Sk,
+                                              const cx<double>* Gk) {
+    const double gap = lam[q] - lam[p];
+    if (gap == 0.0) return INFINITY;
+    const cx<double> spq = Sk[(size_t)q * N + p], gpq = Gk[(size_t)q * N + p];
+    const cx<double> sqp = Sk[(size_t)p * N + q], gqp = Gk[(size_t)p * N + q];
+    const double a = abs2(cx<double>{spq.re - lam[q] * gpq.re, spq.im - lam[q] * gpq.im});
+    const double b = abs2(cx<double>{sqp.re - lam[p] * gqp.re, sqp.im - lam[p] * gqp.im});
+    return sqrt(fmax(a, b)) / fabs(gap);
+}
+
 // Ogita-Aishima correction with Rayleigh-Ritz on near-degenerate clusters.
 // In: S = X^H A X, G = X^H X.  Out: E[kk] (X <- X + X E), lambda[kk*N + i], and
 // stat[kk*4 + {0: ||R||_F, 1: delta, 2: flags, 3: max |E_ij| outside the clusters}] where
@@ -276,12 +336,15 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
                                                     cx<double>* __restrict__ E,
                                                     double* __restrict__ lambda,
                                                     double* __restrict__ stat, double thr,
-                                                    const int* __restrict__ kidx) {
+                                                    const int* __restrict__ kidx,
+                                                    cx<double>* __restrict__ Ug = nullptr) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int maxcl = oa_max_clusters(N);
+    const int maxcl = Ug ? oa_max_clusters_global(N) : oa_max_clusters(N);
     const size_t o_lnew = 8 * (size_t)N, o_cid = o_lnew + 8 * (size_t)N, o_cpos = o_cid + 4 * (size_t)N;
-    const size_t o_cst = o_cpos + 4 * (size_t)N, o_csz = o_cst + 4 * (size_t)maxcl;
-    const size_t o_red = (o_csz + 4 * (size_t)maxcl + 15) & ~(size_t)15;
+    const size_t o_grp = o_cpos + 4 * (size_t)N, o_reach = o_grp + 4 * (size_t)N;
+    const size_t o_cst = o_reach + 4 * (size_t)N, o_csz = o_cst + 4 * (size_t)maxcl;
+    const size_t o_wide = o_csz + 4 * (size_t)maxcl;
+    const size_t o_red = (o_wide + 8 * (size_t)kWideMax + 15) & ~(size_t)15;
     const size_t o_u = o_red + 16 * 8;
     double* lam = reinterpret_cast<double*>(smem);               // [N]  Rayleigh quotients
     double* lnew = reinterpret_cast<double*>(smem + o_lnew);     // [N]  final lambda
@@ -289,10 +352,14 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
     int* cpos = reinterpret_cast<int*>(smem + o_cpos);           // [N]  position in the cluster
     int* cstart = reinterpret_cast<int*>(smem + o_cst);          // [maxcl]
     int* csize = reinterpret_cast<int*>(smem + o_csz);           // [maxcl]
+    int* grp = reinterpret_cast<int*>(smem + o_grp);             // [N]  sub-cluster of a split chain, or -1
+    int* reach = reinterpret_cast<int*>(smem + o_reach);         // [N]  last column coupled to this one
+    int* wide = reinterpret_cast<int*>(smem + o_wide);           // [2 kWideMax] wide chains (start, end)
     double* red = reinterpret_cast<double*>(smem + o_red);       // [16]
-    cx<double>* Ubuf = reinterpret_cast<cx<double>*>(smem + o_u);  // [maxcl][kClusterMax^2]
-    __shared__ int s_ncl, s_flags;
     const int kk = kidx ? kidx[blockIdx.x] : (int)blockIdx.x;
+    cx<double>* Ubuf = Ug ? Ug + (size_t)kk * maxcl * kClusterMax * kClusterMax
+                          : reinterpret_cast<cx<double>*>(smem + o_u);  // [maxcl][kClusterMax^2]
+    __shared__ int s_ncl, s_flags, s_nwide;
     const int tid = threadIdx.x;
     const size_t NN = (size_t)N * N;
     const cx<double>* Sk = S + (size_t)kk * NN;
@@ -302,6 +369,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
         lam[i] = Sk[(size_t)i * N + i].re / Gk[(size_t)i * N + i].re;
         lnew[i] = lam[i];
         cid[i] = -1;
+        grp[i] = -1;
     }
     __syncthreads();
     double r2 = 0.0, amax = 0.0;
@@ -334,9 +402,10 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
     const double an = fmax(fmax(red[8], red[9]), fmax(red[10], red[11]));
     const double delta = 2.0 * (sn_all + an * rn);
     const double tiny = 64.0 * 2.220446049250313e-16 * (an > 0.0 ? an : 1.0);
-    // cluster detection and Rayleigh-Ritz (one thread: clusters are rare and small)
+    // cluster detection and Rayleigh-Ritz (one thread: clusters are rare and small); chains
+    // wider than kClusterMax are listed for splitting
     if (tid == 0) {
-        int ncl = 0, flags = 0;
+        int ncl = 0, flags = 0, nw = 0;
         int i = 0;
         while (i < N) {
             int j = i;
@@ -345,43 +414,15 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
             if (c >= 2) {
                 double spread = 0.0;
                 for (int q = i + 1; q <= j; ++q) spread = fmax(spread, fabs(lam[q] - lam[i]));
-                if (c > kClusterMax || ncl >= maxcl) {
+                if (c > kClusterMax && c <= kWideLen && nw < kWideMax) {
+                    wide[2 * nw] = i;
+                    wide[2 * nw + 1] = j;
+                    ++nw;
+                } else if (c > kClusterMax || ncl >= maxcl) {
                     if (spread > tiny) flags |= 2;
                 } else {
-                    cx<double> K[kClusterMax * kClusterMax], B[kClusterMax * kClusterMax];
-                    // B = I + R_CC / 2 (R = I - G), K = B^H S_CC B
-                    for (int a = 0; a < c; ++a)
-                        for (int b = 0; b < c; ++b) {
-                            const cx<double> g = Gk[(size_t)(i + b) * N + (i + a)];
-                            const double rre = (a == b ? 1.0 : 0.0) - g.re, rim = -g.im;
-                            B[a * kClusterMax + b] = {(a == b ? 1.0 : 0.0) + 0.5 * rre, 0.5 * rim};
-                        }
-                    for (int a = 0; a < c; ++a)
-                        for (int b = 0; b < c; ++b) {
-                            cx<double> acc = {0.0, 0.0};
-                            for (int x = 0; x < c; ++x)
-                                for (int y = 0; y < c; ++y) {
-                                    const cx<double> sxy = Sk[(size_t)(i + y) * N + (i + x)];
-                                    acc = cmad(acc, conj(B[x * kClusterMax + a]) * sxy, B[y * kClusterMax + b]);
-                                }
-                            K[a * kClusterMax + b] = acc;
-                        }
-                    cx<double>* U = Ubuf + (size_t)ncl * kClusterMax * kClusterMax;
-                    jacobi_herm(c, K, U);
-                    for (int a = 0; a < c; ++a) {
-                        lnew[i + a] = K[a * kClusterMax + a].re;
-                        cid[i + a] = ncl;
-                        cpos[i + a] = a;
-                    }
-                    // store BU in place of U (the new coefficients of the cluster columns)
-                    cx<double> BU[kClusterMax * kClusterMax];
-                    for (int a = 0; a < c; ++a)
-                        for (int b = 0; b < c; ++b) {
-                            cx<double> acc = {0.0, 0.0};
-                            for (int x = 0; x < c; ++x) acc = cmad(acc, B[a * kClusterMax + x], U[x * kClusterMax + b]);
-                            BU[a * kClusterMax + b] = acc;
-                        }
-                    for (int q = 0; q < c * kClusterMax; ++q) U[q] = BU[q];
+                    oa_rayleigh_ritz(N, i, c, ncl, Sk, Gk, lnew, cid, cpos,
+                                     Ubuf + (size_t)ncl * kClusterMax * kClusterMax);
                     cstart[ncl] = i;
                     csize[ncl] = c;
                     ++ncl;
@@ -392,8 +433,67 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
         }
         s_ncl = ncl;
         s_flags = flags;
+        s_nwide = nw;
     }
     __syncthreads();
+    if (s_nwide > 0) {
+        // reach[p]: the last column q of p's chain with a first-order coupling above
+        // kSplitCoupling (p itself when none), one thread per column
+        for (int w = 0; w < s_nwide; ++w) {
+            const int i0 = wide[2 * w], j0 = wide[2 * w + 1];
+            for (int p = i0 + tid; p <= j0; p += 256) {
+                int rq = p;
+                for (int q = j0; q > p; --q)
+                    if (!(oa_coupling(N, p, q, lam, Sk, Gk) <= kSplitCoupling)) {
+                        rq = q;
+                        break;
+                    }
+                reach[p] = rq;
+            }
+        }
+        __syncthreads();
+        // sub-clusters: a boundary after p where no column from the sub-cluster's start to p
+        // reaches beyond p; each gets its own group (pairs across groups take the first-order
+        // correction), and groups of 2 .. kClusterMax columns the Rayleigh-Ritz step
+        if (tid == 0) {
+            int ncl = s_ncl, flags = s_flags, gid = 0;
+            for (int w = 0; w < s_nwide; ++w) {
+                const int i0 = wide[2 * w], j0 = wide[2 * w + 1];
+                int b0 = i0, rm = i0;
+                for (int p = i0; p <= j0; ++p) {
+                    rm = max(rm, reach[p]);
+                    if (rm > p) continue;
+                    const int c = p - b0 + 1;
+                    for (int q = b0; q <= p; ++q) grp[q] = gid;
+                    ++gid;
+                    if (c >= 2) {
+                        double spread = 0.0;
+                        for (int q = b0 + 1; q <= p; ++q) spread = fmax(spread, fabs(lam[q] - lam[b0]));
+                        if (c > kClusterMax || ncl >= maxcl) {
+                            if (spread > tiny) flags |= 2;
+                        } else {
+                            oa_rayleigh_ritz(N, b0, c, ncl, Sk, Gk, lnew, cid, cpos,
+                                             Ubuf + (size_t)ncl * kClusterMax * kClusterMax);
+                            cstart[ncl] = b0;
+                            csize[ncl] = c;
+                            ++ncl;
+                            if (spread > tiny) flags |= 1;
+                        }
+                    }
+                    b0 = p + 1;
+                    rm = p + 1;
+                }
+            }
+            s_ncl = ncl;
+            s_flags = flags;
+        }
+        __syncthreads();
+    }
+    // pair (i, j) takes the first-order correction when its Rayleigh quotients are more than
+    // delta apart or it lies in different sub-clusters of a split chain
+    auto separated = [&](int a, int b) {
+        return fabs(lam[b] - lam[a]) > delta || (grp[a] >= 0 && grp[b] >= 0 && grp[a] != grp[b]);
+    };
     // E entries
     double emax = 0.0;
     for (int e = tid; e < N * N; e += 256) {
@@ -413,7 +513,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
                 const double rre = -g.re, rim = -g.im;  // i != ll
                 const double gap = lam[ll] - lam[i];
                 cx<double> est;
-                if (fabs(gap) > delta) {
+                if (separated(i, ll)) {
                     const cx<double> sv = Sk[(size_t)ll * N + i];
                     est = {(sv.re + lam[ll] * rre) / gap, (sv.im + lam[ll] * rim) / gap};
                 } else {
@@ -425,7 +525,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
             const cx<double> g = Gk[e];
             const double rre = (i == j ? 1.0 : 0.0) - g.re, rim = -g.im;
             const double gap = lam[j] - lam[i];
-            if (i != j && fabs(gap) > delta) {
+            if (i != j && separated(i, j)) {
                 const cx<double> sv = Sk[e];
                 out = {(sv.re + lam[j] * rre) / gap, (sv.im + lam[j] * rim) / gap};
             } else {
@@ -449,7 +549,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
         double corr = 0.0;
         for (int i = 0; i < N; ++i) {
             const double gap = lj - lam[i];
-            if (i == j || fabs(gap) <= delta) continue;
+            if (i == j || !separated(i, j)) continue;
             const cx<double> sv = Sk[(size_t)j * N + i], g = Gk[(size_t)j * N + i];
             const double tre = sv.re - lj * g.re, tim = sv.im - lj * g.im;
             corr += (tre * tre + tim * tim) / gap;
@@ -529,7 +629,7 @@ __global__ __launch_bounds__(256) void k_scale_occ(int N, const cx<double>* __re
     }
 }
 
-// Ascending copy of lambda per k-point (rank counting; N <= 1024).
+// Ascending copy of lambda per k-point (rank counting, O(N^2) per k-point).
 __global__ __launch_bounds__(256) void k_sort_evals(int N, const double* __restrict__ lambda,
                                                     double* __restrict__ out) {
     const int kk = blockIdx.x;

@@ -107,7 +107,7 @@ def test_blocked_densitymatrix_occ(pkg, oracle, blocked):
 
 
 def test_default_path_above_512(pkg, oracle, gpu):
-    # N > 512 takes the large-N path by default, in both precision modes
+    # N > 512 takes the large-N path by default, in the precision the executor asks for
     N = 600
     h = pkg.models.random_tb(N, rmax=1, seed=3)
     ks = pkg.regulargrid(nk=4)
@@ -116,6 +116,7 @@ def test_default_path_above_512(pkg, oracle, gpu):
     refb = oracle.bands(R, Tt, ks)
     for ex in (gpu, pkg.api.to_executor("gpu", precision="mixed")):
         b, _ = pkg.api.bandmatrix(h, ks, executor=ex)
+        assert ex.last_precision() == ("fp64" if ex is gpu else "mixed")
         assert np.max(np.abs(b - refb) / np.maximum(np.abs(refb), _hnorm(h, ks)[None, :])) < 1e-10
         rho = pkg.Hops({key: np.zeros((N, N), complex) for key in h.keys()})
         eb = pkg.api.getdensitymatrix(rho, h, ks, 0.0, T=0.02, executor=ex)
@@ -181,3 +182,93 @@ def test_autotune_chooses_eigensolver_layout(pkg, oracle, gpu):
     err = max(np.max(np.abs(rho[tuple(R[:, r])] - ref[:, :, r])) for r in range(R.shape[1]))
     assert err < 1e-9, err
     assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+
+
+# ---- the large-N path in mixed precision: FP32 reduction, FP64 Ogita-Aishima refinement ----
+@pytest.fixture(scope="module")
+def mixed_blocked(pkg, gpu):
+    """Mixed-precision executor with the large-N path forced for every N."""
+    ex = pkg.api.to_executor("gpu", precision="mixed")
+    ex.set_variant(5, 1)
+    yield ex
+    ex.set_variant(5, -1)
+
+
+@pytest.mark.parametrize("N,nk", [(1, 4), (2, 9), (33, 9), (97, 4), (200, 4)])
+def test_mixed_blocked_bands(pkg, oracle, mixed_blocked, N, nk):
+    h = pkg.models.random_tb(N, rmax=1, seed=500 + N)
+    ks = pkg.regulargrid(nk=nk)
+    R, T = h.pack()
+    ref = oracle.bands(R, T, ks)
+    b, _ = pkg.api.bandmatrix(h, ks, executor=mixed_blocked)
+    err = np.abs(b - ref) / np.maximum(np.abs(ref), _hnorm(h, ks)[None, :])
+    assert np.max(err) < 1e-10, np.max(err)
+    assert mixed_blocked.last_precision() == "mixed"
+    assert mixed_blocked.refinement_report()["sweeps"] >= 1
+
+
+@pytest.mark.parametrize("N,nk,mu,T", [(5, 16, 0.1, 0.05), (70, 9, 0.0, 0.02), (130, 4, 0.2, 0.0)])
+def test_mixed_blocked_rho(pkg, oracle, mixed_blocked, N, nk, mu, T):
+    h = pkg.models.random_tb(N, rmax=1, seed=600 + N)
+    ks = pkg.regulargrid(nk=nk)
+    rho = pkg.Hops({R: np.zeros((N, N), complex) for R in h.keys()})
+    eb = pkg.api.getdensitymatrix(rho, h, ks, mu, T=T, executor=mixed_blocked)
+    R, Tt = h.pack()
+    ref, eb_ref = oracle.densitymatrix(R, Tt, ks, R, mu=mu, Temp=T)
+    err = max(np.max(np.abs(rho[tuple(R[:, r])] - ref[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+    rep = mixed_blocked.refinement_report()
+    assert mixed_blocked.last_precision() == "mixed" and rep["max_orth_residual"] < 1e-3
+
+
+def test_mixed_blocked_degenerate_and_occ(pkg, oracle, mixed_blocked):
+    # exact spin degeneracy (clusters resolved by the Rayleigh-Ritz step), then given occupations
+    h = _degenerate_model(pkg, 40, 9)
+    ks = pkg.regulargrid(nk=4)
+    R, Tt = h.pack()
+    ref = oracle.bands(R, Tt, ks)
+    b, _ = pkg.api.bandmatrix(h, ks, executor=mixed_blocked)
+    assert np.max(np.abs(b - ref) / np.maximum(np.abs(ref), _hnorm(h, ks)[None, :])) < 1e-10
+    rho = pkg.Hops({key: np.zeros((80, 80), complex) for key in h.keys()})
+    eb = pkg.api.getdensitymatrix(rho, h, ks, 0.05, T=0.0, executor=mixed_blocked)
+    refr, eb_ref = oracle.densitymatrix(R, Tt, ks, R, mu=0.05, Temp=0.0)
+    err = max(np.max(np.abs(rho[tuple(R[:, r])] - refr[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+    N = 60
+    h = pkg.models.random_tb(N, rmax=1, seed=88)
+    ks = pkg.regulargrid(nk=9)
+    R, Tt = h.pack()
+    b, V = oracle.bands(R, Tt, ks, vectors=True)
+    occ = np.random.default_rng(5).standard_normal(b.shape)
+    rhos = pkg.Hops({key: np.zeros((N, N), complex) for key in [(0, 0), (1, 0), (1, -1)]})
+    pkg.api.densitymatrix_occ(rhos, h, ks, occ, executor=mixed_blocked)
+    for key in rhos.keys():
+        ref = sum(np.exp(-2j * np.pi * np.dot(ks[:, j], key)) * (V[:, :, j] * occ[:, j]) @ V[:, :, j].conj().T
+                  for j in range(ks.shape[1])) / ks.shape[1]
+        assert np.max(np.abs(rhos[key] - ref)) < 1e-9
+
+
+def test_tbg_m11_slater_koster_mixed(pkg, oracle, gpu):
+    # the 1588-orbital twisted bilayer at Gamma and a generic k-point through the FP32 blocked
+    # eigensolver and the FP64 refinement (the default layout above 112 orbitals)
+    ex = pkg.api.to_executor("gpu", precision="mixed")
+    h = pkg.models.tbg_slater_koster(11)
+    N = h[(0, 0)].shape[0]
+    ks = np.array([[0.0, 0.0], [0.137, 0.291]]).T
+    R, Tt = h.pack()
+    refb = oracle.bands(R, Tt, ks)
+    b, _ = pkg.api.bandmatrix(h, ks, executor=ex)
+    assert ex.last_precision() == "mixed"
+    hn = _hnorm(h, ks)
+    assert np.max(np.abs(b - refb) / np.maximum(np.abs(refb), hn[None, :])) < 1e-10
+    keys = [(0, 0), (1, 0), (0, 1)]
+    Rr = np.array(keys).T
+    ref, eb_ref = oracle.densitymatrix(R, Tt, ks, Rr, mu=0.0, Temp=0.01)
+    rho = pkg.Hops({key: np.zeros((N, N), complex) for key in keys})
+    eb = pkg.api.getdensitymatrix(rho, h, ks, 0.0, T=0.01, executor=ex)
+    err = max(np.max(np.abs(rho[key] - ref[:, :, r])) for r, key in enumerate(keys))
+    assert err < 1e-9, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+    assert ex.refinement_report()["sweeps"] >= 1
