@@ -186,8 +186,10 @@ def adjoint_leg(no_cpu: bool) -> dict:
 
 def tbg_leg(no_cpu: bool) -> dict:
     """The 1588-orbital twisted bilayer (twist index 11, Slater-Koster hopping, 9 keys) on
-    the 12 x 12 grid with the FP64 large-N path, device-resident: an eigenvalue pass
-    (H(k) + eigvals) and a full pass (H(k) + eigh + rho on the 9 keys, mu = 0, kT = 0.01)."""
+    the 12 x 12 grid, device-resident, on the large-N path in both precisions: FP64, and FP32
+    reduction + back-transformation with FP64 Ogita-Aishima refinement.  Per precision an
+    eigenvalue pass (H(k) + eigvals) and a full pass (H(k) + eigh + rho on the 9 keys, mu = 0,
+    kT = 0.01)."""
     import torch
 
     h = pkg.models.tbg_slater_koster(11)
@@ -200,22 +202,10 @@ def tbg_leg(no_cpu: bool) -> dict:
     Rr = np.asfortranarray(R.astype(np.int32))
     dev = torch.device("cuda:0")
     ks_d = torch.from_numpy(np.ascontiguousarray(ks.T)).to(dev)
-    ex = api.to_executor("gpu")
-    m = api.GPUModel(h, ex)
-    rho = torch.zeros((nRr, N, N, 2), dtype=torch.float64, device=dev)
-    eb = torch.zeros(1, dtype=torch.float64, device=dev)
-    bands = torch.zeros((Nk, N), dtype=torch.float64, device=dev)
     L = _lib.lib()
     vp = ctypes.c_void_p
-
-    def full():
-        _lib.check(L.lqm_densitymatrix_dev(m.handle, Nk, vp(ks_d.data_ptr()), None, mu, T, nRr,
-                                           Rr.ctypes.data_as(vp), vp(rho.data_ptr()), vp(eb.data_ptr()),
-                                           vp(bands.data_ptr())), "lqm_densitymatrix_dev")
-
-    def evals():
-        _lib.check(L.lqm_bandmatrix_dev(m.handle, Nk, vp(ks_d.data_ptr()), vp(bands.data_ptr()), None),
-                   "lqm_bandmatrix_dev")
+    fl_full = (64.0 / 3.0) * N ** 3 + 8.0 * (R.shape[1] + nRr) * N * N
+    fl_val = (16.0 / 3.0) * N ** 3 + 8.0 * R.shape[1] * N * N
 
     def clock(fn, reps=2):
         fn()
@@ -226,24 +216,52 @@ def tbg_leg(no_cpu: bool) -> dict:
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / reps
 
-    te = clock(evals)
-    tf = clock(full)
-    _lib.call("lqm_context_set_timing", ex.handle, 1)
-    full()
-    ms = (ctypes.c_double * _lib.NSTAGES)()
-    cnt = (ctypes.c_int64 * _lib.NSTAGES)()
-    _lib.call("lqm_context_stage_times", ex.handle, ms, cnt, None, _lib.NSTAGES)
-    _lib.call("lqm_context_set_timing", ex.handle, 0)
-    stages = {L.lqm_stage_name(i).decode(): {"ms": ms[i], "launches": cnt[i]}
-              for i in range(_lib.NSTAGES) if cnt[i]}
-    fl_full = (64.0 / 3.0) * N ** 3 + 8.0 * (R.shape[1] + nRr) * N * N
-    fl_val = (16.0 / 3.0) * N ** 3 + 8.0 * R.shape[1] * N * N
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ex = api.to_executor("gpu", precision=prec)
+        m = api.GPUModel(h, ex)
+        rho = torch.zeros((nRr, N, N, 2), dtype=torch.float64, device=dev)
+        eb = torch.zeros(1, dtype=torch.float64, device=dev)
+        bands = torch.zeros((Nk, N), dtype=torch.float64, device=dev)
+
+        def full():
+            _lib.check(L.lqm_densitymatrix_dev(m.handle, Nk, vp(ks_d.data_ptr()), None, mu, T, nRr,
+                                               Rr.ctypes.data_as(vp), vp(rho.data_ptr()), vp(eb.data_ptr()),
+                                               vp(bands.data_ptr())), "lqm_densitymatrix_dev")
+
+        def evals():
+            _lib.check(L.lqm_bandmatrix_dev(m.handle, Nk, vp(ks_d.data_ptr()), vp(bands.data_ptr()), None),
+                       "lqm_bandmatrix_dev")
+
+        te = clock(evals)
+        tf = clock(full)
+        _lib.call("lqm_context_set_timing", ex.handle, 1)
+        full()
+        ms = (ctypes.c_double * _lib.NSTAGES)()
+        cnt = (ctypes.c_int64 * _lib.NSTAGES)()
+        _lib.call("lqm_context_stage_times", ex.handle, ms, cnt, None, _lib.NSTAGES)
+        _lib.call("lqm_context_set_timing", ex.handle, 0)
+        stages = {L.lqm_stage_name(i).decode(): {"ms": ms[i], "launches": cnt[i]}
+                  for i in range(_lib.NSTAGES) if cnt[i]}
+        res[prec] = {"eigvals_pass": {"seconds": te, "kpoints_per_s": Nk / te,
+                                      "path_tflops_per_s": Nk * fl_val / te / 1e12},
+                     "full_pass": {"seconds": tf, "kpoints_per_s": Nk / tf,
+                                   "path_tflops_per_s": Nk * fl_full / tf / 1e12},
+                     "stages_full_pass_ms": stages, "ran": ex.last_precision(),
+                     "bands": bands.cpu().numpy(), "rho": rho.cpu().numpy(), "eb": eb.item()}
+        if prec == "mixed":
+            res[prec]["refinement"] = ex.refinement_report()
+        del m
+    b64, bmx = res["fp64"].pop("bands"), res["mixed"].pop("bands")
+    r64, rmx = res["fp64"].pop("rho"), res["mixed"].pop("rho")
+    e64, emx = res["fp64"].pop("eb"), res["mixed"].pop("eb")
+    hn = np.array([np.linalg.norm(h(k), 2) for k in ks.T])
     out = {"workload": "tbg_m11_slater_koster_N1588_R9_k12x12", "N": N, "nR": R.shape[1], "nRrho": nRr,
-           "Nk": Nk, "mu": mu, "kT": T, "precision": "fp64",
-           "eigvals_pass": {"seconds": te, "kpoints_per_s": Nk / te,
-                            "path_tflops_per_s": Nk * fl_val / te / 1e12},
-           "full_pass": {"seconds": tf, "kpoints_per_s": Nk / tf, "path_tflops_per_s": Nk * fl_full / tf / 1e12},
-           "stages_full_pass_ms": stages}
+           "Nk": Nk, "mu": mu, "kT": T, "fp64": res["fp64"], "mixed": res["mixed"],
+           "mixed_over_fp64_full_pass": res["fp64"]["full_pass"]["seconds"] / res["mixed"]["full_pass"]["seconds"],
+           "mixed_accuracy_vs_fp64_gpu": {
+               "max_eigval_err": float(np.max(np.abs(bmx - b64) / np.maximum(np.abs(b64), hn[:, None]))),
+               "max_abs_rho_err": float(np.max(np.abs(rmx - r64))), "abs_eband_err": abs(emx - e64)}}
     if not no_cpu:
         from oracle import oracle
 
@@ -258,9 +276,9 @@ def tbg_leg(no_cpu: bool) -> dict:
                                          f"H(k)+zheevr('V')+rho on {nRr} keys"}
         bref = oracle.bands(R, Tt, sub, nthreads=nth)
         idx = np.arange(Nk)[:: max(1, Nk // S)][:S]
-        bg = bands.cpu().numpy()[idx].T
-        hn = np.array([np.linalg.norm(h(k), 2) for k in sub.T])
-        out["max_eigval_err_vs_oracle"] = float(np.max(np.abs(bg - bref) / np.maximum(np.abs(bref), hn[None, :])))
+        for prec, bg in (("fp64", b64), ("mixed", bmx)):
+            out[prec]["max_eigval_err_vs_oracle"] = float(
+                np.max(np.abs(bg[idx].T - bref) / np.maximum(np.abs(bref), hn[idx][None, :])))
     return out
 
 

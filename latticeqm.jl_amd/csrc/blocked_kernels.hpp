@@ -396,6 +396,7 @@ struct GemmArgs {
     cx<R>* C;
     cx<double> alpha, beta;
     const double* ascale;
+    const int* kidx;  // batch entry of grid z (null: z itself)
 };
 using ZGemmArgs = GemmArgs<double>;
 
@@ -505,7 +506,7 @@ __global__ __launch_bounds__(256) void k_zgemm_gen(GemmArgs<R> g) {
     }
     const int m0 = ti * 64 + (wave & 1) * 32;
     const int n0 = tj * 64 + (wave >> 1) * 32;
-    const size_t b = blockIdx.z;
+    const size_t b = g.kidx ? (size_t)g.kidx[blockIdx.z] : (size_t)blockIdx.z;
     typename MM::acc cr[2][2], ci[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)

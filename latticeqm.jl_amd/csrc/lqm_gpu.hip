@@ -1211,6 +1211,7 @@ lqm::GemmArgs<R> gemm_args(int M, int N, int K, const cx<R>* A, int lda, long lo
     g.alpha = alpha;
     g.beta = beta;
     g.ascale = nullptr;
+    g.kidx = nullptr;
     return g;
 }
 
@@ -1603,46 +1604,65 @@ lqm_status run_blocked_mixed(lqm_model* m, int Nk, const double* ks, const doubl
         LQM_HIP(hipGetLastError());
         convert_launch_f2d(c, NN * Kc, X32, X);
         LQM_HIP(hipGetLastError());
-        // Ogita-Aishima sweeps over every k-point of the chunk
+        // Ogita-Aishima sweeps: the first over every k-point of the chunk, later ones over the
+        // k-points whose last correction asked for another (device index list)
         const size_t shc = lqm::oa_lds_bytes(N, true);
         allow_lds(lqm::k_oa_correct, shc);
         int sweeps = 0;
         hstat.resize((size_t)4 * Kc);
+        std::vector<int> cur(Kc), next;
+        for (int k = 0; k < Kc; ++k) cur[k] = k;
+        const int* kidx = nullptr;
+        int nsub = Kc;
         for (;;) {
             const int stg = sweeps == 0 ? ST_OA : ST_OA2;
-            zgemm_gen<0, 0>(c, stg, Kc, Kc, gemm_args(N, N, N, A, N, (long long)NN, X, N, (long long)NN, Wm, N,
-                                                      (long long)NN, one, zero));  // W = A X
-            zgemm_gen<1, 0>(c, stg, 0, Kc, gemm_args(N, N, N, X, N, (long long)NN, Wm, N, (long long)NN, Sm, N,
-                                                     (long long)NN, one, zero));  // S = X^H A X
-            zgemm_gen<1, 0>(c, stg, 0, Kc, gemm_args(N, N, N, X, N, (long long)NN, X, N, (long long)NN, Gm, N,
-                                                     (long long)NN, one, zero));  // G = X^H X
+            auto gg = [&](lqm::ZGemmArgs g) {
+                g.kidx = kidx;
+                return g;
+            };
+            zgemm_gen<0, 0>(c, stg, nsub, nsub, gg(gemm_args(N, N, N, A, N, (long long)NN, X, N, (long long)NN, Wm, N,
+                                                             (long long)NN, one, zero)));  // W = A X
+            zgemm_gen<1, 0>(c, stg, 0, nsub, gg(gemm_args(N, N, N, X, N, (long long)NN, Wm, N, (long long)NN, Sm, N,
+                                                          (long long)NN, one, zero)));  // S = X^H A X
+            zgemm_gen<1, 0>(c, stg, 0, nsub, gg(gemm_args(N, N, N, X, N, (long long)NN, X, N, (long long)NN, Gm, N,
+                                                          (long long)NN, one, zero)));  // G = X^H X
             {
                 StageScope sc(c, stg);
-                hipLaunchKernelGGL(lqm::k_oa_correct, dim3(Kc), dim3(256), shc, s, N, Sm, Gm, Wm, lam, stat,
-                                   c->refine_thr, nullptr, Ug);
+                hipLaunchKernelGGL(lqm::k_oa_correct, dim3(nsub), dim3(256), shc, s, N, Sm, Gm, Wm, lam, stat,
+                                   c->refine_thr, kidx, Ug);
                 // X <- X + X E (E in W), formed in G
-                LQM_HIP(hipMemcpyAsync(Gm, X, sizeof(cx<double>) * NN * Kc, hipMemcpyDeviceToDevice, s));
+                if (!kidx) {
+                    LQM_HIP(hipMemcpyAsync(Gm, X, sizeof(cx<double>) * NN * Kc, hipMemcpyDeviceToDevice, s));
+                } else {
+                    const int bx = (int)std::min<size_t>(1024, (NN + 255) / 256);
+                    hipLaunchKernelGGL(lqm::k_copy_sub, dim3(bx, nsub), dim3(256), 0, s, (int)NN, nsub, kidx, X, Gm);
+                }
             }
-            zgemm_gen<0, 0>(c, stg, 0, Kc, gemm_args(N, N, N, X, N, (long long)NN, Wm, N, (long long)NN, Gm, N,
-                                                     (long long)NN, one, one));
+            zgemm_gen<0, 0>(c, stg, 0, nsub, gg(gemm_args(N, N, N, X, N, (long long)NN, Wm, N, (long long)NN, Gm, N,
+                                                          (long long)NN, one, one)));
+            if (!kidx) {
+                std::swap(X, Gm);
+            } else {
+                const int bx = (int)std::min<size_t>(1024, (NN + 255) / 256);
+                hipLaunchKernelGGL(lqm::k_copy_sub, dim3(bx, nsub), dim3(256), 0, s, (int)NN, nsub, kidx, Gm, X);
+            }
             LQM_HIP(hipGetLastError());
-            std::swap(X, Gm);
             ++sweeps;
             LQM_HIP(hipMemcpyAsync(hstat.data(), stat, sizeof(double) * 4 * Kc, hipMemcpyDeviceToHost, s));
             LQM_HIP(hipStreamSynchronize(s));
-            bool again = false;
+            next.clear();
             int unresolved = 0;
-            for (int k = 0; k < Kc; ++k) {
+            for (int k : cur) {
                 rmax = std::max(rmax, hstat[4 * k]);
                 dmax = std::max(dmax, hstat[4 * k + 1]);
                 const int fl = (int)hstat[4 * k + 2];
                 unresolved += (fl & 2) != 0;
-                again = again || (fl & 4);
+                if (fl & 4) next.push_back(k);
                 if (!std::isfinite(hstat[4 * k]) || !std::isfinite(hstat[4 * k + 3]))
                     return fail(LQM_ERR_NONFINITE, "mixed precision: non-finite refinement residual");
             }
-            c->report[4] += (double)Kc;
-            if (!again) break;
+            c->report[4] += (double)cur.size();
+            if (next.empty()) break;
             if (sweeps >= max_sw) {
                 nnear += unresolved;
                 if (unresolved)
@@ -1650,6 +1670,12 @@ lqm_status run_blocked_mixed(lqm_model* m, int Nk, const double* ks, const doubl
                                                 "eigenvalues could not be resolved; use the FP64 path");
                 break;
             }
+            int* dk;
+            if ((st = wsT(c, W_KIDX, (size_t)Kc, &dk))) return st;
+            LQM_HIP(hipMemcpyAsync(dk, next.data(), sizeof(int) * next.size(), hipMemcpyHostToDevice, s));
+            kidx = dk;
+            nsub = (int)next.size();
+            cur.swap(next);
         }
         sweeps_max = std::max(sweeps_max, sweeps);
         // the workspace slots keep their pointers: X may now be the G slot's buffer

@@ -272,7 +272,7 @@ __device__ void jacobi_herm(int c, cx<double>* K, cx<double>* U) {
 // B = I + R_CC / 2 diagonalised by Jacobi, U <- B U (the new coefficients of the cluster columns).
 // Writes lnew, cid, cpos of the members and U.
 __device__ void oa_rayleigh_ritz(int N, int i, int c, int ncl, const cx<double>* Sk, const cx<double>* Gk,
-                                 double* lnew, int* cid, int* cpos, cx<double>* U) {
+                                   double* lnew, int* cid, int* cpos, cx<double>* U) {
     cx<double> K[kClusterMax * kClusterMax], B[kClusterMax * kClusterMax];
     for (int a = 0; a < c; ++a)
         for (int b = 0; b < c; ++b) {
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
                     if (spread > tiny) flags |= 2;
                 } else {
                     oa_rayleigh_ritz(N, i, c, ncl, Sk, Gk, lnew, cid, cpos,
-                                     Ubuf + (size_t)ncl * kClusterMax * kClusterMax);
+                                                        Ubuf + (size_t)ncl * kClusterMax * kClusterMax);
                     cstart[ncl] = i;
                     csize[ncl] = c;
                     ++ncl;
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
                             if (spread > tiny) flags |= 2;
                         } else {
                             oa_rayleigh_ritz(N, b0, c, ncl, Sk, Gk, lnew, cid, cpos,
-                                             Ubuf + (size_t)ncl * kClusterMax * kClusterMax);
+                                                                Ubuf + (size_t)ncl * kClusterMax * kClusterMax);
                             cstart[ncl] = b0;
                             csize[ncl] = c;
                             ++ncl;
@@ -539,12 +539,31 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
     for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o));
     __syncthreads();
     if ((tid & 63) == 0) red[12 + tid / 64] = emax;
-    // Second-order (Rayleigh-Schroedinger) correction of the Rayleigh quotient of every
-    // column outside a cluster, from the pencil (S, G):
-    //   lambda_j += sum_{i != j, |lambda_j - lambda_i| > delta} |s_ij - lambda_j g_ij|^2 / (lambda_j - lambda_i),
-    // so a single sweep leaves an eigenvalue error of third order in the FP32 error.
+    // Second-order (Rayleigh-Schroedinger) correction of every eigenvalue from the pencil (S, G):
+    //   lambda_j += sum_{i separated from j} |s_ij - lambda_j g_ij|^2 / (lambda_j - lambda_i)
+    // (for a rotated cluster column, its Rayleigh-Ritz value and rotated couplings), so a single
+    // sweep leaves an eigenvalue error of third order in the FP32 error.
     for (int j = tid; j < N; j += 256) {
-        if (cid[j] >= 0) continue;
+        if (cid[j] >= 0) {
+            // a rotated cluster column x~_a = sum_l x_{c0+l} BU_la: the same correction with its
+            // rotated couplings t_i = sum_l (s_{i,c0+l} - l~_a g_{i,c0+l}) BU_la to the columns
+            // outside the cluster that are separated from it
+            const int cj = cid[j], c0 = cstart[cj], cs = csize[cj], a = cpos[j];
+            const cx<double>* BU = Ubuf + (size_t)cj * kClusterMax * kClusterMax;
+            const double lj = lnew[j];
+            double corr = 0.0;
+            for (int i = 0; i < N; ++i) {
+                if (cid[i] == cj || !separated(i, j)) continue;
+                cx<double> t = {0.0, 0.0};
+                for (int l = 0; l < cs; ++l) {
+                    const cx<double> sv = Sk[(size_t)(c0 + l) * N + i], g = Gk[(size_t)(c0 + l) * N + i];
+                    t = cmad(t, cx<double>{sv.re - lj * g.re, sv.im - lj * g.im}, BU[l * kClusterMax + a]);
+                }
+                corr += abs2(t) / (lj - lam[i]);
+            }
+            lnew[j] = lj + corr;
+            continue;
+        }
         const double lj = lam[j];
         double corr = 0.0;
         for (int i = 0; i < N; ++i) {
