@@ -79,6 +79,13 @@ lqm_status lqm_context_synchronize(lqm_context *ctx);
  * names. */
 #define LQM_NSTAGES 26
 lqm_status lqm_context_set_timing(lqm_context *ctx, int enable);
+/* SCF iterations from captured HIP graphs (no reference counterpart; default on).  Where an
+ * iteration needs no host round trip (per-k FP64 kernels, T > 0, the grid in one chunk),
+ * lqm_hartreefock_scf runs the first iteration eagerly, captures 8 iterations in a graph and
+ * replays it until a device-side step kernel reports convergence, the iteration limit or an
+ * error (iterations after that leave the state unchanged); the results are bitwise those of
+ * the host-driven loop.  0 keeps the host-driven loop everywhere; timing mode does too. */
+lqm_status lqm_context_set_graphs(lqm_context *ctx, int enable);
 lqm_status lqm_context_stage_times(lqm_context *ctx, double *ms, int64_t *launches,
                                    int64_t *kpoints, int n);
 const char *lqm_stage_name(int stage);

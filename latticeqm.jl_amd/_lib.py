@@ -37,6 +37,7 @@ _PROTOS = {
     "lqm_context_set_stream": [P, P],
     "lqm_context_synchronize": [P],
     "lqm_context_set_timing": [P, I],
+    "lqm_context_set_graphs": [P, I],
     "lqm_context_stage_times": [P, P, P, P, I],
     "lqm_stage_name": [I],
     "lqm_context_set_variant": [P, I, I],

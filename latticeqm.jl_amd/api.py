@@ -78,6 +78,10 @@ class GPUExec:
                 "kpoints_near_degenerate": int(out[3]), "kpoint_sweeps": int(out[4]),
                 "kpoints_rho_fused": int(out[5])}
 
+    def set_graphs(self, enable: bool) -> None:
+        """Replay SCF iterations from captured HIP graphs where eligible (default on)."""
+        _lib.call("lqm_context_set_graphs", self.handle, int(bool(enable)))
+
     def last_precision(self) -> str | None:
         """Precision the last pass actually ran in ("fp64" / "mixed"; None before any pass).
         A mixed executor runs FP64 for eigenvector output and for the large-N layout."""

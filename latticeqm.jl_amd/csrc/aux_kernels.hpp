@@ -370,6 +370,12 @@ __global__ __launch_bounds__(256) void k_expval_cols(int N, const cx<double>* __
     }
 }
 
+// p[0 .. n) = 0 (32-bit words): the clearing step of passes that may be captured in a graph,
+// where a memset node is avoided.
+__global__ __launch_bounds__(256) void k_zero_words(size_t n, uint32_t* __restrict__ p) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0u;
+}
+
 // out[i] = in[i] converted between complex precisions (n elements, grid-stride).
 template <typename TI, typename TO>
 __global__ __launch_bounds__(256) void k_convert_cx(size_t n, const cx<TI>* __restrict__ in, cx<TO>* __restrict__ out) {

@@ -146,9 +146,13 @@ __global__ __launch_bounds__(256) void k_hf_energy(int N, int nRv, const int* __
 // Linear mixing (Meanfield/fixedpoint.jl:146-161): x1 <- beta x1 + (1 - beta) x0,
 // then x0 <- x1.  part[2b], part[2b+1] = block b's share of ||x1 - x0||^2 and ||x1||^2
 // (x0 before the copy).
+// ctl (may be null): the SCF control block of a device-driven loop; once ctl[0] (done) is set,
+// the mixing leaves x0 and the partials untouched, so later iterations recompute the same state.
 __global__ __launch_bounds__(256) void k_mix(size_t n, double beta, cx<double>* __restrict__ x1,
-                                             cx<double>* __restrict__ x0, double* __restrict__ part) {
+                                             cx<double>* __restrict__ x0, double* __restrict__ part,
+                                             const double* __restrict__ ctl = nullptr) {
     __shared__ double sred[4];
+    if (ctl && ctl[0] != 0.0) return;
     double dd = 0.0, nn = 0.0;
     const double omb = 1.0 - beta;
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
@@ -182,6 +186,39 @@ __global__ __launch_bounds__(256) void k_fold2(int G, const double* __restrict__
         out[0] = a;
         out[1] = b;
     }
+}
+
+// One step of the device-driven SCF loop (one thread), after the iteration's mean field, pass and
+// mixing: scal[0..3] eps_H, eps_F and their imaginary parts, scal[4] eps_band, scal[5..6] the
+// squared norms |x1 - x0|^2, |x1|^2, scal[8..9] mu and its status.  ctl: [0] done, [1] iterations,
+// [2] residual, [3] error (0 ok, 1 non-finite bands, 2 filling not bracketed, 3 non-finite
+// energy, 4 complex energy, 5 non-finite residual), [4] converged, [5] eps_band, [6] mu.  hist[3 (it - 1) ..] = (E,
+// residual, mu), the rows the host loop would write.  Nothing changes once done is set.
+__global__ void k_scf_step(const double* __restrict__ scal, double* __restrict__ hist, double* __restrict__ ctl,
+                           double tol, int max_iter, double imag_tol, int mu_on_device) {
+    if (threadIdx.x != 0 || ctl[0] != 0.0) return;
+    const int it = (int)ctl[1] + 1;
+    ctl[1] = it;
+    int err = 0;
+    if (mu_on_device && scal[9] == 2.0) err = 1;
+    else if (mu_on_device && scal[9] == 1.0) err = 2;
+    else if (!isfinite(scal[0]) || !isfinite(scal[1])) err = 3;
+    else if (fabs(scal[2]) > imag_tol || fabs(scal[3]) > imag_tol) err = 4;
+    const double E = scal[4] - (scal[0] + scal[1]);
+    const double dn = sqrt(scal[5]), xn = sqrt(scal[6]);
+    const double res = xn > 2.220446049250313e-16 ? dn / xn : dn;
+    if (!err && hist) {
+        hist[3 * (it - 1)] = E;
+        hist[3 * (it - 1) + 1] = res;
+        hist[3 * (it - 1) + 2] = scal[8];
+    }
+    if (!err && !isfinite(res)) err = 5;
+    ctl[2] = res;
+    ctl[3] = err;
+    ctl[4] = (!err && res < tol) ? 1.0 : 0.0;
+    ctl[5] = scal[4];  // eps_band and mu of this iteration (later frozen ones recompute others)
+    ctl[6] = scal[8];
+    if (err || res < tol || it >= max_iter) ctl[0] = 1.0;
 }
 
 // occ[i] = fermidirac(e_i - mu; T) with mu read from the device (the chemical potential the

@@ -46,11 +46,29 @@ lqm_status fail(lqm_status st, const std::string& msg) {
             return fail(LQM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Device-wide allocator calls (hipMalloc / hipFree) and the stream captures of SCF graphs are
+// serialised by one mutex: a capture open on one thread's stream must not see another thread
+// synchronise the device (the batch runs workers that allocate and free while others capture).
+std::mutex g_cap_mu;
+hipError_t dev_malloc(void** p, size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_cap_mu);
+    return hipMalloc(p, bytes);
+}
+hipError_t dev_free(void* p) {
+    if (!p) return hipSuccess;
+    std::lock_guard<std::mutex> lk(g_cap_mu);
+    return hipFree(p);
+}
+template <typename T>
+hipError_t dev_malloc(T** p, size_t bytes) {
+    return dev_malloc(reinterpret_cast<void**>(p), bytes);
+}
+
 enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -148,6 +166,8 @@ struct lqm_context {
     const double* occ = nullptr;  // given occupations of the chunk in flight (device), or null
     int last_prec = -1;      // precision the last pass ran in (-1: none yet)
     double* scratch = nullptr;  // this device's scratch array (device address)
+    int graphs = 1;          // SCF iterations replayed from captured HIP graphs where eligible
+    bool capturing = false;  // a stream capture is open: workspace must not grow
 };
 
 struct lqm_model {
@@ -162,16 +182,18 @@ namespace {
 
 lqm_status ws(lqm_context* c, Slot s, size_t bytes, void** out) {
     if (bytes == 0) bytes = 16;
+    if (c->cap[s] < bytes && c->capturing)
+        return fail(LQM_ERR_DEVICE, "workspace growth requested inside a graph capture");
     if (c->cap[s] < bytes) {
         if (c->buf[s]) {
             hipError_t e = hipStreamSynchronize(c->stream);
             if (e == hipSuccess && c->side) e = hipStreamSynchronize(c->side);
             if (e != hipSuccess) return fail(LQM_ERR_DEVICE, hipGetErrorString(e));
-            (void)hipFree(c->buf[s]);
+            (void)dev_free(c->buf[s]);
             c->buf[s] = nullptr;
             c->cap[s] = 0;
         }
-        hipError_t e = hipMalloc(&c->buf[s], bytes);
+        hipError_t e = dev_malloc(&c->buf[s], bytes);
         if (e != hipSuccess)
             return fail(LQM_ERR_DEVICE, "workspace allocation of " + std::to_string(bytes) +
                                             " bytes failed: " + hipGetErrorString(e));
@@ -187,6 +209,15 @@ lqm_status wsT(lqm_context* c, Slot s, size_t count, T** out) {
     lqm_status st = ws(c, s, count * sizeof(T), &p);
     *out = static_cast<T*>(p);
     return st;
+}
+
+// Clears bytes (a multiple of 4) of device memory with a kernel: passes that an SCF graph
+// captures avoid memset nodes.
+void zero_dev(lqm_context* c, void* p, size_t bytes) {
+    const size_t n = bytes / 4;
+    if (n == 0) return;
+    const unsigned g = (unsigned)std::min<size_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(lqm::k_zero_words, dim3(g), dim3(256), 0, c->stream, n, static_cast<uint32_t*>(p));
 }
 
 struct Plan {
@@ -448,7 +479,7 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
             LQM_HIP(hipEventRecord(c->ev_join, c->side));
             c->tfac_pending = true;
         }
-        LQM_HIP(hipMemsetAsync(klist, 0, sizeof(int), c->stream));
+        zero_dev(c, klist, sizeof(int));
         StageScope sc(c, ST_QL, Kc);
         const dim3 g4((Kc + 3) / 4);
         hipLaunchKernelGGL(lqm::k_tri_bisect<R>, g4, dim3(256), 0, c->stream, N, Kc, dd, ee, evals, perm, tn, c->d_status);
@@ -803,9 +834,10 @@ lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, doub
 
 // occ (device, N x Nk, may be null): occupations of the ascending eigenvalues used in
 // place of fermidirac(e - mu; T) -- rho then is sum_k w_k e^{-2 pi i k.R} U diag(occ_k) U^H.
+// dRr_dev (optional): the keys already on the device (no upload: the SCF's captured passes).
 lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
                    int nRr, const int32_t* Rr, double* rho, double* eband, double* bands_out,
-                   const double* occ = nullptr, const MuReq* mq = nullptr) {
+                   const double* occ = nullptr, const MuReq* mq = nullptr, const int* dRr_dev = nullptr) {
     lqm_context* c = m->ctx;
     if (mq && !use_blocked(c, m->N) && c->precision == LQM_PREC_FP32_REFINED)
         return kFallback;  // the refined eigenvalues come with the occupations in one kernel
@@ -814,13 +846,17 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
     if (nRr < 0 || (nRr > 0 && !Rr)) return fail(LQM_ERR_ARG, "bad rho key list");
     if (!ks || !rho || !eband) return fail(LQM_ERR_ARG, "null ks, rho or eband");
     const size_t NN = (size_t)N * N;
-    LQM_HIP(hipMemsetAsync(rho, 0, sizeof(cx<double>) * NN * std::max(nRr, 0), c->stream));
-    LQM_HIP(hipMemsetAsync(eband, 0, sizeof(double), c->stream));
-    int* dRr = nullptr;
+    zero_dev(c, rho, sizeof(cx<double>) * NN * std::max(nRr, 0));
+    zero_dev(c, eband, sizeof(double));
+    const int* dRr = dRr_dev;
     lqm_status st;
-    if ((st = wsT(c, W_RR, (size_t)std::max(1, nRr * d), &dRr))) return st;
-    if (nRr > 0)
-        LQM_HIP(hipMemcpyAsync(dRr, Rr, sizeof(int) * nRr * d, hipMemcpyHostToDevice, c->stream));
+    if (!dRr) {
+        int* up = nullptr;
+        if ((st = wsT(c, W_RR, (size_t)std::max(1, nRr * d), &up))) return st;
+        if (nRr > 0)
+            LQM_HIP(hipMemcpyAsync(up, Rr, sizeof(int) * nRr * d, hipMemcpyHostToDevice, c->stream));
+        dRr = up;
+    }
     if (use_blocked(c, N)) {
         if (c->precision == LQM_PREC_FP32_REFINED)
             return run_blocked_mixed(m, Nk, ks, kw, mu, T, nRr, dRr, reinterpret_cast<cx<double>*>(rho), eband,
@@ -1874,10 +1910,10 @@ lqm_status model_init(lqm_context* ctx, int N, int d, int nR, const int32_t* R, 
     m->d = d;
     m->nR = nR;
     m->R.assign(R, R + (size_t)d * nR);
-    hipError_t e = hipMalloc(&m->dR, sizeof(int) * d * nR);
-    if (e == hipSuccess) e = hipMalloc(&m->dT, sizeof(cx<double>) * (size_t)N * N * nR);
+    hipError_t e = dev_malloc(&m->dR, sizeof(int) * d * nR);
+    if (e == hipSuccess) e = dev_malloc(&m->dT, sizeof(cx<double>) * (size_t)N * N * nR);
     if (e != hipSuccess) {
-        (void)hipFree(m->dR);
+        (void)dev_free(m->dR);
         delete m;
         return fail(LQM_ERR_DEVICE, std::string("model allocation: ") + hipGetErrorString(e));
     }
@@ -1904,7 +1940,7 @@ struct HFState {
     ~HFState() {
         if (c) (void)hipStreamSynchronize(c->stream);
         if (hmf) lqm_model_destroy(hmf);
-        if (mem) (void)hipFree(mem);
+        if (mem) (void)dev_free(mem);
     }
 };
 
@@ -1974,7 +2010,7 @@ lqm_status hf_setup(HFState& S, lqm_context* c, int N, int d, int nRh, const int
     const size_t bytes = ncx * sizeof(cx<double>) + ((nint * sizeof(int) + 255) / 256) * 256 + ndbl * sizeof(double);
     LQM_HIP(hipSetDevice(c->device));
     {
-        hipError_t e = hipMalloc(&S.mem, bytes);
+        hipError_t e = dev_malloc(&S.mem, bytes);
         if (e != hipSuccess) {
             S.mem = nullptr;
             return fail(LQM_ERR_DEVICE, std::string("Hartree-Fock state allocation: ") + hipGetErrorString(e));
@@ -2144,8 +2180,9 @@ lqm_status lqm_context_create(int device, int precision, lqm_context** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipMalloc(&c->d_status, sizeof(int));
-    if (e == hipSuccess) e = hipMemset(c->d_status, 0, sizeof(int));
+    if (e == hipSuccess) e = dev_malloc(&c->d_status, sizeof(int));
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_status, 0, sizeof(int), c->own);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->own);
     if (e != hipSuccess) {
         if (c->own) (void)hipStreamDestroy(c->own);
         if (c->side) (void)hipStreamDestroy(c->side);
@@ -2170,8 +2207,8 @@ lqm_status lqm_context_destroy(lqm_context* c) {
     (void)hipStreamSynchronize(c->stream);
     (void)hipStreamSynchronize(c->side);
     for (int s = 0; s < W_NSLOTS; ++s)
-        if (c->buf[s]) (void)hipFree(c->buf[s]);
-    (void)hipFree(c->d_status);
+        if (c->buf[s]) (void)dev_free(c->buf[s]);
+    (void)dev_free(c->d_status);
     (void)hipStreamDestroy(c->own);
     (void)hipStreamDestroy(c->side);
     (void)hipEventDestroy(c->ev_fork);
@@ -2183,6 +2220,12 @@ lqm_status lqm_context_destroy(lqm_context* c) {
 lqm_status lqm_context_set_stream(lqm_context* c, void* s) {
     if (!c) return fail(LQM_ERR_ARG, "null context");
     c->stream = s ? static_cast<hipStream_t>(s) : c->own;
+    return LQM_OK;
+}
+
+lqm_status lqm_context_set_graphs(lqm_context* c, int enable) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    c->graphs = enable ? 1 : 0;
     return LQM_OK;
 }
 
@@ -2303,8 +2346,8 @@ lqm_status lqm_model_create_dev(lqm_context* ctx, int N, int d, int nR, const in
 lqm_status lqm_model_destroy(lqm_model* m) {
     if (!m) return LQM_OK;
     (void)hipStreamSynchronize(m->ctx->stream);
-    (void)hipFree(m->dR);
-    (void)hipFree(m->dT);
+    (void)dev_free(m->dR);
+    (void)dev_free(m->dT);
     delete m;
     return LQM_OK;
 }
@@ -2566,7 +2609,97 @@ lqm_status lqm_hartreefock_scf(lqm_context* ctx, int N, int d, int nRh, const in
     int it = 0;
     bool conv = false;
     int hstat = 0;
-    while (it < max_iter) {
+    // Graph-driven loop: the per-k FP64 kernels at T > 0 with the grid in one chunk keep mu on
+    // the device, so an iteration needs no host round trip.  A device step kernel does the
+    // host loop's bookkeeping (energy, residual, history row, stop test); iterations are
+    // captured once in a HIP graph of kScfGraphIters iterations and replayed until the control
+    // block says done (iterations after that recompute the frozen state and change nothing).
+    const bool graph_mode = c->graphs && !c->timing && T > 0.0 && c->precision == LQM_PREC_FP64 &&
+                            !use_blocked(c, N) && make_plan(c, N, S.nK, Nk, true, true).Kc >= Nk;
+    if (graph_mode) {
+        constexpr int kScfGraphIters = 8;
+        double* ctl;
+        const size_t nctl = 8, nh = (size_t)3 * max_iter;
+        if ((st = wsT(c, W_SCF, nctl + nh + ((size_t)nRr * d + 1) / 2 + 1, &ctl))) return st;
+        double* dhist = ctl + nctl;
+        int* dRr = reinterpret_cast<int*>(dhist + nh);
+        zero_dev(c, ctl, sizeof(double) * nctl);
+        LQM_HIP(hipMemcpyAsync(dRr, Rr, sizeof(int) * nRr * d, hipMemcpyHostToDevice, c->stream));
+        const double imag_tol = std::sqrt(2.220446049250313e-16) * std::max(1, N);
+        auto iteration = [&]() -> lqm_status {
+            lqm_status s2;
+            if ((s2 = hf_apply(S, S.rho0))) return s2;
+            const MuReq mq{filling, &mu, S.scal + 8};
+            s2 = run_rho(S.hmf, Nk, ksd, kwd, 0.0, T, nRr, Rr, reinterpret_cast<double*>(S.rho1), S.scal + 4,
+                         S.bands, nullptr, &mq, dRr);
+            if (s2) return s2;
+            if (!mq.on_device) return fail(LQM_ERR_DEVICE, "SCF graph: the pass did not keep mu on the device");
+            hipLaunchKernelGGL(lqm::k_mix, dim3(kMixBlocks), dim3(256), 0, c->stream, nrho, beta, S.rho1, S.rho0,
+                               S.part, ctl);
+            hipLaunchKernelGGL(lqm::k_fold2, dim3(1), dim3(256), 0, c->stream, kMixBlocks, S.part, S.scal + 5);
+            hipLaunchKernelGGL(lqm::k_scf_step, dim3(1), dim3(64), 0, c->stream, S.scal, history ? dhist : nullptr,
+                               ctl, tol, max_iter, imag_tol, 1);
+            LQM_HIP(hipGetLastError());
+            return LQM_OK;
+        };
+        double hc[8] = {};
+        auto poll = [&]() -> lqm_status {
+            LQM_HIP(hipMemcpyAsync(hc, ctl, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
+            LQM_HIP(hipMemcpyAsync(&hstat, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            LQM_HIP(hipStreamSynchronize(c->stream));
+            if (hstat) return check_status(c, "densitymatrix");
+            return LQM_OK;
+        };
+        // the first iteration eagerly (it also sizes every workspace the captured ones use)
+        if ((st = iteration()) || (st = poll())) return st;
+        if (hc[0] == 0.0) {
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            lqm_status cst = LQM_OK;
+            hipError_t ee;
+            {
+                std::lock_guard<std::mutex> lk(g_cap_mu);
+                LQM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+                c->capturing = true;
+                for (int g = 0; g < kScfGraphIters && !cst; ++g) cst = iteration();
+                c->capturing = false;
+                ee = hipStreamEndCapture(c->stream, &graph);
+            }
+            if (cst) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return cst;
+            }
+            if (ee != hipSuccess) return fail(LQM_ERR_DEVICE, std::string("SCF graph capture: ") + hipGetErrorString(ee));
+            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                return fail(LQM_ERR_DEVICE, std::string("SCF graph instantiation: ") + hipGetErrorString(ie));
+            }
+            while (hc[0] == 0.0 && !st) {
+                const hipError_t le = hipGraphLaunch(exec, c->stream);
+                if (le != hipSuccess) st = fail(LQM_ERR_DEVICE, std::string("SCF graph launch: ") + hipGetErrorString(le));
+                if (!st) st = poll();
+            }
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            if (st) return st;
+        }
+        it = (int)hc[1];
+        residual = hc[2];
+        conv = hc[4] != 0.0;
+        switch ((int)hc[3]) {
+            case 1: return fail(LQM_ERR_NONFINITE, "non-finite bands");
+            case 2: return fail(LQM_ERR_ARG, "filling not bracketed by the band range");
+            case 3: return fail(LQM_ERR_NONFINITE, "non-finite Hartree/Fock energy");
+            case 4: return fail(LQM_ERR_ARG, "complex-valued Hartree/Fock energy: rho is not Hermitian");
+            case 5: return fail(LQM_ERR_NONFINITE, "non-finite SCF residual");
+            default: break;
+        }
+        eband = hc[5];
+        mu = hc[6];
+        if (history && it > 0 && (st = d2h(c, history, dhist, sizeof(double) * 3 * it))) return st;
+    }
+    while (!graph_mode && it < max_iter) {
         ++it;
         // f!(x1, x0): H_MF[rho0]; mu; rho1 and eps_band  (selfconsistent.jl:281-295, :374-381)
         if ((st = hf_apply(S, S.rho0))) return st;

@@ -257,3 +257,55 @@ def test_scf_large_n_path_vs_oracle(pkg, gpu, mfm, hf):
     assert abs(E - ref["E"]) < 1e-8
     for R in ref["rho"]:
         assert np.max(np.abs(rho[R] - ref["rho"][R])) < 1e-8
+
+
+@pytest.mark.parametrize("case", ["honeycomb_60x60", "spinful_random_N24"])
+def test_scf_graph_replay_equals_host_loop(pkg, gpu, mfm, case):
+    # the graph-driven loop (first iteration eager, then replays of an 8-iteration graph, with
+    # host synchronisations between replays) against the host-driven loop: bitwise equal
+    # density matrix, energies, chemical potential, residual and history, and the same count
+    if case == "honeycomb_60x60":
+        h, v, sub = pkg.models.honeycomb_hubbard(3.0)
+        rho0 = mfm.staggered_guess(v, sub, m=0.1, noise=1e-3, seed=1)
+        ks = pkg.regulargrid(nk=60 ** 2)
+        kw = dict(T=0.01, beta=0.5, tol=1e-8, iterations=500)
+    else:
+        h, v = _spinful_random(pkg, 12, 77, U=2.2)
+        rho0 = mfm.staggered_guess(v, np.arange(12) % 2, m=0.2, noise=1e-3, seed=4)
+        ks = pkg.regulargrid(nk=12 ** 2)
+        kw = dict(T=0.03, beta=0.4, tol=1e-10, iterations=300)
+    out = {}
+    try:
+        for graphs in (True, False, True):
+            gpu.set_graphs(graphs)
+            out.setdefault(graphs, []).append(mfm.solvehartreefock(h, v, rho0, 0.5, ks, executor=gpu, **kw))
+    finally:
+        gpu.set_graphs(True)
+    ref = out[False][0]
+    assert ref[3] and ref[2].history.shape[0] > 16  # several graph replays
+    for res in out[True]:
+        assert res[1] == ref[1] and res[3] == ref[3] and res[4] == ref[4]
+        assert res[2].mu == ref[2].mu and res[2].epsband == ref[2].epsband
+        assert np.array_equal(res[2].history, ref[2].history)
+        for R in ref[0].keys():
+            assert np.array_equal(res[0][R], ref[0][R])
+
+
+def test_scf_graph_iteration_limit(pkg, gpu, mfm):
+    # a limit that falls inside a graph replay stops at exactly that many iterations
+    h, v, sub = pkg.models.honeycomb_hubbard(3.0)
+    rho0 = mfm.staggered_guess(v, sub, m=0.1, noise=1e-3, seed=1)
+    ks = pkg.regulargrid(nk=24 ** 2)
+    res = {}
+    try:
+        for graphs in (True, False):
+            gpu.set_graphs(graphs)
+            with pytest.warns(UserWarning):
+                res[graphs] = mfm.solvehartreefock(h, v, rho0, 0.5, ks, T=0.01, beta=0.5, tol=1e-14, iterations=13,
+                                                   executor=gpu)
+    finally:
+        gpu.set_graphs(True)
+    assert res[True][2].history.shape == (13, 3) and not res[True][3]
+    assert np.array_equal(res[True][2].history, res[False][2].history)
+    for R in res[False][0].keys():
+        assert np.array_equal(res[True][0][R], res[False][0][R])
