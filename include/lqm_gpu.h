@@ -285,6 +285,25 @@ lqm_status lqm_bdg_chemicalpotential(lqm_model *m, int Nk, const double *ks, con
                                      double target, double T, double lo, double hi, double *mu,
                                      int *passes);
 
+/* ---- Berry phases, Chern numbers, Wilson loops ---------------------------------------- */
+/* Link variables of the band topology on eigenvector grids (Spectrum/berry.jl): U holds npts
+ * states, each an N x M block of M eigenvector columns (column-major, block p at
+ * U + 2*p*N*M doubles); pairs [2 x nlink] names the states (a_l, b_l) of each link.
+ * lqm_link_overlaps: O_l = U_a^H U_b [M x M x nlink] (the overlaps of Wilson1D, berry.jl:
+ * 370-383), on the matrix cores.  lqm_link_phases: the unit phase det(O_l) / |det(O_l)|
+ * [2 x nlink doubles] (L(State1, State2), berry.jl:116-119; (0, 0) for a singular overlap) by
+ * LU with partial pivoting on the device.  The plaquette sums of berry / getcherns /
+ * gapcherns are formed from these phases by the host wrappers.  *_dev: U, O, phase are
+ * device pointers, pairs stays on the host. */
+lqm_status lqm_link_overlaps(lqm_context *ctx, int N, int M, int npts, const double *U, int nlink,
+                             const int32_t *pairs, double *O);
+lqm_status lqm_link_phases(lqm_context *ctx, int N, int M, int npts, const double *U, int nlink,
+                           const int32_t *pairs, double *phase);
+lqm_status lqm_link_overlaps_dev(lqm_context *ctx, int N, int M, const double *U_dev, int nlink,
+                                 const int32_t *pairs, double *O_dev);
+lqm_status lqm_link_phases_dev(lqm_context *ctx, int N, int M, const double *U_dev, int nlink,
+                               const int32_t *pairs, double *phase_dev);
+
 /* ---- Hartree-Fock mean field and self-consistency ----------------------------------- */
 /* Flags of the HF functional: which channels HartreeFock(h, v; hartree, fock) keeps
  * (Meanfield/hartreefock.jl:158). */

@@ -69,6 +69,10 @@ _PROTOS = {
     "lqm_densitymatrix_occ_dev": [P, I, P, P, P, I, P, P, P],
     "lqm_hartreefock_scf_batch": [P, I, I, P, D, D, D, I, I, I],
     "lqm_bdg_densitymatrix": [P, I, P, P, D, D, I, P, P, P],
+    "lqm_link_overlaps": [P, I, I, I, P, I, P, P],
+    "lqm_link_phases": [P, I, I, I, P, I, P, P],
+    "lqm_link_overlaps_dev": [P, I, I, P, I, P, P],
+    "lqm_link_phases_dev": [P, I, I, P, I, P, P],
     "lqm_hartreefock_grandpotential": [P, I, I, I, P, P, I, P, P, I, P, P, I, I, P, P, D, D, P, P, P],
     "lqm_bdg_filling": [P, I, P, P, D, D, P],
     "lqm_bdg_chemicalpotential": [P, I, P, P, D, D, D, D, P, P],

@@ -383,6 +383,82 @@ __global__ __launch_bounds__(256) void k_convert_cx(size_t n, const cx<TI>* __re
         out[i] = cx<TO>{(TO)in[i].re, (TO)in[i].im};
 }
 
+// Determinant phase det(O) / |det(O)| of each M x M complex matrix O[l] (column-major, stride
+// M^2), by LU with partial pivoting in place (O is scratch): the link variable of a Berry
+// plaquette (Spectrum/berry.jl:116-119).  One 256-thread block per matrix; out[2l..2l+1] = the
+// unit phase, or (0, 0) for a singular matrix.
+__global__ __launch_bounds__(256) void k_det_phase(int M, cx<double>* __restrict__ O, double* __restrict__ out) {
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    __shared__ cx<double> s_ph;
+    __shared__ int s_piv, s_sing;
+    const int tid = threadIdx.x;
+    cx<double>* A = O + (size_t)blockIdx.x * M * M;
+    if (tid == 0) {
+        s_ph = {1.0, 0.0};
+        s_sing = 0;
+    }
+    __syncthreads();
+    for (int j = 0; j < M; ++j) {
+        // pivot: the largest |a_rj|, r >= j (smallest r among ties)
+        double best = -1.0;
+        int bi = j;
+        for (int r = j + tid; r < M; r += 256) {
+            const double a = abs2(A[(size_t)j * M + r]);
+            if (a > best) {
+                best = a;
+                bi = r;
+            }
+        }
+        sv[tid] = best;
+        si[tid] = bi;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o && (sv[tid + o] > sv[tid] || (sv[tid + o] == sv[tid] && si[tid + o] < si[tid]))) {
+                sv[tid] = sv[tid + o];
+                si[tid] = si[tid + o];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            s_piv = si[0];
+            if (!(sv[0] > 0.0)) s_sing = 1;
+        }
+        __syncthreads();
+        if (s_sing) break;
+        const int p = s_piv;
+        if (p != j)
+            for (int c = tid; c < M; c += 256) {
+                const cx<double> t = A[(size_t)c * M + j];
+                A[(size_t)c * M + j] = A[(size_t)c * M + p];
+                A[(size_t)c * M + p] = t;
+            }
+        __syncthreads();
+        const cx<double> d = A[(size_t)j * M + j];
+        if (tid == 0) {
+            const double n = sqrt(abs2(d));
+            cx<double> u = {d.re / n, d.im / n};
+            if (p != j) u = {-u.re, -u.im};
+            const cx<double> q = s_ph * u;
+            const double qn = sqrt(abs2(q));
+            s_ph = {q.re / qn, q.im / qn};
+        }
+        const cx<double> dinv = crecip(d);
+        for (int r = j + 1 + tid; r < M; r += 256) A[(size_t)j * M + r] = A[(size_t)j * M + r] * dinv;
+        __syncthreads();
+        const int n = M - j - 1;
+        for (int e = tid; e < n * n; e += 256) {
+            const int r = j + 1 + e % n, c = j + 1 + e / n;
+            A[(size_t)c * M + r] = A[(size_t)c * M + r] - A[(size_t)j * M + r] * A[(size_t)c * M + j];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        out[2 * blockIdx.x] = s_sing ? 0.0 : s_ph.re;
+        out[2 * blockIdx.x + 1] = s_sing ? 0.0 : s_ph.im;
+    }
+}
+
 // Nambu (BdG) chemical potential shift (Superconductivity/operators.jl:14-17 applied with -mu,
 // as getdensitymatrix!(::BdGOperator) does, green.jl:38-54): the zero-key block of the 2d x 2d
 // Nambu model becomes base - mu tau_z, i.e. -mu on the electron diagonal (i < d) and +mu on

@@ -396,7 +396,8 @@ struct GemmArgs {
     cx<R>* C;
     cx<double> alpha, beta;
     const double* ascale;
-    const int* kidx;  // batch entry of grid z (null: z itself)
+    const int* kidx;   // batch entry of grid z (null: z itself)
+    const int* pairs;  // (a, b) per grid z: A from entry a, B from entry b, C at z (null: none)
 };
 using ZGemmArgs = GemmArgs<double>;
 
@@ -507,6 +508,8 @@ __global__ __launch_bounds__(256) void k_zgemm_gen(GemmArgs<R> g) {
     const int m0 = ti * 64 + (wave & 1) * 32;
     const int n0 = tj * 64 + (wave >> 1) * 32;
     const size_t b = g.kidx ? (size_t)g.kidx[blockIdx.z] : (size_t)blockIdx.z;
+    const size_t bA = g.pairs ? (size_t)g.pairs[2 * blockIdx.z] : b;
+    const size_t bB = g.pairs ? (size_t)g.pairs[2 * blockIdx.z + 1] : b;
     typename MM::acc cr[2][2], ci[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -517,7 +520,7 @@ __global__ __launch_bounds__(256) void k_zgemm_gen(GemmArgs<R> g) {
         }
     const double* sc = g.ascale ? g.ascale + b * g.sS : nullptr;
     for (int p = 0; p < g.npair; ++p)
-        gemm_acc<OPA, OPB, R>(g.A[p] + b * g.sA, g.lda, g.B[p] + b * g.sB, g.ldb, sc, g.M, g.N, g.K, m0, n0, li,
+        gemm_acc<OPA, OPB, R>(g.A[p] + bA * g.sA, g.lda, g.B[p] + bB * g.sB, g.ldb, sc, g.M, g.N, g.K, m0, n0, li,
                               g4, cr, ci);
     cx<R>* Cb = g.C + b * g.sC;
     const bool zero_beta = g.beta.re == 0.0 && g.beta.im == 0.0;

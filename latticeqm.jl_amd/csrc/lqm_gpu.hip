@@ -68,7 +68,7 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_LNK, W_LNKO, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -1248,6 +1248,7 @@ lqm::GemmArgs<R> gemm_args(int M, int N, int K, const cx<R>* A, int lda, long lo
     g.beta = beta;
     g.ascale = nullptr;
     g.kidx = nullptr;
+    g.pairs = nullptr;
     return g;
 }
 
@@ -3078,6 +3079,77 @@ lqm_status lqm_bdg_densitymatrix(lqm_model* m, int Nk, const double* ks, const d
     if (nRr > 0 && (st = d2h(c, rho, drho, sizeof(cx<double>) * NN * nRr))) return st;
     if (ekin) *ekin = eb / 2.0 + (trh - d * mu) / 2.0;
     return LQM_OK;
+}
+
+lqm_status lqm_link_overlaps_dev(lqm_context* c, int N, int M, const double* U_dev, int nlink,
+                                 const int32_t* pairs, double* O_dev) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    if (N <= 0 || M <= 0 || M > N || nlink < 0) return fail(LQM_ERR_ARG, "bad sizes");
+    if (nlink == 0) return LQM_OK;
+    if (nlink > 65535) return fail(LQM_ERR_UNSUPPORTED, "at most 65535 links per call");
+    if (!U_dev || !pairs || !O_dev) return fail(LQM_ERR_ARG, "null buffer");
+    for (int l = 0; l < 2 * nlink; ++l)
+        if (pairs[l] < 0) return fail(LQM_ERR_ARG, "negative state index in the link list");
+    int* dp;
+    lqm_status st;
+    if ((st = wsT(c, W_LNK, (size_t)2 * nlink, &dp))) return st;
+    LQM_HIP(hipMemcpyAsync(dp, pairs, sizeof(int) * 2 * nlink, hipMemcpyHostToDevice, c->stream));
+    const cx<double>* U = reinterpret_cast<const cx<double>*>(U_dev);
+    lqm::ZGemmArgs g = gemm_args(M, M, N, U, N, (long long)N * M, U, N, (long long)N * M,
+                                 reinterpret_cast<cx<double>*>(O_dev), M, (long long)M * M, {1.0, 0.0}, {0.0, 0.0});
+    g.pairs = dp;
+    zgemm_gen<1, 0>(c, ST_REDUCE, nlink, nlink, g);
+    LQM_HIP(hipGetLastError());
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    return LQM_OK;
+}
+
+lqm_status lqm_link_phases_dev(lqm_context* c, int N, int M, const double* U_dev, int nlink, const int32_t* pairs,
+                               double* phase_dev) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    if (nlink == 0) return LQM_OK;
+    if (!phase_dev) return fail(LQM_ERR_ARG, "null buffer");
+    if (M <= 0 || nlink < 0) return fail(LQM_ERR_ARG, "bad sizes");
+    cx<double>* O;
+    lqm_status st;
+    if ((st = wsT(c, W_LNKO, (size_t)M * M * std::max(nlink, 1), &O))) return st;
+    if ((st = lqm_link_overlaps_dev(c, N, M, U_dev, nlink, pairs, reinterpret_cast<double*>(O)))) return st;
+    hipLaunchKernelGGL(lqm::k_det_phase, dim3(nlink), dim3(256), 0, c->stream, M, O, phase_dev);
+    LQM_HIP(hipGetLastError());
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    return LQM_OK;
+}
+
+lqm_status lqm_link_phases(lqm_context* c, int N, int M, int npts, const double* U, int nlink,
+                           const int32_t* pairs, double* phase) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    if (N <= 0 || M <= 0 || M > N || npts <= 0 || nlink < 0) return fail(LQM_ERR_ARG, "bad sizes");
+    if (nlink == 0) return LQM_OK;
+    if (!U || !pairs || !phase) return fail(LQM_ERR_ARG, "null buffer");
+    for (int l = 0; l < 2 * nlink; ++l)
+        if (pairs[l] < 0 || pairs[l] >= npts) return fail(LQM_ERR_ARG, "state index out of range in the link list");
+    void *dU, *dph;
+    lqm_status st;
+    if ((st = stage_in(c, W_KS, U, sizeof(cx<double>) * (size_t)N * M * npts, &dU))) return st;
+    if ((st = ws(c, W_OUT, sizeof(double) * 2 * nlink, &dph))) return st;
+    if ((st = lqm_link_phases_dev(c, N, M, (const double*)dU, nlink, pairs, (double*)dph))) return st;
+    return d2h(c, phase, dph, sizeof(double) * 2 * nlink);
+}
+
+lqm_status lqm_link_overlaps(lqm_context* c, int N, int M, int npts, const double* U, int nlink,
+                             const int32_t* pairs, double* O) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    if (N <= 0 || M <= 0 || M > N || npts <= 0 || nlink < 0) return fail(LQM_ERR_ARG, "bad sizes");
+    if (nlink == 0) return LQM_OK;
+    if (!U || !pairs || !O) return fail(LQM_ERR_ARG, "null buffer");
+    for (int l = 0; l < 2 * nlink; ++l)
+        if (pairs[l] < 0 || pairs[l] >= npts) return fail(LQM_ERR_ARG, "state index out of range in the link list");
+    void *dU, *dO;
+    lqm_status st;
+    if ((st = stage_in(c, W_KS, U, sizeof(cx<double>) * (size_t)N * M * npts, &dU))) return st;
+    if ((st = ws(c, W_OUT, sizeof(cx<double>) * (size_t)M * M * nlink, &dO))) return st;
+    if ((st = lqm_link_overlaps_dev(c, N, M, (const double*)dU, nlink, pairs, (double*)dO))) return st;
+    return d2h(c, O, dO, sizeof(cx<double>) * (size_t)M * M * nlink);
 }
 
 lqm_status lqm_hartreefock_scf_batch(lqm_context* ctx, int d, int M, lqm_hf_problem* problems,

@@ -286,3 +286,59 @@ def gethubbard(lat: Lattice, neighbors=((0, 0),), mode: str = "nospin", a: float
     v = gethops(lat, t, neighbors=[tuple(L) for L in neighbors])
     spin = {"σx": "sigmax", "σ0": "spinhalf"}.get(mode, mode)
     return addspin(v, spin)
+
+
+def _nnbondlength(lat: Lattice) -> float:
+    """Smallest nonzero interatomic distance over the origin and first-shell cells."""
+    cells = getneighborcells(lat.A, 1, halfspace=False, innerpoints=True, excludeorigin=False)
+    R = lat.positions
+    dmin = np.inf
+    for c in cells:
+        sh = lat.A @ np.asarray(c, dtype=float)
+        d = np.linalg.norm((R + sh[:, None])[:, :, None] - R[:, None, :], axis=0)
+        d = d[d > DEFAULT_PRECISION]
+        if d.size:
+            dmin = min(dmin, float(d.min()))
+    return dmin
+
+
+def haldane(lat: Lattice, t2: float = 0.1, phi: float = np.pi / 2, a0: float | None = None,
+            cellrange: int = 1) -> Hops:
+    """The Haldane next-nearest-neighbour term (Operators/haldane.jl:47-86): for every pair of
+    orbitals at distance sqrt(3) a0 (a0 the nearest-neighbour bond), through their common
+    nearest neighbour R0, block (i, j) of key L gains t2 exp(i phi sign(cross(-y, x))) with
+    x = r_i + A L - R0, y = r_j - R0 (cross(a, b) = a_1 b_2 - a_2 b_1: the chirality)."""
+    a0 = _nnbondlength(lat) if a0 is None else a0
+    cells = getneighborcells(lat.A, cellrange, halfspace=False, innerpoints=True, excludeorigin=False)
+    near = getneighborcells(lat.A, 1, halfspace=False, innerpoints=True, excludeorigin=False)
+    R = lat.positions
+    R0s = np.concatenate([R + (lat.A @ np.asarray(c, dtype=float))[:, None] for c in near], axis=1)
+    h = Hops()
+    d3 = np.sqrt(3.0) * a0
+    for L in cells:
+        sh = lat.A @ np.asarray(L, dtype=float)
+        M = np.zeros((lat.n, lat.n), dtype=np.complex128)
+        for i in range(lat.n):
+            Ri = R[:, i] + sh
+            for j in range(lat.n):
+                Rj = R[:, j]
+                if not np.isclose(np.linalg.norm(Ri - Rj), d3, rtol=1e-6):
+                    continue
+                for q in range(R0s.shape[1]):
+                    x = Ri[:2] - R0s[:2, q]
+                    y = Rj[:2] - R0s[:2, q]
+                    if np.isclose(np.linalg.norm(x), a0, rtol=1e-6) and np.isclose(np.linalg.norm(y), a0, rtol=1e-6):
+                        cr = (-y[0]) * x[1] - (-y[1]) * x[0]
+                        M[i, j] += t2 * np.exp(1j * phi * np.sign(cr))
+                        break
+        if np.any(M != 0):
+            h[tuple(int(v) for v in L)] = M
+    return h
+
+
+def addhops(h: Hops, extra: Hops) -> Hops:
+    """h + extra key by key (addhops!, TightBinding/types.jl)."""
+    out = h.copy()
+    for R, t in extra.items():
+        out[R] = out[R] + t if R in out else t
+    return out
