@@ -75,9 +75,9 @@ lqm_status lqm_context_synchronize(lqm_context *ctx);
  * 16 mix, 17 mu_bisect, 18 blocked_hk, 19 blocked_panel, 20 blocked_update,
  * 21 blocked_bisect, 22 blocked_invit, 23 blocked_backtransform, 24 blocked_rho,
  * 25 oa_correct_later (14 = the first refinement sweep, on every k-point; 25 = later sweeps
- * and cluster corrections, on listed k-points) (LQM_NSTAGES = 26); lqm_stage_name gives the
- * names. */
-#define LQM_NSTAGES 26
+ * and cluster corrections, on listed k-points), 26 response_sums (the ldos / density site
+ * weights and the Kubo pair sums) (LQM_NSTAGES = 27); lqm_stage_name gives the names. */
+#define LQM_NSTAGES 27
 lqm_status lqm_context_set_timing(lqm_context *ctx, int enable);
 /* SCF iterations from captured HIP graphs (no reference counterpart; default on).  Where an
  * iteration needs no host round trip (per-k FP64 kernels, T > 0, the grid in one chunk),
@@ -177,6 +177,36 @@ lqm_status lqm_bandmatrix_expval(lqm_model *m, int Nk, const double *ks, int nP,
 lqm_status lqm_bandmatrix_expval_dev(lqm_model *m, int Nk, const double *ks_dev, int nP,
                                      lqm_model *const *projectors, double *bands_dev,
                                      double *obs_dev);
+
+/* ---- eigenvector consumers ------------------------------------------------------- */
+/* ldos(H, ks, frequencies; G) (Spectrum/dos.jl:57-105): site-resolved local DOS
+ *   ldos[i] = 1/(Nk pi) sum_k sum_n |psi_nk[i]|^2 (1/Nw) sum_o G / ((w_o - e_nk)^2 + G^2)
+ * (with kweights: sum_k w_k ... / pi).  One frequency is the reference's result exactly; over a
+ * window the reference divides its running per-task accumulator by Nw once per k-point (an
+ * executor-dependent result), here the window is averaged once per state. */
+lqm_status lqm_ldos(lqm_model *m, int Nk, const double *ks, const double *kweights, int Nw,
+                    const double *omegas, double gamma, double *ldos);
+lqm_status lqm_ldos_dev(lqm_model *m, int Nk, const double *ks_dev, const double *kweights_dev,
+                        int Nw, const double *omegas_dev, double gamma, double *ldos_dev);
+/* density(H, ks, mu) (Spectrum/dos.jl:20-52): n[i] = 1/Nk sum_k sum_{e_nk <= mu} |psi_nk[i]|^2
+ * (with kweights: sum_k w_k ...). */
+lqm_status lqm_density(lqm_model *m, int Nk, const double *ks, const double *kweights, double mu,
+                       double *n);
+lqm_status lqm_density_dev(lqm_model *m, int Nk, const double *ks_dev, const double *kweights_dev,
+                           double mu, double *n_dev);
+/* opticalconductivity(frequencies, H, J1, J2, ks; mu, G, T) (LinearResponse/
+ * opticalconductivity.jl:101-127 with kubo!, kubo.jl:11-31): the Kubo current-current
+ * correlator, complex chi [Nw] (interleaved re, im),
+ *   chi(w) = -(i/Nk) sum_k sum_{m != n} (f_m - f_n) <m|J1(k)|n> <n|J2(k)^H|m> / (e_m - e_n - w - iG)
+ * with f = fermidirac(e; mu, T) (T = 0: the step with 1/2 at e = mu); with kweights the sum
+ * over k carries w_k instead of 1/Nk.  j1, j2: model handles (Hops of the currents) on the
+ * context of h; j1 == j2 computes U^H J U once. */
+lqm_status lqm_kubo(lqm_model *h, lqm_model *j1, lqm_model *j2, int Nk, const double *ks,
+                    const double *kweights, int Nw, const double *omegas, double mu, double gamma,
+                    double T, double *chi);
+lqm_status lqm_kubo_dev(lqm_model *h, lqm_model *j1, lqm_model *j2, int Nk, const double *ks_dev,
+                        const double *kweights_dev, int Nw, const double *omegas_dev, double mu,
+                        double gamma, double T, double *chi_dev);
 
 /* ---- DOS -------------------------------------------------------------------------- */
 /* dos_compute! (Spectrum/dos.jl:202-232): DOS(w) = sum_k wk sum_n Im[1/(w - i G - e_nk)],

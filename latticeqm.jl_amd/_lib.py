@@ -55,6 +55,12 @@ _PROTOS = {
     "lqm_bandmatrix": [P, I, P, P, P],
     "lqm_bandmatrix_dev": [P, I, P, P, P],
     "lqm_dos": [P, I, P, P, I, P, D, P],
+    "lqm_ldos": [P, I, P, P, I, P, D, P],
+    "lqm_ldos_dev": [P, I, P, P, I, P, D, P],
+    "lqm_density": [P, I, P, P, D, P],
+    "lqm_density_dev": [P, I, P, P, D, P],
+    "lqm_kubo": [P, P, P, I, P, P, I, P, D, D, D, P],
+    "lqm_kubo_dev": [P, P, P, I, P, P, I, P, D, D, D, P],
     "lqm_dos_from_bands_dev": [P, I, I, P, P, I, P, D, P],
     "lqm_dos_from_bands_dev_noalloc": [P, I, I, P, P, I, P, D, P],
     "lqm_chemicalpotential": [P, I, P, P, D, D, P],
@@ -81,7 +87,7 @@ _PROTOS = {
                             P, P, P],
 }
 _RESTYPE = {"lqm_last_error": ctypes.c_char_p, "lqm_abi_version": I, "lqm_stage_name": ctypes.c_char_p}
-NSTAGES = 26
+NSTAGES = 27
 
 
 class HFProblem(ctypes.Structure):

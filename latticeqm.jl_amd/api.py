@@ -288,6 +288,33 @@ def getdos(h, *args, klin: int | None = None, Gamma: float | None = None, kweigh
     return out
 
 
+def ldos(H, ks, frequencies, Gamma: float | None = None, kweights=None, executor=None, **kw) -> np.ndarray:
+    """Site-resolved local DOS (Spectrum/dos.jl:57-105): k-averaged, averaged over the
+    frequency window, with the spectral function's 1/pi, so that at one frequency the sum over
+    sites is getdos / pi.  ``Gamma`` (or ``Γ``) defaults to 0.1 as in the reference."""
+    if Gamma is None:
+        Gamma = kw.pop("Γ", 0.1)
+    om = np.atleast_1d(np.ascontiguousarray(frequencies, dtype=np.float64))
+    if om.size == 0:
+        raise ValueError("at least one frequency")
+    m = _model(H, executor)
+    k = _kpts(ks, m.d)
+    w = _weights(ks, kweights, k.shape[1])
+    out = np.zeros(m.N)
+    _lib.call("lqm_ldos", m.handle, k.shape[1], _vp(k), _vp(w), om.size, _vp(om), float(Gamma), _vp(out))
+    return out
+
+
+def density(H, ks, mu: float = 0.0, kweights=None, executor=None) -> np.ndarray:
+    """Site densities of the states with e <= mu, k-averaged (Spectrum/dos.jl:20-52)."""
+    m = _model(H, executor)
+    k = _kpts(ks, m.d)
+    w = _weights(ks, kweights, k.shape[1])
+    out = np.zeros(m.N)
+    _lib.call("lqm_density", m.handle, k.shape[1], _vp(k), _vp(w), float(mu), _vp(out))
+    return out
+
+
 def chemicalpotential(H, ks, filling: float, T: float = 0.0, executor=None, kweights=None) -> float:
     """mu such that the filling of the bands on ks equals ``filling``."""
     m = _model(H, executor)

@@ -27,6 +27,7 @@
 #include "wy_kernels.hpp"
 #include "blocked_kernels.hpp"
 #include "tri_kernels.hpp"
+#include "response_kernels.hpp"
 
 using lqm::cx;
 
@@ -68,7 +69,7 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_LNK, W_LNKO, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_LNK, W_LNKO, W_RB, W_ROCC, W_RA, W_RC, W_RP, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -114,7 +115,7 @@ lqm_status scratch_address(int device, double** out) {
 enum Stage {
     ST_TRIDIAG, ST_QL, ST_REPLAY, ST_BACKTRANSFORM, ST_ZFZ, ST_RHO2, ST_CONTRACT, ST_FOLD, ST_DOS,
     ST_REDUCE, ST_HK64, ST_GEMM_NN, ST_GEMM_CN, ST_GEMM_NC, ST_OA, ST_HF, ST_MIX, ST_MU, ST_BHK,
-    ST_BTRD, ST_BUPD, ST_BEIG, ST_BVEC, ST_BBT, ST_BRHO, ST_OA2, ST_N
+    ST_BTRD, ST_BUPD, ST_BEIG, ST_BVEC, ST_BBT, ST_BRHO, ST_OA2, ST_RESP, ST_N
 };
 static_assert(ST_N == LQM_NSTAGES, "stage list and LQM_NSTAGES disagree");
 const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "backtransform", "zfz",
@@ -123,7 +124,7 @@ const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "ba
                                        "zgemm_nc_fp64", "oa_correct", "hf_meanfield", "mix",
                                        "mu_bisect", "blocked_hk", "blocked_panel", "blocked_update",
                                        "blocked_bisect", "blocked_invit", "blocked_backtransform",
-                                       "blocked_rho", "oa_correct_later"};
+                                       "blocked_rho", "oa_correct_later", "response_sums"};
 
 struct TimedLaunch {
     int stage;
@@ -2980,6 +2981,187 @@ lqm_status lqm_bandmatrix_expval(lqm_model* m, int Nk, const double* ks, int nP,
     if ((st = d2h(c, bands, db, sizeof(double) * N * Nk))) return st;
     if (nP > 0) return d2h(c, obs, dobs, sizeof(double) * N * Nk * nP);
     return LQM_OK;
+}
+
+// ---- consumers of the eigenvectors: ldos, density, Kubo ----------------------------
+// Each pass walks the grid in chunks: the band pass leaves eigenvalues and eigenvectors of the
+// chunk in HBM, then the consumer kernels read them; nbuf = N x N complex buffers per k-point.
+static int consumer_chunk(int N, int Nk, int nbuf) {
+    const size_t per = (size_t)nbuf * N * N * sizeof(cx<double>) + 16 * (size_t)N;
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)Nk, (size_t(4) << 30) / per));
+}
+
+static lqm_status site_weights_dev(lqm_model* m, int Nk, const double* ks_dev, const double* kw_dev, int kind, int Nw,
+                            const double* om_dev, double gamma, double mu, double* out_dev) {
+    lqm_context* c = m->ctx;
+    const int N = m->N;
+    const size_t NN = (size_t)N * N;
+    const int Kx = consumer_chunk(N, Nk, 1);
+    const int G = std::min(Kx, 1024);
+    cx<double>* V;
+    double *eb, *part;
+    lqm_status st;
+    if ((st = wsT(c, W_PV, NN * Kx, &V))) return st;
+    if ((st = wsT(c, W_RB, (size_t)N * Kx, &eb))) return st;
+    if ((st = wsT(c, W_RP, (size_t)N * G, &part))) return st;
+    const size_t sh = sizeof(double) * N;
+    auto kS = lqm::k_site_weights;
+    allow_lds(kS, sh);
+    for (int k0 = 0; k0 < Nk; k0 += Kx) {
+        const int Kc = std::min(Kx, Nk - k0);
+        if ((st = run_bands(m, Kc, ks_dev + (size_t)k0 * m->d, eb, reinterpret_cast<double*>(V)))) return st;
+        StageScope sc(c, ST_RESP, Kc);
+        hipLaunchKernelGGL(kS, dim3(G), dim3(256), sh, c->stream, N, Kc, eb, V, kw_dev ? kw_dev + k0 : nullptr,
+                           kind, Nw, om_dev, gamma, mu, k0 == 0 ? 1 : 0, part);
+        LQM_HIP(hipGetLastError());
+    }
+    double scale = kw_dev ? 1.0 : 1.0 / (double)Nk;
+    if (kind == 0) scale /= M_PI;
+    {
+        StageScope sc(c, ST_REDUCE);
+        hipLaunchKernelGGL(lqm::k_dos_fold, dim3((N + 255) / 256), dim3(256), 0, c->stream, N, G, part, scale,
+                           out_dev);
+    }
+    LQM_HIP(hipGetLastError());
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    return LQM_OK;
+}
+
+lqm_status lqm_ldos_dev(lqm_model* m, int Nk, const double* ks_dev, const double* kweights_dev, int Nw,
+                        const double* omegas_dev, double gamma, double* ldos_dev) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0 || !ks_dev) return fail(LQM_ERR_ARG, "Nk must be positive");
+    if (Nw <= 0 || !omegas_dev) return fail(LQM_ERR_ARG, "at least one frequency");
+    if (!(gamma > 0.0)) return fail(LQM_ERR_ARG, "the broadening must be positive");
+    if (!ldos_dev) return fail(LQM_ERR_ARG, "null output");
+    return site_weights_dev(m, Nk, ks_dev, kweights_dev, 0, Nw, omegas_dev, gamma, 0.0, ldos_dev);
+}
+
+lqm_status lqm_ldos(lqm_model* m, int Nk, const double* ks, const double* kweights, int Nw, const double* omegas,
+                    double gamma, double* ldos) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0 || !ks || Nw <= 0 || !omegas || !ldos) return fail(LQM_ERR_ARG, "bad sizes or null buffer");
+    lqm_context* c = m->ctx;
+    void *dks, *dkw = nullptr, *dom, *dout;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    if ((st = stage_in(c, W_OM, omegas, sizeof(double) * Nw, &dom))) return st;
+    if ((st = ws(c, W_OUT, sizeof(double) * m->N, &dout))) return st;
+    if ((st = lqm_ldos_dev(m, Nk, (const double*)dks, (const double*)dkw, Nw, (const double*)dom, gamma,
+                           (double*)dout)))
+        return st;
+    return d2h(c, ldos, dout, sizeof(double) * m->N);
+}
+
+lqm_status lqm_density_dev(lqm_model* m, int Nk, const double* ks_dev, const double* kweights_dev, double mu,
+                           double* n_dev) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0 || !ks_dev) return fail(LQM_ERR_ARG, "Nk must be positive");
+    if (!n_dev) return fail(LQM_ERR_ARG, "null output");
+    return site_weights_dev(m, Nk, ks_dev, kweights_dev, 1, 0, nullptr, 0.0, mu, n_dev);
+}
+
+lqm_status lqm_density(lqm_model* m, int Nk, const double* ks, const double* kweights, double mu, double* n) {
+    lqm_status st = check_model(m);
+    if (st) return st;
+    if (Nk <= 0 || !ks || !n) return fail(LQM_ERR_ARG, "bad sizes or null buffer");
+    lqm_context* c = m->ctx;
+    void *dks, *dkw = nullptr, *dout;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    if ((st = ws(c, W_OUT, sizeof(double) * m->N, &dout))) return st;
+    if ((st = lqm_density_dev(m, Nk, (const double*)dks, (const double*)dkw, mu, (double*)dout))) return st;
+    return d2h(c, n, dout, sizeof(double) * m->N);
+}
+
+lqm_status lqm_kubo_dev(lqm_model* h, lqm_model* j1, lqm_model* j2, int Nk, const double* ks_dev,
+                        const double* kweights_dev, int Nw, const double* omegas_dev, double mu, double gamma,
+                        double T, double* chi_dev) {
+    lqm_status st = check_model(h);
+    if (st) return st;
+    if (Nk <= 0 || !ks_dev) return fail(LQM_ERR_ARG, "Nk must be positive");
+    if (Nw <= 0 || !omegas_dev || !chi_dev) return fail(LQM_ERR_ARG, "bad frequencies or null output");
+    if (!(T >= 0.0)) return fail(LQM_ERR_ARG, "negative temperature");
+    for (lqm_model* q : {j1, j2}) {
+        if (!q || q->ctx != h->ctx) return fail(LQM_ERR_ARG, "bad current-operator handle");
+        if (q->N != h->N || q->d != h->d) return fail(LQM_ERR_ARG, "current operator and Hamiltonian differ in N or d");
+    }
+    lqm_context* c = h->ctx;
+    const int N = h->N;
+    const size_t NN = (size_t)N * N;
+    const bool same = j1 == j2;
+    const int Kx = consumer_chunk(N, Nk, same ? 4 : 5);
+    const int G = std::min(Kx, 1024);
+    cx<double> *V, *O, *W, *A, *C, *part;
+    double *eb, *occ;
+    if ((st = wsT(c, W_PV, NN * Kx, &V))) return st;
+    if ((st = wsT(c, W_PO, NN * Kx, &O))) return st;
+    if ((st = wsT(c, W_PW, NN * Kx, &W))) return st;
+    if ((st = wsT(c, W_RA, NN * Kx, &A))) return st;
+    if (same) C = A;
+    else if ((st = wsT(c, W_RC, NN * Kx, &C))) return st;
+    if ((st = wsT(c, W_RB, (size_t)N * Kx, &eb))) return st;
+    if ((st = wsT(c, W_ROCC, (size_t)N * Kx, &occ))) return st;
+    if ((st = wsT(c, W_RP, (size_t)G * Nw, &part))) return st;
+    const cx<double> one = {1.0, 0.0}, zero = {0.0, 0.0};
+    const dim3 gk(G, (unsigned)((Nw + 63) / 64));
+    for (int k0 = 0; k0 < Nk; k0 += Kx) {
+        const int Kc = std::min(Kx, Nk - k0);
+        const double* ksc = ks_dev + (size_t)k0 * h->d;
+        if ((st = run_bands(h, Kc, ksc, eb, reinterpret_cast<double*>(V)))) return st;
+        // U^H J U for each current operator: J(k) assembled like H(k), W = J U on the matrix
+        // cores, then U^H W
+        for (int q = 0; q < (same ? 1 : 2); ++q) {
+            lqm_model* J = q == 0 ? j1 : j2;
+            const size_t sh = sizeof(cx<double>) * J->nR;
+            auto kH = lqm::k_fouriersum;
+            allow_lds(kH, sh);
+            {
+                StageScope sc(c, ST_HK64, Kc);
+                hipLaunchKernelGGL(kH, dim3(Kc), dim3(256), sh, c->stream, N, J->d, J->nR, J->dR, J->dT, ksc, O);
+            }
+            zgemm<0, 0>(c, N, Kc, O, V, W, one, zero);
+            zgemm_gen<1, 0>(c, ST_GEMM_CN, Kc, Kc,
+                            gemm_args(N, N, N, V, N, (long long)NN, W, N, (long long)NN, q == 0 ? A : C, N,
+                                      (long long)NN, one, zero));
+            LQM_HIP(hipGetLastError());
+        }
+        StageScope sc(c, ST_RESP, Kc);
+        const int L = N * Kc;
+        hipLaunchKernelGGL(lqm::k_fd_occ, dim3((L + 255) / 256), dim3(256), 0, c->stream, L, eb, mu, T, occ);
+        hipLaunchKernelGGL(lqm::k_kubo, gk, dim3(256), 0, c->stream, N, Kc, eb, occ, A, C,
+                           kweights_dev ? kweights_dev + k0 : nullptr, Nw, omegas_dev, gamma, k0 == 0 ? 1 : 0,
+                           part);
+        LQM_HIP(hipGetLastError());
+    }
+    {
+        StageScope sc(c, ST_REDUCE);
+        hipLaunchKernelGGL(lqm::k_kubo_fold, dim3((Nw + 255) / 256), dim3(256), 0, c->stream, Nw, G, part,
+                           kweights_dev ? 1.0 : 1.0 / (double)Nk, reinterpret_cast<cx<double>*>(chi_dev));
+    }
+    LQM_HIP(hipGetLastError());
+    LQM_HIP(hipStreamSynchronize(c->stream));
+    return LQM_OK;
+}
+
+lqm_status lqm_kubo(lqm_model* h, lqm_model* j1, lqm_model* j2, int Nk, const double* ks, const double* kweights,
+                    int Nw, const double* omegas, double mu, double gamma, double T, double* chi) {
+    lqm_status st = check_model(h);
+    if (st) return st;
+    if (Nk <= 0 || !ks || Nw <= 0 || !omegas || !chi) return fail(LQM_ERR_ARG, "bad sizes or null buffer");
+    lqm_context* c = h->ctx;
+    void *dks, *dkw = nullptr, *dom, *dout;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * h->d * Nk, &dks))) return st;
+    if (kweights && (st = stage_in(c, W_KW, kweights, sizeof(double) * Nk, &dkw))) return st;
+    if ((st = stage_in(c, W_OM, omegas, sizeof(double) * Nw, &dom))) return st;
+    if ((st = ws(c, W_OUT, sizeof(cx<double>) * Nw, &dout))) return st;
+    if ((st = lqm_kubo_dev(h, j1, j2, Nk, (const double*)dks, (const double*)dkw, Nw, (const double*)dom, mu,
+                           gamma, T, (double*)dout)))
+        return st;
+    return d2h(c, chi, dout, sizeof(cx<double>) * Nw);
 }
 
 lqm_status lqm_bdg_filling(lqm_model* m, int Nk, const double* ks, const double* kweights, double mu,

@@ -336,6 +336,37 @@ def haldane(lat: Lattice, t2: float = 0.1, phi: float = np.pi / 2, a0: float | N
     return h
 
 
+def getcurrentoperators(lat: Lattice, hops, **kw) -> list:
+    """Current operators J_a = dH(k)/dk_a, one Hops per lattice direction a = 0 .. d-1
+    (Operators/current.jl:20-50).
+
+    From a Hops: block (i, j) of key L times -i (r_i + A L - r_j)_a, the factor applied to
+    n x n blocks per orbital pair when the Hops carries an internal multiplicity
+    n = D / (number of orbitals) (spin: kron(t, sigma) blocks).  From a pair function
+    t(r1, r2): the Hops of -i (r1 - r2)_a t(r1, r2) (keywords go to gethops)."""
+    if callable(hops) and not isinstance(hops, Hops):
+        t = hops
+
+        def ta(a):
+            return lambda r1, r2: -1j * (r1 - r2)[a] * t(r1, r2)
+
+        return [gethops(lat, ta(a), **kw) for a in range(lat.d)]
+    N, D = lat.n, hops.N
+    if not (D >= N and D % N == 0):
+        raise AssertionError("Hops dimension must be a multiple of the number of orbitals")
+    n = D // N
+    r = lat.positions
+    ones = np.ones((n, n))
+    Js = []
+    for a in range(lat.d):
+        J = Hops()
+        for L, t in hops.items():
+            r0 = r[a] + (lat.A @ np.asarray(L, dtype=float))[a]
+            J[L] = t * np.kron(-1j * (r0[:, None] - r[a][None, :]), ones)
+        Js.append(J)
+    return Js
+
+
 def addhops(h: Hops, extra: Hops) -> Hops:
     """h + extra key by key (addhops!, TightBinding/types.jl)."""
     out = h.copy()

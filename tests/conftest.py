@@ -29,6 +29,17 @@ def oracle():
     return o
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _torch_runtime_first(request):
+    """torch ships its own HIP runtime next to the one liblqmgpu.so links; torch finds the
+    device only when its runtime starts first, so GPU sessions start it before any test (the
+    tests that hand torch buffers to the library need both)."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        import torch
+
+        torch.cuda.is_available()
+
+
 @pytest.fixture(scope="session")
 def gpu(pkg):
     """The GPU executor; fails loudly (no fallback) if the library or device is missing."""
