@@ -282,6 +282,94 @@ def tbg_leg(no_cpu: bool) -> dict:
     return out
 
 
+def response_leg(no_cpu: bool) -> dict:
+    """Eigenvector consumers on the headline model (64 orbitals, 25 keys, 128 x 128 grid),
+    device-resident, FP64 eigenvectors: the Kubo correlator chi_xx(w) on 256 frequencies in
+    [0, 8] (mu = 0, G = 0.05, kT = 0.02) with J_x(k) = dH/dk_x (block R: 2 pi i R_x t_R), and the
+    site-resolved ldos at one frequency.  The pair-sum kernel's share comes from the stage
+    timer (HIP events on the library's stream)."""
+    import torch
+
+    N, side, Nw = 64, 128, 256
+    mu, G, T = 0.0, 0.05, 0.02
+    h = pkg.models.random_tb(N, rmax=2, seed=2)
+    Jx = pkg.Hops({R: 2j * np.pi * R[0] * t for R, t in h.items()})
+    ks = pkg.regulargrid(nk=side * side)
+    Nk = ks.shape[1]
+    dev = torch.device("cuda:0")
+    ks_d = torch.from_numpy(np.ascontiguousarray(ks.T)).to(dev)
+    om = np.linspace(0.0, 8.0, Nw)
+    om_d = torch.from_numpy(om).to(dev)
+    chi = torch.zeros((Nw, 2), dtype=torch.float64, device=dev)
+    ld = torch.zeros(N, dtype=torch.float64, device=dev)
+    ex = api.to_executor("gpu")
+    mh, mj = api.GPUModel(h, ex), api.GPUModel(Jx, ex)
+    L = _lib.lib()
+    vp = ctypes.c_void_p
+
+    def kubo():
+        _lib.check(L.lqm_kubo_dev(mh.handle, mj.handle, mj.handle, Nk, vp(ks_d.data_ptr()), None, Nw,
+                                  vp(om_d.data_ptr()), ctypes.c_double(mu), ctypes.c_double(G), ctypes.c_double(T),
+                                  vp(chi.data_ptr())), "lqm_kubo_dev")
+
+    def ldos():
+        _lib.check(L.lqm_ldos_dev(mh.handle, Nk, vp(ks_d.data_ptr()), None, 1, vp(om_d.data_ptr() + 8 * 64),
+                                  ctypes.c_double(G), vp(ld.data_ptr())), "lqm_ldos_dev")
+
+    def clock(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    tk = clock(kubo)
+    tl = clock(ldos)
+    _lib.call("lqm_context_set_timing", ex.handle, 1)
+    kubo()
+    ms = (ctypes.c_double * _lib.NSTAGES)()
+    cnt = (ctypes.c_int64 * _lib.NSTAGES)()
+    _lib.call("lqm_context_stage_times", ex.handle, ms, cnt, None, _lib.NSTAGES)
+    _lib.call("lqm_context_set_timing", ex.handle, 0)
+    stages = {L.lqm_stage_name(i).decode(): {"ms": ms[i], "launches": cnt[i]}
+              for i in range(_lib.NSTAGES) if cnt[i]}
+    # algorithmic terms of the pair sums: every ordered pair i != j at every frequency; one term
+    # is a complex division of a complex weight by (D - w) - iG plus the complex accumulation,
+    # 12 FP64 flops counting the reciprocal as one
+    terms = float(Nk) * N * (N - 1) * Nw
+    t_pairs = stages.get("response_sums", {}).get("ms", float("nan")) / 1e3
+    out = {"workload": "random_tb_N64_R25_k128x128_kubo_xx_Nw256", "N": N, "Nk": Nk, "Nw": Nw, "mu": mu,
+           "Gamma": G, "kT": T,
+           "kubo": {"seconds": tk, "kpoints_per_s": Nk / tk, "stages_ms": stages,
+                    "pair_sum_seconds": t_pairs, "pair_terms_per_s": terms / t_pairs,
+                    "pair_sum_fp64_tflops_per_s": 12.0 * terms / t_pairs / 1e12,
+                    "pair_sum_frac_of_fp64_peak": 12.0 * terms / t_pairs / 1e12 / PEAK_FP64_TFLOPS},
+           "ldos_one_frequency": {"seconds": tl, "kpoints_per_s": Nk / tl}}
+    if not no_cpu:
+        from oracle import response as orr
+
+        R, Tt = h.pack()
+        RJ, TJ = Jx.pack()
+        S = 512
+        sub = np.asfortranarray(ks[:, :: Nk // S][:, :S])
+        c0 = time.perf_counter()
+        ref = orr.kubo(R, Tt, RJ, TJ, RJ, TJ, sub, om, mu, G, T)
+        tc = time.perf_counter() - c0
+        out["cpu_baseline"] = {"kpoints_per_s": S / tc, "cores": host_threads(), "kind": "port",
+                               "sample": f"{S} of the {Nk} k-points, chi_xx on the 256 frequencies"}
+        # the whole-grid GPU result against the oracle on the same sample, run on the GPU
+        sub_d = torch.from_numpy(np.ascontiguousarray(sub.T)).to(dev)
+        _lib.check(L.lqm_kubo_dev(mh.handle, mj.handle, mj.handle, S, vp(sub_d.data_ptr()), None, Nw,
+                                  vp(om_d.data_ptr()), ctypes.c_double(mu), ctypes.c_double(G), ctypes.c_double(T),
+                                  vp(chi.data_ptr())), "lqm_kubo_dev")
+        c = chi.cpu().numpy()
+        c = c[:, 0] + 1j * c[:, 1]
+        out["sample_max_rel_err_vs_oracle"] = float(np.max(np.abs(c - ref)) / np.max(np.abs(ref)))
+    return out
+
+
 def batch_leg() -> dict:
     """Uneven models, scaled down from 256 x N in [16, 512]: 48 spinful random models with
     N = 2 x sites drawn log-uniform in [8, 96] (seed 4), onsite Hubbard U ~ U(1, 4) in the
@@ -375,6 +463,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-SCF leg")
     ap.add_argument("--no-tbg", action="store_true", help="skip the 1588-orbital twisted-bilayer leg")
     ap.add_argument("--no-cfg4", action="store_true", help="skip the 256-model batch iteration leg")
+    ap.add_argument("--no-response", action="store_true", help="skip the Kubo / ldos leg")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after timing, copy this many MiB device-to-device once (HBM counter calibration)")
     args = ap.parse_args()
@@ -560,6 +649,7 @@ def main() -> int:
     bat = None if args.no_batch else batch_leg()
     tbg = None if args.no_tbg else tbg_leg(args.no_cpu)
     cfg4 = None if args.no_cfg4 else cfg4_leg()
+    resp = None if args.no_response else response_leg(args.no_cpu)
 
     line = {
         "metric": METRIC,
@@ -594,6 +684,7 @@ def main() -> int:
         "batch_scf": bat,
         "tbg_large_n": tbg,
         "cfg4_batch_iteration": cfg4,
+        "response": resp,
         **out,
     }
     print(json.dumps(line))

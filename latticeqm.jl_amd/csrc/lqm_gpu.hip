@@ -3107,7 +3107,10 @@ lqm_status lqm_kubo_dev(lqm_model* h, lqm_model* j1, lqm_model* j2, int Nk, cons
     if ((st = wsT(c, W_ROCC, (size_t)N * Kx, &occ))) return st;
     if ((st = wsT(c, W_RP, (size_t)G * Nw, &part))) return st;
     const cx<double> one = {1.0, 0.0}, zero = {0.0, 0.0};
-    const dim3 gk(G, (unsigned)((Nw + 63) / 64));
+    // frequencies per lane: 4 for wide windows (one LDS read of a pair serves 4 terms)
+    const int F = Nw > 128 ? 4 : (Nw > 64 ? 2 : 1);
+    const dim3 gk(G, (unsigned)((Nw + 64 * F - 1) / (64 * F)));
+    auto kK = F == 4 ? lqm::k_kubo<4> : (F == 2 ? lqm::k_kubo<2> : lqm::k_kubo<1>);
     for (int k0 = 0; k0 < Nk; k0 += Kx) {
         const int Kc = std::min(Kx, Nk - k0);
         const double* ksc = ks_dev + (size_t)k0 * h->d;
@@ -3132,7 +3135,7 @@ lqm_status lqm_kubo_dev(lqm_model* h, lqm_model* j1, lqm_model* j2, int Nk, cons
         StageScope sc(c, ST_RESP, Kc);
         const int L = N * Kc;
         hipLaunchKernelGGL(lqm::k_fd_occ, dim3((L + 255) / 256), dim3(256), 0, c->stream, L, eb, mu, T, occ);
-        hipLaunchKernelGGL(lqm::k_kubo, gk, dim3(256), 0, c->stream, N, Kc, eb, occ, A, C,
+        hipLaunchKernelGGL(kK, gk, dim3(256), 0, c->stream, N, Kc, eb, occ, A, C,
                            kweights_dev ? kweights_dev + k0 : nullptr, Nw, omegas_dev, gamma, k0 == 0 ? 1 : 0,
                            part);
         LQM_HIP(hipGetLastError());

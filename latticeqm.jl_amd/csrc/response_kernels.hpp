@@ -68,16 +68,19 @@ __global__ __launch_bounds__(256) void k_site_weights(int N, int Kc, const doubl
         for (int i = threadIdx.x; i < N; i += 256) out[i] = 0.0;
 }
 
-// Kubo pair sums (LinearResponse/kubo.jl:11-31) over the k-points of a chunk, for the 64
-// frequencies o = 64 blockIdx.y + lane:
+// Kubo pair sums (LinearResponse/kubo.jl:11-31) over the k-points of a chunk, for the 64 F
+// frequencies o = 64 F blockIdx.y + 64 f + lane (f < F):
 //   acc[o] += sum_k w_k sum_{i != j} (f_i - f_j) A_ij conj(C_ij) / ((e_i - e_j) - (w_o + i G))
 // with A = U^H J1 U and C = U^H J2 U per k-point (A_ij conj(C_ij) = (U^H J1 U)_ij
 // (U^H J2^H U)_ji, the reference's M).  Each block takes the k-points b, b + G, ...; per
 // k-point its 256 threads evaluate the pair weights 256 (i, j) at a time, compact the nonzero
 // ones (f_i = f_j, in particular i = j, gives exactly zero) in order into an LDS list, and
-// every wave then sums a quarter of the list for its lane's frequency.  The four wave sums
+// every wave then sums a quarter of the list for its lane's F frequencies (one LDS read of an
+// entry serves F terms).  A term w / (a - iG), a = D - w_o, is w (a + iG) / (a^2 + G^2) with
+// the reciprocal from the hardware approximation and one Newton step.  The four wave sums
 // meet in LDS at the end: part[b][o] (complex) is written or, after the first chunk, added to.
 constexpr int kKuboList = 1024;
+template <int F>
 __global__ __launch_bounds__(256) void k_kubo(int N, int Kc, const double* __restrict__ bands,
                                               const double* __restrict__ occ, const cx<double>* __restrict__ A,
                                               const cx<double>* __restrict__ C, const double* __restrict__ kw,
@@ -88,10 +91,16 @@ __global__ __launch_bounds__(256) void k_kubo(int N, int Kc, const double* __res
     __shared__ int swc[4];
     __shared__ double sre[4][64], sim[4][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int o = blockIdx.y * 64 + lane;
-    const double wo = o < Nw ? om[o] : 0.0;
-    const size_t NN = (size_t)N * N;
-    double re = 0.0, im = 0.0;
+    const int o0 = blockIdx.y * 64 * F + lane;
+    double wo[F], re[F], im[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        wo[f] = o0 + 64 * f < Nw ? om[o0 + 64 * f] : 0.0;
+        re[f] = 0.0;
+        im[f] = 0.0;
+    }
+    const int NN = N * N;
+    const double g2 = gamma * gamma;
     int cnt = 0;
     // sum the listed pairs: wave wv takes entries [wv q, min((wv + 1) q, cnt))
     auto flush = [&]() {
@@ -99,11 +108,16 @@ __global__ __launch_bounds__(256) void k_kubo(int N, int Kc, const double* __res
         const int q = (cnt + 3) / 4;
         const int e0 = wv * q, e1 = min(cnt, e0 + q);
         for (int e = e0; e < e1; ++e) {
-            const double a = sD[e] - wo;
+            const double D = sD[e];
             const cx<double> w = sW[e];
-            const double r = 1.0 / fma(a, a, gamma * gamma);
-            re += (w.re * a - w.im * gamma) * r;
-            im += (w.im * a + w.re * gamma) * r;
+            const double wgi = -w.im * gamma, wgr = w.re * gamma;
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                const double a = D - wo[f];
+                const double r = rcp_nr(fma(a, a, g2));
+                re[f] = fma(fma(w.re, a, wgi), r, re[f]);
+                im[f] = fma(fma(w.im, a, wgr), r, im[f]);
+            }
         }
         __syncthreads();
         cnt = 0;
@@ -111,16 +125,16 @@ __global__ __launch_bounds__(256) void k_kubo(int N, int Kc, const double* __res
     for (int kk = blockIdx.x; kk < Kc; kk += gridDim.x) {
         const double wk = kw ? kw[kk] : 1.0;
         const double* e = bands + (size_t)kk * N;
-        const double* f = occ + (size_t)kk * N;
+        const double* fo = occ + (size_t)kk * N;
         const cx<double>* a = A + (size_t)kk * NN;
         const cx<double>* c = C + (size_t)kk * NN;
-        for (size_t p0 = 0; p0 < NN; p0 += 256) {
-            const size_t p = p0 + threadIdx.x;
+        for (int p0 = 0; p0 < NN; p0 += 256) {
+            const int p = p0 + threadIdx.x;
             cx<double> w = {0.0, 0.0};
             double D = 0.0;
             if (p < NN) {
-                const int i = (int)(p % N), j = (int)(p / N);
-                const double df = f[i] - f[j];
+                const int i = p % N, j = p / N;
+                const double df = fo[i] - fo[j];
                 if (df != 0.0) {
                     const cx<double> x = a[p], y = c[p];
                     // x conj(y), times (f_i - f_j) w_k
@@ -147,15 +161,20 @@ __global__ __launch_bounds__(256) void k_kubo(int N, int Kc, const double* __res
         }
     }
     if (cnt > 0) flush();
-    sre[wv][lane] = re;
-    sim[wv][lane] = im;
-    __syncthreads();
-    if (wv == 0 && o < Nw) {
-        const double tr = ((sre[0][lane] + sre[1][lane]) + sre[2][lane]) + sre[3][lane];
-        const double ti = ((sim[0][lane] + sim[1][lane]) + sim[2][lane]) + sim[3][lane];
-        cx<double>* out = part + (size_t)blockIdx.x * Nw + o;
-        if (first) *out = {tr, ti};
-        else *out = {out->re + tr, out->im + ti};
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        sre[wv][lane] = re[f];
+        sim[wv][lane] = im[f];
+        __syncthreads();
+        const int o = o0 + 64 * f;
+        if (wv == 0 && o < Nw) {
+            const double tr = ((sre[0][lane] + sre[1][lane]) + sre[2][lane]) + sre[3][lane];
+            const double ti = ((sim[0][lane] + sim[1][lane]) + sim[2][lane]) + sim[3][lane];
+            cx<double>* out = part + (size_t)blockIdx.x * Nw + o;
+            if (first) *out = {tr, ti};
+            else *out = {out->re + tr, out->im + ti};
+        }
+        __syncthreads();
     }
 }
 
