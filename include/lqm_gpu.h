@@ -48,6 +48,7 @@ typedef enum {
 
 typedef struct lqm_context lqm_context;
 typedef struct lqm_model lqm_model;
+typedef struct lqm_sparse_model lqm_sparse_model;
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *lqm_last_error(void);
@@ -76,8 +77,9 @@ lqm_status lqm_context_synchronize(lqm_context *ctx);
  * 21 blocked_bisect, 22 blocked_invit, 23 blocked_backtransform, 24 blocked_rho,
  * 25 oa_correct_later (14 = the first refinement sweep, on every k-point; 25 = later sweeps
  * and cluster corrections, on listed k-points), 26 response_sums (the ldos / density site
- * weights and the Kubo pair sums) (LQM_NSTAGES = 27); lqm_stage_name gives the names. */
-#define LQM_NSTAGES 27
+ * weights and the Kubo pair sums), 27 partial_filter (the Chebyshev filter of the partial
+ * spectrum) (LQM_NSTAGES = 28); lqm_stage_name gives the names. */
+#define LQM_NSTAGES 28
 lqm_status lqm_context_set_timing(lqm_context *ctx, int enable);
 /* SCF iterations from captured HIP graphs (no reference counterpart; default on).  Where an
  * iteration needs no host round trip (per-k FP64 kernels, T > 0, the grid in one chunk),
@@ -207,6 +209,34 @@ lqm_status lqm_kubo(lqm_model *h, lqm_model *j1, lqm_model *j2, int Nk, const do
 lqm_status lqm_kubo_dev(lqm_model *h, lqm_model *j1, lqm_model *j2, int Nk, const double *ks_dev,
                         const double *kweights_dev, int Nw, const double *omegas_dev, double mu,
                         double gamma, double T, double *chi_dev);
+
+/* ---- partial spectrum (format = :sparse, num_bands) -------------------------------- */
+/* A sparse Bloch Hamiltonian H(k) = sum_r T_r e^{2 pi i k.R_r} in one CSR for all keys: row i
+ * holds the entries rowptr[i] .. rowptr[i+1]-1, entry e = (column col[e], key key[e], value
+ * val[2e] + i val[2e+1]); entries of one (i, j) under several keys add.  The sparse Hops of
+ * the reference's format = :sparse (TightBinding/types.jl:102). */
+lqm_status lqm_sparse_model_create(lqm_context *ctx, int N, int d, int nR, const int32_t *R, int64_t nnz,
+                                   const int64_t *rowptr, const int32_t *col, const int32_t *key,
+                                   const double *val, lqm_sparse_model **out);
+lqm_status lqm_sparse_model_destroy(lqm_sparse_model *m);
+/* max_i sum_e |t_e|: a bound on ||H(k)||_2 at every k (the residual scale of the solver). */
+lqm_status lqm_sparse_model_bound(lqm_sparse_model *m, double *bound);
+/* eigen_sparse(H(k); num_bands, sigma, which) (Eigen/eigen_sparse_krylovkit.jl:44-88) at Nk
+ * k-points: which = 0 (:LM) the nb eigenpairs nearest sigma, 1 (:SR) the smallest, 2 (:LR) the
+ * largest; eigenvalues ascending, evals [nb x Nk], evecs (may be NULL) [N x nb x Nk] complex,
+ * orthonormal columns.  The reference's shift-invert Lanczos is replaced by Chebyshev-filtered
+ * subspace iteration (subspace of max(2 nb + 10, 20) vectors, at most 64, so nb <= 56; start
+ * block seeded deterministically): each iteration filters the block with a Chebyshev
+ * polynomial in B = (H - sigma)^2 (LM) or +-H (SR, LR) that damps B's spectrum above the
+ * widest Ritz value, orthonormalises it (shifted CholeskyQR and two plain passes) and takes
+ * Rayleigh-Ritz pairs; a k-point is done when every wanted pair has ||H x - l x|| <=
+ * tol * bound.  iters (may be NULL) [Nk]: the iteration a k-point converged in, -1 if maxiter
+ * ran out (its last Ritz pairs are returned). */
+lqm_status lqm_eigen_partial(lqm_sparse_model *m, int Nk, const double *ks, int nb, int which, double sigma,
+                             double tol, int maxiter, double *evals, double *evecs, int32_t *iters);
+lqm_status lqm_eigen_partial_dev(lqm_sparse_model *m, int Nk, const double *ks_dev, int nb, int which,
+                                 double sigma, double tol, int maxiter, double *evals_dev, double *evecs_dev,
+                                 int32_t *iters);
 
 /* ---- DOS -------------------------------------------------------------------------- */
 /* dos_compute! (Spectrum/dos.jl:202-232): DOS(w) = sum_k wk sum_n Im[1/(w - i G - e_nk)],

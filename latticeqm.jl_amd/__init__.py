@@ -17,7 +17,7 @@ def __getattr__(name):
     # containers stay importable where the library has not been built.
     import importlib
 
-    if name in ("api", "meanfield", "bdg", "berry", "response"):
+    if name in ("api", "meanfield", "bdg", "berry", "response", "sparse"):
         return importlib.import_module(__name__ + "." + name)
     api = importlib.import_module(__name__ + ".api")
     if hasattr(api, name):

@@ -61,6 +61,11 @@ _PROTOS = {
     "lqm_density_dev": [P, I, P, P, D, P],
     "lqm_kubo": [P, P, P, I, P, P, I, P, D, D, D, P],
     "lqm_kubo_dev": [P, P, P, I, P, P, I, P, D, D, D, P],
+    "lqm_sparse_model_create": [P, I, I, I, P, ctypes.c_int64, P, P, P, P, PP],
+    "lqm_sparse_model_destroy": [P],
+    "lqm_sparse_model_bound": [P, P],
+    "lqm_eigen_partial": [P, I, P, I, I, D, D, I, P, P, P],
+    "lqm_eigen_partial_dev": [P, I, P, I, I, D, D, I, P, P, P],
     "lqm_dos_from_bands_dev": [P, I, I, P, P, I, P, D, P],
     "lqm_dos_from_bands_dev_noalloc": [P, I, I, P, P, I, P, D, P],
     "lqm_chemicalpotential": [P, I, P, P, D, D, P],
@@ -87,7 +92,7 @@ _PROTOS = {
                             P, P, P],
 }
 _RESTYPE = {"lqm_last_error": ctypes.c_char_p, "lqm_abi_version": I, "lqm_stage_name": ctypes.c_char_p}
-NSTAGES = 27
+NSTAGES = 28
 
 
 class HFProblem(ctypes.Structure):

@@ -214,9 +214,21 @@ def _projector_models(projectors, m: GPUModel):
     return out
 
 
-def bandmatrix(H, ks, *projectors, executor=None, multimode="gpu"):
+def bandmatrix(H, ks, *projectors, executor=None, multimode="gpu", format="dense", num_bands=None,
+               sigma=1e-8, **kw):
     """Eigenvalues of Hermitian(H(k)) per k: returns (bands (N, Nk), obs (N, Nk, nP)) with
-    obs[n, j, p] = real(psi^H O_p(k_j) psi) for the eigenvector of bands[n, j]."""
+    obs[n, j, p] = real(psi^H O_p(k_j) psi) for the eigenvector of bands[n, j].
+    format="sparse" with num_bands: the num_bands eigenvalues nearest sigma per k (the
+    partial-spectrum path), bands (num_bands, Nk)."""
+    if format == "sparse":
+        from .sparse import eigen_sparse
+
+        if num_bands is None:
+            raise ValueError("format='sparse' needs num_bands")
+        if projectors:
+            raise NotImplementedError("projectors with format='sparse'")
+        b, _ = eigen_sparse(H, ks, num_bands, sigma=sigma, vectors=False, executor=executor, **kw)
+        return b, np.zeros((num_bands, b.shape[1], 0))
     m = _model(H, executor or multimode)
     k = _kpts(ks, m.d)
     Nk = k.shape[1]
@@ -231,9 +243,16 @@ def bandmatrix(H, ks, *projectors, executor=None, multimode="gpu"):
     return bands, obs
 
 
-def eigen(H, ks, executor=None):
+def eigen(H, ks, executor=None, format="dense", num_bands=None, sigma=1e-8, **kw):
     """Eigenvalues and eigenvectors per k (the geteigen! branch, bands.jl:222-224):
-    (bands (N, Nk), vecs (N, N, Nk)) with vecs[:, n, j] the eigenvector of bands[n, j]."""
+    (bands (N, Nk), vecs (N, N, Nk)) with vecs[:, n, j] the eigenvector of bands[n, j].
+    format="sparse" with num_bands: the num_bands eigenpairs nearest sigma per k."""
+    if format == "sparse":
+        from .sparse import eigen_sparse
+
+        if num_bands is None:
+            raise ValueError("format='sparse' needs num_bands")
+        return eigen_sparse(H, ks, num_bands, sigma=sigma, executor=executor, **kw)
     m = _model(H, executor)
     k = _kpts(ks, m.d)
     Nk = k.shape[1]

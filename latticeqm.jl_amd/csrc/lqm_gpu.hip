@@ -28,6 +28,7 @@
 #include "blocked_kernels.hpp"
 #include "tri_kernels.hpp"
 #include "response_kernels.hpp"
+#include "partial_kernels.hpp"
 
 using lqm::cx;
 
@@ -69,7 +70,7 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_LNK, W_LNKO, W_RB, W_ROCC, W_RA, W_RC, W_RP, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_LNK, W_LNKO, W_RB, W_ROCC, W_RA, W_RC, W_RP, W_SPB, W_SPPH, W_SPM, W_SPT, W_SPI, W_SPC, W_SPKS, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -115,7 +116,7 @@ lqm_status scratch_address(int device, double** out) {
 enum Stage {
     ST_TRIDIAG, ST_QL, ST_REPLAY, ST_BACKTRANSFORM, ST_ZFZ, ST_RHO2, ST_CONTRACT, ST_FOLD, ST_DOS,
     ST_REDUCE, ST_HK64, ST_GEMM_NN, ST_GEMM_CN, ST_GEMM_NC, ST_OA, ST_HF, ST_MIX, ST_MU, ST_BHK,
-    ST_BTRD, ST_BUPD, ST_BEIG, ST_BVEC, ST_BBT, ST_BRHO, ST_OA2, ST_RESP, ST_N
+    ST_BTRD, ST_BUPD, ST_BEIG, ST_BVEC, ST_BBT, ST_BRHO, ST_OA2, ST_RESP, ST_SPF, ST_N
 };
 static_assert(ST_N == LQM_NSTAGES, "stage list and LQM_NSTAGES disagree");
 const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "backtransform", "zfz",
@@ -124,7 +125,8 @@ const char* const kStageNames[ST_N] = {"hk_tridiag", "tridiag_ql", "replay", "ba
                                        "zgemm_nc_fp64", "oa_correct", "hf_meanfield", "mix",
                                        "mu_bisect", "blocked_hk", "blocked_panel", "blocked_update",
                                        "blocked_bisect", "blocked_invit", "blocked_backtransform",
-                                       "blocked_rho", "oa_correct_later", "response_sums"};
+                                       "blocked_rho", "oa_correct_later", "response_sums",
+                                       "partial_filter"};
 
 struct TimedLaunch {
     int stage;
@@ -177,6 +179,19 @@ struct lqm_model {
     std::vector<int> R;
     int* dR = nullptr;
     cx<double>* dT = nullptr;
+};
+
+// A sparse Bloch Hamiltonian: one CSR for all keys, entry e of row i = (column, key, value).
+struct lqm_sparse_model {
+    lqm_context* ctx = nullptr;
+    int N = 0, d = 0, nR = 0;
+    long long nnz = 0;
+    std::vector<int> R;
+    int* dR = nullptr;
+    long long* rowptr = nullptr;
+    int2* colkey = nullptr;
+    cx<double>* val = nullptr;
+    double bound = 0.0;  // max_i sum_e |t_e| >= ||H(k)||_2 at every k
 };
 
 namespace {
@@ -3335,6 +3350,388 @@ lqm_status lqm_link_overlaps(lqm_context* c, int N, int M, int npts, const doubl
     if ((st = ws(c, W_OUT, sizeof(cx<double>) * (size_t)M * M * nlink, &dO))) return st;
     if ((st = lqm_link_overlaps_dev(c, N, M, (const double*)dU, nlink, pairs, (double*)dO))) return st;
     return d2h(c, O, dO, sizeof(cx<double>) * (size_t)M * M * nlink);
+}
+
+// ---- partial spectrum: Chebyshev-filtered subspace iteration ------------------------
+lqm_status lqm_sparse_model_create(lqm_context* ctx, int N, int d, int nR, const int32_t* R, int64_t nnz,
+                                   const int64_t* rowptr, const int32_t* col, const int32_t* key, const double* val,
+                                   lqm_sparse_model** out) {
+    if (!ctx || !out) return fail(LQM_ERR_ARG, "null context or output pointer");
+    if (N <= 0 || d <= 0 || nR <= 0 || nnz < 0) return fail(LQM_ERR_ARG, "N, d, nR must be positive, nnz >= 0");
+    if (nR > lqm::kSpMaxKeys) return fail(LQM_ERR_UNSUPPORTED, "more than 2048 keys");
+    if (!R || !rowptr || (nnz > 0 && (!col || !key || !val))) return fail(LQM_ERR_ARG, "null array");
+    if (rowptr[0] != 0 || rowptr[N] != nnz) return fail(LQM_ERR_ARG, "row pointer must run from 0 to nnz");
+    std::vector<int2> ck((size_t)std::max<int64_t>(nnz, 1));
+    double bound = 0.0;
+    for (int i = 0; i < N; ++i) {
+        if (rowptr[i + 1] < rowptr[i]) return fail(LQM_ERR_ARG, "row pointer decreases");
+        double rs = 0.0;
+        for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+            if (col[e] < 0 || col[e] >= N || key[e] < 0 || key[e] >= nR)
+                return fail(LQM_ERR_ARG, "column or key index out of range");
+            if (!std::isfinite(val[2 * e]) || !std::isfinite(val[2 * e + 1])) return fail(LQM_ERR_NONFINITE, "non-finite value");
+            ck[e] = make_int2(col[e], key[e]);
+            rs += std::hypot(val[2 * e], val[2 * e + 1]);
+        }
+        bound = std::max(bound, rs);
+    }
+    auto* m = new lqm_sparse_model();
+    m->ctx = ctx;
+    m->N = N;
+    m->d = d;
+    m->nR = nR;
+    m->nnz = nnz;
+    m->R.assign(R, R + (size_t)d * nR);
+    m->bound = bound;
+    const size_t ne = (size_t)std::max<int64_t>(nnz, 1);
+    hipError_t e = dev_malloc(&m->dR, sizeof(int) * d * nR);
+    if (e == hipSuccess) e = dev_malloc(&m->rowptr, sizeof(long long) * (N + 1));
+    if (e == hipSuccess) e = dev_malloc(&m->colkey, sizeof(int2) * ne);
+    if (e == hipSuccess) e = dev_malloc(&m->val, sizeof(cx<double>) * ne);
+    if (e == hipSuccess) e = hipMemcpyAsync(m->dR, R, sizeof(int) * d * nR, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(m->rowptr, rowptr, sizeof(long long) * (N + 1), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && nnz > 0)
+        e = hipMemcpyAsync(m->colkey, ck.data(), sizeof(int2) * nnz, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && nnz > 0)
+        e = hipMemcpyAsync(m->val, val, sizeof(cx<double>) * nnz, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        (void)dev_free(m->dR);
+        (void)dev_free(m->rowptr);
+        (void)dev_free(m->colkey);
+        (void)dev_free(m->val);
+        delete m;
+        return fail(LQM_ERR_DEVICE, std::string("sparse model allocation: ") + hipGetErrorString(e));
+    }
+    *out = m;
+    return LQM_OK;
+}
+
+lqm_status lqm_sparse_model_destroy(lqm_sparse_model* m) {
+    if (!m) return LQM_OK;
+    (void)hipStreamSynchronize(m->ctx->stream);
+    (void)dev_free(m->dR);
+    (void)dev_free(m->rowptr);
+    (void)dev_free(m->colkey);
+    (void)dev_free(m->val);
+    delete m;
+    return LQM_OK;
+}
+
+lqm_status lqm_sparse_model_bound(lqm_sparse_model* m, double* bound) {
+    if (!m || !bound) return fail(LQM_ERR_ARG, "null model or output");
+    *bound = m->bound;
+    return LQM_OK;
+}
+
+namespace {
+
+// Eigenpairs of Kc Hermitian n x n matrices on the device (A, column-major, upper triangle
+// read): the per-k eigensolver run on them through a stand-in model of one zero key (the
+// variants that read a materialised H(k)).  theta: ascending eigenvalues, P: eigenvectors.
+lqm_status eigh_given(lqm_model* dm, int Kc, const double* ks0, const cx<double>* A, double* theta, cx<double>* P) {
+    lqm_context* c = dm->ctx;
+    const int n = dm->N;
+    Plan PL = make_plan(c, n, 1, Kc, true, false);
+    PL.nqA = 0;  // the one-wave tridiagonalizations assemble H(k) themselves
+    lqm_status st;
+    for (int k0 = 0; k0 < Kc; k0 += PL.Kc) {
+        const int kc = std::min(PL.Kc, Kc - k0);
+        const size_t o = (size_t)k0 * n * n;
+        if ((st = launch_eigvals<double>(dm, PL, kc, ks0, theta + (size_t)k0 * n, true, A + o))) return st;
+        if ((st = launch_backtransform<double, 0>(dm, PL, kc, theta + (size_t)k0 * n, nullptr, 0.0, 0.0, 0.0, P + o,
+                                                  nullptr)))
+            return st;
+    }
+    return LQM_OK;
+}
+
+struct ModelGuard {
+    lqm_model* m = nullptr;
+    ~ModelGuard() { lqm_model_destroy(m); }
+};
+
+}  // namespace
+
+lqm_status lqm_eigen_partial_dev(lqm_sparse_model* m, int Nk, const double* ks_dev, int nb, int which, double sigma,
+                                 double tol, int maxiter, double* evals_dev, double* evecs_dev, int32_t* iters) {
+    if (!m) return fail(LQM_ERR_ARG, "null model");
+    if (Nk <= 0 || !ks_dev || !evals_dev) return fail(LQM_ERR_ARG, "bad k-points or null output");
+    const int N = m->N;
+    if (nb < 1 || nb > N) return fail(LQM_ERR_ARG, "num_bands must be in 1..N");
+    if (which < 0 || which > 2) return fail(LQM_ERR_ARG, "which: 0 closest to sigma, 1 smallest, 2 largest");
+    if (!(tol > 0.0) || maxiter < 1) return fail(LQM_ERR_ARG, "tol must be positive, maxiter >= 1");
+    if (!std::isfinite(sigma)) return fail(LQM_ERR_ARG, "non-finite sigma");
+    const int msub = std::min(N, std::max(2 * nb + 10, 20) > 64 ? std::max(nb + 8, 64) : std::max(2 * nb + 10, 20));
+    if (msub > 64) return fail(LQM_ERR_UNSUPPORTED, "num_bands above 56 (subspace wider than 64)");
+    const int mp = msub <= 16 ? 16 : (msub <= 32 ? 32 : 64);
+    lqm_context* c = m->ctx;
+    begin_pass(c, LQM_PREC_FP64);
+    const hipStream_t s = c->stream;
+    const size_t NP = (size_t)N * mp, MM = (size_t)msub * msub;
+    const size_t per = 4 * 16 * NP + 4 * 16 * MM + 64 * (size_t)msub + 16 * (size_t)m->nR + 64;
+    const int Kx = (int)std::max<size_t>(1, std::min<size_t>({(size_t)Nk, (size_t(16) << 30) / per, (size_t)65535}));
+    lqm_status st;
+    cx<double> *buf, *ph, *mats;
+    double *th, *coef, *ks0;
+    int* ints;
+    if ((st = wsT(c, W_SPB, 4 * NP * Kx, &buf))) return st;
+    if ((st = wsT(c, W_SPPH, (size_t)m->nR * Kx, &ph))) return st;
+    if ((st = wsT(c, W_SPM, 4 * MM * Kx, &mats))) return st;
+    if ((st = wsT(c, W_SPT, 2 * (size_t)msub * Kx, &th))) return st;
+    if ((st = wsT(c, W_SPI, 4 * (size_t)Kx, &ints))) return st;
+    if ((st = wsT(c, W_SPKS, (size_t)Kx, &ks0))) return st;
+    LQM_HIP(hipMemsetAsync(ks0, 0, sizeof(double) * Kx, s));
+    cx<double>* G = mats;
+    cx<double>* Li = mats + MM * Kx;
+    cx<double>* S = mats + 2 * MM * Kx;
+    cx<double>* P = mats + 3 * MM * Kx;
+    double* rn = th + (size_t)msub * Kx;
+    int* kidx = ints;
+    int* flag = ints + Kx;
+    int* lo_d = ints + 2 * Kx;
+    int* outl = ints + 3 * Kx;
+    // stand-in model for the Rayleigh-Ritz eigensolves
+    ModelGuard dm;
+    {
+        const int32_t z[1] = {0};
+        if ((st = model_init(c, msub, 1, 1, z, &dm.m))) return st;
+        LQM_HIP(hipMemsetAsync(dm.m->dR, 0, sizeof(int), s));
+        LQM_HIP(hipMemsetAsync(dm.m->dT, 0, sizeof(cx<double>) * MM, s));
+    }
+    // spectrum of B: LM (H - sigma)^2 in [0, beta^2]; SR H and LR -H in [-rho, rho]
+    const double rho = std::max(m->bound, 1e-300);
+    const double beta = rho + std::fabs(sigma);
+    const double a0 = which == 0 ? 0.0 : -rho, ub = which == 0 ? beta * beta : rho;
+    const double shift = which == 0 ? sigma : 0.0;
+    const double acosh_kappa = std::acosh(1e7);
+    const int nmax = 4000;
+    const double shift_rel = 11.0 * ((double)N * msub + (double)msub * (msub + 1)) * std::ldexp(1.0, -53);
+    const cx<double> one = {1.0, 0.0}, zero = {0.0, 0.0};
+    const long long sB = (long long)NP, sM = (long long)MM;
+    auto spmm = [&](int nact, const double* cf, const cx<double>* X, const cx<double>* pp, const cx<double>* qq,
+                    cx<double>* Y) {
+        hipLaunchKernelGGL(lqm::k_spmm, dim3((unsigned)((N + lqm::kSpRows - 1) / lqm::kSpRows), (unsigned)nact),
+                           dim3(256), 0, s, N, mp, msub, m->rowptr, m->colkey, m->val, m->nR, ph, shift, kidx, cf, X,
+                           pp, qq, Y);
+    };
+    std::vector<double> hth, hrn, hcoef;
+    std::vector<int> hk, hlo;
+    std::vector<double> cut;
+    const size_t chol_lds = sizeof(cx<double>) * 65 * msub;
+    auto kC = lqm::k_chol_inv;
+    allow_lds(kC, chol_lds);
+    for (int k0 = 0; k0 < Nk; k0 += Kx) {
+        const int Kc = std::min(Kx, Nk - k0);
+        const double* ksc = ks_dev + (size_t)k0 * m->d;
+        hipLaunchKernelGGL(lqm::k_sp_phases, dim3((Kc * m->nR + 255) / 256), dim3(256), 0, s, Kc, m->d, m->nR, m->dR,
+                           ksc, ph);
+        cx<double>* B[4] = {buf, buf + NP * Kx, buf + 2 * NP * Kx, buf + 3 * NP * Kx};
+        int ix = 0;  // the buffer holding the current block
+        hipLaunchKernelGGL(lqm::k_sp_start, dim3((unsigned)std::min<size_t>(4096, (NP * Kc + 255) / 256)), dim3(256), 0,
+                           s, N, mp, msub, Kc, (long long)k0, 0x1a771ce9ull, B[0]);
+        LQM_HIP(hipGetLastError());
+        hk.resize(Kc);
+        for (int q = 0; q < Kc; ++q) hk[q] = q;
+        cut.assign(Kc, 0.0);
+        hlo.assign(Kc, 0);
+        {
+            // first cut: the B-value below which about 2m of the N eigenvalues lie for a flat
+            // density of states, kept inside (a0, ub)
+            const double fr = std::min(0.25, 2.0 * msub / (double)N);
+            for (int q = 0; q < Kc; ++q)
+                cut[q] = which == 0 ? std::max(1e-6, fr * fr) * ub : a0 + std::max(1e-6, 2.0 * fr) * (ub - a0);
+        }
+        std::vector<int> done_it(Kc, -1);
+        int it = 0;
+        for (; it < maxiter && !hk.empty(); ++it) {
+            const int nact = (int)hk.size();
+            LQM_HIP(hipMemcpyAsync(kidx, hk.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
+            // filter degree: the conditioning bound T_n(|tau0|) <= 1e7 of the most demanding slot
+            int n = nmax;
+            std::vector<double> ee(Kc), cc(Kc), t0(Kc);
+            for (int q : hk) {
+                const double a = std::min(std::max(cut[q], a0 + 1e-14 * (ub - a0)), ub - 1e-9 * (ub - a0));
+                ee[q] = 0.5 * (ub - a);
+                cc[q] = 0.5 * (ub + a);
+                t0[q] = (a0 - cc[q]) / ee[q];
+                const double ac = std::acosh(std::max(1.0 + 1e-300, std::fabs(t0[q])));
+                n = std::min(n, std::max(1, (int)std::floor(acosh_kappa / ac)));
+            }
+            // per-step coefficients {alpha, beta1, beta2} of the scaled Chebyshev recurrence
+            // q1 = (B - c) q0 / (e r0), q_{s+1} = 2 (B - c) q_s / (e r_s) - q_{s-1} / (r_s r_{s-1}),
+            // r_0 = tau0, r_s = 2 tau0 - 1 / r_{s-1}: the component at a0 stays 1
+            hcoef.assign((size_t)n * Kx * 3, 0.0);
+            for (int q : hk) {
+                double rprev = 0.0, r = t0[q];
+                for (int j = 0; j < n; ++j) {
+                    if (j > 0) {
+                        rprev = r;
+                        r = 2.0 * t0[q] - 1.0 / rprev;
+                    }
+                    const double al = (j == 0 ? 1.0 : 2.0) / (ee[q] * r);
+                    double* cf = &hcoef[((size_t)j * Kx + q) * 3];
+                    cf[0] = which == 2 ? -al : al;
+                    cf[1] = -al * cc[q];
+                    cf[2] = j == 0 ? 0.0 : -1.0 / (r * rprev);
+                }
+            }
+            if ((st = wsT(c, W_SPC, hcoef.size(), &coef))) return st;
+            LQM_HIP(hipMemcpyAsync(coef, hcoef.data(), sizeof(double) * hcoef.size(), hipMemcpyHostToDevice, s));
+            {
+                StageScope sc(c, ST_SPF, (long long)nact * n);
+                // q_0 in B[ix]; q_1 goes to B[c1]; later steps overwrite q_{s-1} in place
+                const int c1 = (ix + 1) % 4, tmp = (ix + 2) % 4;
+                int prev = ix, cur = c1;
+                for (int j = 0; j < n; ++j) {
+                    const double* cf = coef + (size_t)j * Kx * 3;
+                    const int src = j == 0 ? ix : cur;
+                    const int dst = j == 0 ? c1 : prev;
+                    const cx<double>* qq = j == 0 ? nullptr : B[prev];
+                    if (which == 0) {
+                        spmm(nact, nullptr, B[src], nullptr, nullptr, B[tmp]);
+                        spmm(nact, cf, B[tmp], B[src], qq, B[dst]);
+                    } else {
+                        spmm(nact, cf, B[src], B[src], qq, B[dst]);
+                    }
+                    if (j > 0) {
+                        prev = cur;
+                        cur = dst;
+                    }
+                }
+                ix = cur;
+            }
+            LQM_HIP(hipGetLastError());
+            // orthonormal basis: shifted CholeskyQR, then two plain passes (Z <- L^-1 Z)
+            for (int pass = 0; pass < 3; ++pass) {
+                const int dst = (ix + 1) % 4;
+                zgemm_gen<0, 1>(c, ST_GEMM_NC, nact, nact,
+                                [&] {
+                                    auto g = gemm_args(msub, msub, N, B[ix], mp, sB, B[ix], mp, sB, G, msub, sM, one, zero);
+                                    g.kidx = kidx;
+                                    return g;
+                                }());
+                hipLaunchKernelGGL(kC, dim3(nact), dim3(256), chol_lds, s, msub, kidx, G, shift_rel, pass == 0 ? 1 : 0,
+                                   Li, flag);
+                zgemm_gen<0, 0>(c, ST_GEMM_NN, nact, nact,
+                                [&] {
+                                    auto g = gemm_args(msub, N, msub, Li, msub, sM, B[ix], mp, sB, B[dst], mp, sB, one, zero);
+                                    g.kidx = kidx;
+                                    return g;
+                                }());
+                ix = dst;
+            }
+            LQM_HIP(hipGetLastError());
+            // Rayleigh-Ritz: W = (H - shift) Y, conj(S) = Z W_z^H, its eigenpairs, then the Ritz
+            // block X = P^H Z and W Q = P^H W_z, and the residual norms
+            const int iw = (ix + 1) % 4, ixn = (ix + 2) % 4, iwq = (ix + 3) % 4;
+            spmm(nact, nullptr, B[ix], nullptr, nullptr, B[iw]);
+            zgemm_gen<0, 1>(c, ST_GEMM_NC, nact, nact,
+                            [&] {
+                                auto g = gemm_args(msub, msub, N, B[ix], mp, sB, B[iw], mp, sB, S, msub, sM, one, zero);
+                                g.kidx = kidx;
+                                return g;
+                            }());
+            LQM_HIP(hipGetLastError());
+            if ((st = eigh_given(dm.m, Kc, ks0, S, th, P))) return st;
+            zgemm_gen<1, 0>(c, ST_GEMM_CN, nact, nact,
+                            [&] {
+                                auto g = gemm_args(msub, N, msub, P, msub, sM, B[ix], mp, sB, B[ixn], mp, sB, one, zero);
+                                g.kidx = kidx;
+                                return g;
+                            }());
+            zgemm_gen<1, 0>(c, ST_GEMM_CN, nact, nact,
+                            [&] {
+                                auto g = gemm_args(msub, N, msub, P, msub, sM, B[iw], mp, sB, B[iwq], mp, sB, one, zero);
+                                g.kidx = kidx;
+                                return g;
+                            }());
+            hipLaunchKernelGGL(lqm::k_sp_resid, dim3(nact), dim3(256), 0, s, N, mp, msub, kidx, B[ixn], B[iwq], th, rn);
+            LQM_HIP(hipGetLastError());
+            ix = ixn;
+            hth.resize((size_t)msub * Kc);
+            hrn.resize((size_t)msub * Kc);
+            LQM_HIP(hipMemcpyAsync(hth.data(), th, sizeof(double) * msub * Kc, hipMemcpyDeviceToHost, s));
+            LQM_HIP(hipMemcpyAsync(hrn.data(), rn, sizeof(double) * msub * Kc, hipMemcpyDeviceToHost, s));
+            LQM_HIP(hipStreamSynchronize(s));
+            std::vector<int> keep, fin;
+            for (int q : hk) {
+                const double* t = &hth[(size_t)q * msub];
+                int lo = 0;
+                if (which == 2) {
+                    lo = msub - nb;
+                } else if (which == 0) {
+                    // the nb Ritz values nearest 0 (a contiguous window of the ascending list)
+                    int a = 0;
+                    for (int i = 1; i < msub; ++i)
+                        if (std::fabs(t[i]) < std::fabs(t[a])) a = i;
+                    int l = a, r = a;
+                    while (r - l + 1 < nb) {
+                        if (l == 0) ++r;
+                        else if (r == msub - 1) --l;
+                        else if (std::fabs(t[l - 1]) <= std::fabs(t[r + 1])) --l;
+                        else ++r;
+                    }
+                    lo = l;
+                }
+                hlo[q] = lo;
+                bool conv = true;
+                for (int b = lo; b < lo + nb; ++b) conv = conv && hrn[(size_t)q * msub + b] <= tol * rho;
+                if (conv || it + 1 == maxiter) {
+                    if (conv) done_it[q] = it + 1;
+                    fin.push_back(q);
+                    continue;
+                }
+                keep.push_back(q);
+                double nc;
+                if (which == 0) {
+                    nc = 0.0;
+                    for (int i = 0; i < msub; ++i) nc = std::max(nc, t[i] * t[i]);
+                } else {
+                    nc = which == 1 ? t[msub - 1] : -t[0];
+                }
+                cut[q] = nc;
+            }
+            // the finished slots (converged, or at the iteration limit) leave their eigenpairs now:
+            // later iterations no longer update their block
+            if (!fin.empty()) {
+                LQM_HIP(hipMemcpyAsync(lo_d, hlo.data(), sizeof(int) * Kc, hipMemcpyHostToDevice, s));
+                LQM_HIP(hipMemcpyAsync(outl, fin.data(), sizeof(int) * fin.size(), hipMemcpyHostToDevice, s));
+                hipLaunchKernelGGL(lqm::k_sp_output,
+                                   dim3((unsigned)std::min<size_t>(1024, ((size_t)N * nb + 255) / 256), (unsigned)fin.size()),
+                                   dim3(256), 0, s, N, mp, msub, nb, outl, lo_d, th, shift, B[ix],
+                                   evals_dev + (size_t)k0 * nb,
+                                   evecs_dev ? reinterpret_cast<cx<double>*>(evecs_dev) + (size_t)k0 * N * nb : nullptr);
+                LQM_HIP(hipGetLastError());
+                LQM_HIP(hipStreamSynchronize(s));  // outl and lo_d are rewritten next iteration
+            }
+            hk.swap(keep);
+        }
+        if (iters)
+            for (int q = 0; q < Kc; ++q) iters[k0 + q] = done_it[q];
+    }
+    LQM_HIP(hipStreamSynchronize(s));
+    return check_status(c, "partial spectrum");
+}
+
+lqm_status lqm_eigen_partial(lqm_sparse_model* m, int Nk, const double* ks, int nb, int which, double sigma,
+                             double tol, int maxiter, double* evals, double* evecs, int32_t* iters) {
+    if (!m) return fail(LQM_ERR_ARG, "null model");
+    if (Nk <= 0 || !ks || !evals) return fail(LQM_ERR_ARG, "bad k-points or null output");
+    if (nb < 1 || nb > m->N) return fail(LQM_ERR_ARG, "num_bands must be in 1..N");
+    lqm_context* c = m->ctx;
+    lqm_status st;
+    void *dks, *dev, *dvec = nullptr;
+    if ((st = stage_in(c, W_KS, ks, sizeof(double) * m->d * Nk, &dks))) return st;
+    if ((st = ws(c, W_OUT, sizeof(double) * (size_t)nb * Nk, &dev))) return st;
+    if (evecs && (st = ws(c, W_OM, sizeof(cx<double>) * (size_t)m->N * nb * Nk, &dvec))) return st;
+    if ((st = lqm_eigen_partial_dev(m, Nk, (const double*)dks, nb, which, sigma, tol, maxiter, (double*)dev,
+                                    (double*)dvec, iters)))
+        return st;
+    if ((st = d2h(c, evals, dev, sizeof(double) * (size_t)nb * Nk))) return st;
+    if (evecs) return d2h(c, evecs, dvec, sizeof(cx<double>) * (size_t)m->N * nb * Nk);
+    return LQM_OK;
 }
 
 lqm_status lqm_hartreefock_scf_batch(lqm_context* ctx, int d, int M, lqm_hf_problem* problems,

@@ -108,6 +108,45 @@ def honeycomb_hubbard(U: float = 3.0, t: float = -1.0):
     return h, v, np.array([0, 1])
 
 
+def tbg_slater_koster_sparse(m: int = 11, a: float = 1.42, z: float = 3.35, vpppi: float = -2.7,
+                             vppsigma: float = 0.48, decay: float = 0.184, cutoff: float = 3.0):
+    """tbg_slater_koster as a SparseHops (format = :sparse): the same blocks, built from the
+    neighbour pairs within cutoff * a (a k-d tree per key) instead of dense distance arrays, so
+    large twist indices (m = 30: 11164 orbitals) fit in host memory."""
+    from scipy.spatial import cKDTree
+    import scipy.sparse as sp
+
+    from . import lattice
+    from .sparse import SparseHops
+
+    lat = lattice.honeycomb_twisted(m, a=a, z=z)
+    delta = decay * a
+    dmax = cutoff * a
+    pos = lat.positions.T  # [n, 3]
+    n = pos.shape[0]
+    tree = cKDTree(pos)
+    out = SparseHops()
+    for L in [(0, 0), (1, 0), (0, 1), (1, 1), (1, -1)]:
+        shift = lat.A @ np.asarray(L, dtype=float)
+        pairs = cKDTree(pos + shift).query_ball_tree(tree, dmax * (1.0 + 1e-9))
+        rows = np.repeat(np.arange(n), [len(p) for p in pairs])
+        cols = np.fromiter((j for p in pairs for j in p), dtype=np.int64, count=rows.size)
+        dr = (pos[rows] + shift - pos[cols]).T
+        D = np.sqrt(np.sum(dr * dr, axis=0))
+        ok = (D > 1e-6) & (D <= dmax)
+        rows, cols, dr, D = rows[ok], cols[ok], dr[:, ok], D[ok]
+        c2 = dr[2] ** 2 / D ** 2
+        v = vpppi * np.exp(-(D - a) / delta) * (1.0 - c2) + vppsigma * np.exp(-(D - z) / delta) * c2
+        keep = v != 0.0
+        M = sp.csr_matrix((v[keep].astype(np.complex128), (rows[keep], cols[keep])), shape=(n, n))
+        if M.nnz == 0:
+            continue
+        out[L] = M
+        if L != (0, 0):
+            out[tuple(-x for x in L)] = M.conj().T
+    return out
+
+
 def tbg_slater_koster(m: int = 11, a: float = 1.42, z: float = 3.35, vpppi: float = -2.7,
                       vppsigma: float = 0.48, decay: float = 0.184, cutoff: float = 3.0) -> Hops:
     """Twisted bilayer graphene of twist index m (4(3m^2 + 3m + 1) orbitals; m = 11: 1588)
