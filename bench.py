@@ -550,6 +550,69 @@ def cfg4_leg() -> dict:
             "path_tflops_per_s": fl / t / 1e12, "all_finite": bool(all(np.isfinite(r[1]) for r in res))}
 
 
+def cfg4_full_leg(no_cpu: bool) -> dict:
+    """Config 4 to convergence: the 256 models of cfg4_leg (N in [16, 512], 25 keys, Hubbard
+    U ~ U(1, 4), half filling, kT = 0.02, a 32 x 32 grid each) run as one batch on 4 streams,
+    linear mixing beta = 0.5 until the density residual is below 1e-8 (at most 200 iterations),
+    FP64.  The 6 smallest models are checked against the oracle's SCF (order parameters and
+    energies).  Minutes of GPU time: only with --cfg4-full."""
+    import importlib
+
+    import torch
+
+    mf = importlib.import_module("latticeqm_amd.meanfield")
+    rng = np.random.default_rng(4)
+    ks = pkg.regulargrid(nk=32 * 32)
+    Nk = ks.shape[1]
+    sx = np.array([[0, 1], [1, 0]], dtype=complex)
+    probs, Ns = [], []
+    for m in range(256):
+        N = int(2 * round(np.exp(rng.uniform(np.log(16), np.log(512))) / 2))
+        U = float(rng.uniform(1, 4))
+        h = pkg.models.random_tb(N, rmax=2, seed=4000 + m)
+        v = pkg.Hops({(0, 0): U * np.kron(np.eye(N // 2), sx)})
+        rho0 = mf.staggered_guess(v, np.arange(N // 2) % 2, m=0.2, noise=1e-3, seed=m)
+        probs.append((h, v, rho0, 0.5, ks))
+        Ns.append(N)
+        print(f"cfg4-full: built model {m} N={N}", flush=True) if m % 64 == 63 else None
+    kw = dict(T=0.02, beta=0.5, tol=1e-8, iterations=200)
+    ex = api.to_executor("gpu")
+    done = []
+
+    def progress(i, it, conv):
+        done.append(i)
+        if len(done) % 16 == 0 or len(done) == len(probs):
+            print(f"cfg4-full: {len(done)}/{len(probs)} models done (last: N={Ns[i]}, {it} iterations, "
+                  f"converged {conv}) after {time.perf_counter() - t0:.1f} s", flush=True)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = mf.solvehartreefock_batch(probs, nstreams=4, executor=ex, progress=progress, **kw)
+    t = time.perf_counter() - t0
+    its = [int(r[2]["iterations"]) for r in res]
+    nR = 25
+    fl = sum(((64.0 / 3.0) * N ** 3 + 8.0 * (nR + 1) * N * N) * Nk * it for N, it in zip(Ns, its))
+    out = {"workload": "cfg4_256_random_models_N16-512_R25_k32x32_hubbard_to_convergence", "models": len(probs),
+           "seconds": t, "converged": int(sum(bool(r[3]) for r in res)), "iterations_total": int(sum(its)),
+           "iterations_min": int(min(its)), "iterations_max": int(max(its)),
+           "model_iterations_per_s": sum(its) / t, "tflop": fl / 1e12, "path_tflops_per_s": fl / t / 1e12,
+           "max_residual": float(max(float(r[4]) for r in res))}
+    if not no_cpu:
+        from oracle import hartreefock
+
+        small = np.argsort(Ns)[:6]
+        errs, eerr = [], []
+        for i in small:
+            h, v, rho0, fill, _ = probs[i]
+            ref = hartreefock.scf(h.data, v.data, rho0.data, fill, ks, nthreads=host_threads(), **kw)
+            rho, E = res[i][0], res[i][1]
+            errs.append(max(float(np.max(np.abs(rho[R] - ref["rho"][R]))) for R in ref["rho"]))
+            eerr.append(abs(E - ref["E"]))
+        out["oracle_check_6_smallest"] = {"N": [Ns[i] for i in small], "max_abs_rho_err": max(errs),
+                                          "max_abs_energy_err": max(eerr)}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -565,6 +628,7 @@ def main() -> int:
     ap.add_argument("--no-cfg4", action="store_true", help="skip the 256-model batch iteration leg")
     ap.add_argument("--no-response", action="store_true", help="skip the Kubo / ldos leg")
     ap.add_argument("--no-partial", action="store_true", help="skip the partial-spectrum leg")
+    ap.add_argument("--cfg4-full", action="store_true", help="also run the 256-model batch to convergence (minutes)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after timing, copy this many MiB device-to-device once (HBM counter calibration)")
     args = ap.parse_args()
@@ -752,6 +816,7 @@ def main() -> int:
     cfg4 = None if args.no_cfg4 else cfg4_leg()
     resp = None if args.no_response else response_leg(args.no_cpu)
     part = None if args.no_partial else partial_leg(args.no_cpu)
+    cfg4f = cfg4_full_leg(args.no_cpu) if args.cfg4_full else None
 
     line = {
         "metric": METRIC,
@@ -788,6 +853,7 @@ def main() -> int:
         "cfg4_batch_iteration": cfg4,
         "response": resp,
         "partial_spectrum": part,
+        "cfg4_to_convergence": cfg4f,
         **out,
     }
     print(json.dumps(line))

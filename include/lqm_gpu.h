@@ -88,6 +88,11 @@ lqm_status lqm_context_set_timing(lqm_context *ctx, int enable);
  * error (iterations after that leave the state unchanged); the results are bitwise those of
  * the host-driven loop.  0 keeps the host-driven loop everywhere; timing mode does too. */
 lqm_status lqm_context_set_graphs(lqm_context *ctx, int enable);
+/* Progress of lqm_hartreefock_scf_batch (the reference's SCF progress bar, no numerical
+ * counterpart): fn(problem index, iterations (-1 on error), converged, user) is called once per
+ * problem as it finishes, from the batch's worker threads, one call at a time.  NULL: none. */
+typedef void (*lqm_progress_fn)(int problem, int iterations, int converged, void *user);
+lqm_status lqm_context_set_progress(lqm_context *ctx, lqm_progress_fn fn, void *user);
 lqm_status lqm_context_stage_times(lqm_context *ctx, double *ms, int64_t *launches,
                                    int64_t *kpoints, int n);
 const char *lqm_stage_name(int stage);

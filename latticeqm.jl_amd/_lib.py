@@ -38,6 +38,7 @@ _PROTOS = {
     "lqm_context_synchronize": [P],
     "lqm_context_set_timing": [P, I],
     "lqm_context_set_graphs": [P, I],
+    "lqm_context_set_progress": [P, P, P],
     "lqm_context_stage_times": [P, P, P, P, I],
     "lqm_stage_name": [I],
     "lqm_context_set_variant": [P, I, I],
@@ -147,3 +148,6 @@ def check(status: int, what: str = "") -> None:
 
 def call(name: str, *args) -> None:
     check(getattr(lib(), name)(*args), name)
+
+# fn(problem, iterations, converged, user) of lqm_context_set_progress
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p)

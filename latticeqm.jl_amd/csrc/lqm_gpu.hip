@@ -141,6 +141,8 @@ enum VariantKind {
 
 struct lqm_context {
     std::map<int, int> tuned;  // (kind << 20 | N) -> variant
+    lqm_progress_fn progress = nullptr;  // called as each model of a batch finishes
+    void* progress_user = nullptr;
     int forced[VK_N] = {-1, -1, -1, -1, -1, -1, -1};
     int timing = 0;
     std::vector<TimedLaunch> pending;
@@ -2240,6 +2242,13 @@ lqm_status lqm_context_set_stream(lqm_context* c, void* s) {
     return LQM_OK;
 }
 
+lqm_status lqm_context_set_progress(lqm_context* c, lqm_progress_fn fn, void* user) {
+    if (!c) return fail(LQM_ERR_ARG, "null context");
+    c->progress = fn;
+    c->progress_user = user;
+    return LQM_OK;
+}
+
 lqm_status lqm_context_set_graphs(lqm_context* c, int enable) {
     if (!c) return fail(LQM_ERR_ARG, "null context");
     c->graphs = enable ? 1 : 0;
@@ -3751,6 +3760,7 @@ lqm_status lqm_hartreefock_scf_batch(lqm_context* ctx, int d, int M, lqm_hf_prob
     });
     const int W = std::min(nstreams, M);
     std::atomic<int> next{0};
+    std::mutex prog_mu;
     std::vector<std::string> errs(W);
     std::vector<int> errst(W, LQM_OK);
     auto worker = [&](int w) {
@@ -3774,6 +3784,10 @@ lqm_status lqm_hartreefock_scf_batch(lqm_context* ctx, int d, int M, lqm_hf_prob
             if (p.status && errst[w] == LQM_OK) {
                 errst[w] = p.status;
                 errs[w] = g_err;
+            }
+            if (ctx->progress) {
+                std::lock_guard<std::mutex> lk(prog_mu);
+                ctx->progress(order[i], p.status ? -1 : (int)p.result[7], p.converged, ctx->progress_user);
             }
         }
         lqm_context_destroy(wc);

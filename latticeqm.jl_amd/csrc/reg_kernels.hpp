@@ -28,15 +28,17 @@ namespace lqm {
 // traffic.  After rotations by 1, 2, 4 and 8 every lane of the row holds the row sum
 // (each lane adds in its own rotation order, so lanes may differ in the last bit;
 // row_bcast_sum returns one lane's value to every lane of a wave-uniform row).
+// (Row rotations read a valid lane everywhere, so the DPP move needs no old value: mov_dpp
+// leaves the destination undefined where no lane is read, and no zero-fill move is issued.)
 template <int CTRL>
 __device__ __forceinline__ float dpp_r(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, true));
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_r(double v) {
     const long long x = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)x, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), CTRL, 0xf, 0xf, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 __device__ __forceinline__ float rl(float v, int lane) {
@@ -198,14 +200,14 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[T][T], RegTriSmem<R>& S, 
             const int c = off + T * tj + v;
             wc[v] = (c > j && c < N) ? cmad(p[v], mkc<R>(alpha2, 0), vc[v]) : mkc<R>(0, 0);
         }
+        // straight into the tile: a - v conj(w) - w conj(v), the negations folded into the
+        // multiply-adds' source modifiers
 #pragma unroll
         for (int u = 0; u < T; ++u)
 #pragma unroll
             for (int v = 0; v < T; ++v) {
-                cx<R> d = mkc<R>(0, 0);
-                d = cmadcb(d, vr[u], wc[v]);
-                d = cmadcb(d, wr[u], vc[v]);
-                a[u][v] = a[u][v] - d;
+                a[u][v] = cmadcb(a[u][v], mkc<R>(-vr[u].re, -vr[u].im), wc[v]);
+                a[u][v] = cmadcb(a[u][v], mkc<R>(-wr[u].re, -wr[u].im), vc[v]);
             }
     }
 }

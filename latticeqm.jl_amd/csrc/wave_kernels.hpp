@@ -19,8 +19,9 @@ namespace lqm {
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {
     const long long x = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, 0xf, 0xf, false);
+    // row rotations only: every lane reads a valid lane, so no old value (no zero-fill move)
+    const int lo = __builtin_amdgcn_mov_dpp((int)x, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), CTRL, 0xf, 0xf, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 

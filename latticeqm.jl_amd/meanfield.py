@@ -328,13 +328,14 @@ def magnetization(rho: Hops, A=None) -> np.ndarray:
 
 def solvehartreefock_batch(problems, T: float = 0.0, beta: float = 1.0, tol: float = 1e-7,
                            iterations: int = 500, hartree: bool = True, fock: bool = True,
-                           nstreams: int = 4, executor=None):
+                           nstreams: int = 4, executor=None, progress=None):
     """Many independent Hartree-Fock SCFs of uneven size in one call.
 
     ``problems``: sequence of (h, v, rho_init, filling, ks) tuples (ks: points, Mesh or an
     int klin).  They run concurrently on ``nstreams`` device streams, largest first.
     Returns, per problem, ``(rho, E_GS, info, converged, residual)`` with info holding mu,
-    epsband, epsH, epsF, epskin and the iteration count.
+    epsband, epsH, epsF, epskin and the iteration count.  ``progress(i, iterations,
+    converged)``, if given, is called as problem i finishes (from the batch's threads).
     """
     ex = to_executor(executor or "gpu")
     keep = []
@@ -364,8 +365,16 @@ def solvehartreefock_batch(problems, T: float = 0.0, beta: float = 1.0, tol: flo
         p.Nk, p.ks, p.kweights = k.shape[1], k.ctypes.data, (w.ctypes.data if w is not None else None)
         p.filling = float(filling)
     flags = (_lib.LQM_HF_HARTREE if hartree else 0) | (_lib.LQM_HF_FOCK if fock else 0)
-    _lib.call("lqm_hartreefock_scf_batch", ex.handle, int(d or 2), len(problems), arr, float(T), float(beta),
-              float(tol), int(iterations), flags, int(nstreams))
+    cb = None
+    if progress is not None:
+        cb = _lib.PROGRESS_FN(lambda i, it, conv, _u: progress(int(i), int(it), bool(conv)))
+        _lib.call("lqm_context_set_progress", ex.handle, ctypes.cast(cb, ctypes.c_void_p), None)
+    try:
+        _lib.call("lqm_hartreefock_scf_batch", ex.handle, int(d or 2), len(problems), arr, float(T), float(beta),
+                  float(tol), int(iterations), flags, int(nstreams))
+    finally:
+        if cb is not None:
+            _lib.call("lqm_context_set_progress", ex.handle, None, None)
     out = []
     for i, p in enumerate(arr):
         Rr, Tr = keep[i][4], keep[i][5]
