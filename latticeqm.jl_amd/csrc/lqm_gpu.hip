@@ -3143,13 +3143,17 @@ lqm_status lqm_kubo_dev(lqm_model* h, lqm_model* j1, lqm_model* j2, int Nk, cons
         // cores, then U^H W
         for (int q = 0; q < (same ? 1 : 2); ++q) {
             lqm_model* J = q == 0 ? j1 : j2;
-            const size_t sh = sizeof(cx<double>) * J->nR;
-            auto kH = lqm::k_fouriersum;
-            allow_lds(kH, sh);
+            // J(k) for the whole chunk as one product on the matrix cores: the stack of blocks
+            // [NN x nR] times the phase table [nR x Kc]
+            cx<double>* phj;
+            if ((st = wsT(c, W_SPPH, (size_t)J->nR * Kc, &phj))) return st;
             {
-                StageScope sc(c, ST_HK64, Kc);
-                hipLaunchKernelGGL(kH, dim3(Kc), dim3(256), sh, c->stream, N, J->d, J->nR, J->dR, J->dT, ksc, O);
+                StageScope sc(c, ST_HK64);
+                hipLaunchKernelGGL(lqm::k_sp_phases, dim3((Kc * J->nR + 255) / 256), dim3(256), 0, c->stream, Kc,
+                                   J->d, J->nR, J->dR, ksc, phj);
             }
+            zgemm_gen<0, 0>(c, ST_HK64, Kc, 1,
+                            gemm_args((int)NN, Kc, J->nR, J->dT, (int)NN, 0, phj, J->nR, 0, O, (int)NN, 0, one, zero));
             zgemm<0, 0>(c, N, Kc, O, V, W, one, zero);
             zgemm_gen<1, 0>(c, ST_GEMM_CN, Kc, Kc,
                             gemm_args(N, N, N, V, N, (long long)NN, W, N, (long long)NN, q == 0 ? A : C, N,
