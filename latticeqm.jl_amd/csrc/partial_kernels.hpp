@@ -32,6 +32,7 @@ __global__ __launch_bounds__(256) void k_sp_phases(int Kc, int d, int nR, const 
 // wave at a time, kSpRows rows per block.  Columns c >= m are written as zero.
 constexpr int kSpRows = 32;
 constexpr int kSpMaxKeys = 2048;
+constexpr int kSpUnroll = 4;  // gathers in flight per lane (measured: 1.9x over 1; 8 no better)
 __global__ __launch_bounds__(256) void k_spmm(int N, int mp, int m, const long long* __restrict__ rowptr,
                                               const int2* __restrict__ colkey, const cx<double>* __restrict__ val,
                                               int nR, const cx<double>* __restrict__ ph, double shift,
@@ -63,7 +64,21 @@ __global__ __launch_bounds__(256) void k_spmm(int N, int mp, int m, const long l
         }
         cx<double> acc = {0.0, 0.0};
         const long long e1 = rowptr[i + 1];
-        for (long long e = rowptr[i]; e < e1; ++e) {
+        long long e = rowptr[i];
+        // kSpUnroll entries at a time: their column/key loads, then all their gathers in flight
+        for (; e + kSpUnroll <= e1; e += kSpUnroll) {
+            int2 jk[kSpUnroll];
+            cx<double> t[kSpUnroll], xv[kSpUnroll];
+#pragma unroll
+            for (int q = 0; q < kSpUnroll; ++q) jk[q] = colkey[e + q];
+#pragma unroll
+            for (int q = 0; q < kSpUnroll; ++q) xv[q] = x[(size_t)jk[q].x * mp + c];
+#pragma unroll
+            for (int q = 0; q < kSpUnroll; ++q) t[q] = cmad(cx<double>{0.0, 0.0}, val[e + q], sph[jk[q].y]);
+#pragma unroll
+            for (int q = 0; q < kSpUnroll; ++q) acc = cmad(acc, t[q], xv[q]);
+        }
+        for (; e < e1; ++e) {
             const int2 jk = colkey[e];
             const cx<double> t = cmad(cx<double>{0.0, 0.0}, val[e], sph[jk.y]);
             acc = cmad(acc, t, x[(size_t)jk.x * mp + c]);
