@@ -161,3 +161,37 @@ def test_argument_errors(pkg, sparse, gpu):
         sparse.eigen_sparse(sh, ks, 4, which="XX")
     with pytest.raises(Exception):
         sparse.eigen_sparse(_sparse_random(pkg, sparse, 500, seed=2), ks, 57)
+
+
+def test_degenerate_window_and_edges(pkg, sparse, partial, gpu):
+    # two identical decoupled copies of a random model: every eigenvalue exactly doubly
+    # degenerate, so a window of an even number of bands is well defined; compare the
+    # eigenvalues and the spectral projector of the window (gauge free) with the oracle
+    rng = np.random.default_rng(9)
+    n = 40
+    t0 = sp.random(n, n, density=0.1, random_state=rng) + 1j * sp.random(n, n, density=0.1, random_state=rng)
+    t0 = (t0 + t0.conj().T) / 2
+    t1 = sp.random(n, n, density=0.05, random_state=rng) * (1 + 0.5j)
+    blk = lambda a: sp.block_diag([a, a], format="csr")
+    sh = sparse.SparseHops({(0,): blk(t0), (1,): blk(t1), (-1,): blk(t1).conj().T})
+    ks = np.array([[0.0, 0.23, 0.5]])
+    R, T = _dense_pack(sh)
+    for sigma in (0.05, 100.0, -100.0):
+        b, V = sparse.eigen_sparse(sh, ks, 6, sigma=sigma)
+        ref, Vr = partial.eigen_partial(R, T, ks, 6, sigma, "LM", vectors=True)
+        assert np.max(np.abs(b - ref)) < 1e-10 * max(1.0, np.max(np.abs(ref)))
+        for j in range(ks.shape[1]):
+            P, Pr = V[:, :, j] @ V[:, :, j].conj().T, Vr[:, :, j] @ Vr[:, :, j].conj().T
+            assert np.max(np.abs(P - Pr)) < 1e-9
+    # sigma far above the spectrum selects the largest eigenvalues, far below the smallest
+    bl, _ = sparse.eigen_sparse(sh, ks, 4, sigma=100.0)
+    br, _ = sparse.eigen_sparse(sh, ks, 4, which="LR")
+    assert np.max(np.abs(bl - br)) < 1e-10
+    # the whole spectrum (num_bands = N) and a 1 x 1 model
+    small = _sparse_random(pkg, sparse, 20, seed=5, density=0.2)
+    b, _ = sparse.eigen_sparse(small, np.array([[0.1], [0.3]]), 20)
+    R2, T2 = _dense_pack(small)
+    ref = partial.eigen_partial(R2, T2, np.array([[0.1], [0.3]]), 20, 1e-8, "LM")
+    assert np.max(np.abs(b - ref)) < 1e-10 * max(1.0, np.max(np.abs(ref)))
+    e, U = sparse.eigen_sparse_matrix(sp.csr_matrix([[0.7]]), 1)
+    assert abs(e[0] - 0.7) < 1e-15 and abs(abs(U[0, 0]) - 1.0) < 1e-15

@@ -151,3 +151,35 @@ def test_argument_errors(pkg, resp, gpu):
         resp.opticalconductivity([0.1], h, J, J, pkg.regulargrid(nk=4), executor=gpu)
     with pytest.raises(Exception):
         pkg.api.ldos(h, pkg.regulargrid(nk=4), [], executor=gpu)
+
+
+def test_edge_cases(pkg, resp, orr, gpu):
+    # one orbital: no pairs (chi = 0), ldos a single Lorentzian, density 0 / 1 around the band
+    h = pkg.Hops({(0,): [[0.2]], (1,): [[0.5]], (-1,): [[0.5]]})
+    ks = np.linspace(0.0, 1.0, 8, endpoint=False).reshape(1, -1)
+    chi = resp.opticalconductivity([0.0, 0.5], h, h, h, ks, Gamma=0.1, T=0.05, executor=gpu)
+    assert np.max(np.abs(chi)) == 0.0
+    e = 0.2 + 2 * 0.5 * np.cos(2 * np.pi * ks[0])
+    n = pkg.api.ldos(h, ks, 0.3, Gamma=0.1, executor=gpu)
+    assert abs(n[0] - np.mean(0.1 / ((0.3 - e) ** 2 + 0.01)) / np.pi) < 1e-13
+    assert pkg.api.density(h, ks, -5.0, executor=gpu)[0] == 0.0
+    assert abs(pkg.api.density(h, ks, 5.0, executor=gpu)[0] - 1.0) < 1e-15
+    # T = 0 with mu exactly on an eigenvalue: a diagonal model (eigenvalues -1, 0, 1 exact in
+    # every implementation) gives the step with 1/2 at e = mu (fermidirac) in the Kubo
+    # occupations and e <= mu in the density, as the oracle
+    h2 = pkg.Hops({(0,): np.diag([-1.0, 0.0, 1.0])})
+    J = pkg.Hops({(0,): [[0, 0.3, 0.1j], [0.3, 0, 0.2], [-0.1j, 0.2, 0]]})
+    R, T = h2.pack()
+    ks2 = np.zeros((1, 1))
+    ws = np.array([0.0, 0.4, 1.3])
+    chi = resp.opticalconductivity(ws, h2, J, J, ks2, mu=0.0, Gamma=0.1, T=0.0, executor=gpu)
+    ref = orr.kubo(R, T, *J.pack(), *J.pack(), ks2, ws, 0.0, 0.1, 0.0)
+    assert np.max(np.abs(chi - ref)) < 1e-13 * np.max(np.abs(ref))
+    # closed form with f = (1, 1/2, 0): pairs (0, 1), (1, 2), (0, 2)
+    z = ws + 1j * 0.1
+    Jm = np.array([[0, 0.3, 0.1j], [0.3, 0, 0.2], [-0.1j, 0.2, 0]])
+    e, f = np.array([-1.0, 0.0, 1.0]), np.array([1.0, 0.5, 0.0])
+    cf = sum((f[i] - f[j]) * abs(Jm[i, j]) ** 2 / (e[i] - e[j] - z) for i in range(3) for j in range(3) if i != j)
+    assert np.max(np.abs(chi - (-1j) * cf)) < 1e-13
+    d = pkg.api.density(h2, ks2, 0.0, executor=gpu)
+    assert np.array_equal(d, [1.0, 1.0, 0.0])
