@@ -150,6 +150,12 @@ lqm_status lqm_context_refinement_report(lqm_context *ctx, double *out);
 lqm_status lqm_context_last_precision(lqm_context *ctx, int *precision);
 lqm_status lqm_context_get_variant(lqm_context *ctx, int kind, int N, int *variant);
 lqm_status lqm_autotune(lqm_model *m, int Nk, const double *ks_dev);
+/* The autotune cache: the context's tuned variants (kind, N, variant) written to / merged
+ * from a text file whose first line names the device architecture ("lqm-tuning 1 gfx950");
+ * loading a file tuned on another architecture, or one with an entry out of range, is
+ * LQM_ERR_ARG and changes nothing.  No reference counterpart. */
+lqm_status lqm_context_save_tuning(lqm_context *ctx, const char *path);
+lqm_status lqm_context_load_tuning(lqm_context *ctx, const char *path);
 
 /* ---- model: the Hops operator, resident in HBM --------------------------------- */
 /* Upload of Hops{Vector{Int},Matrix{ComplexF64}} (TightBinding/types.jl:54-60):

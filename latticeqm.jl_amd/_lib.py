@@ -39,6 +39,8 @@ _PROTOS = {
     "lqm_context_set_timing": [P, I],
     "lqm_context_set_graphs": [P, I],
     "lqm_context_set_progress": [P, P, P],
+    "lqm_context_save_tuning": [P, ctypes.c_char_p],
+    "lqm_context_load_tuning": [P, ctypes.c_char_p],
     "lqm_context_stage_times": [P, P, P, P, I],
     "lqm_stage_name": [I],
     "lqm_context_set_variant": [P, I, I],

@@ -78,6 +78,14 @@ class GPUExec:
                 "kpoints_near_degenerate": int(out[3]), "kpoint_sweeps": int(out[4]),
                 "kpoints_rho_fused": int(out[5])}
 
+    def save_tuning(self, path) -> None:
+        """Write the autotuned variants (lqm_context_save_tuning)."""
+        _lib.call("lqm_context_save_tuning", self.handle, str(path).encode())
+
+    def load_tuning(self, path) -> None:
+        """Merge autotuned variants from a file written on this architecture."""
+        _lib.call("lqm_context_load_tuning", self.handle, str(path).encode())
+
     def set_graphs(self, enable: bool) -> None:
         """Replay SCF iterations from captured HIP graphs where eligible (default on)."""
         _lib.call("lqm_context_set_graphs", self.handle, int(bool(enable)))

@@ -2307,12 +2307,61 @@ lqm_status lqm_context_refinement_report(lqm_context* c, double* out) {
     return LQM_OK;
 }
 
+namespace {
+// Largest variant index of a kind (the range lqm_context_set_variant accepts).
+int variant_max(int kind) { return kind == VK_TRIDIAG || kind == VK_REFINE ? 2 : kind == VK_QL ? 3 : 1; }
+std::string device_arch(int device) {
+    hipDeviceProp_t p{};
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) return "unknown";
+    std::string a(p.gcnArchName);
+    return a.substr(0, a.find(':'));  // gfx950, without the feature suffixes
+}
+}  // namespace
+
 lqm_status lqm_context_set_variant(lqm_context* c, int kind, int variant) {
     if (!c) return fail(LQM_ERR_ARG, "null context");
-    if (kind < 0 || kind >= VK_N || variant < -1 ||
-        variant > (kind == VK_TRIDIAG || kind == VK_REFINE ? 2 : kind == VK_QL ? 3 : 1))
-        return fail(LQM_ERR_ARG, "bad variant");
+    if (kind < 0 || kind >= VK_N || variant < -1 || variant > variant_max(kind)) return fail(LQM_ERR_ARG, "bad variant");
     c->forced[kind] = variant;
+    return LQM_OK;
+}
+
+
+lqm_status lqm_context_save_tuning(lqm_context* c, const char* path) {
+    if (!c || !path) return fail(LQM_ERR_ARG, "null argument");
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(LQM_ERR_ARG, std::string("cannot write ") + path);
+    std::fprintf(f, "lqm-tuning 1 %s\n", device_arch(c->device).c_str());
+    for (const auto& kv : c->tuned) std::fprintf(f, "%d %d %d\n", kv.first >> 20, kv.first & 0xFFFFF, kv.second);
+    const bool ok = std::fclose(f) == 0;
+    return ok ? LQM_OK : fail(LQM_ERR_ARG, std::string("cannot write ") + path);
+}
+
+lqm_status lqm_context_load_tuning(lqm_context* c, const char* path) {
+    if (!c || !path) return fail(LQM_ERR_ARG, "null argument");
+    FILE* f = std::fopen(path, "r");
+    if (!f) return fail(LQM_ERR_ARG, std::string("cannot read ") + path);
+    char arch[64] = {0};
+    int version = 0;
+    std::map<int, int> got;
+    lqm_status st = LQM_OK;
+    if (std::fscanf(f, "lqm-tuning %d %63s", &version, arch) != 2 || version != 1) {
+        st = fail(LQM_ERR_ARG, "not a tuning file (version 1)");
+    } else if (device_arch(c->device) != arch) {
+        st = fail(LQM_ERR_ARG, std::string("tuning file is for ") + arch + ", this device is " + device_arch(c->device));
+    } else {
+        int kind, N, v, r;
+        while ((r = std::fscanf(f, "%d %d %d", &kind, &N, &v)) == 3) {
+            if (kind < 0 || kind >= VK_N || N < 1 || N > 0xFFFFF || v < 0 || v > variant_max(kind)) {
+                st = fail(LQM_ERR_ARG, "bad entry in the tuning file");
+                break;
+            }
+            got[(kind << 20) | N] = v;
+        }
+        if (st == LQM_OK && r != EOF) st = fail(LQM_ERR_ARG, "unreadable entry in the tuning file");
+    }
+    std::fclose(f);
+    if (st) return st;
+    for (const auto& kv : got) c->tuned[kv.first] = kv.second;  // merged over what is there
     return LQM_OK;
 }
 

@@ -99,6 +99,34 @@ def test_autotune_picks_valid_variants(pkg, oracle, gpu):
     _rho_vs_oracle(pkg, oracle, h, ks, 0.0, 0.05)
 
 
+def test_autotune_cache_roundtrip(pkg, gpu, tmp_path):
+    import torch
+
+    h = pkg.models.random_tb(24, rmax=1, seed=8)
+    ks = pkg.regulargrid(nk=256)
+    ex = pkg.api.GPUExec(0)  # contexts of their own (to_executor shares one per device)
+    m = pkg.api.GPUModel(h, ex)
+    ks_d = torch.from_numpy(np.ascontiguousarray(ks.T)).cuda()
+    chosen = pkg.api.autotune(m, ks_d.data_ptr(), ks.shape[1])
+    path = tmp_path / "tuning.txt"
+    ex.save_tuning(path)
+    fresh = pkg.api.GPUExec(0)
+    assert all(fresh.get_variant(kind, 24) == -1 for kind in chosen)
+    fresh.load_tuning(path)
+    assert {kind: fresh.get_variant(kind, 24) for kind in chosen} == chosen
+    # another architecture's file, and an out-of-range entry, are refused and change nothing
+    lines = path.read_text().splitlines()
+    other = tmp_path / "other.txt"
+    other.write_text("\n".join(["lqm-tuning 1 gfx000"] + lines[1:]) + "\n")
+    bad = tmp_path / "bad.txt"
+    bad.write_text(lines[0] + "\n0 24 9\n")
+    third = pkg.api.GPUExec(0)
+    for f in (other, bad, tmp_path / "missing.txt"):
+        with pytest.raises(Exception):
+            third.load_tuning(f)
+    assert all(third.get_variant(kind, 24) == -1 for kind in chosen)
+
+
 def test_rho_zeeman_graphene_T0(pkg, oracle, gpu):
     # the test_operators.jl:91-127 configuration
     h = pkg.models.addspin(pkg.models.honeycomb_nn(-1.0), "spinhalf")
