@@ -29,6 +29,7 @@
 #include "tri_kernels.hpp"
 #include "response_kernels.hpp"
 #include "partial_kernels.hpp"
+#include "small_kernels.hpp"
 
 using lqm::cx;
 
@@ -70,7 +71,7 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_LNK, W_LNKO, W_RB, W_ROCC, W_RA, W_RC, W_RP, W_SPB, W_SPPH, W_SPM, W_SPT, W_SPI, W_SPC, W_SPKS, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_LNK, W_LNKO, W_RB, W_ROCC, W_RA, W_RC, W_RP, W_SPB, W_SPPH, W_SPM, W_SPT, W_SPI, W_SPC, W_SPKS, W_SMV, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -819,6 +820,42 @@ struct OccScope {
     ~OccScope() { c->occ = nullptr; }
 };
 
+// Models of at most four orbitals: one thread per k-point (k_small_eig / k_small_rho) in
+// place of the per-k workgroup kernels, unless a variant of those kernels is forced.
+bool use_small(const lqm_context* c, int N) {
+    if (N > 4 || c->precision != LQM_PREC_FP64) return false;
+    for (int k = 0; k < VK_N; ++k)
+        if (c->forced[k] >= 0) return false;
+    return true;
+}
+lqm_status launch_small_eig(lqm_model* m, int Kc, const double* ks, double* ev, cx<double>* V) {
+    lqm_context* c = m->ctx;
+    StageScope sc(c, ST_TRIDIAG, Kc);
+    const dim3 g((Kc + 255) / 256);
+    switch (m->N) {
+        case 1: hipLaunchKernelGGL(lqm::k_small_eig<1>, g, dim3(256), 0, c->stream, Kc, m->d, m->nR, m->dR, m->dT, ks, ev, V, c->d_status); break;
+        case 2: hipLaunchKernelGGL(lqm::k_small_eig<2>, g, dim3(256), 0, c->stream, Kc, m->d, m->nR, m->dR, m->dT, ks, ev, V, c->d_status); break;
+        case 3: hipLaunchKernelGGL(lqm::k_small_eig<3>, g, dim3(256), 0, c->stream, Kc, m->d, m->nR, m->dR, m->dT, ks, ev, V, c->d_status); break;
+        default: hipLaunchKernelGGL(lqm::k_small_eig<4>, g, dim3(256), 0, c->stream, Kc, m->d, m->nR, m->dR, m->dT, ks, ev, V, c->d_status); break;
+    }
+    LQM_HIP(hipGetLastError());
+    return LQM_OK;
+}
+lqm_status launch_small_rho(lqm_context* c, int N, int Kc, const double* ev, const cx<double>* V, const double* kw,
+                            double wuni, double mu, const double* mu_dev, double T, const double* occ,
+                            cx<double>* rhok, double* eb) {
+    StageScope sc(c, ST_BACKTRANSFORM, Kc);
+    const dim3 g((Kc + 255) / 256);
+    switch (N) {
+        case 1: hipLaunchKernelGGL(lqm::k_small_rho<1>, g, dim3(256), 0, c->stream, Kc, ev, V, kw, wuni, mu, mu_dev, T, occ, rhok, eb); break;
+        case 2: hipLaunchKernelGGL(lqm::k_small_rho<2>, g, dim3(256), 0, c->stream, Kc, ev, V, kw, wuni, mu, mu_dev, T, occ, rhok, eb); break;
+        case 3: hipLaunchKernelGGL(lqm::k_small_rho<3>, g, dim3(256), 0, c->stream, Kc, ev, V, kw, wuni, mu, mu_dev, T, occ, rhok, eb); break;
+        default: hipLaunchKernelGGL(lqm::k_small_rho<4>, g, dim3(256), 0, c->stream, Kc, ev, V, kw, wuni, mu, mu_dev, T, occ, rhok, eb); break;
+    }
+    LQM_HIP(hipGetLastError());
+    return LQM_OK;
+}
+
 lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, double* evecs) {
     const int N = m->N;
     if (Nk < 0) return fail(LQM_ERR_ARG, "Nk < 0");
@@ -833,6 +870,11 @@ lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, doub
     if (m->ctx->precision == LQM_PREC_FP32_REFINED && !evecs)
         return run_mixed(m, Nk, ks, nullptr, 0.0, 0.0, 0, nullptr, nullptr, nullptr, bands);
     begin_pass(m->ctx, LQM_PREC_FP64);
+    if (use_small(m->ctx, N)) {
+        lqm_status st = launch_small_eig(m, Nk, ks, bands, reinterpret_cast<cx<double>*>(evecs));
+        if (st) return st;
+        return check_status(m->ctx, "bandmatrix");
+    }
     Plan P = make_plan(m->ctx, N, m->nR, Nk, evecs != nullptr, false);
     for (int k0 = 0; k0 < Nk; k0 += P.Kc) {
         const int Kc = std::min(P.Kc, Nk - k0);
@@ -907,7 +949,14 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
         } else if ((st = wsT(c, W_EVALS, (size_t)N * P.Kc, &ev))) {
             return st;
         }
-        if ((st = launch_eigvals(m, P, Kc, ksc, ev, true))) return st;
+        const bool small = use_small(c, N);
+        cx<double>* Vs = nullptr;
+        if (small) {
+            if ((st = wsT(c, W_SMV, NN * P.Kc, &Vs))) return st;
+            if ((st = launch_small_eig(m, Kc, ksc, ev, Vs))) return st;
+        } else if ((st = launch_eigvals(m, P, Kc, ksc, ev, true))) {
+            return st;
+        }
         if (mq && mq->dres && T > 0.0) {
             double *focc, *ework = nullptr;
             if ((st = wsT(c, W_FOCC, (size_t)N * Kc, &focc))) return st;
@@ -931,7 +980,9 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
         double* eb;
         if ((st = wsT(c, W_RHOK, NN * P.Kc, &rhok))) return st;
         if ((st = wsT(c, W_EB, (size_t)P.Kc, &eb))) return st;
-        if (P.rho2) {
+        if (small) {
+            if ((st = launch_small_rho(c, N, Kc, ev, Vs, kwc, wuni, mu, nullptr, T, c->occ, rhok, eb))) return st;
+        } else if (P.rho2) {
             if ((st = launch_rho2(m, P, Kc, ev, kwc, wuni, mu, T, rhok, eb))) return st;
         } else if ((st = launch_backtransform<double, 1>(m, P, Kc, ev, kwc, wuni, mu, T, rhok, eb))) {
             return st;

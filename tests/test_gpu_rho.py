@@ -218,3 +218,36 @@ def test_setfilling_graphene(pkg, gpu):
     h = pkg.models.honeycomb_nn(-1.0)
     mu = pkg.api.setfilling(h, 0.5, nk=100)
     assert abs(mu) < 1e-12
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+def test_small_models_one_thread_per_k(pkg, oracle, N):
+    # N <= 4 runs on one thread per k-point (Jacobi in registers); the same passes with a forced
+    # variant take the per-k workgroup kernels: both against the oracle
+    if N == 4:
+        h = pkg.models.addspin(pkg.models.random_tb(2, rmax=1, seed=31), "spinhalf")  # exact pairs
+    else:
+        h = pkg.models.random_tb(N, rmax=1, seed=30 + N)
+    ks = pkg.regulargrid(nk=100)
+    R, T = h.pack()
+    ex = pkg.api.GPUExec(0)
+    forced = pkg.api.GPUExec(0)
+    forced.set_variant(0, 0)
+    bref = oracle.bands(R, T, ks)
+    for e in (ex, forced):
+        b, _ = pkg.api.bandmatrix(h, ks, executor=e)
+        assert np.max(np.abs(b - bref)) < 1e-12 * max(1.0, np.max(np.abs(bref)))
+        _, V = pkg.api.eigen(h, ks[:, :7], executor=e)
+        for j in range(7):
+            Hk = h(ks[:, j])
+            Hk = np.triu(Hk) + np.triu(Hk, 1).conj().T
+            assert np.max(np.abs(Hk @ V[:, :, j] - V[:, :, j] * b[:, j])) < 1e-12 * max(1.0, np.max(np.abs(bref)))
+            assert np.max(np.abs(V[:, :, j].conj().T @ V[:, :, j] - np.eye(h.N))) < 1e-13
+        w = np.linspace(0.5, 1.5, ks.shape[1]) / ks.shape[1]
+        for mu, Tk, kw in ((0.1, 0.05, None), (0.0, 0.0, None), (-0.2, 0.02, w)):
+            rho = pkg.Hops({key: np.zeros((h.N, h.N), complex) for key in h.keys()})
+            eb = pkg.api.getdensitymatrix(rho, h, ks, mu, T=Tk, kweights=kw, executor=e)
+            ref, ebr = oracle.densitymatrix(R, T, ks, R, mu=mu, Temp=Tk, kweights=kw)
+            for r, key in enumerate(h.keys()):
+                assert np.max(np.abs(rho[key] - ref[:, :, r])) < 1e-12, (mu, Tk, key)
+            assert abs(eb - ebr) < 1e-12 * max(1.0, abs(ebr))
