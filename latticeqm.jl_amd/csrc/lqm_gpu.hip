@@ -965,12 +965,14 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
                 StageScope sc(c, ST_MU);
                 hipLaunchKernelGGL(lqm::k_mu_bisect, dim3(1), dim3(1024), 0, c->stream, N, N * Nk, ev, kw,
                                    mq->filling, T, mq->dres, ework);
+                // the one-thread-per-k-point path forms fermidirac(e - mu) itself from the device mu
                 const int L = N * Kc;
-                hipLaunchKernelGGL(lqm::k_fermi_occ, dim3((L + 255) / 256), dim3(256), 0, c->stream, L, ev,
-                                   mq->dres, T, focc);
+                if (!small)
+                    hipLaunchKernelGGL(lqm::k_fermi_occ, dim3((L + 255) / 256), dim3(256), 0, c->stream, L, ev,
+                                       mq->dres, T, focc);
             }
             LQM_HIP(hipGetLastError());
-            c->occ = focc;
+            if (!small) c->occ = focc;
             mq->on_device = true;
         } else if (mq) {
             if ((st = mu_from_bands(c, N, Nk, ev, kw, mq->filling, T, &mu))) return st;
@@ -981,7 +983,8 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
         if ((st = wsT(c, W_RHOK, NN * P.Kc, &rhok))) return st;
         if ((st = wsT(c, W_EB, (size_t)P.Kc, &eb))) return st;
         if (small) {
-            if ((st = launch_small_rho(c, N, Kc, ev, Vs, kwc, wuni, mu, nullptr, T, c->occ, rhok, eb))) return st;
+            const double* mud = mq && mq->on_device ? mq->dres : nullptr;
+            if ((st = launch_small_rho(c, N, Kc, ev, Vs, kwc, wuni, mu, mud, T, c->occ, rhok, eb))) return st;
         } else if (P.rho2) {
             if ((st = launch_rho2(m, P, Kc, ev, kwc, wuni, mu, T, rhok, eb))) return st;
         } else if ((st = launch_backtransform<double, 1>(m, P, Kc, ev, kwc, wuni, mu, T, rhok, eb))) {
