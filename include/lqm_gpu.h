@@ -99,8 +99,9 @@ const char *lqm_stage_name(int stage);
 
 /* Kernel variants (no reference counterpart; the reference's only tunable is
  * the executor).  kind 0: tridiagonalization (0 = 256-thread full tile, 1 = one
- * wave with the packed Hermitian tile, the default for N <= 8, 2 = register tile on a
- * materialised H(k), N <= 64, the default for 16 < N <= 64);
+ * wave with the packed Hermitian tile, 2 = register tile on a materialised H(k), N <= 64,
+ * 3 = column per lane on a materialised H(k), N <= 64, the default there: one wave up to
+ * 32 orbitals, two waves above, 4 = the same with FP32 on one wave of 64 rows);
  * kind 1: rho_k (0 = V = QZ then V f V^H,
  * 1 = two-sided reflectors on Z f Z^T, the default for N <= 8); kind 2: Ogita-Aishima sweep of the mixed path
  * (0 = batched GEMM chain, 1 = one fused kernel per k-point with X and W in LDS, 2 = the

@@ -100,6 +100,20 @@ __device__ __forceinline__ double blk_sum(double v, double* red) {
     return s;
 }
 
+template <int W>
+__device__ __forceinline__ double blk_max(double v, double* red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) s = fmax(s, red[w]);
+    return s;
+}
+
 // ---------------------------------------------------------------------------------
 // Column step j = p + i of a panel (zlatrd, lower).  Column j receives the panel's
 // pending update A(j:N, j) -= V(j:N, 0:i) conj(W(j, 0:i))^T + W(j:N, 0:i) conj(V(j, 0:i))^T,
@@ -172,7 +186,42 @@ __global__ __launch_bounds__(256) void k_trd_col(int N, int j, int i, int last, 
         beta = are >= 0.0 ? -nrm : nrm;
         tj = {(R)((beta - are) / beta), (R)(-aim / beta)};
         const double dre = are - beta, den = dre * dre + aim * aim;
-        scale = {(R)(dre / den), (R)(-aim / den)};
+        if (den >= num<double>::rot_lo) {
+            scale = {(R)(dre / den), (R)(-aim / den)};
+        } else {
+            // a column of tiny entries (block-uniform branch): the scalars of x / max|x|, then
+            // beta and scale back
+            double mx = 0.0;
+#pragma unroll
+            for (int q = 0; q < kBRows; ++q) {
+                const int r = j + tid + 256 * q;
+                if (r > j && r < N) mx = fmax(mx, fmax(fabs((double)a[q].re), fabs((double)a[q].im)));
+            }
+            mx = blk_max<4>(mx, red);
+            const double inv = 1.0 / mx;
+            double xs = 0.0;
+#pragma unroll
+            for (int q = 0; q < kBRows; ++q) {
+                const int r = j + tid + 256 * q;
+                if (r >= j + 2 && r < N) {
+                    const double xr = inv * a[q].re, xi = inv * a[q].im;
+                    xs += xr * xr + xi * xi;
+                }
+            }
+            xs = blk_sum<4>(xs, red);
+            cx<double> ts, ss;
+            double bs;
+            hh_scalars_scaled(cx<double>{inv * are, inv * aim}, xs, ts, ss, bs);
+            if (inv <= num<double>::huge) {
+                beta = bs * mx;
+                tj = {(R)ts.re, (R)ts.im};
+                scale = {(R)(inv * ss.re), (R)(inv * ss.im)};
+            } else {
+                // entries below 1 / huge (subnormal): left in place, H_j = I
+                beta = are;
+                tj = czero;
+            }
+        }
     }
     const R rbeta = (R)beta;
 #pragma unroll

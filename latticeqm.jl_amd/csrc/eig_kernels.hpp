@@ -105,13 +105,26 @@ __global__ __launch_bounds__(kTPB) void k_hk_tridiag(
         for (int i = j + 2 + lane; i < N; i += kWave) xn2 += abs2(At(i, j));
         xn2 = wave_sum(xn2);
         cx<R> alpha = At(j + 1, j);
-        cx<R> tj = {0, 0}, scale = {0, 0};
-        R beta = alpha.re;
-        if (!(xn2 == (R)0 && alpha.im == (R)0)) {
-            R nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
-            beta = alpha.re >= (R)0 ? -nrm : nrm;
-            tj = mkc<R>((beta - alpha.re) / beta, -alpha.im / beta);
-            scale = crecip(mkc<R>(alpha.re - beta, alpha.im));
+        cx<R> tj, scale;
+        R beta;
+        if (!hh_scalars(alpha, xn2, tj, scale, beta)) {
+            // a column of tiny entries: the scalars of x / max|x|, then beta and scale back
+            R mx = 0;
+            for (int i = j + 1 + lane; i < N; i += kWave) mx = fmax(mx, fmax(fabs(At(i, j).re), fabs(At(i, j).im)));
+#pragma unroll
+            for (int o = kWave / 2; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+            const R inv = (R)1 / mx;
+            R xs2 = 0;
+            for (int i = j + 2 + lane; i < N; i += kWave) xs2 += abs2(inv * At(i, j));
+            xs2 = wave_sum(xs2);
+            hh_scalars_scaled(inv * alpha, xs2, tj, scale, beta);
+            beta *= mx;
+            scale = inv * scale;
+            if (!(inv <= num<R>::huge)) {
+                tj = mkc<R>(0, 0);
+                scale = mkc<R>(0, 0);
+                beta = alpha.re;
+            }
         }
         const bool triv = (tj.re == (R)0 && tj.im == (R)0);
         for (int ip = lane; ip < m; ip += kWave) {

@@ -108,13 +108,52 @@ template <typename R> struct num;
 template <> struct num<double> {
     static constexpr double half_eps = 1.1102230246251565e-16;  // 2^-53
     static constexpr double tiny = 2.2250738585072014e-308;     // smallest normal
+    static constexpr double huge = 1.7976931348623157e308;      // largest finite
     static constexpr double rot_lo = 1e-290, rot_hi = 1e290;
 };
 template <> struct num<float> {
     static constexpr float half_eps = 5.9604645e-08f;  // 2^-24
     static constexpr float tiny = 1.17549435e-38f;     // smallest normal
+    static constexpr float huge = 3.40282347e+38f;     // largest finite
     static constexpr float rot_lo = 1e-30f, rot_hi = 1e30f;
 };
+
+// Householder scalars (zlarfg: H^H x = beta e_1, H = I - tau v v^H, v = [1, x(1:) scale])
+// from alpha = x(0) and xn2 = ||x(1:)||^2: beta = -sign(Re alpha) ||x||, tau = (beta -
+// alpha) / beta, scale = 1 / (alpha - beta); tau = 0 when x(1:) = 0 and alpha is real.
+// Returns false, leaving the outputs undefined, when |alpha - beta|^2 falls below the normal
+// range of R (a column of tiny entries): the caller then rescales x by its largest entry,
+// calls hh_scalars_scaled on the rescaled column and scales beta and scale back.
+template <typename R>
+__device__ __forceinline__ bool hh_scalars(cx<R> alpha, R xn2, cx<R>& tau, cx<R>& scale, R& beta) {
+    tau = mkc<R>(0, 0);
+    scale = mkc<R>(0, 0);
+    beta = alpha.re;
+    if (xn2 == (R)0 && alpha.im == (R)0) return true;
+    const R nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+    beta = alpha.re >= (R)0 ? -nrm : nrm;
+    const R zr = alpha.re - beta, den = zr * zr + alpha.im * alpha.im;
+    if (!(den >= num<R>::rot_lo)) return false;
+    // |alpha - beta|^2 lies in [beta^2, 4 beta^2]: reciprocals from the hardware approximation
+    const R rb = rcp_nr(beta), rz = rcp_nr(den);
+    tau = mkc<R>((beta - alpha.re) * rb, -alpha.im * rb);
+    scale = mkc<R>(zr * rz, -alpha.im * rz);
+    return true;
+}
+// The same for a column rescaled so that its largest component has magnitude 1 (IEEE
+// divisions; |beta| >= 1 there).
+template <typename R>
+__device__ __forceinline__ void hh_scalars_scaled(cx<R> alpha, R xn2, cx<R>& tau, cx<R>& scale, R& beta) {
+    tau = mkc<R>(0, 0);
+    scale = mkc<R>(0, 0);
+    beta = alpha.re;
+    if (xn2 == (R)0 && alpha.im == (R)0) return;
+    const R nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+    beta = alpha.re >= (R)0 ? -nrm : nrm;
+    const R zr = alpha.re - beta, den = zr * zr + alpha.im * alpha.im;
+    tau = mkc<R>((beta - alpha.re) / beta, -alpha.im / beta);
+    scale = mkc<R>(zr / den, -alpha.im / den);
+}
 
 // Device-side status bits, OR-ed into a per-call flag word.
 enum : int {

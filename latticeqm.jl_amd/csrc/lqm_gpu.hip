@@ -24,6 +24,7 @@
 #include "refine_kernels.hpp"
 #include "hf_kernels.hpp"
 #include "reg_kernels.hpp"
+#include "col_kernels.hpp"
 #include "wy_kernels.hpp"
 #include "blocked_kernels.hpp"
 #include "tri_kernels.hpp"
@@ -248,7 +249,9 @@ struct Plan {
     int rch = 0;  // rotation-staging capacity of K_C
     int nm = 0;   // register row length of k_replay (0: replay inside K_C)
     int nqA = 0;  // rows per lane of the one-wave tridiagonalization (0: 256-thread kernel)
-    bool regA = false;  // register-tile tridiagonalization on a materialised H(k) (N <= 64)
+    bool regA = false;  // tridiagonalization on a materialised H(k) (N <= 64)
+    int colA = 0;       // ... 0: register tiles, 1: column per lane (variant 3), 2: the same with
+                        // FP32 on one wave of 64 rows above 32 orbitals (variant 4)
     bool wyC = false;   // compact-WY back-transformation possible (N <= 64, Z from the replay)
     int btv = 1;        // back-transformation variant: 1 compact WY, 0 reflector loop
     size_t shAw = 0;
@@ -287,14 +290,14 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     p.shAw = cC * (np + 2 * N) + sizeof(cx<double>) * nR;
     // forced variant, else the tuned one, else the default (1 = compact WY for the
     // back-transformation when N > 16, 0 otherwise: the WY kernel works on a 64 x 64 tile;
-    // the register-tile tridiagonalization for 16 < N <= 64; for N <= 8 the one-wave
-    // tridiagonalization and the two-sided rho_k, whose tiles fit such matrices -- measured
-    // fastest there: 18.5 vs 21.1 ms for the 105-iteration SCF at 4 orbitals / 3600 k)
+    // the column-per-lane tridiagonalization for N <= 64 -- measured fastest at every N from
+    // 8 to 64 in both precisions; for N <= 8 the two-sided rho_k, whose tiles fit such
+    // matrices -- 18.5 vs 21.1 ms for the 105-iteration SCF at 4 orbitals / 3600 k)
     auto variant = [&](int kind) {
         if (c->forced[kind] >= 0) return c->forced[kind];
         auto it = c->tuned.find((kind << 20) | N);
         if (it != c->tuned.end()) return it->second;
-        if (kind == VK_TRIDIAG) return N > 16 && N <= 64 ? 2 : (N <= 8 ? 1 : 0);
+        if (kind == VK_TRIDIAG) return N <= 64 ? 3 : 0;
         if (kind == VK_RHO) return N <= 8 ? 1 : 0;
         return (kind == VK_BT && N > 16) || kind == VK_TRIEIG ? 1 : 0;
     };
@@ -308,11 +311,14 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
         p.shB = 2 * cR * (size_t)N * L;
     }
     // tridiagonalization: 0 = 256-thread LDS-tile kernel, 1 = one wave, packed tile,
-    // 2 = register tile on a materialised H(k) (N <= 64)
+    // 2 = register tile on a materialised H(k) (N <= 64), 3 / 4 = column per lane on a
+    // materialised H(k) (N <= 64; 3: one wave up to 32 orbitals, two above, 4: FP32 on one
+    // wave of 64 rows above 32 orbitals)
     const int vt = variant(VK_TRIDIAG);
     if (vt == 1 && N <= 128 && p.shAw <= (size_t)c->lds_max) p.nqA = N <= 64 ? 1 : 2;
-    if (vt == 2 && N <= 64) {
+    if (vt >= 2 && N <= 64) {
         p.regA = true;
+        p.colA = vt - 2;
         perk += 16 * NN;
     }
     // back-transformation: 0 = reflector by reflector (LDS), 1 = compact WY blocks of 16 on
@@ -408,6 +414,34 @@ lqm_status launch_hk_full(lqm_context* c, lqm_model* m, int Kc, const double* ks
     return LQM_OK;
 }
 
+// Column-per-lane tridiagonalization of N <= 64: one wave of 16 or 32 rows, two waves of 32
+// rows above 32 orbitals (variant 4 with FP32: one wave of 64 rows).  Row chunks of 4.  FP32
+// reads the row vectors from LDS (its multiply-adds already bound it); FP64 takes them by DPP
+// broadcast inside the multiply-adds (LDS bound otherwise).  Register budgets for 4 (FP32) /
+// 3 (FP64) waves per SIMD.
+template <typename R>
+lqm_status launch_tridiag_col(lqm_context* c, int variant, int N, int Kc, const cx<double>* H, R* dd, R* ee,
+                              cx<R>* refl, cx<R>* tau) {
+    constexpr bool f64 = sizeof(R) == 8;
+    constexpr int BC = f64 ? 3 : 0, WPE = f64 ? 3 : 4;
+    auto kA = lqm::k_tridiag_c<R, 2, 32, 4, BC, WPE>;
+    int threads = 128;
+    if (N <= 16) {
+        kA = lqm::k_tridiag_c<R, 1, 16, 4, BC, 4>;
+        threads = 64;
+    } else if (N <= 32) {
+        kA = lqm::k_tridiag_c<R, 1, 32, 4, BC, WPE>;
+        threads = 64;
+    } else if constexpr (!f64) {
+        if (variant == 4) {
+            kA = lqm::k_tridiag_c<R, 1, 64, 4, 0, 2>;
+            threads = 64;
+        }
+    }
+    hipLaunchKernelGGL(kA, dim3(Kc), dim3(threads), 0, c->stream, N, Kc, H, dd, ee, refl, tau);
+    return LQM_OK;
+}
+
 // Hpre: H(k) of the chunk already materialised in FP64 (full Hermitian), or null.
 template <typename R>
 lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks, R* evals,
@@ -446,7 +480,9 @@ lqm_status launch_eigvals(lqm_model* m, const Plan& P, int Kc, const double* ks,
     }
     {
         StageScope sc(c, ST_TRIDIAG, Kc);
-        if (P.regA) {
+        if (P.regA && P.colA) {
+            launch_tridiag_col<R>(c, P.colA + 2, N, Kc, Hpre, dd, ee, refl, tau);
+        } else if (P.regA) {
             auto kA = lqm::k_tridiag_r<R>;
             hipLaunchKernelGGL(kA, dim3(Kc), dim3(256), 0, c->stream, N, Kc, Hpre, dd, ee, refl, tau);
         } else if (P.nqA == 1) {
@@ -2363,7 +2399,7 @@ lqm_status lqm_context_refinement_report(lqm_context* c, double* out) {
 
 namespace {
 // Largest variant index of a kind (the range lqm_context_set_variant accepts).
-int variant_max(int kind) { return kind == VK_TRIDIAG || kind == VK_REFINE ? 2 : kind == VK_QL ? 3 : 1; }
+int variant_max(int kind) { return kind == VK_TRIDIAG ? 4 : kind == VK_REFINE ? 2 : kind == VK_QL ? 3 : 1; }
 std::string device_arch(int device) {
     hipDeviceProp_t p{};
     if (hipGetDeviceProperties(&p, device) != hipSuccess) return "unknown";
@@ -2627,7 +2663,7 @@ lqm_status lqm_autotune(lqm_model* m, int Nk, const double* ks_dev) {
         if (kind == VK_REFINE) continue;  // a mixed-path choice with a fixed default
         float best = 1e30f;
         int bestv = 0;
-        const int vmax = kind == VK_TRIDIAG ? (N <= 64 ? 2 : 1) : kind == VK_QL ? 3 : 1;
+        const int vmax = kind == VK_TRIDIAG ? (N <= 64 ? 4 : 1) : kind == VK_QL ? 3 : 1;
         if (kind == VK_BT && N > 64) continue;
         if (kind == VK_BLOCKED && (N <= 64 || N > 512)) continue;  // per-k kernels / large-N path only
         if (kind == VK_TRIEIG && (N <= 16 || N > 64)) continue;    // feeds the compact-WY kernel only

@@ -306,6 +306,11 @@ __device__ void oa_rayleigh_ritz(int N, int i, int c, int ncl, const cx<double>*
     for (int q = 0; q < c * kClusterMax; ++q) U[q] = BU[q];
 }
 
+// Near-degenerate pairs (gap <= kNearGap max|lambda|) go through the general correction; there a
+// pair whose first-order coupling exceeds kCoupRR is rotated by Rayleigh-Ritz with its cluster.
+constexpr double kNearGap = 1e-5;
+constexpr double kCoupRR = 1e-5;
+
 // First-order coupling of columns p and q: max(|E_pq|, |E_qp|) with E_pq = (s_pq - l_q g_pq) /
 // (l_q - l_p) (off-diagonal r = -g); infinity when the Rayleigh quotients coincide.
 __device__ __forceinline__ double oa_coupling(int N, int p, int q, const double* lam, const cx<double>* Sk,
@@ -320,7 +325,8 @@ __device__ __forceinline__ double oa_coupling(int N, int p, int q, const double*
 }
 
 // Ogita-Aishima correction with Rayleigh-Ritz on near-degenerate clusters.
-// In: S = X^H A X, G = X^H X.  Out: E[kk] (X <- X + X E), lambda[kk*N + i], and
+// In: S = X^H A X, G = X^H X (both replaced by their Hermitian parts).  Out: E[kk]
+// (X <- X + X E), lambda[kk*N + i], and
 // stat[kk*4 + {0: ||R||_F, 1: delta, 2: flags, 3: max |E_ij| outside the clusters}] where
 // flags bit 0 = some cluster was rotated, bit 1 = a cluster wider than kClusterMax
 // that is not degenerate to working precision (left unresolved), bit 2 = another sweep
@@ -331,8 +337,8 @@ __device__ __forceinline__ double oa_coupling(int N, int p, int q, const double*
 // (columns arrive in ascending FP32 eigenvalue order).  For a cluster C the
 // orthonormalised block K = (I + R/2)^H S_CC (I + R/2) is diagonalised, K = U L U^H,
 // E_CC = (I + R_CC/2) U - I, lambda_C = L, and E_iC (i outside C) = E^std_iC U.
-__global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __restrict__ S,
-                                                    const cx<double>* __restrict__ G,
+__global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restrict__ S,
+                                                    cx<double>* __restrict__ G,
                                                     cx<double>* __restrict__ E,
                                                     double* __restrict__ lambda,
                                                     double* __restrict__ stat, double thr,
@@ -362,9 +368,30 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
     __shared__ int s_ncl, s_flags, s_nwide;
     const int tid = threadIdx.x;
     const size_t NN = (size_t)N * N;
-    const cx<double>* Sk = S + (size_t)kk * NN;
-    const cx<double>* Gk = G + (size_t)kk * NN;
+    cx<double>* Sk = S + (size_t)kk * NN;
+    cx<double>* Gk = G + (size_t)kk * NN;
     cx<double>* Ek = E + (size_t)kk * NN;
+    // S and G <- their Hermitian parts, in place (each pair i < j by one thread): the
+    // first-order corrections then satisfy E_ij + conj(E_ji) = r_ij exactly, whatever the
+    // rounding of X^H A X and X^H X; otherwise a near-degenerate pair turns that rounding
+    // (about eps ||A||) divided by its gap into a loss of orthogonality
+    for (int e = tid; e < N * N; e += 256) {
+        const int i = e % N, j = e / N;
+        if (i < j) {
+            cx<double>* M[2] = {Sk, Gk};
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const cx<double> a = M[q][(size_t)j * N + i], b = M[q][(size_t)i * N + j];
+                const cx<double> h = {0.5 * (a.re + b.re), 0.5 * (a.im - b.im)};
+                M[q][(size_t)j * N + i] = h;
+                M[q][(size_t)i * N + j] = conj(h);
+            }
+        } else if (i == j) {
+            Sk[e].im = 0.0;
+            Gk[e].im = 0.0;
+        }
+    }
+    __syncthreads();
     for (int i = tid; i < N; i += 256) {
         lam[i] = Sk[(size_t)i * N + i].re / Gk[(size_t)i * N + i].re;
         lnew[i] = lam[i];
@@ -409,7 +436,13 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, const cx<double>* __r
         int i = 0;
         while (i < N) {
             int j = i;
-            while (j + 1 < N && fabs(lam[j + 1] - lam[j]) <= delta) ++j;
+            // a near-degenerate pair (gap <= kNearGap ||A||) whose first-order coupling is
+            // above kCoupRR also joins: its first-order step would leave an error of the
+            // coupling's square, which the next sweep then has to undo
+            while (j + 1 < N && (fabs(lam[j + 1] - lam[j]) <= delta ||
+                                 (fabs(lam[j + 1] - lam[j]) <= kNearGap * an &&
+                                  oa_coupling(N, j, j + 1, lam, Sk, Gk) > kCoupRR)))
+                ++j;
             const int c = j - i + 1;
             if (c >= 2) {
                 double spread = 0.0;
@@ -729,12 +762,17 @@ __device__ __forceinline__ void load_x_tile(const XT* __restrict__ X, int N, cx<
     }
 }
 
-// A cluster among the sorted Rayleigh quotients lam[0..N): some consecutive gap <= delta.
-// Wave 0 tests the N - 1 gaps at once and thread 0 records the answer.
-__device__ __forceinline__ void cluster_flag(const double* lam, int N, double delta, int wave, int lane,
-                                             int* flag) {
+// A cluster among the sorted Rayleigh quotients lam[0..N): some consecutive gap <= delta, or a
+// near-degenerate pair (gap <= kNearGap max|lam|) that the first-order correction would
+// take.  The general correction handles both: it takes the Hermitian part of S, which the
+// fused sweeps do not form (a first-order E_ij divides the rounding of S, about eps ||A||,
+// by the gap: at gaps below 1e-5 ||A|| the loss of orthogonality would exceed 1e-11 and
+// stall the iteration).  Wave 0 tests the N - 1 gaps at once and thread 0 records the answer.
+__device__ __forceinline__ void cluster_flag(const double* lam, int N, double delta, double an, int wave,
+                                             int lane, int* flag) {
     if (wave == 0) {
-        const bool close = lane + 1 < N && fabs(lam[min(lane + 1, 63)] - lam[lane]) <= delta;
+        const double lim = fmax(delta, kNearGap * an);
+        const bool close = lane + 1 < N && fabs(lam[min(lane + 1, 63)] - lam[lane]) <= lim;
         const unsigned long long m = __ballot(close);
         if (lane == 0) *flag = m != 0ull;
     }
@@ -906,7 +944,7 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
     const double rn = sqrt(red[0] + red[1] + red[2] + red[3]);
     const double an = fmax(fmax(red[8], red[9]), fmax(red[10], red[11]));
     const double delta = 2.0 * (sqrt(red[4] + red[5] + red[6] + red[7]) + an * rn);
-    cluster_flag(lam, N, delta, wave, lane, &s_cluster);
+    cluster_flag(lam, N, delta, an, wave, lane, &s_cluster);
     __syncthreads();
     if (s_cluster) {
         // hand S and G to the general correction (Rayleigh-Ritz on the clusters)
@@ -1268,7 +1306,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const double rn = sqrt(red[0] + red[1] + red[2] + red[3]);
     const double an = fmax(fmax(red[8], red[9]), fmax(red[10], red[11]));
     const double delta = 2.0 * (sqrt(red[4] + red[5] + red[6] + red[7]) + an * rn);
-    cluster_flag(lam, N, delta, wave, lane, &s_cluster);
+    cluster_flag(lam, N, delta, an, wave, lane, &s_cluster);
     __syncthreads();
     if (s_cluster) {
         cx<double>* Sk = Sg + kk * NN;

@@ -104,24 +104,38 @@ __device__ __forceinline__ void regtri_step(cx<R> (&a)[T][T], RegTriSmem<R>& S, 
         constexpr int UA = (JV + 1) % T;  // row j+1 is row UA of tile row (j+1-off)/T
         const int src = base + (j + 1 - off) / T;
         const cx<R> alpha = mkc<R>(rl(a[UA][JV].re, src), rl(a[UA][JV].im, src));
-        cx<R> tj_ = mkc<R>(0, 0), scale = mkc<R>(0, 0);
-        R beta = alpha.re;
-        if (!(xn2 == (R)0 && alpha.im == (R)0)) {
-            const R nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
-            beta = alpha.re >= (R)0 ? -nrm : nrm;
-            // reciprocals from the hardware approximation while |alpha - beta|^2 (<= 4 beta^2,
-            // >= beta^2) sits in the normal range of R, IEEE division below it (wave-uniform)
-            const R zr = alpha.re - beta, den = zr * zr + alpha.im * alpha.im;
-            R rb, rz;
-            if (den >= num<R>::rot_lo) {
-                rb = rcp_nr(beta);
-                rz = rcp_nr(den);
-            } else {
-                rb = (R)1 / beta;
-                rz = (R)1 / den;
+        cx<R> tj_, scale;
+        R beta;
+        if (!hh_scalars(alpha, xn2, tj_, scale, beta)) {
+            // a column of tiny entries: the scalars of x / max|x|, then beta and scale back
+            R mx = 0;
+#pragma unroll
+            for (int u = 0; u < T; ++u) {
+                const int r = off + T * ti + u;
+                if (r > j && r < N) mx = fmax(mx, fmax(fabs(a[u][JV].re), fabs(a[u][JV].im)));
             }
-            tj_ = mkc<R>((beta - alpha.re) * rb, -alpha.im * rb);
-            scale = mkc<R>(zr * rz, -alpha.im * rz);
+            mx = fmax(mx, dpp_r<0x121>(mx));
+            mx = fmax(mx, dpp_r<0x122>(mx));
+            mx = fmax(mx, dpp_r<0x124>(mx));
+            mx = fmax(mx, dpp_r<0x128>(mx));
+            mx = rl(mx, base);
+            const R inv = (R)1 / mx;
+            R xs2 = 0;
+#pragma unroll
+            for (int u = 0; u < T; ++u) {
+                const int r = off + T * ti + u;
+                if (r >= j + 2 && r < N) xs2 += abs2(inv * a[u][JV]);
+            }
+            xs2 = rl(row16_sum(xs2), base);
+            hh_scalars_scaled(inv * alpha, xs2, tj_, scale, beta);
+            beta *= mx;
+            scale = inv * scale;
+            if (!(inv <= num<R>::huge)) {
+                // entries below 1 / huge (subnormal): left in place, H_j = I
+                tj_ = mkc<R>(0, 0);
+                scale = mkc<R>(0, 0);
+                beta = alpha.re;
+            }
         }
         const bool triv = (tj_.re == (R)0 && tj_.im == (R)0);
 #pragma unroll

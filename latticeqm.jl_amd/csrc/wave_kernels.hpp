@@ -149,13 +149,34 @@ __global__ __launch_bounds__(kWave) void k_hk_tridiag_w(
         }
         xn2 = (R)wave_allsum((double)xn2);
         const cx<R> alpha = A[pj + 1];
-        cx<R> tj = {0, 0}, scale = {0, 0};
-        R beta = alpha.re;
-        if (!(xn2 == R(0) && alpha.im == R(0))) {
-            const R nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
-            beta = alpha.re >= R(0) ? -nrm : nrm;
-            tj = mkc<R>((beta - alpha.re) / beta, -alpha.im / beta);
-            scale = crecip(mkc<R>(alpha.re - beta, alpha.im));
+        cx<R> tj, scale;
+        R beta;
+        if (!hh_scalars(alpha, xn2, tj, scale, beta)) {
+            // a column of tiny entries: the scalars of x / max|x|, then beta and scale back
+            R mx = 0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int r = lane + q * kWave;
+                if (r > j && r < N) mx = fmax(mx, fmax(fabs(A[pj + r - j].re), fabs(A[pj + r - j].im)));
+            }
+#pragma unroll
+            for (int o = kWave / 2; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+            const R inv = R(1) / mx;
+            R xs2 = 0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int r = lane + q * kWave;
+                if (r >= j + 2 && r < N) xs2 += abs2(inv * A[pj + r - j]);
+            }
+            xs2 = (R)wave_allsum((double)xs2);
+            hh_scalars_scaled(inv * alpha, xs2, tj, scale, beta);
+            beta *= mx;
+            scale = inv * scale;
+            if (!(inv <= num<R>::huge)) {
+                tj = mkc<R>(0, 0);
+                scale = mkc<R>(0, 0);
+                beta = alpha.re;
+            }
         }
         const bool triv = (tj.re == R(0) && tj.im == R(0));
         cx<R> vq[NQ];

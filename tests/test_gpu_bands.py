@@ -39,7 +39,8 @@ def test_square_triangular_analytic(pkg, gpu):
     assert np.max(np.abs(b[0] - tri)) < 1e-12
 
 
-@pytest.fixture(params=[(0, -1), (1, -1), (2, -1), (0, 0)], ids=["tri-block", "tri-wave", "tri-reg", "bt-loop"])
+@pytest.fixture(params=[(0, -1), (1, -1), (2, -1), (3, -1), (4, -1), (0, 0)],
+                ids=["tri-block", "tri-wave", "tri-reg", "tri-col", "tri-col2", "bt-loop"])
 def tri_variant(request, gpu):
     """Tridiagonalization variant and back-transformation variant (-1: default, WY)."""
     gpu.set_variant(0, request.param[0])

@@ -9,8 +9,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module", params=[(-1, -1), (2, -1), (-1, 0), (-1, 1)],
-                ids=["default", "tri-reg", "refine-gemm-chain", "refine-fused"])
+@pytest.fixture(scope="module", params=[(-1, -1), (2, -1), (3, -1), (4, -1), (-1, 0), (-1, 1)],
+                ids=["default", "tri-reg", "tri-col", "tri-col2", "refine-gemm-chain", "refine-fused"])
 def mixed(pkg, gpu, request):
     """Mixed-precision executor with (tridiagonalization, refinement sweep) variants forced."""
     ex = pkg.api.to_executor("gpu", precision="mixed")

@@ -27,8 +27,8 @@ def _rho_vs_oracle(pkg, oracle, h, ks, mu, T, kweights=None, keys=None, tol=1e-9
     return rho, eb
 
 
-@pytest.fixture(params=[(0, 0, -1), (1, 1, -1), (0, 1, -1), (1, 0, -1), (2, 0, -1), (2, 1, -1), (0, 0, 0),
-                        (2, 0, 1)], ids=lambda v: f"tri{v[0]}-rho{v[1]}-bt{v[2]}")
+@pytest.fixture(params=[(0, 0, -1), (1, 1, -1), (0, 1, -1), (1, 0, -1), (2, 0, -1), (2, 1, -1), (3, 0, -1),
+                        (4, 1, -1), (0, 0, 0), (2, 0, 1)], ids=lambda v: f"tri{v[0]}-rho{v[1]}-bt{v[2]}")
 def variants(request, gpu):
     """Force each (tridiagonalization, rho_k, back-transformation) kernel variant, then restore."""
     for kind, v in zip((0, 1, 4), request.param):
@@ -95,7 +95,7 @@ def test_autotune_picks_valid_variants(pkg, oracle, gpu):
     m = pkg.api.GPUModel(h, gpu)
     ks_d = torch.from_numpy(np.ascontiguousarray(ks.T)).cuda()
     chosen = pkg.api.autotune(m, ks_d.data_ptr(), ks.shape[1])
-    assert chosen[0] in (0, 1, 2) and chosen[1] in (0, 1) and chosen[3] in (0, 1, 2, 3)
+    assert chosen[0] in (0, 1, 2, 3, 4) and chosen[1] in (0, 1) and chosen[3] in (0, 1, 2, 3)
     _rho_vs_oracle(pkg, oracle, h, ks, 0.0, 0.05)
 
 

@@ -108,6 +108,89 @@ def test_tri_small_register_tiles(pkg, oracle, gpu, N):
         assert np.max(np.abs(v.conj().T @ v - np.eye(N))) < 1e-12 * N
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("var", [3, 4])
+@pytest.mark.parametrize("N", [1, 2, 3, 9, 16, 17, 31, 32, 33, 40, 47, 48, 63, 64])
+def test_tri_column_per_lane_sizes(pkg, oracle, var, prec, N):
+    # column-per-lane tridiagonalization at the sizes where its phases (row chunks of 8) and
+    # its wave split (32 rows per wave) change: eigenpairs and the density matrix vs the
+    # oracle, in FP64 and in FP32 with refinement
+    ex = pkg.api.to_executor("gpu", precision=prec)
+    ex.set_variant(0, var)
+    ex.set_variant(4, 1)
+    try:
+        h = pkg.models.random_tb(N, rmax=1, seed=70 + N)
+        ks = pkg.regulargrid(nk=16)
+        b, V = pkg.api.eigen(h, ks, executor=ex)
+        rho = pkg.Hops({Rk: np.zeros((N, N), complex) for Rk in h.keys()})
+        eb = pkg.api.getdensitymatrix(rho, h, ks, 0.05, T=0.02, executor=ex)
+    finally:
+        ex.set_variant(0, -1)
+        ex.set_variant(4, -1)
+    R, T = h.pack()
+    ref = oracle.bands(R, T, ks)
+    hn = _hnorm(h, ks)
+    assert np.max(np.abs(b - ref) / np.maximum(np.abs(ref), hn[None, :])) < 1e-10
+    for j in range(ks.shape[1]):
+        v = V[:, :, j]
+        assert np.max(np.abs(h(ks[:, j]) @ v - v * b[:, j])) < 1e-11 * N * hn[j]
+        assert np.max(np.abs(v.conj().T @ v - np.eye(N))) < 1e-11 * N
+    rref, eb_ref = oracle.densitymatrix(R, T, ks, R, mu=0.05, Temp=0.02)
+    err = max(np.max(np.abs(rho[tuple(R[:, r])] - rref[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("var", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("tiny", [1e-20, 1e-160])
+def test_tiny_householder_column(pkg, oracle, var, prec, tiny):
+    # a column whose entries square below the normal range (1e-20 in FP32, 1e-160 in FP64):
+    # the Householder scalars come from the column rescaled by its largest entry instead of
+    # overflowing 1 / |alpha - beta|^2 (the converged honeycomb Hubbard mean field has such
+    # columns at its spin-degenerate k-points)
+    t = tiny
+    H0 = np.array([[1.0, 0.3, 0, 0], [0.3, 2.0, 1j * t, t], [0, -1j * t, 3.0, 0.2], [0, t, 0.2, 4.0]])
+    h = pkg.Hops({(0, 0): H0, (1, 0): 0.1 * np.eye(4), (-1, 0): 0.1 * np.eye(4)})
+    ks = pkg.regulargrid(nk=16)
+    ex = pkg.api.to_executor("gpu", precision=prec)
+    ex.set_variant(0, var)
+    try:
+        b, _ = pkg.api.bandmatrix(h, ks, executor=ex)
+        rho = pkg.Hops({Rk: np.zeros((4, 4), complex) for Rk in h.keys()})
+        eb = pkg.api.getdensitymatrix(rho, h, ks, 2.5, T=0.02, executor=ex)
+    finally:
+        ex.set_variant(0, -1)
+    R, T = h.pack()
+    ref = oracle.bands(R, T, ks)
+    assert np.all(np.isfinite(b))
+    assert np.max(np.abs(b - ref)) < 1e-10 * 5
+    rref, eb_ref = oracle.densitymatrix(R, T, ks, R, mu=2.5, Temp=0.02)
+    err = max(np.max(np.abs(rho[tuple(R[:, r])] - rref[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_tiny_householder_column_large_n(pkg, oracle, prec):
+    # the same on the large-N path: two random 65-orbital blocks coupled by entries of 1e-160,
+    # so the reduction meets a column of tiny entries after the first block
+    rng = np.random.default_rng(5)
+    n1 = 65
+    B = [rng.normal(size=(n1, n1)) + 1j * rng.normal(size=(n1, n1)) for _ in range(2)]
+    B = [0.5 * (b + b.conj().T) for b in B]
+    C = 1e-160 * (rng.normal(size=(n1, n1)) + 1j * rng.normal(size=(n1, n1)))
+    H0 = np.block([[B[0], C], [C.conj().T, B[1]]])
+    h = pkg.Hops({(0, 0): H0, (1, 0): 0.1 * np.eye(2 * n1), (-1, 0): 0.1 * np.eye(2 * n1)})
+    ks = pkg.regulargrid(nk=4)
+    ex = pkg.api.to_executor("gpu", precision=prec)
+    b, _ = pkg.api.bandmatrix(h, ks, executor=ex)
+    R, T = h.pack()
+    ref = oracle.bands(R, T, ks)
+    assert np.all(np.isfinite(b))
+    assert np.max(np.abs(b - ref)) < 1e-10 * np.max(np.abs(ref))
+
+
 def test_tri_variants_agree(pkg, gpu):
     # the two tridiagonal eigensolvers give the same eigenvalues to rounding
     h = pkg.models.random_tb(50, rmax=2, seed=77)
