@@ -724,9 +724,9 @@ def main() -> int:
     def flops(stage, nk):
         """Algorithmic FP flops of one launch over nk k-points (complex multiply-add = 8)."""
         if stage == "hk_tridiag":
-            # 16 < N <= 64: the register-tile kernel reduces the materialised H(k) (assembled
-            # in its own stage); otherwise the kernel assembles H(k) itself
-            hk = 0.0 if 16 < N <= 64 else 8.0 * nR * N * N
+            # N <= 64: the column-per-lane kernel reduces the materialised H(k) (assembled in
+            # its own stage); otherwise the kernel assembles H(k) itself
+            hk = 0.0 if N <= 64 else 8.0 * nR * N * N
             return nk * (hk + 16.0 / 3.0 * N ** 3)
         if stage == "backtransform":
             return nk * 8.0 * N ** 3
