@@ -511,7 +511,7 @@ def cfg4_leg() -> dict:
     each a random tight-binding model built like the headline one (25 keys, sub-seed 4000 + m),
     onsite Hubbard U_m ~ U(1, 4) in the sigma_x form on the N/2 site pairs, half filling,
     kT = 0.02, beta = 0.5, a 32 x 32 grid each (Nk = 1024), FP64: ONE mean-field iteration of
-    the whole batch on 4 streams; the large-N path serves N > 112.  Each model's iteration is
+    the whole batch on 4 streams; the large-N path serves N > 64.  Each model's iteration is
     one pass over its grid (H(k), eigenvalues, mu, eigenvectors, rho): the flop count is that
     single factorization, 64/3 N^3 + 8 (nR + 1) N^2 per k-point."""
     import importlib

@@ -405,3 +405,237 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
 }
 
 }  // namespace lqm
+
+namespace lqm {
+
+// ---------------------------------------------------------------------------------
+// Column-per-lane tridiagonalization for the large-N path at 64 < N <= 128 (it replaces the
+// zlatrd panel steps there): CG = 2 groups of 64 columns (lane l of group g holds column
+// 64 g + l) times NRG = 4 groups of RW = 32 rows, one wave per (column group, row group),
+// 512 threads per k-point.  In: A, N x N full Hermitian (column-major, working precision R)
+// per k-point.  Out (the layout of k_trd_col): d[kk N + j] = Re A(j, j), e[kk N + j] = beta,
+// tau[kk N + j], and the reflector entries v_j(r), r >= j + 2, written over A(r, j).
+// Step j, three barriers:
+//   A: the waves holding row j publish x_c = conj(A(j, c)) (c >= j + 2; alpha = x_{j+1}
+//      apart) and their partial ||x||^2; every wave then forms the zlarfg scalars itself.
+//      v_r = scale x_r for r >= j + 2 and v_{j+1} = 1 = scale (alpha - beta), so the product
+//      and the update use the raw x from LDS (row j + 1 taking alpha - beta) with the scalar
+//      applied once per lane.
+//   C: partial products y over each wave's rows and partial s = v^H y, summed in fixed order.
+//   E: w = tau y + alpha2 v by the last row group, read back for the update's rows.
+// The wave holding row j + 1 picks that row out of the update (xr, the next step's x).
+// ---------------------------------------------------------------------------------
+template <typename R>
+struct ColTriSmem2 {
+    alignas(16) cx<R> sx[2][128];   // x_r for r >= j + 2 (zero elsewhere), by step parity
+    alignas(16) cx<R> sy[4][128];   // partial products of the four row groups
+    alignas(16) cx<R> sw[128];      // w
+    cx<R> salpha[2];
+    R spn[2][2];                    // partial ||x(2:)||^2 of the two column groups
+    R smx[2][2];                    // partial max |x| (rescaling of tiny columns)
+    R ss[8];                        // partial s of the eight waves
+};
+
+// Step j on the rows of one wave (column group cg, rows r0 .. r0 + RW - 1); LQ = its first
+// live row chunk (NQ: none live), XR = row j + 1 lies in chunk LQ.
+template <typename R, int RW, int CH, int LQ, bool XR>
+__device__ __forceinline__ void coltri2_step(cx<R> (&a)[RW], cx<R>& xr, ColTriSmem2<R>& S, int N,
+                                             int kk, int cg, int rg, int l, int j, cx<R>* __restrict__ Ak,
+                                             double* __restrict__ dd, double* __restrict__ ee,
+                                             cx<R>* __restrict__ tau) {
+    constexpr int NQ = RW / CH;
+    const int r0 = RW * rg;
+    const int c = 64 * cg + l;
+    const int buf = j & 1;
+    const bool own = (j / RW) == rg;  // this wave holds row j
+    if (own) {
+        const cx<R> x = c > j ? conj(xr) : mkc<R>(0, 0);
+        const R pn = wave_sum64(c > j + 1 ? abs2(x) : (R)0);
+        const R pm = wave_max64(c > j ? fmax(fabs(x.re), fabs(x.im)) : (R)0);
+        S.sx[buf][c] = c > j + 1 ? x : mkc<R>(0, 0);
+        if (c == j + 1) S.salpha[buf] = x;
+        if (l == 0) {
+            S.spn[buf][cg] = pn;
+            S.smx[buf][cg] = pm;
+        }
+        if (c == j) dd[(size_t)kk * N + j] = (double)xr.re;
+    }
+    __syncthreads();  // A
+    // zlarfg scalars, formed by every wave from the published partials (the same bits)
+    const cx<R> alpha = S.salpha[buf];
+    const R xn2 = S.spn[buf][0] + S.spn[buf][1];
+    cx<R> t, scale;
+    R beta;
+    if (!hh_scalars(alpha, xn2, t, scale, beta)) {
+        // a column of tiny entries: the scalars of x / max|x|, then beta and scale back
+        const R mx = fmax(S.smx[buf][0], S.smx[buf][1]);
+        const R inv = (R)1 / mx;
+        const cx<R> x0 = S.sx[buf][l], x1 = S.sx[buf][64 + l];
+        const R xs2 = wave_sum64(abs2(inv * x0) + abs2(inv * x1));
+        hh_scalars_scaled(inv * alpha, xs2, t, scale, beta);
+        beta *= mx;
+        scale = inv * scale;
+        if (!(inv <= num<R>::huge)) {
+            t = mkc<R>(0, 0);
+            scale = mkc<R>(0, 0);
+            beta = alpha.re;
+        }
+    }
+    // own column's v
+    cx<R> vc = mkc<R>(0, 0);
+    if (c == j + 1) vc = mkc<R>(1, 0);
+    else if (c > j + 1) vc = S.sx[buf][c] * scale;
+    if (own && c > j + 1 && c < N) Ak[(size_t)j * N + c] = vc;
+    if (own && cg == 0 && l == 0) {
+        tau[(size_t)kk * N + j] = t;
+        ee[(size_t)kk * N + j] = (double)beta;
+    }
+    // partial y_c over this wave's rows: scale sum_{r >= j + 2} conj(A(r, c)) x_r, plus
+    // conj(A(j + 1, c)) from the wave holding row j + 1
+    // this wave's rows of the LDS vectors from one base address (static offsets after it)
+    const cx<R>* __restrict__ xb = &S.sx[buf][r0];
+    const cx<R>* __restrict__ wb = &S.sw[r0];
+    // v_r = scale x_r with x_{j+1} taken as alpha - beta (scale (alpha - beta) = 1), so the
+    // row j + 1 term needs no separate copy of that row
+    const cx<R> zab = mkc<R>(alpha.re - beta, alpha.im);
+    const int jj = j + 1 - r0 - CH * LQ;  // row j + 1 in chunk LQ (XR)
+    cx<R> ys = mkc<R>(0, 0), ys1 = mkc<R>(0, 0);
+#pragma unroll
+    for (int q = LQ; q < NQ; ++q) {
+        cx<R> xq[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            xq[u] = xb[CH * q + u];
+            if (XR && q == LQ) {
+                xq[u].re = jj == u ? zab.re : xq[u].re;
+                xq[u].im = jj == u ? zab.im : xq[u].im;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u += 2) {
+            ys = cmadc(ys, a[CH * q + u], xq[u]);
+            ys1 = cmadc(ys1, a[CH * q + u + 1], xq[u + 1]);
+        }
+        __builtin_amdgcn_sched_barrier(0);  // one chunk's operands live at a time
+    }
+    cx<R> y = (ys + ys1) * scale;
+    if (c <= j) y = mkc<R>(0, 0);
+    const R sp = wave_sum64(y.re * vc.re + y.im * vc.im);
+    S.sy[rg][c] = y;
+    if (l == 0) S.ss[2 * rg + cg] = sp;
+    __syncthreads();  // C
+    y = ((S.sy[0][c] + S.sy[1][c]) + S.sy[2][c]) + S.sy[3][c];
+    if (c <= j) y = mkc<R>(0, 0);
+    R s = 0;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) s += S.ss[w];
+    const R alpha2 = (R)-0.5 * (t.re * t.re + t.im * t.im) * s;
+    const cx<R> wc = cmad(t * y, mkc<R>(alpha2, 0), vc);
+    if (rg == 3) S.sw[c] = wc;
+    __syncthreads();  // E
+    // live rows: A(r, c) -= v_r conj(w_c) + w_r conj(v_c), v_r = scale x_r
+    const cx<R> g = scale * conj(wc);
+    const cx<R> nv = mkc<R>(-vc.re, -vc.im);
+#pragma unroll
+    for (int q = LQ; q < NQ; ++q) {
+        cx<R> xq[CH], wq[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            xq[u] = xb[CH * q + u];
+            wq[u] = wb[CH * q + u];
+            if (XR && q == LQ) {
+                xq[u].re = jj == u ? zab.re : xq[u].re;
+                xq[u].im = jj == u ? zab.im : xq[u].im;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            a[CH * q + u] = cmad(a[CH * q + u], mkc<R>(-xq[u].re, -xq[u].im), g);
+            a[CH * q + u] = cmadcb(a[CH * q + u], wq[u], nv);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (XR && LQ < NQ) {
+        // row j + 1 (chunk LQ) is the next step's xr
+        cx<R> x = a[CH * LQ];
+#pragma unroll
+        for (int u = 1; u < CH; ++u) {
+            x.re = jj == u ? a[CH * LQ + u].re : x.re;
+            x.im = jj == u ? a[CH * LQ + u].im : x.im;
+        }
+        xr = x;
+    }
+}
+
+// The steps j in [jlo, jhi] with this wave's first live chunk LQ (compile time).
+template <typename R, int RW, int CH, int LQ, bool XR>
+__device__ __forceinline__ void coltri2_range(cx<R> (&a)[RW], cx<R>& xr, ColTriSmem2<R>& S, int N,
+                                              int kk, int cg, int rg, int l, int jlo, int jhi,
+                                              cx<R>* __restrict__ Ak, double* __restrict__ dd,
+                                              double* __restrict__ ee, cx<R>* __restrict__ tau) {
+#pragma nounroll
+    for (int j = jlo; j <= jhi; ++j)
+        coltri2_step<R, RW, CH, LQ, XR>(a, xr, S, N, kk, cg, rg, l, j, Ak, dd, ee, tau);
+}
+// Local phases of the wave's own rows: q = the chunk holding row j + 1.
+template <typename R, int RW, int CH, int Q>
+__device__ __forceinline__ void coltri2_own(cx<R> (&a)[RW], cx<R>& xr, ColTriSmem2<R>& S, int N,
+                                            int kk, int cg, int rg, int l, cx<R>* __restrict__ Ak,
+                                            double* __restrict__ dd, double* __restrict__ ee,
+                                            cx<R>* __restrict__ tau) {
+    if constexpr (Q < RW / CH) {
+        const int r0 = RW * rg;
+        const int jlo = max(r0 + CH * Q - 1, 0), jhi = min(r0 + CH * Q + CH - 2, N - 2);
+        coltri2_range<R, RW, CH, Q, true>(a, xr, S, N, kk, cg, rg, l, jlo, jhi, Ak, dd, ee, tau);
+        coltri2_own<R, RW, CH, Q + 1>(a, xr, S, N, kk, cg, rg, l, Ak, dd, ee, tau);
+    }
+}
+
+template <typename R, int RW, int CH>
+__device__ __forceinline__ void coltri2_wave(ColTriSmem2<R>& S, cx<R>* __restrict__ Ak, int N, int kk, int cg,
+                                             int rg, int l, double* __restrict__ dd, double* __restrict__ ee,
+                                             cx<R>* __restrict__ tau) {
+    constexpr int NQ = RW / CH;
+    const int r0 = RW * rg;
+    const int c = 64 * cg + l;
+    const int cc = min(c, N - 1);
+    // a[i] = A(r, c) = conj(A(c, r)) (element (c, r) at A[r N + c]: lanes read consecutive
+    // entries); eight loads in flight at a time, addresses clamped, padding zeroed.  A column j
+    // is written over only by the waves holding row j, which loaded it themselves.
+    cx<R> a[RW];
+#pragma unroll
+    for (int i0 = 0; i0 < RW; i0 += 8) {
+        cx<R> h[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) h[i] = Ak[(size_t)min(r0 + i0 + i, N - 1) * N + cc];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int r = r0 + i0 + i;
+            a[i0 + i] = (r < N && c < N) ? mkc<R>(h[i].re, -h[i].im) : mkc<R>(0, 0);
+        }
+    }
+    cx<R> xr = a[0];  // row 0 (row group 0)
+    // steps before this wave's rows: all rows live; then the steps whose row j + 1 is its own
+    // (chunk by chunk); then the rest with no live rows (still at every barrier)
+    coltri2_range<R, RW, CH, 0, false>(a, xr, S, N, kk, cg, rg, l, 0, min(r0 - 2, N - 2), Ak, dd, ee, tau);
+    coltri2_own<R, RW, CH, 0>(a, xr, S, N, kk, cg, rg, l, Ak, dd, ee, tau);
+    coltri2_range<R, RW, CH, NQ, false>(a, xr, S, N, kk, cg, rg, l, max(r0 + RW - 1, 0), N - 2, Ak, dd, ee, tau);
+    if ((N - 1) / RW == rg && c == N - 1) {
+        dd[(size_t)kk * N + N - 1] = (double)xr.re;
+        ee[(size_t)kk * N + N - 1] = 0.0;
+        tau[(size_t)kk * N + N - 1] = mkc<R>(0, 0);
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(512) void k_tridiag_cg(int N, cx<R>* __restrict__ A, double* __restrict__ dd,
+                                                    double* __restrict__ ee, cx<R>* __restrict__ tau) {
+    constexpr int RW = 32, CH = 4;
+    __shared__ ColTriSmem2<R> S;
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
+    const int kk = blockIdx.x;
+    coltri2_wave<R, RW, CH>(S, A + (size_t)kk * N * N, N, kk, w & 1, w >> 1, l, dd, ee, tau);
+}
+
+}  // namespace lqm

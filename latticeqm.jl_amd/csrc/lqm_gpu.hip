@@ -166,6 +166,10 @@ struct lqm_context {
     size_t cap[W_NSLOTS] = {};
     int* d_status = nullptr;
     int refine_max = 3;      // Ogita-Aishima sweeps allowed per call (mixed precision)
+    // upper bound on the per-k scratch of one chunk (bytes): the batch SCF gives each worker
+    // context an equal share of the device's free memory so that workers on the largest
+    // models cannot together exceed it
+    size_t ws_cap = SIZE_MAX;
     double refine_thr = 1e-8;  // a k-point stops refining once its squared correction is below
     double report[6] = {};   // last mixed call: sweeps, max ||R||_F, max delta, near-cluster k,
                              // k-point sweeps run (sum over sweeps of the k-points refined),
@@ -337,7 +341,7 @@ Plan make_plan(const lqm_context* c, int N, int nR, int Nk, bool vectors, bool r
     p.tri2 = (vectors || rho) && variant(VK_TRIEIG) == 1 && p.wyC && p.btv == 1 && !p.rho2;
     p.trib = !vectors && !rho && variant(VK_TRIEIG) == 1 && N <= 64;
     perk += 8;
-    size_t kc = kWorkspaceBudget / perk;
+    size_t kc = std::min(kWorkspaceBudget, c->ws_cap) / perk;
     if (kc < 1) kc = 1;
     p.Kc = (int)std::min<size_t>(kc, (size_t)Nk);
     return p;
@@ -1075,7 +1079,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
     const size_t NN = (size_t)N * N;
     // refinement buffers: X, A, W, S, G (+ rho_k) in FP64 complex per k-point, X in FP32
     const size_t per64 = 16 * NN * (rho ? 6 : 5) + 8 * NN + 48 * N;
-    P.Kc = (int)std::max<size_t>(1, std::min<size_t>((size_t)P.Kc, kWorkspaceBudget / per64));
+    P.Kc = (int)std::max<size_t>(1, std::min<size_t>((size_t)P.Kc, std::min(kWorkspaceBudget, c->ws_cap) / per64));
     const double wuni = 1.0 / (double)Nk;
     const cx<double> one = {1.0, 0.0}, zero = {0.0, 0.0};
     lqm_status st;
@@ -1309,7 +1313,10 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
 
 // ---- large N: blocked tridiagonalization, bisection, inverse iteration, compact WY ------
 constexpr size_t kBlockedBudget = size_t(40) << 30;  // bytes of per-k scratch per chunk
-constexpr int kBlockedMinN = 112;    // default: the blocked path above this N (measured crossover)
+// default: the blocked path above these N (measured crossovers of the per-k kernels and the
+// large-N path with the column-per-lane reduction, 4096 k-points of a 9-key model)
+constexpr int kBlockedMinN = 64;         // FP64
+constexpr int kBlockedMinNMixed = 80;    // FP32 with refinement
 constexpr double kClusterRel = 1e-5;  // eigenvalues closer than this x ||T|| share a cluster
 constexpr double kOccTiny = 1e-18;    // occupations below this leave rho_k out
 
@@ -1318,7 +1325,7 @@ bool use_blocked(const lqm_context* c, int N) {
     if (f >= 0) return f == 1;
     auto it = c->tuned.find((VK_BLOCKED << 20) | N);
     if (it != c->tuned.end()) return it->second == 1;
-    return N > kBlockedMinN;
+    return N > (c->precision == LQM_PREC_FP32_REFINED ? kBlockedMinNMixed : kBlockedMinN);
 }
 
 template <int OPA, int OPB, typename R = double>
@@ -1369,6 +1376,14 @@ lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<R>* A, cx<R>* Vp, c
                            double* dd, double* ee, cx<R>* tau) {
     hipStream_t s = c->stream;
     const size_t NN = (size_t)N * N;
+    if (N > 64 && N <= 128 && c->forced[VK_TRIDIAG] != 0) {
+        // the column-per-lane reduction in registers (512 threads per k-point), the same outputs;
+        // variant 0 of kind 0 keeps the zlatrd panels below
+        StageScope sc(c, ST_BTRD, Kc);
+        hipLaunchKernelGGL(lqm::k_tridiag_cg<R>, dim3(Kc), dim3(512), 0, s, N, A, dd, ee, tau);
+        LQM_HIP(hipGetLastError());
+        return LQM_OK;
+    }
     for (int p = 0; p < N - 1; p += lqm::kBnb) {
         const int pw = std::min(lqm::kBnb, N - 1 - p);
         {
@@ -1425,7 +1440,7 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
                   16 * (size_t)nchmax * lqm::kQStride + 8 * (size_t)nbmax + 64 * (size_t)N + 64;
     if (vectors) perk += 8 * NN + 24 * NN + 16 * NN + 16 * (size_t)nblk * lqm::kBwy * lqm::kBwy + 2 * 16 * 64 * (size_t)N;
     if (want_rho) perk += 16 * NN;
-    const int Kmax = (int)std::max<size_t>(1, std::min<size_t>({kBlockedBudget / perk, (size_t)Nk, (size_t)65535}));
+    const int Kmax = (int)std::max<size_t>(1, std::min<size_t>({std::min(kBlockedBudget, c->ws_cap) / perk, (size_t)Nk, (size_t)65535}));
     if (mq && Kmax < Nk) return kFallback;  // mu needs every eigenvalue before the first occupation
     begin_pass(c, LQM_PREC_FP64);
     const double wuni = 1.0 / (double)Nk;
@@ -1649,7 +1664,7 @@ lqm_status run_blocked_mixed(lqm_model* m, int Nk, const double* ks, const doubl
                   8 * NN + 24 * NN + 8 * NN + 8 * (size_t)nblk * lqm::kBwy * lqm::kBwy + 2 * 8 * 64 * (size_t)N +
                   4 * 16 * NN + 16 * (size_t)maxclg * lqm::kClusterMax * lqm::kClusterMax + 48;
     if (want_rho) perk += 16 * NN;
-    const int Kmax = (int)std::max<size_t>(1, std::min<size_t>({kBlockedMixedBudget / perk, (size_t)Nk, (size_t)65535}));
+    const int Kmax = (int)std::max<size_t>(1, std::min<size_t>({std::min(kBlockedMixedBudget, c->ws_cap) / perk, (size_t)Nk, (size_t)65535}));
     if (mq && Kmax < Nk) return kFallback;  // mu needs every refined eigenvalue before the first occupation
     begin_pass(c, LQM_PREC_FP32_REFINED);
     const double wuni = 1.0 / (double)Nk;
@@ -3902,6 +3917,16 @@ lqm_status lqm_hartreefock_scf_batch(lqm_context* ctx, int d, int M, lqm_hf_prob
         return ca > cb;
     });
     const int W = std::min(nstreams, M);
+    // one HBM budget across the workers: each worker context's per-chunk scratch is capped at
+    // an equal share of 85 % of the device memory free now (the largest models then run in
+    // more chunks rather than together exceed the device)
+    size_t cap = SIZE_MAX;
+    {
+        size_t fr = 0, tot = 0;
+        LQM_HIP(hipSetDevice(ctx->device));
+        LQM_HIP(hipMemGetInfo(&fr, &tot));
+        cap = std::max<size_t>(size_t(256) << 20, (size_t)(0.85 * (double)fr) / (size_t)W);
+    }
     std::atomic<int> next{0};
     std::mutex prog_mu;
     std::vector<std::string> errs(W);
@@ -3916,6 +3941,7 @@ lqm_status lqm_hartreefock_scf_batch(lqm_context* ctx, int d, int M, lqm_hf_prob
         }
         wc->refine_max = ctx->refine_max;
         wc->refine_thr = ctx->refine_thr;
+        wc->ws_cap = std::min(ctx->ws_cap, cap);
         wc->tuned = ctx->tuned;
         for (int k = 0; k < VK_N; ++k) wc->forced[k] = ctx->forced[k];
         for (int i = next.fetch_add(1); i < M; i = next.fetch_add(1)) {

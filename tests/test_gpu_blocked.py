@@ -252,7 +252,7 @@ def test_mixed_blocked_degenerate_and_occ(pkg, oracle, mixed_blocked):
 
 def test_tbg_m11_slater_koster_mixed(pkg, oracle, gpu):
     # the 1588-orbital twisted bilayer at Gamma and a generic k-point through the FP32 blocked
-    # eigensolver and the FP64 refinement (the default layout above 112 orbitals)
+    # eigensolver and the FP64 refinement (the default layout above 80 orbitals)
     ex = pkg.api.to_executor("gpu", precision="mixed")
     h = pkg.models.tbg_slater_koster(11)
     N = h[(0, 0)].shape[0]
@@ -272,3 +272,67 @@ def test_tbg_m11_slater_koster_mixed(pkg, oracle, gpu):
     assert err < 1e-9, err
     assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
     assert ex.refinement_report()["sweeps"] >= 1
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("N", [65, 66, 95, 96, 97, 127, 128])
+def test_blocked_column_per_lane_reduction(pkg, oracle, prec, N):
+    # 64 < N <= 128: the large-N path reduces H(k) with the column-per-lane kernel (two
+    # column groups x four row groups); sizes at the column-group and row-group edges.
+    # Eigenpairs and rho vs the oracle, and the eigenvalues vs the zlatrd panels (kind 0
+    # variant 0 keeps them)
+    ex = pkg.api.to_executor("gpu", precision=prec)
+    ex.set_variant(5, 1)
+    h = pkg.models.random_tb(N, rmax=1, seed=900 + N)
+    ks = pkg.regulargrid(nk=9)
+    R, T = h.pack()
+    try:
+        b, _ = pkg.api.bandmatrix(h, ks, executor=ex)
+        rho = pkg.Hops({Rk: np.zeros((N, N), complex) for Rk in h.keys()})
+        eb = pkg.api.getdensitymatrix(rho, h, ks, 0.05, T=0.02, executor=ex)
+        ex.set_variant(0, 0)
+        b0, _ = pkg.api.bandmatrix(h, ks, executor=ex)
+        if prec == "fp64":
+            ex.set_variant(0, -1)
+            bv, V = pkg.api.eigen(h, ks, executor=ex)
+    finally:
+        ex.set_variant(0, -1)
+        ex.set_variant(5, -1)
+    ref = oracle.bands(R, T, ks)
+    hn = _hnorm(h, ks)
+    assert np.max(np.abs(b - ref) / np.maximum(np.abs(ref), hn[None, :])) < 1e-10
+    assert np.max(np.abs(b - b0)) < 1e-10 * np.max(hn)
+    rref, eb_ref = oracle.densitymatrix(R, T, ks, R, mu=0.05, Temp=0.02)
+    err = max(np.max(np.abs(rho[tuple(R[:, r])] - rref[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+    assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+    if prec == "fp64":
+        for j in range(ks.shape[1]):
+            v = V[:, :, j]
+            assert np.max(np.abs(h(ks[:, j]) @ v - v * bv[:, j])) < 1e-10 * hn[j]
+            assert np.max(np.abs(v.conj().T @ v - np.eye(N))) < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_blocked_tiny_column_mid_n(pkg, oracle, prec):
+    # a column of tiny entries in the column-per-lane reduction (two blocks coupled by 1e-20
+    # in FP32 terms, 1e-160 in FP64): rescaled Householder scalars, no overflow
+    rng = np.random.default_rng(8)
+    n1 = 50
+    B = [rng.normal(size=(n1, n1)) + 1j * rng.normal(size=(n1, n1)) for _ in range(2)]
+    B = [0.5 * (b + b.conj().T) for b in B]
+    t = 1e-20 if prec == "mixed" else 1e-160
+    C = t * (rng.normal(size=(n1, n1)) + 1j * rng.normal(size=(n1, n1)))
+    H0 = np.block([[B[0], C], [C.conj().T, B[1]]])
+    h = pkg.Hops({(0, 0): H0, (1, 0): 0.1 * np.eye(2 * n1), (-1, 0): 0.1 * np.eye(2 * n1)})
+    ks = pkg.regulargrid(nk=4)
+    ex = pkg.api.to_executor("gpu", precision=prec)
+    ex.set_variant(5, 1)
+    try:
+        b, _ = pkg.api.bandmatrix(h, ks, executor=ex)
+    finally:
+        ex.set_variant(5, -1)
+    R, T = h.pack()
+    ref = oracle.bands(R, T, ks)
+    assert np.all(np.isfinite(b))
+    assert np.max(np.abs(b - ref)) < 1e-10 * np.max(np.abs(ref))

@@ -12,7 +12,7 @@ dOmega/dt_R against the oracle's rho(-R)^T to 1e-9 and against central differenc
 * random_tb N=128, 9 keys, seed 5, 64 x 64 grid, mu = 0, kT = 0.05: Omega and its gradient;
 * models drawn like the 256-model batch (N = even round(exp U(ln 16, ln 512)), 25 keys, Hubbard
   U ~ U(1, 4) in the sigma_x form, seed 4): the first 16 draws with N <= 200 -- nine models,
-  three of them above the 112-orbital large-N threshold -- on 8 x 8 grids, batch SCF with the
+  three of them above 112 orbitals (large-N path in both precisions) -- on 8 x 8 grids, batch SCF with the
   FP64 and the mixed executor.
 """
 import importlib

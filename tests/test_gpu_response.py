@@ -5,7 +5,7 @@ optical-conductivity invariants (test/test_linearresponse.jl) on the GPU result.
 Bars: ldos and density to 1e-12 relative to the largest entry; chi(w) to 1e-10 relative to
 max |chi| (normwise over the frequencies: the pair sums cancel between occupied and empty
 states, so single entries carry the rounding of the largest terms); the reference's
-invariants with its own tolerances.  Paths covered: per-k eigensolver (N <= 112, FP64 and the
+invariants with its own tolerances.  Paths covered: per-k eigensolver (N <= 64, FP64 and the
 mixed executor, whose eigenvectors come from the FP64 path), the large-N blocked eigensolver
 (N = 160), several frequency tiles (Nw > 64), k-weights, the 128 x 128 headline grid in two
 chunks of k-points (Kubo at N = 64), and J1 == J2 (one U^H J U product).
