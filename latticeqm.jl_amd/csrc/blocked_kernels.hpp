@@ -26,8 +26,9 @@ namespace lqm {
 constexpr int kBnb = 32;      // tridiagonalization panel width
 constexpr int kBwy = 64;      // reflectors per compact-WY block of the back-transformation
 constexpr int kHvCols = 64;   // column chunk of the trailing matrix-vector product
-constexpr int kBRows = 16;    // rows per thread of the one-block-per-k kernels (N <= 4096)
-constexpr int kBMaxN = 256 * kBRows;
+constexpr int kBColT = 256;   // threads of the column-step kernel (one block per k-point)
+constexpr int kBRows = 16;    // rows per thread of the column step (N <= 4096)
+constexpr int kBMaxN = kBColT * kBRows;
 constexpr int kHkK = 8;       // k-points per block of k_hk_full
 
 // ---------------------------------------------------------------------------------
@@ -124,13 +125,13 @@ __device__ __forceinline__ double blk_max(double v, double* red) {
 // One 256-thread block per k-point; thread t owns rows j + t + 256q.
 // ---------------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_trd_col(int N, int j, int i, int last, cx<R>* __restrict__ A,
+__global__ __launch_bounds__(kBColT) void k_trd_col(int N, int j, int i, int last, cx<R>* __restrict__ A,
                                                  cx<R>* __restrict__ V,
                                                  const cx<R>* __restrict__ W,
                                                  double* __restrict__ dd, double* __restrict__ ee,
                                                  cx<R>* __restrict__ tau) {
     __shared__ cx<R> wj[kBnb], vj[kBnb];
-    __shared__ double red[4];
+    __shared__ double red[kBColT / 64];
     __shared__ cx<R> s_alpha;
     __shared__ R s_dj;
     const int tid = threadIdx.x;
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(256) void k_trd_col(int N, int j, int i, int last, 
     double xn = 0.0;
 #pragma unroll
     for (int q = 0; q < kBRows; ++q) {
-        const int r = j + tid + 256 * q;
+        const int r = j + tid + kBColT * q;
         cx<R> x = czero;
         if (r < N) {
             x = Ak[(size_t)j * N + r];
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(256) void k_trd_col(int N, int j, int i, int last, 
     }
     if (tid == 0) s_dj = a[0].re;
     if (tid == 1 && j + 1 < N) s_alpha = a[0];
-    const double xnorm2 = blk_sum<4>(xn, red);  // (its barriers publish s_dj, s_alpha)
+    const double xnorm2 = blk_sum<kBColT / 64>(xn, red);  // (its barriers publish s_dj, s_alpha)
     const R dj = s_dj;
     if (last) {
         if (tid == 0) {
@@ -194,21 +195,21 @@ __global__ __launch_bounds__(256) void k_trd_col(int N, int j, int i, int last, 
             double mx = 0.0;
 #pragma unroll
             for (int q = 0; q < kBRows; ++q) {
-                const int r = j + tid + 256 * q;
+                const int r = j + tid + kBColT * q;
                 if (r > j && r < N) mx = fmax(mx, fmax(fabs((double)a[q].re), fabs((double)a[q].im)));
             }
-            mx = blk_max<4>(mx, red);
+            mx = blk_max<kBColT / 64>(mx, red);
             const double inv = 1.0 / mx;
             double xs = 0.0;
 #pragma unroll
             for (int q = 0; q < kBRows; ++q) {
-                const int r = j + tid + 256 * q;
+                const int r = j + tid + kBColT * q;
                 if (r >= j + 2 && r < N) {
                     const double xr = inv * a[q].re, xi = inv * a[q].im;
                     xs += xr * xr + xi * xi;
                 }
             }
-            xs = blk_sum<4>(xs, red);
+            xs = blk_sum<kBColT / 64>(xs, red);
             cx<double> ts, ss;
             double bs;
             hh_scalars_scaled(cx<double>{inv * are, inv * aim}, xs, ts, ss, bs);
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(256) void k_trd_col(int N, int j, int i, int last, 
     const R rbeta = (R)beta;
 #pragma unroll
     for (int q = 0; q < kBRows; ++q) {
-        const int r = j + tid + 256 * q;
+        const int r = j + tid + kBColT * q;
         if (r < N) {
             if (r == j) {
                 Ak[(size_t)j * N + j] = {dj, R(0)};
@@ -250,7 +251,10 @@ __global__ __launch_bounds__(256) void k_trd_col(int N, int j, int i, int last, 
 // Products of the column step j over the lower triangle of A22 = A(j+1:N, j+1:N) (only
 // elements on or below the diagonal are read; 64 x 64 tiles from row/column j+1).  Tile
 // column J is cut into parts of at most kHvPart tiles (I = J + kHvPart p ...), one
-// 512-thread block per (J, p, k); lane = row of the tile, wave w = columns w, w + 8, ...:
+// 256-thread block per (J, p, k); lane = row of the tile, wave w = columns 16w .. 16w + 15
+// of the tile column.  A wave streams its 16-column slab of each tile of the part with no
+// barrier in between (16 loads per lane in flight, the slab's row sums parked in LDS), the
+// four waves' row sums are added once at the end:
 //   P[k][J][r]       = sum_{c in J, c <= r} A(r, c) v_c           (r in the part's tiles)
 //   P2[k][p][c]      = sum_{r > c, r in the part} conj(A(r, c)) v_r   (c in J)
 //   SS[k][b]         = the block's share of s = v^H A22 v (real)
@@ -258,7 +262,8 @@ __global__ __launch_bounds__(256) void k_trd_col(int N, int j, int i, int last, 
 //   rows of tile J (t < i).
 constexpr int kQStride = 1 + 2 * kBnb;
 constexpr int kHvPart = 8;
-constexpr int kHvWaves = 8;
+constexpr int kHvWaves = 4;
+constexpr int kHvCpw = kHvCols / kHvWaves;  // columns per wave
 __host__ __device__ __forceinline__ int hv_parts(int nt, int J) { return (nt - J + kHvPart - 1) / kHvPart; }
 // number of (J, p) blocks of a column step with nt tile columns
 __host__ __device__ __forceinline__ int hv_blocks(int nt) {
@@ -267,16 +272,15 @@ __host__ __device__ __forceinline__ int hv_blocks(int nt) {
     return b;
 }
 template <typename R>
-__global__ __launch_bounds__(kHvWaves * 64) void k_trd_hemv(int N, int j, int i, int nchmax, int nbmax,
+__global__ __launch_bounds__(kHvWaves * 64) __attribute__((amdgpu_waves_per_eu(3))) void k_trd_hemv(int N, int j, int i, int nchmax, int nbmax,
                                                   const cx<R>* __restrict__ A,
                                                   const cx<R>* __restrict__ V,
                                                   const cx<R>* __restrict__ W,
                                                   cx<R>* __restrict__ P, cx<R>* __restrict__ P2,
                                                   cx<R>* __restrict__ Q, double* __restrict__ SS) {
-    __shared__ cx<R> sv[kHvCols];
-    __shared__ cx<R> red[2][kHvWaves][64];
+    __shared__ cx<R> sy[kHvPart][kHvWaves][64];
     __shared__ double sred[kHvWaves];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const size_t kk = blockIdx.y;
     const int n = N - j - 1;
     const int nt = (n + kHvCols - 1) / kHvCols;
@@ -288,57 +292,66 @@ __global__ __launch_bounds__(kHvWaves * 64) void k_trd_hemv(int N, int j, int i,
     }
     const int I0 = J + kHvPart * p, I1 = min(nt, I0 + kHvPart);
     const int c0 = j + 1 + J * kHvCols;
+    const int cw = c0 + kHvCpw * wave;  // this wave's first column
     const size_t NP = (size_t)N * kBnb;
     const cx<R>* Vk = V + kk * NP;
     const cx<R>* Wk = W + kk * NP;
     const cx<R>* v = Vk + (size_t)i * N;
     const cx<R>* Ak = A + kk * (size_t)N * N;
-    if (tid < kHvCols) sv[tid] = c0 + tid < N ? v[c0 + tid] : cx<R>{R(0), R(0)};
-    __syncthreads();
-    constexpr int CPW = kHvCols / kHvWaves;  // columns per wave
-    cx<R> ycol[CPW];
+    cx<R> vc[kHvCpw];  // v over the wave's columns (wave-uniform)
 #pragma unroll
-    for (int u = 0; u < CPW; ++u) ycol[u] = {R(0), R(0)};
-    double spart = 0.0;
+    for (int u = 0; u < kHvCpw; ++u) vc[u] = cw + u < N ? v[cw + u] : cx<R>{R(0), R(0)};
+    cx<R> ycol[kHvCpw];
+#pragma unroll
+    for (int u = 0; u < kHvCpw; ++u) ycol[u] = {R(0), R(0)};
+#pragma nounroll
     for (int I = I0; I < I1; ++I) {
         const int r = j + 1 + I * kHvCols + lane;
         const bool ok = r < N;
         const int rr = ok ? r : N - 1;  // loads stay in bounds and unconditional (batched)
-        const cx<R> vr = ok ? v[r] : cx<R>{R(0), R(0)};
-        cx<R> a[CPW];
+        cx<R> a[kHvCpw];
 #pragma unroll
-        for (int u = 0; u < CPW; ++u) {
-            const int c = min(c0 + wave + kHvWaves * u, N - 1);
-            a[u] = Ak[(size_t)c * N + rr];
+        for (int u = 0; u < kHvCpw; ++u) {
+            const int c = min(cw + u, N - 1);
+            const cx<R>* col = Ak + (size_t)c * N;  // wave-uniform base
+            a[u] = col[max(rr, c)];  // above the diagonal: a row the wave reads anyway (masked)
         }
-        cx<R> yrow = {0.0, 0.0};
+        const cx<R> vr = ok ? v[r] : cx<R>{R(0), R(0)};
+        cx<R> yrow = {R(0), R(0)};
 #pragma unroll
-        for (int u = 0; u < CPW; ++u) {
-            const int cl = wave + kHvWaves * u, c = c0 + cl;
-            const bool in = ok && c < N && r >= c;
+        for (int u = 0; u < kHvCpw; ++u) {
+            const int c = cw + u;
+            const bool in = ok && r >= c;  // r < N and c <= r: c < N
             const cx<R> x = in ? a[u] : cx<R>{R(0), R(0)};
-            yrow = cmad(yrow, x, sv[cl]);
+            yrow = cmad(yrow, x, vc[u]);
             const cx<R> xt = (in && r > c) ? x : cx<R>{R(0), R(0)};
             ycol[u] = cmadc(ycol[u], xt, vr);
         }
-        const int buf = (I - I0) & 1;
-        red[buf][wave][lane] = yrow;
-        __syncthreads();
-        if (wave == 0 && ok) {
-            cx<R> y = red[buf][0][lane];
+        sy[I - I0][wave][lane] = yrow;
+    }
+    __syncthreads();
+    double spart = 0.0;
+    const int rows = (I1 - I0) * 64;
+    for (int e = tid; e < rows; e += kHvWaves * 64) {
+        const int t = e >> 6, l = e & 63;
+        const int r = j + 1 + (I0 + t) * kHvCols + l;
+        if (r < N) {
+            cx<R> y = sy[t][0][l];
 #pragma unroll
-            for (int w = 1; w < kHvWaves; ++w) y = y + red[buf][w][lane];
+            for (int w = 1; w < kHvWaves; ++w) y = y + sy[t][w][l];
             P[kk * (size_t)nchmax * N + (size_t)J * N + r] = y;
+            const cx<R> vr = v[r];
             spart += (double)(vr.re * y.re + vr.im * y.im);  // Re(conj(v_r) y_r)
         }
     }
-#pragma unroll
-    for (int u = 0; u < CPW; ++u) {
-        const int cl = wave + kHvWaves * u, c = c0 + cl;
-        const cx<R> y = wave_sum(ycol[u]);
-        if (lane == 0 && c < N) {
+    {
+        static_assert(kHvCpw == 16, "column sums by wave_sum16");
+        const cx<R> y = wave_sum16(ycol, lane);
+        const int c = cw + wave_sum16_index(lane);
+        if ((lane & 3) == 0 && c < N) {
             P2[kk * (size_t)nchmax * N + (size_t)p * N + c] = y;
-            spart += (double)(sv[cl].re * y.re + sv[cl].im * y.im);
+            const cx<R> vcol = v[c];
+            spart += (double)(vcol.re * y.re + vcol.im * y.im);
         }
     }
     if (p == 0) {
@@ -347,14 +360,22 @@ __global__ __launch_bounds__(kHvWaves * 64) void k_trd_hemv(int N, int j, int i,
         const int r = c0 + lane;
         const cx<R> vr = r < N ? v[r] : cx<R>{R(0), R(0)};
         const int rr = r < N ? r : N - 1;
-        for (int idx = wave; idx < 2 * i; idx += kHvWaves) {
-            const cx<R>* X = (idx & 1) ? Vk : Wk;
-            const cx<R> x = X[(size_t)(idx >> 1) * N + rr];
-            cx<R> acc = {0.0, 0.0};
-            acc = cmadc(acc, r < N ? x : cx<R>{R(0), R(0)}, vr);
-            acc = wave_sum(acc);
-            if (lane == 0) Qk[1 + (idx & 1) * kBnb + (idx >> 1)] = acc;
+        // the 2i <= 2 kBnb - 2 products, kHvCpw per wave: every load issued before the sums
+        static_assert(2 * kBnb <= kHvCpw * kHvWaves, "one round of products");
+        cx<R> x[kHvCpw];
+#pragma unroll
+        for (int t = 0; t < kHvCpw; ++t) {
+            const int idx = max(0, min(wave + kHvWaves * t, 2 * i - 1));
+            x[t] = ((idx & 1) ? Vk : Wk)[(size_t)(idx >> 1) * N + rr];
         }
+#pragma unroll
+        for (int t = 0; t < kHvCpw; ++t) {
+            const cx<R> z = {R(0), R(0)};
+            x[t] = cmadc(z, r < N ? x[t] : z, vr);
+        }
+        const cx<R> acc = wave_sum16(x, lane);
+        const int idx = wave + kHvWaves * wave_sum16_index(lane);
+        if ((lane & 3) == 0 && idx < 2 * i) Qk[1 + (idx & 1) * kBnb + (idx >> 1)] = acc;
     }
     const double st = blk_sum<kHvWaves>(spart, sred);
     if (tid == 0) SS[kk * nbmax + blockIdx.x] = st;
@@ -411,14 +432,42 @@ __global__ __launch_bounds__(256) void k_trd_w(int N, int j, int i, int nchmax, 
     const cx<R>* Pk = P + kk * (size_t)nchmax * N;
     const cx<R>* P2k = P2 + kk * (size_t)nchmax * N;
     const int I = (r - j - 1) / kHvCols;
+    // the partial sums and the panel columns in batches of 8 loads (indices clamped into range,
+    // the extra terms masked), so a batch is in flight at once
     cx<R> y = {0.0, 0.0};
-    for (int J = 0; J <= I; ++J) y = y + Pk[(size_t)J * N + r];
+    for (int J0 = 0; J0 <= I; J0 += 8) {
+        cx<R> t8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t8[u] = Pk[(size_t)min(J0 + u, I) * N + r];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (J0 + u <= I) y = y + t8[u];
+    }
     const int np = hv_parts(nt, I);
-    for (int q = 0; q < np; ++q) y = y + P2k[(size_t)q * N + r];
+    for (int q0 = 0; q0 < np; q0 += 8) {
+        cx<R> t8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t8[u] = P2k[(size_t)min(q0 + u, np - 1) * N + r];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (q0 + u < np) y = y + t8[u];
+    }
     cx<R> u = {0.0, 0.0};
-    for (int t = 0; t < i; ++t) {
-        u = cmad(u, Vk[(size_t)t * N + r], t12[1 + t]);
-        u = cmad(u, Wk[(size_t)t * N + r], t12[1 + kBnb + t]);
+    for (int t0 = 0; t0 < i; t0 += 8) {
+        cx<R> v8[8], w8[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int t = min(t0 + q, i - 1);
+            v8[q] = Vk[(size_t)t * N + r];
+            w8[q] = Wk[(size_t)t * N + r];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (t0 + q < i) {
+                const int t = t0 + q;
+                u = cmad(u, v8[q], t12[1 + t]);
+                u = cmad(u, w8[q], t12[1 + kBnb + t]);
+            }
     }
     const cx<R> w = tj * (y - u);
     const cx<R> vr = v[r];

@@ -94,6 +94,32 @@ __device__ __forceinline__ cx<R> wave_sum(cx<R> v) {
     return v;
 }
 
+// Sixteen wave sums at once by a halving exchange: each xor step trades half of the values
+// with the partner lane, so 17 exchanges replace 16 x 6.  Lane l ends with the sum over the
+// wave of x[u], u = 8 b5 + 4 b4 + 2 b3 + b2 (b = bits of l; lanes differing only in bits
+// 0 and 1 hold the same sum).
+template <typename R>
+__device__ __forceinline__ cx<R> xor_cx(cx<R> v, int m) {
+    return {__shfl_xor(v.re, m), __shfl_xor(v.im, m)};
+}
+__device__ __forceinline__ int wave_sum16_index(int lane) {
+    return ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> wave_sum16(const cx<R> (&x)[16], int lane) {
+    cx<R> a[8], b[4], c[2];
+    const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = (h5 ? x[8 + u] : x[u]) + xor_cx(h5 ? x[u] : x[8 + u], 32);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) b[u] = (h4 ? a[4 + u] : a[u]) + xor_cx(h4 ? a[u] : a[4 + u], 16);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) c[u] = (h3 ? b[2 + u] : b[u]) + xor_cx(h3 ? b[u] : b[2 + u], 8);
+    cx<R> d = (h2 ? c[1] : c[0]) + xor_cx(h2 ? c[0] : c[1], 4);
+    d = d + xor_cx(d, 2);
+    return d + xor_cx(d, 1);
+}
+
 // e^{2 pi i x} in FP64 (sincospi keeps the argument reduction exact).
 __device__ __forceinline__ cx<double> cis2pi(double x) {
     double s, c;

@@ -1391,7 +1391,7 @@ lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<R>* A, cx<R>* Vp, c
             for (int i = 0; i < pw; ++i) {
                 const int j = p + i, n = N - j - 1;
                 const int nch = (n + lqm::kHvCols - 1) / lqm::kHvCols;
-                hipLaunchKernelGGL(lqm::k_trd_col<R>, dim3(Kc), dim3(256), 0, s, N, j, i, 0, A, Vp, Wp, dd, ee,
+                hipLaunchKernelGGL(lqm::k_trd_col<R>, dim3(Kc), dim3(lqm::kBColT), 0, s, N, j, i, 0, A, Vp, Wp, dd, ee,
                                    tau);
                 hipLaunchKernelGGL(lqm::k_trd_hemv<R>, dim3(lqm::hv_blocks(nch), Kc), dim3(lqm::kHvWaves * 64), 0, s, N, j, i, nchmax,
                                    nbmax, A, Vp, Wp, P, P2, Q, SS);
@@ -1414,7 +1414,7 @@ lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<R>* A, cx<R>* Vp, c
     }
     {
         StageScope sc(c, ST_BTRD, Kc);
-        hipLaunchKernelGGL(lqm::k_trd_col<R>, dim3(Kc), dim3(256), 0, s, N, N - 1, 0, 1, A, Vp, Wp, dd, ee, tau);
+        hipLaunchKernelGGL(lqm::k_trd_col<R>, dim3(Kc), dim3(lqm::kBColT), 0, s, N, N - 1, 0, 1, A, Vp, Wp, dd, ee, tau);
     }
     LQM_HIP(hipGetLastError());
     return LQM_OK;
