@@ -409,11 +409,26 @@ __global__ __launch_bounds__(256) void k_trd_w(int N, int j, int i, int nchmax, 
     const cx<R>* Qk = Q + kk * nchmax * kQStride;
     const cx<R> tj = tau[kk * N + j];
     if (tid < kQStride) {
+        // the block partials in batches of 8 loads (indices clamped, extra terms masked)
         cx<R> acc = {0.0, 0.0};
         if (tid == 0) {
-            for (int b = 0; b < nb; ++b) acc.re += SS[kk * nbmax + b];
+            for (int b0 = 0; b0 < nb; b0 += 8) {
+                double t8[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) t8[u] = SS[kk * nbmax + min(b0 + u, nb - 1)];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (b0 + u < nb) acc.re += t8[u];
+            }
         } else if ((tid - 1) % kBnb < i) {
-            for (int J = 0; J < nt; ++J) acc = acc + Qk[(size_t)J * kQStride + tid];
+            for (int J0 = 0; J0 < nt; J0 += 8) {
+                cx<R> t8[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) t8[u] = Qk[(size_t)min(J0 + u, nt - 1) * kQStride + tid];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (J0 + u < nt) acc = acc + t8[u];
+            }
         }
         t12[tid] = acc;
     }
