@@ -36,11 +36,31 @@ __global__ __launch_bounds__(256) void k_phase_table(int Kc, int d, const double
 // starts at column z * RG), read at wave-uniform addresses (scalar loads, the phases feed
 // the FMAs from scalar registers): thread = element e, RG accumulators, the k-points of
 // split y in order, four rho_k elements in flight.
+//
+// Hermitian mode (Nh = N > 0, the key set closed under R -> -R): thread eu < ne = N(N+1)/2
+// takes the upper-triangle element (r, c), r <= c, packed by columns (eu = c(c+1)/2 + r), and
+// part has row length ne; the fold rebuilds the lower triangle from rho(R)(r, c) =
+// conj(rho(-R)(c, r)), which holds because every rho_k is Hermitian and the weights are real.
+// Otherwise (Nh = 0) thread e < ne = NN is element e.
+__device__ __forceinline__ void upper_rc(int eu, int& r, int& c) {
+    int cc = (int)((sqrt(8.0 * eu + 1.0) - 1.0) * 0.5);
+    while ((cc + 1) * (cc + 2) / 2 <= eu) ++cc;
+    while (cc * (cc + 1) / 2 > eu) --cc;
+    c = cc;
+    r = eu - cc * (cc + 1) / 2;
+}
 template <int RG>
-__global__ __launch_bounds__(256) void k_phase_contract_s(int NN, int Nkc, const cx<double>* __restrict__ ph,
-                                                          int ldp, const cx<double>* __restrict__ rhok,
-                                                          int ksplit, int nRr, cx<double>* __restrict__ part) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void k_phase_contract_s(int NN, int ne, int Nh, int Nkc,
+                                                          const cx<double>* __restrict__ ph, int ldp,
+                                                          const cx<double>* __restrict__ rhok, int ksplit, int nRr,
+                                                          cx<double>* __restrict__ part) {
+    const int eo = blockIdx.x * 256 + threadIdx.x;  // output slot
+    int e = eo;
+    if (Nh > 0 && eo < ne) {
+        int r, c;
+        upper_rc(eo, r, c);
+        e = c * Nh + r;
+    }
     const int s = blockIdx.y;
     const int S = gridDim.y;
     const int rg0 = blockIdx.z * RG;
@@ -51,7 +71,7 @@ __global__ __launch_bounds__(256) void k_phase_contract_s(int NN, int Nkc, const
     cx<double> acc[RG];
 #pragma unroll
     for (int r = 0; r < RG; ++r) acc[r] = {0.0, 0.0};
-    if (e < NN) {
+    if (eo < ne) {
         // slot j holds k-point min(k + j, k1 - 1); the loads are unconditional (clamped
         // index) so the waits can count them
         constexpr int D = 4;
@@ -80,8 +100,21 @@ __global__ __launch_bounds__(256) void k_phase_contract_s(int NN, int Nkc, const
         }
 #pragma unroll
         for (int r = 0; r < RG; ++r)
-            if (r < nr) part[((size_t)(blockIdx.z * S + s) * RG + r) * NN + e] = acc[r];
+            if (r < nr) part[((size_t)(blockIdx.z * S + s) * RG + r) * ne + eo] = acc[r];
     }
+}
+
+// neg[r] = the index of -R_r in the key list, -1 when absent.
+__global__ void k_neg_keys(int nRr, int d, const int* __restrict__ Rr, int* __restrict__ neg) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nRr) return;
+    int found = -1;
+    for (int q = 0; q < nRr && found < 0; ++q) {
+        bool m = true;
+        for (int a = 0; a < d; ++a) m = m && Rr[q * d + a] == -Rr[r * d + a];
+        if (m) found = q;
+    }
+    neg[r] = found;
 }
 
 // A = Hermitian(H(k)), H(k) = sum_R T_R e^{2 pi i k.R} (full storage, as k_hk_full): thread
@@ -157,6 +190,50 @@ __global__ void k_phase_fold(int NN, int nRr, int S, const cx<double>* __restric
         acc.im += v.im;
     }
     rho[idx] = acc;
+}
+
+// The fold of the Hermitian mode: thread (key, eu) sums the partials of upper element
+// eu = (r, c), r <= c, of key in split order (row length nu = N(N+1)/2, coalesced reads) and
+// adds the sum to rho[key](r, c) and, for r < c, its conjugate to rho[neg[key]](c, r); each
+// element of every key is written by exactly one thread.
+template <int RG>
+__global__ void k_phase_fold_h(int N, int nRr, int S, const int* __restrict__ neg,
+                               const cx<double>* __restrict__ part, cx<double>* __restrict__ rho) {
+    const size_t NN = (size_t)N * N;
+    const size_t nu = (size_t)N * (N + 1) / 2;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)nRr * nu) return;
+    const int key = (int)(idx / nu);
+    const int eu = (int)(idx % nu);
+    int r, c;
+    upper_rc(eu, r, c);
+    const int g = key / RG, rr = key % RG;
+    const cx<double>* pp = part + ((size_t)g * S * RG + rr) * nu + eu;  // split s at pp[s * RG * nu]
+    const size_t step = (size_t)RG * nu;
+    cx<double> sum = {0.0, 0.0};
+    int s = 0;
+    for (; s + 8 <= S; s += 8) {
+        cx<double> v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = pp[(size_t)(s + q) * step];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            sum.re += v[q].re;
+            sum.im += v[q].im;
+        }
+    }
+    for (; s < S; ++s) {
+        const cx<double> v = pp[(size_t)s * step];
+        sum.re += v.re;
+        sum.im += v.im;
+    }
+    cx<double>* up = rho + (size_t)key * NN + (size_t)c * N + r;
+    *up = {up->re + sum.re, up->im + sum.im};
+    const int kn = neg[key];
+    if (r < c && kn >= 0) {  // kn < 0 cannot occur: the host takes this mode only for closed key sets
+        cx<double>* lo = rho + (size_t)kn * NN + (size_t)r * N + c;
+        *lo = {lo->re + sum.re, lo->im - sum.im};
+    }
 }
 
 // Sum of n doubles into out[0] (+= when accumulate), one block, fixed order.

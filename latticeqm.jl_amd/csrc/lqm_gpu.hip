@@ -72,7 +72,7 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_LNK, W_LNKO, W_RB, W_ROCC, W_RA, W_RC, W_RP, W_SPB, W_SPPH, W_SPM, W_SPT, W_SPI, W_SPC, W_SPKS, W_SMV, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_LNK, W_LNKO, W_RB, W_ROCC, W_RA, W_RC, W_RP, W_SPB, W_SPPH, W_SPM, W_SPT, W_SPI, W_SPC, W_SPKS, W_SMV, W_RRNEG, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -147,6 +147,7 @@ struct lqm_context {
     void* progress_user = nullptr;
     int forced[VK_N] = {-1, -1, -1, -1, -1, -1, -1};
     int timing = 0;
+    int rr_pairs = 0;  // the rho keys of the current pass are closed under R -> -R
     std::vector<TimedLaunch> pending;
     double stage_ms[ST_N] = {};
     long long stage_n[ST_N] = {};
@@ -747,23 +748,43 @@ int contract_group(int nRr) {
 
 // rho(R) += sum_k w_k e^{-2 pi i k.R} rho_k over a chunk: the phase table of the chunk's
 // k-points and keys (groups x RG columns), the split-K contraction, the fixed-order fold.
-lqm_status launch_contract(int RG, lqm_context* c, int NN, int Kc, int d, const double* ks, const double* kw,
+// With a key set closed under R -> -R (c->rr_pairs) the contraction forms the upper triangle
+// only and the fold takes the lower one from rho(-R) (half the reads and multiply-adds).
+lqm_status launch_contract(int RG, lqm_context* c, int N, int Kc, int d, const double* ks, const double* kw,
                            double wuni, int nRr, const int* Rr, const cx<double>* rhok, int ksplit, int S,
                            int groups, cx<double>* part, cx<double>* rho) {
     hipStream_t s = c->stream;
+    const int NN = N * N;
     const int ldp = groups * RG;
     cx<double>* ph;
     lqm_status st = wsT(c, W_PHC, (size_t)Kc * ldp, &ph);
     if (st) return st;
-    dim3 grid((NN + 255) / 256, S, groups);
+    const bool herm = c->rr_pairs && nRr > 0;
+    int* neg = nullptr;
+    int ne = NN, Nh = 0;
+    if (herm) {
+        if ((st = wsT(c, W_RRNEG, (size_t)nRr, &neg))) return st;
+        hipLaunchKernelGGL(lqm::k_neg_keys, dim3((nRr + 63) / 64), dim3(64), 0, s, nRr, d, Rr, neg);
+        ne = N * (N + 1) / 2;
+        Nh = N;
+        // more k-splits over the smaller element range (the grid stays as wide), within part's
+        // size (groups x S x RG x NN)
+        const long long Smax = std::min<long long>((long long)S * NN / ne, Kc);
+        const int tilesU = (ne + 255) / 256;
+        const int Sw = std::max(1, std::min(std::max(1, Kc / kContractMinK), contract_splits(tilesU * groups)));
+        const int S2 = (int)std::max<long long>(1, std::min<long long>(Smax, std::max(S, Sw)));
+        ksplit = (Kc + S2 - 1) / S2;
+        S = (Kc + ksplit - 1) / ksplit;
+    }
+    dim3 grid((ne + 255) / 256, S, groups);
     {
         StageScope sc(c, ST_CONTRACT, Kc);
         const size_t nph = (size_t)Kc * ldp;
         hipLaunchKernelGGL(lqm::k_phase_table, dim3((unsigned)((nph + 255) / 256)), dim3(256), 0, s, Kc, d, ks, kw,
                            wuni, nRr, ldp, Rr, -1.0, ph);
-#define LQM_CONTRACT(G)                                                                                   \
-    hipLaunchKernelGGL(lqm::k_phase_contract_s<G>, grid, dim3(256), 0, s, NN, Kc, ph, ldp, rhok, ksplit, nRr, \
-                       part)
+#define LQM_CONTRACT(G)                                                                                        \
+    hipLaunchKernelGGL(lqm::k_phase_contract_s<G>, grid, dim3(256), 0, s, NN, ne, Nh, Kc, ph, ldp, rhok, ksplit, \
+                       nRr, part)
         switch (RG) {
             case 1: LQM_CONTRACT(1); break;
             case 4: LQM_CONTRACT(4); break;
@@ -777,20 +798,28 @@ lqm_status launch_contract(int RG, lqm_context* c, int NN, int Kc, int d, const 
         }
 #undef LQM_CONTRACT
     }
-    const size_t total = (size_t)nRr * NN;
+    const size_t total = (size_t)nRr * (herm ? (size_t)ne : (size_t)NN);
     StageScope sc(c, ST_FOLD);
     const dim3 gf((unsigned)((total + 255) / 256));
+#define LQM_FOLD(G)                                                                                       \
+    do {                                                                                                  \
+        if (herm)                                                                                         \
+            hipLaunchKernelGGL(lqm::k_phase_fold_h<G>, gf, dim3(256), 0, s, N, nRr, S, neg, part, rho); \
+        else                                                                                              \
+            hipLaunchKernelGGL(lqm::k_phase_fold<G>, gf, dim3(256), 0, s, NN, nRr, S, part, rho);       \
+    } while (0)
     switch (RG) {
-        case 1: hipLaunchKernelGGL(lqm::k_phase_fold<1>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
-        case 4: hipLaunchKernelGGL(lqm::k_phase_fold<4>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
-        case 8: hipLaunchKernelGGL(lqm::k_phase_fold<8>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
-        case 9: hipLaunchKernelGGL(lqm::k_phase_fold<9>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
-        case 16: hipLaunchKernelGGL(lqm::k_phase_fold<16>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
-        case 24: hipLaunchKernelGGL(lqm::k_phase_fold<24>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
-        case 25: hipLaunchKernelGGL(lqm::k_phase_fold<25>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
-        case 28: hipLaunchKernelGGL(lqm::k_phase_fold<28>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
-        default: hipLaunchKernelGGL(lqm::k_phase_fold<32>, gf, dim3(256), 0, s, NN, nRr, S, part, rho); break;
+        case 1: LQM_FOLD(1); break;
+        case 4: LQM_FOLD(4); break;
+        case 8: LQM_FOLD(8); break;
+        case 9: LQM_FOLD(9); break;
+        case 16: LQM_FOLD(16); break;
+        case 24: LQM_FOLD(24); break;
+        case 25: LQM_FOLD(25); break;
+        case 28: LQM_FOLD(28); break;
+        default: LQM_FOLD(32); break;
     }
+#undef LQM_FOLD
     LQM_HIP(hipGetLastError());
     return LQM_OK;
 }
@@ -935,6 +964,30 @@ lqm_status run_bands(lqm_model* m, int Nk, const double* ks, double* bands, doub
 // occ (device, N x Nk, may be null): occupations of the ascending eigenvalues used in
 // place of fermidirac(e - mu; T) -- rho then is sum_k w_k e^{-2 pi i k.R} U diag(occ_k) U^H.
 // dRr_dev (optional): the keys already on the device (no upload: the SCF's captured passes).
+// Whether the key list (host, d components each) has no repeated key and holds the negation
+// of every key: then R -> -R is a bijection of the list, which the Hermitian contraction needs.
+bool keys_closed_under_negation(int nRr, int d, const int32_t* Rr) {
+    if (nRr <= 0 || !Rr) return false;
+    for (int r = 0; r < nRr; ++r) {
+        int found = 0;
+        for (int q = 0; q < nRr; ++q) {
+            bool neg = true, same = q != r;
+            for (int a = 0; a < d; ++a) {
+                neg = neg && Rr[q * d + a] == -Rr[r * d + a];
+                same = same && Rr[q * d + a] == Rr[r * d + a];
+            }
+            if (same) return false;
+            found += neg;
+        }
+        if (found != 1) return false;
+    }
+    return true;
+}
+struct PairsScope {
+    lqm_context* c;
+    ~PairsScope() { c->rr_pairs = 0; }
+};
+
 lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, double mu, double T,
                    int nRr, const int32_t* Rr, double* rho, double* eband, double* bands_out,
                    const double* occ = nullptr, const MuReq* mq = nullptr, const int* dRr_dev = nullptr) {
@@ -946,6 +999,8 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
     if (nRr < 0 || (nRr > 0 && !Rr)) return fail(LQM_ERR_ARG, "bad rho key list");
     if (!ks || !rho || !eband) return fail(LQM_ERR_ARG, "null ks, rho or eband");
     const size_t NN = (size_t)N * N;
+    c->rr_pairs = keys_closed_under_negation(nRr, d, Rr);
+    PairsScope pairs_scope{c};
     zero_dev(c, rho, sizeof(cx<double>) * NN * std::max(nRr, 0));
     zero_dev(c, eband, sizeof(double));
     const int* dRr = dRr_dev;
@@ -1039,7 +1094,7 @@ lqm_status run_rho(lqm_model* m, int Nk, const double* ks, const double* kw, dou
             const int ksplit = (Kc + Sc - 1) / Sc;
             const int Sused = (Kc + ksplit - 1) / ksplit;
             auto* rh = reinterpret_cast<cx<double>*>(rho);
-            if ((st = launch_contract(RG, c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh))) return st;
+            if ((st = launch_contract(RG, c, N, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rh))) return st;
             LQM_HIP(hipGetLastError());
         }
     }
@@ -1299,7 +1354,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
                 const int Sc = std::min(S, Kc);
                 const int ksplit = (Kc + Sc - 1) / Sc;
                 const int Sused = (Kc + ksplit - 1) / ksplit;
-                if ((st = launch_contract(RG, c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho))) return st;
+                if ((st = launch_contract(RG, c, N, Kc, d, ksc, kwc, wuni, nRr, dRr, Sm, ksplit, Sused, groups, part, rho))) return st;
             }
             LQM_HIP(hipGetLastError());
         }
@@ -1593,7 +1648,7 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
         const int Sc = std::min(S, Kc);
         const int ksplit = (Kc + Sc - 1) / Sc;
         const int Sused = (Kc + ksplit - 1) / ksplit;
-        if ((st = launch_contract(RG, c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho))) return st;
+        if ((st = launch_contract(RG, c, N, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho))) return st;
         LQM_HIP(hipGetLastError());
     }
     return check_status(c, "blocked eigensolver");
@@ -1906,7 +1961,7 @@ lqm_status run_blocked_mixed(lqm_model* m, int Nk, const double* ks, const doubl
         const int Sc = std::min(S, Kc);
         const int ksplit = (Kc + Sc - 1) / Sc;
         const int Sused = (Kc + ksplit - 1) / ksplit;
-        if ((st = launch_contract(RG, c, (int)NN, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho)))
+        if ((st = launch_contract(RG, c, N, Kc, d, ksc, kwc, wuni, nRr, dRr, rhok, ksplit, Sused, groups, part, rho)))
             return st;
         LQM_HIP(hipGetLastError());
     }

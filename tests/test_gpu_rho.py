@@ -251,3 +251,52 @@ def test_small_models_one_thread_per_k(pkg, oracle, N):
             for r, key in enumerate(h.keys()):
                 assert np.max(np.abs(rho[key] - ref[:, :, r])) < 1e-12, (mu, Tk, key)
             assert abs(eb - ebr) < 1e-12 * max(1.0, abs(ebr))
+
+
+def _rho_keys(pkg, h, ks, keys, mu, T, ex):
+    # lqm_densitymatrix on an explicit key list (repeats allowed), through the C ABI
+    import ctypes
+
+    api = pkg.api
+    m = api._model(h, ex)
+    k = api._kpts(ks, m.d)
+    w = api._weights(ks, None, k.shape[1])
+    Rr = np.asfortranarray(np.array(keys, dtype=np.int32).T)
+    rho = np.zeros((m.N, m.N, len(keys)), dtype=np.complex128, order="F")
+    eb = ctypes.c_double()
+    api._lib.call("lqm_densitymatrix", m.handle, k.shape[1], api._vp(k), api._vp(w), float(mu), float(T),
+                  len(keys), api._vp(Rr), api._vp(rho), ctypes.byref(eb))
+    return rho, eb.value
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("N", [3, 7, 33, 64, 96])
+def test_rho_hermitian_contraction(pkg, oracle, prec, N):
+    # a key list closed under R -> -R (no repeats) is contracted over the upper triangle with the
+    # lower one taken from rho(-R); one key without its negation, or a repeated key, takes the
+    # full contraction.  All three vs the oracle; the shared keys agree between the modes
+    h = pkg.models.random_tb(N, rmax=1, seed=910 + N)
+    ks = pkg.regulargrid(nk=16 if N < 90 else 9)
+    ex = pkg.api.to_executor("gpu", precision=prec)
+    closed = list(h.keys())
+    lists = {"closed": closed, "extra": closed + [(2, -1)], "repeat": closed + [closed[1]]}
+    R, Tt = h.pack()
+    out = {}
+    for name, keys in lists.items():
+        rho, eb = _rho_keys(pkg, h, ks, keys, 0.05, 0.02, ex)
+        ref, eb_ref = oracle.densitymatrix(R, Tt, ks, np.array(keys, dtype=np.int32).T, mu=0.05, Temp=0.02)
+        assert np.max(np.abs(rho - ref)) < 1e-9, name
+        assert abs(eb - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
+        out[name] = rho
+    n = len(closed)
+    scale = np.max(np.abs(out["closed"]))
+    for name in ("extra", "repeat"):
+        assert np.max(np.abs(out[name][:, :, :n] - out["closed"])) < 1e-13 * max(1.0, scale), name
+    assert np.array_equal(out["repeat"][:, :, n], out["repeat"][:, :, 1])
+    # the closed list's result: rho(-R) = rho(R)^H, exactly off the diagonal (one sum feeds both)
+    off = ~np.eye(N, dtype=bool)
+    for r, key in enumerate(closed):
+        q = closed.index(tuple(-x for x in key))
+        A, B = out["closed"][:, :, r], out["closed"][:, :, q].conj().T
+        assert np.array_equal(A[off], B[off])
+        assert np.max(np.abs(A - B)) < 1e-14 * max(1.0, scale)
