@@ -1246,7 +1246,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
                         StageScope sc(c, ST_OA2, nc);
                         const size_t sh = lqm::oa_lds_bytes(N);
                         allow_lds(lqm::k_oa_correct, sh);
-                        hipLaunchKernelGGL(lqm::k_oa_correct, dim3(nc), dim3(256), sh, c->stream, N, Sm, G, W,
+                        hipLaunchKernelGGL(lqm::k_oa_correct, dim3(nc), dim3(lqm::kOaT), sh, c->stream, N, Sm, G, W,
                                            lam, stat, c->refine_thr, dk2);
                     }
                     LQM_HIP(hipGetLastError());
@@ -1264,7 +1264,7 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
                 StageScope sc(c, kidx ? ST_OA2 : ST_OA, nsub);
                 const size_t sh = lqm::oa_lds_bytes(N);
                 allow_lds(lqm::k_oa_correct, sh);
-                hipLaunchKernelGGL(lqm::k_oa_correct, dim3(nsub), dim3(256), sh, c->stream, N, Sm, G, W, lam,
+                hipLaunchKernelGGL(lqm::k_oa_correct, dim3(nsub), dim3(lqm::kOaT), sh, c->stream, N, Sm, G, W, lam,
                                    stat, c->refine_thr, kidx);
             }
             LQM_HIP(hipGetLastError());
@@ -1843,7 +1843,7 @@ lqm_status run_blocked_mixed(lqm_model* m, int Nk, const double* ks, const doubl
                                                           (long long)NN, one, zero)));  // G = X^H X
             {
                 StageScope sc(c, stg);
-                hipLaunchKernelGGL(lqm::k_oa_correct, dim3(nsub), dim3(256), shc, s, N, Sm, Gm, Wm, lam, stat,
+                hipLaunchKernelGGL(lqm::k_oa_correct, dim3(nsub), dim3(lqm::kOaT), shc, s, N, Sm, Gm, Wm, lam, stat,
                                    c->refine_thr, kidx, Ug);
                 // X <- X + X E (E in W), formed in G
                 if (!kidx) {

@@ -220,10 +220,12 @@ __host__ __device__ inline int oa_max_clusters_global(int N) { return N / 2 + 1;
 constexpr int kWideMax = 64;
 constexpr int kWideLen = 1024;
 constexpr double kSplitCoupling = 1.0 / 64.0;
+constexpr int kOaT = 512;            // threads of k_oa_correct
+constexpr int kOaW = kOaT / 64;      // its waves
 inline size_t oa_lds_bytes(int N, bool global_u = false) {
     const int maxcl = global_u ? oa_max_clusters_global(N) : oa_max_clusters(N);
     const size_t o_red = (32 * (size_t)N + 8 * (size_t)maxcl + 8 * (size_t)kWideMax + 15) & ~(size_t)15;
-    return o_red + 16 * 8 + (global_u ? 0 : (size_t)maxcl * kClusterMax * kClusterMax * 16);
+    return o_red + 4 * kOaW * 8 + (global_u ? 0 : (size_t)maxcl * kClusterMax * kClusterMax * 16);
 }
 
 __device__ void jacobi_herm(int c, cx<double>* K, cx<double>* U) {
@@ -337,7 +339,7 @@ __device__ __forceinline__ double oa_coupling(int N, int p, int q, const double*
 // (columns arrive in ascending FP32 eigenvalue order).  For a cluster C the
 // orthonormalised block K = (I + R/2)^H S_CC (I + R/2) is diagonalised, K = U L U^H,
 // E_CC = (I + R_CC/2) U - I, lambda_C = L, and E_iC (i outside C) = E^std_iC U.
-__global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restrict__ S,
+__global__ __launch_bounds__(kOaT) void k_oa_correct(int N, cx<double>* __restrict__ S,
                                                     cx<double>* __restrict__ G,
                                                     cx<double>* __restrict__ E,
                                                     double* __restrict__ lambda,
@@ -351,7 +353,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
     const size_t o_cst = o_reach + 4 * (size_t)N, o_csz = o_cst + 4 * (size_t)maxcl;
     const size_t o_wide = o_csz + 4 * (size_t)maxcl;
     const size_t o_red = (o_wide + 8 * (size_t)kWideMax + 15) & ~(size_t)15;
-    const size_t o_u = o_red + 16 * 8;
+    const size_t o_u = o_red + 4 * kOaW * 8;
     double* lam = reinterpret_cast<double*>(smem);               // [N]  Rayleigh quotients
     double* lnew = reinterpret_cast<double*>(smem + o_lnew);     // [N]  final lambda
     int* cid = reinterpret_cast<int*>(smem + o_cid);             // [N]  cluster ordinal or -1
@@ -361,7 +363,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
     int* grp = reinterpret_cast<int*>(smem + o_grp);             // [N]  sub-cluster of a split chain, or -1
     int* reach = reinterpret_cast<int*>(smem + o_reach);         // [N]  last column coupled to this one
     int* wide = reinterpret_cast<int*>(smem + o_wide);           // [2 kWideMax] wide chains (start, end)
-    double* red = reinterpret_cast<double*>(smem + o_red);       // [16]
+    double* red = reinterpret_cast<double*>(smem + o_red);       // [4 kOaW]
     const int kk = kidx ? kidx[blockIdx.x] : (int)blockIdx.x;
     cx<double>* Ubuf = Ug ? Ug + (size_t)kk * maxcl * kClusterMax * kClusterMax
                           : reinterpret_cast<cx<double>*>(smem + o_u);  // [maxcl][kClusterMax^2]
@@ -375,7 +377,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
     // first-order corrections then satisfy E_ij + conj(E_ji) = r_ij exactly, whatever the
     // rounding of X^H A X and X^H X; otherwise a near-degenerate pair turns that rounding
     // (about eps ||A||) divided by its gap into a loss of orthogonality
-    for (int e = tid; e < N * N; e += 256) {
+    for (int e = tid; e < N * N; e += kOaT) {
         const int i = e % N, j = e / N;
         if (i < j) {
             cx<double>* M[2] = {Sk, Gk};
@@ -392,7 +394,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
         }
     }
     __syncthreads();
-    for (int i = tid; i < N; i += 256) {
+    for (int i = tid; i < N; i += kOaT) {
         lam[i] = Sk[(size_t)i * N + i].re / Gk[(size_t)i * N + i].re;
         lnew[i] = lam[i];
         cid[i] = -1;
@@ -400,15 +402,15 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
     }
     __syncthreads();
     double r2 = 0.0, amax = 0.0;
-    for (int e = tid; e < N * N; e += 256) {
+    for (int e = tid; e < N * N; e += kOaT) {
         const int i = e % N, j = e / N;
         const cx<double> g = Gk[e];
         const double rre = (i == j ? 1.0 : 0.0) - g.re, rim = -g.im;
         r2 += rre * rre + rim * rim;
     }
-    for (int i = tid; i < N; i += 256) amax = fmax(amax, fabs(lam[i]));
+    for (int i = tid; i < N; i += kOaT) amax = fmax(amax, fabs(lam[i]));
     double s2all = 0.0;
-    for (int e = tid; e < N * N; e += 256) {
+    for (int e = tid; e < N * N; e += kOaT) {
         const int i = e % N, j = e / N;
         cx<double> sv = Sk[e];
         if (i == j) sv.re -= lam[i];
@@ -420,13 +422,19 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
     for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_xor(amax, o));
     if ((tid & 63) == 0) {
         red[tid / 64] = r2;
-        red[4 + tid / 64] = s2all;
-        red[8 + tid / 64] = amax;
+        red[kOaW + tid / 64] = s2all;
+        red[2 * kOaW + tid / 64] = amax;
     }
     __syncthreads();
-    const double rn = sqrt(red[0] + red[1] + red[2] + red[3]);
-    const double sn_all = sqrt(red[4] + red[5] + red[6] + red[7]);
-    const double an = fmax(fmax(red[8], red[9]), fmax(red[10], red[11]));
+    double rsum = 0.0, ssum = 0.0, an = 0.0;
+#pragma unroll
+    for (int w = 0; w < kOaW; ++w) {
+        rsum += red[w];
+        ssum += red[kOaW + w];
+        an = fmax(an, red[2 * kOaW + w]);
+    }
+    const double rn = sqrt(rsum);
+    const double sn_all = sqrt(ssum);
     const double delta = 2.0 * (sn_all + an * rn);
     const double tiny = 64.0 * 2.220446049250313e-16 * (an > 0.0 ? an : 1.0);
     // cluster detection and Rayleigh-Ritz (one thread: clusters are rare and small); chains
@@ -454,9 +462,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
                 } else if (c > kClusterMax || ncl >= maxcl) {
                     if (spread > tiny) flags |= 2;
                 } else {
-                    oa_rayleigh_ritz(N, i, c, ncl, Sk, Gk, lnew, cid, cpos,
-                                                        Ubuf + (size_t)ncl * kClusterMax * kClusterMax);
-                    cstart[ncl] = i;
+                    cstart[ncl] = i;  // Rayleigh-Ritz below, one thread per cluster
                     csize[ncl] = c;
                     ++ncl;
                     if (spread > tiny) flags |= 1;
@@ -474,7 +480,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
         // kSplitCoupling (p itself when none), one thread per column
         for (int w = 0; w < s_nwide; ++w) {
             const int i0 = wide[2 * w], j0 = wide[2 * w + 1];
-            for (int p = i0 + tid; p <= j0; p += 256) {
+            for (int p = i0 + tid; p <= j0; p += kOaT) {
                 int rq = p;
                 for (int q = j0; q > p; --q)
                     if (!(oa_coupling(N, p, q, lam, Sk, Gk) <= kSplitCoupling)) {
@@ -505,8 +511,6 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
                         if (c > kClusterMax || ncl >= maxcl) {
                             if (spread > tiny) flags |= 2;
                         } else {
-                            oa_rayleigh_ritz(N, b0, c, ncl, Sk, Gk, lnew, cid, cpos,
-                                                                Ubuf + (size_t)ncl * kClusterMax * kClusterMax);
                             cstart[ncl] = b0;
                             csize[ncl] = c;
                             ++ncl;
@@ -522,6 +526,11 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
         }
         __syncthreads();
     }
+    // Rayleigh-Ritz of the clusters found above, one thread each (disjoint columns and slots)
+    for (int cl = tid; cl < s_ncl; cl += kOaT)
+        oa_rayleigh_ritz(N, cstart[cl], csize[cl], cl, Sk, Gk, lnew, cid, cpos,
+                         Ubuf + (size_t)cl * kClusterMax * kClusterMax);
+    __syncthreads();
     // pair (i, j) takes the first-order correction when its Rayleigh quotients are more than
     // delta apart or it lies in different sub-clusters of a split chain
     auto separated = [&](int a, int b) {
@@ -529,7 +538,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
     };
     // E entries
     double emax = 0.0;
-    for (int e = tid; e < N * N; e += 256) {
+    for (int e = tid; e < N * N; e += kOaT) {
         const int i = e % N, j = e / N;
         const int ci = cid[i], cj = cid[j];
         cx<double> out;
@@ -571,12 +580,15 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o));
     __syncthreads();
-    if ((tid & 63) == 0) red[12 + tid / 64] = emax;
+    if ((tid & 63) == 0) red[3 * kOaW + tid / 64] = emax;
     // Second-order (Rayleigh-Schroedinger) correction of every eigenvalue from the pencil (S, G):
     //   lambda_j += sum_{i separated from j} |s_ij - lambda_j g_ij|^2 / (lambda_j - lambda_i)
     // (for a rotated cluster column, its Rayleigh-Ritz value and rotated couplings), so a single
     // sweep leaves an eigenvalue error of third order in the FP32 error.
-    for (int j = tid; j < N; j += 256) {
+    // one wave per column j, its lanes over i (coalesced reads of column j of S and G)
+    const int cw = tid >> 6, cl_lane = tid & 63;
+    for (int j = cw; j < N; j += kOaW) {
+        double corr = 0.0;
         if (cid[j] >= 0) {
             // a rotated cluster column x~_a = sum_l x_{c0+l} BU_la: the same correction with its
             // rotated couplings t_i = sum_l (s_{i,c0+l} - l~_a g_{i,c0+l}) BU_la to the columns
@@ -584,8 +596,7 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
             const int cj = cid[j], c0 = cstart[cj], cs = csize[cj], a = cpos[j];
             const cx<double>* BU = Ubuf + (size_t)cj * kClusterMax * kClusterMax;
             const double lj = lnew[j];
-            double corr = 0.0;
-            for (int i = 0; i < N; ++i) {
+            for (int i = cl_lane; i < N; i += 64) {
                 if (cid[i] == cj || !separated(i, j)) continue;
                 cx<double> t = {0.0, 0.0};
                 for (int l = 0; l < cs; ++l) {
@@ -594,24 +605,27 @@ __global__ __launch_bounds__(256) void k_oa_correct(int N, cx<double>* __restric
                 }
                 corr += abs2(t) / (lj - lam[i]);
             }
-            lnew[j] = lj + corr;
+            corr = wave_sum(corr);
+            if (cl_lane == 0) lnew[j] = lj + corr;
             continue;
         }
         const double lj = lam[j];
-        double corr = 0.0;
-        for (int i = 0; i < N; ++i) {
+        for (int i = cl_lane; i < N; i += 64) {
             const double gap = lj - lam[i];
             if (i == j || !separated(i, j)) continue;
             const cx<double> sv = Sk[(size_t)j * N + i], g = Gk[(size_t)j * N + i];
             const double tre = sv.re - lj * g.re, tim = sv.im - lj * g.im;
             corr += (tre * tre + tim * tim) / gap;
         }
-        lnew[j] = lj + corr;
+        corr = wave_sum(corr);
+        if (cl_lane == 0) lnew[j] = lj + corr;
     }
     __syncthreads();
-    for (int i = tid; i < N; i += 256) lambda[(size_t)kk * N + i] = lnew[i];
+    for (int i = tid; i < N; i += kOaT) lambda[(size_t)kk * N + i] = lnew[i];
     if (tid == 0) {
-        const double em = sqrt(fmax(fmax(red[12], red[13]), fmax(red[14], red[15])));
+        double em2 = 0.0;
+        for (int w = 0; w < kOaW; ++w) em2 = fmax(em2, red[3 * kOaW + w]);
+        const double em = sqrt(em2);
         const double lead = fmax(em, rn);
         int flags = s_flags;
         if ((flags & 3) || thr <= 0.0 || lead * lead > thr) flags |= 4;
