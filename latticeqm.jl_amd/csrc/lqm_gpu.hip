@@ -1837,10 +1837,16 @@ lqm_status run_blocked_mixed(lqm_model* m, int Nk, const double* ks, const doubl
             };
             zgemm_gen<0, 0>(c, stg, nsub, nsub, gg(gemm_args(N, N, N, A, N, (long long)NN, X, N, (long long)NN, Wm, N,
                                                              (long long)NN, one, zero)));  // W = A X
-            zgemm_gen<1, 0>(c, stg, 0, nsub, gg(gemm_args(N, N, N, X, N, (long long)NN, Wm, N, (long long)NN, Sm, N,
-                                                          (long long)NN, one, zero)));  // S = X^H A X
-            zgemm_gen<1, 0>(c, stg, 0, nsub, gg(gemm_args(N, N, N, X, N, (long long)NN, X, N, (long long)NN, Gm, N,
-                                                          (long long)NN, one, zero)));  // G = X^H X
+            // S and G are Hermitian: only their tiles on or below the diagonal are formed;
+            // k_oa_correct completes them from the lower triangle
+            auto lower = [](lqm::ZGemmArgs g) {
+                g.lower = 1;
+                return g;
+            };
+            zgemm_gen<1, 0>(c, stg, 0, nsub, lower(gg(gemm_args(N, N, N, X, N, (long long)NN, Wm, N, (long long)NN, Sm,
+                                                                N, (long long)NN, one, zero))));  // S = X^H A X
+            zgemm_gen<1, 0>(c, stg, 0, nsub, lower(gg(gemm_args(N, N, N, X, N, (long long)NN, X, N, (long long)NN, Gm,
+                                                                N, (long long)NN, one, zero))));  // G = X^H X
             {
                 StageScope sc(c, stg);
                 hipLaunchKernelGGL(lqm::k_oa_correct, dim3(nsub), dim3(lqm::kOaT), shc, s, N, Sm, Gm, Wm, lam, stat,

@@ -327,7 +327,8 @@ __device__ __forceinline__ double oa_coupling(int N, int p, int q, const double*
 }
 
 // Ogita-Aishima correction with Rayleigh-Ritz on near-degenerate clusters.
-// In: S = X^H A X, G = X^H X (both replaced by their Hermitian parts).  Out: E[kk]
+// In: S = X^H A X, G = X^H X, read from their lower triangles (each replaced by the Hermitian
+// matrix of its lower triangle).  Out: E[kk]
 // (X <- X + X E), lambda[kk*N + i], and
 // stat[kk*4 + {0: ||R||_F, 1: delta, 2: flags, 3: max |E_ij| outside the clusters}] where
 // flags bit 0 = some cluster was rotated, bit 1 = a cluster wider than kClusterMax
@@ -373,48 +374,55 @@ __global__ __launch_bounds__(kOaT) void k_oa_correct(int N, cx<double>* __restri
     cx<double>* Sk = S + (size_t)kk * NN;
     cx<double>* Gk = G + (size_t)kk * NN;
     cx<double>* Ek = E + (size_t)kk * NN;
-    // S and G <- their Hermitian parts, in place (each pair i < j by one thread): the
-    // first-order corrections then satisfy E_ij + conj(E_ji) = r_ij exactly, whatever the
-    // rounding of X^H A X and X^H X; otherwise a near-degenerate pair turns that rounding
-    // (about eps ||A||) divided by its gap into a loss of orthogonality
-    for (int e = tid; e < N * N; e += kOaT) {
-        const int i = e % N, j = e / N;
-        if (i < j) {
-            cx<double>* M[2] = {Sk, Gk};
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const cx<double> a = M[q][(size_t)j * N + i], b = M[q][(size_t)i * N + j];
-                const cx<double> h = {0.5 * (a.re + b.re), 0.5 * (a.im - b.im)};
-                M[q][(size_t)j * N + i] = h;
-                M[q][(size_t)i * N + j] = conj(h);
-            }
-        } else if (i == j) {
-            Sk[e].im = 0.0;
-            Gk[e].im = 0.0;
-        }
-    }
-    __syncthreads();
+    // S and G <- the Hermitian matrices of their lower triangles, in place (the products may
+    // form only the tiles on or below the diagonal): the first-order corrections then satisfy
+    // E_ij + conj(E_ji) = r_ij exactly, whatever the rounding of X^H A X and X^H X; otherwise a
+    // near-degenerate pair turns that rounding (about eps ||A||) divided by its gap into a loss
+    // of orthogonality.  The diagonal first (imaginary parts zeroed, the Rayleigh quotients),
+    // then the strict lower triangle: wave w takes column c = 8g + w of column group g, its
+    // lanes the rows (coalesced reads, four row chunks in flight), and writes the conjugates
+    // to row c above the diagonal -- the eight waves fill the same 128-byte lines of those rows
+    // together.  The norms ||R||_F and ||S - D||_F ride along (an off-diagonal pair counts
+    // twice).
+    double r2 = 0.0, s2all = 0.0, amax = 0.0;
     for (int i = tid; i < N; i += kOaT) {
-        lam[i] = Sk[(size_t)i * N + i].re / Gk[(size_t)i * N + i].re;
+        const size_t d = (size_t)i * N + i;
+        const double sd = Sk[d].re, gd = Gk[d].re;
+        Sk[d].im = 0.0;
+        Gk[d].im = 0.0;
+        lam[i] = sd / gd;
         lnew[i] = lam[i];
         cid[i] = -1;
         grp[i] = -1;
+        r2 += (1.0 - gd) * (1.0 - gd);
+        s2all += (sd - lam[i]) * (sd - lam[i]);
+        amax = fmax(amax, fabs(lam[i]));
     }
-    __syncthreads();
-    double r2 = 0.0, amax = 0.0;
-    for (int e = tid; e < N * N; e += kOaT) {
-        const int i = e % N, j = e / N;
-        const cx<double> g = Gk[e];
-        const double rre = (i == j ? 1.0 : 0.0) - g.re, rim = -g.im;
-        r2 += rre * rre + rim * rim;
-    }
-    for (int i = tid; i < N; i += kOaT) amax = fmax(amax, fabs(lam[i]));
-    double s2all = 0.0;
-    for (int e = tid; e < N * N; e += kOaT) {
-        const int i = e % N, j = e / N;
-        cx<double> sv = Sk[e];
-        if (i == j) sv.re -= lam[i];
-        s2all += abs2(sv);
+    {
+        static_assert(kOaW == 8, "one column per wave of a group of eight");
+        const int w = tid >> 6, ln = tid & 63;
+        for (int c = w; c < N - 1; c += kOaW) {
+            for (int r0 = c + 1; r0 < N; r0 += 4 * 64) {
+                cx<double> vs[4], vg[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const size_t e = (size_t)c * N + min(r0 + 64 * q + ln, N - 1);
+                    vs[q] = Sk[e];
+                    vg[q] = Gk[e];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = r0 + 64 * q + ln;
+                    if (r < N) {
+                        r2 += 2.0 * abs2(vg[q]);
+                        s2all += 2.0 * abs2(vs[q]);
+                        const size_t eu = (size_t)r * N + c;  // element (c, r) above the diagonal
+                        Sk[eu] = conj(vs[q]);
+                        Gk[eu] = conj(vg[q]);
+                    }
+                }
+            }
+        }
     }
     r2 = wave_sum(r2);
     s2all = wave_sum(s2all);
@@ -536,90 +544,102 @@ __global__ __launch_bounds__(kOaT) void k_oa_correct(int N, cx<double>* __restri
     auto separated = [&](int a, int b) {
         return fabs(lam[b] - lam[a]) > delta || (grp[a] >= 0 && grp[b] >= 0 && grp[a] != grp[b]);
     };
-    // E entries
-    double emax = 0.0;
-    for (int e = tid; e < N * N; e += kOaT) {
-        const int i = e % N, j = e / N;
-        const int ci = cid[i], cj = cid[j];
-        cx<double> out;
-        if (cj >= 0 && ci == cj) {
-            const cx<double> bu = Ubuf[(size_t)cj * kClusterMax * kClusterMax + cpos[i] * kClusterMax + cpos[j]];
-            out = {bu.re - (i == j ? 1.0 : 0.0), bu.im};
-        } else if (cj >= 0) {
-            // column j of a rotated cluster: E_ij = sum_l Estd_il BU_lj (l in cluster cj)
-            const int c0 = cstart[cj], cs = csize[cj];
-            out = {0.0, 0.0};
-            for (int l = 0; l < cs; ++l) {
-                const int ll = c0 + l;
-                const cx<double> g = Gk[(size_t)ll * N + i];
-                const double rre = -g.re, rim = -g.im;  // i != ll
-                const double gap = lam[ll] - lam[i];
-                cx<double> est;
-                if (separated(i, ll)) {
-                    const cx<double> sv = Sk[(size_t)ll * N + i];
-                    est = {(sv.re + lam[ll] * rre) / gap, (sv.im + lam[ll] * rim) / gap};
-                } else {
-                    est = {0.5 * rre, 0.5 * rim};
-                }
-                out = cmad(out, est, Ubuf[(size_t)cj * kClusterMax * kClusterMax + l * kClusterMax + cpos[j]]);
-            }
-        } else {
-            const cx<double> g = Gk[e];
-            const double rre = (i == j ? 1.0 : 0.0) - g.re, rim = -g.im;
-            const double gap = lam[j] - lam[i];
-            if (i != j && separated(i, j)) {
-                const cx<double> sv = Sk[e];
-                out = {(sv.re + lam[j] * rre) / gap, (sv.im + lam[j] * rim) / gap};
-            } else {
-                out = {0.5 * rre, 0.5 * rim};
-            }
-        }
-        Ek[e] = out;
-        if (i != j && !(ci >= 0 && ci == cj)) emax = fmax(emax, abs2(out));
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o));
-    __syncthreads();
-    if ((tid & 63) == 0) red[3 * kOaW + tid / 64] = emax;
-    // Second-order (Rayleigh-Schroedinger) correction of every eigenvalue from the pencil (S, G):
+    // E entries and the second-order (Rayleigh-Schroedinger) correction of every eigenvalue
+    // from the pencil (S, G), one wave per column j, its lanes over the rows i (coalesced
+    // reads of column j of S and G, four rows in flight):
     //   lambda_j += sum_{i separated from j} |s_ij - lambda_j g_ij|^2 / (lambda_j - lambda_i)
     // (for a rotated cluster column, its Rayleigh-Ritz value and rotated couplings), so a single
     // sweep leaves an eigenvalue error of third order in the FP32 error.
-    // one wave per column j, its lanes over i (coalesced reads of column j of S and G)
-    const int cw = tid >> 6, cl_lane = tid & 63;
-    for (int j = cw; j < N; j += kOaW) {
-        double corr = 0.0;
-        if (cid[j] >= 0) {
-            // a rotated cluster column x~_a = sum_l x_{c0+l} BU_la: the same correction with its
-            // rotated couplings t_i = sum_l (s_{i,c0+l} - l~_a g_{i,c0+l}) BU_la to the columns
-            // outside the cluster that are separated from it
-            const int cj = cid[j], c0 = cstart[cj], cs = csize[cj], a = cpos[j];
-            const cx<double>* BU = Ubuf + (size_t)cj * kClusterMax * kClusterMax;
-            const double lj = lnew[j];
-            for (int i = cl_lane; i < N; i += 64) {
-                if (cid[i] == cj || !separated(i, j)) continue;
-                cx<double> t = {0.0, 0.0};
-                for (int l = 0; l < cs; ++l) {
-                    const cx<double> sv = Sk[(size_t)(c0 + l) * N + i], g = Gk[(size_t)(c0 + l) * N + i];
-                    t = cmad(t, cx<double>{sv.re - lj * g.re, sv.im - lj * g.im}, BU[l * kClusterMax + a]);
+    double emax = 0.0;
+    {
+        const int cw = tid >> 6, ln = tid & 63;
+        for (int j = cw; j < N; j += kOaW) {
+            const int cj = cid[j];
+            double corr = 0.0;
+            cx<double>* Ej = Ek + (size_t)j * N;
+            if (cj >= 0) {
+                // column j of a rotated cluster (x~_a = sum_l x_{c0+l} BU_la): E_ij = sum_l Estd_il
+                // BU_la, and the correction with the rotated couplings t_i = sum_l (s_{i,c0+l} -
+                // l~_a g_{i,c0+l}) BU_la to the columns outside the cluster separated from it; row i
+                // of the cluster's columns of S and G is loaded at once (the cluster's other columns
+                // find it in L2)
+                const int c0 = cstart[cj], cs = csize[cj], a = cpos[j];
+                const cx<double>* BU = Ubuf + (size_t)cj * kClusterMax * kClusterMax;
+                const double lj = lnew[j];
+                for (int i = ln; i < N; i += 64) {
+                    const int ci = cid[i];
+                    cx<double> out;
+                    if (ci == cj) {
+                        const cx<double> bu = BU[cpos[i] * kClusterMax + a];
+                        out = {bu.re - (i == j ? 1.0 : 0.0), bu.im};
+                    } else {
+                        cx<double> gv[kClusterMax], sv[kClusterMax];
+#pragma unroll
+                        for (int l = 0; l < kClusterMax; ++l)
+                            if (l < cs) {
+                                gv[l] = Gk[(size_t)(c0 + l) * N + i];
+                                sv[l] = Sk[(size_t)(c0 + l) * N + i];
+                            }
+                        out = {0.0, 0.0};
+                        cx<double> t = {0.0, 0.0};
+#pragma unroll
+                        for (int l = 0; l < kClusterMax; ++l)
+                            if (l < cs) {
+                                const int ll = c0 + l;
+                                const double rre = -gv[l].re, rim = -gv[l].im;  // i != ll
+                                const double gap = lam[ll] - lam[i];
+                                const cx<double> est = separated(i, ll)
+                                                           ? cx<double>{(sv[l].re + lam[ll] * rre) / gap,
+                                                                        (sv[l].im + lam[ll] * rim) / gap}
+                                                           : cx<double>{0.5 * rre, 0.5 * rim};
+                                const cx<double> bl = BU[l * kClusterMax + a];
+                                out = cmad(out, est, bl);
+                                t = cmad(t, cx<double>{sv[l].re - lj * gv[l].re, sv[l].im - lj * gv[l].im}, bl);
+                            }
+                        if (separated(i, j)) corr += abs2(t) / (lj - lam[i]);
+                        emax = fmax(emax, abs2(out));
+                    }
+                    Ej[i] = out;
                 }
-                corr += abs2(t) / (lj - lam[i]);
+                corr = wave_sum(corr);
+                if (ln == 0) lnew[j] = lj + corr;
+                continue;
+            }
+            const double lj = lam[j];
+            for (int i0 = ln; i0 < N; i0 += 4 * 64) {
+                cx<double> gq[4], sq[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const size_t e = (size_t)j * N + min(i0 + 64 * q, N - 1);
+                    gq[q] = Gk[e];
+                    sq[q] = Sk[e];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + 64 * q;
+                    if (i >= N) continue;
+                    const cx<double> g = gq[q], sv = sq[q];
+                    const double rre = (i == j ? 1.0 : 0.0) - g.re, rim = -g.im;
+                    const double gap = lj - lam[i];
+                    cx<double> out;
+                    if (i != j && separated(i, j)) {
+                        out = {(sv.re + lj * rre) / gap, (sv.im + lj * rim) / gap};
+                        const double tre = sv.re - lj * g.re, tim = sv.im - lj * g.im;
+                        corr += (tre * tre + tim * tim) / gap;
+                    } else {
+                        out = {0.5 * rre, 0.5 * rim};
+                    }
+                    Ej[i] = out;
+                    if (i != j) emax = fmax(emax, abs2(out));
+                }
             }
             corr = wave_sum(corr);
-            if (cl_lane == 0) lnew[j] = lj + corr;
-            continue;
+            if (ln == 0) lnew[j] = lj + corr;
         }
-        const double lj = lam[j];
-        for (int i = cl_lane; i < N; i += 64) {
-            const double gap = lj - lam[i];
-            if (i == j || !separated(i, j)) continue;
-            const cx<double> sv = Sk[(size_t)j * N + i], g = Gk[(size_t)j * N + i];
-            const double tre = sv.re - lj * g.re, tim = sv.im - lj * g.im;
-            corr += (tre * tre + tim * tim) / gap;
-        }
-        corr = wave_sum(corr);
-        if (cl_lane == 0) lnew[j] = lj + corr;
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o));
+    if ((tid & 63) == 0) red[3 * kOaW + tid / 64] = emax;
     __syncthreads();
     for (int i = tid; i < N; i += kOaT) lambda[(size_t)kk * N + i] = lnew[i];
     if (tid == 0) {
