@@ -1425,6 +1425,7 @@ lqm::GemmArgs<R> gemm_args(int M, int N, int K, const cx<R>* A, int lda, long lo
 // zhetrd (lower) of Kc full Hermitian matrices A with zlatrd panels: d, e, tau out, the
 // reflectors left in A below the diagonal.  R: working precision (the tridiagonal d, e is
 // written in FP64 either way).
+constexpr int kTrdSplitK = 32;  // k-points per half of a split tridiagonalization, at least
 template <typename R>
 lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<R>* A, cx<R>* Vp, cx<R>* Wp,
                            cx<R>* P, cx<R>* P2, cx<R>* Q, double* SS, int nchmax, int nbmax,
@@ -1439,38 +1440,80 @@ lqm_status blocked_tridiag(lqm_context* c, int N, int Kc, cx<R>* A, cx<R>* Vp, c
         LQM_HIP(hipGetLastError());
         return LQM_OK;
     }
-    for (int p = 0; p < N - 1; p += lqm::kBnb) {
+    // Otherwise zlatrd panels.  A chunk of at least 2 kTrdSplitK k-points is cut in two
+    // halves whose column steps go to the context stream and the side stream, interleaved
+    // column by column, so that one half's latency-bound steps (k_trd_col, k_trd_w) run beside
+    // the other half's bandwidth-bound k_trd_hemv (every buffer is per k-point, so a half is
+    // the same call on offset pointers).  Not with stage timing on, which times each stage on
+    // one stream.
+    const bool split = !c->timing && c->side && Kc >= 2 * kTrdSplitK;
+    const int nh = split ? 2 : 1;
+    const int Kh[2] = {split ? Kc / 2 : Kc, split ? Kc - Kc / 2 : 0};
+    const size_t k0[2] = {0, (size_t)Kh[0]};
+    hipStream_t st_of[2] = {c->stream, c->side};
+    const hipStream_t s_main = c->stream;
+    if (split) {
+        if (c->tfac_pending) {
+            LQM_HIP(hipStreamWaitEvent(s_main, c->ev_join, 0));
+            c->tfac_pending = false;
+        }
+        LQM_HIP(hipEventRecord(c->ev_fork, s_main));
+        LQM_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    }
+    const size_t NP = (size_t)N * lqm::kBnb;
+    auto col_step = [&](int h, int j, int i, int last) {
+        const size_t o = k0[h];
+        hipLaunchKernelGGL(lqm::k_trd_col<R>, dim3(Kh[h]), dim3(lqm::kBColT), 0, st_of[h], N, j, i, last, A + o * NN,
+                           Vp + o * NP, Wp + o * NP, dd + o * N, ee + o * N, tau + o * N);
+        if (last) return;
+        const int n = N - j - 1;
+        const int nch = (n + lqm::kHvCols - 1) / lqm::kHvCols;
+        hipLaunchKernelGGL(lqm::k_trd_hemv<R>, dim3(lqm::hv_blocks(nch), Kh[h]), dim3(lqm::kHvWaves * 64), 0, st_of[h],
+                           N, j, i, nchmax, nbmax, A + o * NN, Vp + o * NP, Wp + o * NP, P + o * nchmax * N,
+                           P2 + o * nchmax * N, Q + o * nchmax * lqm::kQStride, SS + o * nbmax);
+        hipLaunchKernelGGL(lqm::k_trd_w<R>, dim3((n + 255) / 256, Kh[h]), dim3(256), 0, st_of[h], N, j, i, nchmax,
+                           nbmax, P + o * nchmax * N, P2 + o * nchmax * N, Q + o * nchmax * lqm::kQStride,
+                           SS + o * nbmax, Vp + o * NP, Wp + o * NP, tau + o * N);
+    };
+    lqm_status st = LQM_OK;
+    for (int p = 0; p < N - 1 && !st; p += lqm::kBnb) {
         const int pw = std::min(lqm::kBnb, N - 1 - p);
         {
             StageScope sc(c, ST_BTRD, Kc);
-            for (int i = 0; i < pw; ++i) {
-                const int j = p + i, n = N - j - 1;
-                const int nch = (n + lqm::kHvCols - 1) / lqm::kHvCols;
-                hipLaunchKernelGGL(lqm::k_trd_col<R>, dim3(Kc), dim3(lqm::kBColT), 0, s, N, j, i, 0, A, Vp, Wp, dd, ee,
-                                   tau);
-                hipLaunchKernelGGL(lqm::k_trd_hemv<R>, dim3(lqm::hv_blocks(nch), Kc), dim3(lqm::kHvWaves * 64), 0, s, N, j, i, nchmax,
-                                   nbmax, A, Vp, Wp, P, P2, Q, SS);
-                hipLaunchKernelGGL(lqm::k_trd_w<R>, dim3((n + 255) / 256, Kc), dim3(256), 0, s, N, j, i, nchmax, nbmax,
-                                   P, P2, Q, SS, Vp, Wp, tau);
-            }
+            for (int i = 0; i < pw; ++i)
+                for (int h = 0; h < nh; ++h) col_step(h, p + i, i, 0);
         }
-        LQM_HIP(hipGetLastError());
+        if (hipGetLastError() != hipSuccess) {
+            st = fail(LQM_ERR_DEVICE, "column-step launch failed");
+            break;
+        }
         // A22 -= V W^H + W V^H on rows/columns >= p + pw, tiles on or below the diagonal
         const int q = p + pw, n = N - q;
-        lqm::GemmArgs<R> g = gemm_args<R>(n, n, pw, Vp + q, N, (long long)N * lqm::kBnb, Wp + q, N,
-                                          (long long)N * lqm::kBnb, A + (size_t)q * N + q, N, (long long)NN,
-                                          {-1.0, 0.0}, {1.0, 0.0});
-        g.npair = 2;
-        g.A[1] = Wp + q;
-        g.B[1] = Vp + q;
-        g.lower = 1;
-        zgemm_gen<0, 1, R>(c, ST_BUPD, Kc, Kc, g);
-        LQM_HIP(hipGetLastError());
+        for (int h = 0; h < nh && !st; ++h) {
+            const size_t o = k0[h];
+            lqm::GemmArgs<R> g = gemm_args<R>(n, n, pw, Vp + o * NP + q, N, (long long)NP, Wp + o * NP + q, N,
+                                              (long long)NP, A + o * NN + (size_t)q * N + q, N, (long long)NN,
+                                              {-1.0, 0.0}, {1.0, 0.0});
+            g.npair = 2;
+            g.A[1] = Wp + o * NP + q;
+            g.B[1] = Vp + o * NP + q;
+            g.lower = 1;
+            c->stream = st_of[h];
+            zgemm_gen<0, 1, R>(c, ST_BUPD, Kh[h], Kh[h], g);
+            c->stream = s_main;
+            if (hipGetLastError() != hipSuccess) st = fail(LQM_ERR_DEVICE, "trailing-update launch failed");
+        }
     }
-    {
+    if (!st) {
         StageScope sc(c, ST_BTRD, Kc);
-        hipLaunchKernelGGL(lqm::k_trd_col<R>, dim3(Kc), dim3(lqm::kBColT), 0, s, N, N - 1, 0, 1, A, Vp, Wp, dd, ee, tau);
+        for (int h = 0; h < nh; ++h) col_step(h, N - 1, 0, 1);
     }
+    if (split) {
+        // join the side stream, also after a failed launch (nothing may stay in flight there)
+        (void)hipEventRecord(c->ev_join, c->side);
+        (void)hipStreamWaitEvent(s_main, c->ev_join, 0);
+    }
+    if (st) return st;
     LQM_HIP(hipGetLastError());
     return LQM_OK;
 }
