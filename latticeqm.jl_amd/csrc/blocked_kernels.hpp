@@ -658,18 +658,28 @@ __global__ __launch_bounds__(256) void k_zgemm_gen(GemmArgs<R> g) {
 }
 
 // ---------------------------------------------------------------------------------
-// Eigenvalue n (ascending) of the symmetric tridiagonal (d, e) of k-point blockIdx.y:
-// bisection on the Sturm count (negative pivots of T - x I = L D L^T, a pivot of modulus
-// below pivmin replaced by -pivmin, LAPACK dstebz), inside the Gershgorin interval, until
-// the interval holds no double between its ends or is below
-// max(2 eps max(|a|, |b|), 1e-3 eps ||T||).  Block x = 0 writes ||T|| (Gershgorin bound)
-// to tnorm[k].
+// Eigenvalue n of the tridiagonal (d, e) of k-point blockIdx.y by the Sturm count (LAPACK
+// dstebz pivots: a pivot of modulus below pivmin = tiny max(1, max e^2) becomes -pivmin), one
+// thread per eigenvalue, 256 per block.  d and e^2 are staged in LDS (dynamic, 16 N bytes);
+// the pivots' reciprocals come from the hardware reciprocal and a Newton step (no division in
+// the recurrence).  A first Sturm pass at 256 points of the Gershgorin interval (one per
+// thread, counts in LDS) gives each eigenvalue a bracket 257 times narrower; then bisection,
+// with a Newton step on det(T - x I) -- d/dx log det = sum_i q_i'/q_i from the same recurrence
+// -- taken whenever the bracket holds eigenvalue n alone and the step lands inside it.
+// Converged when the Newton step is at most 4 eps ||T|| and points into the bracket, or the
+// bracket is at most max(2 eps max(|a|, |b|), eps ||T||) (||T|| = the Gershgorin bound).
+// Out: lam[k * N + n] ascending, tnorm[k] = ||T||.
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_sturm_bisect(int N, const double* __restrict__ dd,
                                                       const double* __restrict__ ee,
                                                       double* __restrict__ lam,
                                                       double* __restrict__ tnorm) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double* sd = reinterpret_cast<double*>(smem);  // [N] d
+    double* se2 = sd + N;                          // [N] e^2 (e_{N-1} = 0)
     __shared__ double rlo[4], rhi[4], re2[4];
+    __shared__ double sx[256];
+    __shared__ int scnt[256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t kk = blockIdx.y;
     const double* d = dd + kk * N;
@@ -677,9 +687,11 @@ __global__ __launch_bounds__(256) void k_sturm_bisect(int N, const double* __res
     double lo = 1e308, hi = -1e308, e2 = 0.0;
     for (int i = tid; i < N; i += 256) {
         const double el = i > 0 ? fabs(e[i - 1]) : 0.0, er = i + 1 < N ? fabs(e[i]) : 0.0;
+        sd[i] = d[i];
+        se2[i] = er * er;
         lo = fmin(lo, d[i] - el - er);
         hi = fmax(hi, d[i] + el + er);
-        if (i + 1 < N) e2 = fmax(e2, e[i] * e[i]);
+        e2 = fmax(e2, er * er);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -699,29 +711,98 @@ __global__ __launch_bounds__(256) void k_sturm_bisect(int N, const double* __res
     const double eps = 2.220446049250313e-16;
     const double bnorm = fmax(fabs(lo), fabs(hi));
     const double pivmin = 2.2250738585072014e-308 * fmax(1.0, e2);
+    const double atol = eps * bnorm, ntol = 4.0 * eps * bnorm;
     if (blockIdx.x == 0 && tid == 0) tnorm[kk] = bnorm;
-    const int n = blockIdx.x * 256 + tid;
-    if (n >= N) return;
     double a = lo - 2.0 * eps * bnorm - 2.0 * pivmin, b = hi + 2.0 * eps * bnorm + 2.0 * pivmin;
-    const double atol = 1e-3 * eps * bnorm;
-    for (int it = 0; it < 200; ++it) {
-        const double mid = 0.5 * (a + b);
-        if (!(mid > a && mid < b)) break;
-        if (b - a <= fmax(2.0 * eps * fmax(fabs(a), fabs(b)), atol)) break;
-        double q = d[0] - mid;
+    {
+        // the grid pass: point t = a + (t + 1) h, h = (b - a) / 257
+        const double h = (b - a) / 257.0;
+        const double xg = a + (double)(tid + 1) * h;
+        double q = sd[0] - xg;
         if (fabs(q) <= pivmin) q = -pivmin;
-        int cnt = q < 0.0;
+        int cg = q < 0.0;
+        double r = rcp_nr(q);
         for (int i = 1; i < N; ++i) {
-            const double ei = e[i - 1];
-            q = (d[i] - mid) - ei * ei / q;
-            if (fabs(q) <= pivmin) q = -pivmin;
-            cnt += q < 0.0;
+            double qn = (sd[i] - xg) - se2[i - 1] * r;
+            if (fabs(qn) <= pivmin) qn = -pivmin;
+            r = rcp_nr(qn);
+            cg += qn < 0.0;
         }
-        if (cnt > n) b = mid;
-        else a = mid;
+        sx[tid] = xg;
+        scnt[tid] = cg;
     }
-    lam[kk * N + n] = 0.5 * (a + b);
+    __syncthreads();
+    const int n = blockIdx.x * 256 + tid;
+    if (n >= N) return;  // no barrier below
+    int ca = 0, cb = N;
+    {
+        // the first grid point whose count exceeds n (counts are nondecreasing; 256 = none)
+        int l0 = 0, l1 = 256;
+        while (l0 < l1) {
+            const int mid = (l0 + l1) >> 1;
+            if (scnt[mid] > n) l1 = mid;
+            else l0 = mid + 1;
+        }
+        if (l0 > 0) {
+            a = sx[l0 - 1];
+            ca = scnt[l0 - 1];
+        }
+        if (l0 < 256) {
+            b = sx[l0];
+            cb = scnt[l0];
+        }
+    }
+    double x = 0.5 * (a + b);
+    for (int it = 0; it < 256; ++it) {
+        double q = sd[0] - x;
+        bool bad = false;
+        if (fabs(q) <= pivmin) {
+            q = -pivmin;
+            bad = true;
+        }
+        int cnt = q < 0.0;
+        double r = rcp_nr(q);
+        double qp = -1.0, S = -r;
+        for (int i = 1; i < N; ++i) {
+            const double t = se2[i - 1] * r;
+            double qn = (sd[i] - x) - t;
+            if (fabs(qn) <= pivmin) {
+                qn = -pivmin;
+                bad = true;
+            }
+            qp = fma(t * r, qp, -1.0);
+            r = rcp_nr(qn);
+            S = fma(qp, r, S);
+            cnt += qn < 0.0;
+        }
+        const bool above = cnt > n;  // eigenvalue n lies below x
+        if (above) {
+            b = x;
+            cb = cnt;
+        } else {
+            a = x;
+            ca = cnt;
+        }
+        const double mid = 0.5 * (a + b);
+        const bool iso = cb - ca == 1;
+        const double step = -1.0 / S;
+        const double xn = x + step;
+        const bool fin = !bad && xn - xn == 0.0;
+        // a small step counts only when it points into the bracket: next to an end, a step
+        // toward the neighbouring eigenvalue outside is small as well
+        if (iso && fin && fabs(step) <= ntol && (above ? step <= 0.0 : step >= 0.0)) {
+            x = fmin(fmax(xn, a), b);
+            break;
+        }
+        if (!(mid > a && mid < b) || b - a <= fmax(2.0 * eps * fmax(fabs(a), fabs(b)), atol)) {
+            x = mid;
+            break;
+        }
+        x = (iso && fin && xn > a && xn < b) ? xn : mid;
+    }
+    lam[kk * N + n] = x;
 }
+inline size_t sturm_lds_bytes(int N) { return 16 * (size_t)N; }
 
 __device__ __forceinline__ double invit_start(unsigned long long seed, int i) {
     unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(i + 1);

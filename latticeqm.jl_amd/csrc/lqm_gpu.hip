@@ -1582,7 +1582,10 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
         if ((st = blocked_tridiag(c, N, Kc, A, Vp, Wp, P, P2, Q, SS, nchmax, nbmax, dd, ee, tau))) return st;
         {
             StageScope sc(c, ST_BEIG, Kc);
-            hipLaunchKernelGGL(lqm::k_sturm_bisect, dim3((N + 255) / 256, Kc), dim3(256), 0, s, N, dd, ee, lam, tn);
+            const size_t shs = lqm::sturm_lds_bytes(N);
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(lqm::k_sturm_bisect),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs);
+            hipLaunchKernelGGL(lqm::k_sturm_bisect, dim3((N + 255) / 256, Kc), dim3(256), shs, s, N, dd, ee, lam, tn);
         }
         LQM_HIP(hipGetLastError());
         if (mq) {
@@ -1824,7 +1827,10 @@ lqm_status run_blocked_mixed(lqm_model* m, int Nk, const double* ks, const doubl
         if ((st = blocked_tridiag<float>(c, N, Kc, A32, Vp, Wp, P, P2, Q, SS, nchmax, nbmax, dd, ee, tau))) return st;
         {
             StageScope sc(c, ST_BEIG, Kc);
-            hipLaunchKernelGGL(lqm::k_sturm_bisect, dim3((N + 255) / 256, Kc), dim3(256), 0, s, N, dd, ee, lamw, tn);
+            const size_t shs = lqm::sturm_lds_bytes(N);
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(lqm::k_sturm_bisect),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs);
+            hipLaunchKernelGGL(lqm::k_sturm_bisect, dim3((N + 255) / 256, Kc), dim3(256), shs, s, N, dd, ee, lamw, tn);
         }
         {
             StageScope sc(c, ST_BVEC, Kc);
