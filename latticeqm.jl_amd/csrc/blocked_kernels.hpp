@@ -846,16 +846,19 @@ __global__ __launch_bounds__(64) void k_invit(int N, int ncol, int ldz, const do
         0x51A7E5EEDull ^ ((unsigned long long)(kbase + (long long)kk) << 24) ^ (unsigned long long)m;
     bool bad = false;
     for (int i = 0; i < N; ++i) Z[(size_t)i * ldz] = invit_start(seed, i);
+    // the previous iterate's 1 / max |z| is applied as its entries are read (no scaling pass);
+    // the last back substitution also forms ||z||^2 (no norm pass)
+    double sc = 1.0, nrm2 = 0.0;
     for (int it = 0; it < 3; ++it) {
         // factor + forward substitution (the next right-hand side entry is read one step ahead)
         double acur = d[0] - x, bcur = N > 1 ? e[0] : 0.0;
-        double ycur = Z[0];
-        double ynext = N > 1 ? Z[ldz] : 0.0;
+        double ycur = Z[0] * sc;
+        double ynext = N > 1 ? Z[ldz] * sc : 0.0;
         for (int k = 0; k + 1 < N; ++k) {
             const double c = e[k];
             const double anext = d[k + 1] - x;
             const double bnext = k + 2 < N ? e[k + 1] : 0.0;
-            const double yfar = k + 2 < N ? Z[(size_t)(k + 2) * ldz] : 0.0;
+            const double yfar = k + 2 < N ? Z[(size_t)(k + 2) * ldz] * sc : 0.0;
             const size_t o = (size_t)k * ldz;
             if (fabs(acur) >= fabs(c)) {
                 const double mult = acur != 0.0 ? c / acur : 0.0;
@@ -904,6 +907,7 @@ __global__ __launch_bounds__(64) void k_invit(int N, int ncol, int ldz, const do
             x2 = x1;
             x1 = v;
             amax = fmax(amax, fabs(v));
+            nrm2 = fma(v, v, nrm2);
             o -= ldz;
             p0 = q0;
             p1 = q1;
@@ -914,23 +918,16 @@ __global__ __launch_bounds__(64) void k_invit(int N, int ncol, int ldz, const do
             bad = true;
             amax = 1.0;
         }
-        if (it < 2) {
-            const double sc = 1.0 / amax;
-            for (int i = 0; i < N; ++i) Z[(size_t)i * ldz] *= sc;
-        }
+        sc = 1.0 / amax;
+        if (it < 2) nrm2 = 0.0;
     }
-    double nrm = 0.0;
-    for (int i = 0; i < N; ++i) {
-        const double v = Z[(size_t)i * ldz];
-        nrm += v * v;
-    }
-    nrm = sqrt(nrm);
+    double nrm = sqrt(nrm2);
     if (!(nrm > 0.0) || !isfinite(nrm)) {
         bad = true;
         nrm = 1.0;
     }
-    const double sc = 1.0 / nrm;
-    for (int i = 0; i < N; ++i) Z[(size_t)i * ldz] *= sc;
+    const double scn = 1.0 / nrm;
+    for (int i = 0; i < N; ++i) Z[(size_t)i * ldz] *= scn;
     if (bad) atomicOr(status, kStatNonFinite);
 }
 
