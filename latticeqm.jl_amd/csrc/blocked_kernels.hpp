@@ -1022,21 +1022,24 @@ __global__ __launch_bounds__(256) void k_wy_T(int N, int nblk, const cx<R>* __re
             }
     __syncthreads();
     const cx<R>* tk = tau + kk * N + jb;
-    // T overwrites P column by column (column t of P copied out first)
+    // T overwrites P column by column (column t of P copied out first): T(s, t) =
+    // -tau_t sum_{s <= u < t} T(s, u) P(u, t), the sum of row s split over four consecutive
+    // lanes (u = s + q, s + q + 4, ...) and added by two lane exchanges
+    const int s = threadIdx.x >> 2, q = threadIdx.x & 3;
     for (int t = 0; t < nbb; ++t) {
         if (threadIdx.x < t) col[threadIdx.x] = Ps[t][threadIdx.x];
         __syncthreads();
         const cx<R> tt = tk[t];
-        if (threadIdx.x < 64) {
-            const int s = threadIdx.x;
+        cx<R> acc = {R(0), R(0)};
+        for (int u = s + q; u < t; u += 4) acc = cmad(acc, Ps[u][s], col[u]);
+        acc.re += __shfl_xor(acc.re, 1);
+        acc.im += __shfl_xor(acc.im, 1);
+        acc.re += __shfl_xor(acc.re, 2);
+        acc.im += __shfl_xor(acc.im, 2);
+        if (q == 0) {
             cx<R> v = {R(0), R(0)};
-            if (s < t) {
-                cx<R> acc = {R(0), R(0)};
-                for (int u = s; u < t; ++u) acc = cmad(acc, Ps[u][s], col[u]);
-                v = cx<R>{R(0), R(0)} - tt * acc;
-            } else if (s == t) {
-                v = tt;
-            }
+            if (s < t) v = cx<R>{R(0), R(0)} - tt * acc;
+            else if (s == t) v = tt;
             Ps[t][s] = v;
         }
         __syncthreads();
