@@ -110,10 +110,10 @@ const char *lqm_stage_name(int stage);
  * kind 3: k-points per block of the lane-per-k QL (64 >> variant, 0..3); kind 4:
  * back-transformation (0 = reflector by reflector, 1 = compact WY blocks on the matrix
  * cores, N <= 64, the default for N > 16); kind 5: eigensolver layout (0 = per-k-point
- * kernels, 1 = the large-N path, the default for N > 64 in FP64 and N > 80 in FP32 with
- * refinement: Householder tridiagonalization -- for 64 < N <= 128 one workgroup per k-point
- * with the matrix in registers, a column per lane (kind 0 variant 0 selects the panels
- * instead), above that zlatrd panels of 32 with matrix-core trailing updates -- Sturm
+ * kernels, 1 = the large-N path, the default for N > 64 in both precision modes:
+ * Householder tridiagonalization -- for 64 < N <= 128 one workgroup per k-point with the
+ * matrix in registers, a column per lane (kind 0 variant 0 selects the panels instead),
+ * above that zlatrd panels of 32 with matrix-core trailing updates -- Sturm
  * bisection, inverse iteration with clusters closer than 1e-5 ||T|| orthogonalised,
  * compact-WY back-transformation and rho_k on the matrix cores; FP64, or FP32 followed by
  * FP64 Ogita-Aishima sweeps in the mixed mode, N <= 4096; occupations below 1e-18 leave

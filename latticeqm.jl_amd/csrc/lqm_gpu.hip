@@ -1371,7 +1371,7 @@ constexpr size_t kBlockedBudget = size_t(40) << 30;  // bytes of per-k scratch p
 // default: the blocked path above these N (measured crossovers of the per-k kernels and the
 // large-N path with the column-per-lane reduction, 4096 k-points of a 9-key model)
 constexpr int kBlockedMinN = 64;         // FP64
-constexpr int kBlockedMinNMixed = 80;    // FP32 with refinement
+constexpr int kBlockedMinNMixed = 64;    // FP32 with refinement
 constexpr double kClusterRel = 1e-5;  // eigenvalues closer than this x ||T|| share a cluster
 constexpr double kOccTiny = 1e-18;    // occupations below this leave rho_k out
 

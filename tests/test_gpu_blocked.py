@@ -252,7 +252,7 @@ def test_mixed_blocked_degenerate_and_occ(pkg, oracle, mixed_blocked):
 
 def test_tbg_m11_slater_koster_mixed(pkg, oracle, gpu):
     # the 1588-orbital twisted bilayer at Gamma and a generic k-point through the FP32 blocked
-    # eigensolver and the FP64 refinement (the default layout above 80 orbitals)
+    # eigensolver and the FP64 refinement (the default layout above 64 orbitals)
     ex = pkg.api.to_executor("gpu", precision="mixed")
     h = pkg.models.tbg_slater_koster(11)
     N = h[(0, 0)].shape[0]
