@@ -336,3 +336,36 @@ def test_blocked_tiny_column_mid_n(pkg, oracle, prec):
     ref = oracle.bands(R, T, ks)
     assert np.all(np.isfinite(b))
     assert np.max(np.abs(b - ref)) < 1e-10 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_blocked_split_reduction(pkg, oracle, prec):
+    # a chunk of >= 64 k-points is reduced as two halves on two streams; with stage timing on it
+    # runs on one stream: the same kernels on the same data, so the results agree bit for bit,
+    # and both agree with the oracle (150 orbitals: the zlatrd panels; 81 k-points: halves of
+    # 40 and 41)
+    _lib = pkg.api._lib
+    h = pkg.models.random_tb(150, rmax=1, seed=1150)
+    ks = pkg.regulargrid(nk=81)
+    ex = pkg.api.to_executor("gpu", precision=prec)
+    out = []
+    for timing in (0, 1):
+        _lib.call("lqm_context_set_timing", ex.handle, timing)
+        try:
+            b, _ = pkg.api.bandmatrix(h, ks, executor=ex)
+            rho = pkg.Hops({Rk: np.zeros((150, 150), complex) for Rk in h.keys()})
+            eb = pkg.api.getdensitymatrix(rho, h, ks, 0.05, T=0.02, executor=ex)
+        finally:
+            _lib.call("lqm_context_set_timing", ex.handle, 0)
+        out.append((b, rho, eb))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][2] == out[1][2]
+    for Rk in h.keys():
+        assert np.array_equal(out[0][1][Rk], out[1][1][Rk])
+    R, T = h.pack()
+    ref = oracle.bands(R, T, ks)
+    assert np.max(np.abs(out[0][0] - ref) / np.maximum(np.abs(ref), _hnorm(h, ks)[None, :])) < 1e-10
+    rref, eb_ref = oracle.densitymatrix(R, T, ks, R, mu=0.05, Temp=0.02)
+    err = max(np.max(np.abs(out[0][1][tuple(R[:, r])] - rref[:, :, r])) for r in range(R.shape[1]))
+    assert err < 1e-9, err
+    assert abs(out[0][2] - eb_ref) <= 1e-10 * max(1.0, abs(eb_ref))
