@@ -1177,8 +1177,9 @@ __global__ __launch_bounds__(256) void k_oa_fused(int N, const cx<double>* __res
 // accumulators -- W = A X, G = X^H X, S = X^H W (W straight from the accumulators: the
 // MFMA's K index k = 16a + g + 4s is the accumulator layout's row g + 4s of tile a), E and
 // the second-order correction (a column never leaves its wave), X E (E likewise) and,
-// with rq.want on a finished k-point, rho_k = X f X^H.  Only X lives in LDS (68 KB), so
-// two blocks share a CU.  Outputs and flags as k_oa_fused.
+// with rq.want on a finished k-point, rho_k = X f X^H (there a panel of bands with negligible
+// occupation skips its X E, and rho_k skips such bands in groups of four).  Only X lives in
+// LDS (68 KB), so two blocks share a CU.  Outputs and flags as k_oa_fused.
 // ---------------------------------------------------------------------------------
 inline size_t oa_panel_lds_bytes() { return sizeof(cx<double>) * 64 * kFusedLd + 8 * (64 + 32 + 64 + 64); }
 
@@ -1419,9 +1420,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         stat[kk * 4 + 2] = (more ? 4 : 0) | (form_rho ? 16 : 0);
         stat[kk * 4 + 3] = em;
     }
+    // On a finished k-point with Fermi-Dirac occupations, a panel whose bands all have negligible
+    // occupation does not enter rho_k, so its X E is skipped: every lane holds the refined
+    // eigenvalue ln + cpart of its column, the value lnew[n] the occupations below come from.
+    bool skip_xe = false;
+    if (form_rho && !rq.occ) {
+        const double fo = n < N ? fermi_occ(ln + cpart, rq.mu, rq.T) : 0.0;
+        skip_xe = __ballot(!(fabs(fo) <= kOccNegligible)) == 0ull;
+    }
     // X E: B from E's accumulators (k = 16a + g + 4s), A = X from LDS (three real products)
     lqm_d4 xr_[4], xi_[4];
-    {
+    if (!skip_xe) {
         lqm_d4 p1[4], p2[4], p3[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) p1[t] = p2[t] = p3[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
@@ -1451,14 +1460,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         combine3(p1, p2, p3, xr_, xi_);
     }
     // X + X E
+    if (!skip_xe) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const cx<double> x = Xs[n * kFusedLd + 16 * t + g + 4 * i];
-            xr_[t][i] += x.re;
-            xi_[t][i] += x.im;
-        }
+            for (int i = 0; i < 4; ++i) {
+                const cx<double> x = Xs[n * kFusedLd + 16 * t + g + 4 * i];
+                xr_[t][i] += x.re;
+                xi_[t][i] += x.im;
+            }
+    }
     if (!form_rho) {
 #pragma unroll
         for (int t = 0; t < 4; ++t)
@@ -1469,12 +1480,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             }
         return;
     }
-    // rho_k = X f X^H from the refined X, staged back into LDS
+    // rho_k = X f X^H from the refined X, staged back into LDS (a skipped panel's columns are
+    // not read)
     __syncthreads();  // every wave is done reading the old X
+    if (!skip_xe) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) Xs[n * kFusedLd + 16 * t + g + 4 * i] = cx<double>{xr_[t][i], xi_[t][i]};
+            for (int i = 0; i < 4; ++i) Xs[n * kFusedLd + 16 * t + g + 4 * i] = cx<double>{xr_[t][i], xi_[t][i]};
+    }
     double fn = 0.0, ebp = 0.0;
     if (tid < N) {
         const double l = lnew[tid];
@@ -1505,11 +1519,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         lqm_d4 p1[4], p2[4], p3[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) p1[t] = p2[t] = p3[t] = lqm_d4{0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < 64; k0 += 16) {
-            if (((live >> k0) & 0xFFFFull) == 0) continue;  // 16 bands of negligible occupation
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int c = k0 + 4 * g + s;
+        for (int k0 = 0; k0 < 64; k0 += 4) {
+            if (((live >> k0) & 0xFull) == 0) continue;  // 4 bands of negligible occupation
+            {
+                const int c = k0 + g;
                 const double f = sf[c];
                 const cx<double> y = Xs[c * kFusedLd + n];  // B(c, n) = conj X(n, c)
                 const double yr = y.re, yi = -y.im, ys = y.re - y.im;
