@@ -72,7 +72,7 @@ enum Slot {
     W_D, W_E, W_REFL, W_TAU, W_EVALS, W_PERM, W_ROT, W_SW, W_NSW, W_GA, W_GV, W_RHOK, W_EB,
     W_PART, W_SMALL, W_BANDS, W_KS, W_KW, W_OM, W_OUT, W_RR, W_Z, W_M, W_EV32, W_X, W_HA, W_W,
     W_S, W_G, W_LAM, W_STAT, W_GRADT, W_PV, W_PO, W_PW, W_KIDX, W_HK, W_KIDX2, W_BA, W_BV, W_BZ,
-    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_LNK, W_LNKO, W_RB, W_ROCC, W_RA, W_RC, W_RP, W_SPB, W_SPPH, W_SPM, W_SPT, W_SPI, W_SPC, W_SPKS, W_SMV, W_RRNEG, W_NSLOTS
+    W_BP, W_BVW, W_BT, W_BWT, W_BSC, W_BVEC, W_WYT, W_ZFMT, W_KLIST, W_TNORM, W_PHT, W_PHC, W_FOCC, W_MUE, W_X32, W_BDGB, W_BDGR, W_HFG, W_HFA, W_MA32, W_MX32, W_MUG, W_SCF, W_LNK, W_LNKO, W_RB, W_ROCC, W_RA, W_RC, W_RP, W_SPB, W_SPPH, W_SPM, W_SPT, W_SPI, W_SPC, W_SPKS, W_SMV, W_RRNEG, W_OASCAN, W_OAHDR, W_NSLOTS
 };
 
 constexpr size_t kWorkspaceBudget = size_t(12) << 30;  // bytes of per-k scratch per chunk
@@ -166,6 +166,7 @@ struct lqm_context {
     void* buf[W_NSLOTS] = {};
     size_t cap[W_NSLOTS] = {};
     int* d_status = nullptr;
+    void* pin = nullptr;     // pinned host bytes for the refinement's per-sweep header
     int refine_max = 3;      // Ogita-Aishima sweeps allowed per call (mixed precision)
     // upper bound on the per-k scratch of one chunk (bytes): the batch SCF gives each worker
     // context an equal share of the device's free memory so that workers on the largest
@@ -1146,7 +1147,6 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
     const int S = std::max(1, std::min(std::max(1, P.Kc / kContractMinK), contract_splits(tilesE * groups)));
     cx<double>* part = nullptr;
     if (rho && nRr > 0 && (st = wsT(c, W_PART, (size_t)groups * S * RG * NN, &part))) return st;
-    std::vector<double> hstat;
     for (int k0 = 0; k0 < Nk; k0 += P.Kc) {
         const int Kc = std::min(P.Kc, Nk - k0);
         const double* ksc = ks + (size_t)k0 * d;
@@ -1188,10 +1188,26 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         int sweeps = 0;
         int nsub = Kc;
         const int* kidx = nullptr;
-        std::vector<int> cur, next;
         const int max_sw = std::max(2, c->refine_max);
-        hstat.resize((size_t)4 * Kc);
-        std::vector<int> clus;
+        // the bookkeeping of each sweep stays on the device (k_oa_scan): the k-point lists
+        // [cluster | next sweep (two buffers, read and written alternately) | rho_k not formed]
+        // and a header, of which only the header comes back
+        int* scan_l = nullptr;
+        lqm::OaScanHdr* dhdr = nullptr;
+        if ((st = wsT(c, W_OASCAN, (size_t)4 * Kc, &scan_l))) return st;
+        if ((st = wsT(c, W_OAHDR, (size_t)1, &dhdr))) return st;
+        if (!c->pin) LQM_HIP(hipHostMalloc(&c->pin, sizeof(lqm::OaScanHdr), hipHostMallocDefault));
+        int lbuf = 0;  // the next-sweep buffer the scan writes
+        lqm::OaScanHdr h{};
+        auto scan = [&](const int* curl, int ncur) -> lqm_status {
+            hipLaunchKernelGGL(lqm::k_oa_scan, dim3(1), dim3(lqm::kScanT), 0, c->stream, Kc, stat, curl, ncur, dhdr,
+                               scan_l, scan_l + (size_t)(1 + lbuf) * Kc, scan_l + (size_t)3 * Kc);
+            LQM_HIP(hipGetLastError());
+            LQM_HIP(hipMemcpyAsync(c->pin, dhdr, sizeof(lqm::OaScanHdr), hipMemcpyDeviceToHost, c->stream));
+            LQM_HIP(hipStreamSynchronize(c->stream));
+            h = *static_cast<const lqm::OaScanHdr*>(c->pin);
+            return LQM_OK;
+        };
         // with rho or eband requested, a fused sweep forms rho_k (into Sm) and eb of every
         // k-point it finishes; the rest go through k_scale_occ and the NC product below
         double* eb = nullptr;
@@ -1199,8 +1215,8 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         const double* occk = occ ? occ + (size_t)k0 * N : nullptr;
         const lqm::OaRho rq{(rho || eband) ? 1 : 0, kwc, wuni, mu, T, occk, Sm, eb};
         for (;;) {
-            bool have_stat = false;
             if (fused) {
+                {
                 // the first sweep (every k-point) and the later ones (listed k-points) timed apart
                 StageScope sc(c, kidx ? ST_OA2 : ST_OA, nsub);
                 if (fk == 2 && x32) {
@@ -1222,26 +1238,13 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
                     hipLaunchKernelGGL(lqm::k_oa_fused, dim3(nsub), dim3(256), sh, c->stream, N, A, X, Sm, G, lam,
                                        stat, c->refine_thr, kidx, rq);
                 }
-                LQM_HIP(hipGetLastError());
-                // k-points with a cluster: general correction (Rayleigh-Ritz) on their S, G
-                LQM_HIP(hipMemcpyAsync(hstat.data(), stat, sizeof(double) * 4 * Kc, hipMemcpyDeviceToHost,
-                                       c->stream));
-                LQM_HIP(hipStreamSynchronize(c->stream));
-                clus.clear();
-                if (kidx) {
-                    for (int k : cur)
-                        if ((int)hstat[4 * k + 2] & 8) clus.push_back(k);
-                } else {
-                    for (int k = 0; k < Kc; ++k)
-                        if ((int)hstat[4 * k + 2] & 8) clus.push_back(k);
                 }
-                have_stat = clus.empty();
-                if (!clus.empty()) {
-                    int* dk2;
-                    if ((st = wsT(c, W_KIDX2, (size_t)Kc, &dk2))) return st;
-                    LQM_HIP(hipMemcpyAsync(dk2, clus.data(), sizeof(int) * clus.size(), hipMemcpyHostToDevice,
-                                           c->stream));
-                    const int nc = (int)clus.size();
+                LQM_HIP(hipGetLastError());
+                if ((st = scan(kidx, nsub))) return st;
+                // k-points with a cluster: general correction (Rayleigh-Ritz) on their S, G
+                if (h.n8 > 0) {
+                    const int* dk2 = scan_l;
+                    const int nc = h.n8;
                     {
                         StageScope sc(c, ST_OA2, nc);
                         const size_t sh = lqm::oa_lds_bytes(N);
@@ -1254,6 +1257,8 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
                     const int bx = (int)std::min<size_t>(64, (NN + 255) / 256);
                     hipLaunchKernelGGL(lqm::k_copy_sub, dim3(bx, nc), dim3(256), 0, c->stream, (int)NN, nc, dk2, G, X);
                     LQM_HIP(hipGetLastError());
+                    // the correction rewrote those k-points' status rows
+                    if ((st = scan(kidx, nsub))) return st;
                 }
             } else {
             zgemm<0, 0>(c, N, nsub, A, X, W, one, zero, nullptr, kidx);   // W = A X
@@ -1278,43 +1283,24 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
             } else {
                 std::swap(X, G);
             }
+            if ((st = scan(kidx, nsub))) return st;
             }
             ++sweeps;
-            if (!have_stat) {
-                LQM_HIP(hipMemcpyAsync(hstat.data(), stat, sizeof(double) * 4 * Kc, hipMemcpyDeviceToHost,
-                                       c->stream));
-                LQM_HIP(hipStreamSynchronize(c->stream));
-            }
-            if (!kidx) {
-                cur.resize(Kc);
-                for (int k = 0; k < Kc; ++k) cur[k] = k;
-            }
-            next.clear();
-            int unresolved = 0;
-            for (int k : cur) {
-                rmax = std::max(rmax, hstat[4 * k]);
-                dmax = std::max(dmax, hstat[4 * k + 1]);
-                const int fl = (int)hstat[4 * k + 2];
-                unresolved += (fl & 2) != 0;
-                if (fl & 4) next.push_back(k);
-                if (!std::isfinite(hstat[4 * k]) || !std::isfinite(hstat[4 * k + 3]))
-                    return fail(LQM_ERR_NONFINITE, "mixed precision: non-finite refinement residual");
-            }
-            c->report[4] += (double)cur.size();
-            if (next.empty()) break;
+            rmax = std::max(rmax, h.rmax);
+            dmax = std::max(dmax, h.dmax);
+            if (h.nonfinite) return fail(LQM_ERR_NONFINITE, "mixed precision: non-finite refinement residual");
+            c->report[4] += (double)nsub;
+            if (h.n4 == 0) break;
             if (sweeps >= max_sw) {
-                nnear += unresolved;
-                if (unresolved)
+                nnear += h.unresolved;
+                if (h.unresolved)
                     return fail(LQM_ERR_NOCONV, "mixed precision: a near-degenerate cluster wider than 8 "
                                                 "eigenvalues could not be resolved; use the FP64 path");
                 break;
             }
-            int* dk;
-            if ((st = wsT(c, W_KIDX, (size_t)Kc, &dk))) return st;
-            LQM_HIP(hipMemcpyAsync(dk, next.data(), sizeof(int) * next.size(), hipMemcpyHostToDevice, c->stream));
-            kidx = dk;
-            nsub = (int)next.size();
-            cur.swap(next);
+            kidx = scan_l + (size_t)(1 + lbuf) * Kc;
+            nsub = h.n4;
+            lbuf ^= 1;
         }
         sweeps_max = std::max(sweeps_max, sweeps);
         if (bands) {
@@ -1329,18 +1315,9 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
             int nrest = Kc;
             const int* krest = nullptr;
             if (fused) {
-                clus.clear();
-                for (int k = 0; k < Kc; ++k)
-                    if (!((int)hstat[4 * k + 2] & 16)) clus.push_back(k);
-                nrest = (int)clus.size();
+                nrest = h.nrest;  // from the last sweep's scan, over every k-point of the chunk
                 c->report[5] += (double)(Kc - nrest);
-                if (nrest > 0 && nrest < Kc) {
-                    int* dk2;
-                    if ((st = wsT(c, W_KIDX2, (size_t)Kc, &dk2))) return st;
-                    LQM_HIP(hipMemcpyAsync(dk2, clus.data(), sizeof(int) * nrest, hipMemcpyHostToDevice,
-                                           c->stream));
-                    krest = dk2;
-                }
+                if (nrest > 0 && nrest < Kc) krest = scan_l + (size_t)3 * Kc;
             }
             {
                 StageScope sc(c, ST_REDUCE);
@@ -2443,6 +2420,7 @@ lqm_status lqm_context_destroy(lqm_context* c) {
     for (int s = 0; s < W_NSLOTS; ++s)
         if (c->buf[s]) (void)dev_free(c->buf[s]);
     (void)dev_free(c->d_status);
+    if (c->pin) (void)hipHostFree(c->pin);
     (void)hipStreamDestroy(c->own);
     (void)hipStreamDestroy(c->side);
     (void)hipEventDestroy(c->ev_fork);

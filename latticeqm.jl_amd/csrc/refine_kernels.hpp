@@ -1554,4 +1554,101 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
 }
 
+
+// ---------------------------------------------------------------------------------
+// Bookkeeping between refinement sweeps, on the device.  From the status rows
+// (||R||_F, delta, flags, ||E||) of the k-points the sweep refined -- the list `cur`, or every
+// k-point 0..ncur-1 when it is null -- it lists, in ascending order, the k-points handed to the
+// cluster correction (flag 8) in l8 and those flagged for another sweep (flag 4) in l4, and over
+// all Kc k-points those whose rho_k no sweep formed (flag 16 clear) in lrest; the header takes
+// the three counts, the number of refined k-points with an unresolved cluster (flag 2), whether
+// any refined k-point has a non-finite ||R||_F or ||E||, and the maxima of ||R||_F and delta.
+// One block; each thread owns a contiguous run of entries, so the lists keep the input order.
+// ---------------------------------------------------------------------------------
+struct OaScanHdr {
+    int n8, n4, nrest, unresolved, nonfinite, pad[3];
+    double rmax, dmax;
+};
+constexpr int kScanT = 1024;
+
+__global__ __launch_bounds__(kScanT) void k_oa_scan(int Kc, const double* __restrict__ stat,
+                                                    const int* __restrict__ cur, int ncur,
+                                                    OaScanHdr* __restrict__ hdr, int* __restrict__ l8,
+                                                    int* __restrict__ l4, int* __restrict__ lrest) {
+    __shared__ int wt[3][kScanT / 64];
+    __shared__ int wu[kScanT / 64], wn[kScanT / 64];
+    __shared__ double wr[kScanT / 64], wd[kScanT / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int per = (ncur + kScanT - 1) / kScanT, lo = min(ncur, tid * per), hi = min(ncur, lo + per);
+    const int perA = (Kc + kScanT - 1) / kScanT, loA = min(Kc, tid * perA), hiA = min(Kc, loA + perA);
+    int own[3] = {0, 0, 0}, unres = 0, nonfin = 0;
+    double rmax = 0.0, dmax = 0.0;
+    for (int i = lo; i < hi; ++i) {
+        const int k = cur ? cur[i] : i;
+        const double* r = stat + 4 * (size_t)k;
+        const int fl = (int)r[2];
+        own[0] += (fl & 8) != 0;
+        own[1] += (fl & 4) != 0;
+        unres += (fl & 2) != 0;
+        nonfin |= !isfinite(r[0]) || !isfinite(r[3]);
+        rmax = fmax(rmax, r[0]);
+        dmax = fmax(dmax, r[1]);
+    }
+    for (int k = loA; k < hiA; ++k) own[2] += ((int)stat[4 * (size_t)k + 2] & 16) == 0;
+    // exclusive block scan of the three counts: within the wave, then over the wave totals
+    int inc[3] = {own[0], own[1], own[2]};
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int t = __shfl_up(inc[j], o);
+            if (lane >= o) inc[j] += t;
+        }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        unres += __shfl_xor(unres, o);
+        nonfin |= __shfl_xor(nonfin, o);
+        rmax = fmax(rmax, __shfl_xor(rmax, o));
+        dmax = fmax(dmax, __shfl_xor(dmax, o));
+    }
+    if (lane == 63)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) wt[j][wave] = inc[j];
+    if (lane == 0) {
+        wu[wave] = unres;
+        wn[wave] = nonfin;
+        wr[wave] = rmax;
+        wd[wave] = dmax;
+    }
+    __syncthreads();
+    int off[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        int b = 0;
+        for (int w = 0; w < wave; ++w) b += wt[j][w];
+        off[j] = b + inc[j] - own[j];
+    }
+    for (int i = lo; i < hi; ++i) {
+        const int k = cur ? cur[i] : i;
+        const int fl = (int)stat[4 * (size_t)k + 2];
+        if (fl & 8) l8[off[0]++] = k;
+        if (fl & 4) l4[off[1]++] = k;
+    }
+    for (int k = loA; k < hiA; ++k)
+        if (!((int)stat[4 * (size_t)k + 2] & 16)) lrest[off[2]++] = k;
+    if (tid == 0) {
+        OaScanHdr h{};
+        for (int w = 0; w < kScanT / 64; ++w) {
+            h.n8 += wt[0][w];
+            h.n4 += wt[1][w];
+            h.nrest += wt[2][w];
+            h.unresolved += wu[w];
+            h.nonfinite |= wn[w];
+            h.rmax = fmax(h.rmax, wr[w]);
+            h.dmax = fmax(h.dmax, wd[w]);
+        }
+        *hdr = h;
+    }
+}
+
 }  // namespace lqm
