@@ -1213,7 +1213,9 @@ lqm_status run_mixed(lqm_model* m, int Nk, const double* ks, const double* kw, d
         double* eb = nullptr;
         if ((rho || eband) && (st = wsT(c, W_EB, (size_t)P.Kc, &eb))) return st;
         const double* occk = occ ? occ + (size_t)k0 * N : nullptr;
-        const lqm::OaRho rq{(rho || eband) ? 1 : 0, kwc, wuni, mu, T, occk, Sm, eb};
+        // a key list closed under R -> -R: the contraction reads rho_k on and above the diagonal
+        const lqm::OaRho rq{(rho || eband) ? 1 : 0, kwc, wuni, mu, T, occk, Sm, eb,
+                            (rho && c->rr_pairs && nRr > 0) ? 1 : 0};
         for (;;) {
             if (fused) {
                 {

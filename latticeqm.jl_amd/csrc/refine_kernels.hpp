@@ -772,6 +772,7 @@ struct OaRho {
     const double* occ;
     cx<double>* rhok;
     double* eb;
+    int upper;  // only the elements on and above the diagonal of rho_k are read (k_oa_panel)
 };
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -1514,6 +1515,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     __syncthreads();
     if (tid == 0) rq.eb[kk] = (rq.kw ? rq.kw[kk] : rq.wuni) * (red[16] + red[17] + red[18] + red[19]);
     const unsigned long long live = s_live;
+    // row tiles of rho_k this wave forms: all four, or with rq.upper those on and above the
+    // diagonal (t <= wave; the lower half of the diagonal tile comes along)
+    const int tmax = rq.upper ? __builtin_amdgcn_readfirstlane(wave) : 3;
     lqm_d4 cr[4], ci[4];
     {
         lqm_d4 p1[4], p2[4], p3[4];
@@ -1526,19 +1530,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 const double f = sf[c];
                 const cx<double> y = Xs[c * kFusedLd + n];  // B(c, n) = conj X(n, c)
                 const double yr = y.re, yi = -y.im, ys = y.re - y.im;
-                double xr[4], xi[4], xs[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    const cx<double> x = Xs[c * kFusedLd + 16 * t + li];
-                    xr[t] = f * x.re;
-                    xi[t] = f * x.im;
-                    xs[t] = xr[t] + xi[t];
-                }
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    p1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[t], yr, p1[t], 0, 0, 0);
-                    p2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[t], yi, p2[t], 0, 0, 0);
-                    p3[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[t], ys, p3[t], 0, 0, 0);
+                    if (t <= tmax) {
+                        const cx<double> x = Xs[c * kFusedLd + 16 * t + li];
+                        const double xr = f * x.re, xi = f * x.im, xs = xr + xi;
+                        p1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr, yr, p1[t], 0, 0, 0);
+                        p2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi, yi, p2[t], 0, 0, 0);
+                        p3[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs, ys, p3[t], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -1547,11 +1547,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     cx<double>* Rk = rq.rhok + kk * NN;
 #pragma unroll
     for (int t = 0; t < 4; ++t)
+        if (t <= tmax)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int m = 16 * t + g + 4 * i;
-            if (m < N && n < N) Rk[(size_t)n * N + m] = cx<double>{cr[t][i], ci[t][i]};
-        }
+            for (int i = 0; i < 4; ++i) {
+                const int m = 16 * t + g + 4 * i;
+                if (m < N && n < N) Rk[(size_t)n * N + m] = cx<double>{cr[t][i], ci[t][i]};
+            }
 }
 
 
