@@ -501,7 +501,8 @@ __global__ __launch_bounds__(256) void k_trd_w(int N, int j, int i, int nchmax, 
 template <typename R>
 struct GemmArgs {
     int M, N, K, npair;
-    int lower;  // M == N: only the 64 x 64 tiles on or below the diagonal (grid.x = nt(nt+1)/2)
+    int lower;  // M == N: only the 64 x 64 tiles on or below (1) / on or above (2) the diagonal
+                // (grid.x = nt(nt+1)/2)
     const cx<R>* A[2];
     const cx<R>* B[2];
     int lda, ldb, ldc;
@@ -617,6 +618,11 @@ __global__ __launch_bounds__(256) void k_zgemm_gen(GemmArgs<R> g) {
         while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
         while (ti * (ti + 1) / 2 > t) --ti;
         tj = t - ti * (ti + 1) / 2;
+        if (g.lower == 2) {
+            const int x = ti;
+            ti = tj;
+            tj = x;
+        }
     }
     const int m0 = ti * 64 + (wave & 1) * 32;
     const int n0 = tj * 64 + (wave >> 1) * 32;

@@ -1668,6 +1668,7 @@ lqm_status run_blocked(lqm_model* m, int Nk, const double* ks, const double* kw,
                                          {0.0, 0.0});
             g.ascale = fo;
             g.sS = N;
+            g.lower = c->rr_pairs ? 2 : 0;  // the contraction reads rho_k on and above the diagonal
             zgemm_gen<0, 1>(c, ST_BRHO, Kc, Kc, g);
         }
         const int Sc = std::min(S, Kc);
@@ -1951,9 +1952,12 @@ lqm_status run_blocked_mixed(lqm_model* m, int Nk, const double* ks, const doubl
                 hipLaunchKernelGGL(lqm::k_scale_occ, dim3(Kc), dim3(256), sizeof(double) * N, s, N, X, lam, kwc, wuni,
                                    mu, T, occk, Sm, eb, nullptr);
             }
-            if (rhok)
-                zgemm_gen<0, 1>(c, ST_BRHO, Kc, Kc, gemm_args(N, N, N, Sm, N, (long long)NN, X, N, (long long)NN, rhok,
-                                                              N, (long long)NN, one, zero));
+            if (rhok) {
+                lqm::ZGemmArgs g = gemm_args(N, N, N, Sm, N, (long long)NN, X, N, (long long)NN, rhok, N,
+                                             (long long)NN, one, zero);
+                g.lower = c->rr_pairs ? 2 : 0;  // the contraction reads rho_k on and above the diagonal
+                zgemm_gen<0, 1>(c, ST_BRHO, Kc, Kc, g);
+            }
         } else {
             {
                 StageScope sc(c, ST_REDUCE);
@@ -1981,6 +1985,7 @@ lqm_status run_blocked_mixed(lqm_model* m, int Nk, const double* ks, const doubl
                                              (long long)NN, one, zero);
                 g.ascale = fo;
                 g.sS = N;
+                g.lower = c->rr_pairs ? 2 : 0;  // the contraction reads rho_k on and above the diagonal
                 zgemm_gen<0, 1>(c, ST_BRHO, Kc, Kc, g);
             } else if (rhok) {
                 LQM_HIP(hipMemsetAsync(rhok, 0, sizeof(cx<double>) * NN * Kc, s));
