@@ -670,8 +670,9 @@ lqm_status launch_backtransform(lqm_model* m, const Plan& P, int Kc, const R* ev
                 auto kW = lqm::k_backtransform_wy<R, MODE>;
                 const size_t sh = lqm::wy_lds_bytes();
                 allow_lds(kW, sh);
+                // MODE 1 (rho_k): a key list closed under R -> -R is contracted from the upper triangle
                 hipLaunchKernelGGL(kW, dim3(Kc), dim3(256), sh, c->stream, N, refl, tau, Z, evals, perm, kw, wuni,
-                                   mu, T, c->occ, out, eb, Tg, zfmt);
+                                   mu, T, c->occ, out, eb, Tg, zfmt, (MODE == 1 && c->rr_pairs) ? 1 : 0);
             }
             LQM_HIP(hipGetLastError());
             return LQM_OK;

@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     int N, const cx<R>* __restrict__ refl, const cx<R>* __restrict__ tau, const R* __restrict__ Zg,
     const R* __restrict__ evals, const int* __restrict__ perm, const double* __restrict__ kw,
     double wuni, double mu, double Temp, const double* __restrict__ occ, cx<double>* __restrict__ out,
-    double* __restrict__ eb, const cx<double>* __restrict__ Tg, const int* __restrict__ zfmt) {
+    double* __restrict__ eb, const cx<double>* __restrict__ Tg, const int* __restrict__ zfmt, int upper) {
     extern __shared__ __align__(16) unsigned char smem[];
     WySmem& S = *reinterpret_cast<WySmem*>(smem);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -310,16 +310,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         if (tid == 0) eb[kk] = (kw ? kw[kk] : wuni) * (S.red[0] + S.red[1] + S.red[2] + S.red[3]);
         // columns whose occupation enters the product (the rest are below kOccNegligible)
         const unsigned long long live = __ballot(!(fabs(S.sf[lane]) <= kOccNegligible));
+        // 16 x 16 tiles this wave forms: all four, or with `upper` (the caller reads only the
+        // elements on and above the diagonal) those whose first row is not below their first
+        // column -- every other tile lies wholly below the diagonal
+        bool on[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                on[a][b] = !upper || __builtin_amdgcn_readfirstlane(m0) + 16 * a <= __builtin_amdgcn_readfirstlane(n0) + 16 * b;
         lqm_d4 p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b) p1[a][b] = p2[a][b] = p3[a][b] = lqm_d4{0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < 64; k0 += 16) {
-            if (((live >> k0) & 0xFFFFull) == 0) continue;  // 16 columns of negligible occupation
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int c = k0 + 4 * g + s;
+        for (int k0 = 0; k0 < 64; k0 += 4) {
+            if (((live >> k0) & 0xFull) == 0) continue;  // 4 columns of negligible occupation
+            {
+                const int c = k0 + g;
                 const double f = S.sf[c];
                 double ar[2], ai[2], as[2], br[2], bi[2], bs[2];
 #pragma unroll
@@ -336,11 +344,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
                 for (int a = 0; a < 2; ++a)
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) {
-                        p1[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], p1[a][b], 0, 0, 0);
-                        p2[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bi[b], p2[a][b], 0, 0, 0);
-                        p3[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[a], bs[b], p3[a][b], 0, 0, 0);
-                    }
+                    for (int b = 0; b < 2; ++b)
+                        if (on[a][b]) {
+                            p1[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], p1[a][b], 0, 0, 0);
+                            p2[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bi[b], p2[a][b], 0, 0, 0);
+                            p3[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[a], bs[b], p3[a][b], 0, 0, 0);
+                        }
             }
         }
         lqm_d4 cr[2][2], ci[2][2];
@@ -359,7 +368,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int m = m0 + 16 * a + g + 4 * e, n = n0 + 16 * b + li;
-                    if (m < N && n < N) o[(size_t)n * N + m] = cx<double>{cr[a][b][e], ci[a][b][e]};
+                    if (on[a][b] && m < N && n < N) o[(size_t)n * N + m] = cx<double>{cr[a][b][e], ci[a][b][e]};
                 }
     }
 }
